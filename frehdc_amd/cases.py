@@ -1,0 +1,252 @@
+"""Synthetic scenario builder for the free-surface step.
+
+A scenario is what FrehdC reads from data.dat, bath.dat and the boundary
+files (FREHD.c:30-63, bathymetry.c:20-57, initialize.c:345-378): grid and
+physics settings, a bed elevation per cell, boundary series sampled once per
+step, and the initial fields.  The builders here make the workloads the
+benchmarks and tests use; every random draw is seeded.
+
+Unspecified physics settings take the values of the reference's shipped
+sample (FrehdC_v42/data.dat:67-87): eddy viscosity and diffusivity 1e-4,
+thin-layer drag on (0.1 m, CD cap 1.0), minimum depth and waterfall
+threshold 1e-5 m, advection CFL limiter 0.5/0.7.
+
+Orientation: the reference applies tides only on the y boundaries
+(YP = top row, configuration.c:143-148).  A "west" tidal boundary in a
+workload description is mapped to the YP row, with the bed sloping along y.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import abi
+
+TIDE_PERIOD = 44712.0
+
+
+@dataclasses.dataclass
+class Scenario:
+    cfg: dict
+    bottomZ: np.ndarray            # (N,) raw bed elevation, m about MSL
+    tideP: np.ndarray              # (Nt+1,)
+    tideM: np.ndarray
+    inflow: np.ndarray
+    tidalPS: np.ndarray
+    tidalMS: np.ndarray
+    tideLocP: np.ndarray           # int32 cell indices
+    tideLocM: np.ndarray
+    inflowLoc: np.ndarray
+    manningCell: np.ndarray | None = None
+    surf0: float = 0.0
+    U0: float = 0.0
+    V0: float = 0.0
+    S0: float = 0.0
+    S0cell: np.ndarray | None = None   # (nScalar*N,)
+    name: str = ""
+
+    @property
+    def nx(self):
+        return self.cfg["nx"]
+
+    @property
+    def ny(self):
+        return self.cfg["ny"]
+
+    @property
+    def N(self):
+        return self.nx * self.ny
+
+    @property
+    def Ng(self):
+        return (self.nx + 2) * (self.ny + 2)
+
+    # ---- ctypes views (the scenario object keeps the arrays alive) ----------
+    def c_config(self, **override) -> abi.FrehdConfig:
+        c = abi.FrehdConfig()
+        vals = dict(self.cfg)
+        vals.update(override)
+        for k, v in vals.items():
+            setattr(c, k, v)
+        return c
+
+    def c_forcing(self) -> abi.FrehdForcing:
+        f = abi.FrehdForcing()
+        f.tideLocP = abi.iptr(self.tideLocP)
+        f.tideLocM = abi.iptr(self.tideLocM)
+        f.inflowLoc = abi.iptr(self.inflowLoc)
+        f.tideP = abi.dptr(self.tideP)
+        f.tideM = abi.dptr(self.tideM)
+        f.inflow = abi.dptr(self.inflow)
+        f.tidalPS = abi.dptr(self.tidalPS)
+        f.tidalMS = abi.dptr(self.tidalMS)
+        f.manningCell = abi.dptr(self.manningCell)
+        return f
+
+    def c_ic(self) -> abi.FrehdIC:
+        ic = abi.FrehdIC()
+        ic.bottomZ = abi.dptr(self.bottomZ)
+        ic.surf0, ic.U0, ic.V0, ic.S0 = self.surf0, self.U0, self.V0, self.S0
+        ic.S0cell = abi.dptr(self.S0cell)
+        return ic
+
+
+def default_config(nx, ny, dx, dt, Nt, **kw) -> dict:
+    cfg = dict(
+        nx=nx, ny=ny, Nt=Nt, maxIter=10000, dx=float(dx), dy=float(dx), dt=float(dt), g=9.81,
+        NUx=1e-4, NUy=1e-4, CDx=0.01, CDy=0.01, manningN=0.025, CDnotN=0, useThinLayer=1,
+        hD=0.1, CDmax=1.0, minDepth=1e-5, wtfh=1e-5, CFLl=0.5, CFLh=0.7, eps=1e-10,
+        precond=abi.PRECOND_JACOBI, precision=abi.PREC_FP64, bcType=1, tideLocLengthP=nx,
+        tideLocLengthM=0, inflowLocLength=0, useScalar=1, scalarAdv=1, useConstTidePS=1,
+        useConstTideMS=1, nScalar=1, tidePS=0.0, tideMS=0.0, inflowS=0.0, Kx=1e-4, Ky=1e-4,
+        offset=0.0, epsInner=1e-6, maxRefine=8, reserved0=0,
+    )
+    cfg.update(kw)
+    return cfg
+
+
+def offset_for(bed: np.ndarray) -> float:
+    """Bath.offset: -min(bed) when the bed dips below zero (bathymetry.c:79-83)."""
+    zmin = float(np.min(bed))
+    return 0.0 if zmin >= 0 else -zmin
+
+
+def tide_series(Nt, dt, amp, period=TIDE_PERIOD, phase=0.0):
+    t = np.arange(Nt + 1, dtype=np.float64) * dt
+    return amp * np.sin(2.0 * math.pi * t / period + phase)
+
+
+def fbm_surface(nx, ny, hurst, rng, lo, hi):
+    """Fractional-Brownian surface by spectral synthesis (power ~ k^-(2H+2)),
+    rescaled to [lo, hi]."""
+    kx = np.fft.fftfreq(nx)[None, :]
+    ky = np.fft.fftfreq(ny)[:, None]
+    k = np.sqrt(kx * kx + ky * ky)
+    k[0, 0] = 1.0
+    amp = k ** (-(hurst + 1.0))
+    amp[0, 0] = 0.0
+    phase = rng.uniform(0, 2 * np.pi, size=(ny, nx))
+    noise = rng.normal(size=(ny, nx))
+    spec = amp * noise * np.exp(1j * phase)
+    f = np.real(np.fft.ifft2(spec))
+    f = (f - f.min()) / max(f.max() - f.min(), 1e-300)
+    return lo + (hi - lo) * f
+
+
+def _finish(cfg, bed, tideP, scalar0=None, manning=None, inflowLoc=None, inflow=None,
+            surf0=0.0, name=""):
+    nx, ny, Nt = cfg["nx"], cfg["ny"], cfg["Nt"]
+    N = nx * ny
+    bed = np.ascontiguousarray(bed, dtype=np.float64).reshape(N)
+    cfg["offset"] = offset_for(bed)
+    z = np.zeros(Nt + 1)
+    tideLocP = np.arange(nx * (ny - 1), nx * ny, dtype=np.int32)[: cfg["tideLocLengthP"]]
+    if inflowLoc is None:
+        inflowLoc = np.zeros(0, dtype=np.int32)
+    cfg["inflowLocLength"] = int(len(inflowLoc))
+    return Scenario(
+        cfg=cfg, bottomZ=bed, tideP=np.ascontiguousarray(tideP), tideM=z.copy(),
+        inflow=np.ascontiguousarray(inflow if inflow is not None else z.copy()),
+        tidalPS=z.copy(), tidalMS=z.copy(), tideLocP=np.ascontiguousarray(tideLocP),
+        tideLocM=np.zeros(0, dtype=np.int32), inflowLoc=np.ascontiguousarray(inflowLoc, dtype=np.int32),
+        manningCell=None if manning is None else np.ascontiguousarray(manning, dtype=np.float64),
+        surf0=surf0, S0cell=None if scalar0 is None else np.ascontiguousarray(scalar0, dtype=np.float64),
+        name=name)
+
+
+def basin(nx=128, ny=128, dx=10.0, dt=1.0, Nt=3600, seed=1, amp=0.2, bumps=20):
+    """Rectangular basin, bed sloping from 3.0 m depth at the tidal boundary to
+    0.5 m at the far wall plus seeded Gaussian bumps (0.3 m, sigma 30 m);
+    Manning 0.025; Gaussian scalar plume (peak 1, sigma 50 m) at the centre;
+    Jacobi PCG to 1e-10."""
+    rng = np.random.default_rng(seed)
+    y = np.arange(ny)[:, None] * dx
+    x = np.arange(nx)[None, :] * dx
+    s = (ny - 1 - np.arange(ny)[:, None]) / max(ny - 1, 1)     # 0 at the tidal row
+    bed = (-3.0 + 2.5 * s) * np.ones((1, nx))
+    cx = rng.uniform(0, nx * dx, size=bumps)
+    cy = rng.uniform(0, ny * dx, size=bumps)
+    for a, b in zip(cx, cy):
+        bed = bed + 0.3 * np.exp(-((x - a) ** 2 + (y - b) ** 2) / (2 * 30.0 ** 2))
+    cfg = default_config(nx, ny, dx, dt, Nt, manningN=0.025, eps=1e-10)
+    xc, yc = nx * dx / 2, ny * dx / 2
+    plume = np.exp(-((x - xc) ** 2 + (y - yc) ** 2) / (2 * 50.0 ** 2)) * np.ones((ny, nx))
+    return _finish(cfg, bed, tide_series(Nt, dt, amp), scalar0=plume.reshape(-1),
+                   name=f"basin{nx}x{ny}")
+
+
+def marsh(n=1024, dx=2.0, dt=0.5, Nt=2000, seed=2, hurst=0.7, amp=0.6):
+    """Marsh with wetting and drying: fBm bed (Hurst 0.7) in [-1.0, +1.5] m,
+    tide 0.6 m, per-cell Manning n ~ U(0.02, 0.04), scalar 0 inside with 1 at
+    the tidal boundary."""
+    rng = np.random.default_rng(seed)
+    bed = fbm_surface(n, n, hurst, rng, -1.0, 1.5)
+    manning = np.random.default_rng(seed + 1).uniform(0.02, 0.04, size=n * n)
+    cfg = default_config(n, n, dx, dt, Nt, useConstTidePS=1, tidePS=1.0, eps=1e-10)
+    return _finish(cfg, bed, tide_series(Nt, dt, amp), scalar0=np.zeros(n * n),
+                   manning=manning, name=f"marsh{n}")
+
+
+def channel(n=4096, dx=1.0, dt=0.5, Nt=500, seed=4, hurst=0.8, amp=0.5, q_river=50.0,
+            nscalar=3):
+    """Large domain: fBm bed (Hurst 0.8) with a carved meandering channel
+    (2 m deep, 20 cells wide), tide 0.5 m on the YP row, river inflow of
+    q_river m^3/s spread over the channel cells of row 0, three scalars."""
+    rng = np.random.default_rng(seed)
+    bed = fbm_surface(n, n, hurst, rng, -0.5, 1.5)
+    jj = np.arange(n)
+    centre = n / 2 + 0.2 * n * np.sin(2 * np.pi * jj / n * 2.0)
+    ii = np.arange(n)[None, :]
+    inside = np.abs(ii - centre[:, None]) < 10
+    bed = np.where(inside, bed - 2.0, bed)
+    cfg = default_config(n, n, dx, dt, Nt, eps=1e-10, nScalar=nscalar)
+    row0 = np.nonzero(inside[0])[0].astype(np.int32)
+    inflowLoc = row0.copy()                     # cells of row j = 0
+    inflow = np.full(Nt + 1, q_river)
+    N = n * n
+    s0 = np.concatenate([np.full(N, 35.0 * (q / max(nscalar - 1, 1))) for q in range(nscalar)])
+    return _finish(cfg, bed, tide_series(Nt, dt, amp), scalar0=s0, inflowLoc=inflowLoc,
+                   inflow=inflow, name=f"channel{n}")
+
+
+def ensemble_member(m, n=2048, dx=1.0, dt=0.5, Nt=1000, seed=5, hurst=0.7, base=None):
+    """Ensemble member m: shared fBm base plus a seeded N(0, 0.05) perturbation,
+    Manning n ~ U(0.015, 0.04) and tide amplitude ~ U(0.3, 0.8) per member."""
+    if base is None:
+        base = fbm_surface(n, n, hurst, np.random.default_rng(seed), -1.0, 1.5)
+    r = np.random.default_rng(100 + m)
+    bed = base + r.normal(0.0, 0.05, size=base.shape)
+    cfg = default_config(n, n, dx, dt, Nt, manningN=float(r.uniform(0.015, 0.04)), eps=1e-10)
+    return _finish(cfg, bed, tide_series(Nt, dt, float(r.uniform(0.3, 0.8))),
+                   scalar0=np.zeros(n * n), name=f"member{m}")
+
+
+def serving_requests(count=512, rate=20.0, seed=6, lo=64, hi=1024, dx=2.0, dt=0.5,
+                     steps_lo=200, steps_hi=2000):
+    """Poisson arrivals of scenarios with log-uniform sizes in [lo^2, hi^2]
+    (rounded to multiples of 64 per side), fBm bed with Hurst ~ U(0.6, 0.9) in
+    [-1.5, +1] m, tide ~ U(0.1, 0.7) m, Manning ~ U(0.015, 0.04), 1-3 scalars,
+    U(200, 2000) steps.  Returns [(arrival_s, builder)] where builder() makes
+    the Scenario on demand."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    out = []
+    for q in range(count):
+        side = math.exp(rng.uniform(math.log(lo), math.log(hi)))
+        side = int(max(lo, min(hi, round(side / 64) * 64)))
+        hurst = float(rng.uniform(0.6, 0.9))
+        amp = float(rng.uniform(0.1, 0.7))
+        man = float(rng.uniform(0.015, 0.04))
+        nsc = int(rng.integers(1, 4))
+        steps = int(rng.integers(steps_lo, steps_hi + 1))
+        sseed = int(rng.integers(1 << 30))
+
+        def build(side=side, hurst=hurst, amp=amp, man=man, nsc=nsc, steps=steps, sseed=sseed):
+            bed = fbm_surface(side, side, hurst, np.random.default_rng(sseed), -1.5, 1.0)
+            cfg = default_config(side, side, dx, dt, steps, manningN=man, nScalar=nsc, eps=1e-10)
+            return _finish(cfg, bed, tide_series(steps, dt, amp),
+                           scalar0=np.zeros(nsc * side * side), name=f"req{side}")
+        out.append((float(t[q]), steps, side, build))
+    return out

@@ -1,0 +1,214 @@
+/*
+ * frehd_gpu.h -- C ABI of the MI355X (gfx950) semi-implicit free-surface step.
+ *
+ * The library replaces one Casulli time step of FrehdC v4.2, i.e. the body of
+ * the time loop in FrehdC_v42/nssolve.c:61-104 (SolveAll):
+ *
+ *     oneCompleteStep(...)   nssolve.c:110-118  (solveSourceTerm, solveFreeSurface,
+ *                                                updateData, scalarTransport)
+ *     monitorCFL(...)        nsfunctions.c:1192-1232
+ *
+ * State lives on the device between calls and keeps the reference's flat
+ * array layout (FrehdC_v42/map.c:15-128): interior cell ii = j*nx + i for
+ * 0 <= ii < N = nx*ny, followed by the ghost blocks
+ *     [N, N+nx)              jP ghosts (top row, j = ny)
+ *     [N+nx, N+2nx)          jM ghosts (bottom row, j = -1)
+ *     [N+2nx, N+2nx+ny)      iP ghosts (right column, i = nx)
+ *     [N+2nx+ny, N+2nx+2ny)  iM ghosts (left column, i = -1)
+ *     Ng-4 .. Ng-1           corners iMjP, iPjP, iPjM, iMjM
+ * with Ng = (nx+2)(ny+2).  Face arrays (uuXP, depthXP, ...) hold the i+1/2
+ * (X) or j+1/2 (Y) face of the cell with the same index.
+ *
+ * Every entry point returns 0 on success or a negative FREHD_E* code; the
+ * message of the last failure on the calling thread is frehd_last_error().
+ * Handles are not thread-safe: drive one handle from one host thread.
+ */
+#ifndef FREHD_GPU_H
+#define FREHD_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes -------------------------------------------------------- */
+#define FREHD_OK        0
+#define FREHD_EINVAL   (-1)  /* bad argument or unsupported configuration      */
+#define FREHD_EHIP     (-2)  /* a HIP runtime call failed                       */
+#define FREHD_ENOMEM   (-3)  /* device or host allocation failed               */
+#define FREHD_ECFL     (-4)  /* max CFL > 20 after a step; the reference prints
+                                a warning and calls exit(EXIT_FAILURE) instead
+                                (nsfunctions.c:1226-1231)                       */
+#define FREHD_EBUDGET  (-5)  /* serving: request exceeds the HBM budget         */
+#define FREHD_ESTATE   (-6)  /* call not valid in the handle's current state    */
+
+/* ---- solver and precision selectors --------------------------------------- */
+#define FREHD_PRECOND_JACOBI 0   /* laspack/precond.c:26-46 (JacobiPrecond)       */
+#define FREHD_PRECOND_SSOR   1   /* laspack/precond.c:48-73; sequential, so the
+                                    device path rejects it (the CPU oracle has it) */
+
+#define FREHD_PREC_FP64  0       /* every array and operation in FP64             */
+#define FREHD_PREC_MIXED 1       /* FP32 stencils / PCG, FP64 refinement, FP64
+                                    dot products, eta and volume accumulation      */
+
+/* ---- configuration ----------------------------------------------------------
+ * Scalar fields of the reference Config (FrehdC_v42/configuration.h:4-111) that
+ * the step reads, with the reference's names.  One strip only (np = 1), so
+ * nx = NX and ny = NY.  Fields the reference never assigns (useEvap, useRain,
+ * useSubStep, checkConservation, inflow_rank) are fixed at 0. */
+typedef struct frehd_config {
+    int32_t nx, ny;          /* interior cells in x and y                        */
+    int32_t Nt;              /* steps covered by the forcing series               */
+    int32_t maxIter;         /* PCG iteration cap                                 */
+    double  dx, dy, dt;      /* cell size [m], time step [s]                      */
+    double  g;               /* gravity                                           */
+    double  NUx, NUy;        /* eddy viscosity                                    */
+    double  CDx, CDy;        /* drag coefficient when CDnotN == 1                 */
+    double  manningN;        /* Manning n when CDnotN == 0                        */
+    int32_t CDnotN;          /* 0: Manning (updateCD), 1: constant CD             */
+    int32_t useThinLayer;    /* thinLayerDrag each step (nsfunctions.c:691)       */
+    double  hD, CDmax;       /* thin-layer height (Config.hD) and CD cap          */
+    double  minDepth;        /* limiterCFL shallow-cell threshold                 */
+    double  wtfh;            /* waterfall depth threshold                         */
+    double  CFLl, CFLh;      /* advection CFL limiter bounds                      */
+    double  eps;             /* relative residual stop: |r| < eps |b|             */
+    int32_t precond;         /* FREHD_PRECOND_*                                   */
+    int32_t precision;       /* FREHD_PREC_*                                      */
+    int32_t bcType;          /* 1: tide on YP, 2: tide on YM, 3: both             */
+    int32_t tideLocLengthP;  /* number of YP tidal cells                          */
+    int32_t tideLocLengthM;  /* number of YM tidal cells                          */
+    int32_t inflowLocLength; /* number of inflow cells                            */
+    int32_t useScalar;       /* scalar transport on/off                           */
+    int32_t scalarAdv;       /* 1: first-order upwind (the only device scheme)    */
+    int32_t useConstTidePS, useConstTideMS;
+    int32_t nScalar;         /* scalars carried (>= 1 when useScalar)             */
+    double  tidePS, tideMS;  /* constant tidal scalar values                      */
+    double  inflowS;         /* inflow scalar concentration                       */
+    double  Kx, Ky;          /* scalar diffusivity                                */
+    double  offset;          /* Bath.offset[0]: -min(bed) when the bed dips below 0 */
+    /* refinement controls for FREHD_PREC_MIXED (ignored for FP64)               */
+    double  epsInner;        /* FP32 inner PCG relative tolerance                 */
+    int32_t maxRefine;       /* FP64 refinement sweeps cap                        */
+    int32_t reserved0;
+} frehd_config;
+
+/* ---- forcing ----------------------------------------------------------------
+ * Boundary series hold Nt+1 samples; sample tt is the forcing at step tt
+ * (tNStart + tt*dt), the index the reference uses (nsfunctions.c:386, 921).
+ * tideLocP/M and inflowLoc are interior cell indices (configuration.c:56-60,
+ * 143-150).  manningCell is an optional per-cell Manning n (N values); NULL
+ * means the uniform manningN. */
+typedef struct frehd_forcing {
+    const int32_t *tideLocP, *tideLocM, *inflowLoc;
+    const double  *tideP, *tideM, *inflow;
+    const double  *tidalPS, *tidalMS;   /* used when useConstTide?S == 0          */
+    const double  *manningCell;
+} frehd_forcing;
+
+/* ---- initial condition (the inputs of Init, initialize.c:40-155) ------------
+ * bottomZ is the raw bed elevation of the N interior cells as in bath.dat;
+ * the offset is cfg->offset.  A per-cell initial field overrides the
+ * constant when its pointer is not NULL.  S0cell holds nScalar*N values. */
+typedef struct frehd_ic {
+    const double *bottomZ;
+    double surf0, U0, V0, S0;
+    const double *surf0cell, *U0cell, *V0cell, *S0cell;
+} frehd_ic;
+
+/* ---- state arrays (Data + Bath, initialize.h:10-20, bathymetry.h:6-14) -------
+ * For frehd_dev_download every non-NULL pointer receives that array.  Arrays
+ * marked Ng hold the full ghosted layout; the others hold N interior values.
+ * S and Sm hold nScalar consecutive arrays. */
+typedef struct frehd_fields {
+    double *surf, *surfOld;            /* Ng */
+    double *uuXP, *vvYP;               /* Ng */
+    double *uuYP, *vvXP;               /* N  */
+    double *depth;                     /* N  */
+    double *depthXP, *depthYP;         /* Ng */
+    double *EnXP, *EnYP;               /* Ng */
+    double *DragXP, *DragYP;           /* Ng */
+    double *CDXP, *CDYP;               /* Ng */
+    double *Fuu, *Fvv;                 /* Ng */
+    double *cellV;                     /* N  */
+    double *S;                         /* nScalar x Ng */
+    double *Sm;                        /* nScalar x N  */
+    double *CFLx, *CFLy;               /* Ng (interior written) */
+    double *wtfXP, *wtfYP;             /* Ng */
+    double *bottomZ, *bottomXP, *bottomYP; /* Ng */
+} frehd_fields;
+
+/* ---- per-handle statistics -------------------------------------------------- */
+typedef struct frehd_stats {
+    int64_t steps;            /* steps taken since create                         */
+    int64_t pcgIterTotal;     /* inner PCG iterations summed over all steps       */
+    int32_t pcgIterLast;      /* inner iterations of the last step                */
+    int32_t refineLast;       /* FP64 refinement sweeps of the last step (mixed)  */
+    double  relResLast;       /* |r|/|b| at exit of the last solve                */
+    double  maxCFLx, maxCFLy; /* from the last monitorCFL                         */
+    double  Vloss[4];         /* Vloss1..4 accumulators (nsfunctions.c:446-486,1147) */
+} frehd_stats;
+
+typedef struct frehd_dev frehd_dev_t;
+
+/* Last error message on the calling thread ("" when none). */
+const char *frehd_last_error(void);
+
+/* Library version string. */
+const char *frehd_version(void);
+
+/* Number of visible devices (0 when none). */
+int frehd_device_count(int32_t *count);
+
+/* sizeof of frehd_config, frehd_forcing, frehd_ic, frehd_fields, frehd_stats
+ * as compiled into the library (binding checks). */
+int frehd_abi_sizes(int64_t sizes[5]);
+
+/* Create a device handle from raw inputs and run the Init sequence
+ * (initialize.c:40-155: bathymetry ghosts, face elevations, initial fields,
+ * boundary values, depths, volume, interpolated velocities, thin-layer drag,
+ * scalar mass) on the device.  Replaces Init + the per-step LASpack objects
+ * that SolveAll builds (nssolve.c:72-85). */
+int frehd_dev_create(const frehd_config *cfg, const frehd_forcing *forcing,
+                     const frehd_ic *ic, frehd_dev_t **out);
+
+/* Create a handle from a state that the caller already initialised (the
+ * reference's own Init output): every array of `state` must be non-NULL.
+ * This is the drop-in form: state produced by FrehdC's Init is uploaded once. */
+int frehd_dev_create_from_state(const frehd_config *cfg, const frehd_forcing *forcing,
+                                const frehd_fields *state, frehd_dev_t **out);
+
+/* Advance nsteps steps.  Each step is oneCompleteStep (nssolve.c:110-118)
+ * followed by monitorCFL (nsfunctions.c:1192).  Step numbers continue from
+ * the previous call (the first step is tt = 1).  Returns FREHD_ECFL, leaving
+ * the state after the offending step, where the reference would exit. */
+int frehd_dev_step(frehd_dev_t *dev, int32_t nsteps);
+
+/* Copy the device state into the non-NULL arrays of `out` (DataOutput's
+ * view, fileio.c:62-180, at full precision). */
+int frehd_dev_download(frehd_dev_t *dev, frehd_fields *out);
+
+/* Statistics of the handle. */
+int frehd_dev_stats(frehd_dev_t *dev, frehd_stats *out);
+
+/* Current step counter (number of steps taken). */
+int frehd_dev_step_index(frehd_dev_t *dev, int32_t *tt);
+
+/* Release the handle and its device memory. */
+int frehd_dev_destroy(frehd_dev_t *dev);
+
+/* Device bytes a handle of this configuration occupies. */
+int frehd_dev_footprint(const frehd_config *cfg, int64_t *bytes);
+
+/* Mean device time per launch of each kernel of the last frehd_dev_step call,
+ * measured with HIP events on the handle's stream when timing is enabled.
+ * names/ms/launches receive up to `cap` entries; *n gets the count. */
+int frehd_dev_enable_timing(frehd_dev_t *dev, int32_t on);
+int frehd_dev_kernel_times(frehd_dev_t *dev, int32_t cap, const char **names,
+                           double *ms, int64_t *launches, int32_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FREHD_GPU_H */

@@ -1,0 +1,924 @@
+/*
+ * frehd_oracle.c -- CPU restatement of one FrehdC v4.2 Casulli time step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see frehd_oracle.h): the checker for the device
+ * library, never part of it.  Build: oracle/Makefile (gcc -O2
+ * -ffp-contract=off, so no fused multiply-add changes a rounding).
+ *
+ * Scope: the plain-grid path (useSubgrid = 0) with first-order upwind scalar
+ * transport, Manning or constant drag, optional thin-layer drag, no wind, no
+ * evaporation/rain, one strip (np = 1, irank = 0 = nrank - 1).
+ *
+ * Order of floating-point operations follows the reference expression by
+ * expression (C evaluates a*b*c as (a*b)*c and a+b+c as (a+b)+c), so the
+ * result matches the reference bit for bit; tests/test_oracle_vs_reference.py
+ * checks that against the reference's own functions compiled into
+ * oracle/_ref.  Where the reference reads outside an array (SURVEY-listed
+ * quirks), the value read is defined here as 0.0 and the reference harness
+ * pads every array with zeros so both agree:
+ *   - velocityInterp reads uuXP[ii-1] at ii = 0 and vvYP[ii-nx] for ii < nx
+ *     (nsfunctions.c:342-345);
+ *   - limiterCFL reads depth[] at ghost indices although depth holds N values
+ *     (nsfunctions.c:462-465); depth at a ghost index is 0 here.
+ * CFLx/CFLy are read by the first step's advectionTerm before monitorCFL ever
+ * wrote them (uninitialised malloc in initialize.c:222-223): 0 here.
+ *
+ * Parity pins: every function below is pinned by the reference harness
+ * comparison (bit-exact per phase and over multi-step runs).  The CG is
+ * additionally pinned by a dense solve of the same 5-point system
+ * (tests/test_oracle_math.py); the Jacobi variant is pinned the same way.
+ */
+#include "frehd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* LASpack's tolerance tests (laspack/elcmp.h:31-32). */
+#define LAS_IS_ZERO(a) (fabs(a) < 10.0 * 2.2250738585072014e-308)
+#define LAS_IS_ONE(a)  (fabs((a) - 1.0) < 10.0 * 2.220446049250313e-16)
+
+struct fo_state {
+    frehd_config cfg;
+    int nx, ny, N, Ng;
+    int tt;
+    /* forcing (owned copies) */
+    int32_t *tideLocP, *tideLocM, *inflowLoc;
+    double *tideP, *tideM, *inflow, *tidalPS, *tidalMS, *manningCell;
+    /* Ng arrays */
+    double *surf, *surfOld, *uuXP, *vvYP, *Fuu, *Fvv, *depthXP, *depthYP;
+    double *EnXP, *EnYP, *DragXP, *DragYP, *CDXP, *CDYP, *wtfXP, *wtfYP;
+    double *S, *CFLx, *CFLy, *advX, *advY, *bottomZ, *bottomXP, *bottomYP;
+    /* N arrays (allocated Ng so ghost-index reads are defined zeros) */
+    double *uuYP, *vvXP, *depth, *cellV, *Sm, *rmvCFL;
+    double *GnCt, *GnXP, *GnXM, *GnYP, *GnYM, *z;
+    double Vloss[4];
+    int pcgIters;
+    long long pcgItersTotal;
+    double relres;
+};
+
+/* ---- grid index helpers (map.c:15-128) ------------------------------------ */
+static inline int ci(const fo_state *s, int c) { return c % s->nx; }
+static inline int cj(const fo_state *s, int c) { return c / s->nx; }
+static inline int gjP(const fo_state *s, int i) { return s->N + i; }
+static inline int gjM(const fo_state *s, int i) { return s->N + s->nx + i; }
+static inline int giP(const fo_state *s, int j) { return s->N + 2 * s->nx + j; }
+static inline int giM(const fo_state *s, int j) { return s->N + 2 * s->nx + s->ny + j; }
+static inline int nbP(const fo_state *s, int c) { int i = ci(s, c); return i == s->nx - 1 ? giP(s, cj(s, c)) : c + 1; }
+static inline int nbM(const fo_state *s, int c) { int i = ci(s, c); return i == 0 ? giM(s, cj(s, c)) : c - 1; }
+static inline int nbN(const fo_state *s, int c) { int j = cj(s, c); return j == s->ny - 1 ? gjP(s, ci(s, c)) : c + s->nx; }
+static inline int nbS(const fo_state *s, int c) { int j = cj(s, c); return j == 0 ? gjM(s, ci(s, c)) : c - s->nx; }
+/* corners, map.c:101-104 */
+static inline int cornerIMJP(const fo_state *s) { return s->Ng - 4; }
+static inline int cornerIPJP(const fo_state *s) { return s->Ng - 3; }
+static inline int cornerIPJM(const fo_state *s) { return s->Ng - 2; }
+static inline int cornerIMJM(const fo_state *s) { return s->Ng - 1; }
+/* flat read with a zero for indices below the array (velocityInterp quirk) */
+static inline double rd0(const double *a, long idx) { return idx < 0 ? 0.0 : a[idx]; }
+
+static double *zalloc(int n) { return (double *)calloc((size_t)n, sizeof(double)); }
+
+/* ---- boundary fills ------------------------------------------------------- */
+/* enforceFreeSurfaceBC, nsfunctions.c:317-334 */
+static void fill_surface_ghosts(fo_state *s, double *a)
+{
+    const int nx = s->nx, ny = s->ny, N = s->N;
+    for (int i = 0; i < nx; i++) { a[gjP(s, i)] = a[N - nx + i]; a[gjM(s, i)] = a[i]; }
+    for (int j = 0; j < ny; j++) { a[giP(s, j)] = a[j * nx + nx - 1]; a[giM(s, j)] = a[j * nx]; }
+    a[cornerIPJP(s)] = a[giP(s, ny - 1)];
+    a[cornerIPJM(s)] = a[giM(s, ny - 1)];
+    a[cornerIMJP(s)] = a[giP(s, 0)];
+    a[cornerIMJM(s)] = a[giM(s, 0)];
+}
+
+/* enforceVelocityBC, nsfunctions.c:289-314 */
+static void fill_velocity_ghosts(fo_state *s)
+{
+    const int nx = s->nx, ny = s->ny, N = s->N;
+    double *u = s->uuXP, *v = s->vvYP;
+    for (int i = 0; i < nx; i++) {
+        u[gjP(s, i)] = u[N - nx + i]; u[gjM(s, i)] = u[i];
+        v[gjP(s, i)] = v[N - nx + i]; v[gjM(s, i)] = v[i];
+    }
+    for (int j = 0; j < ny; j++) {
+        u[giP(s, j)] = u[j * nx + nx - 1]; u[giM(s, j)] = u[j * nx];
+        v[giP(s, j)] = v[j * nx + nx - 1]; v[giM(s, j)] = v[j * nx];
+    }
+    u[cornerIPJP(s)] = u[gjP(s, nx - 1)];
+    u[cornerIPJM(s)] = u[gjM(s, nx - 1)];
+    u[cornerIMJP(s)] = u[gjP(s, 0)];
+    u[cornerIMJM(s)] = u[gjM(s, 0)];
+    v[cornerIPJP(s)] = v[giP(s, ny - 1)];
+    v[cornerIPJM(s)] = v[giM(s, ny - 1)];
+    v[cornerIMJP(s)] = v[giP(s, 0)];
+    v[cornerIMJM(s)] = v[giM(s, 0)];
+}
+
+/* ---- Init pieces ----------------------------------------------------------- */
+/* enforceBathBC (initialize.c:158-175) + updateAllFaceElev for useCellEdge = 0,
+   useSubgrid = 0 (nsfunctions.c:76-140) */
+static void init_bathymetry(fo_state *s)
+{
+    const int nx = s->nx, ny = s->ny, N = s->N;
+    double *zb = s->bottomZ, *bx = s->bottomXP, *by = s->bottomYP;
+    for (int i = 0; i < nx; i++) { zb[gjM(s, i)] = zb[i]; zb[gjP(s, i)] = zb[N - nx + i]; }
+    for (int j = 0; j < ny; j++) { zb[giM(s, j)] = zb[j * nx]; zb[giP(s, j)] = zb[j * nx + nx - 1]; }
+    for (int c = 0; c < s->Ng; c++) { bx[c] = 0.0; by[c] = 0.0; }
+    for (int c = 0; c < N; c++) { double a = zb[c], b = zb[nbP(s, c)]; bx[c] = a > b ? a : b; }
+    for (int j = 0; j < ny; j++) {
+        bx[giM(s, j)] = zb[j * nx];
+        bx[giP(s, j)] = bx[j * nx + nx - 1];
+    }
+    for (int i = 0; i < nx; i++) { bx[gjM(s, i)] = bx[i]; bx[gjP(s, i)] = bx[N - nx + i]; }
+    for (int c = 0; c < N; c++) { double a = zb[c], b = zb[nbN(s, c)]; by[c] = a > b ? a : b; }
+    for (int i = 0; i < nx; i++) {
+        double a = zb[gjM(s, i)], b = zb[i];
+        by[gjM(s, i)] = a > b ? a : b;
+        by[gjP(s, i)] = by[N - nx + i];
+    }
+    for (int j = 0; j < ny; j++) { by[giM(s, j)] = by[j * nx]; by[giP(s, j)] = by[j * nx + nx - 1]; }
+}
+
+/* ---- shared pieces of Init and updateData ---------------------------------- */
+/* zeroNegSurf, nsfunctions.c:436-449 */
+static void zero_neg_surf(fo_state *s, int tt)
+{
+    for (int c = 0; c < s->N; c++) {
+        if (s->surf[c] < s->bottomZ[c]) {
+            double diff = s->bottomZ[c] - s->surf[c];
+            s->surf[c] = s->bottomZ[c];
+            if (tt > 1) s->Vloss[2] -= diff * s->cfg.dx * s->cfg.dy;
+        }
+    }
+}
+
+/* enforceTidalBC, nsfunctions.c:412-433 */
+static void tidal_surface(fo_state *s, int tt)
+{
+    const double off = s->cfg.offset;
+    if (s->cfg.bcType != 2)
+        for (int k = 0; k < s->cfg.tideLocLengthP; k++) {
+            int c = s->tideLocP[k];
+            if (s->tideP[tt] + off > s->bottomZ[c]) s->surf[c] = s->tideP[tt] + off;
+        }
+    if (s->cfg.bcType != 1)
+        for (int k = 0; k < s->cfg.tideLocLengthM; k++) {
+            int c = s->tideLocM[k];
+            if (s->tideM[tt] + off > s->bottomZ[c]) s->surf[c] = s->tideM[tt] + off;
+        }
+}
+
+/* updateDepth, nsfunctions.c:239-247 */
+static void update_depth(fo_state *s)
+{
+    for (int c = 0; c < s->N; c++) {
+        double d = s->surf[c] - s->bottomZ[c];
+        s->depth[c] = d < 0 ? 0 : d;
+    }
+}
+
+/* updateAllFaceDepth, nsfunctions.c:143-236 (irank = 0 = nrank - 1) */
+static void update_face_depth(fo_state *s)
+{
+    const int nx = s->nx, ny = s->ny, N = s->N;
+    double *h = s->surf, *dX = s->depthXP, *dY = s->depthYP;
+    const double *bX = s->bottomXP, *bY = s->bottomYP;
+    for (int c = 0; c < s->Ng; c++) { dX[c] = 0; dY[c] = 0; }
+    for (int c = 0; c < N; c++) {
+        double a = h[c], b = h[nbP(s, c)];
+        dX[c] = (a > b ? a : b) - bX[c];
+    }
+    for (int j = 0; j < ny; j++) {
+        dX[giM(s, j)] = h[j * nx] - bX[giM(s, j)];
+        dX[giP(s, j)] = h[giP(s, j)] - bX[giP(s, j)];
+        dX[j * nx + nx - 1] = 0;
+        dX[giP(s, j)] = 0;
+        dX[giM(s, j)] = 0;
+    }
+    for (int i = 0; i < nx; i++) {
+        int gm = gjM(s, i), gp = gjP(s, i);
+        dX[gm] = (h[gm] > h[gm + 1] ? h[gm] : h[gm + 1]) - bX[gm];
+        dX[gp] = (h[gp] > h[gp + 1] ? h[gp] : h[gp + 1]) - bX[gp];
+    }
+    for (int c = 0; c < N; c++) {
+        double a = h[c], b = h[nbN(s, c)];
+        dY[c] = (a > b ? a : b) - bY[c];
+    }
+    for (int i = 0; i < nx; i++) {
+        int gm = gjM(s, i), gp = gjP(s, i);
+        dY[gm] = (h[gm] > h[i] ? h[gm] : h[i]) - bY[gm];
+        dY[gp] = h[gp] - bY[gp];
+        dY[gm] = 0;
+    }
+    for (int j = 0; j < ny; j++) { dY[giM(s, j)] = dY[j * nx]; dY[giP(s, j)] = dY[j * nx + nx - 1]; }
+    for (int i = 0; i < nx; i++) { dY[N - nx + i] = 0; dY[gjP(s, i)] = 0; }
+    for (int i = 0; i < nx; i++) dY[gjM(s, i)] = 0;
+    for (int c = 0; c < s->Ng; c++) { if (dX[c] < 0) dX[c] = 0; if (dY[c] < 0) dY[c] = 0; }
+}
+
+/* computeVolume, nsfunctions.c:361-370 */
+static void compute_volume(fo_state *s)
+{
+    for (int c = 0; c < s->N; c++) {
+        s->cellV[c] = s->cfg.dx * s->cfg.dy * s->depth[c];
+        if (s->cfg.useScalar == 1)
+            for (int q = 0; q < s->cfg.nScalar; q++)
+                s->Sm[q * s->Ng + c] = s->S[q * s->Ng + c] * s->cellV[c];
+    }
+}
+
+/* velocityInterp, nsfunctions.c:337-347 (flat-index neighbours) */
+static void velocity_interp(fo_state *s)
+{
+    const long nx = s->nx;
+    for (long c = 0; c < s->N; c++) {
+        s->uuYP[c] = 0.25 * (s->uuXP[c] + rd0(s->uuXP, c - 1) + s->uuXP[c + nx] + rd0(s->uuXP, c - 1 + nx));
+        s->vvXP[c] = 0.25 * (s->vvYP[c] + s->vvYP[c + 1] + rd0(s->vvYP, c - nx) + rd0(s->vvYP, c + 1 - nx));
+    }
+}
+
+static inline double manning_coeff(const fo_state *s, int c)
+{
+    double n = s->manningCell ? s->manningCell[c] : s->cfg.manningN;
+    return s->cfg.g * n * n;
+}
+
+/* thinLayerDrag, nsfunctions.c:691-727 */
+static void thin_layer_drag(fo_state *s)
+{
+    const frehd_config *k = &s->cfg;
+    if (k->CDnotN == 1) {
+        for (int c = 0; c < s->Ng; c++) {
+            if (s->depthXP[c] <= k->hD)
+                s->CDXP[c] = k->CDx + (k->CDmax - k->CDx) * (k->hD - s->depthXP[c]) / k->hD;
+            if (s->depthYP[c] <= k->hD)
+                s->CDYP[c] = k->CDy + (k->CDmax - k->CDy) * (k->hD - s->depthYP[c]) / k->hD;
+        }
+    } else {
+        for (int c = 0; c < s->N; c++) {
+            if (s->depth[c] > 0) {
+                double coeff = manning_coeff(s, c);
+                if (s->depthXP[c] <= k->hD) s->CDXP[c] = coeff / pow(s->depth[c], 2.0 / 3.0);
+                if (s->depthYP[c] <= k->hD) s->CDYP[c] = coeff / pow(s->depth[c], 2.0 / 3.0);
+            }
+        }
+    }
+}
+
+/* ---- Phase P: solveSourceTerm (nssolve.c:121-155), plain grid -------------- */
+static void phase_source(fo_state *s)
+{
+    const frehd_config *k = &s->cfg;
+    const int N = s->N;
+    const double dx = k->dx, dy = k->dy, dt = k->dt;
+    double *u = s->uuXP, *v = s->vvYP, *uY = s->uuYP, *vX = s->vvXP;
+    double *aX = s->advX, *aY = s->advY;
+
+    /* explicitVelocity, nsfunctions.c:498-507 */
+    for (int c = 0; c < N; c++) { s->EnXP[c] = u[c]; s->EnYP[c] = v[c]; }
+
+    /* advectionTerm, nsfunctions.c:510-586: first-order upwind */
+    for (int c = 0; c < N; c++) {
+        int e = nbP(s, c), w = nbM(s, c), n = nbN(s, c), so = nbS(s, c);
+        double uc = u[c], vx = vX[c];
+        aX[c] = (0.5 / dx) * (uc + fabs(uc)) * (uc - u[w])
+              + (0.5 / dx) * (uc - fabs(uc)) * (u[e] - uc)
+              + (0.5 / dy) * (vx + fabs(vx)) * (uc - u[so])
+              + (0.5 / dy) * (vx - fabs(vx)) * (u[n] - uc);
+        double vc = v[c], uy = uY[c];
+        aY[c] = (0.5 / dx) * (uy + fabs(uy)) * (vc - v[w])
+              + (0.5 / dx) * (uy - fabs(uy)) * (v[e] - vc)
+              + (0.5 / dy) * (vc + fabs(vc)) * (vc - v[so])
+              + (0.5 / dy) * (vc - fabs(vc)) * (v[n] - vc);
+        if (uc == 0) aX[c] = 0;
+        if (vc == 0) aY[c] = 0;
+    }
+    /* CFL limiter applied in place, cell by cell in increasing index: a cell
+       rescales its own term and its four neighbours' (nsfunctions.c:536-585);
+       the y branch's zeroing tests CFLx (nsfunctions.c:574) */
+    const double lo = k->CFLl, hi = k->CFLh;
+    for (int c = 0; c < N; c++) {
+        int t[5] = {c, nbP(s, c), nbM(s, c), nbN(s, c), nbS(s, c)};
+        double cx = s->CFLx[c], cy = s->CFLy[c];
+        if (cx > lo && cx <= hi) {
+            for (int q = 0; q < 5; q++) aX[t[q]] = aX[t[q]] * (hi - s->CFLx[t[q]]) / (hi - lo);
+        } else if (cx > hi) {
+            for (int q = 0; q < 5; q++) aX[t[q]] = 0;
+        }
+        if (cy > lo && cy < hi) {
+            for (int q = 0; q < 5; q++) aY[t[q]] = aY[t[q]] * (hi - s->CFLy[t[q]]) / (hi - lo);
+        } else if (cx > hi) {
+            for (int q = 0; q < 5; q++) aY[t[q]] = 0;
+        }
+        s->EnXP[c] = s->EnXP[c] - dt * aX[c];
+        s->EnYP[c] = s->EnYP[c] - dt * aY[c];
+    }
+
+    /* diffusionTerm, nsfunctions.c:589-606 */
+    for (int c = 0; c < N; c++) {
+        int e = nbP(s, c), w = nbM(s, c), n = nbN(s, c), so = nbS(s, c);
+        double dfx = (k->NUx / (dx * dx)) * (u[e] - 2 * u[c] + u[w])
+                   + (k->NUy / (dy * dy)) * (u[so] - 2 * u[c] + u[n]);
+        double dfy = (k->NUx / (dx * dx)) * (v[e] - 2 * v[c] + v[w])
+                   + (k->NUy / (dy * dy)) * (v[so] - 2 * v[c] + v[n]);
+        s->EnXP[c] = s->EnXP[c] + dt * dfx;
+        s->EnYP[c] = s->EnYP[c] + dt * dfy;
+    }
+
+    /* windTerm: useWind = 0 (nsfunctions.c:609-648 not entered) */
+
+    /* dragTerm, nsfunctions.c:651-688 */
+    for (int c = 0; c < N; c++) {
+        double velx = sqrt(u[c] * u[c] + vX[c] * vX[c]);
+        double vely = sqrt(uY[c] * uY[c] + v[c] * v[c]);
+        s->DragXP[c] = s->depthXP[c] > 0 ? 0.5 * dt * s->CDXP[c] * velx / s->depthXP[c] : 0;
+        s->DragYP[c] = s->depthYP[c] > 0 ? 0.5 * dt * s->CDYP[c] * vely / s->depthYP[c] : 0;
+    }
+    for (int i = 0; i < s->nx; i++) s->DragXP[gjM(s, i)] = s->DragXP[i];
+    for (int j = 0; j < s->ny; j++) s->DragYP[giM(s, j)] = s->DragYP[j * s->nx];
+
+    /* dragInversion, nsfunctions.c:730-746 */
+    for (int c = 0; c < N; c++) {
+        s->EnXP[c] = s->EnXP[c] / (1 + s->DragXP[c]);
+        s->EnYP[c] = s->EnYP[c] / (1 + s->DragYP[c]);
+    }
+    for (int i = 0; i < s->nx; i++) s->EnXP[gjM(s, i)] = s->EnXP[i];
+    for (int j = 0; j < s->ny; j++) s->EnYP[giM(s, j)] = s->EnYP[j * s->nx];
+
+    /* matrixSourceTerm, nsfunctions.c:767-797; z kept in cell order here
+       (the reference stores it in the transposed solver order) */
+    for (int c = 0; c < N; c++) {
+        int w = nbM(s, c), so = nbS(s, c);
+        s->z[c] = s->surf[c]
+                - (dt / dx) * (s->EnXP[c] * s->depthXP[c] - s->EnXP[w] * s->depthXP[w])
+                - (dt / dy) * (s->EnYP[c] * s->depthYP[c] - s->EnYP[so] * s->depthYP[so]);
+    }
+    for (int q = 0; q < k->inflowLocLength; q++) {
+        int c = s->inflowLoc[q];
+        s->z[c] = s->z[c] + (dt / (dx * dy)) * (s->inflow[s->tt] / k->inflowLocLength);
+    }
+}
+
+/* ---- the LASpack CG (itersolv.c:355-456) in the transposed row order ------- */
+typedef struct {
+    int nx, ny, N;
+    /* row k (k = i*ny + j, cell j*nx + i): coefficients of columns
+       k-ny (W), k-1 (S), k (C), k+1 (N), k+ny (E); absent entries are flagged */
+    const double *Ct, *XP, *XM, *YP, *YM;   /* cell order */
+} fo_mat;
+
+static inline int trps_cell(const fo_mat *A, int k) { return (k % A->ny) * A->nx + k / A->ny; }
+
+/* q = A p, row-wise, entries in increasing column (operats.c:870-888) */
+static void mat_vec(const fo_mat *A, const double *p, double *q)
+{
+    const int ny = A->ny, N = A->N;
+    for (int k = 0; k < N; k++) {
+        int c = trps_cell(A, k), i = k / ny, j = k % ny;
+        double sum = 0.0;
+        if (i > 0) sum += (-A->XM[c]) * p[k - ny];
+        if (j > 0) sum += (-A->YM[c]) * p[k - 1];
+        sum += A->Ct[c] * p[k];
+        if (j < ny - 1) sum += (-A->YP[c]) * p[k + 1];
+        if (i < A->nx - 1) sum += (-A->XP[c]) * p[k + ny];
+        q[k] = sum;
+    }
+}
+
+/* SSOR with omega = 1 (precond.c:48-73): forward solve with D+L
+   (operats.c:1136-1160), multiply by D, backward solve with D+U
+   (operats.c:1081-1105); inverse diagonal cached as 1/diag (qmatrix.c:466) */
+static void ssor(const fo_mat *A, const double *invd, const double *r, double *y, double *t)
+{
+    const int ny = A->ny, N = A->N;
+    for (int k = 0; k < N; k++) {
+        int c = trps_cell(A, k), i = k / ny, j = k % ny;
+        double sum = 0.0;
+        if (i > 0) sum -= (-A->XM[c]) * t[k - ny];
+        if (j > 0) sum -= (-A->YM[c]) * t[k - 1];
+        sum += r[k];
+        t[k] = sum * invd[k];
+    }
+    for (int k = 0; k < N; k++) {
+        int c = trps_cell(A, k);
+        double sum = 0.0;
+        sum += A->Ct[c] * t[k];
+        t[k] = sum;
+    }
+    for (int k = N - 1; k >= 0; k--) {
+        int c = trps_cell(A, k), i = k / ny, j = k % ny;
+        double sum = 0.0;
+        if (i < A->nx - 1) sum -= (-A->XP[c]) * y[k + ny];
+        if (j < ny - 1) sum -= (-A->YP[c]) * y[k + 1];
+        sum += t[k];
+        y[k] = sum * invd[k];
+    }
+}
+
+static double dot(const double *a, const double *b, int n)
+{
+    double s = 0.0;
+    for (int k = 0; k < n; k++) s += a[k] * b[k];
+    return s;
+}
+
+static double l2norm(const double *a, int n)
+{
+    double s = 0.0;
+    for (int k = 0; k < n; k++) { double c = a[k]; s += c * c; }
+    return sqrt(s);
+}
+
+/* PCG from x = 0 (solveMatrix, nsfunctions.c:1004-1017 + CGIter) */
+static int cg_solve(const fo_mat *A, const double *b, double *x, double eps, int maxIter,
+                    int precond, double *relres)
+{
+    const int N = A->N;
+    double *r = zalloc(N), *p = zalloc(N), *q = zalloc(N), *zz = zalloc(N), *t = zalloc(N);
+    double *invd = zalloc(N);
+    for (int k = 0; k < N; k++) invd[k] = 1.0 / A->Ct[trps_cell(A, k)];
+    for (int k = 0; k < N; k++) x[k] = 0.0;
+    double bNorm = l2norm(b, N);
+    memcpy(r, b, sizeof(double) * (size_t)N);   /* x = 0: r = b (itersolv.c:395-402) */
+    int iter = 0;
+    double rho, rhoOld = 0.0, alpha, beta, rNorm;
+    for (;;) {
+        rNorm = l2norm(r, N);
+        if (rNorm < eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm))) break;
+        if (iter >= maxIter) break;
+        iter++;
+        if (precond == FREHD_PRECOND_SSOR) ssor(A, invd, r, zz, t);
+        else for (int k = 0; k < N; k++) zz[k] = r[k] * invd[k];
+        rho = dot(r, zz, N);
+        if (iter == 1) {
+            memcpy(p, zz, sizeof(double) * (size_t)N);
+        } else {
+            beta = rho / rhoOld;
+            if (LAS_IS_ONE(beta)) for (int k = 0; k < N; k++) p[k] = zz[k] + p[k];
+            else for (int k = 0; k < N; k++) p[k] = zz[k] + beta * p[k];
+        }
+        mat_vec(A, p, q);
+        alpha = rho / dot(p, q, N);
+        if (LAS_IS_ONE(alpha)) {
+            for (int k = 0; k < N; k++) x[k] += p[k];
+            for (int k = 0; k < N; k++) r[k] -= q[k];
+        } else {
+            for (int k = 0; k < N; k++) x[k] += alpha * p[k];
+            for (int k = 0; k < N; k++) r[k] -= alpha * q[k];
+        }
+        rhoOld = rho;
+    }
+    if (relres) *relres = LAS_IS_ZERO(bNorm) ? 1.0 : rNorm / bNorm;
+    free(r); free(p); free(q); free(zz); free(t); free(invd);
+    return iter;
+}
+
+int fo_solve(int nx, int ny, const double *GnCt, const double *GnXP, const double *GnXM,
+             const double *GnYP, const double *GnYM, const double *z, double *x,
+             double eps, int maxIter, int precond, double *relres)
+{
+    fo_mat A = {nx, ny, nx * ny, GnCt, GnXP, GnXM, GnYP, GnYM};
+    const int N = nx * ny;
+    double *b = zalloc(N), *xt = zalloc(N);
+    for (int k = 0; k < N; k++) b[k] = z[trps_cell(&A, k)];
+    int it = cg_solve(&A, b, xt, eps, maxIter, precond, relres);
+    for (int k = 0; k < N; k++) x[trps_cell(&A, k)] = xt[k];
+    free(b); free(xt);
+    return it;
+}
+
+/* ---- Phase F: solveFreeSurface (nssolve.c:158-178), plain grid ------------- */
+static void phase_surface(fo_state *s)
+{
+    const frehd_config *k = &s->cfg;
+    const int nx = s->nx, ny = s->ny, N = s->N, tt = s->tt;
+    const double dt = k->dt;
+
+    /* matrixCoeff, nsfunctions.c:815-843 */
+    double coefx = k->g * (dt * dt / (k->dx * k->dx));
+    double coefy = k->g * (dt * dt / (k->dy * k->dy));
+    for (int c = 0; c < N; c++) {
+        int w = nbM(s, c), so = nbS(s, c);
+        s->GnXP[c] = coefx * s->depthXP[c] / (1 + s->DragXP[c]);
+        s->GnXM[c] = coefx * s->depthXP[w] / (1 + s->DragXP[w]);
+        s->GnYP[c] = coefy * s->depthYP[c] / (1 + s->DragYP[c]);
+        s->GnYM[c] = coefy * s->depthYP[so] / (1 + s->DragYP[so]);
+        s->GnCt[c] = 1 + s->GnXP[c] + s->GnYP[c] + s->GnXM[c] + s->GnYM[c];
+    }
+    /* adjustMatrixBoundary, nsfunctions.c:846-884: Neumann walls on all sides */
+    for (int j = 0; j < ny; j++) {
+        int a = j * nx, b = j * nx + nx - 1;
+        s->GnCt[a] = s->GnCt[a] - s->GnXM[a]; s->GnXM[a] = 0;
+        s->GnCt[b] = s->GnCt[b] - s->GnXP[b]; s->GnXP[b] = 0;
+    }
+    for (int i = 0; i < nx; i++) {
+        int a = i, b = N - nx + i;
+        s->GnCt[a] = s->GnCt[a] - s->GnYM[a]; s->GnYM[a] = 0;
+        s->GnCt[b] = s->GnCt[b] - s->GnYP[b]; s->GnYP[b] = 0;
+    }
+    /* adjustTidalBoundary, nsfunctions.c:887-924: Dirichlet rows */
+    if (k->bcType != 2)
+        for (int q = 0; q < k->tideLocLengthP; q++) {
+            int c = s->tideLocP[q];
+            s->GnCt[c] = 1; s->GnXM[c] = 0; s->GnXP[c] = 0; s->GnYM[c] = 0; s->GnYP[c] = 0;
+            s->z[c] = s->tideP[tt] + k->offset;
+        }
+    if (k->bcType != 1)
+        for (int q = 0; q < k->tideLocLengthM; q++) {
+            int c = s->tideLocM[q];
+            s->GnCt[c] = 1; s->GnXM[c] = 0; s->GnXP[c] = 0; s->GnYM[c] = 0; s->GnYP[c] = 0;
+            s->z[c] = s->tideM[tt] + k->offset;
+        }
+
+    /* setupMatrix + solveMatrix + getFreeSurface, nsfunctions.c:927-1031 */
+    double *x = zalloc(N);
+    s->pcgIters = fo_solve(nx, ny, s->GnCt, s->GnXP, s->GnXM, s->GnYP, s->GnYM, s->z, x,
+                           k->eps, k->maxIter, k->precond, &s->relres);
+    s->pcgItersTotal += s->pcgIters;
+    memcpy(s->surfOld, s->surf, sizeof(double) * (size_t)s->Ng);
+    for (int c = 0; c < N; c++) s->surf[c] = x[c];
+    free(x);
+
+    zero_neg_surf(s, tt);
+    fill_surface_ghosts(s, s->surf);
+    tidal_surface(s, tt);
+}
+
+/* ---- Phase U: updateData (nssolve.c:181-290), plain grid -------------------- */
+static void phase_update(fo_state *s)
+{
+    const frehd_config *k = &s->cfg;
+    const int nx = s->nx, ny = s->ny, N = s->N, Ng = s->Ng, tt = s->tt;
+    const double dx = k->dx, dy = k->dy, dt = k->dt;
+    double *h = s->surf, *ho = s->surfOld, *zb = s->bottomZ, *d = s->depth;
+
+    /* limiterCFL, nsfunctions.c:452-495 */
+    for (int c = 0; c < N; c++) {
+        double diff = h[c] - zb[c];
+        if (d[c] <= 0 && diff > 0) {
+            double sref = ho[c] + k->minDepth;
+            int e = nbP(s, c), w = nbM(s, c), n = nbN(s, c), so = nbS(s, c);
+            /* depth holds interior values only: a ghost neighbour reads 0 */
+            int dE = ho[e] <= sref || (e < N ? d[e] : 0.0) <= 0;
+            int dW = ho[w] <= sref || (w < N ? d[w] : 0.0) <= 0;
+            int dN = ho[n] <= sref || (n < N ? d[n] : 0.0) <= 0;
+            int dS = ho[so] <= sref || (so < N ? d[so] : 0.0) <= 0;
+            if (dE && dW && dN && dS) {
+                if ((double)c < (double)N - dx) s->Vloss[0] += diff * dx * dy;
+                h[c] = zb[c];
+                s->rmvCFL[c] = 1;
+            }
+        }
+    }
+    for (int c = 0; c < N; c++) {
+        double diff = h[c] - zb[c];
+        if (diff <= k->minDepth && diff > 0) {
+            s->Vloss[1] += diff * dx * dy;
+            h[c] = zb[c];
+            s->cellV[c] = 0;
+            s->rmvCFL[c] = 0;
+        }
+    }
+
+    compute_volume(s);
+    update_depth(s);
+    update_face_depth(s);
+
+    /* detectWaterfallLocation, nsfunctions.c:1080-1130 (dmin = 0) */
+    for (int c = 0; c < N; c++)
+        if (h[c] < s->bottomXP[c] - 0.0 && s->depthXP[c] >= k->wtfh) s->wtfXP[c] = -1;
+    for (int c = 0; c < N; c++) {
+        int w = nbM(s, c);
+        if (h[c] < s->bottomXP[w] - 0.0 && s->depthXP[w] >= k->wtfh) s->wtfXP[w] = 1;
+    }
+    for (int c = 0; c < N; c++)
+        if (h[c] < s->bottomYP[c] - 0.0 && s->depthYP[c] >= k->wtfh) s->wtfYP[c] = -1;
+    for (int c = 0; c < N; c++) {
+        int so = nbS(s, c);
+        if (h[c] < s->bottomYP[so] - 0.0 && s->depthYP[so] >= k->wtfh) s->wtfYP[so] = 1;
+    }
+
+    /* updateVelocity, nsfunctions.c:250-286: not divided by (1 + Drag) */
+    double *u = s->uuXP, *v = s->vvYP;
+    for (int c = 0; c < N; c++) {
+        u[c] = s->EnXP[c] - (k->g * dt / dx) * (h[nbP(s, c)] - h[c]);
+        v[c] = s->EnYP[c] - (k->g * dt / dy) * (h[nbN(s, c)] - h[c]);
+    }
+    for (int c = 0; c < N; c++) {
+        if (s->depthXP[c] == 0) u[c] = 0;
+        if (s->depthYP[c] == 0) v[c] = 0;
+    }
+    for (int c = 0; c < N; c++) {
+        if (s->rmvCFL[c] == 1) {
+            u[c] = 0; v[c] = 0; u[nbM(s, c)] = 0; v[nbS(s, c)] = 0;
+            s->rmvCFL[c] = 0;
+        }
+    }
+
+    /* waterfallCorrection, nsfunctions.c:1133-1166 (Cw = 0.7, CFLmax = 0.24) */
+    for (int c = 0; c < N; c++) {
+        if (s->wtfXP[c] != 0) {
+            double vel0 = u[c];
+            u[c] = s->wtfXP[c] * 0.7 * sqrt(2 * k->g * s->depthXP[c]);
+            if (u[c] > 0.24 * dx / dt) u[c] = 0.24 * dx / dt;
+            s->Vloss[3] -= (vel0 - u[c]) * dt * s->depthXP[c] * dy;
+        }
+        if (s->wtfYP[c] != 0) {
+            double vel0 = v[c];
+            v[c] = s->wtfYP[c] * 0.7 * sqrt(2 * k->g * s->depthYP[c]);
+            if (v[c] > 0.24 * dy / dt) v[c] = 0.24 * dy / dt;
+            s->Vloss[3] -= (vel0 - v[c]) * dt * s->depthYP[c] * dx;
+        }
+    }
+    for (int c = 0; c < N; c++) { s->wtfXP[c] = 0; s->wtfYP[c] = 0; }
+
+    fill_velocity_ghosts(s);
+
+    /* computeFaceFlowRate, nsfunctions.c:350-358 (over Ng) */
+    for (int c = 0; c < Ng; c++) {
+        s->Fuu[c] = u[c] * dy * s->depthXP[c];
+        s->Fvv[c] = v[c] * dx * s->depthYP[c];
+    }
+
+    /* adjustTidalVelocity, nsfunctions.c:1034-1078 */
+    if (k->bcType != 2)
+        for (int q = 0; q < k->tideLocLengthP; q++) {
+            int c = s->tideLocP[q];
+            double Fww = (h[c] - ho[c]) * dx * dy / dt;
+            double all = s->Fvv[nbS(s, c)] + s->Fuu[nbM(s, c)] - s->Fuu[c] - Fww;
+            if (all != 0 && d[c] > 0) v[c] = all / (d[c] * dx);
+            else v[c] = 0;
+        }
+    if (k->bcType != 1)
+        for (int q = 0; q < k->tideLocLengthM; q++) {
+            int c = s->tideLocM[q];
+            double Fww = (h[c] - ho[c]) * dx * dy / dt;
+            double all = -s->Fvv[c] + s->Fuu[nbM(s, c)] - s->Fuu[c] - Fww;
+            if (all != 0 && d[c] > 0) v[nbS(s, c)] = -all / (d[c] * dx);
+        }
+
+    velocity_interp(s);
+
+    /* volumeByFlux, nsfunctions.c:373-409 */
+    for (int c = 0; c < N; c++) {
+        s->cellV[c] = s->cellV[c] + dt * (s->Fuu[nbM(s, c)] - s->Fuu[c] + s->Fvv[nbS(s, c)] - s->Fvv[c]);
+        for (int q = 0; q < k->inflowLocLength; q++)
+            if (c == s->inflowLoc[q]) s->cellV[c] = s->cellV[c] + s->inflow[tt] * dt;
+        if (s->cellV[c] < 0) s->cellV[c] = 0;
+    }
+
+    /* updateCD, nsfunctions.c:1170-1189 */
+    if (k->CDnotN == 0) {
+        for (int c = 0; c < N; c++) {
+            if (d[c] > 0) {
+                double coeff = manning_coeff(s, c);
+                s->CDXP[c] = coeff / pow(d[c], 1.0 / 3.0);
+                s->CDYP[c] = coeff / pow(d[c], 1.0 / 3.0);
+            }
+        }
+    }
+    if (k->useThinLayer == 1) thin_layer_drag(s);
+    (void)ny;
+}
+
+/* ---- Phase T: scalarTransport (scalar.c:30-56), upwind, plain grid ---------- */
+static int is_inflow_cell(const fo_state *s, int c)
+{
+    for (int q = 0; q < s->cfg.inflowLocLength; q++) if (s->inflowLoc[q] == c) return 1;
+    return 0;
+}
+
+static void scalar_pass(fo_state *s, double *S, double *Sm)
+{
+    const frehd_config *k = &s->cfg;
+    const int nx = s->nx, ny = s->ny, N = s->N, tt = s->tt;
+    const double dx = k->dx, dy = k->dy, dt = k->dt;
+    const double *Fu = s->Fuu, *Fv = s->Fvv, *ho = s->surfOld, *zb = s->bottomZ;
+
+    /* scalarAdvection, scalar.c:68-97 */
+    for (int c = 0; c < N; c++) {
+        int e = nbP(s, c), w = nbM(s, c), n = nbN(s, c), so = nbS(s, c);
+        double Sd = Fu[c] > 0 ? S[c] : S[e];
+        double Su = Fu[w] > 0 ? S[w] : S[c];
+        double Sr = Fv[c] > 0 ? S[c] : S[n];
+        double Sl = Fv[so] > 0 ? S[so] : S[c];
+        Sm[c] = Sm[c] + dt * (-Fu[c] * Sd + Fu[w] * Su - Fv[c] * Sr + Fv[so] * Sl);
+    }
+    /* scalarDiffusion, scalar.c:259-274 */
+    for (int c = 0; c < N; c++) {
+        int e = nbP(s, c), w = nbM(s, c), n = nbN(s, c), so = nbS(s, c);
+        Sm[c] = Sm[c] + dt *
+            ((k->Kx * dy * s->depthXP[c] / dx) * (S[e] - S[c]) -
+             (k->Kx * dy * s->depthXP[w] / dx) * (S[c] - S[w]) +
+             (k->Ky * dx * s->depthYP[c] / dy) * (S[n] - S[c]) -
+             (k->Ky * dx * s->depthYP[so] / dy) * (S[c] - S[so]));
+    }
+    /* updateScalar, scalar.c:277-416 */
+    int *cnt = (int *)calloc((size_t)N, sizeof(int));
+    double *smax = zalloc(N), *smin = zalloc(N);
+    for (int c = 0; c < N; c++) {
+        int e = nbP(s, c), w = nbM(s, c), n = nbN(s, c), so = nbS(s, c);
+        double vals[4]; int m = 0;
+        if (s->depthXP[c] > 0 && ho[e] - zb[e] > 0) vals[m++] = S[e];
+        if (s->depthXP[w] > 0 && ho[w] - zb[w] > 0) vals[m++] = S[w];
+        /* last strip: only neighbour indices <= N count (scalar.c:308-315) */
+        if (n <= N && s->depthYP[c] > 0 && ho[n] - zb[n] > 0) vals[m++] = S[n];
+        if (so <= N && s->depthYP[so] > 0 && ho[so] - zb[so] > 0) vals[m++] = S[so];
+        cnt[c] = m;
+        if (m) {
+            double mx = vals[0], mn = vals[0];
+            for (int q = 1; q < m; q++) { if (vals[q] > mx) mx = vals[q]; if (vals[q] < mn) mn = vals[q]; }
+            smax[c] = mx; smin[c] = mn;
+        }
+    }
+    for (int q = 0; q < k->inflowLocLength; q++) {
+        int c = s->inflowLoc[q];
+        Sm[c] = Sm[c] + k->inflowS * s->inflow[tt] * dt / k->inflowLocLength;
+    }
+    for (int c = 0; c < N; c++) {
+        S[c] = s->cellV[c] > 0 ? Sm[c] / s->cellV[c] : 0;
+        int fl = is_inflow_cell(s, c);
+        if (cnt[c] != 0 && S[c] > smax[c] && !fl) S[c] = smax[c];
+        else if (cnt[c] != 0 && S[c] < smin[c] && !fl) S[c] = smin[c];
+    }
+    free(cnt); free(smax); free(smin);
+    for (int c = 0; c < N; c++) {
+        if (S[c] > 100 || S[c] < 0) {
+            int i = c % nx, j = c / nx;
+            int bottomRow = (j == 0);                  /* icjM > N2ci */
+            int topRowNotFirst = (j == ny - 1 && i > 0); /* icjP > N2ci */
+            if (!bottomRow && !topRowNotFirst) {
+                S[c] = 0;
+                s->surf[c] = s->surf[c] - s->depth[c];
+                s->depth[c] = 0;
+            }
+        }
+    }
+    /* enforceScalarBC, scalar.c:419-471 */
+    for (int i = 0; i < nx; i++) { S[gjP(s, i)] = S[N - nx + i]; S[gjM(s, i)] = S[i]; }
+    for (int j = 0; j < ny; j++) { S[giP(s, j)] = S[j * nx + nx - 1]; S[giM(s, j)] = S[j * nx]; }
+    if (k->bcType != 2)
+        for (int q = 0; q < k->tideLocLengthP; q++)
+            S[s->tideLocP[q]] = k->useConstTidePS == 1 ? k->tidePS : s->tidalPS[tt];
+    if (k->bcType != 1)
+        for (int q = 0; q < k->tideLocLengthM; q++)
+            S[s->tideLocM[q]] = k->useConstTideMS == 1 ? k->tideMS : s->tidalMS[tt];
+    for (int c = 0; c < N; c++) if (s->depth[c] <= 0) S[c] = 0;
+}
+
+static void phase_scalar(fo_state *s)
+{
+    if (s->cfg.useScalar != 1) return;
+    for (int q = 0; q < s->cfg.nScalar; q++)
+        scalar_pass(s, s->S + (size_t)q * s->Ng, s->Sm + (size_t)q * s->Ng);
+}
+
+/* ---- monitorCFL, nsfunctions.c:1192-1232 ------------------------------------- */
+static int phase_cfl(fo_state *s)
+{
+    double mx = 0, my = 0;
+    for (int c = 0; c < s->N; c++) {
+        s->CFLx[c] = fabs(s->uuXP[c]) * s->cfg.dt / s->cfg.dx;
+        s->CFLy[c] = fabs(s->vvYP[c]) * s->cfg.dt / s->cfg.dy;
+        if (s->CFLx[c] > mx) mx = s->CFLx[c];
+        if (s->CFLy[c] > my) my = s->CFLy[c];
+    }
+    return (fabs(mx) > 20 || fabs(my) > 20) ? FREHD_ECFL : 0;
+}
+
+int fo_phase(fo_state *s, int phase)
+{
+    switch (phase) {
+    case FO_PHASE_SOURCE: s->tt++; phase_source(s); return 0;
+    case FO_PHASE_SURFACE: phase_surface(s); return 0;
+    case FO_PHASE_UPDATE: phase_update(s); return 0;
+    case FO_PHASE_SCALAR: phase_scalar(s); return 0;
+    case FO_PHASE_CFL: return phase_cfl(s);
+    default: return FREHD_EINVAL;
+    }
+}
+
+int fo_step(fo_state *s)
+{
+    for (int p = FO_PHASE_SOURCE; p < FO_PHASE_CFL; p++) fo_phase(s, p);
+    return fo_phase(s, FO_PHASE_CFL);
+}
+
+/* ---- creation: the Init sequence, initialize.c:40-155 ----------------------- */
+static void *dup_arr(const void *src, size_t bytes)
+{
+    if (!src || bytes == 0) return NULL;
+    void *d = malloc(bytes);
+    memcpy(d, src, bytes);
+    return d;
+}
+
+fo_state *fo_create(const frehd_config *cfg, const frehd_forcing *f, const frehd_ic *ic)
+{
+    if (!cfg || !f || !ic || !ic->bottomZ || cfg->nx < 4 || cfg->ny < 4) return NULL;
+    fo_state *s = (fo_state *)calloc(1, sizeof(fo_state));
+    s->cfg = *cfg;
+    if (s->cfg.nScalar < 1) s->cfg.nScalar = 1;
+    const int nx = cfg->nx, ny = cfg->ny, N = nx * ny, Ng = (nx + 2) * (ny + 2);
+    const int ns = s->cfg.nScalar;
+    s->nx = nx; s->ny = ny; s->N = N; s->Ng = Ng; s->tt = 0;
+    size_t ser = sizeof(double) * (size_t)(cfg->Nt + 1);
+    s->tideLocP = (int32_t *)dup_arr(f->tideLocP, sizeof(int32_t) * (size_t)cfg->tideLocLengthP);
+    s->tideLocM = (int32_t *)dup_arr(f->tideLocM, sizeof(int32_t) * (size_t)cfg->tideLocLengthM);
+    s->inflowLoc = (int32_t *)dup_arr(f->inflowLoc, sizeof(int32_t) * (size_t)cfg->inflowLocLength);
+    s->tideP = (double *)dup_arr(f->tideP, ser);
+    s->tideM = (double *)dup_arr(f->tideM, ser);
+    s->inflow = (double *)dup_arr(f->inflow, ser);
+    s->tidalPS = (double *)dup_arr(f->tidalPS, ser);
+    s->tidalMS = (double *)dup_arr(f->tidalMS, ser);
+    s->manningCell = (double *)dup_arr(f->manningCell, sizeof(double) * (size_t)N);
+    if (!s->inflow) s->inflow = zalloc(cfg->Nt + 1);
+    if (!s->tideP) s->tideP = zalloc(cfg->Nt + 1);
+    if (!s->tideM) s->tideM = zalloc(cfg->Nt + 1);
+    if (!s->tidalPS) s->tidalPS = zalloc(cfg->Nt + 1);
+    if (!s->tidalMS) s->tidalMS = zalloc(cfg->Nt + 1);
+
+    double **ngArrays[] = {&s->surf, &s->surfOld, &s->uuXP, &s->vvYP, &s->Fuu, &s->Fvv,
+        &s->depthXP, &s->depthYP, &s->EnXP, &s->EnYP, &s->DragXP, &s->DragYP, &s->CDXP,
+        &s->CDYP, &s->wtfXP, &s->wtfYP, &s->CFLx, &s->CFLy, &s->advX, &s->advY,
+        &s->bottomZ, &s->bottomXP, &s->bottomYP, &s->uuYP, &s->vvXP, &s->depth,
+        &s->cellV, &s->rmvCFL, &s->GnCt, &s->GnXP, &s->GnXM, &s->GnYP, &s->GnYM, &s->z};
+    for (size_t q = 0; q < sizeof(ngArrays) / sizeof(ngArrays[0]); q++) *ngArrays[q] = zalloc(Ng);
+    s->S = zalloc(ns * Ng);
+    s->Sm = zalloc(ns * Ng);
+
+    /* InitBathymetry (bathymetry.c:74-104): interior bed + offset */
+    for (int c = 0; c < N; c++) s->bottomZ[c] = ic->bottomZ[c] + cfg->offset;
+    init_bathymetry(s);
+    /* initCD (initialize.c:381-389) */
+    for (int c = 0; c < Ng; c++) { s->CDXP[c] = cfg->CDx; s->CDYP[c] = cfg->CDy; }
+    /* initFieldValues (initialize.c:392-440) */
+    for (int c = 0; c < N; c++) {
+        s->surf[c] = (ic->surf0cell ? ic->surf0cell[c] : ic->surf0) + cfg->offset;
+        s->uuXP[c] = ic->U0cell ? ic->U0cell[c] : ic->U0;
+        s->vvYP[c] = ic->V0cell ? ic->V0cell[c] : ic->V0;
+        if (cfg->useScalar == 1)
+            for (int q = 0; q < ns; q++)
+                s->S[(size_t)q * Ng + c] = ic->S0cell ? ic->S0cell[(size_t)q * N + c] : ic->S0;
+    }
+    zero_neg_surf(s, 0);
+    fill_surface_ghosts(s, s->surf);
+    fill_velocity_ghosts(s);
+    tidal_surface(s, 0);
+    update_depth(s);
+    update_face_depth(s);
+    compute_volume(s);
+    velocity_interp(s);
+    thin_layer_drag(s);   /* called unconditionally by Init (initialize.c:138) */
+    if (cfg->useScalar == 1) {
+        for (int q = 0; q < ns; q++) {
+            double *S = s->S + (size_t)q * Ng, *Sm = s->Sm + (size_t)q * Ng;
+            for (int c = 0; c < N; c++) if (s->depth[c] <= 0) S[c] = 0;
+            for (int c = 0; c < N; c++) Sm[c] = S[c] * s->cellV[c];   /* scalarMass */
+        }
+    }
+    return s;
+}
+
+void fo_destroy(fo_state *s)
+{
+    if (!s) return;
+    double *arrs[] = {s->surf, s->surfOld, s->uuXP, s->vvYP, s->Fuu, s->Fvv, s->depthXP,
+        s->depthYP, s->EnXP, s->EnYP, s->DragXP, s->DragYP, s->CDXP, s->CDYP, s->wtfXP,
+        s->wtfYP, s->CFLx, s->CFLy, s->advX, s->advY, s->bottomZ, s->bottomXP, s->bottomYP,
+        s->uuYP, s->vvXP, s->depth, s->cellV, s->rmvCFL, s->GnCt, s->GnXP, s->GnXM, s->GnYP,
+        s->GnYM, s->z, s->S, s->Sm, s->tideP, s->tideM, s->inflow, s->tidalPS, s->tidalMS,
+        s->manningCell};
+    for (size_t q = 0; q < sizeof(arrs) / sizeof(arrs[0]); q++) free(arrs[q]);
+    free(s->tideLocP); free(s->tideLocM); free(s->inflowLoc);
+    free(s);
+}
+
+double *fo_field(fo_state *s, const char *name, int64_t *len)
+{
+    struct { const char *n; double *p; int64_t l; } tab[] = {
+        {"surf", s->surf, s->Ng}, {"surfOld", s->surfOld, s->Ng}, {"uuXP", s->uuXP, s->Ng},
+        {"vvYP", s->vvYP, s->Ng}, {"uuYP", s->uuYP, s->N}, {"vvXP", s->vvXP, s->N},
+        {"depth", s->depth, s->N}, {"depthXP", s->depthXP, s->Ng}, {"depthYP", s->depthYP, s->Ng},
+        {"EnXP", s->EnXP, s->Ng}, {"EnYP", s->EnYP, s->Ng}, {"DragXP", s->DragXP, s->Ng},
+        {"DragYP", s->DragYP, s->Ng}, {"CDXP", s->CDXP, s->Ng}, {"CDYP", s->CDYP, s->Ng},
+        {"Fuu", s->Fuu, s->Ng}, {"Fvv", s->Fvv, s->Ng}, {"cellV", s->cellV, s->N},
+        {"S", s->S, (int64_t)s->cfg.nScalar * s->Ng}, {"Sm", s->Sm, (int64_t)s->cfg.nScalar * s->Ng},
+        {"CFLx", s->CFLx, s->Ng}, {"CFLy", s->CFLy, s->Ng}, {"wtfXP", s->wtfXP, s->Ng},
+        {"wtfYP", s->wtfYP, s->Ng}, {"bottomZ", s->bottomZ, s->Ng}, {"bottomXP", s->bottomXP, s->Ng},
+        {"bottomYP", s->bottomYP, s->Ng}, {"advX", s->advX, s->Ng}, {"advY", s->advY, s->Ng},
+        {"rmvCFL", s->rmvCFL, s->N}, {"GnCt", s->GnCt, s->N}, {"GnXP", s->GnXP, s->N},
+        {"GnXM", s->GnXM, s->N}, {"GnYP", s->GnYP, s->N}, {"GnYM", s->GnYM, s->N},
+        {"z", s->z, s->N}};
+    for (size_t q = 0; q < sizeof(tab) / sizeof(tab[0]); q++)
+        if (strcmp(tab[q].n, name) == 0) { if (len) *len = tab[q].l; return tab[q].p; }
+    if (len) *len = 0;
+    return NULL;
+}
+
+int32_t fo_tt(const fo_state *s) { return s->tt; }
+int32_t fo_pcg_iters(const fo_state *s) { return s->pcgIters; }
+int64_t fo_pcg_iters_total(const fo_state *s) { return s->pcgItersTotal; }
+double  fo_pcg_relres(const fo_state *s) { return s->relres; }
+void    fo_vloss(const fo_state *s, double out[4]) { for (int q = 0; q < 4; q++) out[q] = s->Vloss[q]; }

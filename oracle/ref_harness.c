@@ -1,0 +1,352 @@
+/*
+ * ref_harness.c -- drives FrehdC v4.2's own step functions on given inputs.
+ *
+ * TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
+ * the reference's unmodified sources (FrehdC_v42/*.c except FREHD.c, plus the
+ * LASpack files they use) from where they lie under the reference checkout,
+ * into oracle/_ref/libfrehd_ref.so.  Nothing of the reference is copied into
+ * this repository; the built library stays out of git (.gitignore: *.so).
+ *
+ * The harness replaces only what cannot run as shipped (survey of the
+ * reference's main in FREHD.c:66-118 and Init in initialize.c:40-155):
+ *   - Config is filled directly instead of parsed from data.dat
+ *     (ReadUserSettings/dateNum read an unterminated buffer, utilities.c:114);
+ *     the never-assigned flags useEvap, useRain, useSubStep,
+ *     checkConservation and inflow_rank are set to 0;
+ *   - the grid maps are built here with the same index formulas as
+ *     createMaps (map.c:15-128) but with icjMM sized correctly (the reference
+ *     writes past its end, map.c:69-74);
+ *   - every Data/Bath/BC array is allocated with zeroed guard cells before and
+ *     after it, so the reference's out-of-range reads (velocityInterp's
+ *     uuXP[-1] and vvYP[ii-nx]; limiterCFL's depth[ghost]) read 0.0;
+ *   - boundary series are given per step (Nt+1 samples) instead of being read
+ *     and interpolated from files;
+ *   - the Init call sequence is restated, calling the reference's functions
+ *     for every numerical part.
+ * The step itself is the reference's solveSourceTerm / solveFreeSurface /
+ * updateData / scalarTransport / monitorCFL.  With the Jacobi preconditioner
+ * the harness restates solveFreeSurface (nssolve.c:158-178) with
+ * CGIter(..., JacobiPrecond, 1) in place of the hard-wired SSORPrecond
+ * (nsfunctions.c:1016); everything else in that phase is the reference's.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+#include "configuration.h"
+#include "bathymetry.h"
+#include "initialize.h"
+#include "map.h"
+#include "subgrid.h"
+#include "nsfunctions.h"
+#include "nssolve.h"
+#include "scalar.h"
+#include "laspack/errhandl.h"
+#include "laspack/vector.h"
+#include "laspack/qmatrix.h"
+#include "laspack/operats.h"
+#include "laspack/precond.h"
+#include "laspack/rtc.h"
+#include "laspack/itersolv.h"
+
+#include "../include/frehd_gpu.h"
+
+typedef struct ref_handle {
+    Config *setting;
+    Maps *map;
+    Data *data;
+    Bath *bath;
+    BC *bc;
+    Sub *sub;
+    IC *ic;
+    int tt, precond, lastIter;
+    double *blocks[128];
+    int nblocks;
+} ref_handle;
+
+/* zero-padded allocation: `pad` zero doubles before and after n values */
+static double *padded(ref_handle *h, size_t n, size_t pad)
+{
+    double *b = (double *)calloc(n + 2 * pad, sizeof(double));
+    h->blocks[h->nblocks++] = b;
+    return b + pad;
+}
+
+static int *ialloc(size_t n) { return (int *)calloc(n, sizeof(int)); }
+
+static void build_maps(Maps *m, int nx, int ny)
+{
+    int N = nx * ny, Ng = (nx + 2) * (ny + 2);
+    m->cntr = ialloc(N); m->trps = ialloc(N); m->sprt = ialloc(N);
+    m->iPjc = ialloc(N); m->iMjc = ialloc(N); m->icjP = ialloc(N); m->icjM = ialloc(N);
+    m->iPPjc = ialloc(N); m->iMMjc = ialloc(N); m->icjPP = ialloc(N); m->icjMM = ialloc(N + nx);
+    m->iPjP = ialloc(1); m->iPjM = ialloc(1); m->iMjP = ialloc(1); m->iMjM = ialloc(1);
+    m->ii2d = ialloc(N); m->jj2d = ialloc(N);
+    m->iPbd = ialloc(ny); m->iPgt = ialloc(ny); m->iMbd = ialloc(ny); m->iMgt = ialloc(ny);
+    m->jPbd = ialloc(nx); m->jPgt = ialloc(nx); m->jMbd = ialloc(nx); m->jMgt = ialloc(nx);
+    for (int c = 0; c < N; c++) {
+        int i = c % nx, j = c / nx;
+        m->cntr[c] = c;
+        m->trps[c] = (c % ny) * nx + c / ny;
+        m->sprt[c] = (c % nx) * ny + c / nx;
+        m->icjP[c] = j == ny - 1 ? N + i : c + nx;
+        m->icjM[c] = j == 0 ? N + nx + i : c - nx;
+        m->iPjc[c] = i == nx - 1 ? N + 2 * nx + j : c + 1;
+        m->iMjc[c] = i == 0 ? N + 2 * nx + ny + j : c - 1;
+        m->ii2d[c] = i; m->jj2d[c] = j;
+    }
+    for (int c = 0; c < N; c++) {
+        int i = c % nx, j = c / nx;
+        m->icjPP[c] = j >= ny - 1 ? m->icjP[c] : m->icjP[m->icjP[c]];
+        m->icjMM[c] = j <= 0 ? m->icjM[c] : (j == 1 ? m->icjM[c] : m->icjM[m->icjM[c]]);
+        m->iPPjc[c] = i == nx - 1 ? m->iPjc[c] : m->iPjc[m->iPjc[c]];
+        m->iMMjc[c] = i == 0 ? m->iMjc[c] : m->iMjc[m->iMjc[c]];
+    }
+    m->iMjM[0] = Ng - 1; m->iPjM[0] = Ng - 2; m->iPjP[0] = Ng - 3; m->iMjP[0] = Ng - 4;
+    for (int i = 0; i < nx; i++) {
+        m->jPbd[i] = N - nx + i; m->jMbd[i] = i;
+        m->jPgt[i] = m->icjP[N - nx + i]; m->jMgt[i] = m->icjM[i];
+    }
+    for (int j = 0; j < ny; j++) {
+        m->iPbd[j] = (j + 1) * nx - 1; m->iMbd[j] = j * nx;
+        m->iPgt[j] = m->iPjc[(j + 1) * nx - 1]; m->iMgt[j] = m->iMjc[j * nx];
+    }
+}
+
+static void fill_config(Config *s, const frehd_config *c)
+{
+    memset(s, 0, sizeof(*s));
+    s->NX = s->nx = c->nx; s->NY = s->ny = c->ny;
+    s->N2CI = s->N2ci = c->nx * c->ny;
+    s->N2CT = s->N2ct = (c->nx + 2) * (c->ny + 2);
+    s->dx = c->dx; s->dy = c->dy; s->dt = c->dt; s->Nt = c->Nt; s->OutItvl = c->Nt + 1;
+    s->g = c->g; s->NUx = c->NUx; s->NUy = c->NUy; s->CDx = c->CDx; s->CDy = c->CDy;
+    s->manningN = c->manningN; s->CDnotN = c->CDnotN; s->hD = c->hD; s->CDmax = c->CDmax;
+    s->useThinLayer = c->useThinLayer; s->minDepth = c->minDepth; s->wtfh = c->wtfh;
+    s->CFLl = c->CFLl; s->CFLh = c->CFLh; s->eps = c->eps; s->maxIter = c->maxIter;
+    s->bcType = c->bcType; s->tideLocLengthP = c->tideLocLengthP;
+    s->tideLocLengthM = c->tideLocLengthM; s->inflowLocLength = c->inflowLocLength;
+    s->useScalar = c->useScalar; s->scalarAdv = c->scalarAdv;
+    s->useConstTidePS = c->useConstTidePS; s->useConstTideMS = c->useConstTideMS;
+    s->tidePS = c->tidePS; s->tideMS = c->tideMS; s->inflowS = c->inflowS;
+    s->Kx = c->Kx; s->Ky = c->Ky;
+    s->useSubgrid = 0; s->useMPI = 0; s->np = 1; s->useWind = 0;
+    s->useEvap = 0; s->useRain = 0; s->useSubStep = 0; s->checkConservation = 0;
+    s->inflow_rank = 0; s->useCellEdge = 0;
+}
+
+void ref_destroy(void *hv);
+
+void *ref_create(const frehd_config *c, const frehd_forcing *f, const frehd_ic *icin)
+{
+    if (c->tideLocLengthP > 1000 || c->tideLocLengthM > 1000 || c->inflowLocLength > 1000)
+        return NULL;    /* Config holds at most 1000 locations (configuration.h:60,68) */
+    if (c->useScalar && c->nScalar > 1) return NULL;
+    ref_handle *h = (ref_handle *)calloc(1, sizeof(ref_handle));
+    h->precond = c->precond;
+    const int nx = c->nx, ny = c->ny, N = nx * ny, Ng = (nx + 2) * (ny + 2);
+    const size_t pad = (size_t)nx + 8;
+    h->setting = (Config *)calloc(1, sizeof(Config));
+    fill_config(h->setting, c);
+    Config *s = h->setting;
+    for (int q = 0; q < c->tideLocLengthP; q++) s->tideLocP[q] = f->tideLocP[q];
+    for (int q = 0; q < c->tideLocLengthM; q++) s->tideLocM[q] = f->tideLocM[q];
+    for (int q = 0; q < c->inflowLocLength; q++) s->inflowLoc[q] = f->inflowLoc[q];
+
+    h->map = (Maps *)calloc(1, sizeof(Maps));
+    build_maps(h->map, nx, ny);
+
+    /* Bath (bathymetry.c:20-104) */
+    h->bath = (Bath *)calloc(1, sizeof(Bath));
+    Bath *b = h->bath;
+    b->offset = padded(h, 1, 1);
+    b->offset[0] = c->offset;
+    b->allZ = padded(h, N, pad);
+    b->bottomZ = padded(h, Ng, pad);
+    b->bottomXP = padded(h, Ng, pad);
+    b->bottomYP = padded(h, Ng, pad);
+    b->edgeX = padded(h, Ng, pad);
+    b->edgeY = padded(h, Ng, pad);
+    for (int q = 0; q < N; q++) b->allZ[q] = icin->bottomZ[q];
+    for (int q = 0; q < N; q++) b->bottomZ[q] = b->allZ[q] + b->offset[0];
+
+    /* Data arrays (initDataArrays, initialize.c:178-270), zero padded */
+    h->data = (Data *)calloc(1, sizeof(Data));
+    Data *d = h->data;
+    double **arrs[] = {&d->uuXP, &d->uuYP, &d->vvXP, &d->vvYP, &d->surf, &d->surfOld, &d->Fuu,
+        &d->Fvv, &d->cellV, &d->depth, &d->depthXP, &d->depthYP, &d->EnXP, &d->EnYP,
+        &d->DragXP, &d->DragYP, &d->CDXP, &d->CDYP, &d->GnCt, &d->GnXP, &d->GnYP, &d->GnXM,
+        &d->GnYM, &d->z, &d->wtfXP, &d->wtfYP, &d->S, &d->Sm, &d->rmvCFL, &d->advX, &d->advY,
+        &d->CFLx, &d->CFLy, &d->Vloss, &d->alluuXP, &d->allvvYP, &d->allsurf, &d->alldepth,
+        &d->allS, &d->allCDXP, &d->allCDYP, &d->allVloss};
+    for (size_t q = 0; q < sizeof(arrs) / sizeof(arrs[0]); q++) *arrs[q] = padded(h, Ng, pad);
+    d->Vloss1 = padded(h, 1, 1); d->Vloss2 = padded(h, 1, 1);
+    d->Vloss3 = padded(h, 1, 1); d->Vloss4 = padded(h, 1, 1);
+    d->inflowLoc = ialloc(c->inflowLocLength + 1);
+
+    /* BC series (initBCArrays, initialize.c:330-342), Nt+1 samples */
+    h->bc = (BC *)calloc(1, sizeof(BC));
+    BC *bc = h->bc;
+    const size_t ns = (size_t)c->Nt + 1;
+    double **bcs[] = {&bc->tideP, &bc->tideM, &bc->inflow, &bc->windspd, &bc->winddir,
+        &bc->tidalPS, &bc->tidalMS, &bc->evap, &bc->rain};
+    for (size_t q = 0; q < sizeof(bcs) / sizeof(bcs[0]); q++) *bcs[q] = padded(h, ns, 4);
+    for (size_t t = 0; t < ns; t++) {
+        if (f->tideP) bc->tideP[t] = f->tideP[t];
+        if (f->tideM) bc->tideM[t] = f->tideM[t];
+        if (f->inflow) bc->inflow[t] = f->inflow[t];
+        if (f->tidalPS) bc->tidalPS[t] = f->tidalPS[t];
+        if (f->tidalMS) bc->tidalMS[t] = f->tidalMS[t];
+    }
+
+    h->sub = (Sub *)calloc(1, sizeof(Sub));
+
+    /* IC (readIC, initialize.c:279-327) */
+    h->ic = (IC *)calloc(1, sizeof(IC));
+    IC *ic = h->ic;
+    s->useConstSurf0 = icin->surf0cell ? 0 : 1;
+    s->useConstU0 = icin->U0cell ? 0 : 1;
+    s->useConstV0 = icin->V0cell ? 0 : 1;
+    s->useConstInitS = icin->S0cell ? 0 : 1;
+    ic->surf0 = icin->surf0; ic->U0 = icin->U0; ic->V0 = icin->V0; ic->S0 = icin->S0;
+    ic->allSurf0 = (double *)icin->surf0cell; ic->allU0 = (double *)icin->U0cell;
+    ic->allV0 = (double *)icin->V0cell; ic->allS0 = (double *)icin->S0cell;
+
+    /* Init's call sequence, initialize.c:40-155 (np = 1, plain grid) */
+    enforceBathBC(b, h->map, s, 0, 1);
+    updateAllFaceElev(b, h->map, s);
+    initCD(&h->data, s);
+    initFieldValues(&h->data, b, ic, s, 0);
+    zeroNegSurf(&h->data, b, s, 0);
+    enforceFreeSurfaceBC(&h->data, h->map, s);
+    enforceVelocityBC(&h->data, h->map, s);
+    enforceTidalBC(&h->data, b, bc, s, 0, 0, 1);
+    updateDepth(&h->data, h->map, b, s);
+    updateAllFaceDepth(&h->data, b, h->map, s, 0, 1);
+    computeVolume(&h->data, s);
+    velocityInterp(&h->data, s);
+    thinLayerDrag(&h->data, s);
+    findInflowLocation(&h->data, h->map, s);
+    if (s->useScalar == 1) {
+        for (int q = 0; q < N; q++) if (d->depth[q] <= 0) d->S[q] = 0;
+        scalarMass(&h->data, s);
+    }
+    return h;
+}
+
+/* solveFreeSurface (nssolve.c:158-178) with the preconditioner selectable */
+static void free_surface_phase(ref_handle *h)
+{
+    Config *s = h->setting;
+    QMatrix A; Vector x, z;
+    Q_Constr(&A, "A", s->N2ci, False, Rowws, Normal, True);
+    V_Constr(&z, "z", s->N2ci, Normal, True);
+    V_Constr(&x, "x", s->N2ci, Normal, True);
+    if (h->precond == FREHD_PRECOND_SSOR) {
+        solveFreeSurface(&h->data, h->sub, h->map, h->bc, h->bath, s, h->tt, A, x, z, 0, 1);
+    } else {
+        matrixCoeff(&h->data, h->map, s);
+        adjustMatrixBoundary(&h->data, h->bc, h->map, s, 0, 1);
+        adjustTidalBoundary(&h->data, h->bc, h->bath, s, h->tt, 0, 1);
+        setupMatrix(h->data, h->map, s, A);
+        for (size_t k = 1; k <= (size_t)s->N2ci; k++) V_SetCmp(&z, k, h->data->z[k - 1]);
+        V_SetAllCmp(&x, 0.0);
+        SetRTCAccuracy(s->eps);
+        CGIter(&A, &x, &z, s->maxIter, JacobiPrecond, 1);
+        getFreeSurface(&h->data, h->map, s, x);
+        zeroNegSurf(&h->data, h->bath, s, h->tt);
+        enforceFreeSurfaceBC(&h->data, h->map, s);
+        enforceTidalBC(&h->data, h->bath, h->bc, s, h->tt, 0, 1);
+    }
+    h->lastIter = GetLastNoIter();
+    Q_Destr(&A); V_Destr(&x); V_Destr(&z);
+}
+
+int ref_phase(void *hv, int phase)
+{
+    ref_handle *h = (ref_handle *)hv;
+    Config *s = h->setting;
+    switch (phase) {
+    case 0:
+        h->tt++;
+        solveSourceTerm(&h->data, h->sub, h->map, h->bc, h->bath, s, h->tt, 0, 1);
+        return 0;
+    case 1: free_surface_phase(h); return 0;
+    case 2: updateData(&h->data, &h->sub, h->map, h->bc, h->bath, s, h->tt, 0, 1); return 0;
+    case 3:
+        if (s->useScalar == 1)
+            scalarTransport(&h->data, h->map, h->bath, h->bc, h->sub, s, h->tt, 0, 1);
+        return 0;
+    case 4: {
+        /* monitorCFL exits the process above CFL 20; check first */
+        double mx = 0;
+        for (int c = 0; c < s->N2ci; c++) {
+            double a = fabs(h->data->uuXP[c]) * s->dt / s->dx, b2 = fabs(h->data->vvYP[c]) * s->dt / s->dy;
+            if (a > mx) mx = a;
+            if (b2 > mx) mx = b2;
+        }
+        if (mx > 20) return FREHD_ECFL;
+        monitorCFL(&h->data, h->bath, 0, s, h->tt, 0);
+        return 0;
+    }
+    default: return FREHD_EINVAL;
+    }
+}
+
+int ref_step(void *hv)
+{
+    for (int p = 0; p < 4; p++) ref_phase(hv, p);
+    return ref_phase(hv, 4);
+}
+
+int ref_run(void *hv, int nsteps)
+{
+    for (int t = 0; t < nsteps; t++) {
+        int rc = ref_step(hv);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+double *ref_field(void *hv, const char *name, int64_t *len)
+{
+    ref_handle *h = (ref_handle *)hv;
+    Data *d = h->data;
+    Bath *b = h->bath;
+    int N = h->setting->N2ci, Ng = h->setting->N2ct;
+    struct { const char *n; double *p; int64_t l; } tab[] = {
+        {"surf", d->surf, Ng}, {"surfOld", d->surfOld, Ng}, {"uuXP", d->uuXP, Ng},
+        {"vvYP", d->vvYP, Ng}, {"uuYP", d->uuYP, N}, {"vvXP", d->vvXP, N},
+        {"depth", d->depth, N}, {"depthXP", d->depthXP, Ng}, {"depthYP", d->depthYP, Ng},
+        {"EnXP", d->EnXP, Ng}, {"EnYP", d->EnYP, Ng}, {"DragXP", d->DragXP, Ng},
+        {"DragYP", d->DragYP, Ng}, {"CDXP", d->CDXP, Ng}, {"CDYP", d->CDYP, Ng},
+        {"Fuu", d->Fuu, Ng}, {"Fvv", d->Fvv, Ng}, {"cellV", d->cellV, N},
+        {"S", d->S, Ng}, {"Sm", d->Sm, N}, {"CFLx", d->CFLx, Ng}, {"CFLy", d->CFLy, Ng},
+        {"wtfXP", d->wtfXP, Ng}, {"wtfYP", d->wtfYP, Ng}, {"bottomZ", b->bottomZ, Ng},
+        {"bottomXP", b->bottomXP, Ng}, {"bottomYP", b->bottomYP, Ng}, {"advX", d->advX, Ng},
+        {"advY", d->advY, Ng}, {"rmvCFL", d->rmvCFL, N}, {"GnCt", d->GnCt, N},
+        {"GnXP", d->GnXP, N}, {"GnXM", d->GnXM, N}, {"GnYP", d->GnYP, N}, {"GnYM", d->GnYM, N},
+        {"z_transposed", d->z, N}};
+    for (size_t q = 0; q < sizeof(tab) / sizeof(tab[0]); q++)
+        if (strcmp(tab[q].n, name) == 0) { if (len) *len = tab[q].l; return tab[q].p; }
+    if (len) *len = 0;
+    return NULL;
+}
+
+int ref_tt(void *hv) { return ((ref_handle *)hv)->tt; }
+int ref_pcg_iters(void *hv) { return ((ref_handle *)hv)->lastIter; }
+void ref_vloss(void *hv, double out[4])
+{
+    Data *d = ((ref_handle *)hv)->data;
+    out[0] = d->Vloss1[0]; out[1] = d->Vloss2[0]; out[2] = d->Vloss3[0]; out[3] = d->Vloss4[0];
+}
+
+void ref_destroy(void *hv)
+{
+    ref_handle *h = (ref_handle *)hv;
+    if (!h) return;
+    for (int q = 0; q < h->nblocks; q++) free(h->blocks[q]);
+    /* maps, structs and small int arrays are left to process exit */
+    free(h);
+}
