@@ -136,6 +136,8 @@ def load_library(path: str | None = None):
         "frehd_dev_download": ([V, C.POINTER(FrehdFields)], C.c_int),
         "frehd_dev_stats": ([V, C.POINTER(FrehdStats)], C.c_int),
         "frehd_dev_step_index": ([V, c_int32_p], C.c_int),
+        "frehd_dev_set_step_index": ([V, C.c_int32], C.c_int),
+        "frehd_dev_phase": ([V, C.c_int32], C.c_int),
         "frehd_dev_destroy": ([V], C.c_int),
         "frehd_dev_footprint": ([C.POINTER(FrehdConfig), C.POINTER(C.c_int64)], C.c_int),
         "frehd_dev_enable_timing": ([V, C.c_int32], C.c_int),

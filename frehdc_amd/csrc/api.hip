@@ -1,0 +1,815 @@
+// api.hip -- C ABI of the device step (include/frehd_gpu.h) and its host
+// orchestration: handle creation (Init restated on the host), upload,
+// per-step kernel sequence, PCG convergence control, download, statistics.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/frehd_gpu.h"
+#include "frehd_dev.h"
+#include "step_kernels.h"
+
+using namespace frehd;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) return fail(FREHD_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Host restatement of Init (initialize.c:40-155) for a plain grid, np = 1.
+// One-off setup on the host (as in the reference); the result is uploaded.
+namespace {
+
+struct HostState {
+    int nx, ny, N, Ng, ns;
+    std::vector<double> surf, surfOld, uuXP, vvYP, uuYP, vvXP, depth, depthXP, depthYP, EnXP,
+        EnYP, DragXP, DragYP, CDXP, CDYP, Fuu, Fvv, cellV, S, Sm, CFLx, CFLy, wtfXP, wtfYP,
+        bottomZ, bottomXP, bottomYP;
+};
+
+struct HG {
+    Grid g;
+    int E(int c) const { int i = c % g.nx, j = c / g.nx; return nbE(g, c, i, j); }
+    int W(int c) const { int i = c % g.nx, j = c / g.nx; return nbW(g, c, i, j); }
+    int Nn(int c) const { int i = c % g.nx, j = c / g.nx; return nbN(g, c, i, j); }
+    int S(int c) const { int i = c % g.nx, j = c / g.nx; return nbS(g, c, i, j); }
+};
+
+void host_init(HostState &h, const frehd_config &k, const frehd_forcing &f, const frehd_ic &ic)
+{
+    const int nx = k.nx, ny = k.ny, N = nx * ny, Ng = (nx + 2) * (ny + 2);
+    const int ns = std::max(1, k.nScalar);
+    h.nx = nx; h.ny = ny; h.N = N; h.Ng = Ng; h.ns = ns;
+    std::vector<double> *ng[] = {&h.surf, &h.surfOld, &h.uuXP, &h.vvYP, &h.uuYP, &h.vvXP,
+        &h.depth, &h.depthXP, &h.depthYP, &h.EnXP, &h.EnYP, &h.DragXP, &h.DragYP, &h.CDXP,
+        &h.CDYP, &h.Fuu, &h.Fvv, &h.cellV, &h.CFLx, &h.CFLy, &h.wtfXP, &h.wtfYP, &h.bottomZ,
+        &h.bottomXP, &h.bottomYP};
+    for (auto *v : ng) v->assign(Ng, 0.0);
+    h.S.assign((size_t)ns * Ng, 0.0);
+    h.Sm.assign((size_t)ns * Ng, 0.0);
+    HG G{{nx, ny, N, Ng}};
+    const Grid &g = G.g;
+    double *zb = h.bottomZ.data(), *bx = h.bottomXP.data(), *by = h.bottomYP.data();
+    // InitBathymetry + enforceBathBC + updateAllFaceElev (useCellEdge = 0)
+    for (int c = 0; c < N; c++) zb[c] = ic.bottomZ[c] + k.offset;
+    for (int i = 0; i < nx; i++) { zb[gjM(g, i)] = zb[i]; zb[gjP(g, i)] = zb[N - nx + i]; }
+    for (int j = 0; j < ny; j++) { zb[giM(g, j)] = zb[j * nx]; zb[giP(g, j)] = zb[j * nx + nx - 1]; }
+    for (int c = 0; c < N; c++) { double a = zb[c], b = zb[G.E(c)]; bx[c] = a > b ? a : b; }
+    for (int j = 0; j < ny; j++) { bx[giM(g, j)] = zb[j * nx]; bx[giP(g, j)] = bx[j * nx + nx - 1]; }
+    for (int i = 0; i < nx; i++) { bx[gjM(g, i)] = bx[i]; bx[gjP(g, i)] = bx[N - nx + i]; }
+    for (int c = 0; c < N; c++) { double a = zb[c], b = zb[G.Nn(c)]; by[c] = a > b ? a : b; }
+    for (int i = 0; i < nx; i++) {
+        double a = zb[gjM(g, i)], b = zb[i];
+        by[gjM(g, i)] = a > b ? a : b;
+        by[gjP(g, i)] = by[N - nx + i];
+    }
+    for (int j = 0; j < ny; j++) { by[giM(g, j)] = by[j * nx]; by[giP(g, j)] = by[j * nx + nx - 1]; }
+    // initCD + initFieldValues
+    for (int c = 0; c < Ng; c++) { h.CDXP[c] = k.CDx; h.CDYP[c] = k.CDy; }
+    for (int c = 0; c < N; c++) {
+        h.surf[c] = (ic.surf0cell ? ic.surf0cell[c] : ic.surf0) + k.offset;
+        h.uuXP[c] = ic.U0cell ? ic.U0cell[c] : ic.U0;
+        h.vvYP[c] = ic.V0cell ? ic.V0cell[c] : ic.V0;
+        if (k.useScalar == 1)
+            for (int q = 0; q < ns; q++)
+                h.S[(size_t)q * Ng + c] = ic.S0cell ? ic.S0cell[(size_t)q * N + c] : ic.S0;
+    }
+    // zeroNegSurf(tt = 0)
+    for (int c = 0; c < N; c++) if (h.surf[c] < zb[c]) h.surf[c] = zb[c];
+    // enforceFreeSurfaceBC / enforceVelocityBC
+    for (int kk = N; kk < Ng; kk++) {
+        h.surf[kk] = h.surf[ghost_source(g, kk, false)];
+        h.uuXP[kk] = h.uuXP[ghost_source(g, kk, true)];
+        h.vvYP[kk] = h.vvYP[ghost_source(g, kk, false)];
+    }
+    // enforceTidalBC(tt = 0)
+    if (k.bcType != 2)
+        for (int q = 0; q < k.tideLocLengthP; q++) {
+            int c = f.tideLocP[q];
+            if (f.tideP[0] + k.offset > zb[c]) h.surf[c] = f.tideP[0] + k.offset;
+        }
+    if (k.bcType != 1)
+        for (int q = 0; q < k.tideLocLengthM; q++) {
+            int c = f.tideLocM[q];
+            if (f.tideM[0] + k.offset > zb[c]) h.surf[c] = f.tideM[0] + k.offset;
+        }
+    // updateDepth + updateAllFaceDepth
+    double *hs = h.surf.data(), *dX = h.depthXP.data(), *dY = h.depthYP.data();
+    for (int c = 0; c < N; c++) { double d = hs[c] - zb[c]; h.depth[c] = d < 0 ? 0 : d; }
+    for (int c = 0; c < N; c++) { double a = hs[c], b = hs[G.E(c)]; dX[c] = (a > b ? a : b) - bx[c]; }
+    for (int j = 0; j < ny; j++) {
+        dX[j * nx + nx - 1] = 0; dX[giP(g, j)] = 0; dX[giM(g, j)] = 0;
+    }
+    for (int i = 0; i < nx; i++) {
+        int gm = gjM(g, i), gp = gjP(g, i);
+        dX[gm] = (hs[gm] > hs[gm + 1] ? hs[gm] : hs[gm + 1]) - bx[gm];
+        dX[gp] = (hs[gp] > hs[gp + 1] ? hs[gp] : hs[gp + 1]) - bx[gp];
+    }
+    for (int c = 0; c < N; c++) { double a = hs[c], b = hs[G.Nn(c)]; dY[c] = (a > b ? a : b) - by[c]; }
+    for (int i = 0; i < nx; i++) { dY[gjP(g, i)] = hs[gjP(g, i)] - by[gjP(g, i)]; dY[gjM(g, i)] = 0; }
+    for (int j = 0; j < ny; j++) { dY[giM(g, j)] = dY[j * nx]; dY[giP(g, j)] = dY[j * nx + nx - 1]; }
+    for (int i = 0; i < nx; i++) { dY[N - nx + i] = 0; dY[gjP(g, i)] = 0; dY[gjM(g, i)] = 0; }
+    for (int c = 0; c < Ng; c++) { if (dX[c] < 0) dX[c] = 0; if (dY[c] < 0) dY[c] = 0; }
+    // computeVolume
+    for (int c = 0; c < N; c++) {
+        h.cellV[c] = k.dx * k.dy * h.depth[c];
+        if (k.useScalar == 1)
+            for (int q = 0; q < ns; q++) h.Sm[(size_t)q * Ng + c] = h.S[(size_t)q * Ng + c] * h.cellV[c];
+    }
+    // velocityInterp (flat-index neighbours; indices below 0 read 0)
+    for (long c = 0; c < N; c++) {
+        const double *u = h.uuXP.data(), *v = h.vvYP.data();
+        double um1 = c - 1 >= 0 ? u[c - 1] : 0.0, um1n = c - 1 + nx >= 0 ? u[c - 1 + nx] : 0.0;
+        double vmn = c - nx >= 0 ? v[c - nx] : 0.0, vmn1 = c + 1 - nx >= 0 ? v[c + 1 - nx] : 0.0;
+        h.uuYP[c] = 0.25 * (u[c] + um1 + u[c + nx] + um1n);
+        h.vvXP[c] = 0.25 * (v[c] + v[c + 1] + vmn + vmn1);
+    }
+    // thinLayerDrag (called unconditionally by Init)
+    if (k.CDnotN == 1) {
+        for (int c = 0; c < Ng; c++) {
+            if (dX[c] <= k.hD) h.CDXP[c] = k.CDx + (k.CDmax - k.CDx) * (k.hD - dX[c]) / k.hD;
+            if (dY[c] <= k.hD) h.CDYP[c] = k.CDy + (k.CDmax - k.CDy) * (k.hD - dY[c]) / k.hD;
+        }
+    } else {
+        for (int c = 0; c < N; c++) {
+            if (h.depth[c] > 0) {
+                double n = f.manningCell ? f.manningCell[c] : k.manningN;
+                double coeff = k.g * n * n;
+                if (dX[c] <= k.hD) h.CDXP[c] = coeff / pow(h.depth[c], 2.0 / 3.0);
+                if (dY[c] <= k.hD) h.CDYP[c] = coeff / pow(h.depth[c], 2.0 / 3.0);
+            }
+        }
+    }
+    // scalar: zero on dry cells, scalarMass
+    if (k.useScalar == 1)
+        for (int q = 0; q < ns; q++) {
+            double *S = h.S.data() + (size_t)q * Ng, *Sm = h.Sm.data() + (size_t)q * Ng;
+            for (int c = 0; c < N; c++) if (h.depth[c] <= 0) S[c] = 0;
+            for (int c = 0; c < N; c++) Sm[c] = S[c] * h.cellV[c];
+        }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct KernelTimer {
+    const char *name;
+    double ms;
+    int64_t launches;
+};
+
+struct frehd_dev {
+    frehd_config cfg;
+    Grid g;
+    int device;
+    hipStream_t stream;
+    char *arena;
+    size_t arenaBytes;
+    StepArgs args;
+    double *depthA, *depthB, *pB;
+    int parity;                 // which depth buffer holds the current depth
+    int lastIters;
+    int32_t *hostFlags;         // pinned: {done, halted}
+    Ctl *ctlHost;               // pinned copy of the control block
+    int timing;
+    std::vector<KernelTimer> timers;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> eventPool;
+    size_t eventNext;
+};
+
+namespace {
+
+struct Carve {
+    char *base;
+    size_t off;
+    template <class T>
+    T *take(size_t n)
+    {
+        off = (off + 255) & ~size_t(255);
+        T *p = reinterpret_cast<T *>(base ? base + off : nullptr);
+        off += n * sizeof(T);
+        return p;
+    }
+};
+
+// lay out every device array; with base == nullptr only the size is computed
+size_t layout(frehd_dev *d, char *base)
+{
+    const frehd_config &k = d->cfg;
+    const size_t N = (size_t)k.nx * k.ny, Ng = (size_t)(k.nx + 2) * (k.ny + 2);
+    const int ns = std::max(1, k.nScalar);
+    const size_t Nt1 = (size_t)k.Nt + 1;
+    Carve cv{base, 0};
+    StepArgs &a = d->args;
+    a.ctl = cv.take<Ctl>(1);
+    double **ngArr[] = {&a.surf, &a.surfOld, &a.uuXP, &a.vvYP, &a.depthXP, &a.depthYP, &a.EnXP,
+        &a.EnYP, &a.DragXP, &a.DragYP, &a.CDXP, &a.CDYP, &a.Fuu, &a.Fvv, &a.CFLx, &a.CFLy,
+        &a.wtfXP, &a.wtfYP, &a.bottomZ, &a.bottomXP, &a.bottomYP};
+    for (auto **p : ngArr) *p = cv.take<double>(Ng);
+    a.sStride = (int64_t)((Ng + 31) & ~size_t(31));
+    a.S = cv.take<double>((size_t)a.sStride * ns);
+    a.Sm = cv.take<double>((size_t)a.sStride * ns);
+    double **nArr[] = {&a.uuYP, &a.vvXP, &d->depthA, &d->depthB, &a.cellV, &a.Snew, &a.cX, &a.cY,
+        &a.D, &a.b, &a.x, &a.r, &a.q, &a.pA, &d->pB};
+    for (auto **p : nArr) *p = cv.take<double>(N);
+    a.manning = nullptr;
+    a.rmv = cv.take<uint8_t>(N);
+    a.flags = cv.take<uint8_t>(N);
+    const size_t nb2d = (size_t)((k.nx + 63) / 64) * ((k.ny + 3) / 4);
+    const size_t nb1d = (Ng + 255) / 256;
+    a.partials = cv.take<double>(2 * std::max(nb2d, nb1d) + 64);
+    a.tideP = cv.take<double>(Nt1);
+    a.tideM = cv.take<double>(Nt1);
+    a.inflow = cv.take<double>(Nt1);
+    a.tidalPS = cv.take<double>(Nt1);
+    a.tidalMS = cv.take<double>(Nt1);
+    a.tideLocP = cv.take<int>(std::max(1, k.tideLocLengthP));
+    a.tideLocM = cv.take<int>(std::max(1, k.tideLocLengthM));
+    return cv.off + 256;
+}
+
+size_t layout_manning(frehd_dev *d, char *base, size_t off)
+{
+    Carve cv{base, off};
+    double *m = cv.take<double>((size_t)d->cfg.nx * d->cfg.ny);
+    if (base) d->args.manning = m;
+    return cv.off + 256;
+}
+
+int check_config(const frehd_config *k)
+{
+    if (!k) return fail(FREHD_EINVAL, "config is NULL");
+    if (k->nx < 4 || k->ny < 4) return fail(FREHD_EINVAL, "grid must be at least 4x4 (nx=%d ny=%d)", k->nx, k->ny);
+    if ((int64_t)(k->nx + 2) * (k->ny + 2) >= (int64_t)1 << 31)
+        return fail(FREHD_EINVAL, "grid too large for 32-bit cell indices");
+    if (k->Nt < 1) return fail(FREHD_EINVAL, "Nt must be >= 1");
+    if (!(k->dx > 0 && k->dy > 0 && k->dt > 0)) return fail(FREHD_EINVAL, "dx, dy, dt must be > 0");
+    if (k->precond != FREHD_PRECOND_JACOBI)
+        return fail(FREHD_EINVAL, "the device solver supports only the Jacobi preconditioner "
+                                  "(SSOR's triangular sweeps are sequential)");
+    if (k->precision != FREHD_PREC_FP64 && k->precision != FREHD_PREC_MIXED)
+        return fail(FREHD_EINVAL, "unknown precision %d", k->precision);
+    if (k->precision == FREHD_PREC_MIXED) return fail(FREHD_EINVAL, "mixed precision is not built in this library");
+    if (k->useScalar == 1 && k->scalarAdv != 1)
+        return fail(FREHD_EINVAL, "only first-order upwind scalar advection (scalarAdv = 1) is supported");
+    if (k->bcType < 1 || k->bcType > 3) return fail(FREHD_EINVAL, "bcType must be 1, 2 or 3");
+    if (k->tideLocLengthP < 0 || k->tideLocLengthM < 0 || k->inflowLocLength < 0)
+        return fail(FREHD_EINVAL, "negative location count");
+    if (k->useScalar == 1 && k->nScalar > 16) return fail(FREHD_EINVAL, "at most 16 scalars");
+    return 0;
+}
+
+int build_flags(const frehd_config &k, const frehd_forcing &f, std::vector<uint8_t> &flags)
+{
+    const int N = k.nx * k.ny;
+    flags.assign(N, 0);
+    for (int q = 0; q < k.tideLocLengthP; q++) {
+        int c = f.tideLocP[q];
+        if (c < 0 || c >= N) return fail(FREHD_EINVAL, "tideLocP[%d] = %d out of range", q, c);
+        flags[c] |= 1;
+    }
+    for (int q = 0; q < k.tideLocLengthM; q++) {
+        int c = f.tideLocM[q];
+        if (c < 0 || c >= N) return fail(FREHD_EINVAL, "tideLocM[%d] = %d out of range", q, c);
+        flags[c] |= 2;
+    }
+    for (int q = 0; q < k.inflowLocLength; q++) {
+        int c = f.inflowLoc[q];
+        if (c < 0 || c >= N) return fail(FREHD_EINVAL, "inflowLoc[%d] = %d out of range", q, c);
+        if ((flags[c] >> 2) == 63) return fail(FREHD_EINVAL, "cell %d listed more than 63 times as inflow", c);
+        flags[c] += 4;
+    }
+    return 0;
+}
+
+void fill_params(P &p, const frehd_config &k)
+{
+    memset(&p, 0, sizeof p);
+    p.dx = k.dx; p.dy = k.dy; p.dt = k.dt; p.g = k.g; p.NUx = k.NUx; p.NUy = k.NUy;
+    p.CDx = k.CDx; p.CDy = k.CDy; p.manningN = k.manningN; p.hD = k.hD; p.CDmax = k.CDmax;
+    p.minDepth = k.minDepth; p.wtfh = k.wtfh; p.CFLl = k.CFLl; p.CFLh = k.CFLh; p.eps = k.eps;
+    p.tidePS = k.tidePS; p.tideMS = k.tideMS; p.inflowS = k.inflowS; p.Kx = k.Kx; p.Ky = k.Ky;
+    p.offset = k.offset; p.CDnotN = k.CDnotN; p.useThinLayer = k.useThinLayer; p.bcType = k.bcType;
+    p.tideLocLengthP = k.tideLocLengthP; p.tideLocLengthM = k.tideLocLengthM;
+    p.inflowLocLength = k.inflowLocLength; p.useScalar = k.useScalar;
+    p.nScalar = k.useScalar == 1 ? std::max(1, k.nScalar) : 0;
+    p.useConstTidePS = k.useConstTidePS; p.useConstTideMS = k.useConstTideMS; p.maxIter = k.maxIter;
+}
+
+int upload(frehd_dev *d, const void *host, void *dev, size_t bytes)
+{
+    if (!bytes || !host) return 0;
+    HIPCHK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, d->stream));
+    return 0;
+}
+
+int upload_series(frehd_dev *d, const double *src, const double *dst)
+{
+    const size_t n = (size_t)d->cfg.Nt + 1;
+    if (src) return upload(d, src, (void *)dst, n * sizeof(double));
+    HIPCHK(hipMemsetAsync((void *)dst, 0, n * sizeof(double), d->stream));
+    return 0;
+}
+
+int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **out, frehd_dev **dd)
+{
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    if (!f || !out) return fail(FREHD_EINVAL, "forcing/out is NULL");
+    if (cfg->bcType != 2 && cfg->tideLocLengthP > 0 && !(f->tideLocP && f->tideP))
+        return fail(FREHD_EINVAL, "tideLocP/tideP missing");
+    if (cfg->bcType != 1 && cfg->tideLocLengthM > 0 && !(f->tideLocM && f->tideM))
+        return fail(FREHD_EINVAL, "tideLocM/tideM missing");
+    if (cfg->inflowLocLength > 0 && !(f->inflowLoc && f->inflow))
+        return fail(FREHD_EINVAL, "inflowLoc/inflow missing");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FREHD_EHIP, "no HIP device visible");
+    frehd_dev *d = new frehd_dev();
+    d->cfg = *cfg;
+    if (d->cfg.nScalar < 1) d->cfg.nScalar = 1;
+    d->g = Grid{cfg->nx, cfg->ny, cfg->nx * cfg->ny, (cfg->nx + 2) * (cfg->ny + 2)};
+    HIPCHK(hipGetDevice(&d->device));
+    HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    size_t bytes = layout(d, nullptr);
+    if (f->manningCell) bytes = layout_manning(d, nullptr, bytes);
+    d->arenaBytes = bytes;
+    if (hipMalloc(&d->arena, bytes) != hipSuccess) {
+        hipStreamDestroy(d->stream);
+        delete d;
+        return fail(FREHD_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+    }
+    HIPCHK(hipMemsetAsync(d->arena, 0, bytes, d->stream));
+    size_t used = layout(d, d->arena);
+    if (f->manningCell) layout_manning(d, d->arena, used);
+    d->args.g = d->g;
+    fill_params(d->args.p, d->cfg);
+    HIPCHK(hipHostMalloc((void **)&d->hostFlags, 64, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&d->ctlHost, sizeof(Ctl), hipHostMallocDefault));
+    d->parity = 0;
+    d->lastIters = 16;
+    d->timing = 0;
+    d->eventNext = 0;
+    // forcing
+    const StepArgs &a = d->args;
+    if ((rc = upload_series(d, f->tideP, a.tideP))) return rc;
+    if ((rc = upload_series(d, f->tideM, a.tideM))) return rc;
+    if ((rc = upload_series(d, f->inflow, a.inflow))) return rc;
+    if ((rc = upload_series(d, f->tidalPS, a.tidalPS))) return rc;
+    if ((rc = upload_series(d, f->tidalMS, a.tidalMS))) return rc;
+    if (cfg->tideLocLengthP > 0 && f->tideLocP)
+        if ((rc = upload(d, f->tideLocP, (void *)a.tideLocP, sizeof(int) * cfg->tideLocLengthP))) return rc;
+    if (cfg->tideLocLengthM > 0 && f->tideLocM)
+        if ((rc = upload(d, f->tideLocM, (void *)a.tideLocM, sizeof(int) * cfg->tideLocLengthM))) return rc;
+    if (f->manningCell)
+        if ((rc = upload(d, f->manningCell, d->args.manning, sizeof(double) * d->g.N))) return rc;
+    std::vector<uint8_t> flags;
+    if ((rc = build_flags(d->cfg, *f, flags))) return rc;
+    if ((rc = upload(d, flags.data(), d->args.flags, flags.size()))) return rc;
+    HIPCHK(hipStreamSynchronize(d->stream));
+    *dd = d;
+    return 0;
+}
+
+void set_depth_pointers(frehd_dev *d)
+{
+    d->args.depthOld = d->parity ? d->depthB : d->depthA;
+    d->args.depthNew = d->parity ? d->depthB : d->depthA;
+}
+
+int upload_state(frehd_dev *d, const frehd_fields &s)
+{
+    const size_t N = d->g.N, Ng = d->g.Ng;
+    StepArgs &a = d->args;
+    struct { const double *src; double *dst; size_t n; } ng[] = {
+        {s.surf, a.surf, Ng}, {s.surfOld, a.surfOld, Ng}, {s.uuXP, a.uuXP, Ng}, {s.vvYP, a.vvYP, Ng},
+        {s.uuYP, a.uuYP, N}, {s.vvXP, a.vvXP, N}, {s.depth, d->depthA, N}, {s.depthXP, a.depthXP, Ng},
+        {s.depthYP, a.depthYP, Ng}, {s.EnXP, a.EnXP, Ng}, {s.EnYP, a.EnYP, Ng}, {s.DragXP, a.DragXP, Ng},
+        {s.DragYP, a.DragYP, Ng}, {s.CDXP, a.CDXP, Ng}, {s.CDYP, a.CDYP, Ng}, {s.Fuu, a.Fuu, Ng},
+        {s.Fvv, a.Fvv, Ng}, {s.cellV, a.cellV, N}, {s.CFLx, a.CFLx, Ng}, {s.CFLy, a.CFLy, Ng},
+        {s.wtfXP, a.wtfXP, Ng}, {s.wtfYP, a.wtfYP, Ng}, {s.bottomZ, a.bottomZ, Ng},
+        {s.bottomXP, a.bottomXP, Ng}, {s.bottomYP, a.bottomYP, Ng}};
+    for (auto &e : ng) {
+        int rc = upload(d, e.src, e.dst, e.n * sizeof(double));
+        if (rc) return rc;
+    }
+    const int ns = d->args.p.nScalar;
+    for (int q = 0; q < ns; q++) {
+        if (s.S) { int rc = upload(d, s.S + (size_t)q * Ng, a.S + (size_t)q * a.sStride, Ng * sizeof(double)); if (rc) return rc; }
+        if (s.Sm) { int rc = upload(d, s.Sm + (size_t)q * N, a.Sm + (size_t)q * a.sStride, N * sizeof(double)); if (rc) return rc; }
+    }
+    d->parity = 0;
+    set_depth_pointers(d);
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+// ---- kernel launches with optional per-kernel event timing ----------------
+enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_FLUX, K_TIDEV,
+                K_FINISH, K_SCALAR, K_SCALARBC, K_END, K_COUNT };
+const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble", "k_pcg_pq",
+    "k_pcg_update", "k_surface", "k_limiter", "k_faces", "k_flux", "k_tidal_velocity",
+    "k_finish", "k_scalar", "k_scalar_bc", "k_step_end"};
+
+hipEvent_t next_event(frehd_dev *d)
+{
+    if (d->eventNext == d->eventPool.size()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        d->eventPool.push_back(e);
+    }
+    return d->eventPool[d->eventNext++];
+}
+
+template <class F>
+int launch(frehd_dev *d, KernelId id, F &&fn)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (d->timing) { e0 = next_event(d); e1 = next_event(d); hipEventRecord(e0, d->stream); }
+    fn();
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return fail(FREHD_EHIP, "launch of %s failed: %s", kKernelNames[id], hipGetErrorString(err));
+    if (d->timing) {
+        hipEventRecord(e1, d->stream);
+        d->pending.push_back({id, {e0, e1}});
+    }
+    return 0;
+}
+
+int collect_timing(frehd_dev *d)
+{
+    if (!d->timing) return 0;
+    HIPCHK(hipStreamSynchronize(d->stream));
+    for (auto &pe : d->pending) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
+        d->timers[pe.first].ms += ms;
+        d->timers[pe.first].launches += 1;
+    }
+    d->pending.clear();
+    d->eventNext = 0;
+    return 0;
+}
+
+#define LAUNCH(id, kern, grid, block, ...)                                                     \
+    do {                                                                                      \
+        int rc_ = launch(d, id, [&] { hipLaunchKernelGGL(kern, grid, block, 0, d->stream, __VA_ARGS__); }); \
+        if (rc_) return rc_;                                                                  \
+    } while (0)
+
+int enqueue_pcg_iteration(frehd_dev *d, int it)
+{
+    StepArgs a = d->args;
+    a.pOld = (it & 1) ? d->pB : a.pA;
+    a.pNew = (it & 1) ? a.pA : d->pB;
+    const Grid &g = d->g;
+    dim3 b2(64, 4), g2((g.nx + 63) / 64, (g.ny + 3) / 4);
+    LAUNCH(K_PQ, k_pcg_pq, g2, b2, a);
+    LAUNCH(K_UPD, k_pcg_update, dim3((g.N + 255) / 256), dim3(256), a);
+    return 0;
+}
+
+int read_flags(frehd_dev *d)
+{
+    HIPCHK(hipMemcpyAsync(d->hostFlags, &d->args.ctl->pcg.done, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipMemcpyAsync(d->hostFlags + 1, &d->args.ctl->halted, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipMemcpyAsync(d->hostFlags + 2, &d->args.ctl->pcg.iter, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+int enqueue_phase(frehd_dev *d, int phase)
+{
+    const Grid &g = d->g;
+    StepArgs &a = d->args;
+    dim3 b2(64, 4), g2((g.nx + 63) / 64, (g.ny + 3) / 4);
+    dim3 b1(256), gNg((g.Ng + 255) / 256);
+    double *cur = d->parity ? d->depthB : d->depthA;
+    double *nxt = d->parity ? d->depthA : d->depthB;
+    switch (phase) {
+    case 0:   // solveSourceTerm (up to the RHS, which the assembly computes)
+        LAUNCH(K_BEGIN, k_step_begin, dim3(1), dim3(64), a);
+        LAUNCH(K_PRED, k_predictor, g2, b2, a);
+        return 0;
+    case 1: { // solveFreeSurface
+        LAUNCH(K_ASM, k_assemble, g2, b2, a);
+        // PCG: launch the previous step's iteration count, then small chunks,
+        // until the device reports convergence (or halt); one iteration per
+        // chunk while timing, so no idle launch enters the averages
+        int it = 0;
+        int chunk = d->timing ? 1 : std::max(1, d->lastIters);
+        for (;;) {
+            for (int q = 0; q < chunk; q++, it++) {
+                int rc = enqueue_pcg_iteration(d, it);
+                if (rc) return rc;
+            }
+            int rc = read_flags(d);
+            if (rc) return rc;
+            if (d->hostFlags[0] || d->hostFlags[1]) break;
+            if (it > d->cfg.maxIter + 8) return fail(FREHD_EHIP, "PCG did not report completion");
+            chunk = d->timing ? 1 : 4;
+        }
+        d->lastIters = std::max(1, d->hostFlags[2]);
+        LAUNCH(K_SURF, k_surface, gNg, b1, a);
+        return 0;
+    }
+    case 2: { // updateData (+ the CFL numbers of monitorCFL)
+        a.depthOld = cur;
+        a.depthNew = nxt;
+        LAUNCH(K_LIM, k_limiter, g2, b2, a);
+        LAUNCH(K_FACES, k_faces, gNg, b1, a);
+        LAUNCH(K_FLUX, k_flux, gNg, b1, a);
+        int nt = (a.p.bcType != 2 ? a.p.tideLocLengthP : 0) + (a.p.bcType != 1 ? a.p.tideLocLengthM : 0);
+        if (nt > 0) LAUNCH(K_TIDEV, k_tidal_velocity, dim3((nt + 255) / 256), b1, a);
+        LAUNCH(K_FINISH, k_finish, gNg, b1, a);
+        d->parity ^= 1;
+        return 0;
+    }
+    case 3:   // scalarTransport
+        a.depthNew = cur;
+        if (a.p.useScalar == 1)
+            for (int q = 0; q < a.p.nScalar; q++) {
+                LAUNCH(K_SCALAR, k_scalar, g2, b2, a, q);
+                LAUNCH(K_SCALARBC, k_scalar_bc, gNg, b1, a, q);
+            }
+        return 0;
+    case 4:   // monitorCFL's stop test
+        LAUNCH(K_END, k_step_end, dim3(1), dim3(64), a);
+        return 0;
+    default:
+        return fail(FREHD_EINVAL, "phase %d out of range", phase);
+    }
+}
+
+int enqueue_step(frehd_dev *d)
+{
+    for (int ph = 0; ph < 5; ph++) {
+        int rc = enqueue_phase(d, ph);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *frehd_last_error(void) { return g_err.c_str(); }
+
+const char *frehd_version(void) { return "frehd-mi355x 0.1 (gfx950, FP64 plain-grid step, Jacobi PCG)"; }
+
+int frehd_device_count(int32_t *count)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    if (count) *count = n;
+    return 0;
+}
+
+int frehd_abi_sizes(int64_t sizes[5])
+{
+    sizes[0] = sizeof(frehd_config);
+    sizes[1] = sizeof(frehd_forcing);
+    sizes[2] = sizeof(frehd_ic);
+    sizes[3] = sizeof(frehd_fields);
+    sizes[4] = sizeof(frehd_stats);
+    return 0;
+}
+
+int frehd_dev_footprint(const frehd_config *cfg, int64_t *bytes)
+{
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    frehd_dev tmp;
+    tmp.cfg = *cfg;
+    if (tmp.cfg.nScalar < 1) tmp.cfg.nScalar = 1;
+    size_t b = layout(&tmp, nullptr);
+    b = layout_manning(&tmp, nullptr, b);
+    *bytes = (int64_t)b;
+    return 0;
+}
+
+int frehd_dev_create(const frehd_config *cfg, const frehd_forcing *f, const frehd_ic *ic, frehd_dev_t **out)
+{
+    if (!ic || !ic->bottomZ) return fail(FREHD_EINVAL, "ic/bottomZ is NULL");
+    if (cfg && f) {
+        if (cfg->bcType != 2 && cfg->tideLocLengthP > 0 && !(f->tideLocP && f->tideP))
+            return fail(FREHD_EINVAL, "tideLocP/tideP missing");
+        if (cfg->bcType != 1 && cfg->tideLocLengthM > 0 && !(f->tideLocM && f->tideM))
+            return fail(FREHD_EINVAL, "tideLocM/tideM missing");
+    }
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    if (!f) return fail(FREHD_EINVAL, "forcing is NULL");
+    HostState h;
+    host_init(h, *cfg, *f, *ic);
+    frehd_dev *d = nullptr;
+    if ((rc = create_common(cfg, f, out, &d))) return rc;
+    frehd_fields s;
+    memset(&s, 0, sizeof s);
+    s.surf = h.surf.data(); s.surfOld = h.surfOld.data(); s.uuXP = h.uuXP.data(); s.vvYP = h.vvYP.data();
+    s.uuYP = h.uuYP.data(); s.vvXP = h.vvXP.data(); s.depth = h.depth.data(); s.depthXP = h.depthXP.data();
+    s.depthYP = h.depthYP.data(); s.EnXP = h.EnXP.data(); s.EnYP = h.EnYP.data(); s.DragXP = h.DragXP.data();
+    s.DragYP = h.DragYP.data(); s.CDXP = h.CDXP.data(); s.CDYP = h.CDYP.data(); s.Fuu = h.Fuu.data();
+    s.Fvv = h.Fvv.data(); s.cellV = h.cellV.data(); s.CFLx = h.CFLx.data(); s.CFLy = h.CFLy.data();
+    s.wtfXP = h.wtfXP.data(); s.wtfYP = h.wtfYP.data(); s.bottomZ = h.bottomZ.data();
+    s.bottomXP = h.bottomXP.data(); s.bottomYP = h.bottomYP.data();
+    // S in Ng stride, Sm in N stride for upload_state
+    std::vector<double> Sm((size_t)h.ns * h.N);
+    for (int q = 0; q < h.ns; q++)
+        memcpy(Sm.data() + (size_t)q * h.N, h.Sm.data() + (size_t)q * h.Ng, sizeof(double) * h.N);
+    s.S = h.S.data();
+    s.Sm = Sm.data();
+    if ((rc = upload_state(d, s))) { frehd_dev_destroy(d); return rc; }
+    *out = d;
+    return 0;
+}
+
+int frehd_dev_create_from_state(const frehd_config *cfg, const frehd_forcing *f, const frehd_fields *state,
+                                frehd_dev_t **out)
+{
+    if (!state) return fail(FREHD_EINVAL, "state is NULL");
+    const double *req[] = {state->surf, state->surfOld, state->uuXP, state->vvYP, state->uuYP, state->vvXP,
+        state->depth, state->depthXP, state->depthYP, state->CDXP, state->CDYP, state->cellV,
+        state->CFLx, state->CFLy, state->bottomZ, state->bottomXP, state->bottomYP};
+    for (auto *p : req) if (!p) return fail(FREHD_EINVAL, "state array missing");
+    if (cfg && cfg->useScalar == 1 && !(state->S && state->Sm)) return fail(FREHD_EINVAL, "state S/Sm missing");
+    frehd_dev *d = nullptr;
+    int rc = create_common(cfg, f, out, &d);
+    if (rc) return rc;
+    if ((rc = upload_state(d, *state))) { frehd_dev_destroy(d); return rc; }
+    *out = d;
+    return 0;
+}
+
+int frehd_dev_step(frehd_dev_t *d, int32_t nsteps)
+{
+    if (!d) return fail(FREHD_EINVAL, "handle is NULL");
+    if (nsteps < 0) return fail(FREHD_EINVAL, "nsteps < 0");
+    HIPCHK(hipSetDevice(d->device));
+    Ctl c;
+    HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    if (c.halted) return fail(FREHD_ECFL, "handle halted at step %d: max CFL exceeded 20", c.haltStep);
+    if (c.tt + nsteps > d->cfg.Nt)
+        return fail(FREHD_EINVAL, "stepping to %d exceeds the forcing series (Nt = %d)", c.tt + nsteps, d->cfg.Nt);
+    if (d->timing && d->timers.empty()) {
+        d->timers.resize(K_COUNT);
+        for (int q = 0; q < K_COUNT; q++) d->timers[q] = KernelTimer{kKernelNames[q], 0.0, 0};
+    }
+    for (int s = 0; s < nsteps; s++) {
+        int rc = enqueue_step(d);
+        if (rc) return rc;
+        if (d->timing && (s % 8 == 7)) { rc = collect_timing(d); if (rc) return rc; }
+        if (d->hostFlags[1]) break;
+    }
+    int rc = collect_timing(d);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(d->stream));
+    HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    if (c.halted) return fail(FREHD_ECFL, "max CFL exceeded 20 at step %d (CFLx %.3g, CFLy %.3g)", c.haltStep,
+                              c.lastMaxCFLx, c.lastMaxCFLy);
+    return 0;
+}
+
+int frehd_dev_phase(frehd_dev_t *d, int32_t phase)
+{
+    if (!d) return fail(FREHD_EINVAL, "handle is NULL");
+    HIPCHK(hipSetDevice(d->device));
+    int rc = enqueue_phase(d, phase);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+int frehd_dev_download(frehd_dev_t *d, frehd_fields *o)
+{
+    if (!d || !o) return fail(FREHD_EINVAL, "NULL argument");
+    HIPCHK(hipSetDevice(d->device));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    const size_t N = d->g.N, Ng = d->g.Ng;
+    StepArgs &a = d->args;
+    // the current depth is the buffer the last step wrote
+    const double *depthCur = d->parity ? d->depthB : d->depthA;
+    struct { double *dst; const double *src; size_t n; } tab[] = {
+        {o->surf, a.surf, Ng}, {o->surfOld, a.surfOld, Ng}, {o->uuXP, a.uuXP, Ng}, {o->vvYP, a.vvYP, Ng},
+        {o->uuYP, a.uuYP, N}, {o->vvXP, a.vvXP, N}, {o->depth, depthCur, N}, {o->depthXP, a.depthXP, Ng},
+        {o->depthYP, a.depthYP, Ng}, {o->EnXP, a.EnXP, Ng}, {o->EnYP, a.EnYP, Ng}, {o->DragXP, a.DragXP, Ng},
+        {o->DragYP, a.DragYP, Ng}, {o->CDXP, a.CDXP, Ng}, {o->CDYP, a.CDYP, Ng}, {o->Fuu, a.Fuu, Ng},
+        {o->Fvv, a.Fvv, Ng}, {o->cellV, a.cellV, N}, {o->CFLx, a.CFLx, Ng}, {o->CFLy, a.CFLy, Ng},
+        {o->wtfXP, a.wtfXP, Ng}, {o->wtfYP, a.wtfYP, Ng}, {o->bottomZ, a.bottomZ, Ng},
+        {o->bottomXP, a.bottomXP, Ng}, {o->bottomYP, a.bottomYP, Ng}};
+    for (auto &e : tab)
+        if (e.dst) HIPCHK(hipMemcpy(e.dst, e.src, e.n * sizeof(double), hipMemcpyDeviceToHost));
+    const int ns = std::max(1, a.p.nScalar);
+    for (int q = 0; q < ns; q++) {
+        if (o->S) HIPCHK(hipMemcpy(o->S + (size_t)q * Ng, a.S + (size_t)q * a.sStride, Ng * sizeof(double), hipMemcpyDeviceToHost));
+        if (o->Sm) HIPCHK(hipMemcpy(o->Sm + (size_t)q * N, a.Sm + (size_t)q * a.sStride, N * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int frehd_dev_stats(frehd_dev_t *d, frehd_stats *out)
+{
+    if (!d || !out) return fail(FREHD_EINVAL, "NULL argument");
+    HIPCHK(hipSetDevice(d->device));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    Ctl c;
+    HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    out->steps = c.steps;
+    out->pcgIterTotal = c.pcgIterTotal;
+    out->pcgIterLast = c.pcgIterLast;
+    out->refineLast = 0;
+    out->relResLast = c.relResLast;
+    out->maxCFLx = c.lastMaxCFLx;
+    out->maxCFLy = c.lastMaxCFLy;
+    for (int q = 0; q < 4; q++) out->Vloss[q] = c.Vloss[q];
+    return 0;
+}
+
+int frehd_dev_step_index(frehd_dev_t *d, int32_t *tt)
+{
+    if (!d || !tt) return fail(FREHD_EINVAL, "NULL argument");
+    Ctl c;
+    HIPCHK(hipStreamSynchronize(d->stream));
+    HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    *tt = c.tt;
+    return 0;
+}
+
+int frehd_dev_set_step_index(frehd_dev_t *d, int32_t tt)
+{
+    if (!d) return fail(FREHD_EINVAL, "handle is NULL");
+    if (tt < 0 || tt > d->cfg.Nt) return fail(FREHD_EINVAL, "step index %d outside [0, Nt]", tt);
+    HIPCHK(hipSetDevice(d->device));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    HIPCHK(hipMemcpy(&d->args.ctl->tt, &tt, sizeof(int), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int frehd_dev_destroy(frehd_dev_t *d)
+{
+    if (!d) return 0;
+    hipSetDevice(d->device);
+    hipStreamSynchronize(d->stream);
+    for (auto e : d->eventPool) hipEventDestroy(e);
+    if (d->arena) hipFree(d->arena);
+    if (d->hostFlags) hipHostFree(d->hostFlags);
+    if (d->ctlHost) hipHostFree(d->ctlHost);
+    hipStreamDestroy(d->stream);
+    delete d;
+    return 0;
+}
+
+int frehd_dev_enable_timing(frehd_dev_t *d, int32_t on)
+{
+    if (!d) return fail(FREHD_EINVAL, "handle is NULL");
+    d->timing = on ? 1 : 0;
+    d->timers.clear();
+    d->pending.clear();
+    d->eventNext = 0;
+    return 0;
+}
+
+int frehd_dev_kernel_times(frehd_dev_t *d, int32_t cap, const char **names, double *ms, int64_t *launches,
+                           int32_t *n)
+{
+    if (!d || !n) return fail(FREHD_EINVAL, "NULL argument");
+    int cnt = 0;
+    for (auto &t : d->timers) {
+        if (t.launches == 0) continue;
+        if (cnt < cap) {
+            if (names) names[cnt] = t.name;
+            if (ms) ms[cnt] = t.ms / (double)t.launches;
+            if (launches) launches[cnt] = t.launches;
+        }
+        cnt++;
+    }
+    *n = cnt;
+    return 0;
+}
+
+}  // extern "C"

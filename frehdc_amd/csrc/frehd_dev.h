@@ -1,0 +1,127 @@
+// frehd_dev.h -- device-side layout, index maps and reductions shared by the
+// kernels of the free-surface step (gfx950, wave64).
+//
+// Layout: the reference's flat interior-first array with ghost blocks
+// appended (FrehdC_v42/map.c:15-128), so x is the fastest index and every
+// stencil read of an interior neighbour is a unit-stride (coalesced) access.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace frehd {
+
+struct Grid {
+    int nx, ny, N, Ng;
+};
+
+// ghost block offsets (map.c:58-104)
+__host__ __device__ inline int gjP(const Grid &g, int i) { return g.N + i; }
+__host__ __device__ inline int gjM(const Grid &g, int i) { return g.N + g.nx + i; }
+__host__ __device__ inline int giP(const Grid &g, int j) { return g.N + 2 * g.nx + j; }
+__host__ __device__ inline int giM(const Grid &g, int j) { return g.N + 2 * g.nx + g.ny + j; }
+__host__ __device__ inline int cIMJP(const Grid &g) { return g.Ng - 4; }
+__host__ __device__ inline int cIPJP(const Grid &g) { return g.Ng - 3; }
+__host__ __device__ inline int cIPJM(const Grid &g) { return g.Ng - 2; }
+__host__ __device__ inline int cIMJM(const Grid &g) { return g.Ng - 1; }
+
+// neighbours of interior cell c = j*nx + i
+__host__ __device__ inline int nbE(const Grid &g, int c, int i, int j) { return i == g.nx - 1 ? giP(g, j) : c + 1; }
+__host__ __device__ inline int nbW(const Grid &g, int c, int i, int j) { return i == 0 ? giM(g, j) : c - 1; }
+__host__ __device__ inline int nbN(const Grid &g, int c, int i, int j) { return j == g.ny - 1 ? gjP(g, i) : c + g.nx; }
+__host__ __device__ inline int nbS(const Grid &g, int c, int i, int j) { return j == 0 ? gjM(g, i) : c - g.nx; }
+
+// Interior cell whose value a ghost (or corner) index mirrors under the
+// reference's Neumann ghost fills (enforceFreeSurfaceBC nsfunctions.c:317-334,
+// enforceVelocityBC :289-314 for its u corners, enforceScalarBC scalar.c:423-432).
+// Returns -1 for an interior index.  `vcorners` selects the corner rule of
+// vvYP/surf (corners copy iP/iM ghosts) vs uuXP (corners copy jP/jM ghosts).
+__host__ __device__ inline int ghost_source(const Grid &g, int k, bool ucorners)
+{
+    const int nx = g.nx, ny = g.ny, N = g.N;
+    if (k < N) return -1;
+    int r = k - N;
+    if (r < nx) return N - nx + r;                 // jP ghost <- top row
+    r -= nx;
+    if (r < nx) return r;                          // jM ghost <- bottom row
+    r -= nx;
+    if (r < ny) return r * nx + nx - 1;            // iP ghost <- right column
+    r -= ny;
+    if (r < ny) return r * nx;                     // iM ghost <- left column
+    // corners
+    if (ucorners) {
+        if (k == cIPJP(g)) return N - 1;           // <- jPgt[nx-1] <- cell (nx-1, ny-1)
+        if (k == cIPJM(g)) return nx - 1;          // <- jMgt[nx-1] <- cell (nx-1, 0)
+        if (k == cIMJP(g)) return N - nx;          // <- jPgt[0]    <- cell (0, ny-1)
+        return 0;                                  // iMjM <- jMgt[0] <- cell 0
+    }
+    if (k == cIPJP(g)) return (ny - 1) * nx + nx - 1;  // <- iPgt[ny-1]
+    if (k == cIPJM(g)) return (ny - 1) * nx;           // <- iMgt[ny-1]
+    if (k == cIMJP(g)) return nx - 1;                  // <- iPgt[0]
+    return 0;                                          // iMjM <- iMgt[0]
+}
+
+// ---- reductions ------------------------------------------------------------
+template <int NV>
+struct Sums {
+    double v[NV];
+};
+
+__device__ inline double wave_sum(double x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+__device__ inline double wave_max(double x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+    return x;
+}
+
+// Block sum of NV values in a fixed order (deterministic for a given launch
+// shape).  Result valid in thread 0.  blockDim.x*y*z must be a multiple of 64
+// and at most 1024.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double *lds /* 16*NV */)
+{
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+    const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) v[q] = wave_sum(v[q]);
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) lds[wid * NV + q] = v[q];
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double s = 0.0;
+            for (int w = 0; w < nw; w++) s += lds[w * NV + q];
+            v[q] = s;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ inline double block_max(double v, double *lds)
+{
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+    const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
+    v = wave_max(v);
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    if (tid == 0) {
+        double m = lds[0];
+        for (int w = 1; w < nw; w++) m = fmax(m, lds[w]);
+        v = m;
+    }
+    __syncthreads();
+    return v;
+}
+
+}  // namespace frehd

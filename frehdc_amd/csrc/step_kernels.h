@@ -1,0 +1,76 @@
+// step_kernels.h -- argument block shared by the step kernels and the host
+// orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "frehd_dev.h"
+
+namespace frehd {
+
+// scalar settings (the Config fields the step reads)
+struct P {
+    double dx, dy, dt, g, NUx, NUy, CDx, CDy, manningN, hD, CDmax, minDepth, wtfh, CFLl, CFLh;
+    double eps, tidePS, tideMS, inflowS, Kx, Ky, offset;
+    int CDnotN, useThinLayer, bcType, tideLocLengthP, tideLocLengthM, inflowLocLength;
+    int useScalar, nScalar, useConstTidePS, useConstTideMS, maxIter, pad;
+};
+
+struct PcgScalars {
+    double bb, bNorm, rr, rNorm, rz, pq, alpha, beta;
+    int iter, done;
+};
+
+// device control block (one per handle)
+struct Ctl {
+    int tt;                   // current step number
+    int halted;               // set when max CFL > 20
+    int haltStep;
+    unsigned counter;         // last-block counter of the reductions
+    unsigned long long maxCFLxBits, maxCFLyBits;
+    double lastMaxCFLx, lastMaxCFLy;
+    double Vloss[4];
+    PcgScalars pcg;
+    int pcgIterLast;
+    int pad0;
+    long long pcgIterTotal;
+    long long steps;
+    double relResLast;
+};
+
+struct StepArgs {
+    Grid g;
+    P p;
+    Ctl *ctl;
+    // Ng arrays
+    double *surf, *surfOld, *uuXP, *vvYP, *depthXP, *depthYP, *EnXP, *EnYP, *DragXP, *DragYP;
+    double *CDXP, *CDYP, *Fuu, *Fvv, *CFLx, *CFLy, *wtfXP, *wtfYP, *bottomZ, *bottomXP, *bottomYP;
+    double *S, *Sm;           // nScalar arrays with stride sStride
+    int64_t sStride;
+    // interior arrays
+    double *uuYP, *vvXP, *depthOld, *depthNew, *cellV, *Snew, *manning;
+    uint8_t *rmv, *flags;     // flags: bit0 tideP cell, bit1 tideM cell, bits 2-7 inflow count
+    // operator and solver vectors
+    double *cX, *cY, *D, *b, *x, *r, *q, *pA, *pOld, *pNew, *partials;
+    // forcing (Nt+1 samples) and locations
+    const double *tideP, *tideM, *inflow, *tidalPS, *tidalMS;
+    const int *tideLocP, *tideLocM;
+};
+
+__global__ void k_step_begin(StepArgs a);
+__global__ void k_predictor(StepArgs a);
+__global__ void k_assemble(StepArgs a);
+__global__ void k_pcg_pq(StepArgs a);
+__global__ void k_pcg_update(StepArgs a);
+__global__ void k_surface(StepArgs a);
+__global__ void k_limiter(StepArgs a);
+__global__ void k_faces(StepArgs a);
+__global__ void k_flux(StepArgs a);
+__global__ void k_tidal_velocity(StepArgs a);
+__global__ void k_finish(StepArgs a);
+__global__ void k_scalar(StepArgs a, int q);
+__global__ void k_scalar_bc(StepArgs a, int q);
+__global__ void k_step_end(StepArgs a);
+
+}  // namespace frehd

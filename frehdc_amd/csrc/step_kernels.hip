@@ -1,0 +1,782 @@
+// step_kernels.hip -- FP64 kernels of one Casulli time step (plain grid).
+//
+// Each kernel covers a fused group of the reference's per-cell sweeps; every
+// arithmetic expression keeps the reference's evaluation order and the file
+// is compiled with -ffp-contract=off, so on identical inputs each kernel
+// reproduces the reference bit for bit (pow() of the drag update aside).
+// Sequential in-place sweeps of the reference are restated as per-cell
+// closed forms (see the comment at each kernel).
+#include "frehd_dev.h"
+#include "step_kernels.h"
+
+namespace frehd {
+
+#define LAS_IS_ZERO(a) (fabs(a) < 10.0 * 2.2250738585072014e-308)
+#define LAS_IS_ONE(a) (fabs((a)-1.0) < 10.0 * 2.220446049250313e-16)
+
+__device__ inline int cur_tt(const StepArgs &a) { return a.ctl->tt; }
+
+// ---------------------------------------------------------------------------
+// Step begin: tt += 1 (SolveAll's loop counter, nssolve.c:61).
+__global__ void k_step_begin(StepArgs a)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        Ctl *c = a.ctl;
+        if (c->halted) return;
+        c->tt += 1;
+        c->pcg.iter = 0;
+        c->pcg.done = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Predictor: explicitVelocity + advectionTerm + diffusionTerm + dragTerm +
+// dragInversion (nsfunctions.c:498-746).
+//
+// advectionTerm's CFL limiter walks cells in increasing index and rescales
+// the advection term of the cell and of its four neighbours in place, and
+// each cell's En is updated right after its own turn (nsfunctions.c:536-585).
+// So the term of cell c that enters En[c] has been rescaled by the cells
+// processed up to c that list c as a target: c - nx (c is its north
+// neighbour), c - 1 (when c is its east neighbour, i.e. i > 0) and c itself,
+// in that order.  Each rescale multiplies by (CFLh - CFL[c])/(CFLh - CFLl)
+// or zeroes, depending on the CFL of the cell doing the rescaling.
+__device__ inline void limit_x(double &aX, double cflK, double fX, double lo, double hi, double den)
+{
+    if (cflK > lo && cflK <= hi) aX = aX * fX / den;
+    else if (cflK > hi) aX = 0;
+}
+__device__ inline void limit_y(double &aY, double cflYK, double cflXK, double fY, double lo, double hi,
+                               double den)
+{
+    if (cflYK > lo && cflYK < hi) aY = aY * fY / den;
+    else if (cflXK > hi) aY = 0;   // tests CFLx, nsfunctions.c:574
+}
+
+__global__ void __launch_bounds__(256) k_predictor(StepArgs a)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny) return;
+    const int c = j * g.nx + i;
+    const int e = nbE(g, c, i, j), w = nbW(g, c, i, j), n = nbN(g, c, i, j), s = nbS(g, c, i, j);
+    const P &p = a.p;
+    const double dx = p.dx, dy = p.dy, dt = p.dt;
+    const double *u = a.uuXP, *v = a.vvYP;
+    const double uc = u[c], ue = u[e], uw = u[w], un = u[n], us = u[s];
+    const double vc = v[c], ve = v[e], vw = v[w], vn = v[n], vs = v[s];
+    const double vx = a.vvXP[c], uy = a.uuYP[c];
+
+    double aX = (0.5 / dx) * (uc + fabs(uc)) * (uc - uw) + (0.5 / dx) * (uc - fabs(uc)) * (ue - uc) +
+                (0.5 / dy) * (vx + fabs(vx)) * (uc - us) + (0.5 / dy) * (vx - fabs(vx)) * (un - uc);
+    double aY = (0.5 / dx) * (uy + fabs(uy)) * (vc - vw) + (0.5 / dx) * (uy - fabs(uy)) * (ve - vc) +
+                (0.5 / dy) * (vc + fabs(vc)) * (vc - vs) + (0.5 / dy) * (vc - fabs(vc)) * (vn - vc);
+    if (uc == 0) aX = 0;
+    if (vc == 0) aY = 0;
+
+    const double lo = p.CFLl, hi = p.CFLh, den = hi - lo;
+    const double cxc = a.CFLx[c], cyc = a.CFLy[c];
+    const double fX = hi - cxc, fY = hi - cyc;
+    if (j > 0) {
+        const double kx = a.CFLx[c - g.nx], ky = a.CFLy[c - g.nx];
+        limit_x(aX, kx, fX, lo, hi, den);
+        limit_y(aY, ky, kx, fY, lo, hi, den);
+    }
+    if (i > 0) {
+        const double kx = a.CFLx[c - 1], ky = a.CFLy[c - 1];
+        limit_x(aX, kx, fX, lo, hi, den);
+        limit_y(aY, ky, kx, fY, lo, hi, den);
+    }
+    limit_x(aX, cxc, fX, lo, hi, den);
+    limit_y(aY, cyc, cxc, fY, lo, hi, den);
+
+    double EnX = uc - dt * aX;
+    double EnY = vc - dt * aY;
+
+    const double dfx = (p.NUx / (dx * dx)) * (ue - 2 * uc + uw) + (p.NUy / (dy * dy)) * (us - 2 * uc + un);
+    const double dfy = (p.NUx / (dx * dx)) * (ve - 2 * vc + vw) + (p.NUy / (dy * dy)) * (vs - 2 * vc + vn);
+    EnX = EnX + dt * dfx;
+    EnY = EnY + dt * dfy;
+
+    const double velx = sqrt(uc * uc + vx * vx);
+    const double vely = sqrt(uy * uy + vc * vc);
+    const double dX = a.depthXP[c], dY = a.depthYP[c];
+    const double DragX = dX > 0 ? 0.5 * dt * a.CDXP[c] * velx / dX : 0;
+    const double DragY = dY > 0 ? 0.5 * dt * a.CDYP[c] * vely / dY : 0;
+    EnX = EnX / (1 + DragX);
+    EnY = EnY / (1 + DragY);
+
+    a.EnXP[c] = EnX;
+    a.EnYP[c] = EnY;
+    a.DragXP[c] = DragX;
+    a.DragYP[c] = DragY;
+    // ghost copies (dragTerm :679-686, dragInversion :739-745)
+    if (j == 0) { a.DragXP[gjM(g, i)] = DragX; a.EnXP[gjM(g, i)] = EnX; }
+    if (i == 0) { a.DragYP[giM(g, j)] = DragY; a.EnYP[giM(g, j)] = EnY; }
+}
+
+// ---------------------------------------------------------------------------
+// Assembly: matrixSourceTerm + matrixCoeff + adjustMatrixBoundary +
+// adjustTidalBoundary (nsfunctions.c:767-924) into the matrix-free operator,
+// plus the PCG start (solveMatrix :1004-1017: x = 0, r = b).
+//
+// Operator storage per cell: cX = G of the east face, cY = G of the north
+// face, D = GnCt, with D = -1 marking a Dirichlet (tidal) row, whose matrix
+// row is the identity.  Faces on the domain walls have zero depth, so the
+// wall adjustments of adjustMatrixBoundary subtract zeros and GnCt comes out
+// as (((1 + GnXP) + GnYP) + GnXM) + GnYM with GnXM = cX[west], GnYM = cY[south].
+__device__ inline double face_coef(double coef, double depthF, double dragF)
+{
+    return coef * depthF / (1 + dragF);
+}
+
+__global__ void __launch_bounds__(256) k_assemble(StepArgs a)
+{
+    __shared__ double lds[16 * 3];
+    __shared__ int amLast;
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const P &p = a.p;
+    double part[3] = {0.0, 0.0, 0.0};   // b.b, r.z, (unused)
+    if (i < g.nx && j < g.ny) {
+        const int c = j * g.nx + i;
+        const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
+        const int tt = cur_tt(a);
+        const double dt = p.dt;
+        const double coefx = p.g * (dt * dt / (p.dx * p.dx));
+        const double coefy = p.g * (dt * dt / (p.dy * p.dy));
+        const double GXP = face_coef(coefx, a.depthXP[c], a.DragXP[c]);
+        const double GXM = face_coef(coefx, a.depthXP[w], a.DragXP[w]);
+        const double GYP = face_coef(coefy, a.depthYP[c], a.DragYP[c]);
+        const double GYM = face_coef(coefy, a.depthYP[s], a.DragYP[s]);
+        double D = 1 + GXP + GYP + GXM + GYM;
+
+        double z = a.surf[c] - (dt / p.dx) * (a.EnXP[c] * a.depthXP[c] - a.EnXP[w] * a.depthXP[w]) -
+                   (dt / p.dy) * (a.EnYP[c] * a.depthYP[c] - a.EnYP[s] * a.depthYP[s]);
+        const uint8_t fl = a.flags[c];
+        const int ninflow = fl >> 2;
+        for (int q = 0; q < ninflow; q++)
+            z = z + (dt / (p.dx * p.dy)) * (a.inflow[tt] / p.inflowLocLength);
+        if ((fl & 1) && p.bcType != 2) { z = a.tideP[tt] + p.offset; D = -1.0; }
+        if ((fl & 2) && p.bcType != 1) { z = a.tideM[tt] + p.offset; D = -1.0; }
+        a.cX[c] = GXP;
+        a.cY[c] = GYP;
+        a.D[c] = D;
+        a.b[c] = z;
+        a.r[c] = z;
+        a.x[c] = 0.0;
+        a.pA[c] = 0.0;
+        const double invd = 1.0 / fabs(D);
+        part[0] = z * z;
+        part[1] = z * (z * invd);
+    }
+    block_sum<3>(part, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const int nb = gridDim.x * gridDim.y;
+        const int bid = blockIdx.x + gridDim.x * blockIdx.y;
+        double *pp = a.partials + 2 * bid;
+        __hip_atomic_store(pp + 0, part[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(pp + 1, part[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned prev = __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        amLast = (prev == (unsigned)(nb - 1));
+    }
+    __syncthreads();
+    // the last block to finish reduces every block's partials in block order
+    if (!amLast) return;
+    const int nb = gridDim.x * gridDim.y;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    const int nth = blockDim.x * blockDim.y;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int q = tid; q < nb; q += nth) {
+        acc[0] += __hip_atomic_load(a.partials + 2 * q + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        acc[1] += __hip_atomic_load(a.partials + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum<3>(acc, lds);
+    if (tid == 0) {
+        PcgScalars &sc = a.ctl->pcg;
+        sc.bb = acc[0];
+        sc.rr = acc[0];
+        sc.rz = acc[1];
+        sc.iter = 0;
+        const double bNorm = sqrt(sc.bb), rNorm = bNorm;
+        sc.bNorm = bNorm;
+        sc.rNorm = rNorm;
+        sc.done = (rNorm < p.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm))) ? 1 : 0;
+        if (p.maxIter <= 0) sc.done = 1;
+        a.ctl->counter = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PCG iteration, part 1: p = z + beta p (z = r / D, JacobiPrecond
+// precond.c:26-46) and q = A p (Mul_QV row-wise, operats.c:870-888),
+// plus the block partial of p.q.  p is double-buffered (pA -> pB) so the
+// neighbours' new p can be recomputed from their old p.
+__device__ inline double p_new_at(const StepArgs &a, int k, double beta, int first, int betaOne)
+{
+    const double rk = a.r[k];
+    const double zk = rk * (1.0 / fabs(a.D[k]));
+    if (first) return zk;
+    const double po = a.pOld[k];
+    return betaOne ? zk + po : zk + beta * po;
+}
+
+__global__ void __launch_bounds__(256) k_pcg_pq(StepArgs a)
+{
+    __shared__ double lds[16 * 3];
+    __shared__ int amLast;
+    const Ctl *ctl = a.ctl;
+    if (ctl->halted || ctl->pcg.done) return;
+    const Grid g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int first = ctl->pcg.iter == 0;
+    const double beta = ctl->pcg.beta;
+    const int betaOne = LAS_IS_ONE(beta);
+    double part[3] = {0.0, 0.0, 0.0};
+    if (i < g.nx && j < g.ny) {
+        const int c = j * g.nx + i;
+        const double Dc = a.D[c];
+        const double pc = p_new_at(a, c, beta, first, betaOne);
+        double sum = 0.0;
+        if (Dc < 0) {
+            // identity row: off-diagonal entries are zeros
+            sum = pc;
+        } else {
+            if (i > 0) sum += (-a.cX[c - 1]) * p_new_at(a, c - 1, beta, first, betaOne);
+            if (j > 0) sum += (-a.cY[c - g.nx]) * p_new_at(a, c - g.nx, beta, first, betaOne);
+            sum += Dc * pc;
+            if (j < g.ny - 1) sum += (-a.cY[c]) * p_new_at(a, c + g.nx, beta, first, betaOne);
+            if (i < g.nx - 1) sum += (-a.cX[c]) * p_new_at(a, c + 1, beta, first, betaOne);
+        }
+        a.pNew[c] = pc;
+        a.q[c] = sum;
+        part[0] = pc * sum;
+    }
+    block_sum<3>(part, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const int nb = gridDim.x * gridDim.y;
+        const int bid = blockIdx.x + gridDim.x * blockIdx.y;
+        __hip_atomic_store(a.partials + 2 * bid, part[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned prev = __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        amLast = (prev == (unsigned)(nb - 1));
+    }
+    __syncthreads();
+    if (!amLast) return;
+    const int nb = gridDim.x * gridDim.y;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    const int nth = blockDim.x * blockDim.y;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int q = tid; q < nb; q += nth)
+        acc[0] += __hip_atomic_load(a.partials + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    block_sum<3>(acc, lds);
+    if (tid == 0) {
+        PcgScalars &sc = a.ctl->pcg;
+        sc.pq = acc[0];
+        sc.alpha = sc.rz / sc.pq;
+        a.ctl->counter = 0;
+    }
+}
+
+// PCG iteration, part 2: x += alpha p, r -= alpha q (itersolv.c:420-423),
+// then r.r and r.z for the stopping test (rtc.c:48-72) and the next beta.
+__global__ void __launch_bounds__(256) k_pcg_update(StepArgs a)
+{
+    __shared__ double lds[16 * 3];
+    __shared__ int amLast;
+    const Ctl *ctl = a.ctl;
+    if (ctl->halted || ctl->pcg.done) return;
+    const Grid g = a.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const double alpha = ctl->pcg.alpha;
+    const int alphaOne = LAS_IS_ONE(alpha);
+    double part[3] = {0.0, 0.0, 0.0};
+    if (k < g.N) {
+        const double pk = a.pNew[k], qk = a.q[k];
+        double xk = a.x[k], rk = a.r[k];
+        if (alphaOne) { xk += pk; rk -= qk; }
+        else { xk += alpha * pk; rk -= alpha * qk; }
+        a.x[k] = xk;
+        a.r[k] = rk;
+        part[0] = rk * rk;
+        part[1] = rk * (rk * (1.0 / fabs(a.D[k])));
+    }
+    block_sum<3>(part, lds);
+    if (threadIdx.x == 0) {
+        const int nb = gridDim.x;
+        __hip_atomic_store(a.partials + 2 * blockIdx.x, part[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(a.partials + 2 * blockIdx.x + 1, part[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned prev = __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        amLast = (prev == (unsigned)(nb - 1));
+    }
+    __syncthreads();
+    if (!amLast) return;
+    const int nb = gridDim.x;
+    const int tid = threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int q = tid; q < nb; q += blockDim.x) {
+        acc[0] += __hip_atomic_load(a.partials + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        acc[1] += __hip_atomic_load(a.partials + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum<3>(acc, lds);
+    if (tid == 0) {
+        PcgScalars &sc = a.ctl->pcg;
+        const P &p = a.p;
+        sc.iter += 1;
+        sc.rr = acc[0];
+        const double rzNew = acc[1];
+        sc.beta = rzNew / sc.rz;
+        sc.rz = rzNew;
+        const double rNorm = sqrt(sc.rr);
+        sc.rNorm = rNorm;
+        const double bNorm = sc.bNorm;
+        if (rNorm < p.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm))) sc.done = 1;
+        else if (sc.iter >= p.maxIter) sc.done = 1;
+        a.ctl->counter = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Surface update: getFreeSurface + zeroNegSurf + enforceFreeSurfaceBC +
+// enforceTidalBC (nsfunctions.c:1020-1031, 436-449, 317-334, 412-433), over
+// all Ng indices.  A ghost takes the zero-clipped solution of the interior
+// cell it mirrors (the tidal override comes after the ghost fill).
+__global__ void __launch_bounds__(256) k_surface(StepArgs a)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.Ng) return;
+    const P &p = a.p;
+    const int tt = cur_tt(a);
+    const double old = a.surf[k];
+    a.surfOld[k] = old;
+    const int src = ghost_source(g, k, false);
+    const int cell = src < 0 ? k : src;
+    double h = a.x[cell];
+    const double zb = a.bottomZ[cell];
+    if (h < zb) {
+        const double diff = zb - h;
+        h = zb;
+        if (src < 0 && tt > 1) atomicAdd(&a.ctl->Vloss[2], -(diff * p.dx * p.dy));
+    }
+    if (src < 0) {
+        const uint8_t fl = a.flags[k];
+        if ((fl & 1) && p.bcType != 2 && a.tideP[tt] + p.offset > zb) h = a.tideP[tt] + p.offset;
+        if ((fl & 2) && p.bcType != 1 && a.tideM[tt] + p.offset > zb) h = a.tideM[tt] + p.offset;
+    }
+    a.surf[k] = h;
+}
+
+// ---------------------------------------------------------------------------
+// limiterCFL + computeVolume + updateDepth (nsfunctions.c:452-495, 361-370,
+// 239-247).  limiterCFL's first loop writes only surf[c] of the cell tested
+// and reads surfOld and depth (old) of its neighbours, so it is per-cell; the
+// new depth goes to a second buffer because neighbours read the old one.
+// depth at a ghost index is 0 (the reference reads past the N-array).
+__global__ void __launch_bounds__(256) k_limiter(StepArgs a)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny) return;
+    const P &p = a.p;
+    const int c = j * g.nx + i;
+    const double *dOld = a.depthOld, *ho = a.surfOld;
+    const double zb = a.bottomZ[c];
+    double h = a.surf[c];
+    double diff = h - zb;
+    uint8_t rmv = 0;
+    if (dOld[c] <= 0 && diff > 0) {
+        const double sref = ho[c] + p.minDepth;
+        const int e = nbE(g, c, i, j), w = nbW(g, c, i, j), n = nbN(g, c, i, j), s = nbS(g, c, i, j);
+        const bool dE = ho[e] <= sref || (e < g.N ? dOld[e] : 0.0) <= 0;
+        const bool dW = ho[w] <= sref || (w < g.N ? dOld[w] : 0.0) <= 0;
+        const bool dN = ho[n] <= sref || (n < g.N ? dOld[n] : 0.0) <= 0;
+        const bool dS = ho[s] <= sref || (s < g.N ? dOld[s] : 0.0) <= 0;
+        if (dE && dW && dN && dS) {
+            if ((double)c < (double)g.N - p.dx) atomicAdd(&a.ctl->Vloss[0], diff * p.dx * p.dy);
+            h = zb;
+            rmv = 1;
+        }
+    }
+    diff = h - zb;
+    if (diff <= p.minDepth && diff > 0) {
+        atomicAdd(&a.ctl->Vloss[1], diff * p.dx * p.dy);
+        h = zb;
+        rmv = 0;
+    }
+    a.surf[c] = h;
+    a.rmv[c] = rmv;
+    // computeVolume: depth from the previous step
+    const double cellV = p.dx * p.dy * dOld[c];
+    a.cellV[c] = cellV;
+    for (int q = 0; q < p.nScalar; q++)
+        a.Sm[(size_t)q * a.sStride + c] = a.S[(size_t)q * a.sStride + c] * cellV;
+    // updateDepth
+    const double d = h - zb;
+    a.depthNew[c] = d < 0 ? 0 : d;
+}
+
+// ---------------------------------------------------------------------------
+// updateAllFaceDepth + detectWaterfallLocation + updateVelocity +
+// waterfallCorrection (nsfunctions.c:143-236, 1080-1166, 250-286) for the
+// faces of every Ng index.  A face flag is +1 when the XM/YM test of the cell
+// beyond the face holds (that sweep runs last), else -1 when the XP/YP test of
+// the cell itself holds.  The rmvCFL zeroing of a cell's own and its west /
+// south faces becomes: a face is zeroed when either adjacent interior cell
+// has rmvCFL == 1.
+__device__ inline double face_depth_x(const StepArgs &a, int k)
+{
+    // depthXP per updateAllFaceDepth (irank = 0 = nrank - 1), before the
+    // final negative clip
+    const Grid &g = a.g;
+    const double *h = a.surf, *bX = a.bottomXP;
+    if (k < g.N) {
+        const int i = k % g.nx, j = k / g.nx;
+        if (i == g.nx - 1) return 0.0;               // iPbd faces zeroed
+        const int e = nbE(g, k, i, j);
+        const double hc = h[k], he = h[e];
+        return (hc > he ? hc : he) - bX[k];
+    }
+    int r = k - g.N;
+    if (r < 2 * g.nx) {                              // jP / jM ghost rows
+        const double hc = h[k], hn = h[k + 1];
+        return (hc > hn ? hc : hn) - bX[k];
+    }
+    return 0.0;                                      // iP / iM ghosts, corners
+}
+
+__device__ inline double face_depth_y(const StepArgs &a, int k)
+{
+    const Grid &g = a.g;
+    const double *h = a.surf, *bY = a.bottomYP;
+    if (k < g.N) {
+        const int i = k % g.nx, j = k / g.nx;
+        if (j == g.ny - 1) return 0.0;               // last strip: jPbd zeroed
+        const double hc = h[k], hn = h[k + g.nx];
+        (void)i;
+        return (hc > hn ? hc : hn) - bY[k];
+    }
+    int r = k - g.N;
+    if (r < g.nx) return 0.0;                        // jPgt zeroed (last strip)
+    r -= g.nx;
+    if (r < g.nx) return 0.0;                        // jMgt zeroed
+    r -= g.nx;
+    if (r < g.ny) {                                  // iP ghost <- iPbd value
+        const int src = r * g.nx + g.nx - 1;
+        if (r == g.ny - 1) {
+            // copied before the top row is zeroed (nsfunctions.c:214-231)
+            const double hc = h[src], hn = h[src + g.nx];
+            return (hc > hn ? hc : hn) - bY[src];
+        }
+        return face_depth_y(a, src);
+    }
+    r -= g.ny;
+    if (r < g.ny) {
+        const int src = r * g.nx;
+        if (r == g.ny - 1) {
+            const double hc = h[src], hn = h[src + g.nx];
+            return (hc > hn ? hc : hn) - bY[src];
+        }
+        return face_depth_y(a, src);
+    }
+    return 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_faces(StepArgs a)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.Ng) return;
+    const P &p = a.p;
+    double dX = face_depth_x(a, k);
+    double dY = face_depth_y(a, k);
+    // iP ghosts of depthYP copy the iPbd value before its clip; clip applies to all
+    if (dX < 0) dX = 0;
+    if (dY < 0) dY = 0;
+    a.depthXP[k] = dX;
+    a.depthYP[k] = dY;
+    if (k >= g.N) {
+        // sticky ghost waterfall flags from the XM / YM tests of the adjacent
+        // boundary cell (never reset by the reference)
+        int r = k - g.N;
+        if (r >= g.nx && r < 2 * g.nx) {             // jM ghost: YM face of cell (i, 0)
+            const int i = r - g.nx;
+            if (a.surf[i] < a.bottomYP[k] - 0.0 && dY >= p.wtfh) a.wtfYP[k] = 1;
+        } else if (r >= 2 * g.nx + g.ny && r < 2 * g.nx + 2 * g.ny) {   // iM ghost
+            const int jj = r - 2 * g.nx - g.ny;
+            if (a.surf[jj * g.nx] < a.bottomXP[k] - 0.0 && dX >= p.wtfh) a.wtfXP[k] = 1;
+        }
+        return;
+    }
+    const int c = k, i = c % g.nx, j = c / g.nx;
+    const int e = nbE(g, c, i, j), n = nbN(g, c, i, j);
+    const double hc = a.surf[c];
+    const double bXc = a.bottomXP[c], bYc = a.bottomYP[c];
+    // waterfall flags of this cell's east and north faces
+    double fX = 0, fY = 0;
+    if (hc < bXc - 0.0 && dX >= p.wtfh) fX = -1;
+    if (e < g.N && a.surf[e] < bXc - 0.0 && dX >= p.wtfh) fX = 1;
+    if (hc < bYc - 0.0 && dY >= p.wtfh) fY = -1;
+    if (n < g.N && a.surf[n] < bYc - 0.0 && dY >= p.wtfh) fY = 1;
+    // updateVelocity
+    double uc = a.EnXP[c] - (p.g * p.dt / p.dx) * (a.surf[e] - hc);
+    double vc = a.EnYP[c] - (p.g * p.dt / p.dy) * (a.surf[n] - hc);
+    if (dX == 0) uc = 0;
+    if (dY == 0) vc = 0;
+    const bool rmvC = a.rmv[c] == 1;
+    if (rmvC || (e < g.N && a.rmv[e] == 1)) uc = 0;
+    if (rmvC || (n < g.N && a.rmv[n] == 1)) vc = 0;
+    // waterfallCorrection (Cw = 0.7, CFLmax = 0.24)
+    if (fX != 0) {
+        const double vel0 = uc;
+        uc = fX * 0.7 * sqrt(2 * p.g * dX);
+        if (uc > 0.24 * p.dx / p.dt) uc = 0.24 * p.dx / p.dt;
+        atomicAdd(&a.ctl->Vloss[3], -((vel0 - uc) * p.dt * dX * p.dy));
+    }
+    if (fY != 0) {
+        const double vel0 = vc;
+        vc = fY * 0.7 * sqrt(2 * p.g * dY);
+        if (vc > 0.24 * p.dy / p.dt) vc = 0.24 * p.dy / p.dt;
+        atomicAdd(&a.ctl->Vloss[3], -((vel0 - vc) * p.dt * dY * p.dx));
+    }
+    a.uuXP[c] = uc;
+    a.vvYP[c] = vc;
+    // the rmvCFL pass also zeroes the iM-ghost face of a left-column cell and
+    // the jM-ghost face of a bottom-row cell; those ghosts are overwritten by
+    // enforceVelocityBC right after, so nothing is stored for them here
+}
+
+// ---------------------------------------------------------------------------
+// enforceVelocityBC + computeFaceFlowRate over Ng (nsfunctions.c:289-314,
+// 350-358).
+__global__ void __launch_bounds__(256) k_flux(StepArgs a)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.Ng) return;
+    const P &p = a.p;
+    double u, v;
+    if (k < g.N) {
+        u = a.uuXP[k];
+        v = a.vvYP[k];
+    } else {
+        u = a.uuXP[ghost_source(g, k, true)];
+        v = a.vvYP[ghost_source(g, k, false)];
+        a.uuXP[k] = u;
+        a.vvYP[k] = v;
+    }
+    a.Fuu[k] = u * p.dy * a.depthXP[k];
+    a.Fvv[k] = v * p.dx * a.depthYP[k];
+}
+
+// adjustTidalVelocity (nsfunctions.c:1034-1078): one thread per tidal cell.
+// Fvv is not recomputed afterwards (the reference keeps the old flux).
+__global__ void k_tidal_velocity(StepArgs a)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const P &p = a.p;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nP = p.bcType != 2 ? p.tideLocLengthP : 0;
+    const int nM = p.bcType != 1 ? p.tideLocLengthM : 0;
+    if (q >= nP + nM) return;
+    const bool isP = q < nP;
+    const int c = isP ? a.tideLocP[q] : a.tideLocM[q - nP];
+    const int i = c % g.nx, j = c / g.nx;
+    const double Fww = (a.surf[c] - a.surfOld[c]) * p.dx * p.dy / p.dt;
+    const double dc = a.depthNew[c];
+    if (isP) {
+        const double all = a.Fvv[nbS(g, c, i, j)] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww;
+        if (all != 0 && dc > 0) a.vvYP[c] = all / (dc * p.dx);
+        else a.vvYP[c] = 0;
+    } else {
+        const double all = -a.Fvv[c] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww;
+        if (all != 0 && dc > 0) a.vvYP[nbS(g, c, i, j)] = -all / (dc * p.dx);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// velocityInterp + volumeByFlux + updateCD + thinLayerDrag + monitorCFL
+// (nsfunctions.c:337-347, 373-409, 1170-1189, 691-727, 1192-1232).
+// velocityInterp uses flat-index neighbours (row wrap, ghost reads); indices
+// below 0 read 0.
+__global__ void __launch_bounds__(256) k_finish(StepArgs a)
+{
+    __shared__ double lds[16];
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const P &p = a.p;
+    double cflx = 0, cfly = 0;
+    if (c < g.N) {
+        const int tt = cur_tt(a);
+        const int nx = g.nx, i = c % nx, j = c / nx;
+        const double *u = a.uuXP, *v = a.vvYP;
+        const double um1 = c - 1 >= 0 ? u[c - 1] : 0.0;
+        const double um1n = c - 1 + nx >= 0 ? u[c - 1 + nx] : 0.0;
+        const double vmn = c - nx >= 0 ? v[c - nx] : 0.0;
+        const double vmn1 = c + 1 - nx >= 0 ? v[c + 1 - nx] : 0.0;
+        a.uuYP[c] = 0.25 * (u[c] + um1 + u[c + nx] + um1n);
+        a.vvXP[c] = 0.25 * (v[c] + v[c + 1] + vmn + vmn1);
+
+        double V = a.cellV[c] + p.dt * (a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] + a.Fvv[nbS(g, c, i, j)] - a.Fvv[c]);
+        const int ninflow = a.flags[c] >> 2;
+        for (int q = 0; q < ninflow; q++) V = V + a.inflow[tt] * p.dt;
+        if (V < 0) V = 0;
+        a.cellV[c] = V;
+
+        const double d = a.depthNew[c];
+        if (p.CDnotN == 0 && d > 0) {
+            const double n = a.manning ? a.manning[c] : p.manningN;
+            const double coeff = p.g * n * n;
+            const double cd = coeff / pow(d, 1.0 / 3.0);
+            a.CDXP[c] = cd;
+            a.CDYP[c] = cd;
+        }
+        if (p.useThinLayer == 1 && p.CDnotN == 0 && d > 0) {
+            const double n = a.manning ? a.manning[c] : p.manningN;
+            const double coeff = p.g * n * n;
+            if (a.depthXP[c] <= p.hD) a.CDXP[c] = coeff / pow(d, 2.0 / 3.0);
+            if (a.depthYP[c] <= p.hD) a.CDYP[c] = coeff / pow(d, 2.0 / 3.0);
+        }
+        cflx = fabs(u[c]) * p.dt / p.dx;
+        cfly = fabs(v[c]) * p.dt / p.dy;
+        a.CFLx[c] = cflx;
+        a.CFLy[c] = cfly;
+    }
+    // thinLayerDrag with constant CD covers every Ng face
+    if (p.useThinLayer == 1 && p.CDnotN == 1 && c < g.Ng) {
+        if (a.depthXP[c] <= p.hD) a.CDXP[c] = p.CDx + (p.CDmax - p.CDx) * (p.hD - a.depthXP[c]) / p.hD;
+        if (a.depthYP[c] <= p.hD) a.CDYP[c] = p.CDy + (p.CDmax - p.CDy) * (p.hD - a.depthYP[c]) / p.hD;
+    }
+    const double mx = block_max(cflx, lds);
+    const double my = block_max(cfly, lds);
+    if (threadIdx.x == 0) {
+        atomicMax(&a.ctl->maxCFLxBits, (unsigned long long)__double_as_longlong(mx));
+        atomicMax(&a.ctl->maxCFLyBits, (unsigned long long)__double_as_longlong(my));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Scalar transport (scalar.c:30-56, upwind): scalarAdvection +
+// scalarDiffusion + updateScalar's connectivity limiter, inflow mass and
+// concentration, and the out-of-range reset (scalar.c:68-97, 259-416).
+// The new concentration goes to Snew (neighbours read the old S).
+__global__ void __launch_bounds__(256) k_scalar(StepArgs a, int q)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny) return;
+    const P &p = a.p;
+    const int tt = cur_tt(a);
+    const int c = j * g.nx + i;
+    const int e = nbE(g, c, i, j), w = nbW(g, c, i, j), n = nbN(g, c, i, j), s = nbS(g, c, i, j);
+    const double *S = a.S + (size_t)q * a.sStride;
+    double *Sm = a.Sm + (size_t)q * a.sStride;
+    const double *Fu = a.Fuu, *Fv = a.Fvv;
+    const double dt = p.dt, dx = p.dx, dy = p.dy;
+    const double Sc = S[c], Se = S[e], Sw = S[w], Sn = S[n], Ss = S[s];
+    double sm = Sm[c];
+    {
+        const double Sd = Fu[c] > 0 ? Sc : Se;
+        const double Su = Fu[w] > 0 ? Sw : Sc;
+        const double Sr = Fv[c] > 0 ? Sc : Sn;
+        const double Sl = Fv[s] > 0 ? Ss : Sc;
+        sm = sm + dt * (-Fu[c] * Sd + Fu[w] * Su - Fv[c] * Sr + Fv[s] * Sl);
+    }
+    sm = sm + dt * ((p.Kx * dy * a.depthXP[c] / dx) * (Se - Sc) - (p.Kx * dy * a.depthXP[w] / dx) * (Sc - Sw) +
+                    (p.Ky * dx * a.depthYP[c] / dy) * (Sn - Sc) - (p.Ky * dx * a.depthYP[s] / dy) * (Sc - Ss));
+    // connectivity with the previous-step surface (scalar.c:292-341)
+    const double *ho = a.surfOld, *zb = a.bottomZ;
+    double vals[4];
+    int m = 0;
+    if (a.depthXP[c] > 0 && ho[e] - zb[e] > 0) vals[m++] = Se;
+    if (a.depthXP[w] > 0 && ho[w] - zb[w] > 0) vals[m++] = Sw;
+    if (n <= g.N && a.depthYP[c] > 0 && ho[n] - zb[n] > 0) vals[m++] = Sn;
+    if (s <= g.N && a.depthYP[s] > 0 && ho[s] - zb[s] > 0) vals[m++] = Ss;
+    double smax = 0, smin = 0;
+    if (m) {
+        smax = vals[0];
+        smin = vals[0];
+        for (int t = 1; t < m; t++) {
+            if (vals[t] > smax) smax = vals[t];
+            if (vals[t] < smin) smin = vals[t];
+        }
+    }
+    const int ninflow = a.flags[c] >> 2;
+    for (int t = 0; t < ninflow; t++) sm = sm + p.inflowS * a.inflow[tt] * dt / p.inflowLocLength;
+    Sm[c] = sm;
+    const double V = a.cellV[c];
+    double Sv = V > 0 ? sm / V : 0;
+    if (m != 0 && Sv > smax && ninflow == 0) Sv = smax;
+    else if (m != 0 && Sv < smin && ninflow == 0) Sv = smin;
+    if (Sv > 100 || Sv < 0) {
+        const bool bottomRow = (j == 0);
+        const bool topRowNotFirst = (j == g.ny - 1 && i > 0);
+        if (!bottomRow && !topRowNotFirst) {
+            Sv = 0;
+            a.surf[c] = a.surf[c] - a.depthNew[c];
+            a.depthNew[c] = 0;
+        }
+    }
+    a.Snew[c] = Sv;
+}
+
+// enforceScalarBC (scalar.c:419-471) over the interior and the four ghost
+// blocks (corners untouched).
+__global__ void __launch_bounds__(256) k_scalar_bc(StepArgs a, int q)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.Ng - 4) return;
+    const P &p = a.p;
+    const int tt = cur_tt(a);
+    double *S = a.S + (size_t)q * a.sStride;
+    if (k >= g.N) {
+        S[k] = a.Snew[ghost_source(g, k, false)];
+        return;
+    }
+    double v = a.Snew[k];
+    const uint8_t fl = a.flags[k];
+    if ((fl & 1) && p.bcType != 2) v = p.useConstTidePS == 1 ? p.tidePS : a.tidalPS[tt];
+    if ((fl & 2) && p.bcType != 1) v = p.useConstTideMS == 1 ? p.tideMS : a.tidalMS[tt];
+    if (a.depthNew[k] <= 0) v = 0;
+    S[k] = v;
+}
+
+// Step end: the max CFL test of monitorCFL (nsfunctions.c:1226-1231) halts
+// the handle instead of exiting the process.
+__global__ void k_step_end(StepArgs a)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Ctl *c = a.ctl;
+    if (c->halted) return;
+    const double mx = __longlong_as_double((long long)c->maxCFLxBits);
+    const double my = __longlong_as_double((long long)c->maxCFLyBits);
+    c->lastMaxCFLx = mx;
+    c->lastMaxCFLy = my;
+    c->maxCFLxBits = 0;
+    c->maxCFLyBits = 0;
+    c->pcgIterLast = c->pcg.iter;
+    c->pcgIterTotal += c->pcg.iter;
+    c->relResLast = c->pcg.bNorm > 0 ? c->pcg.rNorm / c->pcg.bNorm : 1.0;
+    c->steps += 1;
+    if (fabs(mx) > 20 || fabs(my) > 20) {
+        c->halted = 1;
+        c->haltStep = c->tt;
+    }
+}
+
+}  // namespace frehd

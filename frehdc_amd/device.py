@@ -1,0 +1,124 @@
+"""Python handle over the device library (libfrehd_gpu.so).
+
+Mirrors the reference's time loop (FrehdC_v42/nssolve.c:48-106): create a
+handle from a scenario (Init), advance steps (oneCompleteStep + monitorCFL),
+download the state (DataOutput's arrays at full precision).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+class DeviceScenario:
+    """One scenario resident on the GPU."""
+
+    def __init__(self, scen, from_state: dict | None = None, **override):
+        self.lib = abi.load_library()
+        self.scen = scen
+        self._cfg = scen.c_config(**override)
+        self._forc = scen.c_forcing()
+        self.nScalar = max(1, self._cfg.nScalar)
+        h = C.c_void_p()
+        if from_state is None:
+            self._ic = scen.c_ic()
+            rc = self.lib.frehd_dev_create(C.byref(self._cfg), C.byref(self._forc),
+                                           C.byref(self._ic), C.byref(h))
+        else:
+            keep = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in from_state.items()}
+            self._keep = keep
+            flds = abi.FrehdFields()
+            for name in abi.FIELD_NAMES:
+                if name in keep:
+                    setattr(flds, name, abi.dptr(keep[name]))
+            rc = self.lib.frehd_dev_create_from_state(C.byref(self._cfg), C.byref(self._forc),
+                                                      C.byref(flds), C.byref(h))
+        abi.check(rc)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.frehd_dev_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1) -> int:
+        """Advance n steps; returns 0 or raises FrehdError (FREHD_ECFL when
+        the CFL stop test fires)."""
+        return abi.check(self.lib.frehd_dev_step(self.h, int(n)))
+
+    def try_step(self, n: int = 1) -> int:
+        return self.lib.frehd_dev_step(self.h, int(n))
+
+    def phase(self, p: int) -> int:
+        return abi.check(self.lib.frehd_dev_phase(self.h, int(p)))
+
+    def download(self, names=None) -> dict:
+        s = self.scen
+        names = list(abi.FIELD_NAMES) if names is None else list(names)
+        out = {}
+        flds = abi.FrehdFields()
+        for n in names:
+            if n == "S":
+                a = np.zeros(self.nScalar * s.Ng)
+            elif n == "Sm":
+                a = np.zeros(self.nScalar * s.N)
+            elif n in abi.INTERIOR_FIELDS:
+                a = np.zeros(s.N)
+            else:
+                a = np.zeros(s.Ng)
+            out[n] = a
+            setattr(flds, n, abi.dptr(a))
+        abi.check(self.lib.frehd_dev_download(self.h, C.byref(flds)))
+        if "S" in out:
+            out["S"] = out["S"].reshape(self.nScalar, s.Ng)
+        if "Sm" in out:
+            out["Sm"] = out["Sm"].reshape(self.nScalar, s.N)
+        return out
+
+    def field(self, name):
+        return self.download([name])[name]
+
+    def stats(self) -> dict:
+        st = abi.FrehdStats()
+        abi.check(self.lib.frehd_dev_stats(self.h, C.byref(st)))
+        return dict(steps=st.steps, pcgIterTotal=st.pcgIterTotal, pcgIterLast=st.pcgIterLast,
+                    refineLast=st.refineLast, relResLast=st.relResLast, maxCFLx=st.maxCFLx,
+                    maxCFLy=st.maxCFLy, Vloss=list(st.Vloss))
+
+    @property
+    def tt(self):
+        v = C.c_int32()
+        abi.check(self.lib.frehd_dev_step_index(self.h, C.byref(v)))
+        return v.value
+
+    @tt.setter
+    def tt(self, value):
+        abi.check(self.lib.frehd_dev_set_step_index(self.h, int(value)))
+
+    def enable_timing(self, on=True):
+        abi.check(self.lib.frehd_dev_enable_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self) -> dict:
+        cap = 64
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        launches = (C.c_int64 * cap)()
+        n = C.c_int32()
+        abi.check(self.lib.frehd_dev_kernel_times(self.h, cap, names, ms, launches, C.byref(n)))
+        return {names[q].decode(): (ms[q], launches[q]) for q in range(min(n.value, cap))}
+
+
+def device_count() -> int:
+    lib = abi.load_library()
+    n = C.c_int32()
+    lib.frehd_device_count(C.byref(n))
+    return n.value

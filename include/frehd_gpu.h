@@ -184,6 +184,13 @@ int frehd_dev_create_from_state(const frehd_config *cfg, const frehd_forcing *fo
  * the state after the offending step, where the reference would exit. */
 int frehd_dev_step(frehd_dev_t *dev, int32_t nsteps);
 
+/* Run one phase of the next step and wait for it (diagnostics and parity
+ * tests): 0 = solveSourceTerm (nssolve.c:121; tt advances here), 1 =
+ * solveFreeSurface (:158), 2 = updateData (:181; also fills CFLx/CFLy),
+ * 3 = scalarTransport (scalar.c:30), 4 = monitorCFL's stop test.  Running
+ * 0..4 in order is one frehd_dev_step. */
+int frehd_dev_phase(frehd_dev_t *dev, int32_t phase);
+
 /* Copy the device state into the non-NULL arrays of `out` (DataOutput's
  * view, fileio.c:62-180, at full precision). */
 int frehd_dev_download(frehd_dev_t *dev, frehd_fields *out);
@@ -193,6 +200,10 @@ int frehd_dev_stats(frehd_dev_t *dev, frehd_stats *out);
 
 /* Current step counter (number of steps taken). */
 int frehd_dev_step_index(frehd_dev_t *dev, int32_t *tt);
+
+/* Set the step counter, e.g. to resume from a state saved at step tt; the
+ * next step taken is tt + 1. */
+int frehd_dev_set_step_index(frehd_dev_t *dev, int32_t tt);
 
 /* Release the handle and its device memory. */
 int frehd_dev_destroy(frehd_dev_t *dev);
