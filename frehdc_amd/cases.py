@@ -118,9 +118,10 @@ def tide_series(Nt, dt, amp, period=TIDE_PERIOD, phase=0.0):
     return amp * np.sin(2.0 * math.pi * t / period + phase)
 
 
-def fbm_surface(nx, ny, hurst, rng, lo, hi):
+def fbm_surface(nx, ny, hurst, rng, lo, hi, uniform=False):
     """Fractional-Brownian surface by spectral synthesis (power ~ k^-(2H+2)),
-    rescaled to [lo, hi]."""
+    rescaled to [lo, hi].  uniform=True maps it monotonically (by rank) so the
+    elevations spread uniformly over [lo, hi], keeping the spatial pattern."""
     kx = np.fft.fftfreq(nx)[None, :]
     ky = np.fft.fftfreq(ny)[:, None]
     k = np.sqrt(kx * kx + ky * ky)
@@ -131,7 +132,12 @@ def fbm_surface(nx, ny, hurst, rng, lo, hi):
     noise = rng.normal(size=(ny, nx))
     spec = amp * noise * np.exp(1j * phase)
     f = np.real(np.fft.ifft2(spec))
-    f = (f - f.min()) / max(f.max() - f.min(), 1e-300)
+    if uniform:
+        r = np.empty(f.size)
+        r[np.argsort(f, axis=None, kind="stable")] = (np.arange(f.size) + 0.5) / f.size
+        f = r.reshape(f.shape)
+    else:
+        f = (f - f.min()) / max(f.max() - f.min(), 1e-300)
     return lo + (hi - lo) * f
 
 
@@ -178,11 +184,11 @@ def basin(nx=128, ny=128, dx=10.0, dt=1.0, Nt=3600, seed=1, amp=0.2, bumps=20):
 
 
 def marsh(n=1024, dx=2.0, dt=0.5, Nt=2000, seed=2, hurst=0.7, amp=0.6):
-    """Marsh with wetting and drying: fBm bed (Hurst 0.7) in [-1.0, +1.5] m,
-    tide 0.6 m, per-cell Manning n ~ U(0.02, 0.04), scalar 0 inside with 1 at
-    the tidal boundary."""
+    """Marsh with wetting and drying: fBm bed (Hurst 0.7) spread uniformly over
+    [-1.0, +1.5] m, tide 0.6 m, per-cell Manning n ~ U(0.02, 0.04), scalar 0
+    inside with 1 at the tidal boundary."""
     rng = np.random.default_rng(seed)
-    bed = fbm_surface(n, n, hurst, rng, -1.0, 1.5)
+    bed = fbm_surface(n, n, hurst, rng, -1.0, 1.5, uniform=True)
     manning = np.random.default_rng(seed + 1).uniform(0.02, 0.04, size=n * n)
     cfg = default_config(n, n, dx, dt, Nt, useConstTidePS=1, tidePS=1.0, eps=1e-10)
     return _finish(cfg, bed, tide_series(Nt, dt, amp), scalar0=np.zeros(n * n),

@@ -170,5 +170,23 @@ class Reference(_CpuModel):
         return out
 
 
+def solve(GnCt, GnXP, GnXM, GnYP, GnYM, z, eps=1e-10, max_iter=10000, precond=0):
+    """The oracle's CG on the 5-point system given by row coefficients in cell
+    order (nx = shape[1], ny = shape[0]); returns (x, iterations, relres)."""
+    lib = _lib(ORACLE_LIB, "fo_")
+    ny, nx = GnCt.shape
+    P = C.POINTER(C.c_double)
+    lib.fo_solve.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P, P, C.c_double, C.c_int, C.c_int,
+                             P]
+    lib.fo_solve.restype = C.c_int
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            for a in (GnCt, GnXP, GnXM, GnYP, GnYM, z)]
+    x = np.zeros(nx * ny)
+    rel = C.c_double()
+    it = lib.fo_solve(nx, ny, *[a.ctypes.data_as(P) for a in arrs], x.ctypes.data_as(P), eps,
+                      max_iter, precond, C.byref(rel))
+    return x.reshape(ny, nx), it, rel.value
+
+
 def have_reference():
     return os.path.exists(REF_LIB)

@@ -1,0 +1,70 @@
+"""The device library builds for the CPU-only container and exports exactly
+the C ABI declared in include/frehd_gpu.h; the ctypes mirror matches the
+compiled struct sizes.  No device call is made here."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from frehdc_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "frehd_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(frehd_[a-z_0-9]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(abi.LIB_PATH):
+        from frehdc_amd import build
+        build.build()
+    return abi.load_library()
+
+
+def test_exports_every_declared_function(lib):
+    names = declared_functions()
+    assert len(names) >= 15
+    out = subprocess.run(["nm", "-D", "--defined-only", abi.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(frehd_\w+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        getattr(lib, n)
+
+
+def test_struct_sizes_match(lib):
+    sizes = (C.c_int64 * 5)()
+    assert lib.frehd_abi_sizes(sizes) == 0
+    ours = [C.sizeof(abi.FrehdConfig), C.sizeof(abi.FrehdForcing), C.sizeof(abi.FrehdIC),
+            C.sizeof(abi.FrehdFields), C.sizeof(abi.FrehdStats)]
+    assert list(sizes) == ours
+
+
+def test_error_paths_need_no_device(lib):
+    # argument checks run before any device call
+    assert lib.frehd_dev_step(None, 1) == abi.FREHD_EINVAL
+    assert b"NULL" in lib.frehd_last_error()
+    cfg = abi.FrehdConfig(nx=2, ny=2, Nt=1, dx=1.0, dy=1.0, dt=1.0)
+    b = C.c_int64()
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == abi.FREHD_EINVAL
+    cfg = abi.FrehdConfig(nx=64, ny=64, Nt=10, dx=1.0, dy=1.0, dt=1.0, bcType=1, precond=1)
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == abi.FREHD_EINVAL
+    assert b"Jacobi" in lib.frehd_last_error()
+
+
+def test_footprint_scales(lib):
+    from frehdc_amd import cases
+    sc = cases.basin(nx=256, ny=256, Nt=10)
+    cfg = sc.c_config()
+    b = C.c_int64()
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
+    per_cell = b.value / sc.N
+    assert 200 < per_cell < 600, per_cell

@@ -1,0 +1,51 @@
+"""The CPU oracle against the reference's own functions, live: every phase of
+every step bit for bit (runs where oracle/_ref/libfrehd_ref.so is built, i.e.
+next to the reference checkout)."""
+import numpy as np
+import pytest
+
+from frehdc_amd import cases
+
+FIELDS = ["surf", "surfOld", "uuXP", "vvYP", "uuYP", "vvXP", "depth", "depthXP", "depthYP",
+          "EnXP", "EnYP", "DragXP", "DragYP", "CDXP", "CDYP", "Fuu", "Fvv", "cellV", "S",
+          "CFLx", "CFLy", "wtfXP", "wtfYP", "bottomZ", "bottomXP", "bottomYP", "GnCt", "GnXP",
+          "GnXM", "GnYP", "GnYM"]
+INTERIOR = {"uuYP", "vvXP", "depth", "cellV", "GnCt", "GnXP", "GnXM", "GnYP", "GnYM"}
+
+
+def marsh_fast():
+    sc = cases.marsh(n=40, dx=2.0, dt=0.5, Nt=200, amp=0.6)
+    sc.manningCell = None
+    sc.tideP[:] = cases.tide_series(200, 0.5, 0.6, period=1200.0)
+    return sc
+
+
+CASES = [
+    ("basin_jacobi", lambda: cases.basin(nx=30, ny=26, Nt=40), {}),
+    ("basin_ssor", lambda: cases.basin(nx=30, ny=26, Nt=40), {"precond": 1}),
+    ("marsh_jacobi", marsh_fast, {}),
+    ("basin_constCD_thin", lambda: cases.basin(nx=28, ny=24, Nt=40), {"CDnotN": 1, "hD": 2.6}),
+]
+
+
+@pytest.mark.parametrize("name,builder,override", CASES, ids=[c[0] for c in CASES])
+def test_every_phase_bit_exact(oracle_built, name, builder, override):
+    import pyoracle
+    if not pyoracle.have_reference():
+        pytest.skip("reference build absent")
+    sc = builder()
+    o = pyoracle.Oracle(sc, **override)
+    r = pyoracle.Reference(sc, **override)
+    steps = 40 if name.startswith("marsh") else 12
+    for t in range(steps):
+        for p in range(5):
+            assert o.phase(p) == r.phase(p)
+            for n in FIELDS:
+                a = o.raw(n)
+                b = r.raw(n)
+                m = sc.N if n in INTERIOR else sc.Ng
+                assert np.array_equal(a[:m].view(np.int64), b[:m].view(np.int64)), (name, t, p, n)
+            zo, zr = o.z(), r.z()
+            assert np.array_equal(zo.view(np.int64), zr.view(np.int64)), (name, t, p, "z")
+        assert o.pcg_iters == r.pcg_iters
+    assert np.array_equal(o.vloss(), r.vloss())
