@@ -32,7 +32,9 @@ class FrehdConfig(C.Structure):
         ("useConstTidePS", C.c_int32), ("useConstTideMS", C.c_int32), ("nScalar", C.c_int32),
         ("tidePS", C.c_double), ("tideMS", C.c_double), ("inflowS", C.c_double),
         ("Kx", C.c_double), ("Ky", C.c_double), ("offset", C.c_double),
-        ("epsInner", C.c_double), ("maxRefine", C.c_int32), ("reserved0", C.c_int32),
+        ("epsInner", C.c_double), ("maxRefine", C.c_int32),
+        ("useSubgrid", C.c_int32), ("Ne", C.c_int32), ("pad1", C.c_int32),
+        ("surfmin", C.c_double), ("dsurf", C.c_double), ("subR", C.c_double),
     ]
 
 
@@ -44,12 +46,20 @@ class FrehdForcing(C.Structure):
     ]
 
 
+SUB_TABLES = ["V", "Z", "Vxp", "Vxm", "Vyp", "Vym", "Np", "Nm", "Op", "Om",
+              "minVxp", "minVxm", "minVyp", "minVym"]
+
+
+class FrehdSubgrid(C.Structure):
+    _fields_ = [("edgeX", c_double_p), ("edgeY", c_double_p)] + [(n, c_double_p) for n in SUB_TABLES]
+
+
 class FrehdIC(C.Structure):
     _fields_ = [
         ("bottomZ", c_double_p),
         ("surf0", C.c_double), ("U0", C.c_double), ("V0", C.c_double), ("S0", C.c_double),
         ("surf0cell", c_double_p), ("U0cell", c_double_p), ("V0cell", c_double_p),
-        ("S0cell", c_double_p),
+        ("S0cell", c_double_p), ("sub", C.POINTER(FrehdSubgrid)),
     ]
 
 
@@ -62,8 +72,11 @@ FIELD_NAMES = [
 INTERIOR_FIELDS = {"uuYP", "vvXP", "depth", "cellV", "Sm"}
 
 
+SUB_FIELD_NAMES = ["V", "Z", "Vx", "Vy", "Zx", "Zy", "Nx", "Oy"]
+
+
 class FrehdFields(C.Structure):
-    _fields_ = [(n, c_double_p) for n in FIELD_NAMES]
+    _fields_ = [(n, c_double_p) for n in FIELD_NAMES + SUB_FIELD_NAMES] + [("ind", c_int32_p)]
 
 
 class FrehdStats(C.Structure):

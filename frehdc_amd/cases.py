@@ -17,6 +17,7 @@ workload description is mapped to the YP row, with the bed sloping along y.
 """
 from __future__ import annotations
 
+import ctypes as C
 import dataclasses
 import math
 
@@ -25,6 +26,12 @@ import numpy as np
 from . import abi
 
 TIDE_PERIOD = 44712.0
+# the tidal (YP) boundary of the marsh workloads is an open-water inlet: the
+# last rows are at least this deep, so the boundary cells never dry (a dry
+# tidal cell with a near-zero face area makes the reference's boundary
+# velocity flux/area blow up)
+INLET_ROWS = 2
+INLET_BED = -0.9
 
 
 @dataclasses.dataclass
@@ -46,6 +53,7 @@ class Scenario:
     S0: float = 0.0
     S0cell: np.ndarray | None = None   # (nScalar*N,)
     name: str = ""
+    subgrid: dict | None = None        # edgeX, edgeY and level tables (subgrid.py)
 
     @property
     def nx(self):
@@ -90,6 +98,13 @@ class Scenario:
         ic.bottomZ = abi.dptr(self.bottomZ)
         ic.surf0, ic.U0, ic.V0, ic.S0 = self.surf0, self.U0, self.V0, self.S0
         ic.S0cell = abi.dptr(self.S0cell)
+        if self.subgrid is not None:
+            sg = abi.FrehdSubgrid()
+            for k in ("edgeX", "edgeY", *abi.SUB_TABLES):
+                a = self.subgrid.get(k)
+                setattr(sg, k, abi.dptr(a) if a is not None else None)
+            self._sg = sg
+            ic.sub = C.pointer(sg)
         return ic
 
 
@@ -101,7 +116,8 @@ def default_config(nx, ny, dx, dt, Nt, **kw) -> dict:
         precond=abi.PRECOND_JACOBI, precision=abi.PREC_FP64, bcType=1, tideLocLengthP=nx,
         tideLocLengthM=0, inflowLocLength=0, useScalar=1, scalarAdv=1, useConstTidePS=1,
         useConstTideMS=1, nScalar=1, tidePS=0.0, tideMS=0.0, inflowS=0.0, Kx=1e-4, Ky=1e-4,
-        offset=0.0, epsInner=1e-6, maxRefine=8, reserved0=0,
+        offset=0.0, epsInner=1e-6, maxRefine=8, useSubgrid=0, Ne=0, surfmin=0.0, dsurf=0.01,
+        subR=1.0,
     )
     cfg.update(kw)
     return cfg
@@ -189,6 +205,7 @@ def marsh(n=1024, dx=2.0, dt=0.5, Nt=2000, seed=2, hurst=0.7, amp=0.6):
     inside with 1 at the tidal boundary."""
     rng = np.random.default_rng(seed)
     bed = fbm_surface(n, n, hurst, rng, -1.0, 1.5, uniform=True)
+    bed[-INLET_ROWS:, :] = np.minimum(bed[-INLET_ROWS:, :], INLET_BED)
     manning = np.random.default_rng(seed + 1).uniform(0.02, 0.04, size=n * n)
     cfg = default_config(n, n, dx, dt, Nt, useConstTidePS=1, tidePS=1.0, eps=1e-10)
     return _finish(cfg, bed, tide_series(Nt, dt, amp), scalar0=np.zeros(n * n),
@@ -256,3 +273,65 @@ def serving_requests(count=512, rate=20.0, seed=6, lo=64, hi=1024, dx=2.0, dt=0.
                            scalar0=np.zeros(nsc * side * side), name=f"req{side}")
         out.append((float(t[q]), steps, side, build))
     return out
+
+
+def with_subgrid(scen: Scenario, fine: np.ndarray, r: int, surfmin: float, surfmax: float,
+                 dsurf: float = 0.01) -> Scenario:
+    """Turn a scenario into its subgrid form: `fine` holds the pixel bed
+    elevations [ny*r, nx*r]; the coarse bed is each block's minimum
+    (bottom.dat), face bottoms and level tables come from subgrid.tables()."""
+    from . import subgrid as sgm
+    nx, ny = scen.nx, scen.ny
+    assert fine.shape == (ny * r, nx * r)
+    bottom, edgeX, edgeY = sgm.coarse_geometry(fine, r)
+    Ne = int(round((surfmax - surfmin) / dsurf))
+    dxf = scen.cfg["dx"] / r
+    tabs = sgm.tables(fine, r, dxf, surfmin, dsurf, Ne)
+    scen.bottomZ = np.ascontiguousarray(bottom.reshape(-1))
+    scen.cfg["offset"] = offset_for(scen.bottomZ)
+    scen.cfg.update(useSubgrid=1, Ne=Ne, surfmin=float(surfmin), dsurf=float(dsurf), subR=float(r))
+    tabs["edgeX"] = np.ascontiguousarray(edgeX.reshape(-1))
+    tabs["edgeY"] = np.ascontiguousarray(edgeY.reshape(-1))
+    scen.subgrid = tabs
+    scen.name += f"_sub{r}"
+    return scen
+
+
+def fine_basin(nx, ny, r, dx=10.0, seed=1, bumps=20):
+    """Pixel bed of the basin workload: slope from -3.0 m at the tidal row to
+    -0.5 m, plus Gaussian bumps (0.3 m, sigma 30 m) and pixel-scale seeded
+    roughness (0.05 m) so the subgrid areas differ from the plain grid."""
+    rng = np.random.default_rng(seed)
+    dxf = dx / r
+    fy, fx = ny * r, nx * r
+    y = (np.arange(fy)[:, None] + 0.5) * dxf
+    x = (np.arange(fx)[None, :] + 0.5) * dxf
+    s = (ny * dx - y) / (ny * dx)
+    bed = (-3.0 + 2.5 * s) * np.ones((1, fx))
+    cx = rng.uniform(0, nx * dx, size=bumps)
+    cy = rng.uniform(0, ny * dx, size=bumps)
+    for a, b in zip(cx, cy):
+        bed = bed + 0.3 * np.exp(-((x - a) ** 2 + (y - b) ** 2) / (2 * 30.0 ** 2))
+    return bed + rng.normal(0.0, 0.05, size=bed.shape)
+
+
+def fine_marsh(n, r, seed=2, hurst=0.7, lo=-1.0, hi=1.5):
+    """Pixel bed of the marsh: fBm (Hurst 0.7) at pixel resolution spread
+    uniformly over [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    f = fbm_surface(n * r, n * r, hurst, rng, lo, hi, uniform=True)
+    f[-INLET_ROWS * r:, :] = np.minimum(f[-INLET_ROWS * r:, :], INLET_BED)
+    return f
+
+
+def marsh_subgrid(n=48, r=8, dx=2.0, dt=0.5, Nt=400, amp=0.6, period=TIDE_PERIOD, seed=2,
+                  dsurf=0.01, manning_cell=True):
+    """The marsh workload on the subgrid model: r x r pixels per cell, tables
+    from surfmin = -1.0 m to surfmax = +1.5 m every dsurf."""
+    sc = marsh(n=n, dx=dx, dt=dt, Nt=Nt, seed=seed, amp=amp)
+    if period != TIDE_PERIOD:
+        sc.tideP[:] = tide_series(Nt, dt, amp, period=period)
+    if not manning_cell:
+        sc.manningCell = None
+    fine = fine_marsh(n, r, seed=seed)
+    return with_subgrid(sc, fine, r, -1.0, 1.5, dsurf)

@@ -14,7 +14,10 @@ struct P {
     double dx, dy, dt, g, NUx, NUy, CDx, CDy, manningN, hD, CDmax, minDepth, wtfh, CFLl, CFLh;
     double eps, tidePS, tideMS, inflowS, Kx, Ky, offset;
     int CDnotN, useThinLayer, bcType, tideLocLengthP, tideLocLengthM, inflowLocLength;
-    int useScalar, nScalar, useConstTidePS, useConstTideMS, maxIter, pad;
+    int useScalar, nScalar, useConstTidePS, useConstTideMS, maxIter, useSubgrid;
+    // subgrid
+    double dsurf, subR;
+    int Ne, pad2;
 };
 
 struct PcgScalars {
@@ -56,7 +59,17 @@ struct StepArgs {
     // forcing (Nt+1 samples) and locations
     const double *tideP, *tideM, *inflow, *tidalPS, *tidalMS;
     const int *tideLocP, *tideLocM;
+    // subgrid: state (Ng), half-cell values (N), level search index (Ng),
+    // face bottoms with offset (N), level elevations (Ne), level tables
+    // V Z Vxp Vxm Vyp Vym Np Nm Op Om, level-major [k*N + c]
+    double *V, *Z, *Vx, *Vy, *Zx, *Zy, *Nx, *Oy;
+    double *Vxp, *Vxm, *Vyp, *Vym, *Np, *Nm, *Op, *Om;
+    int *ind;
+    const double *edgeX, *edgeY, *allSurf;
+    const double *tab[10];
 };
+
+enum SubTable { T_V, T_Z, T_VXP, T_VXM, T_VYP, T_VYM, T_NP, T_NM, T_OP, T_OM };
 
 __global__ void k_step_begin(StepArgs a);
 __global__ void k_predictor(StepArgs a);
@@ -72,5 +85,17 @@ __global__ void k_finish(StepArgs a);
 __global__ void k_scalar(StepArgs a, int q);
 __global__ void k_scalar_bc(StepArgs a, int q);
 __global__ void k_step_end(StepArgs a);
+// subgrid variants
+__global__ void k_predictor_sub(StepArgs a);
+__global__ void k_assemble_sub(StepArgs a);
+__global__ void k_subarea_init(StepArgs a);
+__global__ void k_limiter_sub(StepArgs a);
+__global__ void k_combine(StepArgs a);
+__global__ void k_sub_ghosts(StepArgs a);
+__global__ void k_faces_sub(StepArgs a);
+__global__ void k_flux_sub(StepArgs a);
+__global__ void k_tidal_velocity_sub(StepArgs a);
+__global__ void k_finish_sub(StepArgs a);
+__global__ void k_scalar_sub(StepArgs a, int q);
 
 }  // namespace frehd

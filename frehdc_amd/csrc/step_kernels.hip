@@ -53,7 +53,8 @@ __device__ inline void limit_y(double &aY, double cflYK, double cflXK, double fY
     else if (cflXK > hi) aY = 0;   // tests CFLx, nsfunctions.c:574
 }
 
-__global__ void __launch_bounds__(256) k_predictor(StepArgs a)
+template <bool SUB>
+__device__ inline void predictor_body(const StepArgs &a)
 {
     if (a.ctl->halted) return;
     const Grid g = a.g;
@@ -95,16 +96,42 @@ __global__ void __launch_bounds__(256) k_predictor(StepArgs a)
     double EnX = uc - dt * aX;
     double EnY = vc - dt * aY;
 
-    const double dfx = (p.NUx / (dx * dx)) * (ue - 2 * uc + uw) + (p.NUy / (dy * dy)) * (us - 2 * uc + un);
-    const double dfy = (p.NUx / (dx * dx)) * (ve - 2 * vc + vw) + (p.NUy / (dy * dy)) * (vs - 2 * vc + vn);
+    double dfx, dfy;
+    if (SUB) {
+        // diffusionTermSub, subgrid.c:781-815
+        const double Vxc = a.Vx[c], Vyc = a.Vy[c];
+        const double Nxc = a.Nx[c], Nxe = a.Nx[e], Nxw = a.Nx[w];
+        const double Oyc = a.Oy[c], Oyn = a.Oy[n], Oys = a.Oy[s];
+        dfx = 0;
+        dfy = 0;
+        if (Vxc > 0)
+            dfx = (p.NUx / Vxc) * (Nxe * (ue - uc) / dx - Nxc * (-uw + uc) / dx) +
+                  (p.NUy / Vxc) * (Oyc * (un - uc) / dy - Oys * (-us + uc) / dy);
+        if (Vyc > 0)
+            dfy = (p.NUx / Vyc) * (Nxc * (ve - vc) / dx - Nxw * (-vw + vc) / dx) +
+                  (p.NUy / Vyc) * (Oyn * (vn - vc) / dy - Oyc * (-vs + vc) / dy);
+    } else {
+        dfx = (p.NUx / (dx * dx)) * (ue - 2 * uc + uw) + (p.NUy / (dy * dy)) * (us - 2 * uc + un);
+        dfy = (p.NUx / (dx * dx)) * (ve - 2 * vc + vw) + (p.NUy / (dy * dy)) * (vs - 2 * vc + vn);
+    }
     EnX = EnX + dt * dfx;
     EnY = EnY + dt * dfy;
 
     const double velx = sqrt(uc * uc + vx * vx);
     const double vely = sqrt(uy * uy + vc * vc);
-    const double dX = a.depthXP[c], dY = a.depthYP[c];
-    const double DragX = dX > 0 ? 0.5 * dt * a.CDXP[c] * velx / dX : 0;
-    const double DragY = dY > 0 ? 0.5 * dt * a.CDYP[c] * vely / dY : 0;
+    double DragX, DragY;
+    if (SUB) {
+        // dragTermSub with staggeredV = 1, subgrid.c:857-927
+        const double Vxc = a.Vx[c], Vyc = a.Vy[c];
+        const double effHx = Vxc > 0 ? a.Zx[c] / Vxc : 0;
+        const double effHy = Vyc > 0 ? a.Zy[c] / Vyc : 0;
+        DragX = 0.5 * dt * a.CDXP[c] * velx * effHx;
+        DragY = 0.5 * dt * a.CDYP[c] * vely * effHy;
+    } else {
+        const double dX = a.depthXP[c], dY = a.depthYP[c];
+        DragX = dX > 0 ? 0.5 * dt * a.CDXP[c] * velx / dX : 0;
+        DragY = dY > 0 ? 0.5 * dt * a.CDYP[c] * vely / dY : 0;
+    }
     EnX = EnX / (1 + DragX);
     EnY = EnY / (1 + DragY);
 
@@ -116,6 +143,9 @@ __global__ void __launch_bounds__(256) k_predictor(StepArgs a)
     if (j == 0) { a.DragXP[gjM(g, i)] = DragX; a.EnXP[gjM(g, i)] = EnX; }
     if (i == 0) { a.DragYP[giM(g, j)] = DragY; a.EnYP[giM(g, j)] = EnY; }
 }
+
+__global__ void __launch_bounds__(256) k_predictor(StepArgs a) { predictor_body<false>(a); }
+__global__ void __launch_bounds__(256) k_predictor_sub(StepArgs a) { predictor_body<true>(a); }
 
 // ---------------------------------------------------------------------------
 // Assembly: matrixSourceTerm + matrixCoeff + adjustMatrixBoundary +
@@ -132,10 +162,68 @@ __device__ inline double face_coef(double coef, double depthF, double dragF)
     return coef * depthF / (1 + dragF);
 }
 
-__global__ void __launch_bounds__(256) k_assemble(StepArgs a)
+// Stores the block partials of b.b and r.z; the last block to finish
+// reduces every block's partials in block order and starts the PCG
+// scalars (deterministic for a given launch shape).
+__device__ inline void solver_start_reduce(const StepArgs &a, double (&part)[3])
 {
     __shared__ double lds[16 * 3];
     __shared__ int amLast;
+    const P &p = a.p;
+    block_sum<3>(part, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const int nb = gridDim.x * gridDim.y;
+        const int bid = blockIdx.x + gridDim.x * blockIdx.y;
+        double *pp = a.partials + 2 * bid;
+        __hip_atomic_store(pp + 0, part[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(pp + 1, part[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned prev = __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        amLast = (prev == (unsigned)(nb - 1));
+    }
+    __syncthreads();
+    if (!amLast) return;
+    const int nb = gridDim.x * gridDim.y;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    const int nth = blockDim.x * blockDim.y;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int q = tid; q < nb; q += nth) {
+        acc[0] += __hip_atomic_load(a.partials + 2 * q + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        acc[1] += __hip_atomic_load(a.partials + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum<3>(acc, lds);
+    if (tid == 0) {
+        PcgScalars &sc = a.ctl->pcg;
+        sc.bb = acc[0];
+        sc.rr = acc[0];
+        sc.rz = acc[1];
+        sc.iter = 0;
+        const double bNorm = sqrt(sc.bb), rNorm = bNorm;
+        sc.bNorm = bNorm;
+        sc.rNorm = rNorm;
+        sc.done = (rNorm < p.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm))) ? 1 : 0;
+        if (p.maxIter <= 0) sc.done = 1;
+        a.ctl->counter = 0;
+    }
+}
+
+// stores the operator row and the solver start of cell c
+__device__ inline void store_row(const StepArgs &a, int c, double GXP, double GYP, double D, double z,
+                                 double (&part)[3])
+{
+    a.cX[c] = GXP;
+    a.cY[c] = GYP;
+    a.D[c] = D;
+    a.b[c] = z;
+    a.r[c] = z;
+    a.x[c] = 0.0;
+    a.pA[c] = 0.0;
+    const double invd = 1.0 / fabs(D);
+    part[0] = z * z;
+    part[1] = z * (z * invd);
+}
+
+__global__ void __launch_bounds__(256) k_assemble(StepArgs a)
+{
     if (a.ctl->halted) return;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -163,52 +251,9 @@ __global__ void __launch_bounds__(256) k_assemble(StepArgs a)
             z = z + (dt / (p.dx * p.dy)) * (a.inflow[tt] / p.inflowLocLength);
         if ((fl & 1) && p.bcType != 2) { z = a.tideP[tt] + p.offset; D = -1.0; }
         if ((fl & 2) && p.bcType != 1) { z = a.tideM[tt] + p.offset; D = -1.0; }
-        a.cX[c] = GXP;
-        a.cY[c] = GYP;
-        a.D[c] = D;
-        a.b[c] = z;
-        a.r[c] = z;
-        a.x[c] = 0.0;
-        a.pA[c] = 0.0;
-        const double invd = 1.0 / fabs(D);
-        part[0] = z * z;
-        part[1] = z * (z * invd);
+        store_row(a, c, GXP, GYP, D, z, part);
     }
-    block_sum<3>(part, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const int nb = gridDim.x * gridDim.y;
-        const int bid = blockIdx.x + gridDim.x * blockIdx.y;
-        double *pp = a.partials + 2 * bid;
-        __hip_atomic_store(pp + 0, part[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(pp + 1, part[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned prev = __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        amLast = (prev == (unsigned)(nb - 1));
-    }
-    __syncthreads();
-    // the last block to finish reduces every block's partials in block order
-    if (!amLast) return;
-    const int nb = gridDim.x * gridDim.y;
-    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-    const int nth = blockDim.x * blockDim.y;
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int q = tid; q < nb; q += nth) {
-        acc[0] += __hip_atomic_load(a.partials + 2 * q + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        acc[1] += __hip_atomic_load(a.partials + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<3>(acc, lds);
-    if (tid == 0) {
-        PcgScalars &sc = a.ctl->pcg;
-        sc.bb = acc[0];
-        sc.rr = acc[0];
-        sc.rz = acc[1];
-        sc.iter = 0;
-        const double bNorm = sqrt(sc.bb), rNorm = bNorm;
-        sc.bNorm = bNorm;
-        sc.rNorm = rNorm;
-        sc.done = (rNorm < p.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm))) ? 1 : 0;
-        if (p.maxIter <= 0) sc.done = 1;
-        a.ctl->counter = 0;
-    }
+    solver_start_reduce(a, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -671,7 +716,8 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
 // scalarDiffusion + updateScalar's connectivity limiter, inflow mass and
 // concentration, and the out-of-range reset (scalar.c:68-97, 259-416).
 // The new concentration goes to Snew (neighbours read the old S).
-__global__ void __launch_bounds__(256) k_scalar(StepArgs a, int q)
+template <bool SUB>
+__device__ inline void scalar_body(const StepArgs &a, int q)
 {
     if (a.ctl->halted) return;
     const Grid g = a.g;
@@ -695,16 +741,22 @@ __global__ void __launch_bounds__(256) k_scalar(StepArgs a, int q)
         const double Sl = Fv[s] > 0 ? Ss : Sc;
         sm = sm + dt * (-Fu[c] * Sd + Fu[w] * Su - Fv[c] * Sr + Fv[s] * Sl);
     }
-    sm = sm + dt * ((p.Kx * dy * a.depthXP[c] / dx) * (Se - Sc) - (p.Kx * dy * a.depthXP[w] / dx) * (Sc - Sw) +
-                    (p.Ky * dx * a.depthYP[c] / dy) * (Sn - Sc) - (p.Ky * dx * a.depthYP[s] / dy) * (Sc - Ss));
-    // connectivity with the previous-step surface (scalar.c:292-341)
+    if (SUB)   // scalarDiffusionSub, subgrid.c:1552-1567
+        sm = sm + dt * ((p.Kx * a.Nx[c] / dx) * (Se - Sc) - (p.Kx * a.Nx[w] / dx) * (Sc - Sw) +
+                        (p.Ky * a.Oy[c] / dy) * (Sn - Sc) - (p.Ky * a.Oy[s] / dy) * (Sc - Ss));
+    else
+        sm = sm + dt * ((p.Kx * dy * a.depthXP[c] / dx) * (Se - Sc) - (p.Kx * dy * a.depthXP[w] / dx) * (Sc - Sw) +
+                        (p.Ky * dx * a.depthYP[c] / dy) * (Sn - Sc) - (p.Ky * dx * a.depthYP[s] / dy) * (Sc - Ss));
+    // connectivity with the previous-step surface (scalar.c:292-341); the
+    // subgrid tests face areas instead of face depths (subgrid.c:1588-1630)
     const double *ho = a.surfOld, *zb = a.bottomZ;
+    const double *fx = SUB ? a.Nx : a.depthXP, *fy = SUB ? a.Oy : a.depthYP;
     double vals[4];
     int m = 0;
-    if (a.depthXP[c] > 0 && ho[e] - zb[e] > 0) vals[m++] = Se;
-    if (a.depthXP[w] > 0 && ho[w] - zb[w] > 0) vals[m++] = Sw;
-    if (n <= g.N && a.depthYP[c] > 0 && ho[n] - zb[n] > 0) vals[m++] = Sn;
-    if (s <= g.N && a.depthYP[s] > 0 && ho[s] - zb[s] > 0) vals[m++] = Ss;
+    if (fx[c] > 0 && ho[e] - zb[e] > 0) vals[m++] = Se;
+    if (fx[w] > 0 && ho[w] - zb[w] > 0) vals[m++] = Sw;
+    if (n <= g.N && fy[c] > 0 && ho[n] - zb[n] > 0) vals[m++] = Sn;
+    if (s <= g.N && fy[s] > 0 && ho[s] - zb[s] > 0) vals[m++] = Ss;
     double smax = 0, smin = 0;
     if (m) {
         smax = vals[0];
@@ -714,7 +766,8 @@ __global__ void __launch_bounds__(256) k_scalar(StepArgs a, int q)
             if (vals[t] < smin) smin = vals[t];
         }
     }
-    const int ninflow = a.flags[c] >> 2;
+    // the subgrid path adds inflow only when bcType != 3 (subgrid.c:1660)
+    const int ninflow = (SUB && p.bcType == 3) ? 0 : (a.flags[c] >> 2);
     for (int t = 0; t < ninflow; t++) sm = sm + p.inflowS * a.inflow[tt] * dt / p.inflowLocLength;
     Sm[c] = sm;
     const double V = a.cellV[c];
@@ -732,6 +785,9 @@ __global__ void __launch_bounds__(256) k_scalar(StepArgs a, int q)
     }
     a.Snew[c] = Sv;
 }
+
+__global__ void __launch_bounds__(256) k_scalar(StepArgs a, int q) { scalar_body<false>(a, q); }
+__global__ void __launch_bounds__(256) k_scalar_sub(StepArgs a, int q) { scalar_body<true>(a, q); }
 
 // enforceScalarBC (scalar.c:419-471) over the interior and the four ghost
 // blocks (corners untouched).

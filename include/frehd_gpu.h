@@ -90,7 +90,12 @@ typedef struct frehd_config {
     /* refinement controls for FREHD_PREC_MIXED (ignored for FP64)               */
     double  epsInner;        /* FP32 inner PCG relative tolerance                 */
     int32_t maxRefine;       /* FP64 refinement sweeps cap                        */
-    int32_t reserved0;
+    /* subgrid bathymetry (subgrid.c; Config.useSubgrid ... useminV)            */
+    int32_t useSubgrid;      /* 0: plain grid, 1: subgrid tables                  */
+    int32_t Ne;              /* table levels, round((surfmax - surfmin)/dsurf)    */
+    int32_t pad1;
+    double  surfmin, dsurf;  /* level k is surfmin + k*dsurf (+ offset)           */
+    double  subR;            /* dx / dxf (fine cells per coarse cell side)        */
 } frehd_config;
 
 /* ---- forcing ----------------------------------------------------------------
@@ -106,6 +111,19 @@ typedef struct frehd_forcing {
     const double  *manningCell;
 } frehd_forcing;
 
+/* ---- subgrid tables (initAllSubVar, subgrid.c:2112-2284) --------------------
+ * Level-major tables: value of cell ii at level k is t[k*N + ii] (N interior
+ * cells, Ne levels).  edgeX/edgeY are the raw (offset-free) bottom
+ * elevations of the east and north faces (bottomXP.dat / bottomYP.dat).  Only
+ * staggeredV = 1, useminA = 0, useSubDrag = 0 are supported; with useminA = 0
+ * the minV* tables never change a result (they are read but inert, as in the
+ * reference with useminV = 1) and may be NULL. */
+typedef struct frehd_subgrid {
+    const double *edgeX, *edgeY;
+    const double *V, *Z, *Vxp, *Vxm, *Vyp, *Vym, *Np, *Nm, *Op, *Om;
+    const double *minVxp, *minVxm, *minVyp, *minVym;
+} frehd_subgrid;
+
 /* ---- initial condition (the inputs of Init, initialize.c:40-155) ------------
  * bottomZ is the raw bed elevation of the N interior cells as in bath.dat;
  * the offset is cfg->offset.  A per-cell initial field overrides the
@@ -114,6 +132,7 @@ typedef struct frehd_ic {
     const double *bottomZ;
     double surf0, U0, V0, S0;
     const double *surf0cell, *U0cell, *V0cell, *S0cell;
+    const frehd_subgrid *sub;   /* required when cfg->useSubgrid == 1 */
 } frehd_ic;
 
 /* ---- state arrays (Data + Bath, initialize.h:10-20, bathymetry.h:6-14) -------
@@ -136,6 +155,9 @@ typedef struct frehd_fields {
     double *CFLx, *CFLy;               /* Ng (interior written) */
     double *wtfXP, *wtfYP;             /* Ng */
     double *bottomZ, *bottomXP, *bottomYP; /* Ng */
+    /* subgrid state (Sub, subgrid.h:11-37), Ng each; ignored on a plain grid */
+    double *V, *Z, *Vx, *Vy, *Zx, *Zy, *Nx, *Oy;
+    int32_t *ind;                      /* Ng: level search start per cell     */
 } frehd_fields;
 
 /* ---- per-handle statistics -------------------------------------------------- */

@@ -56,6 +56,14 @@ struct fo_state {
     int pcgIters;
     long long pcgItersTotal;
     double relres;
+    /* subgrid (Sub, subgrid.h:11-37): tables level-major [k*N + ii] */
+    int sub, Ne;
+    double *allSurf, *edgeX, *edgeY;
+    double *tV, *tZ, *tVxp, *tVxm, *tVyp, *tVym, *tNp, *tNm, *tOp, *tOm;
+    double *tminVxp, *tminVxm, *tminVyp, *tminVym;
+    double *V, *Z, *Vxp, *Vyp, *Vxm, *Vym, *Np, *Nm, *Op, *Om, *Vx, *Vy, *Zx, *Zy, *Nx, *Oy;
+    double *minVxp, *minVxm, *minVyp, *minVym, *minVx, *minVy, *Nmin, *Omin, *Nmax, *Omax;
+    int *ind;
 };
 
 /* ---- grid index helpers (map.c:15-128) ------------------------------------ */
@@ -77,7 +85,7 @@ static inline int cornerIMJM(const fo_state *s) { return s->Ng - 1; }
 /* flat read with a zero for indices below the array (velocityInterp quirk) */
 static inline double rd0(const double *a, long idx) { return idx < 0 ? 0.0 : a[idx]; }
 
-static double *zalloc(int n) { return (double *)calloc((size_t)n, sizeof(double)); }
+static double *zalloc(size_t n) { return (double *)calloc(n, sizeof(double)); }
 
 /* ---- boundary fills ------------------------------------------------------- */
 /* enforceFreeSurfaceBC, nsfunctions.c:317-334 */
@@ -125,13 +133,15 @@ static void init_bathymetry(fo_state *s)
     for (int i = 0; i < nx; i++) { zb[gjM(s, i)] = zb[i]; zb[gjP(s, i)] = zb[N - nx + i]; }
     for (int j = 0; j < ny; j++) { zb[giM(s, j)] = zb[j * nx]; zb[giP(s, j)] = zb[j * nx + nx - 1]; }
     for (int c = 0; c < s->Ng; c++) { bx[c] = 0.0; by[c] = 0.0; }
-    for (int c = 0; c < N; c++) { double a = zb[c], b = zb[nbP(s, c)]; bx[c] = a > b ? a : b; }
+    if (s->sub) { for (int c = 0; c < N; c++) bx[c] = s->edgeX[c]; }   /* useSubgrid != 0 */
+    else for (int c = 0; c < N; c++) { double a = zb[c], b = zb[nbP(s, c)]; bx[c] = a > b ? a : b; }
     for (int j = 0; j < ny; j++) {
         bx[giM(s, j)] = zb[j * nx];
         bx[giP(s, j)] = bx[j * nx + nx - 1];
     }
     for (int i = 0; i < nx; i++) { bx[gjM(s, i)] = bx[i]; bx[gjP(s, i)] = bx[N - nx + i]; }
-    for (int c = 0; c < N; c++) { double a = zb[c], b = zb[nbN(s, c)]; by[c] = a > b ? a : b; }
+    if (s->sub) { for (int c = 0; c < N; c++) by[c] = s->edgeY[c]; }
+    else for (int c = 0; c < N; c++) { double a = zb[c], b = zb[nbN(s, c)]; by[c] = a > b ? a : b; }
     for (int i = 0; i < nx; i++) {
         double a = zb[gjM(s, i)], b = zb[i];
         by[gjM(s, i)] = a > b ? a : b;
@@ -266,6 +276,149 @@ static void thin_layer_drag(fo_state *s)
     }
 }
 
+/* ---- subgrid runtime (subgrid.c:55-526) ----------------------------------- */
+/* interpSurf, utilities.c:155-162 */
+static inline double interp_surf(double x, double x1, double y1, double x2, double y2)
+{
+    if (x2 == x1) return y1;
+    return (y2 - y1) * (x - x2) / (x2 - x1) + y2;
+}
+
+/* computeSubArea, subgrid.c:55-251: table lookup per cell.  edgeX/edgeY hold
+   N values; the west/south neighbour of a boundary cell is a ghost index,
+   read as 0 here (the reference reads past the array). */
+static void compute_sub_area(fo_state *s)
+{
+    const frehd_config *k = &s->cfg;
+    const int N = s->N, Ne = s->Ne;
+    const double dsurf = k->dsurf;
+    for (int c = 0; c < N; c++) {
+        const double h = s->surf[c];
+        while (fabs(s->allSurf[s->ind[c]] - h) > 0.5 * dsurf + dsurf / 100) {
+            if (h > s->allSurf[s->ind[c]]) s->ind[c] += 1;
+            else s->ind[c] -= 1;
+            if (s->ind[c] < 0) { s->ind[c] = 0; break; }
+            if (s->ind[c] > Ne - 1) { s->ind[c] = Ne - 1; break; }
+        }
+        const int j1 = s->ind[c];
+        int j2;
+        if (j1 == 0) j2 = 0;
+        else if (j1 == Ne - 1) j2 = Ne - 1;
+        else j2 = h > s->allSurf[j1] ? j1 + 1 : j1 - 1;
+        const size_t k1 = (size_t)N * j1 + c, k2 = (size_t)N * j2 + c;
+        const double s1 = s->allSurf[j1], s2 = s->allSurf[j2];
+        s->V[c] = interp_surf(h, s1, s->tV[k1], s2, s->tV[k2]);
+        s->Z[c] = interp_surf(h, s1, s->tZ[k1], s2, s->tZ[k2]);
+        s->Vxp[c] = interp_surf(h, s1, s->tVxp[k1], s2, s->tVxp[k2]);
+        s->Vyp[c] = interp_surf(h, s1, s->tVyp[k1], s2, s->tVyp[k2]);
+        s->Vxm[c] = interp_surf(h, s1, s->tVxm[k1], s2, s->tVxm[k2]);
+        s->Vym[c] = interp_surf(h, s1, s->tVym[k1], s2, s->tVym[k2]);
+        s->minVxp[c] = interp_surf(h, s1, s->tminVxp[k1], s2, s->tminVxp[k2]);
+        s->minVyp[c] = interp_surf(h, s1, s->tminVyp[k1], s2, s->tminVyp[k2]);
+        s->minVxm[c] = interp_surf(h, s1, s->tminVxm[k1], s2, s->tminVxm[k2]);
+        s->minVym[c] = interp_surf(h, s1, s->tminVym[k1], s2, s->tminVym[k2]);
+        const int w = nbM(s, c), so = nbS(s, c);
+        const double wd[4] = {s->edgeX[c], w < N ? s->edgeX[w] : 0.0, s->edgeY[c], so < N ? s->edgeY[so] : 0.0};
+        const double *tab[4] = {s->tNp, s->tNm, s->tOp, s->tOm};
+        double *out[4] = {s->Np, s->Nm, s->Op, s->Om};
+        for (int f = 0; f < 4; f++) {
+            if (wd[f] > s1 && wd[f] < s2) {
+                if (wd[f] > h) out[f][c] = 0;
+                else out[f][c] = interp_surf(h, wd[f], 0.0, s2, tab[f][k2]);
+            } else {
+                out[f][c] = interp_surf(h, s1, tab[f][k1], s2, tab[f][k2]);
+            }
+        }
+        if (s->depth[c] == 0) { s->Vxp[c] = 0; s->Vyp[c] = 0; s->Vxm[c] = 0; s->Vym[c] = 0; }
+        if (h - s->bottomXP[c] <= 0) s->Np[c] = 0;
+        if (h - s->bottomYP[c] <= 0) s->Op[c] = 0;
+        if (w < N && h - s->bottomXP[w] <= 0) s->Nm[c] = 0;
+        if (so < N && h - s->bottomYP[so] <= 0) s->Om[c] = 0;
+    }
+}
+
+/* combineSubArea, subgrid.c:254-526 (useminA = 0, useminV = 1 whose
+   adjustment needs Nmin > 0 and is therefore inactive) */
+static void combine_sub_area(fo_state *s)
+{
+    const frehd_config *k = &s->cfg;
+    const int N = s->N, nx = s->nx, ny = s->ny;
+    const double dxdy = k->dx * k->dy;
+    for (int c = 0; c < N; c++) {
+        const int e = nbP(s, c), n = nbN(s, c);
+        if (e < N) s->Nx[c] = s->Np[c] >= s->Nm[e] ? s->Np[c] : s->Nm[e];
+        else s->Nx[c] = s->Np[c];
+        if (n < N) s->Oy[c] = s->Op[c] >= s->Om[n] ? s->Op[c] : s->Om[n];
+        else s->Oy[c] = s->Op[c];
+        if (e < N) {
+            if (s->Vxp[c] > 0 && s->Vxm[e] > 0) s->Vx[c] = s->Vxp[c] + s->Vxm[e];
+            else s->Vx[c] = 2.0 * (s->Vxp[c] + s->Vxm[e]);
+            s->Zx[c] = s->Z[c];
+            if (s->Vx[c] > 0 && s->Zx[c] <= 0) s->Zx[c] = dxdy;
+            s->minVx[c] = s->minVxp[c] + s->minVxm[e];
+            if (s->surf[c] < s->bottomXP[c]) { s->Vx[c] = s->Vxm[e]; s->Zx[c] = 0.5 * s->Z[c]; }
+            if (s->surf[e] < s->bottomXP[c]) { s->Vx[c] = s->Vxp[c]; s->Zx[c] = 0.5 * s->Z[c]; }
+        } else {
+            s->Vx[c] = 2 * s->Vxp[c];
+            s->Zx[c] = 2 * s->Z[c];
+            s->minVx[c] = 2 * s->minVxp[c];
+        }
+        if (n < N) {
+            if (s->Vyp[c] > 0 && s->Vym[n] > 0) s->Vy[c] = s->Vyp[c] + s->Vym[n];
+            else s->Vy[c] = 2.0 * (s->Vyp[c] + s->Vym[n]);
+            s->Zy[c] = s->Z[c];
+            if (s->Vy[c] > 0 && s->Zy[c] <= 0) s->Zy[c] = dxdy;
+            s->minVy[c] = s->minVyp[c] + s->minVym[n];
+            if (s->surf[c] < s->bottomYP[c]) { s->Vy[c] = s->Vym[n]; s->Zy[c] = 0.5 * s->Z[c]; }
+            if (s->surf[n] < s->bottomYP[c]) { s->Vy[c] = s->Vyp[c]; s->Zy[c] = 0.5 * s->Z[c]; }
+        } else {
+            s->Vy[c] = 2 * s->Vyp[c];
+            s->Zy[c] = 2 * s->Z[c];
+            s->minVy[c] = 2 * s->minVyp[c];
+        }
+    }
+    const double Vmin = 0.002, Amin = 0.002, Azmin = 0.002;
+    for (int c = 0; c < N; c++) {
+        if (s->Vx[c] < Vmin) s->Vx[c] = 0.0;
+        if (s->Vy[c] < Vmin) s->Vy[c] = 0.0;
+        if (s->V[c] < Vmin) s->V[c] = 0.0;
+        if (s->Z[c] < Azmin) s->Z[c] = 1.0;
+        if (s->Zx[c] < Amin) s->Zx[c] = 0.0;
+        if (s->Zy[c] < Amin) s->Zy[c] = 0.0;
+        if (s->Nx[c] < Amin) s->Nx[c] = 0.0;
+        if (s->Oy[c] < Amin) s->Oy[c] = 0.0;
+    }
+    for (int j = 0; j < ny; j++) {
+        const int gm = giM(s, j), gp = giP(s, j), bm = j * nx, bp = j * nx + nx - 1;
+        s->Vx[gm] = s->Vx[bm]; s->Vy[gm] = s->Vy[bm]; s->Zx[gm] = s->Zx[bm]; s->Zy[gm] = s->Zy[bm];
+        s->Nx[gm] = s->Nx[bm]; s->Oy[gm] = s->Oy[bm]; s->V[gm] = s->V[bm];
+        s->minVx[gm] = s->minVy[bm]; s->minVy[gm] = s->minVx[bm]; s->Z[gm] = s->Z[bm];
+        s->Vx[gp] = s->Vx[bp]; s->Vy[gp] = s->Vy[bp]; s->Zx[gp] = s->Zx[bp]; s->Zy[gp] = s->Zy[bp];
+        s->Nx[gp] = s->Nx[bp]; s->Oy[gp] = s->Oy[bp]; s->V[gp] = s->V[bp];
+        s->minVy[gp] = s->minVx[bp]; s->minVx[gp] = s->minVy[bp]; s->Z[gp] = s->Z[bp];
+        s->Nx[gm] = 0; s->Nx[gp] = 0; s->Nx[bp] = 0;
+    }
+    for (int i = 0; i < nx; i++) {
+        const int gm = gjM(s, i), gp = gjP(s, i), bm = i, bp = N - nx + i;
+        s->Vx[gm] = s->Vx[bm]; s->Vy[gm] = s->Vy[bm]; s->Zx[gm] = s->Zx[bm]; s->Zy[gm] = s->Zy[bm];
+        s->Nx[gm] = s->Nx[bm]; s->Oy[gm] = s->Oy[bm]; s->V[gm] = s->V[bm];
+        s->minVx[gm] = s->minVx[bm]; s->minVy[gm] = s->minVy[bm]; s->Z[gm] = s->Z[bm];
+        s->Vx[gp] = s->Vx[bp]; s->Vy[gp] = s->Vy[bp]; s->Zx[gp] = s->Zx[bp]; s->Zy[gp] = s->Zy[bp];
+        s->Nx[gp] = s->Nx[bp]; s->Oy[gp] = s->Oy[bp]; s->V[gp] = s->V[bp];
+        s->minVx[gp] = s->minVx[bp]; s->minVy[gp] = s->minVy[bp]; s->Z[gp] = s->Z[bp];
+        s->Oy[gm] = 0;
+    }
+    /* YP faces closed except at tidal cells (subgrid.c:508-524) */
+    if (k->bcType != 2) {
+        for (int i = 0; i < nx; i++) {
+            int tidal = 0;
+            for (int q = 0; q < k->tideLocLengthP; q++)
+                if (s->tideLocP[q] - nx * (ny - 1) == i) tidal = 1;
+            if (!tidal) { s->Oy[N - nx + i] = 0; s->Oy[gjP(s, i)] = 0; }
+        }
+    }
+}
+
 /* ---- Phase P: solveSourceTerm (nssolve.c:121-155), plain grid -------------- */
 static void phase_source(fo_state *s)
 {
@@ -315,6 +468,25 @@ static void phase_source(fo_state *s)
         s->EnYP[c] = s->EnYP[c] - dt * aY[c];
     }
 
+    if (s->sub) {
+        /* diffusionTermSub, subgrid.c:781-815 */
+        for (int c = 0; c < N; c++) {
+            int e = nbP(s, c), w = nbM(s, c), n = nbN(s, c), so = nbS(s, c);
+            double dfx = 0, dfy = 0;
+            if (s->Vx[c] > 0)
+                dfx = (k->NUx / s->Vx[c]) *
+                          (s->Nx[e] * (u[e] - u[c]) / dx - s->Nx[c] * (-u[w] + u[c]) / dx) +
+                      (k->NUy / s->Vx[c]) *
+                          (s->Oy[c] * (u[n] - u[c]) / dy - s->Oy[so] * (-u[so] + u[c]) / dy);
+            if (s->Vy[c] > 0)
+                dfy = (k->NUx / s->Vy[c]) *
+                          (s->Nx[c] * (v[e] - v[c]) / dx - s->Nx[w] * (-v[w] + v[c]) / dx) +
+                      (k->NUy / s->Vy[c]) *
+                          (s->Oy[n] * (v[n] - v[c]) / dy - s->Oy[c] * (-v[so] + v[c]) / dy);
+            s->EnXP[c] = s->EnXP[c] + dt * dfx;
+            s->EnYP[c] = s->EnYP[c] + dt * dfy;
+        }
+    } else
     /* diffusionTerm, nsfunctions.c:589-606 */
     for (int c = 0; c < N; c++) {
         int e = nbP(s, c), w = nbM(s, c), n = nbN(s, c), so = nbS(s, c);
@@ -328,12 +500,20 @@ static void phase_source(fo_state *s)
 
     /* windTerm: useWind = 0 (nsfunctions.c:609-648 not entered) */
 
-    /* dragTerm, nsfunctions.c:651-688 */
+    /* dragTerm, nsfunctions.c:651-688 / dragTermSub (staggeredV = 1),
+       subgrid.c:857-927 */
     for (int c = 0; c < N; c++) {
         double velx = sqrt(u[c] * u[c] + vX[c] * vX[c]);
         double vely = sqrt(uY[c] * uY[c] + v[c] * v[c]);
-        s->DragXP[c] = s->depthXP[c] > 0 ? 0.5 * dt * s->CDXP[c] * velx / s->depthXP[c] : 0;
-        s->DragYP[c] = s->depthYP[c] > 0 ? 0.5 * dt * s->CDYP[c] * vely / s->depthYP[c] : 0;
+        if (s->sub) {
+            double effHx = s->Vx[c] > 0 ? s->Zx[c] / s->Vx[c] : 0;
+            double effHy = s->Vy[c] > 0 ? s->Zy[c] / s->Vy[c] : 0;
+            s->DragXP[c] = 0.5 * dt * s->CDXP[c] * velx * effHx;
+            s->DragYP[c] = 0.5 * dt * s->CDYP[c] * vely * effHy;
+        } else {
+            s->DragXP[c] = s->depthXP[c] > 0 ? 0.5 * dt * s->CDXP[c] * velx / s->depthXP[c] : 0;
+            s->DragYP[c] = s->depthYP[c] > 0 ? 0.5 * dt * s->CDYP[c] * vely / s->depthYP[c] : 0;
+        }
     }
     for (int i = 0; i < s->nx; i++) s->DragXP[gjM(s, i)] = s->DragXP[i];
     for (int j = 0; j < s->ny; j++) s->DragYP[giM(s, j)] = s->DragYP[j * s->nx];
@@ -346,6 +526,23 @@ static void phase_source(fo_state *s)
     for (int i = 0; i < s->nx; i++) s->EnXP[gjM(s, i)] = s->EnXP[i];
     for (int j = 0; j < s->ny; j++) s->EnYP[giM(s, j)] = s->EnYP[j * s->nx];
 
+    if (s->sub) {
+        /* matrixSourceTermSub (staggeredV = 1), subgrid.c:930-975 */
+        for (int c = 0; c < N; c++) {
+            int w = nbM(s, c), so = nbS(s, c);
+            s->z[c] = s->surf[c] * s->Z[c]
+                    - dt * (s->EnXP[c] * s->Nx[c] - s->EnXP[w] * s->Nx[w])
+                    - dt * (s->EnYP[c] * s->Oy[c] - s->EnYP[so] * s->Oy[so]);
+        }
+        if (k->bcType != 3)
+            for (int q = 0; q < k->inflowLocLength; q++) {
+                int c = s->inflowLoc[q];
+                s->z[c] = s->z[c] + dt * (s->inflow[s->tt] / k->inflowLocLength);
+            }
+        for (int c = 0; c < N; c++)
+            if (s->z[c] == 0 || s->z[c] == s->surf[c] * dx * dy) s->z[c] = s->surf[c] * dx * dy;
+        return;
+    }
     /* matrixSourceTerm, nsfunctions.c:767-797; z kept in cell order here
        (the reference stores it in the transposed solver order) */
     for (int c = 0; c < N; c++) {
@@ -495,6 +692,29 @@ static void phase_surface(fo_state *s)
     const int nx = s->nx, ny = s->ny, N = s->N, tt = s->tt;
     const double dt = k->dt;
 
+    if (s->sub) {
+        /* matrixCoeffSub (useSubgrid = 1, staggeredV = 1), subgrid.c:978-1081 */
+        const double cx = k->g * (dt * dt), cy = k->g * (dt * dt), dxdy = k->dx * k->dy;
+        const double divider = 1.0;
+        for (int c = 0; c < N; c++) {
+            int w = nbM(s, c), so = nbS(s, c);
+            s->GnXP[c] = 0; s->GnXM[c] = 0; s->GnYP[c] = 0; s->GnYM[c] = 0;
+            if (s->Vx[c] > 0) s->GnXP[c] = cx * s->Nx[c] * s->Nx[c] / s->Vx[c] / divider / (1 + s->DragXP[c]);
+            if (s->Vx[w] > 0) s->GnXM[c] = cx * s->Nx[w] * s->Nx[w] / s->Vx[w] / divider / (1 + s->DragXP[w]);
+            if (s->Vy[c] > 0) s->GnYP[c] = cy * s->Oy[c] * s->Oy[c] / s->Vy[c] / divider / (1 + s->DragYP[c]);
+            if (s->Vy[so] > 0) s->GnYM[c] = cy * s->Oy[so] * s->Oy[so] / s->Vy[so] / divider / (1 + s->DragYP[so]);
+            s->GnCt[c] = s->Z[c] + s->GnXP[c] + s->GnYP[c] + s->GnXM[c] + s->GnYM[c];
+        }
+        for (int c = 0; c < N; c++) {
+            if (s->GnCt[c] == 0.0 || s->GnCt[c] == dxdy) s->GnCt[c] = dxdy;
+            if (s->depth[c] == 0 && s->uuXP[c] == 0 && s->uuXP[nbM(s, c)] == 0 && s->vvYP[c] == 0 &&
+                s->vvYP[nbS(s, c)] == 0) {
+                s->GnCt[c] = dxdy;
+                s->GnXM[c] = 0.0; s->GnXP[c] = 0.0; s->GnYM[c] = 0.0; s->GnYP[c] = 0.0;
+                s->z[c] = s->surf[c] * dxdy;
+            }
+        }
+    } else {
     /* matrixCoeff, nsfunctions.c:815-843 */
     double coefx = k->g * (dt * dt / (k->dx * k->dx));
     double coefy = k->g * (dt * dt / (k->dy * k->dy));
@@ -505,6 +725,7 @@ static void phase_surface(fo_state *s)
         s->GnYP[c] = coefy * s->depthYP[c] / (1 + s->DragYP[c]);
         s->GnYM[c] = coefy * s->depthYP[so] / (1 + s->DragYP[so]);
         s->GnCt[c] = 1 + s->GnXP[c] + s->GnYP[c] + s->GnXM[c] + s->GnYM[c];
+    }
     }
     /* adjustMatrixBoundary, nsfunctions.c:846-884: Neumann walls on all sides */
     for (int j = 0; j < ny; j++) {
@@ -581,10 +802,90 @@ static void phase_update(fo_state *s)
         }
     }
 
-    compute_volume(s);
+    if (s->sub) {
+        /* computeVolumeSub (subgrid.c:1215-1224): the previous step's V */
+        for (int c = 0; c < N; c++) {
+            s->cellV[c] = s->V[c];
+            if (k->useScalar == 1)
+                for (int q = 0; q < k->nScalar; q++)
+                    s->Sm[q * Ng + c] = s->S[q * Ng + c] * s->cellV[c];
+        }
+        compute_sub_area(s);
+        combine_sub_area(s);
+    } else {
+        compute_volume(s);
+    }
     update_depth(s);
     update_face_depth(s);
 
+    double *u = s->uuXP, *v = s->vvYP;
+    if (s->sub) {
+        /* detectWaterfallLocationSub, subgrid.c:1335-1397: flags reset here,
+           depth test on the neighbour's surface above the face bottom */
+        for (int c = 0; c < N; c++) { s->wtfXP[c] = 0; s->wtfYP[c] = 0; }
+        for (int c = 0; c < N; c++) {
+            double bot = s->bottomXP[c];
+            if (h[c] < bot - 0.0 && h[nbP(s, c)] - bot >= k->wtfh) s->wtfXP[c] = -1;
+        }
+        for (int c = 0; c < N; c++) {
+            int w = nbM(s, c);
+            double bot = s->bottomXP[w];
+            if (h[c] < bot - 0.0 && h[w] - bot >= k->wtfh) s->wtfXP[w] = 1;
+        }
+        for (int c = 0; c < N; c++) {
+            double bot = s->bottomYP[c];
+            if (h[c] < bot - 0.0 && h[nbN(s, c)] - bot >= k->wtfh) s->wtfYP[c] = -1;
+        }
+        for (int c = 0; c < N; c++) {
+            int so = nbS(s, c);
+            double bot = s->bottomYP[so];
+            if (h[c] < bot - 0.0 && h[so] - bot >= k->wtfh) s->wtfYP[so] = 1;
+        }
+        /* updateVelocitySub (useSubgrid = 1, staggeredV = 1), subgrid.c:1084-1187 */
+        const double divider = 1.0;
+        for (int c = 0; c < N; c++) {
+            double effHx = s->Vx[c] > 0 ? s->Nx[c] / s->Vx[c] : 0;
+            double effHy = s->Vy[c] > 0 ? s->Oy[c] / s->Vy[c] : 0;
+            u[c] = s->EnXP[c] - (k->g * dt * effHx) * (h[nbP(s, c)] - h[c]) / divider;
+            v[c] = s->EnYP[c] - (k->g * dt * effHy) * (h[nbN(s, c)] - h[c]) / divider;
+        }
+        for (int c = 0; c < N; c++) {
+            if (s->Nx[c] == 0) u[c] = 0;
+            if (s->Oy[c] == 0) v[c] = 0;
+        }
+        /* no outflow from a dry cell (in place, neighbour faces included) */
+        for (int c = 0; c < N; c++) {
+            if (d[c] == 0) {
+                int w = nbM(s, c), so = nbS(s, c);
+                if (u[c] > 0) u[c] = 0;
+                if (u[w] < 0) u[w] = 0;
+                if (v[c] > 0) v[c] = 0;
+                if (v[so] < 0) v[so] = 0;
+            }
+        }
+        for (int c = 0; c < N; c++) {
+            if (s->rmvCFL[c] == 1) {
+                u[c] = 0; v[c] = 0; u[nbM(s, c)] = 0; v[nbS(s, c)] = 0;
+                s->rmvCFL[c] = 0;
+            }
+        }
+        /* waterfallCorrectionSub, subgrid.c:1400-1444 (flags kept) */
+        const double CFLmax = 0.24 / k->subR, Cw = 0.7;
+        for (int c = 0; c < N; c++) {
+            if (s->wtfXP[c] != 0) {
+                double vel0 = u[c];
+                u[c] = s->wtfXP[c] * Cw * sqrt(2 * k->g * s->Nx[c] / dy);
+                if (fabs(u[c]) > CFLmax * dx / dt) u[c] = s->wtfXP[c] * CFLmax * dx / dt;
+                s->Vloss[3] -= (vel0 - u[c]) * dt * s->Nx[c];
+            }
+            if (s->wtfYP[c] != 0) {
+                double vel0 = v[c];
+                v[c] = s->wtfYP[c] * Cw * sqrt(2 * k->g * s->Oy[c] / dx);
+                if (fabs(v[c]) > CFLmax * dy / dt) v[c] = s->wtfYP[c] * CFLmax * dy / dt;
+                s->Vloss[3] -= (vel0 - v[c]) * dt * s->Oy[c];
+            }
+        }
+    } else {
     /* detectWaterfallLocation, nsfunctions.c:1080-1130 (dmin = 0) */
     for (int c = 0; c < N; c++)
         if (h[c] < s->bottomXP[c] - 0.0 && s->depthXP[c] >= k->wtfh) s->wtfXP[c] = -1;
@@ -600,7 +901,6 @@ static void phase_update(fo_state *s)
     }
 
     /* updateVelocity, nsfunctions.c:250-286: not divided by (1 + Drag) */
-    double *u = s->uuXP, *v = s->vvYP;
     for (int c = 0; c < N; c++) {
         u[c] = s->EnXP[c] - (k->g * dt / dx) * (h[nbP(s, c)] - h[c]);
         v[c] = s->EnYP[c] - (k->g * dt / dy) * (h[nbN(s, c)] - h[c]);
@@ -632,46 +932,76 @@ static void phase_update(fo_state *s)
         }
     }
     for (int c = 0; c < N; c++) { s->wtfXP[c] = 0; s->wtfYP[c] = 0; }
+    }
 
     fill_velocity_ghosts(s);
 
-    /* computeFaceFlowRate, nsfunctions.c:350-358 (over Ng) */
+    /* computeFaceFlowRate[Sub], nsfunctions.c:350-358 / subgrid.c:1190-1212 (over Ng) */
     for (int c = 0; c < Ng; c++) {
-        s->Fuu[c] = u[c] * dy * s->depthXP[c];
-        s->Fvv[c] = v[c] * dx * s->depthYP[c];
+        if (s->sub) {
+            s->Fuu[c] = u[c] * s->Nx[c];
+            s->Fvv[c] = v[c] * s->Oy[c];
+        } else {
+            s->Fuu[c] = u[c] * dy * s->depthXP[c];
+            s->Fvv[c] = v[c] * dx * s->depthYP[c];
+        }
     }
 
-    /* adjustTidalVelocity, nsfunctions.c:1034-1078 */
+    /* adjustTidalVelocity[Sub], nsfunctions.c:1034-1078 / subgrid.c:1291-1332 */
     if (k->bcType != 2)
         for (int q = 0; q < k->tideLocLengthP; q++) {
             int c = s->tideLocP[q];
-            double Fww = (h[c] - ho[c]) * dx * dy / dt;
-            double all = s->Fvv[nbS(s, c)] + s->Fuu[nbM(s, c)] - s->Fuu[c] - Fww;
-            if (all != 0 && d[c] > 0) v[c] = all / (d[c] * dx);
-            else v[c] = 0;
+            if (s->sub) {
+                double Fww = (h[c] - ho[c]) * s->Z[c] / dt;
+                double all = s->Fvv[nbS(s, c)] + s->Fuu[nbM(s, c)] - s->Fuu[c] - Fww;
+                if (all != 0 && s->Oy[c] > 0) v[c] = all / s->Oy[c];
+                else v[c] = 0;
+            } else {
+                double Fww = (h[c] - ho[c]) * dx * dy / dt;
+                double all = s->Fvv[nbS(s, c)] + s->Fuu[nbM(s, c)] - s->Fuu[c] - Fww;
+                if (all != 0 && d[c] > 0) v[c] = all / (d[c] * dx);
+                else v[c] = 0;
+            }
         }
     if (k->bcType != 1)
         for (int q = 0; q < k->tideLocLengthM; q++) {
             int c = s->tideLocM[q];
-            double Fww = (h[c] - ho[c]) * dx * dy / dt;
-            double all = -s->Fvv[c] + s->Fuu[nbM(s, c)] - s->Fuu[c] - Fww;
-            if (all != 0 && d[c] > 0) v[nbS(s, c)] = -all / (d[c] * dx);
+            if (s->sub) {
+                double Fww = (h[c] - ho[c]) * s->Z[c] / dt;
+                double all = -s->Fvv[c] + s->Fuu[nbM(s, c)] - s->Fuu[c] - Fww;
+                if (all != 0 && s->Oy[c] > 0) v[nbS(s, c)] = -all / s->Oy[c];
+                else v[nbS(s, c)] = 0;
+            } else {
+                double Fww = (h[c] - ho[c]) * dx * dy / dt;
+                double all = -s->Fvv[c] + s->Fuu[nbM(s, c)] - s->Fuu[c] - Fww;
+                if (all != 0 && d[c] > 0) v[nbS(s, c)] = -all / (d[c] * dx);
+            }
         }
 
     velocity_interp(s);
 
-    /* volumeByFlux, nsfunctions.c:373-409 */
+    /* volumeByFlux[Sub], nsfunctions.c:373-409 / subgrid.c:1227-1263 */
+    const int inflowOn = s->sub ? (k->bcType != 3) : 1;
     for (int c = 0; c < N; c++) {
         s->cellV[c] = s->cellV[c] + dt * (s->Fuu[nbM(s, c)] - s->Fuu[c] + s->Fvv[nbS(s, c)] - s->Fvv[c]);
-        for (int q = 0; q < k->inflowLocLength; q++)
-            if (c == s->inflowLoc[q]) s->cellV[c] = s->cellV[c] + s->inflow[tt] * dt;
+        if (inflowOn)
+            for (int q = 0; q < k->inflowLocLength; q++)
+                if (c == s->inflowLoc[q]) s->cellV[c] = s->cellV[c] + s->inflow[tt] * dt;
         if (s->cellV[c] < 0) s->cellV[c] = 0;
     }
 
-    /* updateCD, nsfunctions.c:1170-1189 */
+    /* updateCD, nsfunctions.c:1170-1189 / updateCDSub (useSubDrag = 0),
+       subgrid.c:1520-1535 with h = V/Z */
     if (k->CDnotN == 0) {
         for (int c = 0; c < N; c++) {
-            if (d[c] > 0) {
+            if (s->sub) {
+                if (s->V[c] > 0) {
+                    double coeff = manning_coeff(s, c);
+                    double hh = s->V[c] / s->Z[c];
+                    s->CDXP[c] = coeff / pow(hh, 1.0 / 3.0);
+                    s->CDYP[c] = coeff / pow(hh, 1.0 / 3.0);
+                }
+            } else if (d[c] > 0) {
                 double coeff = manning_coeff(s, c);
                 s->CDXP[c] = coeff / pow(d[c], 1.0 / 3.0);
                 s->CDYP[c] = coeff / pow(d[c], 1.0 / 3.0);
@@ -679,7 +1009,7 @@ static void phase_update(fo_state *s)
         }
     }
     if (k->useThinLayer == 1) thin_layer_drag(s);
-    (void)ny;
+    (void)ny; (void)nx;
 }
 
 /* ---- Phase T: scalarTransport (scalar.c:30-56), upwind, plain grid ---------- */
@@ -705,6 +1035,17 @@ static void scalar_pass(fo_state *s, double *S, double *Sm)
         double Sl = Fv[so] > 0 ? S[so] : S[c];
         Sm[c] = Sm[c] + dt * (-Fu[c] * Sd + Fu[w] * Su - Fv[c] * Sr + Fv[so] * Sl);
     }
+    if (s->sub) {
+        /* scalarDiffusionSub, subgrid.c:1552-1567 */
+        for (int c = 0; c < N; c++) {
+            int e = nbP(s, c), w = nbM(s, c), n = nbN(s, c), so = nbS(s, c);
+            Sm[c] = Sm[c] + dt *
+                ((k->Kx * s->Nx[c] / dx) * (S[e] - S[c]) -
+                 (k->Kx * s->Nx[w] / dx) * (S[c] - S[w]) +
+                 (k->Ky * s->Oy[c] / dy) * (S[n] - S[c]) -
+                 (k->Ky * s->Oy[so] / dy) * (S[c] - S[so]));
+        }
+    } else
     /* scalarDiffusion, scalar.c:259-274 */
     for (int c = 0; c < N; c++) {
         int e = nbP(s, c), w = nbM(s, c), n = nbN(s, c), so = nbS(s, c);
@@ -720,11 +1061,14 @@ static void scalar_pass(fo_state *s, double *S, double *Sm)
     for (int c = 0; c < N; c++) {
         int e = nbP(s, c), w = nbM(s, c), n = nbN(s, c), so = nbS(s, c);
         double vals[4]; int m = 0;
-        if (s->depthXP[c] > 0 && ho[e] - zb[e] > 0) vals[m++] = S[e];
-        if (s->depthXP[w] > 0 && ho[w] - zb[w] > 0) vals[m++] = S[w];
+        /* faces open: depth (plain) or subgrid face area (updateScalarSub,
+           subgrid.c:1588-1630) */
+        const double *fx = s->sub ? s->Nx : s->depthXP, *fy = s->sub ? s->Oy : s->depthYP;
+        if (fx[c] > 0 && ho[e] - zb[e] > 0) vals[m++] = S[e];
+        if (fx[w] > 0 && ho[w] - zb[w] > 0) vals[m++] = S[w];
         /* last strip: only neighbour indices <= N count (scalar.c:308-315) */
-        if (n <= N && s->depthYP[c] > 0 && ho[n] - zb[n] > 0) vals[m++] = S[n];
-        if (so <= N && s->depthYP[so] > 0 && ho[so] - zb[so] > 0) vals[m++] = S[so];
+        if (n <= N && fy[c] > 0 && ho[n] - zb[n] > 0) vals[m++] = S[n];
+        if (so <= N && fy[so] > 0 && ho[so] - zb[so] > 0) vals[m++] = S[so];
         cnt[c] = m;
         if (m) {
             double mx = vals[0], mn = vals[0];
@@ -732,13 +1076,15 @@ static void scalar_pass(fo_state *s, double *S, double *Sm)
             smax[c] = mx; smin[c] = mn;
         }
     }
-    for (int q = 0; q < k->inflowLocLength; q++) {
-        int c = s->inflowLoc[q];
-        Sm[c] = Sm[c] + k->inflowS * s->inflow[tt] * dt / k->inflowLocLength;
-    }
+    const int inflowOn = s->sub ? (k->bcType != 3) : 1;
+    if (inflowOn)
+        for (int q = 0; q < k->inflowLocLength; q++) {
+            int c = s->inflowLoc[q];
+            Sm[c] = Sm[c] + k->inflowS * s->inflow[tt] * dt / k->inflowLocLength;
+        }
     for (int c = 0; c < N; c++) {
         S[c] = s->cellV[c] > 0 ? Sm[c] / s->cellV[c] : 0;
-        int fl = is_inflow_cell(s, c);
+        int fl = inflowOn && is_inflow_cell(s, c);
         if (cnt[c] != 0 && S[c] > smax[c] && !fl) S[c] = smax[c];
         else if (cnt[c] != 0 && S[c] < smin[c] && !fl) S[c] = smin[c];
     }
@@ -790,7 +1136,9 @@ static int phase_cfl(fo_state *s)
 int fo_phase(fo_state *s, int phase)
 {
     switch (phase) {
-    case FO_PHASE_SOURCE: s->tt++; phase_source(s); return 0;
+    case FO_PHASE_SOURCE:
+        if (s->tt + 1 > s->cfg.Nt) return FREHD_EINVAL;   /* forcing series exhausted */
+        s->tt++; phase_source(s); return 0;
     case FO_PHASE_SURFACE: phase_surface(s); return 0;
     case FO_PHASE_UPDATE: phase_update(s); return 0;
     case FO_PHASE_SCALAR: phase_scalar(s); return 0;
@@ -801,7 +1149,8 @@ int fo_phase(fo_state *s, int phase)
 
 int fo_step(fo_state *s)
 {
-    for (int p = FO_PHASE_SOURCE; p < FO_PHASE_CFL; p++) fo_phase(s, p);
+    if (fo_phase(s, FO_PHASE_SOURCE)) return FREHD_EINVAL;
+    for (int p = FO_PHASE_SURFACE; p < FO_PHASE_CFL; p++) fo_phase(s, p);
     return fo_phase(s, FO_PHASE_CFL);
 }
 
@@ -848,8 +1197,21 @@ fo_state *fo_create(const frehd_config *cfg, const frehd_forcing *f, const frehd
     s->S = zalloc(ns * Ng);
     s->Sm = zalloc(ns * Ng);
 
-    /* InitBathymetry (bathymetry.c:74-104): interior bed + offset */
+    /* InitBathymetry (bathymetry.c:74-104): interior bed (and face edges for
+       the subgrid) + offset */
     for (int c = 0; c < N; c++) s->bottomZ[c] = ic->bottomZ[c] + cfg->offset;
+    if (cfg->useSubgrid) {
+        const frehd_subgrid *sg = ic->sub;
+        if (!sg || !sg->edgeX || !sg->edgeY || cfg->Ne < 1) { fo_destroy(s); return NULL; }
+        s->sub = 1;
+        s->Ne = cfg->Ne;
+        s->edgeX = zalloc(Ng);
+        s->edgeY = zalloc(Ng);
+        for (int c = 0; c < N; c++) {
+            s->edgeX[c] = sg->edgeX[c] + cfg->offset;
+            s->edgeY[c] = sg->edgeY[c] + cfg->offset;
+        }
+    }
     init_bathymetry(s);
     /* initCD (initialize.c:381-389) */
     for (int c = 0; c < Ng; c++) { s->CDXP[c] = cfg->CDx; s->CDYP[c] = cfg->CDy; }
@@ -868,7 +1230,33 @@ fo_state *fo_create(const frehd_config *cfg, const frehd_forcing *f, const frehd
     tidal_surface(s, 0);
     update_depth(s);
     update_face_depth(s);
-    compute_volume(s);
+    if (s->sub) {
+        /* initAllSubVar (subgrid.c:2112-2284), computeSubArea, combineSubArea */
+        const frehd_subgrid *sg = ic->sub;
+        const int Ne = s->Ne;
+        const size_t tn = (size_t)Ne * N;
+        s->allSurf = zalloc(Ne);
+        for (int q = 0; q < Ne; q++) s->allSurf[q] = cfg->surfmin + q * cfg->dsurf + cfg->offset;
+        double **tabs[] = {&s->tV, &s->tZ, &s->tVxp, &s->tVxm, &s->tVyp, &s->tVym, &s->tNp,
+                           &s->tNm, &s->tOp, &s->tOm, &s->tminVxp, &s->tminVxm, &s->tminVyp,
+                           &s->tminVym};
+        const double *src[] = {sg->V, sg->Z, sg->Vxp, sg->Vxm, sg->Vyp, sg->Vym, sg->Np, sg->Nm,
+                               sg->Op, sg->Om, sg->minVxp, sg->minVxm, sg->minVyp, sg->minVym};
+        for (int q = 0; q < 14; q++) {
+            *tabs[q] = (double *)calloc(tn, sizeof(double));
+            if (src[q]) memcpy(*tabs[q], src[q], sizeof(double) * tn);
+        }
+        double **subArr[] = {&s->V, &s->Z, &s->Vxp, &s->Vyp, &s->Vxm, &s->Vym, &s->Np, &s->Nm,
+                             &s->Op, &s->Om, &s->Vx, &s->Vy, &s->Zx, &s->Zy, &s->Nx, &s->Oy,
+                             &s->minVxp, &s->minVxm, &s->minVyp, &s->minVym, &s->minVx,
+                             &s->minVy, &s->Nmin, &s->Omin, &s->Nmax, &s->Omax};
+        for (size_t q = 0; q < sizeof(subArr) / sizeof(subArr[0]); q++) *subArr[q] = zalloc(Ng);
+        s->ind = (int *)calloc((size_t)Ng, sizeof(int));
+        for (int c = 0; c < Ng; c++) s->ind[c] = Ne / 2;   /* round(Ne / 2), integer division */
+        compute_sub_area(s);
+        combine_sub_area(s);
+    }
+    compute_volume(s);    /* the plain computeVolume, also with the subgrid (initialize.c:134) */
     velocity_interp(s);
     thin_layer_drag(s);   /* called unconditionally by Init (initialize.c:138) */
     if (cfg->useScalar == 1) {
@@ -891,6 +1279,13 @@ void fo_destroy(fo_state *s)
         s->GnYM, s->z, s->S, s->Sm, s->tideP, s->tideM, s->inflow, s->tidalPS, s->tidalMS,
         s->manningCell};
     for (size_t q = 0; q < sizeof(arrs) / sizeof(arrs[0]); q++) free(arrs[q]);
+    double *subs[] = {s->allSurf, s->edgeX, s->edgeY, s->tV, s->tZ, s->tVxp, s->tVxm, s->tVyp,
+        s->tVym, s->tNp, s->tNm, s->tOp, s->tOm, s->tminVxp, s->tminVxm, s->tminVyp, s->tminVym,
+        s->V, s->Z, s->Vxp, s->Vyp, s->Vxm, s->Vym, s->Np, s->Nm, s->Op, s->Om, s->Vx, s->Vy,
+        s->Zx, s->Zy, s->Nx, s->Oy, s->minVxp, s->minVxm, s->minVyp, s->minVym, s->minVx,
+        s->minVy, s->Nmin, s->Omin, s->Nmax, s->Omax};
+    for (size_t q = 0; q < sizeof(subs) / sizeof(subs[0]); q++) free(subs[q]);
+    free(s->ind);
     free(s->tideLocP); free(s->tideLocM); free(s->inflowLoc);
     free(s);
 }
@@ -910,12 +1305,18 @@ double *fo_field(fo_state *s, const char *name, int64_t *len)
         {"bottomYP", s->bottomYP, s->Ng}, {"advX", s->advX, s->Ng}, {"advY", s->advY, s->Ng},
         {"rmvCFL", s->rmvCFL, s->N}, {"GnCt", s->GnCt, s->N}, {"GnXP", s->GnXP, s->N},
         {"GnXM", s->GnXM, s->N}, {"GnYP", s->GnYP, s->N}, {"GnYM", s->GnYM, s->N},
-        {"z", s->z, s->N}};
+        {"z", s->z, s->N}, {"V", s->V, s->Ng}, {"Z", s->Z, s->Ng}, {"Vx", s->Vx, s->Ng},
+        {"Vy", s->Vy, s->Ng}, {"Zx", s->Zx, s->Ng}, {"Zy", s->Zy, s->Ng}, {"Nx", s->Nx, s->Ng},
+        {"Oy", s->Oy, s->Ng}, {"Vxp", s->Vxp, s->N}, {"Vxm", s->Vxm, s->N}, {"Vyp", s->Vyp, s->N},
+        {"Vym", s->Vym, s->N}, {"Np", s->Np, s->N}, {"Nm", s->Nm, s->N}, {"Op", s->Op, s->N},
+        {"Om", s->Om, s->N}, {"minVx", s->minVx, s->Ng}, {"minVy", s->minVy, s->Ng}};
     for (size_t q = 0; q < sizeof(tab) / sizeof(tab[0]); q++)
-        if (strcmp(tab[q].n, name) == 0) { if (len) *len = tab[q].l; return tab[q].p; }
+        if (strcmp(tab[q].n, name) == 0 && tab[q].p) { if (len) *len = tab[q].l; return tab[q].p; }
     if (len) *len = 0;
     return NULL;
 }
+
+int32_t *fo_ind(fo_state *s) { return s->ind; }
 
 int32_t fo_tt(const fo_state *s) { return s->tt; }
 int32_t fo_pcg_iters(const fo_state *s) { return s->pcgIters; }

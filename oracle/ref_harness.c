@@ -61,7 +61,7 @@ typedef struct ref_handle {
     Sub *sub;
     IC *ic;
     int tt, precond, lastIter;
-    double *blocks[128];
+    double *blocks[256];
     int nblocks;
 } ref_handle;
 
@@ -131,7 +131,11 @@ static void fill_config(Config *s, const frehd_config *c)
     s->useConstTidePS = c->useConstTidePS; s->useConstTideMS = c->useConstTideMS;
     s->tidePS = c->tidePS; s->tideMS = c->tideMS; s->inflowS = c->inflowS;
     s->Kx = c->Kx; s->Ky = c->Ky;
-    s->useSubgrid = 0; s->useMPI = 0; s->np = 1; s->useWind = 0;
+    s->useSubgrid = c->useSubgrid; s->useMPI = 0; s->np = 1; s->useWind = 0;
+    s->staggeredV = 1; s->useminA = 0; s->useminV = 1; s->useSubDrag = 0; s->beta = 0;
+    s->dsurf = c->dsurf; s->surfmin = c->surfmin; s->surfmax = c->surfmin + c->Ne * c->dsurf;
+    s->subR = c->subR; s->dxf = c->dx / c->subR; s->dyf = c->dy / c->subR;
+    s->dA = s->dxf * s->dyf; s->z0 = c->hD;
     s->useEvap = 0; s->useRain = 0; s->useSubStep = 0; s->checkConservation = 0;
     s->inflow_rank = 0; s->useCellEdge = 0;
 }
@@ -170,6 +174,17 @@ void *ref_create(const frehd_config *c, const frehd_forcing *f, const frehd_ic *
     b->edgeY = padded(h, Ng, pad);
     for (int q = 0; q < N; q++) b->allZ[q] = icin->bottomZ[q];
     for (int q = 0; q < N; q++) b->bottomZ[q] = b->allZ[q] + b->offset[0];
+    b->allEdgeX = padded(h, N, pad);
+    b->allEdgeY = padded(h, N, pad);
+    if (c->useSubgrid) {
+        if (!icin->sub || !icin->sub->edgeX || !icin->sub->edgeY) return NULL;
+        for (int q = 0; q < N; q++) {
+            b->allEdgeX[q] = icin->sub->edgeX[q];
+            b->allEdgeY[q] = icin->sub->edgeY[q];
+            b->edgeX[q] = b->allEdgeX[q] + b->offset[0];
+            b->edgeY[q] = b->allEdgeY[q] + b->offset[0];
+        }
+    }
 
     /* Data arrays (initDataArrays, initialize.c:178-270), zero padded */
     h->data = (Data *)calloc(1, sizeof(Data));
@@ -224,6 +239,33 @@ void *ref_create(const frehd_config *c, const frehd_forcing *f, const frehd_ic *
     enforceTidalBC(&h->data, b, bc, s, 0, 0, 1);
     updateDepth(&h->data, h->map, b, s);
     updateAllFaceDepth(&h->data, b, h->map, s, 0, 1);
+    if (c->useSubgrid) {
+        /* initAllSubVar (subgrid.c:2112-2284) without the file reads */
+        Sub *sb = h->sub;
+        const int Ne = c->Ne;
+        sb->Ne = Ne;
+        sb->allSurf = padded(h, Ne, 4);
+        for (int q = 0; q < Ne; q++) sb->allSurf[q] = s->surfmin + q * s->dsurf + b->offset[0];
+        const frehd_subgrid *sg = icin->sub;
+        double **tabs[] = {&sb->allV, &sb->allZ, &sb->allVxp, &sb->allVxm, &sb->allVyp, &sb->allVym,
+            &sb->allNp, &sb->allNm, &sb->allOp, &sb->allOm, &sb->allminVxp, &sb->allminVxm,
+            &sb->allminVyp, &sb->allminVym};
+        const double *src[] = {sg->V, sg->Z, sg->Vxp, sg->Vxm, sg->Vyp, sg->Vym, sg->Np, sg->Nm,
+            sg->Op, sg->Om, sg->minVxp, sg->minVxm, sg->minVyp, sg->minVym};
+        for (int q = 0; q < 14; q++) {
+            *tabs[q] = padded(h, (size_t)Ne * N, pad);
+            if (src[q]) memcpy(*tabs[q], src[q], sizeof(double) * (size_t)Ne * N);
+        }
+        double **arr[] = {&sb->V, &sb->Z, &sb->Vxp, &sb->Vyp, &sb->Vxm, &sb->Vym, &sb->Np, &sb->Op,
+            &sb->Nm, &sb->Om, &sb->Vx, &sb->Vy, &sb->Nx, &sb->Oy, &sb->Zx, &sb->Zy, &sb->Nmin,
+            &sb->Omin, &sb->Nmax, &sb->Omax, &sb->minVyp, &sb->minVxp, &sb->minVym, &sb->minVxm,
+            &sb->minVy, &sb->minVx};
+        for (size_t q = 0; q < sizeof(arr) / sizeof(arr[0]); q++) *arr[q] = padded(h, Ng, pad);
+        sb->ind = ialloc(Ng);
+        for (int q = 0; q < Ng; q++) sb->ind[q] = round(Ne / 2);
+        computeSubArea(&h->sub, h->data, b, h->map, s);
+        combineSubArea(&h->sub, h->data, b, h->map, s, 0, 1);
+    }
     computeVolume(&h->data, s);
     velocityInterp(&h->data, s);
     thinLayerDrag(&h->data, s);
@@ -246,7 +288,8 @@ static void free_surface_phase(ref_handle *h)
     if (h->precond == FREHD_PRECOND_SSOR) {
         solveFreeSurface(&h->data, h->sub, h->map, h->bc, h->bath, s, h->tt, A, x, z, 0, 1);
     } else {
-        matrixCoeff(&h->data, h->map, s);
+        if (s->useSubgrid == 0) matrixCoeff(&h->data, h->map, s);
+        else matrixCoeffSub(&h->data, h->map, h->sub, s);
         adjustMatrixBoundary(&h->data, h->bc, h->map, s, 0, 1);
         adjustTidalBoundary(&h->data, h->bc, h->bath, s, h->tt, 0, 1);
         setupMatrix(h->data, h->map, s, A);
@@ -269,6 +312,7 @@ int ref_phase(void *hv, int phase)
     Config *s = h->setting;
     switch (phase) {
     case 0:
+        if (h->tt + 1 > s->Nt) return FREHD_EINVAL;   /* forcing series exhausted */
         h->tt++;
         solveSourceTerm(&h->data, h->sub, h->map, h->bc, h->bath, s, h->tt, 0, 1);
         return 0;
@@ -296,7 +340,8 @@ int ref_phase(void *hv, int phase)
 
 int ref_step(void *hv)
 {
-    for (int p = 0; p < 4; p++) ref_phase(hv, p);
+    if (ref_phase(hv, 0)) return FREHD_EINVAL;
+    for (int p = 1; p < 4; p++) ref_phase(hv, p);
     return ref_phase(hv, 4);
 }
 
@@ -327,12 +372,20 @@ double *ref_field(void *hv, const char *name, int64_t *len)
         {"bottomXP", b->bottomXP, Ng}, {"bottomYP", b->bottomYP, Ng}, {"advX", d->advX, Ng},
         {"advY", d->advY, Ng}, {"rmvCFL", d->rmvCFL, N}, {"GnCt", d->GnCt, N},
         {"GnXP", d->GnXP, N}, {"GnXM", d->GnXM, N}, {"GnYP", d->GnYP, N}, {"GnYM", d->GnYM, N},
-        {"z_transposed", d->z, N}};
+        {"z_transposed", d->z, N},
+        {"V", h->sub->V, Ng}, {"Z", h->sub->Z, Ng}, {"Vx", h->sub->Vx, Ng}, {"Vy", h->sub->Vy, Ng},
+        {"Zx", h->sub->Zx, Ng}, {"Zy", h->sub->Zy, Ng}, {"Nx", h->sub->Nx, Ng}, {"Oy", h->sub->Oy, Ng},
+        {"Vxp", h->sub->Vxp, N}, {"Vxm", h->sub->Vxm, N}, {"Vyp", h->sub->Vyp, N},
+        {"Vym", h->sub->Vym, N}, {"Np", h->sub->Np, N}, {"Nm", h->sub->Nm, N},
+        {"Op", h->sub->Op, N}, {"Om", h->sub->Om, N}, {"minVx", h->sub->minVx, Ng},
+        {"minVy", h->sub->minVy, Ng}};
     for (size_t q = 0; q < sizeof(tab) / sizeof(tab[0]); q++)
-        if (strcmp(tab[q].n, name) == 0) { if (len) *len = tab[q].l; return tab[q].p; }
+        if (strcmp(tab[q].n, name) == 0 && tab[q].p) { if (len) *len = tab[q].l; return tab[q].p; }
     if (len) *len = 0;
     return NULL;
 }
+
+int *ref_ind(void *hv) { return ((ref_handle *)hv)->sub->ind; }
 
 int ref_tt(void *hv) { return ((ref_handle *)hv)->tt; }
 int ref_pcg_iters(void *hv) { return ((ref_handle *)hv)->lastIter; }
