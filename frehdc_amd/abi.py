@@ -144,7 +144,8 @@ def load_library(path: str | None = None):
         "frehd_dev_create": ([C.POINTER(FrehdConfig), C.POINTER(FrehdForcing),
                               C.POINTER(FrehdIC), C.POINTER(V)], C.c_int),
         "frehd_dev_create_from_state": ([C.POINTER(FrehdConfig), C.POINTER(FrehdForcing),
-                                         C.POINTER(FrehdFields), C.POINTER(V)], C.c_int),
+                                         C.POINTER(FrehdSubgrid), C.POINTER(FrehdFields),
+                                         C.POINTER(V)], C.c_int),
         "frehd_dev_step": ([V, C.c_int32], C.c_int),
         "frehd_dev_download": ([V, C.POINTER(FrehdFields)], C.c_int),
         "frehd_dev_stats": ([V, C.POINTER(FrehdStats)], C.c_int),

@@ -334,4 +334,6 @@ def marsh_subgrid(n=48, r=8, dx=2.0, dt=0.5, Nt=400, amp=0.6, period=TIDE_PERIOD
     if not manning_cell:
         sc.manningCell = None
     fine = fine_marsh(n, r, seed=seed)
-    return with_subgrid(sc, fine, r, -1.0, 1.5, dsurf)
+    sc = with_subgrid(sc, fine, r, -1.0, 1.5, dsurf)
+    sc.name = f"marsh{n}_sub{r}"
+    return sc

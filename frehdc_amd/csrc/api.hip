@@ -60,6 +60,7 @@ struct HG {
 
 void host_init(HostState &h, const frehd_config &k, const frehd_forcing &f, const frehd_ic &ic)
 {
+    const bool sub = k.useSubgrid == 1;
     const int nx = k.nx, ny = k.ny, N = nx * ny, Ng = (nx + 2) * (ny + 2);
     const int ns = std::max(1, k.nScalar);
     h.nx = nx; h.ny = ny; h.N = N; h.Ng = Ng; h.ns = ns;
@@ -77,10 +78,18 @@ void host_init(HostState &h, const frehd_config &k, const frehd_forcing &f, cons
     for (int c = 0; c < N; c++) zb[c] = ic.bottomZ[c] + k.offset;
     for (int i = 0; i < nx; i++) { zb[gjM(g, i)] = zb[i]; zb[gjP(g, i)] = zb[N - nx + i]; }
     for (int j = 0; j < ny; j++) { zb[giM(g, j)] = zb[j * nx]; zb[giP(g, j)] = zb[j * nx + nx - 1]; }
-    for (int c = 0; c < N; c++) { double a = zb[c], b = zb[G.E(c)]; bx[c] = a > b ? a : b; }
+    // face bottoms: max of the two cells, or the subgrid edges (updateAllFaceElev
+    // takes edgeX/edgeY when useSubgrid != 0, nsfunctions.c:84-96)
+    for (int c = 0; c < N; c++) {
+        if (sub) bx[c] = ic.sub->edgeX[c] + k.offset;
+        else { double a = zb[c], b = zb[G.E(c)]; bx[c] = a > b ? a : b; }
+    }
     for (int j = 0; j < ny; j++) { bx[giM(g, j)] = zb[j * nx]; bx[giP(g, j)] = bx[j * nx + nx - 1]; }
     for (int i = 0; i < nx; i++) { bx[gjM(g, i)] = bx[i]; bx[gjP(g, i)] = bx[N - nx + i]; }
-    for (int c = 0; c < N; c++) { double a = zb[c], b = zb[G.Nn(c)]; by[c] = a > b ? a : b; }
+    for (int c = 0; c < N; c++) {
+        if (sub) by[c] = ic.sub->edgeY[c] + k.offset;
+        else { double a = zb[c], b = zb[G.Nn(c)]; by[c] = a > b ? a : b; }
+    }
     for (int i = 0; i < nx; i++) {
         double a = zb[gjM(g, i)], b = zb[i];
         by[gjM(g, i)] = a > b ? a : b;
@@ -216,6 +225,10 @@ struct Carve {
     }
 };
 
+// blocks of the grid-stride solver kernels: enough to fill the chip, few
+// enough that every block can re-reduce all block partials cheaply
+inline size_t pcg_blocks(size_t N) { return std::min<size_t>(1024, (N + 255) / 256); }
+
 // lay out every device array; with base == nullptr only the size is computed
 size_t layout(frehd_dev *d, char *base)
 {
@@ -241,7 +254,15 @@ size_t layout(frehd_dev *d, char *base)
     a.flags = cv.take<uint8_t>(N);
     const size_t nb2d = (size_t)((k.nx + 63) / 64) * ((k.ny + 3) / 4);
     const size_t nb1d = (Ng + 255) / 256;
-    a.partials = cv.take<double>(2 * std::max(nb2d, nb1d) + 64);
+    const size_t G = pcg_blocks(N);
+    const size_t maxb = std::max(std::max(nb2d, nb1d), G);
+    a.maxBlocks = (int)maxb;
+    a.nb1 = (int)nb1d;
+    a.nb2 = (int)nb2d;
+    a.partA = cv.take<double>(2 * G);
+    a.partB = cv.take<double>(G);
+    a.vpart = cv.take<double>(4 * maxb);
+    a.cflpart = cv.take<double>(2 * maxb);
     a.tideP = cv.take<double>(Nt1);
     a.tideM = cv.take<double>(Nt1);
     a.inflow = cv.take<double>(Nt1);
@@ -249,6 +270,17 @@ size_t layout(frehd_dev *d, char *base)
     a.tidalMS = cv.take<double>(Nt1);
     a.tideLocP = cv.take<int>(std::max(1, k.tideLocLengthP));
     a.tideLocM = cv.take<int>(std::max(1, k.tideLocLengthM));
+    if (k.useSubgrid == 1) {
+        double **ngs[] = {&a.V, &a.Z, &a.Vx, &a.Vy, &a.Zx, &a.Zy, &a.Nx, &a.Oy};
+        for (auto **p : ngs) *p = cv.take<double>(Ng);
+        double **ns2[] = {&a.Vxp, &a.Vxm, &a.Vyp, &a.Vym, &a.Np, &a.Nm, &a.Op, &a.Om};
+        for (auto **p : ns2) *p = cv.take<double>(N);
+        a.ind = cv.take<int>(Ng);
+        a.edgeX = cv.take<double>(N);
+        a.edgeY = cv.take<double>(N);
+        a.allSurf = cv.take<double>((size_t)k.Ne);
+        for (int t = 0; t < 10; t++) a.tab[t] = cv.take<double>((size_t)k.Ne * N);
+    }
     return cv.off + 256;
 }
 
@@ -280,6 +312,9 @@ int check_config(const frehd_config *k)
     if (k->tideLocLengthP < 0 || k->tideLocLengthM < 0 || k->inflowLocLength < 0)
         return fail(FREHD_EINVAL, "negative location count");
     if (k->useScalar == 1 && k->nScalar > 16) return fail(FREHD_EINVAL, "at most 16 scalars");
+    if (k->useSubgrid != 0 && k->useSubgrid != 1) return fail(FREHD_EINVAL, "useSubgrid must be 0 or 1");
+    if (k->useSubgrid == 1 && !(k->Ne >= 1 && k->dsurf > 0 && k->subR > 0))
+        return fail(FREHD_EINVAL, "subgrid needs Ne >= 1, dsurf > 0, subR > 0");
     return 0;
 }
 
@@ -318,6 +353,7 @@ void fill_params(P &p, const frehd_config &k)
     p.inflowLocLength = k.inflowLocLength; p.useScalar = k.useScalar;
     p.nScalar = k.useScalar == 1 ? std::max(1, k.nScalar) : 0;
     p.useConstTidePS = k.useConstTidePS; p.useConstTideMS = k.useConstTideMS; p.maxIter = k.maxIter;
+    p.useSubgrid = k.useSubgrid; p.dsurf = k.dsurf; p.subR = k.subR; p.Ne = k.Ne;
 }
 
 int upload(frehd_dev *d, const void *host, void *dev, size_t bytes)
@@ -421,18 +457,51 @@ int upload_state(frehd_dev *d, const frehd_fields &s)
         if (s.S) { int rc = upload(d, s.S + (size_t)q * Ng, a.S + (size_t)q * a.sStride, Ng * sizeof(double)); if (rc) return rc; }
         if (s.Sm) { int rc = upload(d, s.Sm + (size_t)q * N, a.Sm + (size_t)q * a.sStride, N * sizeof(double)); if (rc) return rc; }
     }
+    if (a.p.useSubgrid == 1) {
+        struct { const double *src; double *dst; } sb[] = {{s.V, a.V}, {s.Z, a.Z}, {s.Vx, a.Vx},
+            {s.Vy, a.Vy}, {s.Zx, a.Zx}, {s.Zy, a.Zy}, {s.Nx, a.Nx}, {s.Oy, a.Oy}};
+        for (auto &e : sb) {
+            int rc = upload(d, e.src, e.dst, Ng * sizeof(double));
+            if (rc) return rc;
+        }
+        if (s.ind) { int rc = upload(d, s.ind, a.ind, Ng * sizeof(int)); if (rc) return rc; }
+    }
     d->parity = 0;
     set_depth_pointers(d);
     HIPCHK(hipStreamSynchronize(d->stream));
     return 0;
 }
 
+// subgrid inputs: face bottoms (+ offset), level elevations, level tables
+int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
+{
+    const frehd_config &k = d->cfg;
+    StepArgs &a = d->args;
+    const size_t N = d->g.N, tn = (size_t)k.Ne * N;
+    if (!sg || !sg->edgeX || !sg->edgeY) return fail(FREHD_EINVAL, "subgrid tables missing");
+    const double *tabs[10] = {sg->V, sg->Z, sg->Vxp, sg->Vxm, sg->Vyp, sg->Vym, sg->Np, sg->Nm, sg->Op, sg->Om};
+    for (int t = 0; t < 10; t++)
+        if (!tabs[t]) return fail(FREHD_EINVAL, "subgrid table %d missing", t);
+    std::vector<double> ex(N), ey(N), lev(k.Ne);
+    for (size_t c = 0; c < N; c++) { ex[c] = sg->edgeX[c] + k.offset; ey[c] = sg->edgeY[c] + k.offset; }
+    for (int q = 0; q < k.Ne; q++) lev[q] = k.surfmin + q * k.dsurf + k.offset;
+    int rc;
+    if ((rc = upload(d, ex.data(), (void *)a.edgeX, N * sizeof(double)))) return rc;
+    if ((rc = upload(d, ey.data(), (void *)a.edgeY, N * sizeof(double)))) return rc;
+    if ((rc = upload(d, lev.data(), (void *)a.allSurf, k.Ne * sizeof(double)))) return rc;
+    for (int t = 0; t < 10; t++)
+        if ((rc = upload(d, tabs[t], (void *)a.tab[t], tn * sizeof(double)))) return rc;
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
 // ---- kernel launches with optional per-kernel event timing ----------------
 enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_FLUX, K_TIDEV,
-                K_FINISH, K_SCALAR, K_SCALARBC, K_END, K_COUNT };
+                K_FINISH, K_SCALAR, K_SCALARBC, K_END, K_COMBINE, K_SUBGHOST, K_SUBINIT, K_COUNT };
 const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble", "k_pcg_pq",
     "k_pcg_update", "k_surface", "k_limiter", "k_faces", "k_flux", "k_tidal_velocity",
-    "k_finish", "k_scalar", "k_scalar_bc", "k_step_end"};
+    "k_finish", "k_scalar", "k_scalar_bc", "k_step_end", "k_combine", "k_sub_ghosts",
+    "k_subarea_init"};
 
 hipEvent_t next_event(frehd_dev *d)
 {
@@ -487,8 +556,9 @@ int enqueue_pcg_iteration(frehd_dev *d, int it)
     a.pNew = (it & 1) ? a.pA : d->pB;
     const Grid &g = d->g;
     dim3 b2(64, 4), g2((g.nx + 63) / 64, (g.ny + 3) / 4);
-    LAUNCH(K_PQ, k_pcg_pq, g2, b2, a);
-    LAUNCH(K_UPD, k_pcg_update, dim3((g.N + 255) / 256), dim3(256), a);
+    const dim3 gP((unsigned)pcg_blocks(g.N)), bP(256);
+    LAUNCH(K_PQ, k_pcg_pq, gP, bP, a, it);
+    LAUNCH(K_UPD, k_pcg_update, gP, bP, a, it);
     return 0;
 }
 
@@ -509,13 +579,17 @@ int enqueue_phase(frehd_dev *d, int phase)
     dim3 b1(256), gNg((g.Ng + 255) / 256);
     double *cur = d->parity ? d->depthB : d->depthA;
     double *nxt = d->parity ? d->depthA : d->depthB;
+    const bool sub = a.p.useSubgrid == 1;
     switch (phase) {
     case 0:   // solveSourceTerm (up to the RHS, which the assembly computes)
         LAUNCH(K_BEGIN, k_step_begin, dim3(1), dim3(64), a);
-        LAUNCH(K_PRED, k_predictor, g2, b2, a);
+        if (sub) LAUNCH(K_PRED, k_predictor_sub, g2, b2, a);
+        else LAUNCH(K_PRED, k_predictor, g2, b2, a);
         return 0;
     case 1: { // solveFreeSurface
-        LAUNCH(K_ASM, k_assemble, g2, b2, a);
+        a.depthOld = cur;   // matrixCoeffSub tests the previous step's depth
+        if (sub) LAUNCH(K_ASM, k_assemble_sub, dim3((unsigned)pcg_blocks(g.N)), dim3(256), a);
+        else LAUNCH(K_ASM, k_assemble, dim3((unsigned)pcg_blocks(g.N)), dim3(256), a);
         // PCG: launch the previous step's iteration count, then small chunks,
         // until the device reports convergence (or halt); one iteration per
         // chunk while timing, so no idle launch enters the averages
@@ -539,12 +613,23 @@ int enqueue_phase(frehd_dev *d, int phase)
     case 2: { // updateData (+ the CFL numbers of monitorCFL)
         a.depthOld = cur;
         a.depthNew = nxt;
-        LAUNCH(K_LIM, k_limiter, g2, b2, a);
-        LAUNCH(K_FACES, k_faces, gNg, b1, a);
-        LAUNCH(K_FLUX, k_flux, gNg, b1, a);
         int nt = (a.p.bcType != 2 ? a.p.tideLocLengthP : 0) + (a.p.bcType != 1 ? a.p.tideLocLengthM : 0);
-        if (nt > 0) LAUNCH(K_TIDEV, k_tidal_velocity, dim3((nt + 255) / 256), b1, a);
-        LAUNCH(K_FINISH, k_finish, gNg, b1, a);
+        if (sub) {
+            const dim3 gN((g.N + 255) / 256), gGh((2 * g.nx + 2 * g.ny + 255) / 256);
+            LAUNCH(K_LIM, k_limiter_sub, g2, b2, a);
+            LAUNCH(K_COMBINE, k_combine, gN, b1, a);
+            LAUNCH(K_SUBGHOST, k_sub_ghosts, gGh, b1, a);
+            LAUNCH(K_FACES, k_faces_sub, gNg, b1, a);
+            LAUNCH(K_FLUX, k_flux_sub, gNg, b1, a);
+            if (nt > 0) LAUNCH(K_TIDEV, k_tidal_velocity_sub, dim3((nt + 255) / 256), b1, a);
+            LAUNCH(K_FINISH, k_finish_sub, gNg, b1, a);
+        } else {
+            LAUNCH(K_LIM, k_limiter, g2, b2, a);
+            LAUNCH(K_FACES, k_faces, gNg, b1, a);
+            LAUNCH(K_FLUX, k_flux, gNg, b1, a);
+            if (nt > 0) LAUNCH(K_TIDEV, k_tidal_velocity, dim3((nt + 255) / 256), b1, a);
+            LAUNCH(K_FINISH, k_finish, gNg, b1, a);
+        }
         d->parity ^= 1;
         return 0;
     }
@@ -552,16 +637,35 @@ int enqueue_phase(frehd_dev *d, int phase)
         a.depthNew = cur;
         if (a.p.useScalar == 1)
             for (int q = 0; q < a.p.nScalar; q++) {
-                LAUNCH(K_SCALAR, k_scalar, g2, b2, a, q);
+                if (sub) LAUNCH(K_SCALAR, k_scalar_sub, g2, b2, a, q);
+                else LAUNCH(K_SCALAR, k_scalar, g2, b2, a, q);
                 LAUNCH(K_SCALARBC, k_scalar_bc, gNg, b1, a, q);
             }
         return 0;
     case 4:   // monitorCFL's stop test
-        LAUNCH(K_END, k_step_end, dim3(1), dim3(64), a);
+        LAUNCH(K_END, k_step_end, dim3(1), dim3(256), a);
         return 0;
     default:
         return fail(FREHD_EINVAL, "phase %d out of range", phase);
     }
+}
+
+// Init's subgrid part (initialize.c:103-124): level index at round(Ne/2)
+// (integer division first, subgrid.c:2272), computeSubArea, combineSubArea
+int init_subgrid_device(frehd_dev *d)
+{
+    const Grid &g = d->g;
+    StepArgs &a = d->args;
+    std::vector<int> ind(g.Ng, d->cfg.Ne / 2);
+    int rc = upload(d, ind.data(), a.ind, ind.size() * sizeof(int));
+    if (rc) return rc;
+    a.depthOld = d->parity ? d->depthB : d->depthA;
+    const dim3 b1(256), gN((g.N + 255) / 256), gGh((2 * g.nx + 2 * g.ny + 255) / 256);
+    LAUNCH(K_SUBINIT, k_subarea_init, gN, b1, a);
+    LAUNCH(K_COMBINE, k_combine, gN, b1, a);
+    LAUNCH(K_SUBGHOST, k_sub_ghosts, gGh, b1, a);
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return 0;
 }
 
 int enqueue_step(frehd_dev *d)
@@ -580,7 +684,7 @@ extern "C" {
 
 const char *frehd_last_error(void) { return g_err.c_str(); }
 
-const char *frehd_version(void) { return "frehd-mi355x 0.1 (gfx950, FP64 plain-grid step, Jacobi PCG)"; }
+const char *frehd_version(void) { return "frehd-mi355x 0.2 (gfx950, FP64 plain and subgrid step, Jacobi PCG)"; }
 
 int frehd_device_count(int32_t *count)
 {
@@ -625,6 +729,8 @@ int frehd_dev_create(const frehd_config *cfg, const frehd_forcing *f, const freh
     int rc = check_config(cfg);
     if (rc) return rc;
     if (!f) return fail(FREHD_EINVAL, "forcing is NULL");
+    if (cfg->useSubgrid == 1 && !(ic->sub && ic->sub->edgeX && ic->sub->edgeY))
+        return fail(FREHD_EINVAL, "subgrid inputs (ic->sub) missing");
     HostState h;
     host_init(h, *cfg, *f, *ic);
     frehd_dev *d = nullptr;
@@ -645,14 +751,26 @@ int frehd_dev_create(const frehd_config *cfg, const frehd_forcing *f, const freh
     s.S = h.S.data();
     s.Sm = Sm.data();
     if ((rc = upload_state(d, s))) { frehd_dev_destroy(d); return rc; }
+    if (cfg->useSubgrid == 1) {
+        if ((rc = upload_subgrid(d, ic->sub)) || (rc = init_subgrid_device(d))) {
+            frehd_dev_destroy(d);
+            return rc;
+        }
+    }
     *out = d;
     return 0;
 }
 
-int frehd_dev_create_from_state(const frehd_config *cfg, const frehd_forcing *f, const frehd_fields *state,
-                                frehd_dev_t **out)
+int frehd_dev_create_from_state(const frehd_config *cfg, const frehd_forcing *f, const frehd_subgrid *sub,
+                                const frehd_fields *state, frehd_dev_t **out)
 {
     if (!state) return fail(FREHD_EINVAL, "state is NULL");
+    if (cfg && cfg->useSubgrid == 1) {
+        const double *sreq[] = {state->V, state->Z, state->Vx, state->Vy, state->Zx, state->Zy,
+                                state->Nx, state->Oy};
+        for (auto *p : sreq) if (!p) return fail(FREHD_EINVAL, "subgrid state array missing");
+        if (!state->ind) return fail(FREHD_EINVAL, "subgrid level index missing");
+    }
     const double *req[] = {state->surf, state->surfOld, state->uuXP, state->vvYP, state->uuYP, state->vvXP,
         state->depth, state->depthXP, state->depthYP, state->CDXP, state->CDYP, state->cellV,
         state->CFLx, state->CFLy, state->bottomZ, state->bottomXP, state->bottomYP};
@@ -662,6 +780,7 @@ int frehd_dev_create_from_state(const frehd_config *cfg, const frehd_forcing *f,
     int rc = create_common(cfg, f, out, &d);
     if (rc) return rc;
     if ((rc = upload_state(d, *state))) { frehd_dev_destroy(d); return rc; }
+    if (cfg->useSubgrid == 1 && (rc = upload_subgrid(d, sub))) { frehd_dev_destroy(d); return rc; }
     *out = d;
     return 0;
 }
@@ -724,6 +843,13 @@ int frehd_dev_download(frehd_dev_t *d, frehd_fields *o)
         {o->bottomXP, a.bottomXP, Ng}, {o->bottomYP, a.bottomYP, Ng}};
     for (auto &e : tab)
         if (e.dst) HIPCHK(hipMemcpy(e.dst, e.src, e.n * sizeof(double), hipMemcpyDeviceToHost));
+    if (a.p.useSubgrid == 1) {
+        struct { double *dst; const double *src; } sb[] = {{o->V, a.V}, {o->Z, a.Z}, {o->Vx, a.Vx},
+            {o->Vy, a.Vy}, {o->Zx, a.Zx}, {o->Zy, a.Zy}, {o->Nx, a.Nx}, {o->Oy, a.Oy}};
+        for (auto &e : sb)
+            if (e.dst) HIPCHK(hipMemcpy(e.dst, e.src, Ng * sizeof(double), hipMemcpyDeviceToHost));
+        if (o->ind) HIPCHK(hipMemcpy(o->ind, a.ind, Ng * sizeof(int), hipMemcpyDeviceToHost));
+    }
     const int ns = std::max(1, a.p.nScalar);
     for (int q = 0; q < ns; q++) {
         if (o->S) HIPCHK(hipMemcpy(o->S + (size_t)q * Ng, a.S + (size_t)q * a.sStride, Ng * sizeof(double), hipMemcpyDeviceToHost));

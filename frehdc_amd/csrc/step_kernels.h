@@ -21,7 +21,8 @@ struct P {
 };
 
 struct PcgScalars {
-    double bb, bNorm, rr, rNorm, rz, pq, alpha, beta;
+    double bb, bNorm, rr, rNorm, pq, alpha;
+    double rhoHist[2];        // r.z of iterations of even / odd index
     int iter, done;
 };
 
@@ -30,8 +31,7 @@ struct Ctl {
     int tt;                   // current step number
     int halted;               // set when max CFL > 20
     int haltStep;
-    unsigned counter;         // last-block counter of the reductions
-    unsigned long long maxCFLxBits, maxCFLyBits;
+    int pad;
     double lastMaxCFLx, lastMaxCFLy;
     double Vloss[4];
     PcgScalars pcg;
@@ -55,7 +55,12 @@ struct StepArgs {
     double *uuYP, *vvXP, *depthOld, *depthNew, *cellV, *Snew, *manning;
     uint8_t *rmv, *flags;     // flags: bit0 tideP cell, bit1 tideM cell, bits 2-7 inflow count
     // operator and solver vectors
-    double *cX, *cY, *D, *b, *x, *r, *q, *pA, *pOld, *pNew, *partials;
+    double *cX, *cY, *D, *b, *x, *r, *q, *pA, *pOld, *pNew;
+    // per-block partial sums: solver (partA: r.r, r.z; partB: p.q), volume
+    // losses (vpart: 4 slots) and CFL maxima (cflpart: 2 slots), each slot
+    // maxBlocks long; nb1 = blocks of an Ng sweep, nb2 = blocks of a 2-D sweep
+    double *partA, *partB, *vpart, *cflpart;
+    int maxBlocks, nb1, nb2, pad3;
     // forcing (Nt+1 samples) and locations
     const double *tideP, *tideM, *inflow, *tidalPS, *tidalMS;
     const int *tideLocP, *tideLocM;
@@ -74,8 +79,8 @@ enum SubTable { T_V, T_Z, T_VXP, T_VXM, T_VYP, T_VYM, T_NP, T_NM, T_OP, T_OM };
 __global__ void k_step_begin(StepArgs a);
 __global__ void k_predictor(StepArgs a);
 __global__ void k_assemble(StepArgs a);
-__global__ void k_pcg_pq(StepArgs a);
-__global__ void k_pcg_update(StepArgs a);
+__global__ void k_pcg_pq(StepArgs a, int it);
+__global__ void k_pcg_update(StepArgs a, int it);
 __global__ void k_surface(StepArgs a);
 __global__ void k_limiter(StepArgs a);
 __global__ void k_faces(StepArgs a);

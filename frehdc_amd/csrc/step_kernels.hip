@@ -26,6 +26,7 @@ __global__ void k_step_begin(StepArgs a)
         c->tt += 1;
         c->pcg.iter = 0;
         c->pcg.done = 0;
+        c->pcg.rhoHist[0] = c->pcg.rhoHist[1] = 0.0;
     }
 }
 
@@ -162,51 +163,44 @@ __device__ inline double face_coef(double coef, double depthF, double dragF)
     return coef * depthF / (1 + dragF);
 }
 
-// Stores the block partials of b.b and r.z; the last block to finish
-// reduces every block's partials in block order and starts the PCG
-// scalars (deterministic for a given launch shape).
-__device__ inline void solver_start_reduce(const StepArgs &a, double (&part)[3])
+// ---- reductions without atomics ------------------------------------------
+// Every reduction kernel runs a fixed grid of G blocks (grid-stride over the
+// cells) and writes one partial per block.  The kernel that consumes the sum
+// reduces all G partials itself, in every block, in the same fixed order, so
+// every block obtains bitwise the same value and the result is identical run
+// to run.  (Same-address atomics serialise across the chip, and agent-scope
+// release/acquire fences write back and invalidate the per-XCD L2s; kernel
+// boundaries give the visibility for free.)
+template <int NV>
+__device__ inline void store_partials(double *part, const double (&v)[NV])
 {
-    __shared__ double lds[16 * 3];
-    __shared__ int amLast;
-    const P &p = a.p;
-    block_sum<3>(part, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const int nb = gridDim.x * gridDim.y;
-        const int bid = blockIdx.x + gridDim.x * blockIdx.y;
-        double *pp = a.partials + 2 * bid;
-        __hip_atomic_store(pp + 0, part[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(pp + 1, part[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned prev = __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        amLast = (prev == (unsigned)(nb - 1));
-    }
-    __syncthreads();
-    if (!amLast) return;
-    const int nb = gridDim.x * gridDim.y;
-    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-    const int nth = blockDim.x * blockDim.y;
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int q = tid; q < nb; q += nth) {
-        acc[0] += __hip_atomic_load(a.partials + 2 * q + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        acc[1] += __hip_atomic_load(a.partials + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<3>(acc, lds);
-    if (tid == 0) {
-        PcgScalars &sc = a.ctl->pcg;
-        sc.bb = acc[0];
-        sc.rr = acc[0];
-        sc.rz = acc[1];
-        sc.iter = 0;
-        const double bNorm = sqrt(sc.bb), rNorm = bNorm;
-        sc.bNorm = bNorm;
-        sc.rNorm = rNorm;
-        sc.done = (rNorm < p.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm))) ? 1 : 0;
-        if (p.maxIter <= 0) sc.done = 1;
-        a.ctl->counter = 0;
-    }
+    // v already block-reduced into thread 0
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
 }
 
-// stores the operator row and the solver start of cell c
+template <int NV>
+__device__ inline void reduce_partials(const double *part, int nparts, double (&out)[NV], double *lds)
+{
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) acc[q] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NV; q++) acc[q] += part[(size_t)q * nparts + b];
+    block_sum<NV>(acc, lds);
+    __shared__ double bc[NV];
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) bc[q] = acc[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; q++) out[q] = bc[q];
+}
+
+// stores the operator row and the solver start of cell c, accumulating b.b
+// and r.z (z = r / D) of the thread's cells
 __device__ inline void store_row(const StepArgs &a, int c, double GXP, double GYP, double D, double z,
                                  double (&part)[3])
 {
@@ -216,27 +210,25 @@ __device__ inline void store_row(const StepArgs &a, int c, double GXP, double GY
     a.b[c] = z;
     a.r[c] = z;
     a.x[c] = 0.0;
-    a.pA[c] = 0.0;
     const double invd = 1.0 / fabs(D);
-    part[0] = z * z;
-    part[1] = z * (z * invd);
+    part[0] += z * z;
+    part[1] += z * (z * invd);
 }
 
 __global__ void __launch_bounds__(256) k_assemble(StepArgs a)
 {
+    __shared__ double lds[16 * 3];
     if (a.ctl->halted) return;
     const Grid g = a.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const P &p = a.p;
-    double part[3] = {0.0, 0.0, 0.0};   // b.b, r.z, (unused)
-    if (i < g.nx && j < g.ny) {
-        const int c = j * g.nx + i;
+    double part[3] = {0.0, 0.0, 0.0};   // b.b, r.z
+    const int tt = cur_tt(a);
+    const double dt = p.dt;
+    const double coefx = p.g * (dt * dt / (p.dx * p.dx));
+    const double coefy = p.g * (dt * dt / (p.dy * p.dy));
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += gridDim.x * blockDim.x) {
+        const int i = c % g.nx, j = c / g.nx;
         const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
-        const int tt = cur_tt(a);
-        const double dt = p.dt;
-        const double coefx = p.g * (dt * dt / (p.dx * p.dx));
-        const double coefy = p.g * (dt * dt / (p.dy * p.dy));
         const double GXP = face_coef(coefx, a.depthXP[c], a.DragXP[c]);
         const double GXM = face_coef(coefx, a.depthXP[w], a.DragXP[w]);
         const double GYP = face_coef(coefy, a.depthYP[c], a.DragYP[c]);
@@ -253,14 +245,17 @@ __global__ void __launch_bounds__(256) k_assemble(StepArgs a)
         if ((fl & 2) && p.bcType != 1) { z = a.tideM[tt] + p.offset; D = -1.0; }
         store_row(a, c, GXP, GYP, D, z, part);
     }
-    solver_start_reduce(a, part);
+    block_sum<3>(part, lds);
+    store_partials<2>(a.partA, {part[0], part[1]});
 }
 
 // ---------------------------------------------------------------------------
-// PCG iteration, part 1: p = z + beta p (z = r / D, JacobiPrecond
-// precond.c:26-46) and q = A p (Mul_QV row-wise, operats.c:870-888),
-// plus the block partial of p.q.  p is double-buffered (pA -> pB) so the
-// neighbours' new p can be recomputed from their old p.
+// PCG iteration `it`, part 1: the stopping test on the residual of the
+// previous iteration (RTCResult, rtc.c:48-72, checked before every
+// iteration, itersolv.c:407-409), then p = z + beta p (z = r / D,
+// JacobiPrecond precond.c:26-46) and q = A p (Mul_QV row-wise,
+// operats.c:870-888) with the block partials of p.q.  p is double-buffered
+// (pOld -> pNew) so the neighbours' new p can be recomputed from their old p.
 __device__ inline double p_new_at(const StepArgs &a, int k, double beta, int first, int betaOne)
 {
     const double rk = a.r[k];
@@ -270,27 +265,43 @@ __device__ inline double p_new_at(const StepArgs &a, int k, double beta, int fir
     return betaOne ? zk + po : zk + beta * po;
 }
 
-__global__ void __launch_bounds__(256) k_pcg_pq(StepArgs a)
+__global__ void __launch_bounds__(256) k_pcg_pq(StepArgs a, int it)
 {
     __shared__ double lds[16 * 3];
-    __shared__ int amLast;
-    const Ctl *ctl = a.ctl;
+    Ctl *ctl = a.ctl;
     if (ctl->halted || ctl->pcg.done) return;
     const Grid g = a.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int first = ctl->pcg.iter == 0;
-    const double beta = ctl->pcg.beta;
+    const P &p = a.p;
+    // r.r and r.z of the current residual (from the assembly or the last update)
+    double rs[2];
+    reduce_partials<2>(a.partA, gridDim.x, rs, lds);
+    const double rr = rs[0], rz = rs[1];
+    const double rNorm = sqrt(rr);
+    const double bNorm = it == 0 ? rNorm : ctl->pcg.bNorm;
+    const bool stop = rNorm < p.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm)) ||
+                      it >= p.maxIter;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        PcgScalars &sc = ctl->pcg;
+        if (it == 0) { sc.bb = rr; sc.bNorm = bNorm; }
+        sc.rr = rr;
+        sc.rNorm = rNorm;
+        sc.rhoHist[it & 1] = rz;
+        sc.iter = it;
+        if (stop) sc.done = 1;
+    }
+    if (stop) return;
+    const int first = it == 0;
+    const double beta = first ? 0.0 : rz / ctl->pcg.rhoHist[(it - 1) & 1];
     const int betaOne = LAS_IS_ONE(beta);
     double part[3] = {0.0, 0.0, 0.0};
-    if (i < g.nx && j < g.ny) {
-        const int c = j * g.nx + i;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += gridDim.x * blockDim.x) {
+        const int i = c % g.nx, j = c / g.nx;
         const double Dc = a.D[c];
         const double pc = p_new_at(a, c, beta, first, betaOne);
         double sum = 0.0;
         if (Dc < 0) {
-            // identity row: off-diagonal entries are zeros
-            sum = pc;
+            // identity row (tidal Dirichlet or isolated dry cell): zero off-diagonals
+            sum = -Dc * pc;
         } else {
             if (i > 0) sum += (-a.cX[c - 1]) * p_new_at(a, c - 1, beta, first, betaOne);
             if (j > 0) sum += (-a.cY[c - g.nx]) * p_new_at(a, c - g.nx, beta, first, betaOne);
@@ -300,90 +311,46 @@ __global__ void __launch_bounds__(256) k_pcg_pq(StepArgs a)
         }
         a.pNew[c] = pc;
         a.q[c] = sum;
-        part[0] = pc * sum;
+        part[0] += pc * sum;
     }
     block_sum<3>(part, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const int nb = gridDim.x * gridDim.y;
-        const int bid = blockIdx.x + gridDim.x * blockIdx.y;
-        __hip_atomic_store(a.partials + 2 * bid, part[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned prev = __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        amLast = (prev == (unsigned)(nb - 1));
-    }
-    __syncthreads();
-    if (!amLast) return;
-    const int nb = gridDim.x * gridDim.y;
-    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-    const int nth = blockDim.x * blockDim.y;
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int q = tid; q < nb; q += nth)
-        acc[0] += __hip_atomic_load(a.partials + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    block_sum<3>(acc, lds);
-    if (tid == 0) {
-        PcgScalars &sc = a.ctl->pcg;
-        sc.pq = acc[0];
-        sc.alpha = sc.rz / sc.pq;
-        a.ctl->counter = 0;
-    }
+    store_partials<1>(a.partB, {part[0]});
 }
 
-// PCG iteration, part 2: x += alpha p, r -= alpha q (itersolv.c:420-423),
-// then r.r and r.z for the stopping test (rtc.c:48-72) and the next beta.
-__global__ void __launch_bounds__(256) k_pcg_update(StepArgs a)
+// PCG iteration `it`, part 2: alpha = rho / p.q, x += alpha p, r -= alpha q
+// (itersolv.c:418-423, LASpack's IsOne(alpha) shortcut kept), and the block
+// partials of r.r and r.z for the next stopping test and beta.
+__global__ void __launch_bounds__(256) k_pcg_update(StepArgs a, int it)
 {
     __shared__ double lds[16 * 3];
-    __shared__ int amLast;
-    const Ctl *ctl = a.ctl;
+    Ctl *ctl = a.ctl;
     if (ctl->halted || ctl->pcg.done) return;
     const Grid g = a.g;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const double alpha = ctl->pcg.alpha;
+    double pqv[1];
+    reduce_partials<1>(a.partB, gridDim.x, pqv, lds);
+    const double alpha = ctl->pcg.rhoHist[it & 1] / pqv[0];
     const int alphaOne = LAS_IS_ONE(alpha);
     double part[3] = {0.0, 0.0, 0.0};
-    if (k < g.N) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < g.N; k += gridDim.x * blockDim.x) {
         const double pk = a.pNew[k], qk = a.q[k];
         double xk = a.x[k], rk = a.r[k];
         if (alphaOne) { xk += pk; rk -= qk; }
         else { xk += alpha * pk; rk -= alpha * qk; }
         a.x[k] = xk;
         a.r[k] = rk;
-        part[0] = rk * rk;
-        part[1] = rk * (rk * (1.0 / fabs(a.D[k])));
+        part[0] += rk * rk;
+        part[1] += rk * (rk * (1.0 / fabs(a.D[k])));
     }
     block_sum<3>(part, lds);
-    if (threadIdx.x == 0) {
-        const int nb = gridDim.x;
-        __hip_atomic_store(a.partials + 2 * blockIdx.x, part[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(a.partials + 2 * blockIdx.x + 1, part[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned prev = __hip_atomic_fetch_add(&a.ctl->counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        amLast = (prev == (unsigned)(nb - 1));
-    }
-    __syncthreads();
-    if (!amLast) return;
-    const int nb = gridDim.x;
-    const int tid = threadIdx.x;
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int q = tid; q < nb; q += blockDim.x) {
-        acc[0] += __hip_atomic_load(a.partials + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        acc[1] += __hip_atomic_load(a.partials + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<3>(acc, lds);
-    if (tid == 0) {
-        PcgScalars &sc = a.ctl->pcg;
-        const P &p = a.p;
-        sc.iter += 1;
-        sc.rr = acc[0];
-        const double rzNew = acc[1];
-        sc.beta = rzNew / sc.rz;
-        sc.rz = rzNew;
-        const double rNorm = sqrt(sc.rr);
-        sc.rNorm = rNorm;
-        const double bNorm = sc.bNorm;
-        if (rNorm < p.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm))) sc.done = 1;
-        else if (sc.iter >= p.maxIter) sc.done = 1;
-        a.ctl->counter = 0;
+    store_partials<2>(a.partA, {part[0], part[1]});
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->pcg.pq = pqv[0];
+        ctl->pcg.alpha = alpha;
+        ctl->pcg.iter = it + 1;
     }
 }
+
+__device__ inline void surface_cell(const StepArgs &a, int k, double &loss);
 
 // ---------------------------------------------------------------------------
 // Surface update: getFreeSurface + zeroNegSurf + enforceFreeSurfaceBC +
@@ -392,10 +359,19 @@ __global__ void __launch_bounds__(256) k_pcg_update(StepArgs a)
 // cell it mirrors (the tidal override comes after the ghost fill).
 __global__ void __launch_bounds__(256) k_surface(StepArgs a)
 {
+    __shared__ double lds[16];
     if (a.ctl->halted) return;
     const Grid g = a.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= g.Ng) return;
+    double loss[1] = {0.0};
+    if (k < g.Ng) surface_cell(a, k, loss[0]);
+    block_sum<1>(loss, lds);
+    if (threadIdx.x == 0) a.vpart[2 * a.maxBlocks + blockIdx.x] = loss[0];
+}
+
+__device__ inline void surface_cell(const StepArgs &a, int k, double &loss)
+{
+    const Grid &g = a.g;
     const P &p = a.p;
     const int tt = cur_tt(a);
     const double old = a.surf[k];
@@ -407,7 +383,7 @@ __global__ void __launch_bounds__(256) k_surface(StepArgs a)
     if (h < zb) {
         const double diff = zb - h;
         h = zb;
-        if (src < 0 && tt > 1) atomicAdd(&a.ctl->Vloss[2], -(diff * p.dx * p.dy));
+        if (src < 0 && tt > 1) loss -= diff * p.dx * p.dy;
     }
     if (src < 0) {
         const uint8_t fl = a.flags[k];
@@ -423,20 +399,17 @@ __global__ void __launch_bounds__(256) k_surface(StepArgs a)
 // and reads surfOld and depth (old) of its neighbours, so it is per-cell; the
 // new depth goes to a second buffer because neighbours read the old one.
 // depth at a ghost index is 0 (the reference reads past the N-array).
-__global__ void __launch_bounds__(256) k_limiter(StepArgs a)
+// limiterCFL for cell c (both sweeps); returns the new surface, sets rmv and
+// adds the Vloss1/Vloss2 contributions
+__device__ inline double limiter_cell(const StepArgs &a, int c, int i, int j, double (&loss)[2], uint8_t &rmv)
 {
-    if (a.ctl->halted) return;
-    const Grid g = a.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= g.nx || j >= g.ny) return;
+    const Grid &g = a.g;
     const P &p = a.p;
-    const int c = j * g.nx + i;
     const double *dOld = a.depthOld, *ho = a.surfOld;
     const double zb = a.bottomZ[c];
     double h = a.surf[c];
     double diff = h - zb;
-    uint8_t rmv = 0;
+    rmv = 0;
     if (dOld[c] <= 0 && diff > 0) {
         const double sref = ho[c] + p.minDepth;
         const int e = nbE(g, c, i, j), w = nbW(g, c, i, j), n = nbN(g, c, i, j), s = nbS(g, c, i, j);
@@ -445,27 +418,55 @@ __global__ void __launch_bounds__(256) k_limiter(StepArgs a)
         const bool dN = ho[n] <= sref || (n < g.N ? dOld[n] : 0.0) <= 0;
         const bool dS = ho[s] <= sref || (s < g.N ? dOld[s] : 0.0) <= 0;
         if (dE && dW && dN && dS) {
-            if ((double)c < (double)g.N - p.dx) atomicAdd(&a.ctl->Vloss[0], diff * p.dx * p.dy);
+            if ((double)c < (double)g.N - p.dx) loss[0] += diff * p.dx * p.dy;
             h = zb;
             rmv = 1;
         }
     }
     diff = h - zb;
     if (diff <= p.minDepth && diff > 0) {
-        atomicAdd(&a.ctl->Vloss[1], diff * p.dx * p.dy);
+        loss[1] += diff * p.dx * p.dy;
         h = zb;
         rmv = 0;
     }
-    a.surf[c] = h;
-    a.rmv[c] = rmv;
-    // computeVolume: depth from the previous step
-    const double cellV = p.dx * p.dy * dOld[c];
-    a.cellV[c] = cellV;
-    for (int q = 0; q < p.nScalar; q++)
-        a.Sm[(size_t)q * a.sStride + c] = a.S[(size_t)q * a.sStride + c] * cellV;
-    // updateDepth
-    const double d = h - zb;
-    a.depthNew[c] = d < 0 ? 0 : d;
+    return h;
+}
+
+__device__ inline void store_vloss2(const StepArgs &a, double (&loss)[2], double *lds)
+{
+    block_sum<2>(loss, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const int bid = blockIdx.x + gridDim.x * blockIdx.y;
+        a.vpart[0 * a.maxBlocks + bid] = loss[0];
+        a.vpart[1 * a.maxBlocks + bid] = loss[1];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_limiter(StepArgs a)
+{
+    __shared__ double lds[16 * 2];
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    double loss[2] = {0.0, 0.0};
+    if (i < g.nx && j < g.ny) {
+        const P &p = a.p;
+        const int c = j * g.nx + i;
+        uint8_t rmv;
+        const double h = limiter_cell(a, c, i, j, loss, rmv);
+        a.surf[c] = h;
+        a.rmv[c] = rmv;
+        // computeVolume: depth from the previous step
+        const double cellV = p.dx * p.dy * a.depthOld[c];
+        a.cellV[c] = cellV;
+        for (int q = 0; q < p.nScalar; q++)
+            a.Sm[(size_t)q * a.sStride + c] = a.S[(size_t)q * a.sStride + c] * cellV;
+        // updateDepth
+        const double d = h - a.bottomZ[c];
+        a.depthNew[c] = d < 0 ? 0 : d;
+    }
+    store_vloss2(a, loss, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -534,12 +535,22 @@ __device__ inline double face_depth_y(const StepArgs &a, int k)
     return 0.0;
 }
 
+__device__ inline void faces_cell(const StepArgs &a, int k, double &loss);
+
 __global__ void __launch_bounds__(256) k_faces(StepArgs a)
 {
+    __shared__ double lds[16];
     if (a.ctl->halted) return;
-    const Grid g = a.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= g.Ng) return;
+    double loss[1] = {0.0};
+    if (k < a.g.Ng) faces_cell(a, k, loss[0]);
+    block_sum<1>(loss, lds);
+    if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blockIdx.x] = loss[0];
+}
+
+__device__ inline void faces_cell(const StepArgs &a, int k, double &loss)
+{
+    const Grid &g = a.g;
     const P &p = a.p;
     double dX = face_depth_x(a, k);
     double dY = face_depth_y(a, k);
@@ -584,13 +595,13 @@ __global__ void __launch_bounds__(256) k_faces(StepArgs a)
         const double vel0 = uc;
         uc = fX * 0.7 * sqrt(2 * p.g * dX);
         if (uc > 0.24 * p.dx / p.dt) uc = 0.24 * p.dx / p.dt;
-        atomicAdd(&a.ctl->Vloss[3], -((vel0 - uc) * p.dt * dX * p.dy));
+        loss -= (vel0 - uc) * p.dt * dX * p.dy;
     }
     if (fY != 0) {
         const double vel0 = vc;
         vc = fY * 0.7 * sqrt(2 * p.g * dY);
         if (vc > 0.24 * p.dy / p.dt) vc = 0.24 * p.dy / p.dt;
-        atomicAdd(&a.ctl->Vloss[3], -((vel0 - vc) * p.dt * dY * p.dx));
+        loss -= (vel0 - vc) * p.dt * dY * p.dx;
     }
     a.uuXP[c] = uc;
     a.vvYP[c] = vc;
@@ -706,8 +717,8 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
     const double mx = block_max(cflx, lds);
     const double my = block_max(cfly, lds);
     if (threadIdx.x == 0) {
-        atomicMax(&a.ctl->maxCFLxBits, (unsigned long long)__double_as_longlong(mx));
-        atomicMax(&a.ctl->maxCFLyBits, (unsigned long long)__double_as_longlong(my));
+        a.cflpart[blockIdx.x] = mx;
+        a.cflpart[a.maxBlocks + blockIdx.x] = my;
     }
 }
 
@@ -812,19 +823,35 @@ __global__ void __launch_bounds__(256) k_scalar_bc(StepArgs a, int q)
     S[k] = v;
 }
 
-// Step end: the max CFL test of monitorCFL (nsfunctions.c:1226-1231) halts
-// the handle instead of exiting the process.
-__global__ void k_step_end(StepArgs a)
+// Step end: volume-loss and CFL reductions (block partials in block order),
+// and the max CFL test of monitorCFL (nsfunctions.c:1226-1231), which halts
+// the handle instead of exiting the process.  One block of 256 threads.
+__global__ void __launch_bounds__(256) k_step_end(StepArgs a)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double lds[16 * 4];
     Ctl *c = a.ctl;
     if (c->halted) return;
-    const double mx = __longlong_as_double((long long)c->maxCFLxBits);
-    const double my = __longlong_as_double((long long)c->maxCFLyBits);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < a.nb2; b += blockDim.x) {
+        v[0] += a.vpart[b];
+        v[1] += a.vpart[a.maxBlocks + b];
+    }
+    for (int b = threadIdx.x; b < a.nb1; b += blockDim.x) {
+        v[2] += a.vpart[2 * a.maxBlocks + b];
+        v[3] += a.vpart[3 * a.maxBlocks + b];
+    }
+    block_sum<4>(v, lds);
+    double mx = 0.0, my = 0.0;
+    for (int b = threadIdx.x; b < a.nb1; b += blockDim.x) {
+        mx = fmax(mx, a.cflpart[b]);
+        my = fmax(my, a.cflpart[a.maxBlocks + b]);
+    }
+    mx = block_max(mx, lds);
+    my = block_max(my, lds);
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < 4; q++) c->Vloss[q] += v[q];
     c->lastMaxCFLx = mx;
     c->lastMaxCFLy = my;
-    c->maxCFLxBits = 0;
-    c->maxCFLyBits = 0;
     c->pcgIterLast = c->pcg.iter;
     c->pcgIterTotal += c->pcg.iter;
     c->relResLast = c->pcg.bNorm > 0 ? c->pcg.rNorm / c->pcg.bNorm : 1.0;
@@ -833,6 +860,435 @@ __global__ void k_step_end(StepArgs a)
         c->halted = 1;
         c->haltStep = c->tt;
     }
+}
+
+}  // namespace frehd
+
+// ===========================================================================
+// Subgrid bathymetry (useSubgrid = 1, staggeredV = 1, useminA = 0,
+// useSubDrag = 0): the step kernels that differ from the plain grid.
+// ===========================================================================
+namespace frehd {
+
+// interpSurf, utilities.c:155-162
+__device__ inline double interp_surf(double x, double x1, double y1, double x2, double y2)
+{
+    if (x2 == x1) return y1;
+    return (y2 - y1) * (x - x2) / (x2 - x1) + y2;
+}
+
+// computeSubArea for cell c (subgrid.c:55-251): walk the level index to the
+// surface, interpolate the half-cell volumes and face areas between the two
+// bracketing levels, zero what is dry.  edgeX/edgeY of a ghost neighbour
+// read 0 (the reference reads past its N-array).
+__device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, double h, double depthOld)
+{
+    const Grid &g = a.g;
+    const P &p = a.p;
+    const int N = g.N, Ne = p.Ne;
+    const double dsurf = p.dsurf;
+    const double *as = a.allSurf;
+    int ind = a.ind[c];
+    while (fabs(as[ind] - h) > 0.5 * dsurf + dsurf / 100) {
+        if (h > as[ind]) ind += 1;
+        else ind -= 1;
+        if (ind < 0) { ind = 0; break; }
+        if (ind > Ne - 1) { ind = Ne - 1; break; }
+    }
+    a.ind[c] = ind;
+    const int j1 = ind;
+    const int j2 = j1 == 0 ? 0 : (j1 == Ne - 1 ? Ne - 1 : (h > as[j1] ? j1 + 1 : j1 - 1));
+    const size_t k1 = (size_t)N * j1 + c, k2 = (size_t)N * j2 + c;
+    const double s1 = as[j1], s2 = as[j2];
+    const double *const *t = a.tab;
+    a.V[c] = interp_surf(h, s1, t[T_V][k1], s2, t[T_V][k2]);
+    a.Z[c] = interp_surf(h, s1, t[T_Z][k1], s2, t[T_Z][k2]);
+    double vxp = interp_surf(h, s1, t[T_VXP][k1], s2, t[T_VXP][k2]);
+    double vyp = interp_surf(h, s1, t[T_VYP][k1], s2, t[T_VYP][k2]);
+    double vxm = interp_surf(h, s1, t[T_VXM][k1], s2, t[T_VXM][k2]);
+    double vym = interp_surf(h, s1, t[T_VYM][k1], s2, t[T_VYM][k2]);
+    const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
+    const double wd[4] = {a.edgeX[c], w < N ? a.edgeX[w] : 0.0, a.edgeY[c], s < N ? a.edgeY[s] : 0.0};
+    const int tid[4] = {T_NP, T_NM, T_OP, T_OM};
+    double f[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (wd[q] > s1 && wd[q] < s2) {
+            f[q] = wd[q] > h ? 0.0 : interp_surf(h, wd[q], 0.0, s2, t[tid[q]][k2]);
+        } else {
+            f[q] = interp_surf(h, s1, t[tid[q]][k1], s2, t[tid[q]][k2]);
+        }
+    }
+    if (depthOld == 0) { vxp = 0; vyp = 0; vxm = 0; vym = 0; }
+    if (h - a.bottomXP[c] <= 0) f[0] = 0;
+    if (h - a.bottomYP[c] <= 0) f[2] = 0;
+    if (w < N && h - a.bottomXP[w] <= 0) f[1] = 0;
+    if (s < N && h - a.bottomYP[s] <= 0) f[3] = 0;
+    a.Vxp[c] = vxp; a.Vyp[c] = vyp; a.Vxm[c] = vxm; a.Vym[c] = vym;
+    a.Np[c] = f[0]; a.Nm[c] = f[1]; a.Op[c] = f[2]; a.Om[c] = f[3];
+}
+
+// Init: computeSubArea on the initial surface (initialize.c:107)
+__global__ void __launch_bounds__(256) k_subarea_init(StepArgs a)
+{
+    const Grid g = a.g;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    subarea_cell(a, c, c % g.nx, c / g.nx, a.surf[c], a.depthOld[c]);
+}
+
+// limiterCFL + computeVolumeSub + computeSubArea + updateDepth
+// (nsfunctions.c:452-495, subgrid.c:1215-1224, 55-251, nsfunctions.c:239-247)
+__global__ void __launch_bounds__(256) k_limiter_sub(StepArgs a)
+{
+    __shared__ double lds[16 * 2];
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    double loss[2] = {0.0, 0.0};
+    if (i < g.nx && j < g.ny) {
+        const P &p = a.p;
+        const int c = j * g.nx + i;
+        uint8_t rmv;
+        const double h = limiter_cell(a, c, i, j, loss, rmv);
+        a.surf[c] = h;
+        a.rmv[c] = rmv;
+        const double cellV = a.V[c];   // the previous step's subgrid volume
+        a.cellV[c] = cellV;
+        for (int q = 0; q < p.nScalar; q++)
+            a.Sm[(size_t)q * a.sStride + c] = a.S[(size_t)q * a.sStride + c] * cellV;
+        subarea_cell(a, c, i, j, h, a.depthOld[c]);
+        const double d = h - a.bottomZ[c];
+        a.depthNew[c] = d < 0 ? 0 : d;
+    }
+    store_vloss2(a, loss, lds);
+}
+
+// combineSubArea for the interior (subgrid.c:254-462 and the clip loop
+// :466-478), plus the interior faces the boundary loop closes: Nx of the
+// right column and Oy of the non-tidal top row.  useminV = 1 changes volumes
+// only where Nmin > 0, which stays 0 with useminA = 0.
+__global__ void __launch_bounds__(256) k_combine(StepArgs a)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const P &p = a.p;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const int i = c % g.nx, j = c / g.nx, N = g.N;
+    const int e = nbE(g, c, i, j), n = nbN(g, c, i, j);
+    const double dxdy = p.dx * p.dy;
+    const double Zc = a.Z[c], h = a.surf[c];
+    double Nx, Oy, Vx, Vy, Zx, Zy;
+    if (e < N) { const double A = a.Np[c], B = a.Nm[e]; Nx = A >= B ? A : B; }
+    else Nx = a.Np[c];
+    if (n < N) { const double A = a.Op[c], B = a.Om[n]; Oy = A >= B ? A : B; }
+    else Oy = a.Op[c];
+    if (e < N) {
+        const double vp = a.Vxp[c], vm = a.Vxm[e];
+        if (vp > 0 && vm > 0) Vx = vp + vm;
+        else Vx = 2.0 * (vp + vm);
+        Zx = Zc;
+        if (Vx > 0 && Zx <= 0) Zx = dxdy;
+        if (h < a.bottomXP[c]) { Vx = vm; Zx = 0.5 * Zc; }
+        if (a.surf[e] < a.bottomXP[c]) { Vx = vp; Zx = 0.5 * Zc; }
+    } else {
+        Vx = 2 * a.Vxp[c];
+        Zx = 2 * Zc;
+    }
+    if (n < N) {
+        const double vp = a.Vyp[c], vm = a.Vym[n];
+        if (vp > 0 && vm > 0) Vy = vp + vm;
+        else Vy = 2.0 * (vp + vm);
+        Zy = Zc;
+        if (Vy > 0 && Zy <= 0) Zy = dxdy;
+        if (h < a.bottomYP[c]) { Vy = vm; Zy = 0.5 * Zc; }
+        if (a.surf[n] < a.bottomYP[c]) { Vy = vp; Zy = 0.5 * Zc; }
+    } else {
+        Vy = 2 * a.Vyp[c];
+        Zy = 2 * Zc;
+    }
+    const double Vmin = 0.002, Amin = 0.002, Azmin = 0.002;
+    double V = a.V[c], Z = Zc;
+    if (Vx < Vmin) Vx = 0.0;
+    if (Vy < Vmin) Vy = 0.0;
+    if (V < Vmin) V = 0.0;
+    if (Z < Azmin) Z = 1.0;
+    if (Zx < Amin) Zx = 0.0;
+    if (Zy < Amin) Zy = 0.0;
+    if (Nx < Amin) Nx = 0.0;
+    if (Oy < Amin) Oy = 0.0;
+    if (i == g.nx - 1) Nx = 0;                                    // subgrid.c:486
+    if (j == g.ny - 1 && p.bcType != 2 && !(a.flags[c] & 1)) Oy = 0;   // :508-524
+    a.Nx[c] = Nx; a.Oy[c] = Oy; a.Vx[c] = Vx; a.Vy[c] = Vy;
+    a.Zx[c] = Zx; a.Zy[c] = Zy; a.V[c] = V; a.Z[c] = Z;
+}
+
+// combineSubArea's ghost copies (subgrid.c:480-506) over the 2nx + 2ny
+// ghosts.  The iM/iP ghosts of the top row take Oy before the top-row
+// closure, i.e. the clipped Op of the corner cell.
+__global__ void __launch_bounds__(256) k_sub_ghosts(StepArgs a)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int k = g.N + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.Ng - 4) return;
+    const int src = ghost_source(g, k, false);
+    const int r = k - g.N;
+    a.V[k] = a.V[src]; a.Z[k] = a.Z[src];
+    a.Vx[k] = a.Vx[src]; a.Vy[k] = a.Vy[src];
+    a.Zx[k] = a.Zx[src]; a.Zy[k] = a.Zy[src];
+    if (r < 2 * g.nx) {                      // jP / jM ghosts
+        a.Nx[k] = a.Nx[src];
+        a.Oy[k] = r < g.nx ? a.Oy[src] : 0.0;
+    } else {                                 // iP / iM ghosts
+        const int jj = (r - 2 * g.nx) % g.ny;
+        a.Nx[k] = 0.0;
+        double oy = a.Oy[src];
+        if (jj == g.ny - 1) { oy = a.Op[src]; if (oy < 0.002) oy = 0.0; }
+        a.Oy[k] = oy;
+    }
+}
+
+// updateAllFaceDepth (plain, nssolve.c:230) + detectWaterfallLocationSub +
+// updateVelocitySub + waterfallCorrectionSub (subgrid.c:1335-1444, 1084-1187)
+// for the faces of every Ng index.  Interior flags are recomputed every step
+// (the Sub variant resets them before detecting); the ghost flags written by
+// the XM/YM tests of boundary cells stay set, as in the reference.
+__device__ inline void faces_sub_cell(const StepArgs &a, int k, double &loss)
+{
+    const Grid &g = a.g;
+    const P &p = a.p;
+    double dX = face_depth_x(a, k);
+    double dY = face_depth_y(a, k);
+    if (dX < 0) dX = 0;
+    if (dY < 0) dY = 0;
+    a.depthXP[k] = dX;
+    a.depthYP[k] = dY;
+    if (k >= g.N) {
+        int r = k - g.N;
+        if (r >= g.nx && r < 2 * g.nx) {             // jM ghost: YM test of cell (i, 0)
+            const int i = r - g.nx;
+            const double bot = a.bottomYP[k];
+            if (a.surf[i] < bot - 0.0 && a.surf[k] - bot >= p.wtfh) a.wtfYP[k] = 1;
+        } else if (r >= 2 * g.nx + g.ny && r < 2 * g.nx + 2 * g.ny) {   // iM ghost
+            const int jj = r - 2 * g.nx - g.ny;
+            const double bot = a.bottomXP[k];
+            if (a.surf[jj * g.nx] < bot - 0.0 && a.surf[k] - bot >= p.wtfh) a.wtfXP[k] = 1;
+        }
+        return;
+    }
+    const int c = k, i = c % g.nx, j = c / g.nx;
+    const int e = nbE(g, c, i, j), n = nbN(g, c, i, j);
+    const double hc = a.surf[c], he = a.surf[e], hn = a.surf[n];
+    const double bXc = a.bottomXP[c], bYc = a.bottomYP[c];
+    double fX = 0, fY = 0;
+    if (hc < bXc - 0.0 && he - bXc >= p.wtfh) fX = -1;
+    if (e < g.N && he < bXc - 0.0 && hc - bXc >= p.wtfh) fX = 1;
+    if (hc < bYc - 0.0 && hn - bYc >= p.wtfh) fY = -1;
+    if (n < g.N && hn < bYc - 0.0 && hc - bYc >= p.wtfh) fY = 1;
+    a.wtfXP[c] = fX;
+    a.wtfYP[c] = fY;
+    const double Nxc = a.Nx[c], Oyc = a.Oy[c], Vxc = a.Vx[c], Vyc = a.Vy[c];
+    const double effHx = Vxc > 0 ? Nxc / Vxc : 0;
+    const double effHy = Vyc > 0 ? Oyc / Vyc : 0;
+    const double divider = 1.0;
+    double uc = a.EnXP[c] - (p.g * p.dt * effHx) * (he - hc) / divider;
+    double vc = a.EnYP[c] - (p.g * p.dt * effHy) * (hn - hc) / divider;
+    if (Nxc == 0) uc = 0;
+    if (Oyc == 0) vc = 0;
+    // no outflow from a dry cell: the face's own cell stops positive flow,
+    // the cell beyond stops negative flow
+    const double *dn = a.depthNew;
+    const bool dryC = dn[c] == 0;
+    if (dryC && uc > 0) uc = 0;
+    if (e < g.N && dn[e] == 0 && uc < 0) uc = 0;
+    if (dryC && vc > 0) vc = 0;
+    if (n < g.N && dn[n] == 0 && vc < 0) vc = 0;
+    const bool rmvC = a.rmv[c] == 1;
+    if (rmvC || (e < g.N && a.rmv[e] == 1)) uc = 0;
+    if (rmvC || (n < g.N && a.rmv[n] == 1)) vc = 0;
+    const double CFLmax = 0.24 / p.subR, Cw = 0.7;
+    if (fX != 0) {
+        const double vel0 = uc;
+        uc = fX * Cw * sqrt(2 * p.g * Nxc / p.dy);
+        if (fabs(uc) > CFLmax * p.dx / p.dt) uc = fX * CFLmax * p.dx / p.dt;
+        loss -= (vel0 - uc) * p.dt * Nxc;
+    }
+    if (fY != 0) {
+        const double vel0 = vc;
+        vc = fY * Cw * sqrt(2 * p.g * Oyc / p.dx);
+        if (fabs(vc) > CFLmax * p.dy / p.dt) vc = fY * CFLmax * p.dy / p.dt;
+        loss -= (vel0 - vc) * p.dt * Oyc;
+    }
+    a.uuXP[c] = uc;
+    a.vvYP[c] = vc;
+}
+
+__global__ void __launch_bounds__(256) k_faces_sub(StepArgs a)
+{
+    __shared__ double lds[16];
+    if (a.ctl->halted) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double loss[1] = {0.0};
+    if (k < a.g.Ng) faces_sub_cell(a, k, loss[0]);
+    block_sum<1>(loss, lds);
+    if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blockIdx.x] = loss[0];
+}
+
+// enforceVelocityBC + computeFaceFlowRateSub over Ng (subgrid.c:1190-1212)
+__global__ void __launch_bounds__(256) k_flux_sub(StepArgs a)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.Ng) return;
+    double u, v;
+    if (k < g.N) {
+        u = a.uuXP[k];
+        v = a.vvYP[k];
+    } else {
+        u = a.uuXP[ghost_source(g, k, true)];
+        v = a.vvYP[ghost_source(g, k, false)];
+        a.uuXP[k] = u;
+        a.vvYP[k] = v;
+    }
+    a.Fuu[k] = u * a.Nx[k];
+    a.Fvv[k] = v * a.Oy[k];
+}
+
+// adjustTidalVelocitySub (subgrid.c:1291-1332)
+__global__ void k_tidal_velocity_sub(StepArgs a)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const P &p = a.p;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nP = p.bcType != 2 ? p.tideLocLengthP : 0;
+    const int nM = p.bcType != 1 ? p.tideLocLengthM : 0;
+    if (q >= nP + nM) return;
+    const bool isP = q < nP;
+    const int c = isP ? a.tideLocP[q] : a.tideLocM[q - nP];
+    const int i = c % g.nx, j = c / g.nx;
+    const double Fww = (a.surf[c] - a.surfOld[c]) * a.Z[c] / p.dt;
+    const double Oyc = a.Oy[c];
+    if (isP) {
+        const double all = a.Fvv[nbS(g, c, i, j)] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww;
+        a.vvYP[c] = (all != 0 && Oyc > 0) ? all / Oyc : 0;
+    } else {
+        const double all = -a.Fvv[c] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww;
+        a.vvYP[nbS(g, c, i, j)] = (all != 0 && Oyc > 0) ? -all / Oyc : 0;
+    }
+}
+
+// velocityInterp + volumeByFluxSub + updateCDSub (useSubDrag = 0: h = V/Z)
+// + thinLayerDrag + CFL numbers (nsfunctions.c:337-347, subgrid.c:1227-1263,
+// 1520-1535, nsfunctions.c:691-727, 1192-1232)
+__global__ void __launch_bounds__(256) k_finish_sub(StepArgs a)
+{
+    __shared__ double lds[16];
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const P &p = a.p;
+    double cflx = 0, cfly = 0;
+    if (c < g.N) {
+        const int tt = cur_tt(a);
+        const int nx = g.nx, i = c % nx, j = c / nx;
+        const double *u = a.uuXP, *v = a.vvYP;
+        const double um1 = c - 1 >= 0 ? u[c - 1] : 0.0;
+        const double um1n = c - 1 + nx >= 0 ? u[c - 1 + nx] : 0.0;
+        const double vmn = c - nx >= 0 ? v[c - nx] : 0.0;
+        const double vmn1 = c + 1 - nx >= 0 ? v[c + 1 - nx] : 0.0;
+        a.uuYP[c] = 0.25 * (u[c] + um1 + u[c + nx] + um1n);
+        a.vvXP[c] = 0.25 * (v[c] + v[c + 1] + vmn + vmn1);
+
+        double V = a.cellV[c] + p.dt * (a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] + a.Fvv[nbS(g, c, i, j)] - a.Fvv[c]);
+        const int ninflow = p.bcType != 3 ? (a.flags[c] >> 2) : 0;
+        for (int q = 0; q < ninflow; q++) V = V + a.inflow[tt] * p.dt;
+        if (V < 0) V = 0;
+        a.cellV[c] = V;
+
+        if (p.CDnotN == 0 && a.V[c] > 0) {
+            const double n = a.manning ? a.manning[c] : p.manningN;
+            const double coeff = p.g * n * n;
+            const double hh = a.V[c] / a.Z[c];
+            const double cd = coeff / pow(hh, 1.0 / 3.0);
+            a.CDXP[c] = cd;
+            a.CDYP[c] = cd;
+        }
+        const double d = a.depthNew[c];
+        if (p.useThinLayer == 1 && p.CDnotN == 0 && d > 0) {
+            const double n = a.manning ? a.manning[c] : p.manningN;
+            const double coeff = p.g * n * n;
+            if (a.depthXP[c] <= p.hD) a.CDXP[c] = coeff / pow(d, 2.0 / 3.0);
+            if (a.depthYP[c] <= p.hD) a.CDYP[c] = coeff / pow(d, 2.0 / 3.0);
+        }
+        cflx = fabs(u[c]) * p.dt / p.dx;
+        cfly = fabs(v[c]) * p.dt / p.dy;
+        a.CFLx[c] = cflx;
+        a.CFLy[c] = cfly;
+    }
+    if (p.useThinLayer == 1 && p.CDnotN == 1 && c < g.Ng) {
+        if (a.depthXP[c] <= p.hD) a.CDXP[c] = p.CDx + (p.CDmax - p.CDx) * (p.hD - a.depthXP[c]) / p.hD;
+        if (a.depthYP[c] <= p.hD) a.CDYP[c] = p.CDy + (p.CDmax - p.CDy) * (p.hD - a.depthYP[c]) / p.hD;
+    }
+    const double mx = block_max(cflx, lds);
+    const double my = block_max(cfly, lds);
+    if (threadIdx.x == 0) {
+        a.cflpart[blockIdx.x] = mx;
+        a.cflpart[a.maxBlocks + blockIdx.x] = my;
+    }
+}
+
+// matrixSourceTermSub + matrixCoeffSub (useSubgrid = 1, staggeredV = 1) +
+// adjustMatrixBoundary + adjustTidalBoundary (subgrid.c:930-1081,
+// nsfunctions.c:846-924) and the solver start.  A face coefficient is
+// g dt^2 N^2 / V / (1 + Drag) (0 where V = 0); the closed wall faces have
+// N = 0, so the wall adjustment subtracts zeros.  GnCt equal to 0 or dx*dy
+// becomes dx*dy; a dry cell with no velocity on its four faces gets the
+// identity-like row dx*dy * x = dx*dy * eta (flagged by a negative D).
+__global__ void __launch_bounds__(256) k_assemble_sub(StepArgs a)
+{
+    __shared__ double lds[16 * 3];
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const P &p = a.p;
+    double part[3] = {0.0, 0.0, 0.0};
+    const int tt = cur_tt(a);
+    const double dt = p.dt, dxdy = p.dx * p.dy;
+    const double cx = p.g * (dt * dt), cy = p.g * (dt * dt);
+    const double divider = 1.0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += gridDim.x * blockDim.x) {
+        const int i = c % g.nx, j = c / g.nx;
+        const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
+        const double Nxc = a.Nx[c], Nxw = a.Nx[w], Oyc = a.Oy[c], Oys = a.Oy[s];
+        const double Vxc = a.Vx[c], Vxw = a.Vx[w], Vyc = a.Vy[c], Vys = a.Vy[s];
+        double GXP = 0, GXM = 0, GYP = 0, GYM = 0;
+        if (Vxc > 0) GXP = cx * Nxc * Nxc / Vxc / divider / (1 + a.DragXP[c]);
+        if (Vxw > 0) GXM = cx * Nxw * Nxw / Vxw / divider / (1 + a.DragXP[w]);
+        if (Vyc > 0) GYP = cy * Oyc * Oyc / Vyc / divider / (1 + a.DragYP[c]);
+        if (Vys > 0) GYM = cy * Oys * Oys / Vys / divider / (1 + a.DragYP[s]);
+        double D = a.Z[c] + GXP + GYP + GXM + GYM;
+        const double hc = a.surf[c];
+        double z = hc * a.Z[c] - dt * (a.EnXP[c] * Nxc - a.EnXP[w] * Nxw) -
+                   dt * (a.EnYP[c] * Oyc - a.EnYP[s] * Oys);
+        const uint8_t fl = a.flags[c];
+        const int ninflow = p.bcType != 3 ? (fl >> 2) : 0;
+        for (int q = 0; q < ninflow; q++) z = z + dt * (a.inflow[tt] / p.inflowLocLength);
+        if (z == 0 || z == hc * p.dx * p.dy) z = hc * p.dx * p.dy;
+        if (D == 0.0 || D == dxdy) D = dxdy;
+        if (a.depthOld[c] == 0 && a.uuXP[c] == 0 && a.uuXP[w] == 0 && a.vvYP[c] == 0 && a.vvYP[s] == 0) {
+            D = -dxdy;
+            z = hc * p.dx * p.dy;
+        }
+        if ((fl & 1) && p.bcType != 2) { z = a.tideP[tt] + p.offset; D = -1.0; }
+        if ((fl & 2) && p.bcType != 1) { z = a.tideM[tt] + p.offset; D = -1.0; }
+        store_row(a, c, GXP, GYP, D, z, part);
+    }
+    block_sum<3>(part, lds);
+    store_partials<2>(a.partA, {part[0], part[1]});
 }
 
 }  // namespace frehd

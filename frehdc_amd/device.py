@@ -28,14 +28,19 @@ class DeviceScenario:
             rc = self.lib.frehd_dev_create(C.byref(self._cfg), C.byref(self._forc),
                                            C.byref(self._ic), C.byref(h))
         else:
-            keep = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in from_state.items()}
+            keep = {k: np.ascontiguousarray(v, dtype=np.int32 if k == "ind" else np.float64)
+                    for k, v in from_state.items()}
             self._keep = keep
             flds = abi.FrehdFields()
-            for name in abi.FIELD_NAMES:
+            for name in abi.FIELD_NAMES + abi.SUB_FIELD_NAMES:
                 if name in keep:
                     setattr(flds, name, abi.dptr(keep[name]))
+            if "ind" in keep:
+                flds.ind = abi.iptr(keep["ind"])
+            self._ic = scen.c_ic()
+            sub = self._ic.sub if scen.subgrid is not None else None
             rc = self.lib.frehd_dev_create_from_state(C.byref(self._cfg), C.byref(self._forc),
-                                                      C.byref(flds), C.byref(h))
+                                                      sub, C.byref(flds), C.byref(h))
         abi.check(rc)
         self.h = h
 
@@ -67,6 +72,11 @@ class DeviceScenario:
         out = {}
         flds = abi.FrehdFields()
         for n in names:
+            if n == "ind":
+                a = np.zeros(s.Ng, dtype=np.int32)
+                out[n] = a
+                flds.ind = abi.iptr(a)
+                continue
             if n == "S":
                 a = np.zeros(self.nScalar * s.Ng)
             elif n == "Sm":

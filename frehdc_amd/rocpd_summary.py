@@ -1,35 +1,67 @@
-"""Per-kernel duration summary of a rocprofv3 --kernel-trace run (the rocpd
-SQLite output), as text: kernel, launches, mean/min/max duration, total.
+"""Per-kernel duration summary of a rocprofv3 --kernel-trace run, as text:
+kernel, launches, mean/min/max duration, total.  Reads either the rocpd
+SQLite output (``*_results.db``) or the CSV kernel trace
+(``--output-format csv`` -> ``*_kernel_trace.csv``).
 
-    python -m frehdc_amd.rocpd_summary <results.db> [--cmd "the profiled command"]
+    python -m frehdc_amd.rocpd_summary <results.db | kernel_trace.csv> [--cmd "..."] [--active-us T]
+
+``--active-us T`` adds a second table over the launches longer than T us
+only: the PCG's chunked launches include iterations that exit at once after
+convergence, and the working launches are the ones the roofline is about.
 """
 from __future__ import annotations
 
 import argparse
+import collections
+import csv
 import sqlite3
 
 
-def summarize(db: str):
-    con = sqlite3.connect(db)
-    rows = con.execute(
-        "select name, count(*), avg(end-start), min(end-start), max(end-start), sum(end-start) "
-        "from kernels group by name order by sum(end-start) desc").fetchall()
-    con.close()
-    return rows
+def durations(path: str):
+    """{kernel name: [duration ns, ...]} in launch order."""
+    out = collections.defaultdict(list)
+    if path.endswith(".csv"):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                out[r["Kernel_Name"]].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    else:
+        con = sqlite3.connect(path)
+        for name, d in con.execute("select name, end-start from kernels order by start"):
+            out[name].append(int(d))
+        con.close()
+    return out
+
+
+def table(d, min_ns=0):
+    rows = []
+    for name, v in d.items():
+        v = [x for x in v if x > min_ns]
+        if v:
+            rows.append((name, len(v), sum(v) / len(v), min(v), max(v), sum(v)))
+    rows.sort(key=lambda r: -r[5])
+    tot = sum(r[5] for r in rows)
+    lines = [f"{'kernel':55s} {'launches':>8s} {'mean_us':>10s} {'min_us':>10s} {'max_us':>10s} "
+             f"{'total_ms':>10s} {'pct':>6s}"]
+    for name, n, avg, mn, mx, s in rows:
+        lines.append(f"{name[:55]:55s} {n:8d} {avg / 1e3:10.3f} {mn / 1e3:10.3f} {mx / 1e3:10.3f} "
+                     f"{s / 1e6:10.3f} {100 * s / tot:6.1f}")
+    return "\n".join(lines)
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("db")
+    ap.add_argument("path")
     ap.add_argument("--cmd", default="")
+    ap.add_argument("--active-us", type=float, default=0.0)
     a = ap.parse_args()
-    rows = summarize(a.db)
-    tot = sum(r[5] for r in rows)
+    d = durations(a.path)
     if a.cmd:
         print(f"command: {a.cmd}")
-    print(f"{'kernel':55s} {'launches':>8s} {'mean_us':>10s} {'min_us':>10s} {'max_us':>10s} {'total_ms':>10s} {'pct':>6s}")
-    for name, n, avg, mn, mx, s in rows:
-        print(f"{name[:55]:55s} {n:8d} {avg / 1e3:10.3f} {mn / 1e3:10.3f} {mx / 1e3:10.3f} {s / 1e6:10.3f} {100 * s / tot:6.1f}")
+    print("all launches")
+    print(table(d))
+    if a.active_us > 0:
+        print(f"\nlaunches longer than {a.active_us:g} us")
+        print(table(d, int(a.active_us * 1e3)))
 
 
 if __name__ == "__main__":

@@ -196,9 +196,12 @@ int frehd_dev_create(const frehd_config *cfg, const frehd_forcing *forcing,
 
 /* Create a handle from a state that the caller already initialised (the
  * reference's own Init output): every array of `state` must be non-NULL.
- * This is the drop-in form: state produced by FrehdC's Init is uploaded once. */
+ * This is the drop-in form: state produced by FrehdC's Init is uploaded once.
+ * With cfg->useSubgrid == 1, `sub` gives the tables and the subgrid state
+ * arrays (V ... Oy, ind) of `state` are required; `sub` may be NULL otherwise. */
 int frehd_dev_create_from_state(const frehd_config *cfg, const frehd_forcing *forcing,
-                                const frehd_fields *state, frehd_dev_t **out);
+                                const frehd_subgrid *sub, const frehd_fields *state,
+                                frehd_dev_t **out);
 
 /* Advance nsteps steps.  Each step is oneCompleteStep (nssolve.c:110-118)
  * followed by monitorCFL (nsfunctions.c:1192).  Step numbers continue from

@@ -54,6 +54,8 @@ def _bind(lib, prefix):
     getattr(lib, prefix + "pcg_iters").restype = C.c_int
     getattr(lib, prefix + "vloss").argtypes = [V, C.POINTER(C.c_double)]
     getattr(lib, prefix + "vloss").restype = None
+    getattr(lib, prefix + "ind").argtypes = [V]
+    getattr(lib, prefix + "ind").restype = C.POINTER(C.c_int32)
 
 
 _libs = {}
@@ -120,6 +122,13 @@ class _CpuModel:
     @property
     def pcg_iters(self):
         return getattr(self.lib, self.PREFIX + "pcg_iters")(self.h)
+
+    def ind(self):
+        """Level search index per cell (Ng), subgrid only."""
+        p = getattr(self.lib, self.PREFIX + "ind")(self.h)
+        if not p:
+            raise KeyError("ind")
+        return np.ctypeslib.as_array(p, shape=(self.scen.Ng,)).copy()
 
     def vloss(self):
         out = (C.c_double * 4)()

@@ -25,6 +25,7 @@ import pyoracle  # noqa: E402
 FIELDS = ["surf", "surfOld", "uuXP", "vvYP", "uuYP", "vvXP", "depth", "depthXP", "depthYP",
           "EnXP", "EnYP", "DragXP", "DragYP", "CDXP", "CDYP", "Fuu", "Fvv", "cellV", "S", "Sm",
           "CFLx", "CFLy"]
+SUB_FIELDS = ["V", "Z", "Vx", "Vy", "Zx", "Zy", "Nx", "Oy"]
 CHECKPOINTS = [1, 5, 40]
 
 
@@ -45,7 +46,12 @@ def inflow_basin():
     return sc
 
 
+def marsh_subgrid():
+    return cases.marsh_subgrid(n=12, r=4, Nt=60, period=1200.0, manning_cell=False, dsurf=0.05)
+
+
 CASES = {
+    "marsh12_subgrid4_jacobi": (marsh_subgrid, {}),
     "basin40x36_jacobi": (lambda: cases.basin(nx=40, ny=36, Nt=100), {}),
     "basin40x36_ssor": (lambda: cases.basin(nx=40, ny=36, Nt=100), {"precond": 1}),
     "marsh48_wetdry_jacobi": (fast_marsh, {}),
@@ -54,7 +60,11 @@ CASES = {
 
 
 def scenario_arrays(sc):
-    return {
+    sub = {}
+    if sc.subgrid is not None:
+        for k in ("edgeX", "edgeY", "V", "Z", "Vxp", "Vxm", "Vyp", "Vym", "Np", "Nm", "Op", "Om"):
+            sub[f"sub_{k}"] = sc.subgrid[k]
+    return {**sub,
         "in_bottomZ": sc.bottomZ, "in_tideP": sc.tideP, "in_tideM": sc.tideM,
         "in_inflow": sc.inflow, "in_tideLocP": sc.tideLocP, "in_tideLocM": sc.tideLocM,
         "in_inflowLoc": sc.inflowLoc,
@@ -74,7 +84,8 @@ def main():
         out["cfg_json"] = np.frombuffer(json.dumps(cfg).encode(), dtype=np.uint8)
         out["surf0"] = np.array([sc.surf0])
         iters, vloss = [], []
-        for n in FIELDS:
+        names = FIELDS + (SUB_FIELDS if sc.subgrid is not None else [])
+        for n in names:
             out[f"init_{n}"] = ref.field(n).reshape(-1)
         t = 0
         for cp in CHECKPOINTS:
@@ -84,7 +95,7 @@ def main():
                 iters.append(ref.pcg_iters)
                 vloss.append(ref.vloss())
                 t += 1
-            for n in FIELDS:
+            for n in names:
                 out[f"t{cp}_{n}"] = ref.field(n).reshape(-1)
         out["pcg_iters"] = np.array(iters, dtype=np.int32)
         out["vloss"] = np.array(vloss)

@@ -27,6 +27,11 @@ def load_fixture(path):
         inflowLoc=z["in_inflowLoc"].astype(np.int32), manningCell=None,
         surf0=float(z["surf0"][0]), S0cell=S0.copy() if S0.size else None,
         name=os.path.basename(path)[:-4])
+    if "sub_V" in z.files:
+        sg = {k[4:]: z[k].copy() for k in z.files if k.startswith("sub_")}
+        for m, src in (("minVxp", "Vxp"), ("minVxm", "Vxm"), ("minVyp", "Vyp"), ("minVym", "Vym")):
+            sg[m] = sg[src]
+        sc.subgrid = sg
     return sc, z
 
 

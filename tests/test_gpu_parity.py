@@ -49,6 +49,8 @@ def compare(dev_fields, ref_fields, names, tag):
         a, b = a[:m], b[:m]
         if n in ULP_FIELDS:
             ok = np.all(np.abs(a - b) <= 2 * np.spacing(np.maximum(np.abs(a), np.abs(b))))
+        elif a.dtype != np.float64 or b.dtype != np.float64:
+            ok = np.array_equal(a, b)
         else:
             ok = np.array_equal(a.view(np.int64), b.view(np.int64))
         if not ok:

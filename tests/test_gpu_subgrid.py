@@ -1,0 +1,101 @@
+"""Subgrid bathymetry on the device vs the CPU oracle (which matches the
+reference's subgrid.c bit for bit): Init, every phase and whole runs.
+
+Bit-exact except the PCG sums (tolerance as for the plain grid) and pow()
+in the drag update (2 ulp)."""
+import numpy as np
+import pytest
+
+from frehdc_amd import abi, cases
+import pyoracle as po
+
+from test_gpu_parity import compare
+
+pytestmark = pytest.mark.gpu
+
+SUB = abi.SUB_FIELD_NAMES
+
+
+def oracle_state(o, sc):
+    out = {}
+    for n in abi.FIELD_NAMES + SUB:
+        out[n] = np.ascontiguousarray(o.field(n) if n in abi.FIELD_NAMES else o.raw(n)).reshape(-1)
+    out["ind"] = o.ind()
+    return out
+
+
+def sub_cases():
+    return [cases.marsh_subgrid(n=32, r=4, Nt=300, period=1200.0),
+            cases.marsh_subgrid(n=24, r=8, Nt=300, period=1200.0, manning_cell=False)]
+
+
+@pytest.fixture(scope="module")
+def dev_mod():
+    from frehdc_amd import device
+    if device.device_count() < 1:
+        pytest.skip("no GPU")
+    return device
+
+
+def dl(d, sc):
+    return d.download(abi.FIELD_NAMES + SUB + ["ind"])
+
+
+@pytest.mark.parametrize("sc", sub_cases(), ids=lambda s: s.name)
+def test_subgrid_init_bit_exact(dev_mod, oracle_built, sc):
+    print(f"case {sc.name}: init", flush=True)
+    d = dev_mod.DeviceScenario(sc)
+    o = po.Oracle(sc)
+    compare(dl(d, sc), oracle_state(o, sc), abi.FIELD_NAMES + SUB + ["ind"], "init")
+
+
+@pytest.mark.parametrize("k", [0, 5, 40])
+@pytest.mark.parametrize("sc", sub_cases(), ids=lambda s: s.name)
+def test_subgrid_phases_bit_exact(dev_mod, oracle_built, sc, k):
+    print(f"case {sc.name}: phases after {k} steps", flush=True)
+    o = po.Oracle(sc)
+    o.step(k)
+    d = dev_mod.DeviceScenario(sc, from_state=oracle_state(o, sc))
+    d.tt = o.tt
+    o.phase(po.PHASE_SOURCE)
+    d.phase(0)
+    compare(dl(d, sc), oracle_state(o, sc), ["EnXP", "EnYP", "DragXP", "DragYP"], f"k={k} phase0")
+    d.phase(1)
+    o.phase(po.PHASE_SURFACE)
+    got, ref = dl(d, sc), oracle_state(o, sc)
+    compare(got, ref, ["surfOld"], "phase1")
+    assert np.max(np.abs(got["surf"] - ref["surf"])) <= 1e-9
+    d = dev_mod.DeviceScenario(sc, from_state=oracle_state(o, sc))
+    d.tt = o.tt
+    d.phase(2)
+    o.phase(po.PHASE_UPDATE)
+    names = [n for n in abi.FIELD_NAMES + SUB + ["ind"] if n not in ("CFLx", "CFLy", "S", "Sm")]
+    compare(dl(d, sc), oracle_state(o, sc), names, f"k={k} phase2")
+    d.phase(3)
+    o.phase(po.PHASE_SCALAR)
+    compare(dl(d, sc), oracle_state(o, sc), ["S", "Sm", "surf", "depth"], f"k={k} phase3")
+    d.phase(4)
+    o.phase(po.PHASE_CFL)
+    compare(dl(d, sc), oracle_state(o, sc), ["CFLx", "CFLy"], f"k={k} phase4")
+
+
+@pytest.mark.parametrize("sc", sub_cases(), ids=lambda s: s.name)
+def test_subgrid_trajectory(dev_mod, oracle_built, sc):
+    steps = 200
+    print(f"case {sc.name}: {steps}-step trajectory", flush=True)
+    d = dev_mod.DeviceScenario(sc)
+    o = po.Oracle(sc)
+    worst = {}
+    for t in range(steps // 20):
+        d.step(20)
+        o.step(20)
+        got = d.download(["surf", "uuXP", "vvYP", "depth", "S", "V"])
+        ref = oracle_state(o, sc)
+        N = sc.N
+        for n in ("surf", "uuXP", "vvYP", "S", "V"):
+            e = float(np.max(np.abs(got[n].reshape(-1)[:N] - ref[n][:N])))
+            worst[n] = max(worst.get(n, 0.0), e)
+        assert np.array_equal(got["depth"][:N] > 0, ref["depth"][:N] > 0), f"wet mask at {(t + 1) * 20}"
+    print("max errors", worst)
+    for n, e in worst.items():
+        assert e <= 1e-7, f"{n} error {e}"

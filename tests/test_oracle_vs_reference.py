@@ -10,7 +10,8 @@ FIELDS = ["surf", "surfOld", "uuXP", "vvYP", "uuYP", "vvXP", "depth", "depthXP",
           "EnXP", "EnYP", "DragXP", "DragYP", "CDXP", "CDYP", "Fuu", "Fvv", "cellV", "S",
           "CFLx", "CFLy", "wtfXP", "wtfYP", "bottomZ", "bottomXP", "bottomYP", "GnCt", "GnXP",
           "GnXM", "GnYP", "GnYM"]
-INTERIOR = {"uuYP", "vvXP", "depth", "cellV", "GnCt", "GnXP", "GnXM", "GnYP", "GnYM"}
+INTERIOR = {"uuYP", "vvXP", "depth", "cellV", "GnCt", "GnXP", "GnXM", "GnYP", "GnYM", "Vxp",
+            "Vxm", "Vyp", "Vym", "Np", "Nm", "Op", "Om"}
 
 
 def marsh_fast():
@@ -20,7 +21,16 @@ def marsh_fast():
     return sc
 
 
+def marsh_sub():
+    return cases.marsh_subgrid(n=20, r=4, Nt=200, period=1200.0, manning_cell=False)
+
+
+SUB_FIELDS = ["V", "Z", "Vx", "Vy", "Zx", "Zy", "Nx", "Oy", "Vxp", "Vxm", "Vyp", "Vym", "Np",
+              "Nm", "Op", "Om", "minVx", "minVy"]
+
 CASES = [
+    ("marsh_subgrid4_jacobi", marsh_sub, {}),
+    ("marsh_subgrid4_ssor", marsh_sub, {"precond": 1}),
     ("basin_jacobi", lambda: cases.basin(nx=30, ny=26, Nt=40), {}),
     ("basin_ssor", lambda: cases.basin(nx=30, ny=26, Nt=40), {"precond": 1}),
     ("marsh_jacobi", marsh_fast, {}),
@@ -37,10 +47,13 @@ def test_every_phase_bit_exact(oracle_built, name, builder, override):
     o = pyoracle.Oracle(sc, **override)
     r = pyoracle.Reference(sc, **override)
     steps = 40 if name.startswith("marsh") else 12
+    names = FIELDS + (SUB_FIELDS if sc.subgrid is not None else [])
     for t in range(steps):
         for p in range(5):
             assert o.phase(p) == r.phase(p)
-            for n in FIELDS:
+            if sc.subgrid is not None:
+                assert np.array_equal(o.ind(), r.ind()), (name, t, p, "ind")
+            for n in names:
                 a = o.raw(n)
                 b = r.raw(n)
                 m = sc.N if n in INTERIOR else sc.Ng
