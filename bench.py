@@ -112,12 +112,12 @@ def main():
     kt = dev.kernel_times()
     dev.enable_timing(False)
     log("kernel times (ms/launch, launches):", json.dumps(kt))
-    t_pq = kt.get("k_pcg_pq", (0, 0))[0]
-    t_up = kt.get("k_pcg_update", (0, 0))[0]
+    t_pq = kt.get("k_pcg_dir", (0, 0))[0]
+    t_up = kt.get("k_pcg_res", (0, 0))[0]
     t_iter_ms = t_pq + t_up
     alg_bytes = PCG_ITER_BYTES * scen.N
     achieved = alg_bytes / (t_iter_ms * 1e-3) / 1e9 if t_iter_ms > 0 else 0.0
-    roof = {"bound": "hbm", "kernel": "PCG iteration (k_pcg_pq + k_pcg_update)",
+    roof = {"bound": "hbm", "kernel": "PCG iteration (k_pcg_dir + k_pcg_res)",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_launch": alg_bytes, "ms_per_iteration": round(t_iter_ms, 5)}

@@ -158,6 +158,7 @@ def load_library(path: str | None = None):
         "frehd_dev_kernel_times": ([V, C.c_int32, C.POINTER(C.c_char_p), c_double_p,
                                     C.POINTER(C.c_int64), c_int32_p], C.c_int),
         "frehd_abi_sizes": ([C.POINTER(C.c_int64)], C.c_int),
+        "frehd_dev_set_pcg_tiling": ([V, C.c_int32, C.c_int32], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -10,11 +10,11 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfrehd_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["api.hip", "step_kernels.hip"]
+SOURCES = ["api.hip", "step_kernels.hip", "pcg_kernels.hip"]
 
 # -ffp-contract=off: no fused multiply-add, so every FP64 expression rounds as
-# the reference's (x86-64, no FMA) does.  Packed FP32 instructions are
-# disabled (they have given wrong results on this hardware pool).
+# the reference's (x86-64, no FMA) does.  Packed FP32 instructions
+# (v_pk_*_f32) are disabled as a precaution.
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off",

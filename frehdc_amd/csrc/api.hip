@@ -199,6 +199,7 @@ struct frehd_dev {
     size_t arenaBytes;
     StepArgs args;
     double *depthA, *depthB, *pB;
+    PcgArgs<double> pcg;        // FP64 solve: vectors, tiling, partial counts
     int parity;                 // which depth buffer holds the current depth
     int lastIters;
     int32_t *hostFlags;         // pinned: {done, halted}
@@ -228,6 +229,29 @@ struct Carve {
 // blocks of the grid-stride solver kernels: enough to fill the chip, few
 // enough that every block can re-reduce all block partials cheaply
 inline size_t pcg_blocks(size_t N) { return std::min<size_t>(1024, (N + 255) / 256); }
+
+// k_pcg_dir tiling: tiles of 256 columns x H rows, H chosen so that about
+// 1024 tiles cover the grid (16 waves per CU on 256 CUs, and ~1024 partials
+// for k_pcg_res to re-reduce); the XCD remap pads the tile count to a
+// multiple of 8
+struct DirTiling {
+    int gx, gy, H, remap, nb;
+};
+
+inline DirTiling dir_tiling(int nx, int ny, int H, int remap)
+{
+    DirTiling t;
+    t.gx = (nx + 255) / 256;
+    if (H <= 0) H = std::max(1, (int)((int64_t)t.gx * ny / 1024));
+    t.H = H;
+    t.gy = (ny + H - 1) / H;
+    t.remap = remap;
+    t.nb = t.gx * t.gy;
+    if (remap) t.nb = (t.nb + 7) & ~7;
+    return t;
+}
+
+inline size_t max_dir_blocks(int nx, int ny) { return (size_t)((nx + 255) / 256) * ny + 8; }
 
 // lay out every device array; with base == nullptr only the size is computed
 size_t layout(frehd_dev *d, char *base)
@@ -260,7 +284,7 @@ size_t layout(frehd_dev *d, char *base)
     a.nb1 = (int)nb1d;
     a.nb2 = (int)nb2d;
     a.partA = cv.take<double>(2 * G);
-    a.partB = cv.take<double>(G);
+    a.partB = cv.take<double>(max_dir_blocks(k.nx, k.ny));
     a.vpart = cv.take<double>(4 * maxb);
     a.cflpart = cv.take<double>(2 * maxb);
     a.tideP = cv.take<double>(Nt1);
@@ -371,6 +395,8 @@ int upload_series(frehd_dev *d, const double *src, const double *dst)
     return 0;
 }
 
+void set_pcg_args(frehd_dev *d, int H, int remap);
+
 int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **out, frehd_dev **dd)
 {
     int rc = check_config(cfg);
@@ -403,6 +429,7 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     if (f->manningCell) layout_manning(d, d->arena, used);
     d->args.g = d->g;
     fill_params(d->args.p, d->cfg);
+    set_pcg_args(d, 0, 0);
     HIPCHK(hipHostMalloc((void **)&d->hostFlags, 64, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&d->ctlHost, sizeof(Ctl), hipHostMallocDefault));
     d->parity = 0;
@@ -498,8 +525,8 @@ int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
 // ---- kernel launches with optional per-kernel event timing ----------------
 enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_FLUX, K_TIDEV,
                 K_FINISH, K_SCALAR, K_SCALARBC, K_END, K_COMBINE, K_SUBGHOST, K_SUBINIT, K_COUNT };
-const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble", "k_pcg_pq",
-    "k_pcg_update", "k_surface", "k_limiter", "k_faces", "k_flux", "k_tidal_velocity",
+const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble", "k_pcg_dir",
+    "k_pcg_res", "k_surface", "k_limiter", "k_faces", "k_flux", "k_tidal_velocity",
     "k_finish", "k_scalar", "k_scalar_bc", "k_step_end", "k_combine", "k_sub_ghosts",
     "k_subarea_init"};
 
@@ -551,20 +578,47 @@ int collect_timing(frehd_dev *d)
 
 int enqueue_pcg_iteration(frehd_dev *d, int it)
 {
-    StepArgs a = d->args;
-    a.pOld = (it & 1) ? d->pB : a.pA;
-    a.pNew = (it & 1) ? a.pA : d->pB;
-    const Grid &g = d->g;
-    dim3 b2(64, 4), g2((g.nx + 63) / 64, (g.ny + 3) / 4);
-    const dim3 gP((unsigned)pcg_blocks(g.N)), bP(256);
-    LAUNCH(K_PQ, k_pcg_pq, gP, bP, a, it);
-    LAUNCH(K_UPD, k_pcg_update, gP, bP, a, it);
+    PcgArgs<double> p = d->pcg;
+    p.pOld = (it & 1) ? d->pB : d->args.pA;
+    p.pNew = (it & 1) ? d->args.pA : d->pB;
+    LAUNCH(K_PQ, k_pcg_dir<double>, dim3((unsigned)p.nbDir), dim3(256), p, it);
+    LAUNCH(K_UPD, k_pcg_res<double>, dim3((unsigned)p.nbRes), dim3(256), p, it);
     return 0;
+}
+
+void set_pcg_args(frehd_dev *d, int H, int remap)
+{
+    const StepArgs &a = d->args;
+    PcgArgs<double> &p = d->pcg;
+    p.cX = a.cX;
+    p.cY = a.cY;
+    p.D = a.D;
+    p.r = a.r;
+    p.q = a.q;
+    p.pOld = a.pA;
+    p.pNew = d->pB;
+    p.x = a.x;
+    p.partA = a.partA;
+    p.partB = a.partB;
+    p.sc = &a.ctl->pcg;
+    p.halted = &a.ctl->halted;
+    p.eps = d->cfg.eps;
+    p.maxIter = d->cfg.maxIter;
+    p.nx = d->g.nx;
+    p.ny = d->g.ny;
+    p.N = d->g.N;
+    const DirTiling t = dir_tiling(p.nx, p.ny, H, remap);
+    p.gx = t.gx;
+    p.gy = t.gy;
+    p.H = t.H;
+    p.remap = t.remap;
+    p.nbDir = t.nb;
+    p.nbRes = (int)pcg_blocks(p.N);
 }
 
 int read_flags(frehd_dev *d)
 {
-    HIPCHK(hipMemcpyAsync(d->hostFlags, &d->args.ctl->pcg.done, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipMemcpyAsync(d->hostFlags, &d->args.ctl->pcg.doneAt, sizeof(int), hipMemcpyDeviceToHost, d->stream));
     HIPCHK(hipMemcpyAsync(d->hostFlags + 1, &d->args.ctl->halted, sizeof(int), hipMemcpyDeviceToHost, d->stream));
     HIPCHK(hipMemcpyAsync(d->hostFlags + 2, &d->args.ctl->pcg.iter, sizeof(int), hipMemcpyDeviceToHost, d->stream));
     HIPCHK(hipStreamSynchronize(d->stream));
@@ -602,7 +656,7 @@ int enqueue_phase(frehd_dev *d, int phase)
             }
             int rc = read_flags(d);
             if (rc) return rc;
-            if (d->hostFlags[0] || d->hostFlags[1]) break;
+            if (d->hostFlags[0] >= 0 || d->hostFlags[1]) break;
             if (it > d->cfg.maxIter + 8) return fail(FREHD_EHIP, "PCG did not report completion");
             chunk = d->timing ? 1 : 4;
         }
@@ -935,6 +989,17 @@ int frehd_dev_kernel_times(frehd_dev_t *d, int32_t cap, const char **names, doub
         cnt++;
     }
     *n = cnt;
+    return 0;
+}
+
+int frehd_dev_set_pcg_tiling(frehd_dev_t *d, int32_t rows_per_tile, int32_t xcd_remap)
+{
+    if (!d) return fail(FREHD_EINVAL, "handle is NULL");
+    if (rows_per_tile < 0 || rows_per_tile > d->g.ny)
+        return fail(FREHD_EINVAL, "rows_per_tile %d out of range [0, %d]", rows_per_tile, d->g.ny);
+    if ((size_t)dir_tiling(d->g.nx, d->g.ny, rows_per_tile, xcd_remap).nb > max_dir_blocks(d->g.nx, d->g.ny))
+        return fail(FREHD_EINVAL, "too many tiles");
+    set_pcg_args(d, rows_per_tile, xcd_remap ? 1 : 0);
     return 0;
 }
 

@@ -124,4 +124,45 @@ __device__ inline double block_max(double v, double *lds)
     return v;
 }
 
+// LASpack's IsZero / IsOne tests (elcmp.h:31-32): |a| < 10 DBL_MIN and
+// |a - 1| < 10 DBL_EPSILON
+#define LAS_IS_ZERO(a) (fabs(a) < 10.0 * 2.2250738585072014e-308)
+#define LAS_IS_ONE(a) (fabs((a)-1.0) < 10.0 * 2.220446049250313e-16)
+
+// ---- reductions without atomics ------------------------------------------
+// Every reduction kernel runs a fixed grid of G blocks (grid-stride over the
+// cells) and writes one partial per block.  The kernel that consumes the sum
+// reduces all G partials itself, in every block, in the same fixed order, so
+// every block obtains bitwise the same value and the result is identical run
+// to run.  (Same-address atomics serialise across the chip, and agent-scope
+// release/acquire fences write back and invalidate the per-XCD L2s; kernel
+// boundaries give the visibility for free.)
+template <int NV>
+__device__ inline void store_partials(double *part, const double (&v)[NV])
+{
+    // v already block-reduced into thread 0
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
+}
+
+template <int NV>
+__device__ inline void reduce_partials(const double *part, int nparts, double (&out)[NV], double *lds)
+{
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) acc[q] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NV; q++) acc[q] += part[(size_t)q * nparts + b];
+    block_sum<NV>(acc, lds);
+    __shared__ double bc[NV];
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) bc[q] = acc[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; q++) out[q] = bc[q];
+}
+
 }  // namespace frehd

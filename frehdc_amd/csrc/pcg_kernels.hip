@@ -1,0 +1,250 @@
+// pcg_kernels.hip -- the Jacobi-preconditioned conjugate gradient of the
+// free-surface solve (CGIter, LASpack itersolv.c:355-456, with
+// JacobiPrecond precond.c:26-46: z = r * (1/|diag|)), two kernels per
+// iteration:
+//
+//   k_pcg_dir(it)  stop test on the residual left by iteration it-1
+//                  (RTCResult rtc.c:48-72, checked before every iteration,
+//                  itersolv.c:407-409); x += alpha_{it-1} p_{it-1}
+//                  (itersolv.c:418, deferred from iteration it-1 to here so
+//                  that p_{it-1} is read once); p_it = z + beta p_{it-1};
+//                  q = A p_it (Mul_QV row-wise, operats.c:870-888); block
+//                  partials of p.q.
+//   k_pcg_res(it)  alpha = rho / p.q; r -= alpha q (itersolv.c:419-423,
+//                  LASpack's IsOne(alpha) shortcut kept); block partials
+//                  of r.r and r.z for the next stop test and beta.
+//
+// Every block of a consumer kernel re-reduces all block partials of the
+// producer in the same fixed order, so the scalars are bitwise identical in
+// every block and from run to run, without atomics.
+//
+// k_pcg_dir walks a tile of 256 columns x H rows: each wave owns 64
+// adjacent columns and slides down the rows, keeping p of rows j-1, j, j+1
+// in registers, so every array is read once per iteration (east/west
+// neighbours come from the adjacent lanes by shuffle; only lanes 0 and 63
+// read one halo column).  Loads run one row ahead of the arithmetic.
+#include "frehd_dev.h"
+#include "step_kernels.h"
+
+namespace frehd {
+
+template <typename T>
+__device__ inline T jacobi_z(T r, T D)
+{
+    return r * (T(1) / fabs(D));
+}
+
+template <typename T>
+__device__ inline T direction(T z, T po, T beta, int first, int betaOne)
+{
+    if (first) return z;
+    return betaOne ? z + po : z + beta * po;
+}
+
+// raw values of one cell needed to form its new direction
+template <typename T>
+struct Raw {
+    T r, D, po;
+};
+
+template <typename T>
+__device__ inline Raw<T> load_raw(const PcgArgs<T> &a, int c, int first)
+{
+    Raw<T> v;
+    v.r = a.r[c];
+    v.D = a.D[c];
+    v.po = first ? T(0) : a.pOld[c];
+    return v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_pcg_dir(PcgArgs<T> a, int it)
+{
+    __shared__ double lds[16 * 3];
+    PcgScalars *sc = a.sc;
+    if (*a.halted) return;
+    const int doneAt = sc->doneAt;
+    if (doneAt >= 0 && doneAt < it) return;
+
+    double rs[2];
+    reduce_partials<2>(a.partA, a.nbRes, rs, lds);
+    const double rr = rs[0], rz = rs[1];
+    const double rNorm = sqrt(rr);
+    const double bNorm = it == 0 ? rNorm : sc->bNorm;
+    const bool stop = rNorm < a.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm)) ||
+                      it >= a.maxIter;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (it == 0) { sc->bb = rr; sc->bNorm = bNorm; }
+        sc->rr = rr;
+        sc->rNorm = rNorm;
+        sc->rhoHist[it & 1] = rz;
+        sc->iter = it;
+        if (stop) sc->doneAt = it;
+    }
+    const int first = it == 0;
+    const double alphaPrev = first ? 0.0 : sc->alpha;
+    const int alphaOne = LAS_IS_ONE(alphaPrev);
+    const T al = (T)alphaPrev;
+
+    int lb = blockIdx.x;
+    if (a.remap) lb = (lb & 7) * (int)(gridDim.x >> 3) + (lb >> 3);
+    const int bx = lb % a.gx, by = lb / a.gx;
+    double part[1] = {0.0};
+    if (by < a.gy) {
+        const int nx = a.nx, ny = a.ny;
+        const int lane = threadIdx.x & 63;
+        const int i0 = bx * 256 + (threadIdx.x & ~63);
+        const int i = i0 + lane;
+        const bool valid = i < nx;
+        const int j0 = by * a.H, j1 = min(j0 + a.H, ny);
+        if (stop) {
+            // converged: only the deferred x update of iteration it-1
+            if (!first && valid)
+                for (int j = j0; j < j1; j++) {
+                    const int c = j * nx + i;
+                    const T po = a.pOld[c];
+                    a.x[c] = alphaOne ? a.x[c] + po : a.x[c] + al * po;
+                }
+        } else {
+            const double betaD = first ? 0.0 : rz / sc->rhoHist[(it - 1) & 1];
+            const int betaOne = LAS_IS_ONE(betaD);
+            const T beta = (T)betaD;
+            const int ih = lane == 0 ? i0 - 1 : (lane == 63 ? i0 + 64 : -1);
+            const bool halo = ih >= 0 && ih < nx;
+
+            // row j0-1 (south neighbour of the first row)
+            T pS = 0, cyS = 0;
+            if (valid && j0 > 0) {
+                const int c = (j0 - 1) * nx + i;
+                const Raw<T> v = load_raw(a, c, first);
+                pS = direction(jacobi_z(v.r, v.D), v.po, beta, first, betaOne);
+                cyS = a.cY[c];
+            }
+            // row j0
+            Raw<T> cur = {0, 1, 0}, nxt = {0, 1, 0}, hcur = {0, 1, 0};
+            T cxc = 0, cyc = 0, xc = 0;
+            if (valid) {
+                const int c = j0 * nx + i;
+                cur = load_raw(a, c, first);
+                cxc = a.cX[c];
+                cyc = j0 < ny - 1 ? a.cY[c] : T(0);
+                if (!first) xc = a.x[c];
+                if (j0 + 1 < ny) nxt = load_raw(a, c + nx, first);
+            }
+            if (halo) hcur = load_raw(a, j0 * nx + ih, first);
+            T pc = direction(jacobi_z(cur.r, cur.D), cur.po, beta, first, betaOne);
+
+            for (int j = j0; j < j1; j++) {
+                const int c = j * nx + i;
+                // loads one row ahead: raw of row j+2, the rest of row j+1
+                Raw<T> nn = {0, 1, 0}, hn = {0, 1, 0};
+                T cxn = 0, cyn = 0, xn = 0;
+                if (valid && j + 1 < j1) {
+                    if (j + 2 < ny) nn = load_raw(a, c + 2 * nx, first);
+                    cxn = a.cX[c + nx];
+                    cyn = j + 1 < ny - 1 ? a.cY[c + nx] : T(0);
+                    if (!first) xn = a.x[c + nx];
+                }
+                if (halo && j + 1 < j1) hn = load_raw(a, c + nx - i + ih, first);
+
+                const T pn = (valid && j + 1 < ny) ? direction(jacobi_z(nxt.r, nxt.D), nxt.po, beta, first, betaOne)
+                                                   : T(0);
+                const T ph = halo ? direction(jacobi_z(hcur.r, hcur.D), hcur.po, beta, first, betaOne) : T(0);
+                T pw = __shfl_up(pc, 1, 64), pe = __shfl_down(pc, 1, 64);
+                T cxw = __shfl_up(cxc, 1, 64);
+                if (lane == 0) {
+                    pw = ph;
+                    cxw = (valid && i > 0) ? a.cX[c - 1] : T(0);
+                }
+                if (lane == 63) pe = ph;
+                if (valid) {
+                    const T Dc = cur.D;
+                    T sum;
+                    if (Dc < 0) {
+                        sum = -Dc * pc;
+                    } else {
+                        sum = 0;
+                        if (i > 0) sum += (-cxw) * pw;
+                        if (j > 0) sum += (-cyS) * pS;
+                        sum += Dc * pc;
+                        if (j < ny - 1) sum += (-cyc) * pn;
+                        if (i < nx - 1) sum += (-cxc) * pe;
+                    }
+                    a.pNew[c] = pc;
+                    a.q[c] = sum;
+                    part[0] += (double)pc * (double)sum;
+                    if (!first) a.x[c] = alphaOne ? xc + cur.po : xc + al * cur.po;
+                }
+                pS = pc;
+                cyS = cyc;
+                pc = pn;
+                cur = nxt;
+                nxt = nn;
+                hcur = hn;
+                cxc = cxn;
+                cyc = cyn;
+                xc = xn;
+            }
+        }
+    }
+    block_sum<1>(part, lds);
+    if (!stop) store_partials<1>(a.partB, {part[0]});
+}
+
+template <typename T>
+struct alignas(16) Pack16 {
+    T v[16 / sizeof(T)];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_pcg_res(PcgArgs<T> a, int it)
+{
+    __shared__ double lds[16 * 3];
+    PcgScalars *sc = a.sc;
+    if (*a.halted) return;
+    const int doneAt = sc->doneAt;
+    if (doneAt >= 0 && doneAt <= it) return;
+    double pqv[1];
+    reduce_partials<1>(a.partB, a.nbDir, pqv, lds);
+    const double alpha = sc->rhoHist[it & 1] / pqv[0];
+    const int alphaOne = LAS_IS_ONE(alpha);
+    const T al = (T)alpha;
+    constexpr int V = 16 / sizeof(T);
+    double part[2] = {0.0, 0.0};
+    auto cell = [&](T &rk, T qk, T Dk) {
+        rk = alphaOne ? rk - qk : rk - al * qk;
+        part[0] += (double)rk * (double)rk;
+        part[1] += (double)rk * (double)(rk * (T(1) / fabs(Dk)));
+    };
+    const int nv = a.N / V;
+    const Pack16<T> *q2 = reinterpret_cast<const Pack16<T> *>(a.q);
+    const Pack16<T> *D2 = reinterpret_cast<const Pack16<T> *>(a.D);
+    Pack16<T> *r2 = reinterpret_cast<Pack16<T> *>(a.r);
+    for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < nv; h += gridDim.x * blockDim.x) {
+        const Pack16<T> qv = q2[h], dv = D2[h];
+        Pack16<T> rv = r2[h];
+#pragma unroll
+        for (int u = 0; u < V; u++) cell(rv.v[u], qv.v[u], dv.v[u]);
+        r2[h] = rv;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int k = nv * V; k < a.N; k++) {
+            T rk = a.r[k];
+            cell(rk, a.q[k], a.D[k]);
+            a.r[k] = rk;
+        }
+    block_sum<2>(part, lds);
+    store_partials<2>(a.partA, {part[0], part[1]});
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->pq = pqv[0];
+        sc->alpha = alpha;
+        sc->iter = it + 1;
+    }
+}
+
+template __global__ void k_pcg_dir<double>(PcgArgs<double>, int);
+template __global__ void k_pcg_res<double>(PcgArgs<double>, int);
+template __global__ void k_pcg_dir<float>(PcgArgs<float>, int);
+template __global__ void k_pcg_res<float>(PcgArgs<float>, int);
+
+}  // namespace frehd

@@ -23,7 +23,8 @@ struct P {
 struct PcgScalars {
     double bb, bNorm, rr, rNorm, pq, alpha;
     double rhoHist[2];        // r.z of iterations of even / odd index
-    int iter, done;
+    int iter;
+    int doneAt;               // iteration whose stop test succeeded, -1 while running
 };
 
 // device control block (one per handle)
@@ -74,13 +75,35 @@ struct StepArgs {
     const double *tab[10];
 };
 
+// One Jacobi-PCG solve over device vectors of type T (double for the FP64
+// path, float for the inner solve of the mixed-precision path).  Operator
+// row c (= j*nx + i): W -cX[c-1], S -cY[c-nx], C D[c], N -cY[c], E -cX[c];
+// D < 0 marks an identity row scaled by -D (tidal Dirichlet rows, isolated
+// dry subgrid cells).
+template <typename T>
+struct PcgArgs {
+    const T *cX, *cY, *D;
+    T *r, *q, *pOld, *pNew, *x;
+    double *partA;            // r.r, r.z partials of k_pcg_res (nbRes blocks)
+    double *partB;            // p.q partials of k_pcg_dir (nbDir blocks)
+    PcgScalars *sc;
+    const int *halted;
+    double eps;
+    int maxIter, nx, ny, N;
+    // k_pcg_dir tiling: gx x gy tiles of 256 columns x H rows; remap != 0
+    // gives each XCD a contiguous band of tiles (nbDir a multiple of 8)
+    int gx, gy, H, remap;
+    int nbDir, nbRes;
+};
+
+template <typename T> __global__ void k_pcg_dir(PcgArgs<T> a, int it);
+template <typename T> __global__ void k_pcg_res(PcgArgs<T> a, int it);
+
 enum SubTable { T_V, T_Z, T_VXP, T_VXM, T_VYP, T_VYM, T_NP, T_NM, T_OP, T_OM };
 
 __global__ void k_step_begin(StepArgs a);
 __global__ void k_predictor(StepArgs a);
 __global__ void k_assemble(StepArgs a);
-__global__ void k_pcg_pq(StepArgs a, int it);
-__global__ void k_pcg_update(StepArgs a, int it);
 __global__ void k_surface(StepArgs a);
 __global__ void k_limiter(StepArgs a);
 __global__ void k_faces(StepArgs a);

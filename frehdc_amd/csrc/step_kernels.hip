@@ -11,9 +11,6 @@
 
 namespace frehd {
 
-#define LAS_IS_ZERO(a) (fabs(a) < 10.0 * 2.2250738585072014e-308)
-#define LAS_IS_ONE(a) (fabs((a)-1.0) < 10.0 * 2.220446049250313e-16)
-
 __device__ inline int cur_tt(const StepArgs &a) { return a.ctl->tt; }
 
 // ---------------------------------------------------------------------------
@@ -25,7 +22,7 @@ __global__ void k_step_begin(StepArgs a)
         if (c->halted) return;
         c->tt += 1;
         c->pcg.iter = 0;
-        c->pcg.done = 0;
+        c->pcg.doneAt = -1;
         c->pcg.rhoHist[0] = c->pcg.rhoHist[1] = 0.0;
     }
 }
@@ -163,42 +160,6 @@ __device__ inline double face_coef(double coef, double depthF, double dragF)
     return coef * depthF / (1 + dragF);
 }
 
-// ---- reductions without atomics ------------------------------------------
-// Every reduction kernel runs a fixed grid of G blocks (grid-stride over the
-// cells) and writes one partial per block.  The kernel that consumes the sum
-// reduces all G partials itself, in every block, in the same fixed order, so
-// every block obtains bitwise the same value and the result is identical run
-// to run.  (Same-address atomics serialise across the chip, and agent-scope
-// release/acquire fences write back and invalidate the per-XCD L2s; kernel
-// boundaries give the visibility for free.)
-template <int NV>
-__device__ inline void store_partials(double *part, const double (&v)[NV])
-{
-    // v already block-reduced into thread 0
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < NV; q++) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
-}
-
-template <int NV>
-__device__ inline void reduce_partials(const double *part, int nparts, double (&out)[NV], double *lds)
-{
-    double acc[NV];
-#pragma unroll
-    for (int q = 0; q < NV; q++) acc[q] = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
-#pragma unroll
-        for (int q = 0; q < NV; q++) acc[q] += part[(size_t)q * nparts + b];
-    block_sum<NV>(acc, lds);
-    __shared__ double bc[NV];
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < NV; q++) bc[q] = acc[q];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NV; q++) out[q] = bc[q];
-}
-
 // stores the operator row and the solver start of cell c, accumulating b.b
 // and r.z (z = r / D) of the thread's cells
 __device__ inline void store_row(const StepArgs &a, int c, double GXP, double GYP, double D, double z,
@@ -249,106 +210,6 @@ __global__ void __launch_bounds__(256) k_assemble(StepArgs a)
     store_partials<2>(a.partA, {part[0], part[1]});
 }
 
-// ---------------------------------------------------------------------------
-// PCG iteration `it`, part 1: the stopping test on the residual of the
-// previous iteration (RTCResult, rtc.c:48-72, checked before every
-// iteration, itersolv.c:407-409), then p = z + beta p (z = r / D,
-// JacobiPrecond precond.c:26-46) and q = A p (Mul_QV row-wise,
-// operats.c:870-888) with the block partials of p.q.  p is double-buffered
-// (pOld -> pNew) so the neighbours' new p can be recomputed from their old p.
-__device__ inline double p_new_at(const StepArgs &a, int k, double beta, int first, int betaOne)
-{
-    const double rk = a.r[k];
-    const double zk = rk * (1.0 / fabs(a.D[k]));
-    if (first) return zk;
-    const double po = a.pOld[k];
-    return betaOne ? zk + po : zk + beta * po;
-}
-
-__global__ void __launch_bounds__(256) k_pcg_pq(StepArgs a, int it)
-{
-    __shared__ double lds[16 * 3];
-    Ctl *ctl = a.ctl;
-    if (ctl->halted || ctl->pcg.done) return;
-    const Grid g = a.g;
-    const P &p = a.p;
-    // r.r and r.z of the current residual (from the assembly or the last update)
-    double rs[2];
-    reduce_partials<2>(a.partA, gridDim.x, rs, lds);
-    const double rr = rs[0], rz = rs[1];
-    const double rNorm = sqrt(rr);
-    const double bNorm = it == 0 ? rNorm : ctl->pcg.bNorm;
-    const bool stop = rNorm < p.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm)) ||
-                      it >= p.maxIter;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        PcgScalars &sc = ctl->pcg;
-        if (it == 0) { sc.bb = rr; sc.bNorm = bNorm; }
-        sc.rr = rr;
-        sc.rNorm = rNorm;
-        sc.rhoHist[it & 1] = rz;
-        sc.iter = it;
-        if (stop) sc.done = 1;
-    }
-    if (stop) return;
-    const int first = it == 0;
-    const double beta = first ? 0.0 : rz / ctl->pcg.rhoHist[(it - 1) & 1];
-    const int betaOne = LAS_IS_ONE(beta);
-    double part[3] = {0.0, 0.0, 0.0};
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += gridDim.x * blockDim.x) {
-        const int i = c % g.nx, j = c / g.nx;
-        const double Dc = a.D[c];
-        const double pc = p_new_at(a, c, beta, first, betaOne);
-        double sum = 0.0;
-        if (Dc < 0) {
-            // identity row (tidal Dirichlet or isolated dry cell): zero off-diagonals
-            sum = -Dc * pc;
-        } else {
-            if (i > 0) sum += (-a.cX[c - 1]) * p_new_at(a, c - 1, beta, first, betaOne);
-            if (j > 0) sum += (-a.cY[c - g.nx]) * p_new_at(a, c - g.nx, beta, first, betaOne);
-            sum += Dc * pc;
-            if (j < g.ny - 1) sum += (-a.cY[c]) * p_new_at(a, c + g.nx, beta, first, betaOne);
-            if (i < g.nx - 1) sum += (-a.cX[c]) * p_new_at(a, c + 1, beta, first, betaOne);
-        }
-        a.pNew[c] = pc;
-        a.q[c] = sum;
-        part[0] += pc * sum;
-    }
-    block_sum<3>(part, lds);
-    store_partials<1>(a.partB, {part[0]});
-}
-
-// PCG iteration `it`, part 2: alpha = rho / p.q, x += alpha p, r -= alpha q
-// (itersolv.c:418-423, LASpack's IsOne(alpha) shortcut kept), and the block
-// partials of r.r and r.z for the next stopping test and beta.
-__global__ void __launch_bounds__(256) k_pcg_update(StepArgs a, int it)
-{
-    __shared__ double lds[16 * 3];
-    Ctl *ctl = a.ctl;
-    if (ctl->halted || ctl->pcg.done) return;
-    const Grid g = a.g;
-    double pqv[1];
-    reduce_partials<1>(a.partB, gridDim.x, pqv, lds);
-    const double alpha = ctl->pcg.rhoHist[it & 1] / pqv[0];
-    const int alphaOne = LAS_IS_ONE(alpha);
-    double part[3] = {0.0, 0.0, 0.0};
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < g.N; k += gridDim.x * blockDim.x) {
-        const double pk = a.pNew[k], qk = a.q[k];
-        double xk = a.x[k], rk = a.r[k];
-        if (alphaOne) { xk += pk; rk -= qk; }
-        else { xk += alpha * pk; rk -= alpha * qk; }
-        a.x[k] = xk;
-        a.r[k] = rk;
-        part[0] += rk * rk;
-        part[1] += rk * (rk * (1.0 / fabs(a.D[k])));
-    }
-    block_sum<3>(part, lds);
-    store_partials<2>(a.partA, {part[0], part[1]});
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->pcg.pq = pqv[0];
-        ctl->pcg.alpha = alpha;
-        ctl->pcg.iter = it + 1;
-    }
-}
 
 __device__ inline void surface_cell(const StepArgs &a, int k, double &loss);
 

@@ -114,6 +114,9 @@ class DeviceScenario:
     def tt(self, value):
         abi.check(self.lib.frehd_dev_set_step_index(self.h, int(value)))
 
+    def set_pcg_tiling(self, rows_per_tile=0, xcd_remap=False):
+        abi.check(self.lib.frehd_dev_set_pcg_tiling(self.h, int(rows_per_tile), 1 if xcd_remap else 0))
+
     def enable_timing(self, on=True):
         abi.check(self.lib.frehd_dev_enable_timing(self.h, 1 if on else 0))
 

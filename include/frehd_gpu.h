@@ -243,6 +243,13 @@ int frehd_dev_enable_timing(frehd_dev_t *dev, int32_t on);
 int frehd_dev_kernel_times(frehd_dev_t *dev, int32_t cap, const char **names,
                            double *ms, int64_t *launches, int32_t *n);
 
+/* Launch shape of the solver's direction kernel (no counterpart in the
+ * reference; a tuning knob for the autotuner): tiles of 256 columns x
+ * rows_per_tile rows (0 = automatic, about 1024 tiles), and xcd_remap != 0
+ * to give each of the 8 XCDs a contiguous band of tiles.  Results do not
+ * depend on the shape beyond the order of the solver's dot-product sums. */
+int frehd_dev_set_pcg_tiling(frehd_dev_t *dev, int32_t rows_per_tile, int32_t xcd_remap);
+
 #ifdef __cplusplus
 }
 #endif
