@@ -33,7 +33,7 @@ class FrehdConfig(C.Structure):
         ("tidePS", C.c_double), ("tideMS", C.c_double), ("inflowS", C.c_double),
         ("Kx", C.c_double), ("Ky", C.c_double), ("offset", C.c_double),
         ("epsInner", C.c_double), ("maxRefine", C.c_int32),
-        ("useSubgrid", C.c_int32), ("Ne", C.c_int32), ("pad1", C.c_int32),
+        ("useSubgrid", C.c_int32), ("Ne", C.c_int32), ("warmStart", C.c_int32),
         ("surfmin", C.c_double), ("dsurf", C.c_double), ("subR", C.c_double),
     ]
 

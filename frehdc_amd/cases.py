@@ -116,7 +116,7 @@ def default_config(nx, ny, dx, dt, Nt, **kw) -> dict:
         precond=abi.PRECOND_JACOBI, precision=abi.PREC_FP64, bcType=1, tideLocLengthP=nx,
         tideLocLengthM=0, inflowLocLength=0, useScalar=1, scalarAdv=1, useConstTidePS=1,
         useConstTideMS=1, nScalar=1, tidePS=0.0, tideMS=0.0, inflowS=0.0, Kx=1e-4, Ky=1e-4,
-        offset=0.0, epsInner=1e-6, maxRefine=8, useSubgrid=0, Ne=0, surfmin=0.0, dsurf=0.01,
+        offset=0.0, epsInner=1e-5, maxRefine=8, warmStart=0, useSubgrid=0, Ne=0, surfmin=0.0, dsurf=0.01,
         subR=1.0,
     )
     cfg.update(kw)

@@ -200,6 +200,11 @@ struct frehd_dev {
     StepArgs args;
     double *depthA, *depthB, *pB;
     PcgArgs<double> pcg;        // FP64 solve: vectors, tiling, partial counts
+    PcgArgs<float> pcg32;       // mixed precision: inner FP32 solve
+    RefineArgs ref;             // mixed precision: FP64 refinement
+    float *pA32, *pB32;
+    double *xFinal;             // where the last solve left x
+    std::vector<int> mixPlan;   // inner iterations per sweep of the last mixed solve
     int parity;                 // which depth buffer holds the current depth
     int lastIters;
     int32_t *hostFlags;         // pinned: {done, halted}
@@ -283,8 +288,17 @@ size_t layout(frehd_dev *d, char *base)
     a.maxBlocks = (int)maxb;
     a.nb1 = (int)nb1d;
     a.nb2 = (int)nb2d;
-    a.partA = cv.take<double>(2 * G);
-    a.partB = cv.take<double>(max_dir_blocks(k.nx, k.ny));
+    const size_t maxDir = max_dir_blocks(k.nx, k.ny);
+    a.partA = cv.take<double>(2 * std::max(G, maxDir));
+    a.partB = cv.take<double>(maxDir);
+    a.cX32 = a.cY32 = a.D32 = nullptr;
+    if (k.precision == FREHD_PREC_MIXED) {
+        float **fArr[] = {&a.cX32, &a.cY32, &a.D32, &d->pcg32.r, &d->pcg32.q, &d->pA32, &d->pB32,
+                          &d->pcg32.x};
+        for (auto **p : fArr) *p = cv.take<float>(N);
+        d->ref.xbuf[1] = cv.take<double>(N);
+        d->ref.partC = cv.take<double>(2 * maxDir);
+    }
     a.vpart = cv.take<double>(4 * maxb);
     a.cflpart = cv.take<double>(2 * maxb);
     a.tideP = cv.take<double>(Nt1);
@@ -329,7 +343,8 @@ int check_config(const frehd_config *k)
                                   "(SSOR's triangular sweeps are sequential)");
     if (k->precision != FREHD_PREC_FP64 && k->precision != FREHD_PREC_MIXED)
         return fail(FREHD_EINVAL, "unknown precision %d", k->precision);
-    if (k->precision == FREHD_PREC_MIXED) return fail(FREHD_EINVAL, "mixed precision is not built in this library");
+    if (k->precision == FREHD_PREC_MIXED && !(k->epsInner > 0 && k->epsInner < 1 && k->maxRefine >= 1))
+        return fail(FREHD_EINVAL, "mixed precision needs 0 < epsInner < 1 and maxRefine >= 1");
     if (k->useScalar == 1 && k->scalarAdv != 1)
         return fail(FREHD_EINVAL, "only first-order upwind scalar advection (scalarAdv = 1) is supported");
     if (k->bcType < 1 || k->bcType > 3) return fail(FREHD_EINVAL, "bcType must be 1, 2 or 3");
@@ -378,6 +393,7 @@ void fill_params(P &p, const frehd_config &k)
     p.nScalar = k.useScalar == 1 ? std::max(1, k.nScalar) : 0;
     p.useConstTidePS = k.useConstTidePS; p.useConstTideMS = k.useConstTideMS; p.maxIter = k.maxIter;
     p.useSubgrid = k.useSubgrid; p.dsurf = k.dsurf; p.subR = k.subR; p.Ne = k.Ne;
+    p.warmStart = k.precision == FREHD_PREC_MIXED ? k.warmStart : 0;
 }
 
 int upload(frehd_dev *d, const void *host, void *dev, size_t bytes)
@@ -524,11 +540,12 @@ int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
 
 // ---- kernel launches with optional per-kernel event timing ----------------
 enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_FLUX, K_TIDEV,
-                K_FINISH, K_SCALAR, K_SCALARBC, K_END, K_COMBINE, K_SUBGHOST, K_SUBINIT, K_COUNT };
+                K_FINISH, K_SCALAR, K_SCALARBC, K_END, K_COMBINE, K_SUBGHOST, K_SUBINIT,
+                K_PQ32, K_UPD32, K_REFINE, K_RTEST, K_COUNT };
 const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble", "k_pcg_dir",
     "k_pcg_res", "k_surface", "k_limiter", "k_faces", "k_flux", "k_tidal_velocity",
     "k_finish", "k_scalar", "k_scalar_bc", "k_step_end", "k_combine", "k_sub_ghosts",
-    "k_subarea_init"};
+    "k_subarea_init", "k_pcg_dir_f32", "k_pcg_res_f32", "k_refine", "k_refine_test"};
 
 hipEvent_t next_event(frehd_dev *d)
 {
@@ -614,6 +631,135 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     p.remap = t.remap;
     p.nbDir = t.nb;
     p.nbRes = (int)pcg_blocks(p.N);
+    p.nbStart = p.nbRes;
+    p.outerDone = nullptr;
+    p.epsDyn = nullptr;
+    p.decouple = 0;
+    d->xFinal = a.x;
+    if (d->cfg.precision != FREHD_PREC_MIXED) return;
+    // inner FP32 solve: same tiling, FP32 operator and vectors (r, q, x = the
+    // correction d were carved by layout()), started by k_refine
+    PcgArgs<float> &f = d->pcg32;
+    f.cX = a.cX32;
+    f.cY = a.cY32;
+    f.D = a.D32;
+    f.pOld = d->pA32;
+    f.pNew = d->pB32;
+    f.partA = a.partA;
+    f.partB = a.partB;
+    f.sc = &a.ctl->pcgIn;
+    f.halted = &a.ctl->halted;
+    f.outerDone = &a.ctl->ref.doneAt;
+    f.epsDyn = &a.ctl->ref.epsIn;
+    f.eps = d->cfg.epsInner;
+    f.maxIter = p.maxIter;
+    f.nx = p.nx; f.ny = p.ny; f.N = p.N;
+    f.gx = t.gx; f.gy = t.gy; f.H = t.H; f.remap = t.remap;
+    f.nbDir = t.nb;
+    f.nbRes = p.nbRes;
+    f.nbStart = t.nb;
+    f.decouple = 1;
+    RefineArgs &r = d->ref;
+    r.cX = a.cX; r.cY = a.cY; r.D = a.D; r.b = a.b;
+    r.xbuf[0] = a.x;
+    r.d = f.x;
+    r.r32 = f.r;
+    r.D32 = a.D32;
+    r.partA = a.partA;
+    r.ctl = a.ctl;
+    r.eps = d->cfg.eps;
+    r.epsInner = d->cfg.epsInner;
+    r.maxRefine = d->cfg.maxRefine;
+    r.nx = p.nx; r.ny = p.ny; r.N = p.N;
+    r.gx = t.gx; r.gy = t.gy; r.H = t.H; r.remap = t.remap;
+    r.nb = t.nb;
+}
+
+int enqueue_inner_iteration(frehd_dev *d, int it)
+{
+    PcgArgs<float> p = d->pcg32;
+    p.pOld = (it & 1) ? d->pB32 : d->pA32;
+    p.pNew = (it & 1) ? d->pA32 : d->pB32;
+    LAUNCH(K_PQ32, k_pcg_dir<float>, dim3((unsigned)p.nbDir), dim3(256), p, it);
+    LAUNCH(K_UPD32, k_pcg_res<float>, dim3((unsigned)p.nbRes), dim3(256), p, it);
+    return 0;
+}
+
+int enqueue_refine(frehd_dev *d, int s)
+{
+    LAUNCH(K_REFINE, k_refine, dim3((unsigned)d->ref.nb), dim3(256), d->ref, s);
+    LAUNCH(K_RTEST, k_refine_test, dim3(1), dim3(256), d->ref, s);
+    return 0;
+}
+
+// one inner solve of n iterations plus the stop test / deferred x update of
+// iteration n (a no-op when it converged earlier)
+int enqueue_inner_solve(frehd_dev *d, int n)
+{
+    for (int it = 0; it < n; it++) {
+        int rc = enqueue_inner_iteration(d, it);
+        if (rc) return rc;
+    }
+    PcgArgs<float> p = d->pcg32;
+    p.pOld = (n & 1) ? d->pB32 : d->pA32;
+    p.pNew = (n & 1) ? d->pA32 : d->pB32;
+    LAUNCH(K_PQ32, k_pcg_dir<float>, dim3((unsigned)p.nbDir), dim3(256), p, n);
+    return 0;
+}
+
+int read_refine(frehd_dev *d)
+{
+    const Ctl *c = d->args.ctl;
+    HIPCHK(hipMemcpyAsync(&d->ctlHost->ref, &c->ref, sizeof(RefineScalars), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipMemcpyAsync(&d->ctlHost->halted, &c->halted, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipMemcpyAsync(&d->ctlHost->pcgIn, &c->pcgIn, sizeof(PcgScalars), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+// Mixed-precision solve: sweeps of FP64 refinement around FP32 inner
+// solves.  The launch pattern of the last solve (sweeps and inner
+// iterations per sweep) is enqueued without waiting; launches past
+// convergence exit at once, and an inner solve cut short by the pattern
+// still yields a valid correction (the next sweep recomputes the true
+// residual).  If the pattern ends before the outer test passes, further
+// sweeps follow, each inner solve run to its own stop test.
+int enqueue_mixed_solve(frehd_dev *d)
+{
+    int rc, s = 0;
+    if ((rc = enqueue_refine(d, 0))) return rc;
+    for (size_t k = 0; k < d->mixPlan.size(); k++) {
+        if ((rc = enqueue_inner_solve(d, d->mixPlan[k]))) return rc;
+        if ((rc = enqueue_refine(d, ++s))) return rc;
+    }
+    if ((rc = read_refine(d))) return rc;
+    const Ctl *h = d->ctlHost;
+    while (h->ref.doneAt < 0 && !h->halted) {
+        if (s >= d->cfg.maxRefine) return fail(FREHD_EHIP, "refinement did not report completion");
+        int it = 0, chunk = 8;
+        for (;;) {
+            for (int q = 0; q < chunk; q++, it++)
+                if ((rc = enqueue_inner_iteration(d, it))) return rc;
+            if ((rc = read_refine(d))) return rc;
+            if (h->pcgIn.doneAt >= 0 || h->halted) break;
+            if (it > d->cfg.maxIter + 8) return fail(FREHD_EHIP, "inner PCG did not report completion");
+        }
+        if ((rc = enqueue_refine(d, ++s))) return rc;
+        if ((rc = read_refine(d))) return rc;
+    }
+    if (h->halted) return 0;
+    const int done = h->ref.doneAt;
+    d->xFinal = d->ref.xbuf[done & 1];
+    // pattern for the next step: the inner iterations each sweep used, one
+    // more where the pattern cut a solve short
+    std::vector<int> plan;
+    for (int k = 0; k < done && k < 8; k++) {
+        int n = h->ref.innerIters[k];
+        if (k < (int)d->mixPlan.size() && n >= d->mixPlan[k]) n = d->mixPlan[k] + 2;
+        plan.push_back(std::max(1, n));
+    }
+    d->mixPlan = plan;
+    return 0;
 }
 
 int read_flags(frehd_dev *d)
@@ -647,6 +793,14 @@ int enqueue_phase(frehd_dev *d, int phase)
         // PCG: launch the previous step's iteration count, then small chunks,
         // until the device reports convergence (or halt); one iteration per
         // chunk while timing, so no idle launch enters the averages
+        if (d->cfg.precision == FREHD_PREC_MIXED) {
+            int rc = enqueue_mixed_solve(d);
+            if (rc) return rc;
+            StepArgs as = a;
+            as.x = d->xFinal;
+            LAUNCH(K_SURF, k_surface, gNg, b1, as);
+            return 0;
+        }
         int it = 0;
         int chunk = d->timing ? 1 : std::max(1, d->lastIters);
         for (;;) {
@@ -738,7 +892,7 @@ extern "C" {
 
 const char *frehd_last_error(void) { return g_err.c_str(); }
 
-const char *frehd_version(void) { return "frehd-mi355x 0.2 (gfx950, FP64 plain and subgrid step, Jacobi PCG)"; }
+const char *frehd_version(void) { return "frehd-mi355x 0.3 (gfx950, plain and subgrid step, Jacobi PCG in FP64 or FP32 with FP64 refinement)"; }
 
 int frehd_device_count(int32_t *count)
 {
@@ -922,7 +1076,7 @@ int frehd_dev_stats(frehd_dev_t *d, frehd_stats *out)
     out->steps = c.steps;
     out->pcgIterTotal = c.pcgIterTotal;
     out->pcgIterLast = c.pcgIterLast;
-    out->refineLast = 0;
+    out->refineLast = c.refineLast;
     out->relResLast = c.relResLast;
     out->maxCFLx = c.lastMaxCFLx;
     out->maxCFLy = c.lastMaxCFLy;

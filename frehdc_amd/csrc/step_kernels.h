@@ -17,7 +17,8 @@ struct P {
     int useScalar, nScalar, useConstTidePS, useConstTideMS, maxIter, useSubgrid;
     // subgrid
     double dsurf, subR;
-    int Ne, pad2;
+    int Ne;
+    int warmStart;            // mixed precision: refinement starts from eta^n instead of 0
 };
 
 struct PcgScalars {
@@ -25,6 +26,15 @@ struct PcgScalars {
     double rhoHist[2];        // r.z of iterations of even / odd index
     int iter;
     int doneAt;               // iteration whose stop test succeeded, -1 while running
+};
+
+// FP64 refinement of the mixed-precision solve
+struct RefineScalars {
+    double bNorm, rNorm;
+    double epsIn;             // relative tolerance of the current inner FP32 solve
+    int sweep;                // last refinement sweep tested
+    int doneAt;               // sweep whose residual test succeeded, -1 while running
+    int innerIters[8];        // inner iterations of sweeps 0..7 of the last solve
 };
 
 // device control block (one per handle)
@@ -41,6 +51,10 @@ struct Ctl {
     long long pcgIterTotal;
     long long steps;
     double relResLast;
+    RefineScalars ref;
+    PcgScalars pcgIn;         // inner FP32 solve of the mixed-precision path
+    int refineLast;
+    int pad1;
 };
 
 struct StepArgs {
@@ -57,6 +71,7 @@ struct StepArgs {
     uint8_t *rmv, *flags;     // flags: bit0 tideP cell, bit1 tideM cell, bits 2-7 inflow count
     // operator and solver vectors
     double *cX, *cY, *D, *b, *x, *r, *q, *pA, *pOld, *pNew;
+    float *cX32, *cY32, *D32;  // FP32 operator copies (mixed precision only, else NULL)
     // per-block partial sums: solver (partA: r.r, r.z; partB: p.q), volume
     // losses (vpart: 4 slots) and CFL maxima (cflpart: 2 slots), each slot
     // maxBlocks long; nb1 = blocks of an Ng sweep, nb2 = blocks of a 2-D sweep
@@ -84,20 +99,45 @@ template <typename T>
 struct PcgArgs {
     const T *cX, *cY, *D;
     T *r, *q, *pOld, *pNew, *x;
-    double *partA;            // r.r, r.z partials of k_pcg_res (nbRes blocks)
+    double *partA;            // r.r, r.z partials: of the start residual (nbStart
+                              // blocks) at iteration 0, of k_pcg_res (nbRes) after
     double *partB;            // p.q partials of k_pcg_dir (nbDir blocks)
     PcgScalars *sc;
     const int *halted;
+    const int *outerDone;     // inner solve: the refinement has converged (>= 0)
+    const double *epsDyn;     // inner solve: tolerance set by the refinement test
     double eps;
     int maxIter, nx, ny, N;
     // k_pcg_dir tiling: gx x gy tiles of 256 columns x H rows; remap != 0
     // gives each XCD a contiguous band of tiles (nbDir a multiple of 8)
     int gx, gy, H, remap;
-    int nbDir, nbRes;
+    int nbDir, nbRes, nbStart;
+    int decouple;             // inner solve: skip couplings to identity rows
 };
 
 template <typename T> __global__ void k_pcg_dir(PcgArgs<T> a, int it);
 template <typename T> __global__ void k_pcg_res(PcgArgs<T> a, int it);
+
+// FP64 iterative refinement around the FP32 solve (mixed precision): sweep
+// s forms x_s = x_{s-1} + d (d: the inner solve's correction) and the true
+// residual r = b - A x_s with the FP64 operator; k_refine_test decides
+// whether the solve has converged and sets up the next inner solve.
+struct RefineArgs {
+    const double *cX, *cY, *D, *b;
+    double *xbuf[2];          // x_s lives in xbuf[s & 1]
+    const float *d;           // inner correction
+    float *r32;               // residual, rounded: the next inner right-hand side
+    const float *D32;
+    double *partA;            // r32.r32, r32.z32 partials (inner start)
+    double *partC;            // b.b, r.r partials (FP64)
+    Ctl *ctl;
+    double eps, epsInner;
+    int maxRefine, nx, ny, N;
+    int gx, gy, H, remap, nb, pad;
+};
+
+__global__ void k_refine(RefineArgs a, int s);
+__global__ void k_refine_test(RefineArgs a, int s);
 
 enum SubTable { T_V, T_Z, T_VXP, T_VXM, T_VYP, T_VYM, T_NP, T_NM, T_OP, T_OM };
 

@@ -23,6 +23,10 @@ __global__ void k_step_begin(StepArgs a)
         c->tt += 1;
         c->pcg.iter = 0;
         c->pcg.doneAt = -1;
+        c->ref.doneAt = -1;
+        c->ref.sweep = 0;
+        c->pcgIn.doneAt = -1;
+        c->pcgIn.iter = 0;
         c->pcg.rhoHist[0] = c->pcg.rhoHist[1] = 0.0;
     }
 }
@@ -169,6 +173,15 @@ __device__ inline void store_row(const StepArgs &a, int c, double GXP, double GY
     a.cY[c] = GYP;
     a.D[c] = D;
     a.b[c] = z;
+    if (a.cX32) {
+        // mixed precision: FP32 operator for the inner solve; the refinement
+        // starts from 0 or from the current surface and forms r itself
+        a.cX32[c] = (float)GXP;
+        a.cY32[c] = (float)GYP;
+        a.D32[c] = (float)D;
+        a.x[c] = a.p.warmStart ? a.surf[c] : 0.0;
+        return;
+    }
     a.r[c] = z;
     a.x[c] = 0.0;
     const double invd = 1.0 / fabs(D);

@@ -93,7 +93,8 @@ typedef struct frehd_config {
     /* subgrid bathymetry (subgrid.c; Config.useSubgrid ... useminV)            */
     int32_t useSubgrid;      /* 0: plain grid, 1: subgrid tables                  */
     int32_t Ne;              /* table levels, round((surfmax - surfmin)/dsurf)    */
-    int32_t pad1;
+    int32_t warmStart;       /* FREHD_PREC_MIXED: 1 starts the refinement from the
+                                current surface instead of the reference's x = 0  */
     double  surfmin, dsurf;  /* level k is surfmin + k*dsurf (+ offset)           */
     double  subR;            /* dx / dxf (fine cells per coarse cell side)        */
 } frehd_config;
