@@ -1,8 +1,10 @@
 """Throughput of the semi-implicit free-surface step on one MI355X.
 
 Workload (default): the wetting/drying marsh, 1024 x 1024 cells, dx = 2 m,
-dt = 0.5 s, fBm bed (Hurst 0.7) in [-1.0, +1.5] m, tide 0.6 m / 44712 s on
-the YP row, Manning n ~ U(0.02, 0.04) per cell, one scalar, Jacobi PCG to
+subgrid 8 x 8 pixels per cell (tables generated on the device from the
+pixel bed, 250 levels of 0.01 m), dt = 0.5 s, fBm bed (Hurst 0.7) in
+[-1.0, +1.5] m, tide 0.6 m / 44712 s on the YP row, Manning n ~
+U(0.02, 0.04) per cell, one scalar (0, boundary value 1), Jacobi PCG to
 1e-10.  Forms timed on the same scenario: fp64 (every operation FP64, the
 reference's cold start), mixed_cold (FP32 operator and inner PCG, FP64
 refinement to the same |b - Ax| < eps |b| test, cold start) and mixed_warm
@@ -41,39 +43,44 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scen_builder, n_steps):
-    """Time the reference's own step functions (oracle/_ref) on the host, one
-    core, on a bounded sample: the same grid for n_steps steps.  Falls back to
-    the C restatement (same algorithm, bit-identical results) when the
-    reference build is absent."""
+def cpu_baseline(n, r, budget_s):
+    """Time the reference's own step functions (oracle/_ref, one host core)
+    on a bounded sample of the workload: an n x n marsh from the same
+    generator (fBm bed, r x r subgrid, tables generated on the device and
+    downloaded, uniform Manning n since the reference has no per-cell n),
+    stepping until about budget_s seconds have passed.  Falls back to the C
+    restatement (same algorithm, bit-identical results) when the reference
+    build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
     import pyoracle
-    scen = scen_builder()
+    from frehdc_amd import cases, device
+    scen = cases.marsh_subgrid(n=n, r=r, Nt=4000, device_tables=True, manning_cell=False)
+    d = device.DeviceScenario(scen)
+    tabs = d.tables()
+    d.close()
+    sg = {k: np.ascontiguousarray(v.reshape(-1)) for k, v in tabs.items()}
+    sg["edgeX"], sg["edgeY"] = scen.subgrid["edgeX"], scen.subgrid["edgeY"]
+    scen.subgrid = sg
     kind = "reference"
-    ntide = scen.cfg["tideLocLengthP"]
-    note = ""
     try:
         if not pyoracle.have_reference():
             raise FileNotFoundError
-        if ntide > 1000:
-            # the reference Config holds at most 1000 tidal cells
-            scen.cfg["tideLocLengthP"] = 1000
-            scen.tideLocP = scen.tideLocP[:1000].copy()
-            note = ", tide on the first 1000 cells of the YP row (reference limit)"
-        scen.manningCell = None      # the reference has one global Manning n
         model = pyoracle.Reference(scen)
     except (FileNotFoundError, ValueError, OSError):
         kind = "port"
-        scen = scen_builder()
         model = pyoracle.Oracle(scen)
     model.step(1)                    # first step outside the timing
+    steps = 0
     t0 = time.perf_counter()
-    model.step(n_steps)
+    while time.perf_counter() - t0 < budget_s and steps < 3000:
+        model.step(1)
+        steps += 1
     dt = time.perf_counter() - t0
-    val = scen.N * n_steps / dt
+    val = scen.N * steps / dt
     return {"value": val, "unit": "cell-steps/s", "cores": 1, "kind": kind,
-            "sample": f"{scen.nx}x{scen.ny} marsh, steps 2-{n_steps + 1}, Jacobi PCG, FP64, "
-                      f"{dt:.1f} s on 1 host core{note}"}
+            "sample": f"{n}x{n} marsh, subgrid {r}x{r} (Ne={scen.cfg['Ne']}), steps 2-{steps + 1}, "
+                      f"Jacobi PCG eps 1e-10, FP64, uniform Manning n, {dt:.1f} s on 1 host core"}
 
 
 def run_form(scen, steps, warmup, **over):
@@ -106,7 +113,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--subgrid", type=int, default=8, help="subgrid pixels per cell side (0: plain grid)")
+    ap.add_argument("--cpu-size", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed_cold,mixed_warm",
                     help="precision forms to time (the headline value is the last mixed form)")
@@ -121,11 +130,14 @@ def main():
 
     total = args.warmup + args.steps + 8
 
-    def builder():
-        return cases.marsh(n=args.size, Nt=max(total, 2000))
-
-    scen = builder()
+    t0 = time.perf_counter()
+    if args.subgrid > 0:
+        scen = cases.marsh_subgrid(n=args.size, r=args.subgrid, Nt=max(total, 2000), device_tables=True)
+    else:
+        scen = cases.marsh(n=args.size, Nt=max(total, 2000))
+    log(f"bench: scenario built in {time.perf_counter() - t0:.1f} s")
     N = scen.N
+    grid = f"subgrid {args.subgrid}x{args.subgrid} (Ne={scen.cfg['Ne']})" if args.subgrid > 0 else "plain grid"
     forms = {"fp64": {}, "mixed_cold": {"precision": abi.PREC_MIXED, "warmStart": 0},
              "mixed_warm": {"precision": abi.PREC_MIXED, "warmStart": 1}}
     names = [f for f in args.forms.split(",") if f]
@@ -176,7 +188,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(builder, args.cpu_steps)
+        cpu = cpu_baseline(args.cpu_size, max(args.subgrid, 1), args.cpu_seconds)
         log("cpu baseline:", json.dumps(cpu))
 
     out = {
@@ -184,10 +196,10 @@ def main():
         "value": hr["value"], "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": hr["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
-        "config": {"workload": f"marsh {args.size}x{args.size}, dx=2 m, dt=0.5 s, plain grid, "
-                               f"Jacobi PCG eps=1e-10, form {head}", "cells": N,
-                   "form": head, "forms": forms_out,
-                   "subgrid": "plain grid (subgrid path is parity-tested, not benched here)"},
+        "config": {"workload": f"marsh {args.size}x{args.size}, dx=2 m, {grid}, dt=0.5 s, fBm bed "
+                               f"[-1, 1.5] m, tide 0.6 m / 44712 s, Manning n U(0.02, 0.04) per cell, "
+                               f"one scalar, Jacobi PCG eps=1e-10, form {head}", "cells": N,
+                   "form": head, "forms": forms_out},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -51,7 +51,8 @@ SUB_TABLES = ["V", "Z", "Vxp", "Vxm", "Vyp", "Vym", "Np", "Nm", "Op", "Om",
 
 
 class FrehdSubgrid(C.Structure):
-    _fields_ = [("edgeX", c_double_p), ("edgeY", c_double_p)] + [(n, c_double_p) for n in SUB_TABLES]
+    _fields_ = ([("edgeX", c_double_p), ("edgeY", c_double_p)] + [(n, c_double_p) for n in SUB_TABLES]
+                + [("fine", c_double_p), ("fineR", C.c_int32), ("pad", C.c_int32), ("dxf", C.c_double)])
 
 
 class FrehdIC(C.Structure):
@@ -159,6 +160,7 @@ def load_library(path: str | None = None):
                                     C.POINTER(C.c_int64), c_int32_p], C.c_int),
         "frehd_abi_sizes": ([C.POINTER(C.c_int64)], C.c_int),
         "frehd_dev_set_pcg_tiling": ([V, C.c_int32, C.c_int32], C.c_int),
+        "frehd_dev_download_tables": ([V, C.POINTER(c_double_p)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

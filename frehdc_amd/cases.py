@@ -103,6 +103,10 @@ class Scenario:
             for k in ("edgeX", "edgeY", *abi.SUB_TABLES):
                 a = self.subgrid.get(k)
                 setattr(sg, k, abi.dptr(a) if a is not None else None)
+            if "fine" in self.subgrid:
+                sg.fine = abi.dptr(self.subgrid["fine"])
+                sg.fineR = int(self.subgrid["fineR"])
+                sg.dxf = float(self.subgrid["dxf"])
             self._sg = sg
             ic.sub = C.pointer(sg)
         return ic
@@ -276,17 +280,22 @@ def serving_requests(count=512, rate=20.0, seed=6, lo=64, hi=1024, dx=2.0, dt=0.
 
 
 def with_subgrid(scen: Scenario, fine: np.ndarray, r: int, surfmin: float, surfmax: float,
-                 dsurf: float = 0.01) -> Scenario:
+                 dsurf: float = 0.01, device_tables: bool = False) -> Scenario:
     """Turn a scenario into its subgrid form: `fine` holds the pixel bed
     elevations [ny*r, nx*r]; the coarse bed is each block's minimum
-    (bottom.dat), face bottoms and level tables come from subgrid.tables()."""
+    (bottom.dat), face bottoms and level tables come from subgrid.tables(),
+    or, with device_tables, the fine bathymetry is handed to the device
+    library, which generates the tables itself."""
     from . import subgrid as sgm
     nx, ny = scen.nx, scen.ny
     assert fine.shape == (ny * r, nx * r)
     bottom, edgeX, edgeY = sgm.coarse_geometry(fine, r)
     Ne = int(round((surfmax - surfmin) / dsurf))
     dxf = scen.cfg["dx"] / r
-    tabs = sgm.tables(fine, r, dxf, surfmin, dsurf, Ne)
+    if device_tables:
+        tabs = {"fine": np.ascontiguousarray(fine, dtype=np.float64), "fineR": r, "dxf": dxf}
+    else:
+        tabs = sgm.tables(fine, r, dxf, surfmin, dsurf, Ne)
     scen.bottomZ = np.ascontiguousarray(bottom.reshape(-1))
     scen.cfg["offset"] = offset_for(scen.bottomZ)
     scen.cfg.update(useSubgrid=1, Ne=Ne, surfmin=float(surfmin), dsurf=float(dsurf), subR=float(r))
@@ -325,15 +334,16 @@ def fine_marsh(n, r, seed=2, hurst=0.7, lo=-1.0, hi=1.5):
 
 
 def marsh_subgrid(n=48, r=8, dx=2.0, dt=0.5, Nt=400, amp=0.6, period=TIDE_PERIOD, seed=2,
-                  dsurf=0.01, manning_cell=True):
+                  dsurf=0.01, manning_cell=True, device_tables=False):
     """The marsh workload on the subgrid model: r x r pixels per cell, tables
-    from surfmin = -1.0 m to surfmax = +1.5 m every dsurf."""
+    from surfmin = -1.0 m to surfmax = +1.5 m every dsurf (generated on the
+    device with device_tables)."""
     sc = marsh(n=n, dx=dx, dt=dt, Nt=Nt, seed=seed, amp=amp)
     if period != TIDE_PERIOD:
         sc.tideP[:] = tide_series(Nt, dt, amp, period=period)
     if not manning_cell:
         sc.manningCell = None
     fine = fine_marsh(n, r, seed=seed)
-    sc = with_subgrid(sc, fine, r, -1.0, 1.5, dsurf)
+    sc = with_subgrid(sc, fine, r, -1.0, 1.5, dsurf, device_tables=device_tables)
     sc.name = f"marsh{n}_sub{r}"
     return sc

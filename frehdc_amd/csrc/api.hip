@@ -216,6 +216,12 @@ struct frehd_dev {
     size_t eventNext;
 };
 
+namespace frehd {
+int launch_subgrid_tables(const double *fineDev, double *tabs, int nx, int ny, int r, int Ne,
+                          double surfmin, double dsurf, double dxf, hipStream_t stream);
+int launch_tables_reorder(double *tabs, double *lm, int N, int Ne, int T, int toInterleaved, hipStream_t stream);
+}
+
 namespace {
 
 struct Carve {
@@ -317,7 +323,7 @@ size_t layout(frehd_dev *d, char *base)
         a.edgeX = cv.take<double>(N);
         a.edgeY = cv.take<double>(N);
         a.allSurf = cv.take<double>((size_t)k.Ne);
-        for (int t = 0; t < 10; t++) a.tab[t] = cv.take<double>((size_t)k.Ne * N);
+        a.tabs = cv.take<double>((size_t)k.Ne * N * 10);
     }
     return cv.off + 256;
 }
@@ -523,7 +529,10 @@ int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
     const size_t N = d->g.N, tn = (size_t)k.Ne * N;
     if (!sg || !sg->edgeX || !sg->edgeY) return fail(FREHD_EINVAL, "subgrid tables missing");
     const double *tabs[10] = {sg->V, sg->Z, sg->Vxp, sg->Vxm, sg->Vyp, sg->Vym, sg->Np, sg->Nm, sg->Op, sg->Om};
-    for (int t = 0; t < 10; t++)
+    const bool generate = sg->fine != nullptr && !sg->V;
+    if (generate && !(sg->fineR >= 1 && sg->dxf > 0))
+        return fail(FREHD_EINVAL, "fine bathymetry needs fineR >= 1 and dxf > 0");
+    for (int t = 0; t < 10 && !generate; t++)
         if (!tabs[t]) return fail(FREHD_EINVAL, "subgrid table %d missing", t);
     std::vector<double> ex(N), ey(N), lev(k.Ne);
     for (size_t c = 0; c < N; c++) { ex[c] = sg->edgeX[c] + k.offset; ey[c] = sg->edgeY[c] + k.offset; }
@@ -532,10 +541,31 @@ int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
     if ((rc = upload(d, ex.data(), (void *)a.edgeX, N * sizeof(double)))) return rc;
     if ((rc = upload(d, ey.data(), (void *)a.edgeY, N * sizeof(double)))) return rc;
     if ((rc = upload(d, lev.data(), (void *)a.allSurf, k.Ne * sizeof(double)))) return rc;
-    for (int t = 0; t < 10; t++)
-        if ((rc = upload(d, tabs[t], (void *)a.tab[t], tn * sizeof(double)))) return rc;
-    HIPCHK(hipStreamSynchronize(d->stream));
-    return 0;
+    if (generate) {
+        const size_t nf = N * (size_t)sg->fineR * sg->fineR;
+        double *fine = nullptr;
+        if (hipMalloc(&fine, nf * sizeof(double)) != hipSuccess)
+            return fail(FREHD_ENOMEM, "hipMalloc of the fine bathymetry (%zu bytes) failed", nf * sizeof(double));
+        rc = upload(d, sg->fine, fine, nf * sizeof(double));
+        if (!rc && launch_subgrid_tables(fine, const_cast<double *>(a.tabs), k.nx, k.ny, sg->fineR, k.Ne,
+                                         k.surfmin, k.dsurf, sg->dxf, d->stream))
+            rc = fail(FREHD_EHIP, "table generator launch failed");
+        hipStreamSynchronize(d->stream);
+        hipFree(fine);
+        return rc;
+    }
+    // level-major host tables -> staging -> interleaved
+    double *stage = nullptr;
+    if (hipMalloc(&stage, tn * sizeof(double)) != hipSuccess)
+        return fail(FREHD_ENOMEM, "hipMalloc of the table staging buffer (%zu bytes) failed", tn * sizeof(double));
+    for (int t = 0; t < 10 && !rc; t++) {
+        rc = upload(d, tabs[t], stage, tn * sizeof(double));
+        if (!rc && launch_tables_reorder(const_cast<double *>(a.tabs), stage, (int)N, k.Ne, t, 1, d->stream))
+            rc = fail(FREHD_EHIP, "table reorder launch failed");
+    }
+    hipStreamSynchronize(d->stream);
+    hipFree(stage);
+    return rc;
 }
 
 // ---- kernel launches with optional per-kernel event timing ----------------
@@ -1144,6 +1174,30 @@ int frehd_dev_kernel_times(frehd_dev_t *d, int32_t cap, const char **names, doub
     }
     *n = cnt;
     return 0;
+}
+
+int frehd_dev_download_tables(frehd_dev_t *d, double *const tables[10])
+{
+    if (!d || !tables) return fail(FREHD_EINVAL, "NULL argument");
+    if (d->cfg.useSubgrid != 1) return fail(FREHD_ESTATE, "the handle has no subgrid tables");
+    HIPCHK(hipSetDevice(d->device));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    const size_t tn = (size_t)d->cfg.Ne * d->g.N;
+    double *stage = nullptr;
+    if (hipMalloc(&stage, tn * sizeof(double)) != hipSuccess)
+        return fail(FREHD_ENOMEM, "hipMalloc of the table staging buffer failed");
+    int rc = 0;
+    for (int t = 0; t < 10 && !rc; t++) {
+        if (!tables[t]) continue;
+        if (launch_tables_reorder(const_cast<double *>(d->args.tabs), stage, (int)d->g.N, d->cfg.Ne, t, 0,
+                                  d->stream))
+            rc = fail(FREHD_EHIP, "table reorder launch failed");
+        else if (hipMemcpyAsync(tables[t], stage, tn * sizeof(double), hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
+                 hipStreamSynchronize(d->stream) != hipSuccess)
+            rc = fail(FREHD_EHIP, "table download failed");
+    }
+    hipFree(stage);
+    return rc;
 }
 
 int frehd_dev_set_pcg_tiling(frehd_dev_t *d, int32_t rows_per_tile, int32_t xcd_remap)

@@ -82,13 +82,21 @@ struct StepArgs {
     const int *tideLocP, *tideLocM;
     // subgrid: state (Ng), half-cell values (N), level search index (Ng),
     // face bottoms with offset (N), level elevations (Ne), level tables
-    // V Z Vxp Vxm Vyp Vym Np Nm Op Om, level-major [k*N + c]
+    // V Z Vxp Vxm Vyp Vym Np Nm Op Om interleaved per cell and level:
+    // tabs[(c*Ne + k)*10 + T], so the two levels a cell interpolates between
+    // are 160 contiguous bytes (the reference's files are level-major,
+    // [k*N + c], and are reordered on upload)
     double *V, *Z, *Vx, *Vy, *Zx, *Zy, *Nx, *Oy;
     double *Vxp, *Vxm, *Vyp, *Vym, *Np, *Nm, *Op, *Om;
     int *ind;
     const double *edgeX, *edgeY, *allSurf;
-    const double *tab[10];
+    const double *tabs;
 };
+
+__host__ __device__ inline size_t tab_index(size_t c, int k, int Ne, int T)
+{
+    return (c * (size_t)Ne + (size_t)k) * 10 + (size_t)T;
+}
 
 // One Jacobi-PCG solve over device vectors of type T (double for the FP64
 // path, float for the inner solve of the mixed-precision path).  Operator

@@ -762,6 +762,7 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
     const int N = g.N, Ne = p.Ne;
     const double dsurf = p.dsurf;
     const double *as = a.allSurf;
+    (void)N;
     int ind = a.ind[c];
     while (fabs(as[ind] - h) > 0.5 * dsurf + dsurf / 100) {
         if (h > as[ind]) ind += 1;
@@ -772,15 +773,21 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
     a.ind[c] = ind;
     const int j1 = ind;
     const int j2 = j1 == 0 ? 0 : (j1 == Ne - 1 ? Ne - 1 : (h > as[j1] ? j1 + 1 : j1 - 1));
-    const size_t k1 = (size_t)N * j1 + c, k2 = (size_t)N * j2 + c;
     const double s1 = as[j1], s2 = as[j2];
-    const double *const *t = a.tab;
-    a.V[c] = interp_surf(h, s1, t[T_V][k1], s2, t[T_V][k2]);
-    a.Z[c] = interp_surf(h, s1, t[T_Z][k1], s2, t[T_Z][k2]);
-    double vxp = interp_surf(h, s1, t[T_VXP][k1], s2, t[T_VXP][k2]);
-    double vyp = interp_surf(h, s1, t[T_VYP][k1], s2, t[T_VYP][k2]);
-    double vxm = interp_surf(h, s1, t[T_VXM][k1], s2, t[T_VXM][k2]);
-    double vym = interp_surf(h, s1, t[T_VYM][k1], s2, t[T_VYM][k2]);
+    // the two levels' 10 values each (adjacent in the interleaved layout)
+    double t1[10], t2[10];
+    const double *p1 = a.tabs + tab_index(c, j1, Ne, 0), *p2 = a.tabs + tab_index(c, j2, Ne, 0);
+#pragma unroll
+    for (int q = 0; q < 10; q++) {
+        t1[q] = p1[q];
+        t2[q] = p2[q];
+    }
+    a.V[c] = interp_surf(h, s1, t1[T_V], s2, t2[T_V]);
+    a.Z[c] = interp_surf(h, s1, t1[T_Z], s2, t2[T_Z]);
+    double vxp = interp_surf(h, s1, t1[T_VXP], s2, t2[T_VXP]);
+    double vyp = interp_surf(h, s1, t1[T_VYP], s2, t2[T_VYP]);
+    double vxm = interp_surf(h, s1, t1[T_VXM], s2, t2[T_VXM]);
+    double vym = interp_surf(h, s1, t1[T_VYM], s2, t2[T_VYM]);
     const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
     const double wd[4] = {a.edgeX[c], w < N ? a.edgeX[w] : 0.0, a.edgeY[c], s < N ? a.edgeY[s] : 0.0};
     const int tid[4] = {T_NP, T_NM, T_OP, T_OM};
@@ -788,9 +795,9 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         if (wd[q] > s1 && wd[q] < s2) {
-            f[q] = wd[q] > h ? 0.0 : interp_surf(h, wd[q], 0.0, s2, t[tid[q]][k2]);
+            f[q] = wd[q] > h ? 0.0 : interp_surf(h, wd[q], 0.0, s2, t2[tid[q]]);
         } else {
-            f[q] = interp_surf(h, s1, t[tid[q]][k1], s2, t[tid[q]][k2]);
+            f[q] = interp_surf(h, s1, t1[tid[q]], s2, t2[tid[q]]);
         }
     }
     if (depthOld == 0) { vxp = 0; vyp = 0; vxm = 0; vym = 0; }

@@ -114,6 +114,15 @@ class DeviceScenario:
     def tt(self, value):
         abi.check(self.lib.frehd_dev_set_step_index(self.h, int(value)))
 
+    def tables(self) -> dict:
+        """The handle's subgrid level tables, {name: (Ne, N) array}."""
+        cfg = self._cfg
+        N = cfg.nx * cfg.ny
+        out = {k: np.empty((cfg.Ne, N)) for k in abi.SUB_TABLES[:10]}
+        ptrs = (abi.c_double_p * 10)(*[abi.dptr(out[k]) for k in abi.SUB_TABLES[:10]])
+        abi.check(self.lib.frehd_dev_download_tables(self.h, ptrs))
+        return out
+
     def set_pcg_tiling(self, rows_per_tile=0, xcd_remap=False):
         abi.check(self.lib.frehd_dev_set_pcg_tiling(self.h, int(rows_per_tile), 1 if xcd_remap else 0))
 

@@ -123,6 +123,14 @@ typedef struct frehd_subgrid {
     const double *edgeX, *edgeY;
     const double *V, *Z, *Vxp, *Vxm, *Vyp, *Vym, *Np, *Nm, *Op, *Om;
     const double *minVxp, *minVxm, *minVyp, *minVym;
+    /* Alternative to the ten tables (V ... Om all NULL): the fine bathymetry,
+     * (ny*fineR) rows of (nx*fineR) raw elevations, x fastest, from which the
+     * device generates the tables (the reference's preprocessing script,
+     * script_CreateSubAreaWithBlockV16.m:129-262 with checkBlock = 0).  dxf
+     * is the fine pixel size.  No counterpart in the reference's C code. */
+    const double *fine;
+    int32_t fineR, pad;
+    double dxf;
 } frehd_subgrid;
 
 /* ---- initial condition (the inputs of Init, initialize.c:40-155) ------------
@@ -243,6 +251,12 @@ int frehd_dev_footprint(const frehd_config *cfg, int64_t *bytes);
 int frehd_dev_enable_timing(frehd_dev_t *dev, int32_t on);
 int frehd_dev_kernel_times(frehd_dev_t *dev, int32_t cap, const char **names,
                            double *ms, int64_t *launches, int32_t *n);
+
+/* Download the subgrid level tables of a handle (e.g. the ones generated on
+ * the device): tables[0..9] receive V Z Vxp Vxm Vyp Vym Np Nm Op Om, Ne*N
+ * values each, level-major; NULL entries are skipped.  No counterpart in the
+ * reference (its tables come from files, subgrid.c:2112-2284). */
+int frehd_dev_download_tables(frehd_dev_t *dev, double *const tables[10]);
 
 /* Launch shape of the solver's direction kernel (no counterpart in the
  * reference; a tuning knob for the autotuner): tiles of 256 columns x

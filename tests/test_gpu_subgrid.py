@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from frehdc_amd import abi, cases
+from frehdc_amd.device import DeviceScenario
 import pyoracle as po
 
 from test_gpu_parity import compare
@@ -99,3 +100,36 @@ def test_subgrid_trajectory(dev_mod, oracle_built, sc):
     print("max errors", worst)
     for n, e in worst.items():
         assert e <= 1e-7, f"{n} error {e}"
+
+
+@pytest.mark.parametrize("r", [1, 4, 8])
+def test_device_tables_match_host(r):
+    """Tables generated on the device from the fine bathymetry agree with the
+    host generator (pinned by tests/test_subgrid_tables.py) to rounding: the
+    sums run in a different order."""
+    print(f"case device tables r={r}", flush=True)
+    host = cases.marsh_subgrid(n=24, r=r, Nt=50, manning_cell=False)
+    dev_sc = cases.marsh_subgrid(n=24, r=r, Nt=50, manning_cell=False, device_tables=True)
+    d = DeviceScenario(dev_sc)
+    t = d.tables()
+    d.close()
+    for k, v in t.items():
+        ref = host.subgrid[k].reshape(v.shape)
+        scale = max(1.0, float(np.max(np.abs(ref))))
+        err = float(np.max(np.abs(v - ref)))
+        assert err <= 1e-13 * scale, (k, err)
+
+
+def test_device_tables_run_matches_host_tables():
+    print("case marsh32_sub4 device vs host tables, 100 steps", flush=True)
+    host = cases.marsh_subgrid(n=32, r=4, Nt=200, period=1200.0)
+    dev_sc = cases.marsh_subgrid(n=32, r=4, Nt=200, period=1200.0, device_tables=True)
+    a = DeviceScenario(host)
+    b = DeviceScenario(dev_sc)
+    a.step(100)
+    b.step(100)
+    fa = a.download(["surf", "uuXP", "vvYP"])
+    fb = b.download(["surf", "uuXP", "vvYP"])
+    N = host.N
+    for k in fa:
+        assert np.max(np.abs(fa[k][:N] - fb[k][:N])) <= 1e-9, k
