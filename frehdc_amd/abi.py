@@ -161,6 +161,15 @@ def load_library(path: str | None = None):
         "frehd_abi_sizes": ([C.POINTER(C.c_int64)], C.c_int),
         "frehd_dev_set_pcg_tiling": ([V, C.c_int32, C.c_int32], C.c_int),
         "frehd_dev_download_tables": ([V, C.POINTER(c_double_p)], C.c_int),
+        "frehd_batch_create": ([C.c_int32, C.POINTER(V)], C.c_int),
+        "frehd_batch_add": ([V, V], C.c_int),
+        "frehd_batch_remove": ([V, V], C.c_int),
+        "frehd_batch_size": ([V, c_int32_p], C.c_int),
+        "frehd_batch_step": ([V, C.c_int32], C.c_int),
+        "frehd_batch_enable_timing": ([V, C.c_int32], C.c_int),
+        "frehd_batch_kernel_times": ([V, C.c_int32, C.POINTER(C.c_char_p), c_double_p,
+                                      C.POINTER(C.c_int64), c_int32_p], C.c_int),
+        "frehd_batch_destroy": ([V], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -10,7 +10,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfrehd_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["api.hip", "step_kernels.hip", "pcg_kernels.hip", "subgrid_tables.hip"]
+SOURCES = ["api.hip", "exec.hip", "step_kernels.hip", "pcg_kernels.hip", "subgrid_tables.hip"]
 
 # -ffp-contract=off: no fused multiply-add, so every FP64 expression rounds as
 # the reference's (x86-64, no FMA) does.  Packed FP32 instructions
@@ -26,7 +26,7 @@ FLAGS = [
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]
+    cmd = [HIPCC, *FLAGS, "-I", os.path.join(os.path.dirname(HERE), "include"), "-o", tmp, *srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -1,6 +1,7 @@
-// api.hip -- C ABI of the device step (include/frehd_gpu.h) and its host
-// orchestration: handle creation (Init restated on the host), upload,
-// per-step kernel sequence, PCG convergence control, download, statistics.
+// api.hip -- C ABI of the device step (include/frehd_gpu.h): handle
+// creation (Init restated on the host), device layout, upload, download,
+// statistics, and groups of handles stepped together (frehd_batch_t).  The
+// kernel sequencing itself is in exec.hip.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -13,15 +14,16 @@
 #include <string>
 #include <vector>
 
-#include "../../include/frehd_gpu.h"
 #include "frehd_dev.h"
+#include "handle.h"
 #include "step_kernels.h"
 
 using namespace frehd;
 
 static thread_local std::string g_err;
 
-static int fail(int code, const char *fmt, ...)
+namespace frehd {
+int fail(int code, const char *fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -31,12 +33,7 @@ static int fail(int code, const char *fmt, ...)
     g_err = buf;
     return code;
 }
-
-#define HIPCHK(call)                                                                      \
-    do {                                                                                  \
-        hipError_t e_ = (call);                                                           \
-        if (e_ != hipSuccess) return fail(FREHD_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
-    } while (0)
+}  // namespace frehd
 
 // ---------------------------------------------------------------------------
 // Host restatement of Init (initialize.c:40-155) for a plain grid, np = 1.
@@ -183,39 +180,6 @@ void host_init(HostState &h, const frehd_config &k, const frehd_forcing &f, cons
 
 }  // namespace
 
-// ---------------------------------------------------------------------------
-struct KernelTimer {
-    const char *name;
-    double ms;
-    int64_t launches;
-};
-
-struct frehd_dev {
-    frehd_config cfg;
-    Grid g;
-    int device;
-    hipStream_t stream;
-    char *arena;
-    size_t arenaBytes;
-    StepArgs args;
-    double *depthA, *depthB, *pB;
-    PcgArgs<double> pcg;        // FP64 solve: vectors, tiling, partial counts
-    PcgArgs<float> pcg32;       // mixed precision: inner FP32 solve
-    RefineArgs ref;             // mixed precision: FP64 refinement
-    float *pA32, *pB32;
-    double *xFinal;             // where the last solve left x
-    std::vector<int> mixPlan;   // inner iterations per sweep of the last mixed solve
-    int parity;                 // which depth buffer holds the current depth
-    int lastIters;
-    int32_t *hostFlags;         // pinned: {done, halted}
-    Ctl *ctlHost;               // pinned copy of the control block
-    int timing;
-    std::vector<KernelTimer> timers;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
-    std::vector<hipEvent_t> eventPool;
-    size_t eventNext;
-};
-
 namespace frehd {
 int launch_subgrid_tables(const double *fineDev, double *tabs, int nx, int ny, int r, int Ne,
                           double surfmin, double dsurf, double dxf, hipStream_t stream);
@@ -281,8 +245,8 @@ size_t layout(frehd_dev *d, char *base)
     a.sStride = (int64_t)((Ng + 31) & ~size_t(31));
     a.S = cv.take<double>((size_t)a.sStride * ns);
     a.Sm = cv.take<double>((size_t)a.sStride * ns);
-    double **nArr[] = {&a.uuYP, &a.vvXP, &d->depthA, &d->depthB, &a.cellV, &a.Snew, &a.cX, &a.cY,
-        &a.D, &a.b, &a.x, &a.r, &a.q, &a.pA, &d->pB};
+    double **nArr[] = {&a.uuYP, &a.vvXP, &a.depthA, &a.depthB, &a.cellV, &a.Snew, &a.cX, &a.cY,
+        &a.D, &a.b, &a.x, &a.r, &a.q, &a.pA, &a.pB};
     for (auto **p : nArr) *p = cv.take<double>(N);
     a.manning = nullptr;
     a.rmv = cv.take<uint8_t>(N);
@@ -298,13 +262,22 @@ size_t layout(frehd_dev *d, char *base)
     a.partA = cv.take<double>(2 * std::max(G, maxDir));
     a.partB = cv.take<double>(maxDir);
     a.cX32 = a.cY32 = a.D32 = nullptr;
+    a.xB = nullptr;
     if (k.precision == FREHD_PREC_MIXED) {
-        float **fArr[] = {&a.cX32, &a.cY32, &a.D32, &d->pcg32.r, &d->pcg32.q, &d->pA32, &d->pB32,
-                          &d->pcg32.x};
+        float **fArr[] = {&a.cX32, &a.cY32, &a.D32, &d->pcg32.r, &d->pcg32.q, &d->pcg32.pBuf[0],
+                          &d->pcg32.pBuf[1], &d->pcg32.x};
         for (auto **p : fArr) *p = cv.take<float>(N);
-        d->ref.xbuf[1] = cv.take<double>(N);
+        a.xB = cv.take<double>(N);
         d->ref.partC = cv.take<double>(2 * maxDir);
     }
+    // the handle as a group of one: device copies of its argument blocks
+    // and the flag gather buffer
+    d->self.capacity = 1;
+    d->self.dArgs = cv.take<StepArgs>(1);
+    d->self.dPcg = cv.take<PcgArgs<double>>(1);
+    d->self.dPcg32 = cv.take<PcgArgs<float>>(1);
+    d->self.dRef = cv.take<RefineArgs>(1);
+    d->self.dFlags = cv.take<int32_t>(F_STRIDE);
     a.vpart = cv.take<double>(4 * maxb);
     a.cflpart = cv.take<double>(2 * maxb);
     a.tideP = cv.take<double>(Nt1);
@@ -419,8 +392,23 @@ int upload_series(frehd_dev *d, const double *src, const double *dst)
 
 void set_pcg_args(frehd_dev *d, int H, int remap);
 
+// this handle's launch extents (see StepArgs)
+void fill_extents(frehd_dev *d)
+{
+    StepArgs &a = d->args;
+    const frehd_config &k = d->cfg;
+    a.gx2 = (k.nx + 63) / 64;
+    a.gy2 = (k.ny + 3) / 4;
+    a.nbN = (d->g.N + 255) / 256;
+    a.nbGhost = (2 * k.nx + 2 * k.ny + 255) / 256;
+    const int nt = (k.bcType != 2 ? k.tideLocLengthP : 0) + (k.bcType != 1 ? k.tideLocLengthM : 0);
+    a.nbTide = (nt + 255) / 256;
+    a.nbRes = (int)pcg_blocks(d->g.N);
+}
+
 int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **out, frehd_dev **dd)
 {
+    // (the handle gets a stream of its own, default priority)
     int rc = check_config(cfg);
     if (rc) return rc;
     if (!f || !out) return fail(FREHD_EINVAL, "forcing/out is NULL");
@@ -438,6 +426,7 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     d->g = Grid{cfg->nx, cfg->ny, cfg->nx * cfg->ny, (cfg->nx + 2) * (cfg->ny + 2)};
     HIPCHK(hipGetDevice(&d->device));
     HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    d->ownStream = true;
     size_t bytes = layout(d, nullptr);
     if (f->manningCell) bytes = layout_manning(d, nullptr, bytes);
     d->arenaBytes = bytes;
@@ -451,13 +440,14 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     if (f->manningCell) layout_manning(d, d->arena, used);
     d->args.g = d->g;
     fill_params(d->args.p, d->cfg);
+    fill_extents(d);
     set_pcg_args(d, 0, 0);
-    HIPCHK(hipHostMalloc((void **)&d->hostFlags, 64, hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void **)&d->ctlHost, sizeof(Ctl), hipHostMallocDefault));
+    d->self.device = d->device;
+    d->self.stream = d->stream;
+    d->self.tim = &d->timing;
+    HIPCHK(hipHostMalloc((void **)&d->self.hFlags, sizeof(int32_t) * F_STRIDE, hipHostMallocDefault));
     d->parity = 0;
-    d->lastIters = 16;
-    d->timing = 0;
-    d->eventNext = 0;
+    d->tt = 0;
     // forcing
     const StepArgs &a = d->args;
     if ((rc = upload_series(d, f->tideP, a.tideP))) return rc;
@@ -475,15 +465,11 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     if ((rc = build_flags(d->cfg, *f, flags))) return rc;
     if ((rc = upload(d, flags.data(), d->args.flags, flags.size()))) return rc;
     HIPCHK(hipStreamSynchronize(d->stream));
+    if ((rc = upload_member_args(d))) return rc;
     *dd = d;
     return 0;
 }
 
-void set_depth_pointers(frehd_dev *d)
-{
-    d->args.depthOld = d->parity ? d->depthB : d->depthA;
-    d->args.depthNew = d->parity ? d->depthB : d->depthA;
-}
 
 int upload_state(frehd_dev *d, const frehd_fields &s)
 {
@@ -491,7 +477,7 @@ int upload_state(frehd_dev *d, const frehd_fields &s)
     StepArgs &a = d->args;
     struct { const double *src; double *dst; size_t n; } ng[] = {
         {s.surf, a.surf, Ng}, {s.surfOld, a.surfOld, Ng}, {s.uuXP, a.uuXP, Ng}, {s.vvYP, a.vvYP, Ng},
-        {s.uuYP, a.uuYP, N}, {s.vvXP, a.vvXP, N}, {s.depth, d->depthA, N}, {s.depthXP, a.depthXP, Ng},
+        {s.uuYP, a.uuYP, N}, {s.vvXP, a.vvXP, N}, {s.depth, a.depthA, N}, {s.depthXP, a.depthXP, Ng},
         {s.depthYP, a.depthYP, Ng}, {s.EnXP, a.EnXP, Ng}, {s.EnYP, a.EnYP, Ng}, {s.DragXP, a.DragXP, Ng},
         {s.DragYP, a.DragYP, Ng}, {s.CDXP, a.CDXP, Ng}, {s.CDYP, a.CDYP, Ng}, {s.Fuu, a.Fuu, Ng},
         {s.Fvv, a.Fvv, Ng}, {s.cellV, a.cellV, N}, {s.CFLx, a.CFLx, Ng}, {s.CFLy, a.CFLy, Ng},
@@ -516,7 +502,6 @@ int upload_state(frehd_dev *d, const frehd_fields &s)
         if (s.ind) { int rc = upload(d, s.ind, a.ind, Ng * sizeof(int)); if (rc) return rc; }
     }
     d->parity = 0;
-    set_depth_pointers(d);
     HIPCHK(hipStreamSynchronize(d->stream));
     return 0;
 }
@@ -568,71 +553,6 @@ int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
     return rc;
 }
 
-// ---- kernel launches with optional per-kernel event timing ----------------
-enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_FLUX, K_TIDEV,
-                K_FINISH, K_SCALAR, K_SCALARBC, K_END, K_COMBINE, K_SUBGHOST, K_SUBINIT,
-                K_PQ32, K_UPD32, K_REFINE, K_RTEST, K_COUNT };
-const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble", "k_pcg_dir",
-    "k_pcg_res", "k_surface", "k_limiter", "k_faces", "k_flux", "k_tidal_velocity",
-    "k_finish", "k_scalar", "k_scalar_bc", "k_step_end", "k_combine", "k_sub_ghosts",
-    "k_subarea_init", "k_pcg_dir_f32", "k_pcg_res_f32", "k_refine", "k_refine_test"};
-
-hipEvent_t next_event(frehd_dev *d)
-{
-    if (d->eventNext == d->eventPool.size()) {
-        hipEvent_t e;
-        hipEventCreate(&e);
-        d->eventPool.push_back(e);
-    }
-    return d->eventPool[d->eventNext++];
-}
-
-template <class F>
-int launch(frehd_dev *d, KernelId id, F &&fn)
-{
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (d->timing) { e0 = next_event(d); e1 = next_event(d); hipEventRecord(e0, d->stream); }
-    fn();
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return fail(FREHD_EHIP, "launch of %s failed: %s", kKernelNames[id], hipGetErrorString(err));
-    if (d->timing) {
-        hipEventRecord(e1, d->stream);
-        d->pending.push_back({id, {e0, e1}});
-    }
-    return 0;
-}
-
-int collect_timing(frehd_dev *d)
-{
-    if (!d->timing) return 0;
-    HIPCHK(hipStreamSynchronize(d->stream));
-    for (auto &pe : d->pending) {
-        float ms = 0;
-        hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
-        d->timers[pe.first].ms += ms;
-        d->timers[pe.first].launches += 1;
-    }
-    d->pending.clear();
-    d->eventNext = 0;
-    return 0;
-}
-
-#define LAUNCH(id, kern, grid, block, ...)                                                     \
-    do {                                                                                      \
-        int rc_ = launch(d, id, [&] { hipLaunchKernelGGL(kern, grid, block, 0, d->stream, __VA_ARGS__); }); \
-        if (rc_) return rc_;                                                                  \
-    } while (0)
-
-int enqueue_pcg_iteration(frehd_dev *d, int it)
-{
-    PcgArgs<double> p = d->pcg;
-    p.pOld = (it & 1) ? d->pB : d->args.pA;
-    p.pNew = (it & 1) ? d->args.pA : d->pB;
-    LAUNCH(K_PQ, k_pcg_dir<double>, dim3((unsigned)p.nbDir), dim3(256), p, it);
-    LAUNCH(K_UPD, k_pcg_res<double>, dim3((unsigned)p.nbRes), dim3(256), p, it);
-    return 0;
-}
-
 void set_pcg_args(frehd_dev *d, int H, int remap)
 {
     const StepArgs &a = d->args;
@@ -642,8 +562,8 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     p.D = a.D;
     p.r = a.r;
     p.q = a.q;
-    p.pOld = a.pA;
-    p.pNew = d->pB;
+    p.pBuf[0] = a.pA;
+    p.pBuf[1] = a.pB;
     p.x = a.x;
     p.partA = a.partA;
     p.partB = a.partB;
@@ -665,16 +585,13 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     p.outerDone = nullptr;
     p.epsDyn = nullptr;
     p.decouple = 0;
-    d->xFinal = a.x;
     if (d->cfg.precision != FREHD_PREC_MIXED) return;
-    // inner FP32 solve: same tiling, FP32 operator and vectors (r, q, x = the
-    // correction d were carved by layout()), started by k_refine
+    // inner FP32 solve: same tiling, FP32 operator and vectors (r, q, p, x =
+    // the correction d were carved by layout()), started by k_refine
     PcgArgs<float> &f = d->pcg32;
     f.cX = a.cX32;
     f.cY = a.cY32;
     f.D = a.D32;
-    f.pOld = d->pA32;
-    f.pNew = d->pB32;
     f.partA = a.partA;
     f.partB = a.partB;
     f.sc = &a.ctl->pcgIn;
@@ -692,6 +609,7 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     RefineArgs &r = d->ref;
     r.cX = a.cX; r.cY = a.cY; r.D = a.D; r.b = a.b;
     r.xbuf[0] = a.x;
+    r.xbuf[1] = a.xB;
     r.d = f.x;
     r.r32 = f.r;
     r.D32 = a.D32;
@@ -705,214 +623,14 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     r.nb = t.nb;
 }
 
-int enqueue_inner_iteration(frehd_dev *d, int it)
-{
-    PcgArgs<float> p = d->pcg32;
-    p.pOld = (it & 1) ? d->pB32 : d->pA32;
-    p.pNew = (it & 1) ? d->pA32 : d->pB32;
-    LAUNCH(K_PQ32, k_pcg_dir<float>, dim3((unsigned)p.nbDir), dim3(256), p, it);
-    LAUNCH(K_UPD32, k_pcg_res<float>, dim3((unsigned)p.nbRes), dim3(256), p, it);
-    return 0;
-}
-
-int enqueue_refine(frehd_dev *d, int s)
-{
-    LAUNCH(K_REFINE, k_refine, dim3((unsigned)d->ref.nb), dim3(256), d->ref, s);
-    LAUNCH(K_RTEST, k_refine_test, dim3(1), dim3(256), d->ref, s);
-    return 0;
-}
-
-// one inner solve of n iterations plus the stop test / deferred x update of
-// iteration n (a no-op when it converged earlier)
-int enqueue_inner_solve(frehd_dev *d, int n)
-{
-    for (int it = 0; it < n; it++) {
-        int rc = enqueue_inner_iteration(d, it);
-        if (rc) return rc;
-    }
-    PcgArgs<float> p = d->pcg32;
-    p.pOld = (n & 1) ? d->pB32 : d->pA32;
-    p.pNew = (n & 1) ? d->pA32 : d->pB32;
-    LAUNCH(K_PQ32, k_pcg_dir<float>, dim3((unsigned)p.nbDir), dim3(256), p, n);
-    return 0;
-}
-
-int read_refine(frehd_dev *d)
-{
-    const Ctl *c = d->args.ctl;
-    HIPCHK(hipMemcpyAsync(&d->ctlHost->ref, &c->ref, sizeof(RefineScalars), hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(hipMemcpyAsync(&d->ctlHost->halted, &c->halted, sizeof(int), hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(hipMemcpyAsync(&d->ctlHost->pcgIn, &c->pcgIn, sizeof(PcgScalars), hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(hipStreamSynchronize(d->stream));
-    return 0;
-}
-
-// Mixed-precision solve: sweeps of FP64 refinement around FP32 inner
-// solves.  The launch pattern of the last solve (sweeps and inner
-// iterations per sweep) is enqueued without waiting; launches past
-// convergence exit at once, and an inner solve cut short by the pattern
-// still yields a valid correction (the next sweep recomputes the true
-// residual).  If the pattern ends before the outer test passes, further
-// sweeps follow, each inner solve run to its own stop test.
-int enqueue_mixed_solve(frehd_dev *d)
-{
-    int rc, s = 0;
-    if ((rc = enqueue_refine(d, 0))) return rc;
-    for (size_t k = 0; k < d->mixPlan.size(); k++) {
-        if ((rc = enqueue_inner_solve(d, d->mixPlan[k]))) return rc;
-        if ((rc = enqueue_refine(d, ++s))) return rc;
-    }
-    if ((rc = read_refine(d))) return rc;
-    const Ctl *h = d->ctlHost;
-    while (h->ref.doneAt < 0 && !h->halted) {
-        if (s >= d->cfg.maxRefine) return fail(FREHD_EHIP, "refinement did not report completion");
-        int it = 0, chunk = 8;
-        for (;;) {
-            for (int q = 0; q < chunk; q++, it++)
-                if ((rc = enqueue_inner_iteration(d, it))) return rc;
-            if ((rc = read_refine(d))) return rc;
-            if (h->pcgIn.doneAt >= 0 || h->halted) break;
-            if (it > d->cfg.maxIter + 8) return fail(FREHD_EHIP, "inner PCG did not report completion");
-        }
-        if ((rc = enqueue_refine(d, ++s))) return rc;
-        if ((rc = read_refine(d))) return rc;
-    }
-    if (h->halted) return 0;
-    const int done = h->ref.doneAt;
-    d->xFinal = d->ref.xbuf[done & 1];
-    // pattern for the next step: the inner iterations each sweep used, one
-    // more where the pattern cut a solve short
-    std::vector<int> plan;
-    for (int k = 0; k < done && k < 8; k++) {
-        int n = h->ref.innerIters[k];
-        if (k < (int)d->mixPlan.size() && n >= d->mixPlan[k]) n = d->mixPlan[k] + 2;
-        plan.push_back(std::max(1, n));
-    }
-    d->mixPlan = plan;
-    return 0;
-}
-
-int read_flags(frehd_dev *d)
-{
-    HIPCHK(hipMemcpyAsync(d->hostFlags, &d->args.ctl->pcg.doneAt, sizeof(int), hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(hipMemcpyAsync(d->hostFlags + 1, &d->args.ctl->halted, sizeof(int), hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(hipMemcpyAsync(d->hostFlags + 2, &d->args.ctl->pcg.iter, sizeof(int), hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(hipStreamSynchronize(d->stream));
-    return 0;
-}
-
-int enqueue_phase(frehd_dev *d, int phase)
-{
-    const Grid &g = d->g;
-    StepArgs &a = d->args;
-    dim3 b2(64, 4), g2((g.nx + 63) / 64, (g.ny + 3) / 4);
-    dim3 b1(256), gNg((g.Ng + 255) / 256);
-    double *cur = d->parity ? d->depthB : d->depthA;
-    double *nxt = d->parity ? d->depthA : d->depthB;
-    const bool sub = a.p.useSubgrid == 1;
-    switch (phase) {
-    case 0:   // solveSourceTerm (up to the RHS, which the assembly computes)
-        LAUNCH(K_BEGIN, k_step_begin, dim3(1), dim3(64), a);
-        if (sub) LAUNCH(K_PRED, k_predictor_sub, g2, b2, a);
-        else LAUNCH(K_PRED, k_predictor, g2, b2, a);
-        return 0;
-    case 1: { // solveFreeSurface
-        a.depthOld = cur;   // matrixCoeffSub tests the previous step's depth
-        if (sub) LAUNCH(K_ASM, k_assemble_sub, dim3((unsigned)pcg_blocks(g.N)), dim3(256), a);
-        else LAUNCH(K_ASM, k_assemble, dim3((unsigned)pcg_blocks(g.N)), dim3(256), a);
-        // PCG: launch the previous step's iteration count, then small chunks,
-        // until the device reports convergence (or halt); one iteration per
-        // chunk while timing, so no idle launch enters the averages
-        if (d->cfg.precision == FREHD_PREC_MIXED) {
-            int rc = enqueue_mixed_solve(d);
-            if (rc) return rc;
-            StepArgs as = a;
-            as.x = d->xFinal;
-            LAUNCH(K_SURF, k_surface, gNg, b1, as);
-            return 0;
-        }
-        int it = 0;
-        int chunk = d->timing ? 1 : std::max(1, d->lastIters);
-        for (;;) {
-            for (int q = 0; q < chunk; q++, it++) {
-                int rc = enqueue_pcg_iteration(d, it);
-                if (rc) return rc;
-            }
-            int rc = read_flags(d);
-            if (rc) return rc;
-            if (d->hostFlags[0] >= 0 || d->hostFlags[1]) break;
-            if (it > d->cfg.maxIter + 8) return fail(FREHD_EHIP, "PCG did not report completion");
-            chunk = d->timing ? 1 : 4;
-        }
-        d->lastIters = std::max(1, d->hostFlags[2]);
-        LAUNCH(K_SURF, k_surface, gNg, b1, a);
-        return 0;
-    }
-    case 2: { // updateData (+ the CFL numbers of monitorCFL)
-        a.depthOld = cur;
-        a.depthNew = nxt;
-        int nt = (a.p.bcType != 2 ? a.p.tideLocLengthP : 0) + (a.p.bcType != 1 ? a.p.tideLocLengthM : 0);
-        if (sub) {
-            const dim3 gN((g.N + 255) / 256), gGh((2 * g.nx + 2 * g.ny + 255) / 256);
-            LAUNCH(K_LIM, k_limiter_sub, g2, b2, a);
-            LAUNCH(K_COMBINE, k_combine, gN, b1, a);
-            LAUNCH(K_SUBGHOST, k_sub_ghosts, gGh, b1, a);
-            LAUNCH(K_FACES, k_faces_sub, gNg, b1, a);
-            LAUNCH(K_FLUX, k_flux_sub, gNg, b1, a);
-            if (nt > 0) LAUNCH(K_TIDEV, k_tidal_velocity_sub, dim3((nt + 255) / 256), b1, a);
-            LAUNCH(K_FINISH, k_finish_sub, gNg, b1, a);
-        } else {
-            LAUNCH(K_LIM, k_limiter, g2, b2, a);
-            LAUNCH(K_FACES, k_faces, gNg, b1, a);
-            LAUNCH(K_FLUX, k_flux, gNg, b1, a);
-            if (nt > 0) LAUNCH(K_TIDEV, k_tidal_velocity, dim3((nt + 255) / 256), b1, a);
-            LAUNCH(K_FINISH, k_finish, gNg, b1, a);
-        }
-        d->parity ^= 1;
-        return 0;
-    }
-    case 3:   // scalarTransport
-        a.depthNew = cur;
-        if (a.p.useScalar == 1)
-            for (int q = 0; q < a.p.nScalar; q++) {
-                if (sub) LAUNCH(K_SCALAR, k_scalar_sub, g2, b2, a, q);
-                else LAUNCH(K_SCALAR, k_scalar, g2, b2, a, q);
-                LAUNCH(K_SCALARBC, k_scalar_bc, gNg, b1, a, q);
-            }
-        return 0;
-    case 4:   // monitorCFL's stop test
-        LAUNCH(K_END, k_step_end, dim3(1), dim3(256), a);
-        return 0;
-    default:
-        return fail(FREHD_EINVAL, "phase %d out of range", phase);
-    }
-}
-
 // Init's subgrid part (initialize.c:103-124): level index at round(Ne/2)
 // (integer division first, subgrid.c:2272), computeSubArea, combineSubArea
 int init_subgrid_device(frehd_dev *d)
 {
-    const Grid &g = d->g;
-    StepArgs &a = d->args;
-    std::vector<int> ind(g.Ng, d->cfg.Ne / 2);
-    int rc = upload(d, ind.data(), a.ind, ind.size() * sizeof(int));
+    std::vector<int> ind(d->g.Ng, d->cfg.Ne / 2);
+    int rc = upload(d, ind.data(), d->args.ind, ind.size() * sizeof(int));
     if (rc) return rc;
-    a.depthOld = d->parity ? d->depthB : d->depthA;
-    const dim3 b1(256), gN((g.N + 255) / 256), gGh((2 * g.nx + 2 * g.ny + 255) / 256);
-    LAUNCH(K_SUBINIT, k_subarea_init, gN, b1, a);
-    LAUNCH(K_COMBINE, k_combine, gN, b1, a);
-    LAUNCH(K_SUBGHOST, k_sub_ghosts, gGh, b1, a);
-    HIPCHK(hipStreamSynchronize(d->stream));
-    return 0;
-}
-
-int enqueue_step(frehd_dev *d)
-{
-    for (int ph = 0; ph < 5; ph++) {
-        int rc = enqueue_phase(d, ph);
-        if (rc) return rc;
-    }
-    return 0;
+    return exec_init_subgrid(d->self);
 }
 
 }  // namespace
@@ -1033,17 +751,15 @@ int frehd_dev_step(frehd_dev_t *d, int32_t nsteps)
     if (c.halted) return fail(FREHD_ECFL, "handle halted at step %d: max CFL exceeded 20", c.haltStep);
     if (c.tt + nsteps > d->cfg.Nt)
         return fail(FREHD_EINVAL, "stepping to %d exceeds the forcing series (Nt = %d)", c.tt + nsteps, d->cfg.Nt);
-    if (d->timing && d->timers.empty()) {
-        d->timers.resize(K_COUNT);
-        for (int q = 0; q < K_COUNT; q++) d->timers[q] = KernelTimer{kKernelNames[q], 0.0, 0};
-    }
+    d->tt = c.tt;
+    Exec &e = d->self;
     for (int s = 0; s < nsteps; s++) {
-        int rc = enqueue_step(d);
+        int rc = exec_step(e);
         if (rc) return rc;
-        if (d->timing && (s % 8 == 7)) { rc = collect_timing(d); if (rc) return rc; }
-        if (d->hostFlags[1]) break;
+        if (d->timing.on && (s % 8 == 7)) { rc = exec_collect_timing(e); if (rc) return rc; }
+        if (e.hFlags[F_HALTED]) break;
     }
-    int rc = collect_timing(d);
+    int rc = exec_collect_timing(e);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(d->stream));
     HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
@@ -1056,7 +772,7 @@ int frehd_dev_phase(frehd_dev_t *d, int32_t phase)
 {
     if (!d) return fail(FREHD_EINVAL, "handle is NULL");
     HIPCHK(hipSetDevice(d->device));
-    int rc = enqueue_phase(d, phase);
+    int rc = exec_phase(d->self, phase);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(d->stream));
     return 0;
@@ -1070,7 +786,7 @@ int frehd_dev_download(frehd_dev_t *d, frehd_fields *o)
     const size_t N = d->g.N, Ng = d->g.Ng;
     StepArgs &a = d->args;
     // the current depth is the buffer the last step wrote
-    const double *depthCur = d->parity ? d->depthB : d->depthA;
+    const double *depthCur = d->parity ? a.depthB : a.depthA;
     struct { double *dst; const double *src; size_t n; } tab[] = {
         {o->surf, a.surf, Ng}, {o->surfOld, a.surfOld, Ng}, {o->uuXP, a.uuXP, Ng}, {o->vvYP, a.vvYP, Ng},
         {o->uuYP, a.uuYP, N}, {o->vvXP, a.vvXP, N}, {o->depth, depthCur, N}, {o->depthXP, a.depthXP, Ng},
@@ -1137,13 +853,12 @@ int frehd_dev_set_step_index(frehd_dev_t *d, int32_t tt)
 int frehd_dev_destroy(frehd_dev_t *d)
 {
     if (!d) return 0;
+    if (d->batch) frehd_batch_remove(static_cast<frehd_batch_t *>(d->batch), d);
     hipSetDevice(d->device);
     hipStreamSynchronize(d->stream);
-    for (auto e : d->eventPool) hipEventDestroy(e);
     if (d->arena) hipFree(d->arena);
-    if (d->hostFlags) hipHostFree(d->hostFlags);
-    if (d->ctlHost) hipHostFree(d->ctlHost);
-    hipStreamDestroy(d->stream);
+    if (d->self.hFlags) hipHostFree(d->self.hFlags);
+    if (d->ownStream) hipStreamDestroy(d->stream);
     delete d;
     return 0;
 }
@@ -1151,10 +866,24 @@ int frehd_dev_destroy(frehd_dev_t *d)
 int frehd_dev_enable_timing(frehd_dev_t *d, int32_t on)
 {
     if (!d) return fail(FREHD_EINVAL, "handle is NULL");
-    d->timing = on ? 1 : 0;
-    d->timers.clear();
-    d->pending.clear();
-    d->eventNext = 0;
+    d->timing.on = on ? 1 : 0;
+    d->timing.reset();
+    return 0;
+}
+
+static int kernel_times(const Timing &t, int32_t cap, const char **names, double *ms, int64_t *launches, int32_t *n)
+{
+    int cnt = 0;
+    for (auto &k : t.timers) {
+        if (k.launches == 0) continue;
+        if (cnt < cap) {
+            if (names) names[cnt] = k.name;
+            if (ms) ms[cnt] = k.ms / (double)k.launches;
+            if (launches) launches[cnt] = k.launches;
+        }
+        cnt++;
+    }
+    *n = cnt;
     return 0;
 }
 
@@ -1162,18 +891,7 @@ int frehd_dev_kernel_times(frehd_dev_t *d, int32_t cap, const char **names, doub
                            int32_t *n)
 {
     if (!d || !n) return fail(FREHD_EINVAL, "NULL argument");
-    int cnt = 0;
-    for (auto &t : d->timers) {
-        if (t.launches == 0) continue;
-        if (cnt < cap) {
-            if (names) names[cnt] = t.name;
-            if (ms) ms[cnt] = t.ms / (double)t.launches;
-            if (launches) launches[cnt] = t.launches;
-        }
-        cnt++;
-    }
-    *n = cnt;
-    return 0;
+    return kernel_times(d->timing, cap, names, ms, launches, n);
 }
 
 int frehd_dev_download_tables(frehd_dev_t *d, double *const tables[10])
@@ -1207,7 +925,140 @@ int frehd_dev_set_pcg_tiling(frehd_dev_t *d, int32_t rows_per_tile, int32_t xcd_
         return fail(FREHD_EINVAL, "rows_per_tile %d out of range [0, %d]", rows_per_tile, d->g.ny);
     if ((size_t)dir_tiling(d->g.nx, d->g.ny, rows_per_tile, xcd_remap).nb > max_dir_blocks(d->g.nx, d->g.ny))
         return fail(FREHD_EINVAL, "too many tiles");
+    HIPCHK(hipSetDevice(d->device));
+    HIPCHK(hipStreamSynchronize(d->stream));
     set_pcg_args(d, rows_per_tile, xcd_remap ? 1 : 0);
+    return upload_member_args(d);
+}
+
+// ---- batches ----------------------------------------------------------------
+int frehd_batch_create(int32_t capacity, frehd_batch_t **out)
+{
+    if (!out) return fail(FREHD_EINVAL, "out is NULL");
+    if (capacity < 1 || capacity > 65535) return fail(FREHD_EINVAL, "capacity %d outside [1, 65535]", capacity);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FREHD_EHIP, "no HIP device visible");
+    frehd_batch *b = new frehd_batch();
+    Exec &e = b->e;
+    e.capacity = capacity;
+    HIPCHK(hipGetDevice(&e.device));
+    HIPCHK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+    const size_t n = (size_t)capacity;
+    if (hipMalloc(&e.dArgs, n * sizeof(StepArgs)) != hipSuccess ||
+        hipMalloc(&e.dPcg, n * sizeof(PcgArgs<double>)) != hipSuccess ||
+        hipMalloc(&e.dPcg32, n * sizeof(PcgArgs<float>)) != hipSuccess ||
+        hipMalloc(&e.dRef, n * sizeof(RefineArgs)) != hipSuccess ||
+        hipMalloc(&e.dFlags, n * F_STRIDE * sizeof(int32_t)) != hipSuccess ||
+        hipHostMalloc((void **)&e.hFlags, n * F_STRIDE * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+        frehd_batch_destroy(b);
+        return fail(FREHD_ENOMEM, "batch allocation failed");
+    }
+    e.tim = &b->timing;
+    *out = b;
+    return 0;
+}
+
+int frehd_batch_add(frehd_batch_t *b, frehd_dev_t *d)
+{
+    if (!b || !d) return fail(FREHD_EINVAL, "NULL argument");
+    if (d->batch) return fail(FREHD_ESTATE, "the handle is already in a batch");
+    if (d->device != b->e.device) return fail(FREHD_EINVAL, "the handle lives on another device");
+    if ((int)b->e.members.size() >= b->e.capacity) return fail(FREHD_EBUDGET, "batch full (%d)", b->e.capacity);
+    HIPCHK(hipSetDevice(d->device));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    b->e.members.push_back(d);
+    int rc = exec_refresh(b->e);
+    if (rc) {
+        b->e.members.pop_back();
+        exec_refresh(b->e);
+        return rc;
+    }
+    d->batch = b;
+    return 0;
+}
+
+int frehd_batch_remove(frehd_batch_t *b, frehd_dev_t *d)
+{
+    if (!b || !d) return fail(FREHD_EINVAL, "NULL argument");
+    auto &m = b->e.members;
+    auto it = std::find(m.begin(), m.end(), static_cast<frehd_dev_impl *>(d));
+    if (it == m.end()) return fail(FREHD_EINVAL, "the handle is not in this batch");
+    HIPCHK(hipStreamSynchronize(b->e.stream));
+    m.erase(it);
+    d->batch = nullptr;
+    return exec_refresh(b->e);
+}
+
+int frehd_batch_size(frehd_batch_t *b, int32_t *n)
+{
+    if (!b || !n) return fail(FREHD_EINVAL, "NULL argument");
+    *n = (int32_t)b->e.members.size();
+    return 0;
+}
+
+int frehd_batch_step(frehd_batch_t *b, int32_t nsteps)
+{
+    if (!b) return fail(FREHD_EINVAL, "batch is NULL");
+    if (nsteps < 0) return fail(FREHD_EINVAL, "nsteps < 0");
+    Exec &e = b->e;
+    if (e.members.empty() || nsteps == 0) return 0;
+    HIPCHK(hipSetDevice(e.device));
+    for (auto *d : e.members) {
+        Ctl c;
+        HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+        if (!c.halted && c.tt + nsteps > d->cfg.Nt)
+            return fail(FREHD_EINVAL, "a member steps to %d beyond its forcing series (Nt = %d)", c.tt + nsteps,
+                        d->cfg.Nt);
+        d->tt = c.tt;
+    }
+    for (int s = 0; s < nsteps; s++) {
+        int rc = exec_step(e);
+        if (rc) return rc;
+        if (b->timing.on && (s % 8 == 7)) { rc = exec_collect_timing(e); if (rc) return rc; }
+    }
+    int rc = exec_collect_timing(e);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(e.stream));
+    int halted = 0;
+    for (auto *d : e.members) {
+        Ctl c;
+        HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+        halted += c.halted != 0;
+    }
+    if (halted) return fail(FREHD_ECFL, "%d batch member(s) halted: max CFL exceeded 20", halted);
+    return 0;
+}
+
+int frehd_batch_enable_timing(frehd_batch_t *b, int32_t on)
+{
+    if (!b) return fail(FREHD_EINVAL, "batch is NULL");
+    b->timing.on = on ? 1 : 0;
+    b->timing.reset();
+    return 0;
+}
+
+int frehd_batch_kernel_times(frehd_batch_t *b, int32_t cap, const char **names, double *ms, int64_t *launches,
+                             int32_t *n)
+{
+    if (!b || !n) return fail(FREHD_EINVAL, "NULL argument");
+    return kernel_times(b->timing, cap, names, ms, launches, n);
+}
+
+int frehd_batch_destroy(frehd_batch_t *b)
+{
+    if (!b) return 0;
+    Exec &e = b->e;
+    hipSetDevice(e.device);
+    if (e.stream) hipStreamSynchronize(e.stream);
+    for (auto *d : e.members) d->batch = nullptr;
+    if (e.dArgs) hipFree(e.dArgs);
+    if (e.dPcg) hipFree(e.dPcg);
+    if (e.dPcg32) hipFree(e.dPcg32);
+    if (e.dRef) hipFree(e.dRef);
+    if (e.dFlags) hipFree(e.dFlags);
+    if (e.hFlags) hipHostFree(e.hFlags);
+    if (e.stream) hipStreamDestroy(e.stream);
+    delete b;
     return 0;
 }
 

@@ -1,0 +1,384 @@
+// exec.hip -- kernel sequencing for a group of scenarios (Exec): the phases
+// of a time step, the solver's convergence loop, the mixed-precision
+// refinement loop, per-kernel timing.  Every launch covers all members of
+// the group (grid z = member); see handle.h.
+#include <algorithm>
+#include <cstring>
+
+#include "handle.h"
+
+namespace frehd {
+
+const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble", "k_pcg_dir",
+    "k_pcg_res", "k_surface", "k_limiter", "k_faces", "k_flux", "k_tidal_velocity",
+    "k_finish", "k_scalar", "k_scalar_bc", "k_step_end", "k_combine", "k_sub_ghosts",
+    "k_subarea_init", "k_pcg_dir_f32", "k_pcg_res_f32", "k_refine", "k_refine_test", "k_gather_flags"};
+
+void Timing::reset()
+{
+    timers.clear();
+    pending.clear();
+    next = 0;
+}
+
+Timing::~Timing()
+{
+    for (auto e : pool) hipEventDestroy(e);
+}
+
+// the solver flags of every member, gathered into one array so that one copy
+// brings them to the host
+__global__ void k_gather_flags(const StepArgs *__restrict__ A, int count, int32_t *out)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= count) return;
+    const Ctl *c = A[m].ctl;
+    int32_t *o = out + (size_t)m * F_STRIDE;
+    o[F_DONE] = c->pcg.doneAt;
+    o[F_HALTED] = c->halted;
+    o[F_ITER] = c->pcg.iter;
+    o[F_REFDONE] = c->ref.doneAt;
+    o[F_INDONE] = c->pcgIn.doneAt;
+    o[F_INITER] = c->pcgIn.iter;
+    o[F_TT] = c->tt;
+    o[F_PAD] = 0;
+    for (int k = 0; k < 8; k++) o[F_INNER0 + k] = c->ref.innerIters[k];
+}
+
+namespace {
+
+hipEvent_t next_event(Timing &t)
+{
+    if (t.next == t.pool.size()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        t.pool.push_back(e);
+    }
+    return t.pool[t.next++];
+}
+
+template <class F>
+int launch(Exec &e, KernelId id, F &&fn)
+{
+    Timing *t = e.tim;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (t && t->on) {
+        e0 = next_event(*t);
+        e1 = next_event(*t);
+        hipEventRecord(e0, e.stream);
+    }
+    fn();
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return fail(FREHD_EHIP, "launch of %s failed: %s", kKernelNames[id], hipGetErrorString(err));
+    if (t && t->on) {
+        hipEventRecord(e1, e.stream);
+        t->pending.push_back({id, {e0, e1}});
+    }
+    return 0;
+}
+
+#define LAUNCH(id, kern, grid, block, ...)                                                            \
+    do {                                                                                             \
+        int rc_ = launch(e, id, [&] { hipLaunchKernelGGL(kern, grid, block, 0, e.stream, __VA_ARGS__); }); \
+        if (rc_) return rc_;                                                                         \
+    } while (0)
+
+inline dim3 grid1(int nb, int n) { return dim3((unsigned)std::max(nb, 1), 1, (unsigned)n); }
+
+int gather(Exec &e)
+{
+    LAUNCH(K_GATHER, k_gather_flags, dim3((unsigned)((e.count + 63) / 64)), dim3(64), e.dArgs, e.count, e.dFlags);
+    HIPCHK(hipMemcpyAsync(e.hFlags, e.dFlags, sizeof(int32_t) * F_STRIDE * e.count, hipMemcpyDeviceToHost, e.stream));
+    HIPCHK(hipStreamSynchronize(e.stream));
+    return 0;
+}
+
+inline const int32_t *flags(const Exec &e, int m) { return e.hFlags + (size_t)m * F_STRIDE; }
+
+int enqueue_pcg_iteration(Exec &e, int it)
+{
+    LAUNCH(K_PQ, k_pcg_dir<double>, grid1(e.ext.nbDir, e.count), dim3(256), e.dPcg, it);
+    LAUNCH(K_UPD, k_pcg_res<double>, grid1(e.ext.nbRes, e.count), dim3(256), e.dPcg, it);
+    return 0;
+}
+
+// FP64 PCG: the previous step's iteration count, then small chunks, until
+// every member reports convergence (or a halt); one iteration per chunk
+// while timing, so no idle launch enters the averages
+int solve_fp64(Exec &e)
+{
+    const bool timing = e.tim && e.tim->on;
+    int it = 0;
+    int chunk = timing ? 1 : std::max(1, e.lastIters);
+    for (;;) {
+        for (int q = 0; q < chunk; q++, it++) {
+            int rc = enqueue_pcg_iteration(e, it);
+            if (rc) return rc;
+        }
+        int rc = gather(e);
+        if (rc) return rc;
+        bool all = true;
+        for (int m = 0; m < e.count; m++)
+            if (flags(e, m)[F_DONE] < 0 && !flags(e, m)[F_HALTED]) all = false;
+        if (all) break;
+        if (it > e.maxIter + 8) return fail(FREHD_EHIP, "PCG did not report completion");
+        chunk = timing ? 1 : 4;
+    }
+    int last = 1;
+    for (int m = 0; m < e.count; m++)
+        if (!flags(e, m)[F_HALTED]) last = std::max(last, (int)flags(e, m)[F_ITER]);
+    e.lastIters = last;
+    return 0;
+}
+
+int enqueue_inner_iteration(Exec &e, int it)
+{
+    LAUNCH(K_PQ32, k_pcg_dir<float>, grid1(e.ext.nbDir, e.count), dim3(256), e.dPcg32, it);
+    LAUNCH(K_UPD32, k_pcg_res<float>, grid1(e.ext.nbRes, e.count), dim3(256), e.dPcg32, it);
+    return 0;
+}
+
+int enqueue_refine(Exec &e, int s)
+{
+    LAUNCH(K_REFINE, k_refine, grid1(e.ext.nbRef, e.count), dim3(256), e.dRef, s);
+    LAUNCH(K_RTEST, k_refine_test, dim3(1, 1, (unsigned)e.count), dim3(256), e.dRef, s);
+    return 0;
+}
+
+// one inner solve of n iterations plus the stop test / deferred x update of
+// iteration n (a no-op when it converged earlier)
+int enqueue_inner_solve(Exec &e, int n)
+{
+    for (int it = 0; it < n; it++) {
+        int rc = enqueue_inner_iteration(e, it);
+        if (rc) return rc;
+    }
+    LAUNCH(K_PQ32, k_pcg_dir<float>, grid1(e.ext.nbDir, e.count), dim3(256), e.dPcg32, n);
+    return 0;
+}
+
+// Mixed-precision solve: sweeps of FP64 refinement around FP32 inner
+// solves.  The launch pattern of the last solve (sweeps and inner
+// iterations per sweep, the maximum over the members) is enqueued without
+// waiting; launches past a member's convergence exit at once, and an inner
+// solve cut short by the pattern still yields a valid correction (the next
+// sweep recomputes the true residual).  If the pattern ends before every
+// member's outer test passes, further sweeps follow, each inner solve run to
+// its own stop test.
+int solve_mixed(Exec &e)
+{
+    int rc, s = 0;
+    if ((rc = enqueue_refine(e, 0))) return rc;
+    for (size_t k = 0; k < e.mixPlan.size(); k++) {
+        if ((rc = enqueue_inner_solve(e, e.mixPlan[k]))) return rc;
+        if ((rc = enqueue_refine(e, ++s))) return rc;
+    }
+    if ((rc = gather(e))) return rc;
+    auto outer_open = [&](int m) { return flags(e, m)[F_REFDONE] < 0 && !flags(e, m)[F_HALTED]; };
+    auto any_open = [&] {
+        for (int m = 0; m < e.count; m++)
+            if (outer_open(m)) return true;
+        return false;
+    };
+    while (any_open()) {
+        if (s >= e.maxRefine) return fail(FREHD_EHIP, "refinement did not report completion");
+        int it = 0;
+        for (;;) {
+            for (int q = 0; q < 8; q++, it++)
+                if ((rc = enqueue_inner_iteration(e, it))) return rc;
+            if ((rc = gather(e))) return rc;
+            bool all = true;
+            for (int m = 0; m < e.count; m++)
+                if (outer_open(m) && flags(e, m)[F_INDONE] < 0) all = false;
+            if (all) break;
+            if (it > e.maxIter + 8) return fail(FREHD_EHIP, "inner PCG did not report completion");
+        }
+        if ((rc = enqueue_refine(e, ++s))) return rc;
+        if ((rc = gather(e))) return rc;
+    }
+    // pattern for the next step: per sweep, the most inner iterations any
+    // member used, two more where the pattern cut a solve short
+    int sweeps = 0;
+    for (int m = 0; m < e.count; m++)
+        if (!flags(e, m)[F_HALTED]) sweeps = std::max(sweeps, (int)flags(e, m)[F_REFDONE]);
+    std::vector<int> plan;
+    for (int k = 0; k < sweeps && k < 8; k++) {
+        int n = 1;
+        for (int m = 0; m < e.count; m++)
+            if (!flags(e, m)[F_HALTED] && flags(e, m)[F_REFDONE] > k) n = std::max(n, (int)flags(e, m)[F_INNER0 + k]);
+        if (k < (int)e.mixPlan.size() && n >= e.mixPlan[k]) n = e.mixPlan[k] + 2;
+        plan.push_back(n);
+    }
+    e.mixPlan = plan;
+    return 0;
+}
+
+}  // namespace
+
+int exec_collect_timing(Exec &e)
+{
+    Timing *t = e.tim;
+    if (!t || !t->on) return 0;
+    HIPCHK(hipStreamSynchronize(e.stream));
+    if (t->timers.empty()) {
+        t->timers.resize(K_COUNT);
+        for (int q = 0; q < K_COUNT; q++) t->timers[q] = KernelTimer{kKernelNames[q], 0.0, 0};
+    }
+    for (auto &pe : t->pending) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
+        t->timers[pe.first].ms += ms;
+        t->timers[pe.first].launches += 1;
+    }
+    t->pending.clear();
+    t->next = 0;
+    return 0;
+}
+
+int exec_phase(Exec &e, int phase)
+{
+    const int n = e.count;
+    if (n == 0) return 0;
+    const Extents &x = e.ext;
+    const dim3 b2(64, 4), g2((unsigned)x.gx2, (unsigned)x.gy2, (unsigned)n), b1(256);
+    switch (phase) {
+    case 0:   // solveSourceTerm (up to the RHS, which the assembly computes)
+        LAUNCH(K_BEGIN, k_step_begin, dim3(1, 1, n), dim3(64), e.dArgs);
+        if (e.sub) LAUNCH(K_PRED, k_predictor_sub, g2, b2, e.dArgs);
+        else LAUNCH(K_PRED, k_predictor, g2, b2, e.dArgs);
+        return 0;
+    case 1: { // solveFreeSurface
+        if (e.sub) LAUNCH(K_ASM, k_assemble_sub, grid1(x.nbRes, n), b1, e.dArgs);
+        else LAUNCH(K_ASM, k_assemble, grid1(x.nbRes, n), b1, e.dArgs);
+        int rc = e.mixed ? solve_mixed(e) : solve_fp64(e);
+        if (rc) return rc;
+        LAUNCH(K_SURF, k_surface, grid1(x.nb1, n), b1, e.dArgs);
+        return 0;
+    }
+    case 2: { // updateData (+ the CFL numbers of monitorCFL)
+        if (e.sub) {
+            LAUNCH(K_LIM, k_limiter_sub, g2, b2, e.dArgs);
+            LAUNCH(K_COMBINE, k_combine, grid1(x.nbN, n), b1, e.dArgs);
+            LAUNCH(K_SUBGHOST, k_sub_ghosts, grid1(x.nbGhost, n), b1, e.dArgs);
+            LAUNCH(K_FACES, k_faces_sub, grid1(x.nb1, n), b1, e.dArgs);
+            LAUNCH(K_FLUX, k_flux_sub, grid1(x.nb1, n), b1, e.dArgs);
+            if (x.nbTide > 0) LAUNCH(K_TIDEV, k_tidal_velocity_sub, grid1(x.nbTide, n), b1, e.dArgs);
+            LAUNCH(K_FINISH, k_finish_sub, grid1(x.nb1, n), b1, e.dArgs);
+        } else {
+            LAUNCH(K_LIM, k_limiter, g2, b2, e.dArgs);
+            LAUNCH(K_FACES, k_faces, grid1(x.nb1, n), b1, e.dArgs);
+            LAUNCH(K_FLUX, k_flux, grid1(x.nb1, n), b1, e.dArgs);
+            if (x.nbTide > 0) LAUNCH(K_TIDEV, k_tidal_velocity, grid1(x.nbTide, n), b1, e.dArgs);
+            LAUNCH(K_FINISH, k_finish, grid1(x.nb1, n), b1, e.dArgs);
+        }
+        // the new depth is now the current one for downloads; the device
+        // flips its own parity at the end of the step
+        for (auto *m : e.members) m->parity ^= 1;
+        return 0;
+    }
+    case 3:   // scalarTransport (members without scalars skip inside)
+        for (int q = 0; q < e.nScalarMax; q++) {
+            if (e.sub) LAUNCH(K_SCALAR, k_scalar_sub, g2, b2, e.dArgs, q);
+            else LAUNCH(K_SCALAR, k_scalar, g2, b2, e.dArgs, q);
+            LAUNCH(K_SCALARBC, k_scalar_bc, grid1(x.nb1, n), b1, e.dArgs, q);
+        }
+        return 0;
+    case 4:   // monitorCFL's stop test
+        LAUNCH(K_END, k_step_end, dim3(1, 1, n), dim3(256), e.dArgs);
+        for (auto *m : e.members) m->tt += 1;
+        return 0;
+    default:
+        return fail(FREHD_EINVAL, "phase %d out of range", phase);
+    }
+}
+
+int exec_step(Exec &e)
+{
+    for (int ph = 0; ph < 5; ph++) {
+        int rc = exec_phase(e, ph);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int exec_init_subgrid(Exec &e)
+{
+    const Extents &x = e.ext;
+    const int n = e.count;
+    LAUNCH(K_SUBINIT, k_subarea_init, grid1(x.nbN, n), dim3(256), e.dArgs);
+    LAUNCH(K_COMBINE, k_combine, grid1(x.nbN, n), dim3(256), e.dArgs);
+    LAUNCH(K_SUBGHOST, k_sub_ghosts, grid1(x.nbGhost, n), dim3(256), e.dArgs);
+    HIPCHK(hipStreamSynchronize(e.stream));
+    return 0;
+}
+
+Extents member_extents(const frehd_dev_impl *d)
+{
+    const StepArgs &a = d->args;
+    Extents x;
+    x.gx2 = a.gx2;
+    x.gy2 = a.gy2;
+    x.nb1 = a.nb1;
+    x.nbN = a.nbN;
+    x.nbGhost = a.nbGhost;
+    x.nbTide = a.nbTide;
+    x.nbRes = a.nbRes;
+    x.nbDir = d->pcg.nbDir;
+    x.nbRef = d->cfg.precision == FREHD_PREC_MIXED ? d->ref.nb : 0;
+    return x;
+}
+
+static void merge(Exec &e, const frehd_dev_impl *d, bool first)
+{
+    const Extents m = member_extents(d);
+    Extents &x = e.ext;
+    if (first) {
+        x = m;
+        e.maxIter = d->cfg.maxIter;
+        e.maxRefine = d->cfg.maxRefine;
+        e.nScalarMax = d->args.p.nScalar;
+        return;
+    }
+    x.gx2 = std::max(x.gx2, m.gx2);
+    x.gy2 = std::max(x.gy2, m.gy2);
+    x.nb1 = std::max(x.nb1, m.nb1);
+    x.nbN = std::max(x.nbN, m.nbN);
+    x.nbGhost = std::max(x.nbGhost, m.nbGhost);
+    x.nbTide = std::max(x.nbTide, m.nbTide);
+    x.nbRes = std::max(x.nbRes, m.nbRes);
+    x.nbDir = std::max(x.nbDir, m.nbDir);
+    x.nbRef = std::max(x.nbRef, m.nbRef);
+    e.maxIter = std::max(e.maxIter, d->cfg.maxIter);
+    e.maxRefine = std::max(e.maxRefine, d->cfg.maxRefine);
+    e.nScalarMax = std::max(e.nScalarMax, d->args.p.nScalar);
+}
+
+int exec_refresh(Exec &e)
+{
+    e.count = (int)e.members.size();
+    if (e.count > e.capacity) return fail(FREHD_EINVAL, "group holds %d members, capacity %d", e.count, e.capacity);
+    for (int m = 0; m < e.count; m++) {
+        const frehd_dev_impl *d = e.members[m];
+        if (m == 0) {
+            e.sub = d->cfg.useSubgrid == 1;
+            e.mixed = d->cfg.precision == FREHD_PREC_MIXED;
+        } else if ((d->cfg.useSubgrid == 1) != (bool)e.sub || (d->cfg.precision == FREHD_PREC_MIXED) != (bool)e.mixed) {
+            return fail(FREHD_EINVAL, "a group's members must share useSubgrid and precision");
+        }
+        merge(e, d, m == 0);
+        HIPCHK(hipMemcpyAsync(e.dArgs + m, &d->args, sizeof(StepArgs), hipMemcpyHostToDevice, e.stream));
+        HIPCHK(hipMemcpyAsync(e.dPcg + m, &d->pcg, sizeof(PcgArgs<double>), hipMemcpyHostToDevice, e.stream));
+        HIPCHK(hipMemcpyAsync(e.dPcg32 + m, &d->pcg32, sizeof(PcgArgs<float>), hipMemcpyHostToDevice, e.stream));
+        HIPCHK(hipMemcpyAsync(e.dRef + m, &d->ref, sizeof(RefineArgs), hipMemcpyHostToDevice, e.stream));
+    }
+    HIPCHK(hipStreamSynchronize(e.stream));
+    return 0;
+}
+
+int upload_member_args(frehd_dev_impl *d)
+{
+    d->self.members.assign(1, d);
+    return exec_refresh(d->self);
+}
+
+}  // namespace frehd

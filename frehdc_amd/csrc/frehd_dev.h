@@ -138,12 +138,12 @@ __device__ inline double block_max(double v, double *lds)
 // release/acquire fences write back and invalidate the per-XCD L2s; kernel
 // boundaries give the visibility for free.)
 template <int NV>
-__device__ inline void store_partials(double *part, const double (&v)[NV])
+__device__ inline void store_partials(double *part, int nparts, const double (&v)[NV])
 {
-    // v already block-reduced into thread 0
+    // v already block-reduced into thread 0; nparts = the member's block count
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int q = 0; q < NV; q++) part[(size_t)q * gridDim.x + blockIdx.x] = v[q];
+        for (int q = 0; q < NV; q++) part[(size_t)q * nparts + blockIdx.x] = v[q];
 }
 
 template <int NV>

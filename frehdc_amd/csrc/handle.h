@@ -1,0 +1,123 @@
+// handle.h -- host-side state of the device library (internal; the public
+// interface is include/frehd_gpu.h).
+//
+// frehd_dev   one scenario resident on the device: its arena, the host copy
+//             of its kernel argument blocks and their device copies.
+// Exec        a group of scenarios advanced by the same kernel launches: each
+//             kernel reads its scenario's argument block from a device array
+//             indexed by blockIdx.z and is sized for the group's largest
+//             member (smaller members' surplus blocks exit at once).  A
+//             handle alone is a group of one; frehd_batch_t is a group of
+//             many.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "frehd_gpu.h"
+#include "step_kernels.h"
+
+namespace frehd {
+
+int fail(int code, const char *fmt, ...);
+
+#define HIPCHK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) return ::frehd::fail(FREHD_EHIP, "%s failed: %s", #call,    \
+                                                   hipGetErrorString(e_));                \
+    } while (0)
+
+enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_FLUX, K_TIDEV,
+                K_FINISH, K_SCALAR, K_SCALARBC, K_END, K_COMBINE, K_SUBGHOST, K_SUBINIT,
+                K_PQ32, K_UPD32, K_REFINE, K_RTEST, K_GATHER, K_COUNT };
+extern const char *kKernelNames[K_COUNT];
+
+struct KernelTimer {
+    const char *name;
+    double ms;
+    int64_t launches;
+};
+
+// per-kernel event timing on the group's stream
+struct Timing {
+    int on = 0;
+    std::vector<KernelTimer> timers;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> pool;
+    size_t next = 0;
+    void reset();
+    ~Timing();
+};
+
+// launch extents of a group: the maximum over its members
+struct Extents {
+    int gx2, gy2, nb1, nbN, nbGhost, nbTide, nbRes, nbDir, nbRef;
+};
+
+// flags gathered from every member's control block after a solver chunk
+enum { F_DONE, F_HALTED, F_ITER, F_REFDONE, F_INDONE, F_INITER, F_TT, F_PAD, F_INNER0, F_STRIDE = 16 };
+
+struct frehd_dev_impl;
+
+struct Exec {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int count = 0, capacity = 0;
+    StepArgs *dArgs = nullptr;          // device arrays of member argument blocks
+    PcgArgs<double> *dPcg = nullptr;
+    PcgArgs<float> *dPcg32 = nullptr;
+    RefineArgs *dRef = nullptr;
+    int32_t *dFlags = nullptr, *hFlags = nullptr;   // F_STRIDE ints per member
+    Extents ext{};
+    int sub = 0, mixed = 0, nScalarMax = 0, maxIter = 0, maxRefine = 0;
+    int lastIters = 16;
+    std::vector<int> mixPlan;           // inner iterations per sweep of the last mixed solve
+    Timing *tim = nullptr;
+    std::vector<frehd_dev_impl *> members;
+};
+
+struct frehd_dev_impl {
+    frehd_config cfg;
+    Grid g;
+    int device;
+    hipStream_t stream;
+    bool ownStream;
+    char *arena;
+    size_t arenaBytes;
+    StepArgs args;
+    PcgArgs<double> pcg;                // FP64 solve
+    PcgArgs<float> pcg32;               // mixed precision: inner FP32 solve
+    RefineArgs ref;                     // mixed precision: FP64 refinement
+    int parity;                         // host view: buffer holding the current depth
+    int tt;                             // host view of the step counter
+    Exec self;                          // the handle as a group of one
+    Timing timing;
+    void *batch;                        // the batch holding the handle, or NULL
+};
+
+// member extents
+Extents member_extents(const frehd_dev_impl *d);
+// (re)build a group's device argument arrays and extents from its members
+int exec_refresh(Exec &e);
+// run one phase (0..4) / one whole step for every member of the group
+int exec_phase(Exec &e, int phase);
+int exec_step(Exec &e);
+// Init's subgrid part for one handle (level index, computeSubArea,
+// combineSubArea)
+int exec_init_subgrid(Exec &e);
+int exec_collect_timing(Exec &e);
+// upload one handle's argument blocks to its own group arrays
+int upload_member_args(frehd_dev_impl *d);
+
+}  // namespace frehd
+
+struct frehd_dev : frehd::frehd_dev_impl {};
+
+struct frehd_batch {
+    frehd::Exec e;
+    frehd::Timing timing;
+};

@@ -56,19 +56,23 @@ struct Raw {
 };
 
 template <typename T>
-__device__ inline Raw<T> load_raw(const PcgArgs<T> &a, int c, int first)
+__device__ inline Raw<T> load_raw(const PcgArgs<T> &a, const T *pOld, int c, int first)
 {
     Raw<T> v;
     v.r = a.r[c];
     v.D = a.D[c];
-    v.po = first ? T(0) : a.pOld[c];
+    v.po = first ? T(0) : pOld[c];
     return v;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_pcg_dir(PcgArgs<T> a, int it)
+__global__ void __launch_bounds__(256) k_pcg_dir(const PcgArgs<T> *__restrict__ A, int it)
 {
+    const PcgArgs<T> &a = A[blockIdx.z];
+    if ((int)blockIdx.x >= a.nbDir) return;
     __shared__ double lds[16 * 3];
+    const T *pOld = a.pBuf[it & 1];
+    T *pNew = a.pBuf[(it + 1) & 1];
     PcgScalars *sc = a.sc;
     if (*a.halted) return;
     if (a.outerDone && *a.outerDone >= 0) return;
@@ -97,7 +101,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(PcgArgs<T> a, int it)
     const T al = (T)alphaPrev;
 
     int lb = blockIdx.x;
-    if (a.remap) lb = (lb & 7) * (int)(gridDim.x >> 3) + (lb >> 3);
+    if (a.remap) lb = (lb & 7) * (a.nbDir >> 3) + (lb >> 3);
     const int bx = lb % a.gx, by = lb / a.gx;
     double part[1] = {0.0};
     if (by < a.gy) {
@@ -116,7 +120,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(PcgArgs<T> a, int it)
                     if (first) {
                         a.x[c] = T(0);
                     } else {
-                        const T po = a.pOld[c];
+                        const T po = pOld[c];
                         a.x[c] = alphaOne ? a.x[c] + po : a.x[c] + al * po;
                     }
                 }
@@ -131,7 +135,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(PcgArgs<T> a, int it)
             T pS = 0, cyS = 0, DS = 1;
             if (valid && j0 > 0) {
                 const int c = (j0 - 1) * nx + i;
-                const Raw<T> v = load_raw(a, c, first);
+                const Raw<T> v = load_raw(a, pOld, c, first);
                 pS = direction(jacobi_z(v.r, v.D), v.po, beta, first, betaOne);
                 cyS = a.cY[c];
                 DS = v.D;
@@ -142,13 +146,13 @@ __global__ void __launch_bounds__(256) k_pcg_dir(PcgArgs<T> a, int it)
             T cxc = 0, cyc = 0, xc = 0;
             if (valid) {
                 const int c = j0 * nx + i;
-                cur = load_raw(a, c, first);
+                cur = load_raw(a, pOld, c, first);
                 cxc = a.cX[c];
                 cyc = j0 < ny - 1 ? a.cY[c] : T(0);
                 if (!first) xc = a.x[c];
-                if (j0 + 1 < ny) nxt = load_raw(a, c + nx, first);
+                if (j0 + 1 < ny) nxt = load_raw(a, pOld, c + nx, first);
             }
-            if (halo) hcur = load_raw(a, j0 * nx + ih, first);
+            if (halo) hcur = load_raw(a, pOld, j0 * nx + ih, first);
             T pc = direction(jacobi_z(cur.r, cur.D), cur.po, beta, first, betaOne);
 
             for (int j = j0; j < j1; j++) {
@@ -157,12 +161,12 @@ __global__ void __launch_bounds__(256) k_pcg_dir(PcgArgs<T> a, int it)
                 Raw<T> nn = {0, 1, 0}, hn = {0, 1, 0};
                 T cxn = 0, cyn = 0, xn = 0;
                 if (valid && j + 1 < j1) {
-                    if (j + 2 < ny) nn = load_raw(a, c + 2 * nx, first);
+                    if (j + 2 < ny) nn = load_raw(a, pOld, c + 2 * nx, first);
                     cxn = a.cX[c + nx];
                     cyn = j + 1 < ny - 1 ? a.cY[c + nx] : T(0);
                     if (!first) xn = a.x[c + nx];
                 }
-                if (halo && j + 1 < j1) hn = load_raw(a, c + nx - i + ih, first);
+                if (halo && j + 1 < j1) hn = load_raw(a, pOld, c + nx - i + ih, first);
 
                 const T pn = (valid && j + 1 < ny) ? direction(jacobi_z(nxt.r, nxt.D), nxt.po, beta, first, betaOne)
                                                    : T(0);
@@ -200,7 +204,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(PcgArgs<T> a, int it)
                         if (j < ny - 1 && !(nxt.D < 0)) sum += (-cyc) * pn;
                         if (i < nx - 1 && !(De < 0)) sum += (-cxc) * pe;
                     }
-                    a.pNew[c] = pc;
+                    pNew[c] = pc;
                     a.q[c] = sum;
                     part[0] += (double)pc * (double)sum;
                     a.x[c] = first ? T(0) : (alphaOne ? xc + cur.po : xc + al * cur.po);
@@ -219,7 +223,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(PcgArgs<T> a, int it)
         }
     }
     block_sum<1>(part, lds);
-    if (!stop) store_partials<1>(a.partB, {part[0]});
+    if (!stop) store_partials<1>(a.partB, a.nbDir, {part[0]});
 }
 
 template <typename T>
@@ -228,8 +232,10 @@ struct alignas(16) Pack16 {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_pcg_res(PcgArgs<T> a, int it)
+__global__ void __launch_bounds__(256) k_pcg_res(const PcgArgs<T> *__restrict__ A, int it)
 {
+    const PcgArgs<T> &a = A[blockIdx.z];
+    if ((int)blockIdx.x >= a.nbRes) return;
     __shared__ double lds[16 * 3];
     PcgScalars *sc = a.sc;
     if (*a.halted) return;
@@ -252,7 +258,7 @@ __global__ void __launch_bounds__(256) k_pcg_res(PcgArgs<T> a, int it)
     const Pack16<T> *q2 = reinterpret_cast<const Pack16<T> *>(a.q);
     const Pack16<T> *D2 = reinterpret_cast<const Pack16<T> *>(a.D);
     Pack16<T> *r2 = reinterpret_cast<Pack16<T> *>(a.r);
-    for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < nv; h += gridDim.x * blockDim.x) {
+    for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < nv; h += a.nbRes * blockDim.x) {
         const Pack16<T> qv = q2[h], dv = D2[h];
         Pack16<T> rv = r2[h];
 #pragma unroll
@@ -266,7 +272,7 @@ __global__ void __launch_bounds__(256) k_pcg_res(PcgArgs<T> a, int it)
             a.r[k] = rk;
         }
     block_sum<2>(part, lds);
-    store_partials<2>(a.partA, {part[0], part[1]});
+    store_partials<2>(a.partA, a.nbRes, {part[0], part[1]});
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->pq = pqv[0];
         sc->alpha = alpha;
@@ -287,15 +293,17 @@ __global__ void __launch_bounds__(256) k_pcg_res(PcgArgs<T> a, int it)
 // the decoupled system then has the coupled system's solution.  Writes x_s,
 // r32 = (float) (r + lift) and the partials b.b (s = 0), r.r (the FP64 true
 // residual) and r32.r32, r32.z32 (the inner solve's first stop test and rho).
-__global__ void __launch_bounds__(256) k_refine(RefineArgs a, int s)
+__global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A, int s)
 {
+    const RefineArgs &a = A[blockIdx.z];
+    if ((int)blockIdx.x >= a.nb) return;
     __shared__ double lds[16 * 4];
     Ctl *ctl = a.ctl;
     if (ctl->halted || (ctl->ref.doneAt >= 0 && ctl->ref.doneAt < s)) return;
     const double *xin = s == 0 ? a.xbuf[0] : a.xbuf[(s + 1) & 1];   // x_0 or x_{s-1}
     double *xout = a.xbuf[s & 1];
     int lb = blockIdx.x;
-    if (a.remap) lb = (lb & 7) * (int)(gridDim.x >> 3) + (lb >> 3);
+    if (a.remap) lb = (lb & 7) * (a.nb >> 3) + (lb >> 3);
     const int bx = lb % a.gx, by = lb / a.gx;
     double part[4] = {0.0, 0.0, 0.0, 0.0};     // r32.r32, r32.z32, b.b, r.r
     if (by < a.gy) {
@@ -396,9 +404,9 @@ __global__ void __launch_bounds__(256) k_refine(RefineArgs a, int s)
     block_sum<4>(part, lds);
     if (threadIdx.x == 0) {
         a.partA[blockIdx.x] = part[0];
-        a.partA[gridDim.x + blockIdx.x] = part[1];
+        a.partA[a.nb + blockIdx.x] = part[1];
         a.partC[blockIdx.x] = part[2];
-        a.partC[gridDim.x + blockIdx.x] = part[3];
+        a.partC[a.nb + blockIdx.x] = part[3];
     }
 }
 
@@ -407,8 +415,9 @@ __global__ void __launch_bounds__(256) k_refine(RefineArgs a, int s)
 // sweep cap; otherwise resets the inner solve with tolerance
 // max(epsInner, eps |b| / (2 |r|)), so the last inner solve stops about when
 // the outer test will pass.
-__global__ void __launch_bounds__(256) k_refine_test(RefineArgs a, int s)
+__global__ void __launch_bounds__(256) k_refine_test(const RefineArgs *__restrict__ A, int s)
 {
+    const RefineArgs &a = A[blockIdx.z];
     __shared__ double lds[16 * 2];
     Ctl *ctl = a.ctl;
     if (ctl->halted || ctl->ref.doneAt >= 0) return;
@@ -439,9 +448,9 @@ __global__ void __launch_bounds__(256) k_refine_test(RefineArgs a, int s)
     }
 }
 
-template __global__ void k_pcg_dir<double>(PcgArgs<double>, int);
-template __global__ void k_pcg_res<double>(PcgArgs<double>, int);
-template __global__ void k_pcg_dir<float>(PcgArgs<float>, int);
-template __global__ void k_pcg_res<float>(PcgArgs<float>, int);
+template __global__ void k_pcg_dir<double>(const PcgArgs<double> *__restrict__, int);
+template __global__ void k_pcg_res<double>(const PcgArgs<double> *__restrict__, int);
+template __global__ void k_pcg_dir<float>(const PcgArgs<float> *__restrict__, int);
+template __global__ void k_pcg_res<float>(const PcgArgs<float> *__restrict__, int);
 
 }  // namespace frehd

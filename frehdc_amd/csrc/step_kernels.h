@@ -38,11 +38,13 @@ struct RefineScalars {
 };
 
 // device control block (one per handle)
+// (kernels read their member's argument block from a device array,
+// A[blockIdx.z]: one launch can advance many scenarios)
 struct Ctl {
     int tt;                   // current step number
     int halted;               // set when max CFL > 20
     int haltStep;
-    int pad;
+    int parity;               // depth buffer of the step's start: 0 -> depthA
     double lastMaxCFLx, lastMaxCFLy;
     double Vloss[4];
     PcgScalars pcg;
@@ -66,17 +68,24 @@ struct StepArgs {
     double *CDXP, *CDYP, *Fuu, *Fvv, *CFLx, *CFLy, *wtfXP, *wtfYP, *bottomZ, *bottomXP, *bottomYP;
     double *S, *Sm;           // nScalar arrays with stride sStride
     int64_t sStride;
-    // interior arrays
-    double *uuYP, *vvXP, *depthOld, *depthNew, *cellV, *Snew, *manning;
+    // interior arrays; the depth is double-buffered: Ctl.parity selects the
+    // buffer holding the depth at the start of the step
+    double *uuYP, *vvXP, *depthA, *depthB, *cellV, *Snew, *manning;
     uint8_t *rmv, *flags;     // flags: bit0 tideP cell, bit1 tideM cell, bits 2-7 inflow count
     // operator and solver vectors
-    double *cX, *cY, *D, *b, *x, *r, *q, *pA, *pOld, *pNew;
+    double *cX, *cY, *D, *b, *x, *r, *q, *pA, *pB;
     float *cX32, *cY32, *D32;  // FP32 operator copies (mixed precision only, else NULL)
+    double *xB;                // mixed precision: second refinement x buffer
     // per-block partial sums: solver (partA: r.r, r.z; partB: p.q), volume
     // losses (vpart: 4 slots) and CFL maxima (cflpart: 2 slots), each slot
     // maxBlocks long; nb1 = blocks of an Ng sweep, nb2 = blocks of a 2-D sweep
     double *partA, *partB, *vpart, *cflpart;
     int maxBlocks, nb1, nb2, pad3;
+    // this member's launch extents (a batched launch is sized for its
+    // largest member; blocks past these exit at once): 2-D stencil grid of
+    // 64x4 blocks, sweeps of 256 over N, ghost cells and tidal cells, and
+    // the grid-stride solver grid
+    int gx2, gy2, nbN, nbGhost, nbTide, nbRes;
     // forcing (Nt+1 samples) and locations
     const double *tideP, *tideM, *inflow, *tidalPS, *tidalMS;
     const int *tideLocP, *tideLocM;
@@ -106,7 +115,8 @@ __host__ __device__ inline size_t tab_index(size_t c, int k, int Ne, int T)
 template <typename T>
 struct PcgArgs {
     const T *cX, *cY, *D;
-    T *r, *q, *pOld, *pNew, *x;
+    T *r, *q, *x;
+    T *pBuf[2];               // iteration it reads p from pBuf[it&1], writes pBuf[(it+1)&1]
     double *partA;            // r.r, r.z partials: of the start residual (nbStart
                               // blocks) at iteration 0, of k_pcg_res (nbRes) after
     double *partB;            // p.q partials of k_pcg_dir (nbDir blocks)
@@ -123,8 +133,8 @@ struct PcgArgs {
     int decouple;             // inner solve: skip couplings to identity rows
 };
 
-template <typename T> __global__ void k_pcg_dir(PcgArgs<T> a, int it);
-template <typename T> __global__ void k_pcg_res(PcgArgs<T> a, int it);
+template <typename T> __global__ void k_pcg_dir(const PcgArgs<T> *__restrict__ A, int it);
+template <typename T> __global__ void k_pcg_res(const PcgArgs<T> *__restrict__ A, int it);
 
 // FP64 iterative refinement around the FP32 solve (mixed precision): sweep
 // s forms x_s = x_{s-1} + d (d: the inner solve's correction) and the true
@@ -144,34 +154,37 @@ struct RefineArgs {
     int gx, gy, H, remap, nb, pad;
 };
 
-__global__ void k_refine(RefineArgs a, int s);
-__global__ void k_refine_test(RefineArgs a, int s);
+__global__ void k_refine(const RefineArgs *__restrict__ A, int s);
+__global__ void k_refine_test(const RefineArgs *__restrict__ A, int s);
+
+__device__ inline const double *depth_cur(const StepArgs &a) { return a.ctl->parity ? a.depthB : a.depthA; }
+__device__ inline double *depth_nxt(const StepArgs &a) { return a.ctl->parity ? a.depthA : a.depthB; }
 
 enum SubTable { T_V, T_Z, T_VXP, T_VXM, T_VYP, T_VYM, T_NP, T_NM, T_OP, T_OM };
 
-__global__ void k_step_begin(StepArgs a);
-__global__ void k_predictor(StepArgs a);
-__global__ void k_assemble(StepArgs a);
-__global__ void k_surface(StepArgs a);
-__global__ void k_limiter(StepArgs a);
-__global__ void k_faces(StepArgs a);
-__global__ void k_flux(StepArgs a);
-__global__ void k_tidal_velocity(StepArgs a);
-__global__ void k_finish(StepArgs a);
-__global__ void k_scalar(StepArgs a, int q);
-__global__ void k_scalar_bc(StepArgs a, int q);
-__global__ void k_step_end(StepArgs a);
+__global__ void k_step_begin(const StepArgs *__restrict__ A);
+__global__ void k_predictor(const StepArgs *__restrict__ A);
+__global__ void k_assemble(const StepArgs *__restrict__ A);
+__global__ void k_surface(const StepArgs *__restrict__ A);
+__global__ void k_limiter(const StepArgs *__restrict__ A);
+__global__ void k_faces(const StepArgs *__restrict__ A);
+__global__ void k_flux(const StepArgs *__restrict__ A);
+__global__ void k_tidal_velocity(const StepArgs *__restrict__ A);
+__global__ void k_finish(const StepArgs *__restrict__ A);
+__global__ void k_scalar(const StepArgs *__restrict__ A, int q);
+__global__ void k_scalar_bc(const StepArgs *__restrict__ A, int q);
+__global__ void k_step_end(const StepArgs *__restrict__ A);
 // subgrid variants
-__global__ void k_predictor_sub(StepArgs a);
-__global__ void k_assemble_sub(StepArgs a);
-__global__ void k_subarea_init(StepArgs a);
-__global__ void k_limiter_sub(StepArgs a);
-__global__ void k_combine(StepArgs a);
-__global__ void k_sub_ghosts(StepArgs a);
-__global__ void k_faces_sub(StepArgs a);
-__global__ void k_flux_sub(StepArgs a);
-__global__ void k_tidal_velocity_sub(StepArgs a);
-__global__ void k_finish_sub(StepArgs a);
-__global__ void k_scalar_sub(StepArgs a, int q);
+__global__ void k_predictor_sub(const StepArgs *__restrict__ A);
+__global__ void k_assemble_sub(const StepArgs *__restrict__ A);
+__global__ void k_subarea_init(const StepArgs *__restrict__ A);
+__global__ void k_limiter_sub(const StepArgs *__restrict__ A);
+__global__ void k_combine(const StepArgs *__restrict__ A);
+__global__ void k_sub_ghosts(const StepArgs *__restrict__ A);
+__global__ void k_faces_sub(const StepArgs *__restrict__ A);
+__global__ void k_flux_sub(const StepArgs *__restrict__ A);
+__global__ void k_tidal_velocity_sub(const StepArgs *__restrict__ A);
+__global__ void k_finish_sub(const StepArgs *__restrict__ A);
+__global__ void k_scalar_sub(const StepArgs *__restrict__ A, int q);
 
 }  // namespace frehd

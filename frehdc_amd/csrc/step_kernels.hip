@@ -15,8 +15,9 @@ __device__ inline int cur_tt(const StepArgs &a) { return a.ctl->tt; }
 
 // ---------------------------------------------------------------------------
 // Step begin: tt += 1 (SolveAll's loop counter, nssolve.c:61).
-__global__ void k_step_begin(StepArgs a)
+__global__ void k_step_begin(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         Ctl *c = a.ctl;
         if (c->halted) return;
@@ -146,8 +147,18 @@ __device__ inline void predictor_body(const StepArgs &a)
     if (i == 0) { a.DragYP[giM(g, j)] = DragY; a.EnYP[giM(g, j)] = EnY; }
 }
 
-__global__ void __launch_bounds__(256) k_predictor(StepArgs a) { predictor_body<false>(a); }
-__global__ void __launch_bounds__(256) k_predictor_sub(StepArgs a) { predictor_body<true>(a); }
+__global__ void __launch_bounds__(256) k_predictor(const StepArgs *__restrict__ A)
+{
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
+    predictor_body<false>(a);
+}
+__global__ void __launch_bounds__(256) k_predictor_sub(const StepArgs *__restrict__ A)
+{
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
+    predictor_body<true>(a);
+}
 
 // ---------------------------------------------------------------------------
 // Assembly: matrixSourceTerm + matrixCoeff + adjustMatrixBoundary +
@@ -189,8 +200,10 @@ __device__ inline void store_row(const StepArgs &a, int c, double GXP, double GY
     part[1] += z * (z * invd);
 }
 
-__global__ void __launch_bounds__(256) k_assemble(StepArgs a)
+__global__ void __launch_bounds__(256) k_assemble(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nbRes) return;
     __shared__ double lds[16 * 3];
     if (a.ctl->halted) return;
     const Grid g = a.g;
@@ -200,7 +213,7 @@ __global__ void __launch_bounds__(256) k_assemble(StepArgs a)
     const double dt = p.dt;
     const double coefx = p.g * (dt * dt / (p.dx * p.dx));
     const double coefy = p.g * (dt * dt / (p.dy * p.dy));
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += gridDim.x * blockDim.x) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += a.nbRes * blockDim.x) {
         const int i = c % g.nx, j = c / g.nx;
         const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
         const double GXP = face_coef(coefx, a.depthXP[c], a.DragXP[c]);
@@ -220,7 +233,7 @@ __global__ void __launch_bounds__(256) k_assemble(StepArgs a)
         store_row(a, c, GXP, GYP, D, z, part);
     }
     block_sum<3>(part, lds);
-    store_partials<2>(a.partA, {part[0], part[1]});
+    store_partials<2>(a.partA, a.nbRes, {part[0], part[1]});
 }
 
 
@@ -231,8 +244,10 @@ __device__ inline void surface_cell(const StepArgs &a, int k, double &loss);
 // enforceTidalBC (nsfunctions.c:1020-1031, 436-449, 317-334, 412-433), over
 // all Ng indices.  A ghost takes the zero-clipped solution of the interior
 // cell it mirrors (the tidal override comes after the ghost fill).
-__global__ void __launch_bounds__(256) k_surface(StepArgs a)
+__global__ void __launch_bounds__(256) k_surface(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
     if (a.ctl->halted) return;
     const Grid g = a.g;
@@ -252,7 +267,10 @@ __device__ inline void surface_cell(const StepArgs &a, int k, double &loss)
     a.surfOld[k] = old;
     const int src = ghost_source(g, k, false);
     const int cell = src < 0 ? k : src;
-    double h = a.x[cell];
+    // the solution: x, or for the mixed path the refinement buffer of the
+    // sweep that converged
+    const double *xs = (a.cX32 && (a.ctl->ref.doneAt & 1)) ? a.xB : a.x;
+    double h = xs[cell];
     const double zb = a.bottomZ[cell];
     if (h < zb) {
         const double diff = zb - h;
@@ -279,7 +297,7 @@ __device__ inline double limiter_cell(const StepArgs &a, int c, int i, int j, do
 {
     const Grid &g = a.g;
     const P &p = a.p;
-    const double *dOld = a.depthOld, *ho = a.surfOld;
+    const double *dOld = depth_cur(a), *ho = a.surfOld;
     const double zb = a.bottomZ[c];
     double h = a.surf[c];
     double diff = h - zb;
@@ -310,14 +328,16 @@ __device__ inline void store_vloss2(const StepArgs &a, double (&loss)[2], double
 {
     block_sum<2>(loss, lds);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const int bid = blockIdx.x + gridDim.x * blockIdx.y;
+        const int bid = blockIdx.x + a.gx2 * blockIdx.y;
         a.vpart[0 * a.maxBlocks + bid] = loss[0];
         a.vpart[1 * a.maxBlocks + bid] = loss[1];
     }
 }
 
-__global__ void __launch_bounds__(256) k_limiter(StepArgs a)
+__global__ void __launch_bounds__(256) k_limiter(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
     __shared__ double lds[16 * 2];
     if (a.ctl->halted) return;
     const Grid g = a.g;
@@ -332,13 +352,13 @@ __global__ void __launch_bounds__(256) k_limiter(StepArgs a)
         a.surf[c] = h;
         a.rmv[c] = rmv;
         // computeVolume: depth from the previous step
-        const double cellV = p.dx * p.dy * a.depthOld[c];
+        const double cellV = p.dx * p.dy * depth_cur(a)[c];
         a.cellV[c] = cellV;
         for (int q = 0; q < p.nScalar; q++)
             a.Sm[(size_t)q * a.sStride + c] = a.S[(size_t)q * a.sStride + c] * cellV;
         // updateDepth
         const double d = h - a.bottomZ[c];
-        a.depthNew[c] = d < 0 ? 0 : d;
+        depth_nxt(a)[c] = d < 0 ? 0 : d;
     }
     store_vloss2(a, loss, lds);
 }
@@ -411,8 +431,10 @@ __device__ inline double face_depth_y(const StepArgs &a, int k)
 
 __device__ inline void faces_cell(const StepArgs &a, int k, double &loss);
 
-__global__ void __launch_bounds__(256) k_faces(StepArgs a)
+__global__ void __launch_bounds__(256) k_faces(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
     if (a.ctl->halted) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -487,8 +509,10 @@ __device__ inline void faces_cell(const StepArgs &a, int k, double &loss)
 // ---------------------------------------------------------------------------
 // enforceVelocityBC + computeFaceFlowRate over Ng (nsfunctions.c:289-314,
 // 350-358).
-__global__ void __launch_bounds__(256) k_flux(StepArgs a)
+__global__ void __launch_bounds__(256) k_flux(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nb1) return;
     if (a.ctl->halted) return;
     const Grid g = a.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -510,8 +534,10 @@ __global__ void __launch_bounds__(256) k_flux(StepArgs a)
 
 // adjustTidalVelocity (nsfunctions.c:1034-1078): one thread per tidal cell.
 // Fvv is not recomputed afterwards (the reference keeps the old flux).
-__global__ void k_tidal_velocity(StepArgs a)
+__global__ void k_tidal_velocity(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nbTide) return;
     if (a.ctl->halted) return;
     const Grid g = a.g;
     const P &p = a.p;
@@ -523,7 +549,7 @@ __global__ void k_tidal_velocity(StepArgs a)
     const int c = isP ? a.tideLocP[q] : a.tideLocM[q - nP];
     const int i = c % g.nx, j = c / g.nx;
     const double Fww = (a.surf[c] - a.surfOld[c]) * p.dx * p.dy / p.dt;
-    const double dc = a.depthNew[c];
+    const double dc = depth_nxt(a)[c];
     if (isP) {
         const double all = a.Fvv[nbS(g, c, i, j)] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww;
         if (all != 0 && dc > 0) a.vvYP[c] = all / (dc * p.dx);
@@ -539,8 +565,10 @@ __global__ void k_tidal_velocity(StepArgs a)
 // (nsfunctions.c:337-347, 373-409, 1170-1189, 691-727, 1192-1232).
 // velocityInterp uses flat-index neighbours (row wrap, ghost reads); indices
 // below 0 read 0.
-__global__ void __launch_bounds__(256) k_finish(StepArgs a)
+__global__ void __launch_bounds__(256) k_finish(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
     if (a.ctl->halted) return;
     const Grid g = a.g;
@@ -564,7 +592,7 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
         if (V < 0) V = 0;
         a.cellV[c] = V;
 
-        const double d = a.depthNew[c];
+        const double d = depth_nxt(a)[c];
         if (p.CDnotN == 0 && d > 0) {
             const double n = a.manning ? a.manning[c] : p.manningN;
             const double coeff = p.g * n * n;
@@ -604,7 +632,7 @@ __global__ void __launch_bounds__(256) k_finish(StepArgs a)
 template <bool SUB>
 __device__ inline void scalar_body(const StepArgs &a, int q)
 {
-    if (a.ctl->halted) return;
+    if (a.ctl->halted || q >= a.p.nScalar) return;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -664,21 +692,33 @@ __device__ inline void scalar_body(const StepArgs &a, int q)
         const bool topRowNotFirst = (j == g.ny - 1 && i > 0);
         if (!bottomRow && !topRowNotFirst) {
             Sv = 0;
-            a.surf[c] = a.surf[c] - a.depthNew[c];
-            a.depthNew[c] = 0;
+            a.surf[c] = a.surf[c] - depth_nxt(a)[c];
+            depth_nxt(a)[c] = 0;
         }
     }
     a.Snew[c] = Sv;
 }
 
-__global__ void __launch_bounds__(256) k_scalar(StepArgs a, int q) { scalar_body<false>(a, q); }
-__global__ void __launch_bounds__(256) k_scalar_sub(StepArgs a, int q) { scalar_body<true>(a, q); }
+__global__ void __launch_bounds__(256) k_scalar(const StepArgs *__restrict__ A, int q)
+{
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
+    scalar_body<false>(a, q);
+}
+__global__ void __launch_bounds__(256) k_scalar_sub(const StepArgs *__restrict__ A, int q)
+{
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
+    scalar_body<true>(a, q);
+}
 
 // enforceScalarBC (scalar.c:419-471) over the interior and the four ghost
 // blocks (corners untouched).
-__global__ void __launch_bounds__(256) k_scalar_bc(StepArgs a, int q)
+__global__ void __launch_bounds__(256) k_scalar_bc(const StepArgs *__restrict__ A, int q)
 {
-    if (a.ctl->halted) return;
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nb1) return;
+    if (a.ctl->halted || q >= a.p.nScalar) return;
     const Grid g = a.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= g.Ng - 4) return;
@@ -693,15 +733,16 @@ __global__ void __launch_bounds__(256) k_scalar_bc(StepArgs a, int q)
     const uint8_t fl = a.flags[k];
     if ((fl & 1) && p.bcType != 2) v = p.useConstTidePS == 1 ? p.tidePS : a.tidalPS[tt];
     if ((fl & 2) && p.bcType != 1) v = p.useConstTideMS == 1 ? p.tideMS : a.tidalMS[tt];
-    if (a.depthNew[k] <= 0) v = 0;
+    if (depth_nxt(a)[k] <= 0) v = 0;
     S[k] = v;
 }
 
 // Step end: volume-loss and CFL reductions (block partials in block order),
 // and the max CFL test of monitorCFL (nsfunctions.c:1226-1231), which halts
 // the handle instead of exiting the process.  One block of 256 threads.
-__global__ void __launch_bounds__(256) k_step_end(StepArgs a)
+__global__ void __launch_bounds__(256) k_step_end(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
     __shared__ double lds[16 * 4];
     Ctl *c = a.ctl;
     if (c->halted) return;
@@ -730,6 +771,7 @@ __global__ void __launch_bounds__(256) k_step_end(StepArgs a)
     c->pcgIterTotal += c->pcg.iter;
     c->relResLast = c->pcg.bNorm > 0 ? c->pcg.rNorm / c->pcg.bNorm : 1.0;
     c->steps += 1;
+    c->parity ^= 1;     // the new depth becomes the next step's start
     if (fabs(mx) > 20 || fabs(my) > 20) {
         c->halted = 1;
         c->haltStep = c->tt;
@@ -810,18 +852,22 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
 }
 
 // Init: computeSubArea on the initial surface (initialize.c:107)
-__global__ void __launch_bounds__(256) k_subarea_init(StepArgs a)
+__global__ void __launch_bounds__(256) k_subarea_init(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nbN) return;
     const Grid g = a.g;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
-    subarea_cell(a, c, c % g.nx, c / g.nx, a.surf[c], a.depthOld[c]);
+    subarea_cell(a, c, c % g.nx, c / g.nx, a.surf[c], depth_cur(a)[c]);
 }
 
 // limiterCFL + computeVolumeSub + computeSubArea + updateDepth
 // (nsfunctions.c:452-495, subgrid.c:1215-1224, 55-251, nsfunctions.c:239-247)
-__global__ void __launch_bounds__(256) k_limiter_sub(StepArgs a)
+__global__ void __launch_bounds__(256) k_limiter_sub(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
     __shared__ double lds[16 * 2];
     if (a.ctl->halted) return;
     const Grid g = a.g;
@@ -839,9 +885,9 @@ __global__ void __launch_bounds__(256) k_limiter_sub(StepArgs a)
         a.cellV[c] = cellV;
         for (int q = 0; q < p.nScalar; q++)
             a.Sm[(size_t)q * a.sStride + c] = a.S[(size_t)q * a.sStride + c] * cellV;
-        subarea_cell(a, c, i, j, h, a.depthOld[c]);
+        subarea_cell(a, c, i, j, h, depth_cur(a)[c]);
         const double d = h - a.bottomZ[c];
-        a.depthNew[c] = d < 0 ? 0 : d;
+        depth_nxt(a)[c] = d < 0 ? 0 : d;
     }
     store_vloss2(a, loss, lds);
 }
@@ -850,8 +896,10 @@ __global__ void __launch_bounds__(256) k_limiter_sub(StepArgs a)
 // :466-478), plus the interior faces the boundary loop closes: Nx of the
 // right column and Oy of the non-tidal top row.  useminV = 1 changes volumes
 // only where Nmin > 0, which stays 0 with useminA = 0.
-__global__ void __launch_bounds__(256) k_combine(StepArgs a)
+__global__ void __launch_bounds__(256) k_combine(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nbN) return;
     if (a.ctl->halted) return;
     const Grid g = a.g;
     const P &p = a.p;
@@ -909,8 +957,10 @@ __global__ void __launch_bounds__(256) k_combine(StepArgs a)
 // combineSubArea's ghost copies (subgrid.c:480-506) over the 2nx + 2ny
 // ghosts.  The iM/iP ghosts of the top row take Oy before the top-row
 // closure, i.e. the clipped Op of the corner cell.
-__global__ void __launch_bounds__(256) k_sub_ghosts(StepArgs a)
+__global__ void __launch_bounds__(256) k_sub_ghosts(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nbGhost) return;
     if (a.ctl->halted) return;
     const Grid g = a.g;
     const int k = g.N + blockIdx.x * blockDim.x + threadIdx.x;
@@ -981,7 +1031,7 @@ __device__ inline void faces_sub_cell(const StepArgs &a, int k, double &loss)
     if (Oyc == 0) vc = 0;
     // no outflow from a dry cell: the face's own cell stops positive flow,
     // the cell beyond stops negative flow
-    const double *dn = a.depthNew;
+    const double *dn = depth_nxt(a);
     const bool dryC = dn[c] == 0;
     if (dryC && uc > 0) uc = 0;
     if (e < g.N && dn[e] == 0 && uc < 0) uc = 0;
@@ -1007,8 +1057,10 @@ __device__ inline void faces_sub_cell(const StepArgs &a, int k, double &loss)
     a.vvYP[c] = vc;
 }
 
-__global__ void __launch_bounds__(256) k_faces_sub(StepArgs a)
+__global__ void __launch_bounds__(256) k_faces_sub(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
     if (a.ctl->halted) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1019,8 +1071,10 @@ __global__ void __launch_bounds__(256) k_faces_sub(StepArgs a)
 }
 
 // enforceVelocityBC + computeFaceFlowRateSub over Ng (subgrid.c:1190-1212)
-__global__ void __launch_bounds__(256) k_flux_sub(StepArgs a)
+__global__ void __launch_bounds__(256) k_flux_sub(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nb1) return;
     if (a.ctl->halted) return;
     const Grid g = a.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1040,8 +1094,10 @@ __global__ void __launch_bounds__(256) k_flux_sub(StepArgs a)
 }
 
 // adjustTidalVelocitySub (subgrid.c:1291-1332)
-__global__ void k_tidal_velocity_sub(StepArgs a)
+__global__ void k_tidal_velocity_sub(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nbTide) return;
     if (a.ctl->halted) return;
     const Grid g = a.g;
     const P &p = a.p;
@@ -1066,8 +1122,10 @@ __global__ void k_tidal_velocity_sub(StepArgs a)
 // velocityInterp + volumeByFluxSub + updateCDSub (useSubDrag = 0: h = V/Z)
 // + thinLayerDrag + CFL numbers (nsfunctions.c:337-347, subgrid.c:1227-1263,
 // 1520-1535, nsfunctions.c:691-727, 1192-1232)
-__global__ void __launch_bounds__(256) k_finish_sub(StepArgs a)
+__global__ void __launch_bounds__(256) k_finish_sub(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
     if (a.ctl->halted) return;
     const Grid g = a.g;
@@ -1099,7 +1157,7 @@ __global__ void __launch_bounds__(256) k_finish_sub(StepArgs a)
             a.CDXP[c] = cd;
             a.CDYP[c] = cd;
         }
-        const double d = a.depthNew[c];
+        const double d = depth_nxt(a)[c];
         if (p.useThinLayer == 1 && p.CDnotN == 0 && d > 0) {
             const double n = a.manning ? a.manning[c] : p.manningN;
             const double coeff = p.g * n * n;
@@ -1130,8 +1188,10 @@ __global__ void __launch_bounds__(256) k_finish_sub(StepArgs a)
 // N = 0, so the wall adjustment subtracts zeros.  GnCt equal to 0 or dx*dy
 // becomes dx*dy; a dry cell with no velocity on its four faces gets the
 // identity-like row dx*dy * x = dx*dy * eta (flagged by a negative D).
-__global__ void __launch_bounds__(256) k_assemble_sub(StepArgs a)
+__global__ void __launch_bounds__(256) k_assemble_sub(const StepArgs *__restrict__ A)
 {
+    const StepArgs &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nbRes) return;
     __shared__ double lds[16 * 3];
     if (a.ctl->halted) return;
     const Grid g = a.g;
@@ -1141,7 +1201,7 @@ __global__ void __launch_bounds__(256) k_assemble_sub(StepArgs a)
     const double dt = p.dt, dxdy = p.dx * p.dy;
     const double cx = p.g * (dt * dt), cy = p.g * (dt * dt);
     const double divider = 1.0;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += gridDim.x * blockDim.x) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += a.nbRes * blockDim.x) {
         const int i = c % g.nx, j = c / g.nx;
         const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
         const double Nxc = a.Nx[c], Nxw = a.Nx[w], Oyc = a.Oy[c], Oys = a.Oy[s];
@@ -1160,7 +1220,7 @@ __global__ void __launch_bounds__(256) k_assemble_sub(StepArgs a)
         for (int q = 0; q < ninflow; q++) z = z + dt * (a.inflow[tt] / p.inflowLocLength);
         if (z == 0 || z == hc * p.dx * p.dy) z = hc * p.dx * p.dy;
         if (D == 0.0 || D == dxdy) D = dxdy;
-        if (a.depthOld[c] == 0 && a.uuXP[c] == 0 && a.uuXP[w] == 0 && a.vvYP[c] == 0 && a.vvYP[s] == 0) {
+        if (depth_cur(a)[c] == 0 && a.uuXP[c] == 0 && a.uuXP[w] == 0 && a.vvYP[c] == 0 && a.vvYP[s] == 0) {
             D = -dxdy;
             z = hc * p.dx * p.dy;
         }
@@ -1169,7 +1229,7 @@ __global__ void __launch_bounds__(256) k_assemble_sub(StepArgs a)
         store_row(a, c, GXP, GYP, D, z, part);
     }
     block_sum<3>(part, lds);
-    store_partials<2>(a.partA, {part[0], part[1]});
+    store_partials<2>(a.partA, a.nbRes, {part[0], part[1]});
 }
 
 }  // namespace frehd

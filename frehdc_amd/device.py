@@ -139,6 +139,64 @@ class DeviceScenario:
         return {names[q].decode(): (ms[q], launches[q]) for q in range(min(n.value, cap))}
 
 
+class Batch:
+    """Many DeviceScenario handles advanced by shared kernel launches
+    (frehd_batch_*): members may differ in grid size, forcing and physics but
+    share useSubgrid and precision."""
+
+    def __init__(self, capacity: int):
+        self.lib = abi.load_library()
+        h = C.c_void_p()
+        abi.check(self.lib.frehd_batch_create(int(capacity), C.byref(h)))
+        self.h = h
+        self.members = []
+
+    def add(self, dev: DeviceScenario):
+        abi.check(self.lib.frehd_batch_add(self.h, dev.h))
+        self.members.append(dev)
+
+    def remove(self, dev: DeviceScenario):
+        abi.check(self.lib.frehd_batch_remove(self.h, dev.h))
+        self.members.remove(dev)
+
+    def __len__(self):
+        n = C.c_int32()
+        abi.check(self.lib.frehd_batch_size(self.h, C.byref(n)))
+        return n.value
+
+    def step(self, n: int = 1) -> int:
+        """Advance every member n steps; returns 0 or raises FrehdError
+        (FREHD_ECFL when a member is halted)."""
+        return abi.check(self.lib.frehd_batch_step(self.h, int(n)))
+
+    def try_step(self, n: int = 1) -> int:
+        return self.lib.frehd_batch_step(self.h, int(n))
+
+    def enable_timing(self, on=True):
+        abi.check(self.lib.frehd_batch_enable_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self) -> dict:
+        cap = 64
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        launches = (C.c_int64 * cap)()
+        n = C.c_int32()
+        abi.check(self.lib.frehd_batch_kernel_times(self.h, cap, names, ms, launches, C.byref(n)))
+        return {names[q].decode(): (ms[q], launches[q]) for q in range(min(n.value, cap))}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.frehd_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def device_count() -> int:
     lib = abi.load_library()
     n = C.c_int32()

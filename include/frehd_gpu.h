@@ -265,6 +265,29 @@ int frehd_dev_download_tables(frehd_dev_t *dev, double *const tables[10]);
  * depend on the shape beyond the order of the solver's dot-product sums. */
 int frehd_dev_set_pcg_tiling(frehd_dev_t *dev, int32_t rows_per_tile, int32_t xcd_remap);
 
+/* ---- batches: many handles advanced by shared launches ---------------------
+ * No counterpart in the reference (one scenario per process there).  A batch
+ * holds up to `capacity` handles of one device that share useSubgrid and
+ * precision (grid sizes, forcing and physics may differ); every kernel of a
+ * step is launched once for all of them.  Handles stay usable on their own
+ * (download, stats, ...) between batch calls; a handle is in at most one
+ * batch at a time, and removing it (or destroying the batch) leaves it
+ * intact.  frehd_batch_step advances every member that is not halted by
+ * nsteps steps and returns FREHD_ECFL if a member is halted at the end (its
+ * frehd_dev_stats / frehd_dev_step_index tell which); FREHD_EINVAL if a
+ * member's forcing series is too short for nsteps, before any step. */
+typedef struct frehd_batch frehd_batch_t;
+
+int frehd_batch_create(int32_t capacity, frehd_batch_t **out);
+int frehd_batch_add(frehd_batch_t *batch, frehd_dev_t *dev);
+int frehd_batch_remove(frehd_batch_t *batch, frehd_dev_t *dev);
+int frehd_batch_size(frehd_batch_t *batch, int32_t *n);
+int frehd_batch_step(frehd_batch_t *batch, int32_t nsteps);
+int frehd_batch_enable_timing(frehd_batch_t *batch, int32_t on);
+int frehd_batch_kernel_times(frehd_batch_t *batch, int32_t cap, const char **names,
+                             double *ms, int64_t *launches, int32_t *n);
+int frehd_batch_destroy(frehd_batch_t *batch);
+
 #ifdef __cplusplus
 }
 #endif
