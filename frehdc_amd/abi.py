@@ -84,7 +84,8 @@ class FrehdStats(C.Structure):
     _fields_ = [
         ("steps", C.c_int64), ("pcgIterTotal", C.c_int64), ("pcgIterLast", C.c_int32),
         ("refineLast", C.c_int32), ("relResLast", C.c_double), ("maxCFLx", C.c_double),
-        ("maxCFLy", C.c_double), ("Vloss", C.c_double * 4),
+        ("maxCFLy", C.c_double), ("Vloss", C.c_double * 4), ("halted", C.c_int32),
+        ("haltStep", C.c_int32),
     ]
 
 

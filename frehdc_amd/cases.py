@@ -251,12 +251,14 @@ def ensemble_member(m, n=2048, dx=1.0, dt=0.5, Nt=1000, seed=5, hurst=0.7, base=
 
 
 def serving_requests(count=512, rate=20.0, seed=6, lo=64, hi=1024, dx=2.0, dt=0.5,
-                     steps_lo=200, steps_hi=2000):
+                     steps_lo=200, steps_hi=2000, subgrid_r=0):
     """Poisson arrivals of scenarios with log-uniform sizes in [lo^2, hi^2]
     (rounded to multiples of 64 per side), fBm bed with Hurst ~ U(0.6, 0.9) in
-    [-1.5, +1] m, tide ~ U(0.1, 0.7) m, Manning ~ U(0.015, 0.04), 1-3 scalars,
-    U(200, 2000) steps.  Returns [(arrival_s, builder)] where builder() makes
-    the Scenario on demand."""
+    [-1.5, +1] m (an always-wet inlet at the tidal rows), tide ~ U(0.1, 0.7) m,
+    Manning ~ U(0.015, 0.04), 1-3 scalars, U(200, 2000) steps; with
+    subgrid_r > 0 the bed is drawn at r x r pixels per cell and the tables
+    are generated on the device.  Returns [(arrival_s, steps, side, builder)]
+    where builder() makes the Scenario on demand."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, size=count))
     out = []
@@ -271,10 +273,16 @@ def serving_requests(count=512, rate=20.0, seed=6, lo=64, hi=1024, dx=2.0, dt=0.
         sseed = int(rng.integers(1 << 30))
 
         def build(side=side, hurst=hurst, amp=amp, man=man, nsc=nsc, steps=steps, sseed=sseed):
-            bed = fbm_surface(side, side, hurst, np.random.default_rng(sseed), -1.5, 1.0)
+            r = max(1, subgrid_r)
+            bed = fbm_surface(side * r, side * r, hurst, np.random.default_rng(sseed), -1.5, 1.0)
+            bed[-INLET_ROWS * r:, :] = np.minimum(bed[-INLET_ROWS * r:, :], INLET_BED)
             cfg = default_config(side, side, dx, dt, steps, manningN=man, nScalar=nsc, eps=1e-10)
-            return _finish(cfg, bed, tide_series(steps, dt, amp),
-                           scalar0=np.zeros(nsc * side * side), name=f"req{side}")
+            coarse = bed.reshape(side, r, side, r).min(axis=(1, 3)) if r > 1 else bed
+            sc = _finish(cfg, coarse, tide_series(steps, dt, amp),
+                         scalar0=np.zeros(nsc * side * side), name=f"req{side}")
+            if subgrid_r > 0:
+                sc = with_subgrid(sc, bed, r, -1.5, 1.0, 0.01, device_tables=True)
+            return sc
         out.append((float(t[q]), steps, side, build))
     return out
 

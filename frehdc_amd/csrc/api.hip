@@ -827,6 +827,8 @@ int frehd_dev_stats(frehd_dev_t *d, frehd_stats *out)
     out->maxCFLx = c.lastMaxCFLx;
     out->maxCFLy = c.lastMaxCFLy;
     for (int q = 0; q < 4; q++) out->Vloss[q] = c.Vloss[q];
+    out->halted = c.halted;
+    out->haltStep = c.haltStep;
     return 0;
 }
 

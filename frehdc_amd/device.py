@@ -102,7 +102,8 @@ class DeviceScenario:
         abi.check(self.lib.frehd_dev_stats(self.h, C.byref(st)))
         return dict(steps=st.steps, pcgIterTotal=st.pcgIterTotal, pcgIterLast=st.pcgIterLast,
                     refineLast=st.refineLast, relResLast=st.relResLast, maxCFLx=st.maxCFLx,
-                    maxCFLy=st.maxCFLy, Vloss=list(st.Vloss))
+                    maxCFLy=st.maxCFLy, Vloss=list(st.Vloss), halted=bool(st.halted),
+                    haltStep=st.haltStep)
 
     @property
     def tt(self):

@@ -178,6 +178,8 @@ typedef struct frehd_stats {
     double  relResLast;       /* |r|/|b| at exit of the last solve                */
     double  maxCFLx, maxCFLy; /* from the last monitorCFL                         */
     double  Vloss[4];         /* Vloss1..4 accumulators (nsfunctions.c:446-486,1147) */
+    int32_t halted;           /* 1 once max CFL > 20 stopped the handle            */
+    int32_t haltStep;         /* the step at which it stopped                      */
 } frehd_stats;
 
 typedef struct frehd_dev frehd_dev_t;
