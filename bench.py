@@ -107,6 +107,97 @@ def run_form(scen, steps, warmup, **over):
             "relres": st["relResLast"], "kt": kt, "fields": fields}
 
 
+def run_batch(members, steps, warmup, **over):
+    """One batch of ensemble members (shared launches): warmup steps, then
+    `steps` timed steps, then 2 steps with per-kernel event timing."""
+    from frehdc_amd import abi, device
+    devs = [device.DeviceScenario(sc, **over) for sc in members]
+    b = device.Batch(len(devs))
+    for d in devs:
+        b.add(d)
+    b.try_step(warmup)
+    t0 = time.perf_counter()
+    rc = b.try_step(steps)
+    t1 = time.perf_counter()
+    if rc not in (0, abi.FREHD_ECFL):
+        abi.check(rc)
+    halted = sum(1 for d in devs if d.stats()["halted"])
+    surf = [d.download(["surf"])["surf"][: sc.N] for d, sc in zip(devs, members)]
+    iters = max(d.stats()["pcgIterLast"] for d in devs)
+    b.enable_timing(True)
+    b.try_step(2)
+    kt = b.kernel_times()
+    b.close()
+    for d in devs:
+        d.close()
+    cells = sum(sc.N for sc in members)
+    return {"sec": t1 - t0, "value": cells * steps / (t1 - t0), "ms_per_step": (t1 - t0) * 1e3 / steps,
+            "surf": surf, "iters": iters, "kt": kt, "halted": halted}
+
+
+def batched_ensemble(n_members, size, steps, warmup):
+    """The batched-ensemble workload: members of one fBm base (Hurst 0.7) plus
+    a seeded N(0, 0.05 m) perturbation, Manning n ~ U(0.015, 0.04) and tide
+    amplitude ~ U(0.3, 0.8) m per member, one scalar, dx = 1 m, dt = 0.5 s,
+    plain grid; FP64 and mixed (warm start) forms on the same members."""
+    from frehdc_amd import abi, cases
+    import numpy as np
+    t0 = time.perf_counter()
+    base = cases.fbm_surface(size, size, 0.7, np.random.default_rng(5), -1.0, 1.5)
+    Nt = warmup + steps + 8
+    members = [cases.ensemble_member(m, n=size, Nt=Nt, base=base) for m in range(n_members)]
+    log(f"bench: batch of {n_members} x {size}^2 built in {time.perf_counter() - t0:.1f} s")
+    out = {}
+    ref = None
+    for f, over in (("fp64", {}), ("mixed_warm", {"precision": abi.PREC_MIXED, "warmStart": 1})):
+        r = run_batch(members, steps, warmup, **over)
+        log(f"bench: batch {f}: {r['ms_per_step']:.2f} ms/step, {r['value'] / 1e9:.3f} G cell-steps/s")
+        log(f"kernel times batch {f} (ms/launch, launches):",
+            json.dumps({k: [round(v[0], 4), v[1]] for k, v in r["kt"].items()}))
+        o = {"cell_steps_per_s": r["value"], "ms_per_step": r["ms_per_step"], "pcg_iterations_last": r["iters"],
+             "halted_members": r["halted"]}
+        if ref is None:
+            ref = r["surf"]
+        else:
+            o["max_abs_eta_err_m"] = float(max(np.max(np.abs(a - b)) for a, b in zip(r["surf"], ref)))
+        out[f] = o
+    return {"workload": f"{n_members} members x {size}x{size}, dx=1 m, dt=0.5 s, plain grid, shared fBm "
+                        f"base + N(0, 0.05 m) per member, Manning U(0.015, 0.04), tide U(0.3, 0.8) m, "
+                        f"one scalar, {steps} timed steps after {warmup}", "forms": out}
+
+
+def serving_run(count, subgrid):
+    from frehdc_amd import abi, serving
+    t0 = time.perf_counter()
+    reqs = serving.make_requests(count, 20.0, subgrid_r=subgrid, workers=16)
+    log(f"bench: {count} serving requests built in {time.perf_counter() - t0:.1f} s")
+    out = {}
+    for f, over in (("fp64", {}), ("mixed_warm", {"precision": abi.PREC_MIXED, "warmStart": 1})):
+        fresh = [serving.Request(rid=r.rid, arrival=r.arrival, steps=r.steps, scen=r.scen) for r in reqs]
+        st = serving.Server(budget_bytes=200e9, **over).run(fresh)
+        log(f"bench: serving {f}: p50 {st['latency_p50_s']:.2f} s, p99 {st['latency_p99_s']:.2f} s, "
+            f"{st['cell_steps_per_s'] / 1e9:.3f} G cell-steps/s")
+        out[f] = st
+    return {"workload": f"{count} requests, Poisson 20/s, sides log-uniform 64..1024 (multiples of 64), "
+                        f"dx=2 m, subgrid {subgrid}x{subgrid}, fBm Hurst U(0.6, 0.9), tide U(0.1, 0.7) m, "
+                        f"Manning U(0.015, 0.04), 1-3 scalars, U(200, 2000) steps, HBM budget 200 GB",
+            "forms": out}
+
+
+def traffic_of(kernels, size, subgrid):
+    """HBM bytes per launch of the kernels, from the committed counter
+    summary of the default workload (profiles/traffic_marsh1024_sub8.json);
+    None for any other workload or when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "traffic_marsh1024_sub8.json")
+    if size != 1024 or subgrid != 8 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f)["kernels"]
+    if not all(n in k for n in kernels):
+        return None
+    return int(sum(k[n]["bytes"] for n in kernels))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +210,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,mixed_cold,mixed_warm",
                     help="precision forms to time (the headline value is the last mixed form)")
+    ap.add_argument("--batch-members", type=int, default=64)
+    ap.add_argument("--batch-size", type=int, default=2048)
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-count", type=int, default=48)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -181,10 +278,20 @@ def main():
     t_iter_ms = kt.get(kd, (0, 0))[0] + kt.get(kr, (0, 0))[0]
     alg_bytes = per_cell * N
     achieved = alg_bytes / (t_iter_ms * 1e-3) / 1e9 if t_iter_ms > 0 else 0.0
+    traffic = traffic_of([kd, kr], args.size, args.subgrid)
     roof = {"bound": "hbm", "kernel": f"PCG iteration ({kd} + {kr})",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_launch": alg_bytes, "ms_per_iteration": round(t_iter_ms, 5)}
+    if traffic:
+        roof["traffic_GBps"] = round(traffic / (t_iter_ms * 1e-3) / 1e9, 1)
+
+    batched = None
+    if not args.no_batch and args.batch_members > 0:
+        batched = batched_ensemble(args.batch_members, args.batch_size, args.batch_steps, 3)
+    served = None
+    if not args.no_serve and args.serve_count > 0:
+        served = serving_run(args.serve_count, 4)
 
     cpu = None
     if not args.no_cpu:
@@ -202,6 +309,8 @@ def main():
                    "form": head, "forms": forms_out},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": served,
     }
     print(json.dumps(out), flush=True)
 

@@ -245,6 +245,7 @@ def ensemble_member(m, n=2048, dx=1.0, dt=0.5, Nt=1000, seed=5, hurst=0.7, base=
         base = fbm_surface(n, n, hurst, np.random.default_rng(seed), -1.0, 1.5)
     r = np.random.default_rng(100 + m)
     bed = base + r.normal(0.0, 0.05, size=base.shape)
+    bed[-INLET_ROWS:, :] = np.minimum(bed[-INLET_ROWS:, :], INLET_BED)   # always-wet tidal inlet
     cfg = default_config(n, n, dx, dt, Nt, manningN=float(r.uniform(0.015, 0.04)), eps=1e-10)
     return _finish(cfg, bed, tide_series(Nt, dt, float(r.uniform(0.3, 0.8))),
                    scalar0=np.zeros(n * n), name=f"member{m}")
