@@ -324,6 +324,8 @@ int check_config(const frehd_config *k)
         return fail(FREHD_EINVAL, "unknown precision %d", k->precision);
     if (k->precision == FREHD_PREC_MIXED && !(k->epsInner > 0 && k->epsInner < 1 && k->maxRefine >= 1))
         return fail(FREHD_EINVAL, "mixed precision needs 0 < epsInner < 1 and maxRefine >= 1");
+    if (k->precision == FREHD_PREC_MIXED && (k->warmStart < 0 || k->warmStart > 2))
+        return fail(FREHD_EINVAL, "warmStart must be 0, 1 or 2");
     if (k->useScalar == 1 && k->scalarAdv != 1)
         return fail(FREHD_EINVAL, "only first-order upwind scalar advection (scalarAdv = 1) is supported");
     if (k->bcType < 1 || k->bcType > 3) return fail(FREHD_EINVAL, "bcType must be 1, 2 or 3");

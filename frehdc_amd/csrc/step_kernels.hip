@@ -190,7 +190,9 @@ __device__ inline void store_row(const StepArgs &a, int c, double GXP, double GY
         a.cX32[c] = (float)GXP;
         a.cY32[c] = (float)GYP;
         a.D32[c] = (float)D;
-        a.x[c] = a.p.warmStart ? a.surf[c] : 0.0;
+        // start: 0 (the reference's), the current surface, or its linear
+        // extrapolation in time 2 eta^n - eta^(n-1)
+        a.x[c] = a.p.warmStart == 2 ? 2.0 * a.surf[c] - a.surfOld[c] : (a.p.warmStart ? a.surf[c] : 0.0);
         return;
     }
     a.r[c] = z;

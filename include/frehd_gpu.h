@@ -93,8 +93,9 @@ typedef struct frehd_config {
     /* subgrid bathymetry (subgrid.c; Config.useSubgrid ... useminV)            */
     int32_t useSubgrid;      /* 0: plain grid, 1: subgrid tables                  */
     int32_t Ne;              /* table levels, round((surfmax - surfmin)/dsurf)    */
-    int32_t warmStart;       /* FREHD_PREC_MIXED: 1 starts the refinement from the
-                                current surface instead of the reference's x = 0  */
+    int32_t warmStart;       /* FREHD_PREC_MIXED: refinement start; 0 the
+                                reference's x = 0, 1 the current surface, 2 its
+                                extrapolation 2 eta^n - eta^(n-1)                  */
     double  surfmin, dsurf;  /* level k is surfmin + k*dsurf (+ offset)           */
     double  subR;            /* dx / dxf (fine cells per coarse cell side)        */
 } frehd_config;

@@ -96,7 +96,7 @@ def test_batch_subgrid_bitwise():
 def test_batch_mixed_within_tolerance():
     print("case batch of 2 mixed-precision scenarios, 80 steps", flush=True)
     scs = [cases.basin(nx=40, ny=36, Nt=200), cases.marsh(n=96, Nt=200)]
-    over = dict(precision=abi.PREC_MIXED, warmStart=1)
+    over = dict(precision=abi.PREC_MIXED, warmStart=2)
     ref = [solo(sc, 80, **over) for sc in scs]
     b = Batch(2)
     devs = [DeviceScenario(sc, **over) for sc in scs]
