@@ -53,6 +53,9 @@ class Scenario:
     S0: float = 0.0
     S0cell: np.ndarray | None = None   # (nScalar*N,)
     name: str = ""
+    surf0cell: np.ndarray | None = None  # (N,) per-cell initial fields (surfic/uuic/vvic.dat)
+    U0cell: np.ndarray | None = None
+    V0cell: np.ndarray | None = None
     subgrid: dict | None = None        # edgeX, edgeY and level tables (subgrid.py)
 
     @property
@@ -98,6 +101,9 @@ class Scenario:
         ic.bottomZ = abi.dptr(self.bottomZ)
         ic.surf0, ic.U0, ic.V0, ic.S0 = self.surf0, self.U0, self.V0, self.S0
         ic.S0cell = abi.dptr(self.S0cell)
+        ic.surf0cell = abi.dptr(self.surf0cell)
+        ic.U0cell = abi.dptr(self.U0cell)
+        ic.V0cell = abi.dptr(self.V0cell)
         if self.subgrid is not None:
             sg = abi.FrehdSubgrid()
             for k in ("edgeX", "edgeY", *abi.SUB_TABLES):

@@ -23,6 +23,10 @@
  *     and interpolated from files;
  *   - the Init call sequence is restated, calling the reference's functions
  *     for every numerical part.
+ * The ref_io_* functions call the reference's own input and output routines
+ * (readOneBC, ReadBathymetry + InitBathymetry, initAllSubVar, WriteFile) on
+ * given files, for the parity tests of the deck reader; paths must stay
+ * short (readOneBC builds them in a 50-byte buffer, initialize.c:405).
  * The step itself is the reference's solveSourceTerm / solveFreeSurface /
  * updateData / scalarTransport / monitorCFL.  With the Jacobi preconditioner
  * the harness restates solveFreeSurface (nssolve.c:158-178) with
@@ -402,4 +406,89 @@ void ref_destroy(void *hv)
     for (int q = 0; q < h->nblocks; q++) free(h->blocks[q]);
     /* maps, structs and small int arrays are left to process exit */
     free(h);
+}
+
+/* ---- the reference's file input/output, for the deck reader's tests ------ */
+#include "fileio.h"
+
+static Config *io_config(const char *inputFolder, const char *subgridFolder, int NX, int NY)
+{
+    Config *s = (Config *)calloc(1, sizeof(Config));
+    snprintf(s->inputFolder, sizeof s->inputFolder, "%s", inputFolder);
+    if (subgridFolder) snprintf(s->subgridFolder, sizeof s->subgridFolder, "%s", subgridFolder);
+    s->NX = NX; s->NY = NY; s->nx = NX; s->ny = NY; s->np = 1;
+    s->N2CI = NX * NY; s->N2ci = NX * NY; s->N2CT = (NX + 2) * (NY + 2); s->N2ct = s->N2CT;
+    return s;
+}
+
+/* readOneBC (initialize.c:400-460): Nt samples of series `name` */
+int ref_io_read_bc(const char *inputFolder, const char *name, int N, double tNStart, double dt, int Nt,
+                   double *out)
+{
+    Config *s = io_config(inputFolder, NULL, 1, 1);
+    s->tNStart = tNStart;
+    s->dt = dt;
+    s->Nt = Nt;
+    s->tNEnd = tNStart + dt * Nt;
+    readOneBC(out, (char *)name, s, N);
+    free(s);
+    return 0;
+}
+
+/* ReadBathymetry + InitBathymetry (bathymetry.c:19-104) */
+int ref_io_bathymetry(const char *inputFolder, const char *subgridFolder, int useSubgrid, int NX, int NY,
+                      double *allZ, double *allEdgeX, double *allEdgeY, double *offset)
+{
+    Config *s = io_config(inputFolder, subgridFolder, NX, NY);
+    s->useSubgrid = useSubgrid;
+    Bath *b = NULL;
+    ReadBathymetry(&b, s, 0);
+    InitBathymetry(b, s, 0);
+    memcpy(allZ, b->allZ, sizeof(double) * NX * NY);
+    if (useSubgrid) {
+        memcpy(allEdgeX, b->allEdgeX, sizeof(double) * NX * NY);
+        memcpy(allEdgeY, b->allEdgeY, sizeof(double) * NX * NY);
+    }
+    *offset = b->offset[0];
+    free(s);
+    return 0;
+}
+
+/* initAllSubVar (subgrid.c:2112-2284): one level-major table (0..13 = V Z
+ * Vxp Vxm Vyp Vym Np Nm Op Om minVxp minVxm minVyp minVym) */
+int ref_io_subtables(const char *inputFolder, const char *subgridFolder, int NX, int NY, double surfmin,
+                     double surfmax, double dsurf, double offset, int useminV, double *out[14], int *Ne)
+{
+    Config *s = io_config(inputFolder, subgridFolder, NX, NY);
+    s->useSubgrid = 1;
+    s->surfmin = surfmin;
+    s->surfmax = surfmax;
+    s->dsurf = dsurf;
+    s->useminV = useminV;
+    s->useminA = 0;
+    s->useSubDrag = 0;
+    Bath b;
+    memset(&b, 0, sizeof b);
+    double off = offset;
+    b.offset = &off;
+    Sub *sub = NULL;
+    initAllSubVar(&sub, &b, s, 0);
+    *Ne = sub->Ne;
+    double *src[14] = {sub->allV, sub->allZ, sub->allVxp, sub->allVxm, sub->allVyp, sub->allVym, sub->allNp,
+                       sub->allNm, sub->allOp, sub->allOm, sub->allminVxp, sub->allminVxm, sub->allminVyp,
+                       sub->allminVym};
+    for (int t = 0; t < 14; t++)
+        if (out[t] && src[t]) memcpy(out[t], src[t], sizeof(double) * NX * NY * sub->Ne);
+    free(s);
+    return 0;
+}
+
+/* WriteFile (fileio.c:181-206) */
+int ref_io_write(const char *saveFolder, const char *name, int tt, const double *arr, int N)
+{
+    Config *s = io_config("", NULL, 1, 1);
+    snprintf(s->saveFolder, sizeof s->saveFolder, "%s", saveFolder);
+    WriteFile((double *)arr, (char *)name, s, tt, N);
+    free(s);
+    return 0;
 }
