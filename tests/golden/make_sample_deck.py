@@ -2,7 +2,7 @@
 its plain-grid input files) into tests/golden/sample_deck/ and record what
 the reference's own readers make of it (oracle/_ref: readOneBC for the
 boundary series, ReadBathymetry + InitBathymetry for the bed and offset)
-in tests/golden/sample_deck_reference.npz.
+in tests/golden/sample_deck/reference_readers.npz.
 
     TZ=CST6 python tests/golden/make_sample_deck.py
 
@@ -54,7 +54,7 @@ def main():
         out["offset"] = np.array([off.value])
     finally:
         os.chdir(cwd)
-    np.savez_compressed(os.path.join(HERE, "sample_deck_reference.npz"), **out)
+    np.savez_compressed(os.path.join(DST, "reference_readers.npz"), **out)
     print({k: v.shape for k, v in out.items()})
 
 

@@ -6,7 +6,7 @@ tests/golden/make_sample_deck.py):
 * settings: the derived values ReadUserSettings computes
   (configuration.c:18-158) for that deck;
 * boundary series: readOneBC's output (initialize.c:400-460, via
-  oracle/_ref) recorded in tests/golden/sample_deck_reference.npz, bit for
+  oracle/_ref) recorded in tests/golden/sample_deck/reference_readers.npz, bit for
   bit;
 * bed and offset: ReadBathymetry + InitBathymetry's, bit for bit;
 * with the reference present: the writer's files equal WriteFile's byte for
@@ -25,7 +25,7 @@ from frehdc_amd.case import Case
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DECK = os.path.join(HERE, "golden", "sample_deck")
-GOLD = os.path.join(HERE, "golden", "sample_deck_reference.npz")
+GOLD = os.path.join(HERE, "golden", "sample_deck", "reference_readers.npz")
 REFDIR = "/root/reference/FrehdC_v42"
 
 
