@@ -290,8 +290,10 @@ size_t layout(frehd_dev *d, char *base)
     if (k.useSubgrid == 1) {
         double **ngs[] = {&a.V, &a.Z, &a.Vx, &a.Vy, &a.Zx, &a.Zy, &a.Nx, &a.Oy};
         for (auto **p : ngs) *p = cv.take<double>(Ng);
-        double **ns2[] = {&a.Vxp, &a.Vxm, &a.Vyp, &a.Vym, &a.Np, &a.Nm, &a.Op, &a.Om};
+        double **ns2[] = {&a.Vxp, &a.Vxm, &a.Vyp, &a.Vym, &a.Np, &a.Nm, &a.Op, &a.Om, &a.Vraw, &a.Zraw,
+                          &a.hSub};
         for (auto **p : ns2) *p = cv.take<double>(N);
+        a.dSub = cv.take<uint8_t>(N);
         a.ind = cv.take<int>(Ng);
         a.edgeX = cv.take<double>(N);
         a.edgeY = cv.take<double>(N);
@@ -440,6 +442,10 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     HIPCHK(hipMemsetAsync(d->arena, 0, bytes, d->stream));
     size_t used = layout(d, d->arena);
     if (f->manningCell) layout_manning(d, d->arena, used);
+    // no surface seen yet by the subgrid lookup (all-ones bits: a NaN no
+    // surface equals)
+    if (cfg->useSubgrid == 1)
+        HIPCHK(hipMemsetAsync(d->args.hSub, 0xFF, sizeof(double) * d->g.N, d->stream));
     d->args.g = d->g;
     fill_params(d->args.p, d->cfg);
     fill_extents(d);

@@ -243,7 +243,6 @@ int exec_phase(Exec &e, int phase)
     const dim3 b2(64, 4), g2((unsigned)x.gx2, (unsigned)x.gy2, (unsigned)n), b1(256);
     switch (phase) {
     case 0:   // solveSourceTerm (up to the RHS, which the assembly computes)
-        LAUNCH(K_BEGIN, k_step_begin, dim3(1, 1, n), dim3(64), e.dArgs);
         if (e.sub) LAUNCH(K_PRED, k_predictor_sub, g2, b2, e.dArgs);
         else LAUNCH(K_PRED, k_predictor, g2, b2, e.dArgs);
         return 0;
@@ -284,7 +283,7 @@ int exec_phase(Exec &e, int phase)
         }
         return 0;
     case 4:   // monitorCFL's stop test
-        LAUNCH(K_END, k_step_end, dim3(1, 1, n), dim3(256), e.dArgs);
+        LAUNCH(K_END, k_step_end, dim3(1, 1, n), dim3(1024), e.dArgs);
         for (auto *m : e.members) m->tt += 1;
         return 0;
     default:

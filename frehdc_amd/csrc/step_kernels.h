@@ -97,6 +97,11 @@ struct StepArgs {
     // [k*N + c], and are reordered on upload)
     double *V, *Z, *Vx, *Vy, *Zx, *Zy, *Nx, *Oy;
     double *Vxp, *Vxm, *Vyp, *Vym, *Np, *Nm, *Op, *Om;
+    // computeSubArea's own V and Z (before combineSubArea clips them in
+    // place), and the inputs it last saw per cell (surface, dry-start flag):
+    // a cell whose inputs are bitwise unchanged keeps its outputs
+    double *Vraw, *Zraw, *hSub;
+    uint8_t *dSub;
     int *ind;
     const double *edgeX, *edgeY, *allSurf;
     const double *tabs;
@@ -162,7 +167,6 @@ __device__ inline double *depth_nxt(const StepArgs &a) { return a.ctl->parity ? 
 
 enum SubTable { T_V, T_Z, T_VXP, T_VXM, T_VYP, T_VYM, T_NP, T_NM, T_OP, T_OM };
 
-__global__ void k_step_begin(const StepArgs *__restrict__ A);
 __global__ void k_predictor(const StepArgs *__restrict__ A);
 __global__ void k_assemble(const StepArgs *__restrict__ A);
 __global__ void k_surface(const StepArgs *__restrict__ A);
