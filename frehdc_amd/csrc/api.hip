@@ -180,6 +180,22 @@ void host_init(HostState &h, const frehd_config &k, const frehd_forcing &f, cons
 
 }  // namespace
 
+// Copies between host and device go on the handle's own stream and wait
+// for it, never through the null stream: a synchronous null-stream copy from
+// one host thread would interfere with another thread's stream capture.
+static int copy_wait(hipStream_t st, void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+#define COPYW(st, dst, src, bytes, kind)                        \
+    do {                                                        \
+        int rc_ = copy_wait((st), (dst), (src), (bytes), (kind)); \
+        if (rc_) return rc_;                                    \
+    } while (0)
+
 namespace frehd {
 int launch_subgrid_tables(const double *fineDev, double *tabs, int nx, int ny, int r, int Ne,
                           double surfmin, double dsurf, double dxf, hipStream_t stream);
@@ -755,7 +771,7 @@ int frehd_dev_step(frehd_dev_t *d, int32_t nsteps)
     if (nsteps < 0) return fail(FREHD_EINVAL, "nsteps < 0");
     HIPCHK(hipSetDevice(d->device));
     Ctl c;
-    HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
     if (c.halted) return fail(FREHD_ECFL, "handle halted at step %d: max CFL exceeded 20", c.haltStep);
     if (c.tt + nsteps > d->cfg.Nt)
         return fail(FREHD_EINVAL, "stepping to %d exceeds the forcing series (Nt = %d)", c.tt + nsteps, d->cfg.Nt);
@@ -770,7 +786,7 @@ int frehd_dev_step(frehd_dev_t *d, int32_t nsteps)
     int rc = exec_collect_timing(e);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(d->stream));
-    HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
     if (c.halted) return fail(FREHD_ECFL, "max CFL exceeded 20 at step %d (CFLx %.3g, CFLy %.3g)", c.haltStep,
                               c.lastMaxCFLx, c.lastMaxCFLy);
     return 0;
@@ -804,18 +820,18 @@ int frehd_dev_download(frehd_dev_t *d, frehd_fields *o)
         {o->wtfXP, a.wtfXP, Ng}, {o->wtfYP, a.wtfYP, Ng}, {o->bottomZ, a.bottomZ, Ng},
         {o->bottomXP, a.bottomXP, Ng}, {o->bottomYP, a.bottomYP, Ng}};
     for (auto &e : tab)
-        if (e.dst) HIPCHK(hipMemcpy(e.dst, e.src, e.n * sizeof(double), hipMemcpyDeviceToHost));
+        if (e.dst) COPYW(d->stream, e.dst, e.src, e.n * sizeof(double), hipMemcpyDeviceToHost);
     if (a.p.useSubgrid == 1) {
         struct { double *dst; const double *src; } sb[] = {{o->V, a.V}, {o->Z, a.Z}, {o->Vx, a.Vx},
             {o->Vy, a.Vy}, {o->Zx, a.Zx}, {o->Zy, a.Zy}, {o->Nx, a.Nx}, {o->Oy, a.Oy}};
         for (auto &e : sb)
-            if (e.dst) HIPCHK(hipMemcpy(e.dst, e.src, Ng * sizeof(double), hipMemcpyDeviceToHost));
-        if (o->ind) HIPCHK(hipMemcpy(o->ind, a.ind, Ng * sizeof(int), hipMemcpyDeviceToHost));
+            if (e.dst) COPYW(d->stream, e.dst, e.src, Ng * sizeof(double), hipMemcpyDeviceToHost);
+        if (o->ind) COPYW(d->stream, o->ind, a.ind, Ng * sizeof(int), hipMemcpyDeviceToHost);
     }
     const int ns = std::max(1, a.p.nScalar);
     for (int q = 0; q < ns; q++) {
-        if (o->S) HIPCHK(hipMemcpy(o->S + (size_t)q * Ng, a.S + (size_t)q * a.sStride, Ng * sizeof(double), hipMemcpyDeviceToHost));
-        if (o->Sm) HIPCHK(hipMemcpy(o->Sm + (size_t)q * N, a.Sm + (size_t)q * a.sStride, N * sizeof(double), hipMemcpyDeviceToHost));
+        if (o->S) COPYW(d->stream, o->S + (size_t)q * Ng, a.S + (size_t)q * a.sStride, Ng * sizeof(double), hipMemcpyDeviceToHost);
+        if (o->Sm) COPYW(d->stream, o->Sm + (size_t)q * N, a.Sm + (size_t)q * a.sStride, N * sizeof(double), hipMemcpyDeviceToHost);
     }
     return 0;
 }
@@ -826,7 +842,7 @@ int frehd_dev_stats(frehd_dev_t *d, frehd_stats *out)
     HIPCHK(hipSetDevice(d->device));
     HIPCHK(hipStreamSynchronize(d->stream));
     Ctl c;
-    HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
     out->steps = c.steps;
     out->pcgIterTotal = c.pcgIterTotal;
     out->pcgIterLast = c.pcgIterLast;
@@ -845,7 +861,7 @@ int frehd_dev_step_index(frehd_dev_t *d, int32_t *tt)
     if (!d || !tt) return fail(FREHD_EINVAL, "NULL argument");
     Ctl c;
     HIPCHK(hipStreamSynchronize(d->stream));
-    HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
     *tt = c.tt;
     return 0;
 }
@@ -856,7 +872,7 @@ int frehd_dev_set_step_index(frehd_dev_t *d, int32_t tt)
     if (tt < 0 || tt > d->cfg.Nt) return fail(FREHD_EINVAL, "step index %d outside [0, Nt]", tt);
     HIPCHK(hipSetDevice(d->device));
     HIPCHK(hipStreamSynchronize(d->stream));
-    HIPCHK(hipMemcpy(&d->args.ctl->tt, &tt, sizeof(int), hipMemcpyHostToDevice));
+    COPYW(d->stream, &d->args.ctl->tt, &tt, sizeof(int), hipMemcpyHostToDevice);
     return 0;
 }
 
@@ -866,6 +882,7 @@ int frehd_dev_destroy(frehd_dev_t *d)
     if (d->batch) frehd_batch_remove(static_cast<frehd_batch_t *>(d->batch), d);
     hipSetDevice(d->device);
     hipStreamSynchronize(d->stream);
+    exec_release_graphs(d->self);
     if (d->arena) hipFree(d->arena);
     if (d->self.hFlags) hipHostFree(d->self.hFlags);
     if (d->ownStream) hipStreamDestroy(d->stream);
@@ -1015,7 +1032,7 @@ int frehd_batch_step(frehd_batch_t *b, int32_t nsteps)
     HIPCHK(hipSetDevice(e.device));
     for (auto *d : e.members) {
         Ctl c;
-        HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+        COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
         if (!c.halted && c.tt + nsteps > d->cfg.Nt)
             return fail(FREHD_EINVAL, "a member steps to %d beyond its forcing series (Nt = %d)", c.tt + nsteps,
                         d->cfg.Nt);
@@ -1032,7 +1049,7 @@ int frehd_batch_step(frehd_batch_t *b, int32_t nsteps)
     int halted = 0;
     for (auto *d : e.members) {
         Ctl c;
-        HIPCHK(hipMemcpy(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+        COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
         halted += c.halted != 0;
     }
     if (halted) return fail(FREHD_ECFL, "%d batch member(s) halted: max CFL exceeded 20", halted);
@@ -1061,6 +1078,7 @@ int frehd_batch_destroy(frehd_batch_t *b)
     hipSetDevice(e.device);
     if (e.stream) hipStreamSynchronize(e.stream);
     for (auto *d : e.members) d->batch = nullptr;
+    exec_release_graphs(e);
     if (e.dArgs) hipFree(e.dArgs);
     if (e.dPcg) hipFree(e.dPcg);
     if (e.dPcg32) hipFree(e.dPcg32);

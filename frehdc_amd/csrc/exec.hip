@@ -85,10 +85,17 @@ int launch(Exec &e, KernelId id, F &&fn)
 
 inline dim3 grid1(int nb, int n) { return dim3((unsigned)std::max(nb, 1), 1, (unsigned)n); }
 
-int gather(Exec &e)
+int enqueue_gather(Exec &e)
 {
     LAUNCH(K_GATHER, k_gather_flags, dim3((unsigned)((e.count + 63) / 64)), dim3(64), e.dArgs, e.count, e.dFlags);
     HIPCHK(hipMemcpyAsync(e.hFlags, e.dFlags, sizeof(int32_t) * F_STRIDE * e.count, hipMemcpyDeviceToHost, e.stream));
+    return 0;
+}
+
+int gather(Exec &e)
+{
+    int rc = enqueue_gather(e);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(e.stream));
     return 0;
 }
@@ -102,27 +109,38 @@ int enqueue_pcg_iteration(Exec &e, int it)
     return 0;
 }
 
-// FP64 PCG: the previous step's iteration count, then small chunks, until
-// every member reports convergence (or a halt); one iteration per chunk
-// while timing, so no idle launch enters the averages
-int solve_fp64(Exec &e)
+// FP64 PCG.  The step enqueues the previous step's iteration count (one
+// iteration while timing, so no idle launch enters the averages) and gathers
+// the flags; solve_fp64_finish then adds small chunks until every member
+// reports convergence (or a halt).
+int solve_fp64_pattern(Exec &e, int *it)
 {
     const bool timing = e.tim && e.tim->on;
-    int it = 0;
-    int chunk = timing ? 1 : std::max(1, e.lastIters);
+    const int n = timing ? 1 : std::max(1, e.lastIters);
+    for (int q = 0; q < n; q++) {
+        int rc = enqueue_pcg_iteration(e, q);
+        if (rc) return rc;
+    }
+    *it = n;
+    return 0;
+}
+
+int solve_fp64_finish(Exec &e, int it)
+{
+    const bool timing = e.tim && e.tim->on;
     for (;;) {
+        bool all = true;
+        for (int m = 0; m < e.count; m++)
+            if (flags(e, m)[F_DONE] < 0 && !flags(e, m)[F_HALTED]) all = false;
+        if (all) break;
+        if (it > e.maxIter + 8) return fail(FREHD_EHIP, "PCG did not report completion");
+        const int chunk = timing ? 1 : 4;
         for (int q = 0; q < chunk; q++, it++) {
             int rc = enqueue_pcg_iteration(e, it);
             if (rc) return rc;
         }
         int rc = gather(e);
         if (rc) return rc;
-        bool all = true;
-        for (int m = 0; m < e.count; m++)
-            if (flags(e, m)[F_DONE] < 0 && !flags(e, m)[F_HALTED]) all = false;
-        if (all) break;
-        if (it > e.maxIter + 8) return fail(FREHD_EHIP, "PCG did not report completion");
-        chunk = timing ? 1 : 4;
     }
     int last = 1;
     for (int m = 0; m < e.count; m++)
@@ -165,7 +183,7 @@ int enqueue_inner_solve(Exec &e, int n)
 // sweep recomputes the true residual).  If the pattern ends before every
 // member's outer test passes, further sweeps follow, each inner solve run to
 // its own stop test.
-int solve_mixed(Exec &e)
+int solve_mixed_pattern(Exec &e, int *sweeps)
 {
     int rc, s = 0;
     if ((rc = enqueue_refine(e, 0))) return rc;
@@ -173,7 +191,13 @@ int solve_mixed(Exec &e)
         if ((rc = enqueue_inner_solve(e, e.mixPlan[k]))) return rc;
         if ((rc = enqueue_refine(e, ++s))) return rc;
     }
-    if ((rc = gather(e))) return rc;
+    *sweeps = s;
+    return 0;
+}
+
+int solve_mixed_finish(Exec &e, int s)
+{
+    int rc;
     auto outer_open = [&](int m) { return flags(e, m)[F_REFDONE] < 0 && !flags(e, m)[F_HALTED]; };
     auto any_open = [&] {
         for (int m = 0; m < e.count; m++)
@@ -235,6 +259,14 @@ int exec_collect_timing(Exec &e)
     return 0;
 }
 
+int enqueue_assemble(Exec &e)
+{
+    const int n = e.count;
+    if (e.sub) LAUNCH(K_ASM, k_assemble_sub, grid1(e.ext.nbRes, n), dim3(256), e.dArgs);
+    else LAUNCH(K_ASM, k_assemble, grid1(e.ext.nbRes, n), dim3(256), e.dArgs);
+    return 0;
+}
+
 int exec_phase(Exec &e, int phase)
 {
     const int n = e.count;
@@ -247,9 +279,10 @@ int exec_phase(Exec &e, int phase)
         else LAUNCH(K_PRED, k_predictor, g2, b2, e.dArgs);
         return 0;
     case 1: { // solveFreeSurface
-        if (e.sub) LAUNCH(K_ASM, k_assemble_sub, grid1(x.nbRes, n), b1, e.dArgs);
-        else LAUNCH(K_ASM, k_assemble, grid1(x.nbRes, n), b1, e.dArgs);
-        int rc = e.mixed ? solve_mixed(e) : solve_fp64(e);
+        int rc = enqueue_assemble(e), k = 0;
+        if (!rc) rc = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
+        if (!rc) rc = gather(e);
+        if (!rc) rc = e.mixed ? solve_mixed_finish(e, k) : solve_fp64_finish(e, k);
         if (rc) return rc;
         LAUNCH(K_SURF, k_surface, grid1(x.nb1, n), b1, e.dArgs);
         return 0;
@@ -291,8 +324,99 @@ int exec_phase(Exec &e, int phase)
     }
 }
 
+void exec_release_graphs(Exec &e)
+{
+    for (auto &h : e.heads) hipGraphExecDestroy(h.second);
+    e.heads.clear();
+    if (e.tail) hipGraphExecDestroy(e.tail);
+    e.tail = nullptr;
+}
+
+// capture what `enqueue` launches on the group's stream into an executable
+// graph
+template <class F>
+static int capture(Exec &e, F &&enqueue, hipGraphExec_t *out)
+{
+    HIPCHK(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue();
+    hipGraph_t g = nullptr;
+    hipError_t err = hipStreamEndCapture(e.stream, &g);
+    if (rc || err != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return rc ? rc : fail(FREHD_EHIP, "stream capture failed: %s", hipGetErrorString(err));
+    }
+    err = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (err != hipSuccess) return fail(FREHD_EHIP, "graph instantiation failed: %s", hipGetErrorString(err));
+    return 0;
+}
+
+// One step as two graph launches and one synchronisation: the head
+// (predictor, assembly, the solver's launch pattern of the last step, the
+// flag gather) and, once the flags show every member's solve finished (after
+// eager extra iterations where needed), the tail.  The same kernels with the
+// same arguments as the eager path, so the results are identical.
+static int exec_step_graphs(Exec &e)
+{
+    const int n = e.count;
+    std::vector<int> key;
+    if (e.mixed) {
+        key.push_back(1);
+        key.insert(key.end(), e.mixPlan.begin(), e.mixPlan.end());
+    } else {
+        key = {0, std::max(1, e.lastIters)};
+    }
+    hipGraphExec_t head = nullptr;
+    for (auto &h : e.heads)
+        if (h.first == key) head = h.second;
+    int k = 0, rc;
+    if (!head) {
+        if (e.heads.size() >= 16) exec_release_graphs(e);
+        rc = capture(e, [&] {
+            const dim3 b2(64, 4), g2((unsigned)e.ext.gx2, (unsigned)e.ext.gy2, (unsigned)n);
+            if (e.sub) LAUNCH(K_PRED, k_predictor_sub, g2, b2, e.dArgs);
+            else LAUNCH(K_PRED, k_predictor, g2, b2, e.dArgs);
+            int r = enqueue_assemble(e);
+            if (!r) r = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
+            if (!r) r = enqueue_gather(e);
+            return r;
+        }, &head);
+        if (rc) return rc;
+        e.heads.push_back({key, head});
+    } else {
+        k = e.mixed ? (int)e.mixPlan.size() : std::max(1, e.lastIters);
+    }
+    HIPCHK(hipGraphLaunch(head, e.stream));
+    HIPCHK(hipStreamSynchronize(e.stream));
+    rc = e.mixed ? solve_mixed_finish(e, k) : solve_fp64_finish(e, k);
+    if (rc) return rc;
+    if (!e.tail) {
+        rc = capture(e, [&] {
+            LAUNCH(K_SURF, k_surface, grid1(e.ext.nb1, n), dim3(256), e.dArgs);
+            for (int ph = 2; ph <= 4; ph++) {
+                int r = exec_phase(e, ph);
+                if (r) return r;
+            }
+            return 0;
+        }, &e.tail);
+        if (rc) return rc;
+        // the capture ran the host bookkeeping of phases 2 and 4 once
+        for (auto *m : e.members) {
+            m->parity ^= 1;
+            m->tt -= 1;
+        }
+    }
+    HIPCHK(hipGraphLaunch(e.tail, e.stream));
+    for (auto *m : e.members) {
+        m->parity ^= 1;
+        m->tt += 1;
+    }
+    return 0;
+}
+
 int exec_step(Exec &e)
 {
+    if (e.useGraphs && !(e.tim && e.tim->on) && e.count > 0) return exec_step_graphs(e);
     for (int ph = 0; ph < 5; ph++) {
         int rc = exec_phase(e, ph);
         if (rc) return rc;
@@ -354,6 +478,7 @@ static void merge(Exec &e, const frehd_dev_impl *d, bool first)
 
 int exec_refresh(Exec &e)
 {
+    exec_release_graphs(e);
     e.count = (int)e.members.size();
     if (e.count > e.capacity) return fail(FREHD_EINVAL, "group holds %d members, capacity %d", e.count, e.capacity);
     for (int m = 0; m < e.count; m++) {

@@ -78,7 +78,18 @@ struct Exec {
     std::vector<int> mixPlan;           // inner iterations per sweep of the last mixed solve
     Timing *tim = nullptr;
     std::vector<frehd_dev_impl *> members;
+    // captured steps (hipGraph): the step's head (predictor, assembly, the
+    // solver's launch pattern, flag gather) per pattern, and its tail
+    // (surface update, updateData, scalars, CFL test); rebuilt whenever the
+    // members' argument blocks change
+    int useGraphs = 1;
+    std::vector<std::pair<std::vector<int>, hipGraphExec_t>> heads;
+    hipGraphExec_t tail = nullptr;
 };
+
+// drop the captured steps of a group (after its members or their arguments
+// change, and before it is destroyed)
+void exec_release_graphs(Exec &e);
 
 struct frehd_dev_impl {
     frehd_config cfg;

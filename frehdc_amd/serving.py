@@ -131,12 +131,12 @@ class Server:
                 if mem:
                     k = min(self.quantum, min(r.steps - r.done_steps for r in mem))
                     work.append((key, mem, k))
-            futs = [(key, mem, k, self.pool.submit(self.batches[key].try_step, k)) for key, mem, k in work]
+            futs = [(key, mem, k, self.pool.submit(_step_in_thread, self.batches[key], k)) for key, mem, k in work]
             for key, mem, k, fut in futs:
-                rc = fut.result()
+                rc, msg = fut.result()
                 b = self.batches[key]
                 if rc not in (0, abi.FREHD_ECFL):
-                    abi.check(rc)
+                    raise abi.FrehdError(rc, msg)
                 for r in mem:
                     r.done_steps += k
                 halted = {id(r) for r in mem if rc == abi.FREHD_ECFL and r.dev.stats()["halted"]}
@@ -172,6 +172,13 @@ class Server:
             "peak_resident_scenarios": int(peak_members),
             "rounds": rounds,
         }
+
+
+def _step_in_thread(batch, k):
+    """Step a batch; the library's error message is per thread, so read it
+    in the thread that made the call."""
+    rc = batch.try_step(k)
+    return rc, (batch.lib.frehd_last_error().decode() if rc else "")
 
 
 def make_requests(count, rate, seed=6, lo=64, hi=1024, steps_lo=200, steps_hi=2000, subgrid_r=0, workers=1):
