@@ -35,6 +35,7 @@ class FrehdConfig(C.Structure):
         ("epsInner", C.c_double), ("maxRefine", C.c_int32),
         ("useSubgrid", C.c_int32), ("Ne", C.c_int32), ("warmStart", C.c_int32),
         ("surfmin", C.c_double), ("dsurf", C.c_double), ("subR", C.c_double),
+        ("subgridOnTheFly", C.c_int32), ("pad2", C.c_int32),
     ]
 
 

@@ -127,7 +127,7 @@ def default_config(nx, ny, dx, dt, Nt, **kw) -> dict:
         tideLocLengthM=0, inflowLocLength=0, useScalar=1, scalarAdv=1, useConstTidePS=1,
         useConstTideMS=1, nScalar=1, tidePS=0.0, tideMS=0.0, inflowS=0.0, Kx=1e-4, Ky=1e-4,
         offset=0.0, epsInner=1e-5, maxRefine=8, warmStart=0, useSubgrid=0, Ne=0, surfmin=0.0, dsurf=0.01,
-        subR=1.0,
+        subR=1.0, subgridOnTheFly=0,
     )
     cfg.update(kw)
     return cfg
@@ -244,6 +244,74 @@ def channel(n=4096, dx=1.0, dt=0.5, Nt=500, seed=4, hurst=0.8, amp=0.5, q_river=
                    inflow=inflow, name=f"channel{n}")
 
 
+def fbm_pixels(nx, ny, hurst, rng, lo, hi, workers=None):
+    """fBm surface as fbm_surface (rescaled to [lo, hi]) for pixel grids of
+    10^8 points and more: single-precision spectrum, multithreaded inverse
+    FFT (scipy.fft), float64 result."""
+    import os
+    import scipy.fft
+    kx = np.fft.fftfreq(nx).astype(np.float32)[None, :]
+    ky = np.fft.fftfreq(ny).astype(np.float32)[:, None]
+    k = np.sqrt(kx * kx + ky * ky)
+    k[0, 0] = 1.0
+    amp = k ** np.float32(-(hurst + 1.0))
+    amp[0, 0] = 0.0
+    del k
+    spec = np.empty((ny, nx), dtype=np.complex64)
+    for j0 in range(0, ny, 1024):                     # row blocks: bounded temporaries
+        j1 = min(ny, j0 + 1024)
+        phase = rng.uniform(0, 2 * np.pi, size=(j1 - j0, nx)).astype(np.float32)
+        noise = rng.normal(size=(j1 - j0, nx)).astype(np.float32)
+        spec[j0:j1] = amp[j0:j1] * noise * (np.cos(phase) + 1j * np.sin(phase))
+    del amp
+    f = scipy.fft.ifft2(spec, workers=workers or min(16, os.cpu_count() or 1), overwrite_x=True).real
+    del spec
+    fmin, fmax = float(f.min()), float(f.max())
+    out = np.empty((ny, nx), dtype=np.float64)
+    np.subtract(f, fmin, out=out)
+    out *= (hi - lo) / max(fmax - fmin, 1e-30)
+    out += lo
+    return out
+
+
+def channel_subgrid(n=4096, r=4, dx=1.0, dt=0.5, Nt=500, seed=4, hurst=0.8, amp=0.5, q_river=50.0,
+                    nscalar=3, lo=-1.5, hi=1.0, width=20, depth=2.0, dsurf=0.01, on_the_fly=True):
+    """The large domain on the subgrid model: r x r pixels per cell of an fBm
+    bed (Hurst 0.8) in [lo, hi] with a meandering channel (depth m deep,
+    `width` cells wide) carved at pixel resolution; tide amp on the YP row,
+    river inflow q_river m^3/s spread over the channel cells of row 0; three
+    scalars: a salinity-like 0..35 gradient along y, a temperature-like
+    10..30 gradient along x, a tracer 0/1 patch; boundary scalar 1 at the
+    tide.  Levels span [lo - depth, hi] every dsurf; the device keeps the
+    pixels and evaluates levels per lookup (on_the_fly) since the tables of
+    a 4096^2 grid would need 10 x Ne x N values."""
+    rng = np.random.default_rng(seed)
+    fy = fx = n * r
+    fine = fbm_pixels(fx, fy, hurst, rng, lo, hi)
+    yp = np.arange(fy, dtype=np.float64)
+    centre = fx / 2 + 0.2 * fx * np.sin(2 * np.pi * yp / fy * 2.0)
+    xp = np.arange(fx, dtype=np.float64)
+    for y0 in range(0, fy, 2048):
+        y1 = min(fy, y0 + 2048)
+        inside = np.abs((xp[None, :] + 0.5) - centre[y0:y1, None]) < width * r / 2
+        fine[y0:y1][inside] -= depth
+    cfg = default_config(n, n, dx, dt, Nt, eps=1e-10, nScalar=nscalar, useConstTidePS=1, tidePS=1.0)
+    N = n * n
+    jj, ii = np.divmod(np.arange(N), n)
+    s0 = [35.0 * jj / (n - 1), 10.0 + 20.0 * ii / (n - 1),
+          ((np.abs(ii - n // 2) < n // 8) & (np.abs(jj - n // 2) < n // 8)).astype(np.float64)]
+    s0 = np.concatenate([s0[q % 3] for q in range(nscalar)])
+    # inflow cells: row 0 cells whose middle pixel lies in the channel
+    mid = np.abs((xp[r // 2::r] + 0.5) - centre[r // 2]) < width * r / 2
+    row0 = np.nonzero(mid)[0].astype(np.int32)
+    inflow = np.full(Nt + 1, q_river / len(row0))
+    sc = _finish(cfg, np.zeros(N), tide_series(Nt, dt, amp), scalar0=s0, inflowLoc=row0, inflow=inflow,
+                 name=f"channel{n}")
+    sc = with_subgrid(sc, fine, r, lo - depth, hi, dsurf, on_the_fly=on_the_fly)
+    sc.name = f"channel{n}_sub{r}"
+    return sc
+
+
 def ensemble_member(m, n=2048, dx=1.0, dt=0.5, Nt=1000, seed=5, hurst=0.7, base=None):
     """Ensemble member m: shared fBm base plus a seeded N(0, 0.05) perturbation,
     Manning n ~ U(0.015, 0.04) and tide amplitude ~ U(0.3, 0.8) per member."""
@@ -295,12 +363,14 @@ def serving_requests(count=512, rate=20.0, seed=6, lo=64, hi=1024, dx=2.0, dt=0.
 
 
 def with_subgrid(scen: Scenario, fine: np.ndarray, r: int, surfmin: float, surfmax: float,
-                 dsurf: float = 0.01, device_tables: bool = False) -> Scenario:
+                 dsurf: float = 0.01, device_tables: bool = False, on_the_fly: bool = False) -> Scenario:
     """Turn a scenario into its subgrid form: `fine` holds the pixel bed
     elevations [ny*r, nx*r]; the coarse bed is each block's minimum
     (bottom.dat), face bottoms and level tables come from subgrid.tables(),
     or, with device_tables, the fine bathymetry is handed to the device
-    library, which generates the tables itself."""
+    library, which generates the tables itself (with on_the_fly it keeps the
+    pixels and evaluates the levels at each lookup instead)."""
+    device_tables = device_tables or on_the_fly
     from . import subgrid as sgm
     nx, ny = scen.nx, scen.ny
     assert fine.shape == (ny * r, nx * r)
@@ -313,7 +383,8 @@ def with_subgrid(scen: Scenario, fine: np.ndarray, r: int, surfmin: float, surfm
         tabs = sgm.tables(fine, r, dxf, surfmin, dsurf, Ne)
     scen.bottomZ = np.ascontiguousarray(bottom.reshape(-1))
     scen.cfg["offset"] = offset_for(scen.bottomZ)
-    scen.cfg.update(useSubgrid=1, Ne=Ne, surfmin=float(surfmin), dsurf=float(dsurf), subR=float(r))
+    scen.cfg.update(useSubgrid=1, Ne=Ne, surfmin=float(surfmin), dsurf=float(dsurf), subR=float(r),
+                    subgridOnTheFly=1 if on_the_fly else 0)
     tabs["edgeX"] = np.ascontiguousarray(edgeX.reshape(-1))
     tabs["edgeY"] = np.ascontiguousarray(edgeY.reshape(-1))
     scen.subgrid = tabs
@@ -349,16 +420,16 @@ def fine_marsh(n, r, seed=2, hurst=0.7, lo=-1.0, hi=1.5):
 
 
 def marsh_subgrid(n=48, r=8, dx=2.0, dt=0.5, Nt=400, amp=0.6, period=TIDE_PERIOD, seed=2,
-                  dsurf=0.01, manning_cell=True, device_tables=False):
+                  dsurf=0.01, manning_cell=True, device_tables=False, on_the_fly=False):
     """The marsh workload on the subgrid model: r x r pixels per cell, tables
     from surfmin = -1.0 m to surfmax = +1.5 m every dsurf (generated on the
-    device with device_tables)."""
+    device with device_tables, evaluated per lookup with on_the_fly)."""
     sc = marsh(n=n, dx=dx, dt=dt, Nt=Nt, seed=seed, amp=amp)
     if period != TIDE_PERIOD:
         sc.tideP[:] = tide_series(Nt, dt, amp, period=period)
     if not manning_cell:
         sc.manningCell = None
     fine = fine_marsh(n, r, seed=seed)
-    sc = with_subgrid(sc, fine, r, -1.0, 1.5, dsurf, device_tables=device_tables)
+    sc = with_subgrid(sc, fine, r, -1.0, 1.5, dsurf, device_tables=device_tables, on_the_fly=on_the_fly)
     sc.name = f"marsh{n}_sub{r}"
     return sc

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -244,6 +245,15 @@ inline DirTiling dir_tiling(int nx, int ny, int H, int remap)
 
 inline size_t max_dir_blocks(int nx, int ny) { return (size_t)((nx + 255) / 256) * ny + 8; }
 
+// fine pixels per coarse cell side of the on-the-fly subgrid, 0 when subR
+// is not a whole number
+static int fine_r(const frehd_config &k)
+{
+    const double r = std::floor(k.subR + 0.5);
+    if (!(r >= 1 && r <= 64 && std::fabs(k.subR - r) < 1e-9)) return 0;
+    return (int)r;
+}
+
 // lay out every device array; with base == nullptr only the size is computed
 size_t layout(frehd_dev *d, char *base)
 {
@@ -314,7 +324,15 @@ size_t layout(frehd_dev *d, char *base)
         a.edgeX = cv.take<double>(N);
         a.edgeY = cv.take<double>(N);
         a.allSurf = cv.take<double>((size_t)k.Ne);
-        a.tabs = cv.take<double>((size_t)k.Ne * N * 10);
+        a.tabs = nullptr;
+        a.bed = FineBed{nullptr, k.nx, k.ny, 0, 0.0};
+        if (k.subgridOnTheFly == 1) {
+            const int r = fine_r(k);
+            a.bed.r = r;
+            a.bed.z = cv.take<double>(N * (size_t)r * r);
+        } else {
+            a.tabs = cv.take<double>((size_t)k.Ne * N * 10);
+        }
     }
     return cv.off + 256;
 }
@@ -353,6 +371,12 @@ int check_config(const frehd_config *k)
     if (k->useSubgrid != 0 && k->useSubgrid != 1) return fail(FREHD_EINVAL, "useSubgrid must be 0 or 1");
     if (k->useSubgrid == 1 && !(k->Ne >= 1 && k->dsurf > 0 && k->subR > 0))
         return fail(FREHD_EINVAL, "subgrid needs Ne >= 1, dsurf > 0, subR > 0");
+    if (k->subgridOnTheFly != 0 && k->subgridOnTheFly != 1)
+        return fail(FREHD_EINVAL, "subgridOnTheFly must be 0 or 1");
+    if (k->subgridOnTheFly == 1 && k->useSubgrid != 1)
+        return fail(FREHD_EINVAL, "subgridOnTheFly needs useSubgrid = 1");
+    if (k->subgridOnTheFly == 1 && fine_r(*k) < 1)
+        return fail(FREHD_EINVAL, "subgridOnTheFly needs an integral subR >= 1 (subR = %g)", k->subR);
     return 0;
 }
 
@@ -391,7 +415,7 @@ void fill_params(P &p, const frehd_config &k)
     p.inflowLocLength = k.inflowLocLength; p.useScalar = k.useScalar;
     p.nScalar = k.useScalar == 1 ? std::max(1, k.nScalar) : 0;
     p.useConstTidePS = k.useConstTidePS; p.useConstTideMS = k.useConstTideMS; p.maxIter = k.maxIter;
-    p.useSubgrid = k.useSubgrid; p.dsurf = k.dsurf; p.subR = k.subR; p.Ne = k.Ne;
+    p.useSubgrid = k.useSubgrid; p.dsurf = k.dsurf; p.subR = k.subR; p.Ne = k.Ne; p.surfmin = k.surfmin;
     p.warmStart = k.precision == FREHD_PREC_MIXED ? k.warmStart : 0;
 }
 
@@ -538,9 +562,12 @@ int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
     const size_t N = d->g.N, tn = (size_t)k.Ne * N;
     if (!sg || !sg->edgeX || !sg->edgeY) return fail(FREHD_EINVAL, "subgrid tables missing");
     const double *tabs[10] = {sg->V, sg->Z, sg->Vxp, sg->Vxm, sg->Vyp, sg->Vym, sg->Np, sg->Nm, sg->Op, sg->Om};
-    const bool generate = sg->fine != nullptr && !sg->V;
+    const bool onTheFly = k.subgridOnTheFly == 1;
+    const bool generate = sg->fine != nullptr && (!sg->V || onTheFly);
     if (generate && !(sg->fineR >= 1 && sg->dxf > 0))
         return fail(FREHD_EINVAL, "fine bathymetry needs fineR >= 1 and dxf > 0");
+    if (onTheFly && !(generate && sg->fineR == a.bed.r))
+        return fail(FREHD_EINVAL, "subgridOnTheFly needs the fine bathymetry with fineR = subR");
     for (int t = 0; t < 10 && !generate; t++)
         if (!tabs[t]) return fail(FREHD_EINVAL, "subgrid table %d missing", t);
     std::vector<double> ex(N), ey(N), lev(k.Ne);
@@ -550,6 +577,13 @@ int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
     if ((rc = upload(d, ex.data(), (void *)a.edgeX, N * sizeof(double)))) return rc;
     if ((rc = upload(d, ey.data(), (void *)a.edgeY, N * sizeof(double)))) return rc;
     if ((rc = upload(d, lev.data(), (void *)a.allSurf, k.Ne * sizeof(double)))) return rc;
+    if (onTheFly) {
+        // the argument block went to the device before the pixel size was
+        // known: send it again
+        a.bed.dxf = sg->dxf;
+        rc = upload(d, sg->fine, (void *)a.bed.z, N * (size_t)a.bed.r * a.bed.r * sizeof(double));
+        return rc ? rc : upload_member_args(d);
+    }
     if (generate) {
         const size_t nf = N * (size_t)sg->fineR * sg->fineR;
         double *fine = nullptr;
@@ -928,20 +962,38 @@ int frehd_dev_download_tables(frehd_dev_t *d, double *const tables[10])
     HIPCHK(hipSetDevice(d->device));
     HIPCHK(hipStreamSynchronize(d->stream));
     const size_t tn = (size_t)d->cfg.Ne * d->g.N;
+    const StepArgs &a = d->args;
+    // on the fly: generate the interleaved tables from the pixels first
+    double *gen = nullptr;
+    if (!a.tabs) {
+        if (hipMalloc(&gen, tn * 10 * sizeof(double)) != hipSuccess)
+            return fail(FREHD_ENOMEM, "hipMalloc of the generated tables (%zu bytes) failed", tn * 10 * sizeof(double));
+        if (launch_subgrid_tables(a.bed.z, gen, d->cfg.nx, d->cfg.ny, a.bed.r, d->cfg.Ne, d->cfg.surfmin,
+                                  d->cfg.dsurf, a.bed.dxf, d->stream)) {
+            hipFree(gen);
+            return fail(FREHD_EHIP, "table generator launch failed");
+        }
+    }
+    const double *src = a.tabs ? a.tabs : gen;
     double *stage = nullptr;
-    if (hipMalloc(&stage, tn * sizeof(double)) != hipSuccess)
+    if (hipMalloc(&stage, tn * sizeof(double)) != hipSuccess) {
+        hipStreamSynchronize(d->stream);
+        if (gen) hipFree(gen);
         return fail(FREHD_ENOMEM, "hipMalloc of the table staging buffer failed");
+    }
     int rc = 0;
     for (int t = 0; t < 10 && !rc; t++) {
         if (!tables[t]) continue;
-        if (launch_tables_reorder(const_cast<double *>(d->args.tabs), stage, (int)d->g.N, d->cfg.Ne, t, 0,
+        if (launch_tables_reorder(const_cast<double *>(src), stage, (int)d->g.N, d->cfg.Ne, t, 0,
                                   d->stream))
             rc = fail(FREHD_EHIP, "table reorder launch failed");
         else if (hipMemcpyAsync(tables[t], stage, tn * sizeof(double), hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
                  hipStreamSynchronize(d->stream) != hipSuccess)
             rc = fail(FREHD_EHIP, "table download failed");
     }
+    hipStreamSynchronize(d->stream);
     hipFree(stage);
+    if (gen) hipFree(gen);
     return rc;
 }
 

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "frehd_dev.h"
+#include "subgrid_eval.h"
 
 namespace frehd {
 
@@ -16,7 +17,7 @@ struct P {
     int CDnotN, useThinLayer, bcType, tideLocLengthP, tideLocLengthM, inflowLocLength;
     int useScalar, nScalar, useConstTidePS, useConstTideMS, maxIter, useSubgrid;
     // subgrid
-    double dsurf, subR;
+    double dsurf, subR, surfmin;   // surfmin: raw elevation of level 0
     int Ne;
     int warmStart;            // mixed precision: refinement starts from eta^n instead of 0
 };
@@ -105,6 +106,9 @@ struct StepArgs {
     int *ind;
     const double *edgeX, *edgeY, *allSurf;
     const double *tabs;
+    // instead of tabs (NULL): the fine bathymetry, from which a lookup
+    // evaluates its two levels (cell_levels, bitwise the table values)
+    FineBed bed;
 };
 
 __host__ __device__ inline size_t tab_index(size_t c, int k, int Ne, int T)

@@ -826,13 +826,25 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
     const int j1 = ind;
     const int j2 = j1 == 0 ? 0 : (j1 == Ne - 1 ? Ne - 1 : (h > as[j1] ? j1 + 1 : j1 - 1));
     const double s1 = as[j1], s2 = as[j2];
-    // the two levels' 10 values each (adjacent in the interleaved layout)
+    // the two levels' 10 values each: adjacent in the interleaved tables, or
+    // evaluated from the cell's pixels
     double t1[10], t2[10];
-    const double *p1 = a.tabs + tab_index(c, j1, Ne, 0), *p2 = a.tabs + tab_index(c, j2, Ne, 0);
+    if (a.tabs) {
+        const double *p1 = a.tabs + tab_index(c, j1, Ne, 0), *p2 = a.tabs + tab_index(c, j2, Ne, 0);
 #pragma unroll
-    for (int q = 0; q < 10; q++) {
-        t1[q] = p1[q];
-        t2[q] = p2[q];
+        for (int q = 0; q < 10; q++) {
+            t1[q] = p1[q];
+            t2[q] = p2[q];
+        }
+    } else {
+        const double hl[2] = {level_raw(p.surfmin, dsurf, j1), level_raw(p.surfmin, dsurf, j2)};
+        double v[2][10];
+        cell_levels<2>(a.bed, i, j, hl, v);
+#pragma unroll
+        for (int q = 0; q < 10; q++) {
+            t1[q] = v[0][q];
+            t2[q] = v[1][q];
+        }
     }
     a.Vraw[c] = interp_surf(h, s1, t1[T_V], s2, t2[T_V]);
     a.Zraw[c] = interp_surf(h, s1, t1[T_Z], s2, t2[T_Z]);

@@ -98,6 +98,14 @@ typedef struct frehd_config {
                                 extrapolation 2 eta^n - eta^(n-1)                  */
     double  surfmin, dsurf;  /* level k is surfmin + k*dsurf (+ offset)           */
     double  subR;            /* dx / dxf (fine cells per coarse cell side)        */
+    /* useSubgrid == 1 only: 0 keeps the Ne x N x 10 level tables on the device;
+     * 1 keeps the fine bathymetry instead (frehd_subgrid.fine, fineR == subR)
+     * and evaluates the two levels each lookup needs from the cell's pixels,
+     * bitwise the values the tables hold.  For grids whose tables do not fit
+     * (4096^2 at r = 4 and 450 levels: 600 GB of tables, 2 GB of pixels).
+     * No counterpart in the reference, which reads tables from files. */
+    int32_t subgridOnTheFly;
+    int32_t pad2;
 } frehd_config;
 
 /* ---- forcing ----------------------------------------------------------------

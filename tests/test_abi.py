@@ -68,3 +68,21 @@ def test_footprint_scales(lib):
     assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
     per_cell = b.value / sc.N
     assert 200 < per_cell < 600, per_cell
+
+
+def test_on_the_fly_footprint():
+    """Without tables the handle holds r^2 pixels per cell instead of
+    10 x Ne table values."""
+    from frehdc_amd import cases
+    sc = cases.marsh_subgrid(n=64, r=4, Nt=10, device_tables=True)
+    fly = dict(sc.cfg, subgridOnTheFly=1)
+    lib = abi.load_library()
+    import ctypes as C
+    t, f = C.c_int64(), C.c_int64()
+    abi.check(lib.frehd_dev_footprint(C.byref(sc.c_config()), C.byref(t)))
+    cf = abi.FrehdConfig()
+    for k, v in fly.items():
+        setattr(cf, k, v)
+    abi.check(lib.frehd_dev_footprint(C.byref(cf), C.byref(f)))
+    N = sc.N
+    assert t.value - f.value == pytest.approx(8 * N * (10 * sc.cfg["Ne"] - 16), abs=4096)

@@ -133,3 +133,28 @@ def test_device_tables_run_matches_host_tables():
     N = host.N
     for k in fa:
         assert np.max(np.abs(fa[k][:N] - fb[k][:N])) <= 1e-9, k
+
+
+@pytest.mark.parametrize("n,r,prec", [(40, 1, abi.PREC_FP64), (36, 3, abi.PREC_FP64), (32, 4, abi.PREC_MIXED),
+                                      (24, 8, abi.PREC_FP64)])
+def test_on_the_fly_bitwise_equal_to_tables(n, r, prec):
+    """The on-the-fly lookup (pixels kept, two levels evaluated per lookup)
+    gives bitwise the state of the stored device tables: both evaluate the
+    levels with the same function.  Whole runs over the wet/dry cycle, every
+    state array and the level index compared."""
+    print(f"case on-the-fly marsh{n}_sub{r} precision {prec}, 150 steps", flush=True)
+    a = DeviceScenario(cases.marsh_subgrid(n=n, r=r, Nt=300, period=1200.0, device_tables=True), precision=prec)
+    b = DeviceScenario(cases.marsh_subgrid(n=n, r=r, Nt=300, period=1200.0, on_the_fly=True), precision=prec)
+    assert a._cfg.subgridOnTheFly == 0 and b._cfg.subgridOnTheFly == 1
+    for t in range(3):
+        a.step(50)
+        b.step(50)
+        fa, fb = dl(a, None), dl(b, None)
+        for k in fa:
+            assert np.array_equal(fa[k], fb[k]), f"{k} differs after {(t + 1) * 50} steps"
+    ta, tb = a.tables(), b.tables()
+    for k in ta:
+        assert np.array_equal(ta[k], tb[k]), k
+    a.close()
+    b.close()
+
