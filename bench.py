@@ -269,18 +269,20 @@ def main():
     head = mixed[-1] if mixed else names[-1]
     hr = res[head]
 
-    # roofline of the dominant kernel pair of the headline form (one PCG
-    # iteration), from HIP events on the handle's stream
+    # roofline of the solver iteration of the headline form, from HIP events
+    # on the handle's stream: FP64, the two kernels of one PCG iteration;
+    # mixed, the one launch of an inner FP32 iteration (working launches:
+    # the set-up and stop-test launches are timed apart)
     kt = hr["kt"]
     if head == "fp64":
-        kd, kr, per_cell, dtype = "k_pcg_dir", "k_pcg_res", PCG_ITER_BYTES, "f64"
+        ks, per_cell, dtype = ["k_pcg_dir", "k_pcg_res"], PCG_ITER_BYTES, "f64"
     else:
-        kd, kr, per_cell, dtype = "k_pcg_dir_f32", "k_pcg_res_f32", PCG_ITER_BYTES_F32, "f32+f64"
-    t_iter_ms = kt.get(kd, (0, 0))[0] + kt.get(kr, (0, 0))[0]
+        ks, per_cell, dtype = ["k_cg1_f32"], PCG_ITER_BYTES_F32, "f32+f64"
+    t_iter_ms = sum(kt.get(k, (0, 0))[0] for k in ks)
     alg_bytes = per_cell * N
     achieved = alg_bytes / (t_iter_ms * 1e-3) / 1e9 if t_iter_ms > 0 else 0.0
-    traffic = traffic_of([kd, kr], args.size, args.subgrid)
-    roof = {"bound": "hbm", "kernel": f"PCG iteration ({kd} + {kr})",
+    traffic = traffic_of(ks, args.size, args.subgrid)
+    roof = {"bound": "hbm", "kernel": f"solver iteration ({' + '.join(ks)})",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_launch": alg_bytes, "ms_per_iteration": round(t_iter_ms, 5)}

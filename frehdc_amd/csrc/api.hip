@@ -287,12 +287,16 @@ size_t layout(frehd_dev *d, char *base)
     const size_t maxDir = max_dir_blocks(k.nx, k.ny);
     a.partA = cv.take<double>(2 * std::max(G, maxDir));
     a.partB = cv.take<double>(maxDir);
-    a.cX32 = a.cY32 = a.D32 = nullptr;
+    a.cX32 = a.cY32 = nullptr;
     a.xB = nullptr;
     if (k.precision == FREHD_PREC_MIXED) {
-        float **fArr[] = {&a.cX32, &a.cY32, &a.D32, &d->pcg32.r, &d->pcg32.q, &d->pcg32.pBuf[0],
+        // scaled coefficients, two residual and two direction buffers, the
+        // correction
+        float **fArr[] = {&a.cX32, &a.cY32, &d->pcg32.r, &d->pcg32.q, &d->pcg32.pBuf[0],
                           &d->pcg32.pBuf[1], &d->pcg32.x};
         for (auto **p : fArr) *p = cv.take<float>(N);
+        d->pcg32.partF[0] = cv.take<double>(4 * maxDir);
+        d->pcg32.partF[1] = cv.take<double>(4 * maxDir);
         a.xB = cv.take<double>(N);
         d->ref.partC = cv.take<double>(2 * maxDir);
     }
@@ -649,7 +653,7 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     PcgArgs<float> &f = d->pcg32;
     f.cX = a.cX32;
     f.cY = a.cY32;
-    f.D = a.D32;
+    f.D = nullptr;
     f.partA = a.partA;
     f.partB = a.partB;
     f.sc = &a.ctl->pcgIn;
@@ -670,7 +674,8 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     r.xbuf[1] = a.xB;
     r.d = f.x;
     r.r32 = f.r;
-    r.D32 = a.D32;
+    r.cXs = a.cX32;
+    r.cYs = a.cY32;
     r.partA = a.partA;
     r.ctl = a.ctl;
     r.eps = d->cfg.eps;

@@ -33,7 +33,7 @@ int fail(int code, const char *fmt, ...);
 
 enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_FLUX, K_TIDEV,
                 K_FINISH, K_SCALAR, K_SCALARBC, K_END, K_COMBINE, K_SUBGHOST, K_SUBINIT,
-                K_PQ32, K_UPD32, K_REFINE, K_RTEST, K_GATHER, K_COUNT };
+                K_CG32, K_CG32_SET, K_CG32_STOP, K_REFINE, K_RTEST, K_GATHER, K_COUNT };
 extern const char *kKernelNames[K_COUNT];
 
 struct KernelTimer {

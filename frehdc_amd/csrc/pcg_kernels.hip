@@ -283,16 +283,19 @@ __global__ void __launch_bounds__(256) k_pcg_res(const PcgArgs<T> *__restrict__ 
 // ---------------------------------------------------------------------------
 // Mixed precision: FP64 iterative refinement around the FP32 solve.
 //
-// k_refine(s): x_s = x_{s-1} + d (s > 0; x_0 is the start the assembly
-// wrote) and the true residual r = b - A x_s with the FP64 operator, in the
-// row order of the operator (W, S, C, N, E); the same sliding-window walk as
-// k_pcg_dir, over x_s recomputed at the neighbours.  The inner solve runs on
-// the decoupled operator (rows ignore identity-row neighbours), so its
-// right-hand side is lifted: a row next to an identity row t gets
-// + G * d_t, with d_t = r_t / (-D_t) the identity row's exact correction;
-// the decoupled system then has the coupled system's solution.  Writes x_s,
-// r32 = (float) (r + lift) and the partials b.b (s = 0), r.r (the FP64 true
-// residual) and r32.r32, r32.z32 (the inner solve's first stop test and rho).
+// k_refine(s): x_s = x_{s-1} + S d (s > 0; x_0 is the start the assembly
+// wrote; S = |D|^-1/2) and the true residual r = b - A x_s with the FP64
+// operator, in the row order of the operator (W, S, C, N, E); the same
+// sliding-window walk as k_pcg_dir, over x_s recomputed at the neighbours.
+// The inner solve runs on the decoupled operator (rows ignore identity-row
+// neighbours), so its right-hand side is lifted: a row next to an identity
+// row t gets + G * d_t, with d_t = r_t / (-D_t) the identity row's exact
+// correction; the decoupled system then has the coupled system's solution.
+// It is also Jacobi-scaled, S A S y = S (r + lift), d = S y: unit diagonal,
+// so the inner CG needs no preconditioner and reads no diagonal.  Writes
+// x_s, r32 = (float) S (r + lift), at s = 0 the scaled FP32 coefficients,
+// and the partials b.b (s = 0), r.r (the FP64 true residual) and r32.r32
+// (the inner solve's first stop test).
 __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A, int s)
 {
     const RefineArgs &a = A[blockIdx.z];
@@ -315,25 +318,27 @@ __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A
         const int j0 = by * a.H, j1 = min(j0 + a.H, ny);
         const int ih = lane == 0 ? i0 - 1 : (lane == 63 ? i0 + 64 : -1);
         const bool halo = ih >= 0 && ih < nx;
-        auto xs = [&](int c) { return s > 0 ? xin[c] + (double)a.d[c] : xin[c]; };
+        // x_s at a cell from its diagonal: the inner solve returns the
+        // scaled correction, d = |D|^-1/2 y
+        auto xs = [&](int c, double Dv) { return s > 0 ? xin[c] + (double)a.d[c] / sqrt(fabs(Dv)) : xin[c]; };
         // rows j-1 (S), j (C), j+1 (N): x_s, diagonal and right-hand side
         double xS = 0, DS = 1, bS = 0, cyS = 0;
         if (valid && j0 > 0) {
             const int c = (j0 - 1) * nx + i;
-            xS = xs(c);
             DS = a.D[c];
+            xS = xs(c, DS);
             bS = a.b[c];
             cyS = a.cY[c];
         }
         double xc = 0, Dc = 1, bc = 0, xn = 0, Dn = 1, bn = 0;
         if (valid) {
             const int c = j0 * nx + i;
-            xc = xs(c);
             Dc = a.D[c];
+            xc = xs(c, Dc);
             bc = a.b[c];
             if (j0 + 1 < ny) {
-                xn = xs(c + nx);
                 Dn = a.D[c + nx];
+                xn = xs(c + nx, Dn);
                 bn = a.b[c + nx];
             }
         }
@@ -342,8 +347,8 @@ __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A
             double xh = 0, Dh = 1, bh = 0;
             if (halo) {
                 const int h = j * nx + ih;
-                xh = xs(h);
                 Dh = a.D[h];
+                xh = xs(h, Dh);
                 bh = a.b[h];
             }
             const double cxc = valid ? a.cX[c] : 0.0;
@@ -365,8 +370,8 @@ __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A
             // next row (j+2) ahead
             double xnn = 0, Dnn = 1, bnn = 0;
             if (valid && j + 2 < ny && j + 1 < j1) {
-                xnn = xs(c + 2 * nx);
                 Dnn = a.D[c + 2 * nx];
+                xnn = xs(c + 2 * nx, Dnn);
                 bnn = a.b[c + 2 * nx];
             }
             double cyc = 0.0;
@@ -388,11 +393,18 @@ __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A
                     if (i < nx - 1 && De < 0) lift += cxc * ((be + De * xe) / -De);
                 }
                 const double r = bc - sum;
-                const float r32 = (float)(r + lift);
+                // the inner right-hand side, scaled: |D|^-1/2 (r + lift)
+                const float r32 = (float)((r + lift) / sqrt(fabs(Dc)));
                 if (s > 0) xout[c] = xc;
                 a.r32[c] = r32;
+                if (s == 0) {
+                    // the scaled inner operator: c / sqrt(D_c D_nb), zero on
+                    // faces that touch an identity row or the domain edge
+                    a.cXs[c] = (i < nx - 1 && Dc > 0 && De > 0) ? (float)(cxc / sqrt(Dc * De)) : 0.0f;
+                    a.cYs[c] = (j < ny - 1 && Dc > 0 && Dn > 0) ? (float)(cyc / sqrt(Dc * Dn)) : 0.0f;
+                }
                 part[0] += (double)r32 * (double)r32;
-                part[1] += (double)r32 * (double)(r32 * (1.0f / fabsf(a.D32[c])));
+                part[1] += (double)r32 * (double)r32;
                 if (s == 0) part[2] += bc * bc;
                 part[3] += r * r;
             }
@@ -408,6 +420,235 @@ __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A
         a.partC[blockIdx.x] = part[2];
         a.partC[a.nb + blockIdx.x] = part[3];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Inner FP32 solve of the mixed path: CG on the scaled, decoupled operator
+// A' = S A S (unit diagonal; off-diagonals -cXs, -cYs, zero on faces that
+// touch an identity row or the domain edge), one kernel and one reduction
+// per iteration.
+//
+// k_cg1(it), it >= 1, from r_{it-1}, p_{it-1} and the block partials of
+// k_cg1(it-1) (p.q, r.q, q.q, r.r with q = A' p):
+//   alpha = r.r / p.q                          (direct sums)
+//   rho   = r.r - 2 alpha r.q + alpha^2 q.q     (|r_it|^2 by recurrence)
+//   beta  = rho / r.r
+//   q_{it-1} = A' p_{it-1}, recomputed here rather than stored
+//   r_it = r_{it-1} - alpha q_{it-1}     x_it = x_{it-1} + alpha p_{it-1}
+//   p_it = r_it + beta p_{it-1}          q_it = A' p_it -> block partials
+// It reads r, p, x, cXs, cYs and writes r, p, x: the 8 accesses per cell of
+// one iteration.  r and p alternate between two buffers, because blocks
+// read their neighbours' previous values.  k_cg1(0) forms the partials of
+// p_0 = r_0 (the scaled residual of k_refine) and zeroes x.  The stop test
+// |r| < eps |r_0| runs at the start of a launch on the last directly summed
+// r.r (r_0 at it = 0, r_{it-1} from it = 2 on); a stopped solve's x is
+// final, and so is the x of a solve the launch pattern cuts short.
+//
+// Tile walk as in k_pcg_dir: a wave owns 64 columns and slides down the
+// rows, ingesting row t (q_{it-1}, r_it, p_it of row t) and finishing row
+// t-1 (q_it, the stores, the partials).  A' p_{it-1} at a neighbour needs
+// p_{it-1} two cells away: lanes 0 and 63 also ingest the column beyond
+// the wave (v) from the column after it (o), with the same expressions the
+// owner of v uses, so both obtain bitwise the same p_it there.
+namespace {
+
+struct Cg1Row {
+    float P2, R, cX, cY, X;                 // own column: p(t+1), r(t), cXs(t), cYs(t), x(t-1)
+    float hP2, hR, hcXv, hcY, hPo, hcXo;    // lanes 0 / 63: column v and its outer neighbour o
+};
+
+// A' p at one cell, the terms in the owner's order (W, E, S, N)
+__device__ inline float apply_scaled(float pc, float cW, float pW, float cE, float pE, float cS, float pS, float cN,
+                                     float pN)
+{
+#pragma clang fp contract(off)
+    return pc - cW * pW - cE * pE - cS * pS - cN * pN;
+}
+
+// r_it and p_it of a cell from r_{it-1}, q_{it-1}, p_{it-1}
+__device__ inline void advance(float R, float q, float P, float al, float be, float &rn, float &pn)
+{
+#pragma clang fp contract(off)
+    rn = R - al * q;
+    pn = rn + be * P;
+}
+
+}  // namespace
+
+template <bool First>
+__global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ A, int it)
+{
+    it = First ? 0 : it;
+    const PcgArgs<float> &a = A[blockIdx.z];
+    if ((int)blockIdx.x >= a.nbDir) return;
+    __shared__ double lds[16 * 4];
+    PcgScalars *sc = a.sc;
+    if (*a.halted) return;
+    if (a.outerDone && *a.outerDone >= 0) return;
+    const int doneAt = sc->doneAt;
+    if (doneAt >= 0 && doneAt < it) return;
+
+    const double eps = a.epsDyn ? *a.epsDyn : a.eps;
+    double alphaD = 0.0, betaD = 0.0;
+    bool stop;
+    if (it == 0) {
+        double v[1];
+        reduce_partials<1>(a.partA, a.nbStart, v, lds);
+        const double rNorm = sqrt(v[0]);
+        stop = (LAS_IS_ZERO(rNorm) && LAS_IS_ONE(1.0 + rNorm)) || a.maxIter <= 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->bb = v[0];
+            sc->bNorm = rNorm;
+            sc->rr = v[0];
+            sc->rNorm = rNorm;
+            sc->iter = 0;
+            if (stop) sc->doneAt = 0;
+        }
+    } else {
+        double v[4];
+        reduce_partials<4>(a.partF[(it - 1) & 1], a.nbDir, v, lds);
+        const double pq = v[0], rq = v[1], qq = v[2], rr = v[3];
+        const double rNorm = sqrt(rr);
+        stop = (it >= 2 && (rNorm < eps * sc->bNorm || it - 1 >= a.maxIter)) || !(pq > 0);
+        alphaD = rr / pq;
+        double rho = rr - 2.0 * alphaD * rq + alphaD * alphaD * qq;
+        if (!(rho > 0)) rho = 0;               // cancellation at convergence: restart the directions
+        betaD = rho / rr;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->rr = rr;
+            sc->rNorm = rNorm;
+            sc->iter = stop ? it - 1 : it;
+            if (stop) {
+                sc->doneAt = it;
+            } else {
+                sc->pq = pq;
+                sc->alpha = alphaD;
+            }
+        }
+    }
+    const bool first = it == 0;
+    const float al = (float)alphaD, be = (float)betaD;
+
+    int lb = blockIdx.x;
+    if (a.remap) lb = (lb & 7) * (a.nbDir >> 3) + (lb >> 3);
+    const int bx = lb % a.gx, by = lb / a.gx;
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    if (by < a.gy && (!stop || first)) {
+        const int nx = a.nx, ny = a.ny;
+        const int lane = threadIdx.x & 63;
+        const int i0 = bx * 256 + (threadIdx.x & ~63);
+        const int i = i0 + lane;
+        const bool valid = i < nx;
+        const int j0 = by * a.H, j1 = min(j0 + a.H, ny);
+        // buffers: r and p of iteration it-1 (p_0 = r_0) and of it
+        float *const rBuf[2] = {a.r, a.q};
+        const float *Rin = first ? a.r : rBuf[(it - 1) & 1];
+        const float *Pin = it <= 1 ? a.r : a.pBuf[(it - 1) & 1];
+        float *Rout = rBuf[it & 1];
+        float *Pout = a.pBuf[it & 1];
+        float *X = a.x;
+        if (stop) {
+            // stopped at it = 0: the correction is zero
+            if (valid)
+                for (int j = j0; j < j1; j++) X[j * nx + i] = 0.0f;
+        } else {
+            const float *cXs = a.cX, *cYs = a.cY;
+            // halo lanes: v = the column beyond the wave, o = the one after it
+            const int v = lane == 0 ? i0 - 1 : (lane == 63 ? i0 + 64 : -1);
+            const bool halo = v >= 0 && v < nx;
+            const int o = lane == 0 ? v - 1 : v + 1;
+            const bool oin = halo && o >= 0 && o < nx;
+            auto at = [&](const float *p, int row, int col, bool ok) -> float {
+                return (ok && row >= 0 && row < ny) ? p[row * nx + col] : 0.0f;
+            };
+            auto load = [&](int t) {
+                Cg1Row L;
+                L.P2 = at(Pin, t + 1, i, valid);
+                L.R = at(Rin, t, i, valid);
+                L.cX = at(cXs, t, i, valid);
+                L.cY = at(cYs, t, i, valid);
+                L.X = (!first && t - 1 >= j0) ? at(X, t - 1, i, valid) : 0.0f;
+                L.hP2 = at(Pin, t + 1, v, halo);
+                L.hR = at(Rin, t, v, halo);
+                L.hcXv = at(cXs, t, v, halo);
+                L.hcY = at(cYs, t, v, halo);
+                L.hPo = at(Pin, t, o, oin);
+                L.hcXo = lane == 0 ? at(cXs, t, o, oin) : 0.0f;
+                return L;
+            };
+            // window at the start: rows j0-2 and j0-1 of p, the south face of row j0-1
+            float P0 = at(Pin, j0 - 2, i, valid), P1 = at(Pin, j0 - 1, i, valid);
+            float cYm = at(cYs, j0 - 2, i, valid), cYmm = 0.0f, cXc = 0.0f;
+            float pnm = 0.0f, pnc = 0.0f, rnc = 0.0f;
+            float hP0 = at(Pin, j0 - 2, v, halo), hP1 = at(Pin, j0 - 1, v, halo);
+            float hcYm = at(cYs, j0 - 2, v, halo), hpn = 0.0f, hcVIprev = 0.0f;
+            Cg1Row cur = load(j0 - 1);
+            for (int t = j0 - 1; t <= j1; t++) {
+                Cg1Row nxt;
+                if (t < j1) nxt = load(t + 1);
+                else nxt = Cg1Row{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                // own column: q_{it-1}, r_it, p_it of row t
+                float pW = __shfl_up(P1, 1, 64), pE = __shfl_down(P1, 1, 64);
+                float cW = __shfl_up(cur.cX, 1, 64);
+                if (lane == 0) { pW = hP1; cW = cur.hcXv; }
+                if (lane == 63) pE = hP1;
+                float rn, pn;
+                if (first) {
+                    rn = cur.R;
+                    pn = cur.R;
+                } else {
+                    const float q = apply_scaled(P1, cW, pW, cur.cX, pE, cYm, P0, cur.cY, cur.P2);
+                    advance(cur.R, q, P1, al, be, rn, pn);
+                }
+                // halo column v of row t, the owner's expressions
+                float hpnT = 0.0f;
+                if (halo) {
+                    if (first) {
+                        hpnT = cur.hR;
+                    } else {
+                        const float hq = lane == 0
+                            ? apply_scaled(hP1, cur.hcXo, cur.hPo, cur.hcXv, P1, hcYm, hP0, cur.hcY, cur.hP2)
+                            : apply_scaled(hP1, cur.cX, P1, cur.hcXv, cur.hPo, hcYm, hP0, cur.hcY, cur.hP2);
+                        float hr;
+                        advance(cur.hR, hq, hP1, al, be, hr, hpnT);
+                    }
+                }
+                // finish row j = t-1: q_it = A' p_it, stores, partials
+                const int j = t - 1;
+                float qW = __shfl_up(pnc, 1, 64), qE = __shfl_down(pnc, 1, 64);
+                float cWj = __shfl_up(cXc, 1, 64);
+                if (lane == 0) { qW = hpn; cWj = hcVIprev; }
+                if (lane == 63) qE = hpn;
+                if (valid && j >= j0) {
+                    const float qn = apply_scaled(pnc, cWj, qW, cXc, qE, cYmm, pnm, cYm, pn);
+                    const int c = j * nx + i;
+                    if (!first) {
+                        Rout[c] = rnc;
+                        Pout[c] = pnc;
+                        X[c] = cur.X + al * P0;
+                    } else {
+                        X[c] = 0.0f;
+                    }
+                    part[0] += (double)pnc * (double)qn;
+                    part[1] += (double)rnc * (double)qn;
+                    part[2] += (double)qn * (double)qn;
+                    part[3] += (double)rnc * (double)rnc;
+                }
+                // slide
+                P0 = P1; P1 = cur.P2;
+                cYmm = cYm; cYm = cur.cY;
+                cXc = cur.cX;
+                pnm = pnc; pnc = pn; rnc = rn;
+                hP0 = hP1; hP1 = cur.hP2;
+                hcYm = cur.hcY;
+                hpn = hpnT;
+                hcVIprev = cur.hcXv;
+                cur = nxt;
+            }
+        }
+    }
+    block_sum<4>(part, lds);
+    if (!stop) store_partials<4>(a.partF[it & 1], a.nbDir, {part[0], part[1], part[2], part[3]});
 }
 
 // k_refine_test(s), one block: the outer stop test |b - A x_s| < eps |b|
@@ -450,7 +691,7 @@ __global__ void __launch_bounds__(256) k_refine_test(const RefineArgs *__restric
 
 template __global__ void k_pcg_dir<double>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_pcg_res<double>(const PcgArgs<double> *__restrict__, int);
-template __global__ void k_pcg_dir<float>(const PcgArgs<float> *__restrict__, int);
-template __global__ void k_pcg_res<float>(const PcgArgs<float> *__restrict__, int);
+template __global__ void k_cg1<true>(const PcgArgs<float> *__restrict__, int);
+template __global__ void k_cg1<false>(const PcgArgs<float> *__restrict__, int);
 
 }  // namespace frehd

@@ -75,7 +75,8 @@ struct StepArgs {
     uint8_t *rmv, *flags;     // flags: bit0 tideP cell, bit1 tideM cell, bits 2-7 inflow count
     // operator and solver vectors
     double *cX, *cY, *D, *b, *x, *r, *q, *pA, *pB;
-    float *cX32, *cY32, *D32;  // FP32 operator copies (mixed precision only, else NULL)
+    float *cX32, *cY32;       // mixed precision only (else NULL): the scaled FP32
+                              // operator of the inner solve, written by k_refine
     double *xB;                // mixed precision: second refinement x buffer
     // per-block partial sums: solver (partA: r.r, r.z; partB: p.q), volume
     // losses (vpart: 4 slots) and CFL maxima (cflpart: 2 slots), each slot
@@ -140,10 +141,16 @@ struct PcgArgs {
     int gx, gy, H, remap;
     int nbDir, nbRes, nbStart;
     int decouple;             // inner solve: skip couplings to identity rows
+    double *partF[2];         // k_cg1: p.q, r.q, q.q, r.r partials (nbDir each),
+                              // launch it writes partF[it&1]
 };
 
 template <typename T> __global__ void k_pcg_dir(const PcgArgs<T> *__restrict__ A, int it);
 template <typename T> __global__ void k_pcg_res(const PcgArgs<T> *__restrict__ A, int it);
+// the inner FP32 solve of the mixed path, one launch per iteration (First:
+// the set-up launch, it = 0): cX, cY hold the scaled coefficients, r and q
+// the two residual buffers, D unused
+template <bool First> __global__ void k_cg1(const PcgArgs<float> *__restrict__ A, int it);
 
 // FP64 iterative refinement around the FP32 solve (mixed precision): sweep
 // s forms x_s = x_{s-1} + d (d: the inner solve's correction) and the true
@@ -153,9 +160,9 @@ struct RefineArgs {
     const double *cX, *cY, *D, *b;
     double *xbuf[2];          // x_s lives in xbuf[s & 1]
     const float *d;           // inner correction
-    float *r32;               // residual, rounded: the next inner right-hand side
-    const float *D32;
-    double *partA;            // r32.r32, r32.z32 partials (inner start)
+    float *r32;               // scaled residual, rounded: the next inner right-hand side
+    float *cXs, *cYs;         // scaled FP32 coefficients of the inner solve (written at s = 0)
+    double *partA;            // r32.r32 partials (inner start)
     double *partC;            // b.b, r.r partials (FP64)
     Ctl *ctl;
     double eps, epsInner;

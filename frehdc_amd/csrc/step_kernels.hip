@@ -186,12 +186,9 @@ __device__ inline void store_row(const StepArgs &a, int c, double GXP, double GY
     a.D[c] = D;
     a.b[c] = z;
     if (a.cX32) {
-        // mixed precision: FP32 operator for the inner solve; the refinement
-        // starts from 0 or from the current surface and forms r itself
-        a.cX32[c] = (float)GXP;
-        a.cY32[c] = (float)GYP;
-        a.D32[c] = (float)D;
-        // start: 0 (the reference's), the current surface, or its linear
+        // mixed precision: the refinement starts from 0 or from the current
+        // surface, forms r and the scaled FP32 operator itself.  Start: 0
+        // (the reference's), the current surface, or its linear
         // extrapolation in time 2 eta^n - eta^(n-1)
         a.x[c] = a.p.warmStart == 2 ? 2.0 * a.surf[c] - a.surfOld[c] : (a.p.warmStart ? a.surf[c] : 0.0);
         return;

@@ -96,3 +96,31 @@ def test_mixed_within_fp64_tolerance_sensitivity(warm):
     # one face moves its velocity by the front's speed
     for k in ("uuXP", "vvYP"):
         assert e_mix[k] <= max(1e-4, 30 * e_tight[k]), k
+
+
+@pytest.mark.parametrize("H,remap", [(0, 0), (1, 0), (3, 1), (70, 0)])
+def test_inner_solver_tilings(H, remap):
+    """The inner FP32 CG walks tiles of 256 columns x H rows; a 300 x 70 grid
+    (two column tiles, a width that is not a multiple of the 64-lane wave)
+    under several row tilings must converge like the FP64 CG and give the
+    FP64 path's surface."""
+    print(f"case basin300x70 mixed cold, tiling H={H} remap={remap}", flush=True)
+    sc = cases.basin(nx=300, ny=70, Nt=60)
+    ref = DeviceScenario(sc)
+    ref.step(20)
+    f64 = ref.download(["surf"])
+    it64 = ref.stats()["pcgIterTotal"] / 20
+    ref.close()
+    dev = DeviceScenario(sc, precision=abi.PREC_MIXED, warmStart=0)
+    dev.set_pcg_tiling(H, remap)
+    dev.step(20)
+    st = dev.stats()
+    got = dev.download(["surf"])
+    dev.close()
+    N = sc.N
+    err = float(np.max(np.abs(got["surf"][:N] - f64["surf"][:N])))
+    inner = st["pcgIterTotal"] / 20
+    print(f"  eta err {err:.3e}, inner iterations/step {inner:.1f} vs FP64 {it64:.1f}, sweeps {st['refineLast']}")
+    assert st["relResLast"] < sc.cfg["eps"]
+    assert err <= 1e-8
+    assert inner <= 1.5 * it64 + 10
