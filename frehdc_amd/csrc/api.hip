@@ -646,7 +646,6 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     p.nbStart = p.nbRes;
     p.outerDone = nullptr;
     p.epsDyn = nullptr;
-    p.decouple = 0;
     if (d->cfg.precision != FREHD_PREC_MIXED) return;
     // inner FP32 solve: same tiling, FP32 operator and vectors (r, q, p, x =
     // the correction d were carved by layout()), started by k_refine
@@ -667,7 +666,6 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     f.nbDir = t.nb;
     f.nbRes = p.nbRes;
     f.nbStart = t.nb;
-    f.decouple = 1;
     RefineArgs &r = d->ref;
     r.cX = a.cX; r.cY = a.cY; r.D = a.D; r.b = a.b;
     r.xbuf[0] = a.x;
@@ -694,6 +692,74 @@ int init_subgrid_device(frehd_dev *d)
     int rc = upload(d, ind.data(), d->args.ind, ind.size() * sizeof(int));
     if (rc) return rc;
     return exec_init_subgrid(d->self);
+}
+
+// Solver tiling of a group chosen by timing: every candidate (rows per tile
+// 1, 2, 4, ..., up to the smallest member height, and the default; XCD remap
+// off and on) runs `iters` solver iterations on the members' current
+// systems with a zero tolerance, and the fastest is applied to every member.
+// Control blocks (statistics, step counter, flags) are restored, and the
+// solver's work arrays are rewritten at the next step's assembly, so the
+// model state is untouched.
+int autotune_group(Exec &e, int iters, int32_t *bestH, int32_t *bestRemap, double *bestMs)
+{
+    if (e.count == 0) return fail(FREHD_ESTATE, "nothing to tune");
+    if (iters < 1 || iters > 1000) return fail(FREHD_EINVAL, "iters %d outside [1, 1000]", iters);
+    HIPCHK(hipSetDevice(e.device));
+    HIPCHK(hipStreamSynchronize(e.stream));
+    std::vector<Ctl> saved(e.count);
+    int nyMin = 1 << 30;
+    for (int m = 0; m < e.count; m++) {
+        COPYW(e.stream, &saved[m], e.members[m]->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
+        nyMin = std::min(nyMin, e.members[m]->g.ny);
+    }
+    std::vector<int> Hs = {0};
+    for (int H = 1; H <= nyMin && H <= 256; H *= 2) Hs.push_back(H);
+    int rc = 0, bH = 0, bR = 0;
+    double best = -1.0;
+    for (int H : Hs) {
+        for (int remap = 0; remap < 2 && !rc; remap++) {
+            bool ok = true;
+            for (auto *d : e.members)
+                if ((size_t)dir_tiling(d->g.nx, d->g.ny, H, remap).nb > max_dir_blocks(d->g.nx, d->g.ny)) ok = false;
+            if (!ok) continue;
+            for (auto *d : e.members) {
+                set_pcg_args(static_cast<frehd_dev *>(d), H, remap);
+                d->pcg.eps = 0.0;                     // the trial iterations must all run
+                d->pcg.maxIter = 1 << 30;
+                d->pcg32.maxIter = 1 << 30;
+            }
+            if ((rc = exec_refresh(e))) break;
+            for (int m = 0; m < e.count && !rc; m++) {
+                Ctl c = saved[m];
+                c.halted = 0;
+                c.pcg.doneAt = -1;
+                c.pcgIn.doneAt = -1;
+                c.ref.doneAt = -1;
+                c.ref.epsIn = 0.0;
+                rc = copy_wait(e.stream, e.members[m]->args.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice);
+            }
+            double ms = 0.0;
+            if (!rc) rc = exec_time_solver(e, iters, &ms);
+            if (!rc && ms > 0 && (best < 0 || ms < best)) {
+                best = ms;
+                bH = H;
+                bR = remap;
+            }
+        }
+        if (rc) break;
+    }
+    // the winner (or, after an error, the default), then the saved control blocks
+    for (auto *d : e.members) set_pcg_args(static_cast<frehd_dev *>(d), rc ? 0 : bH, rc ? 0 : bR);
+    int rc2 = exec_refresh(e);
+    for (int m = 0; m < e.count && !rc2; m++)
+        rc2 = copy_wait(e.stream, e.members[m]->args.ctl, &saved[m], sizeof(Ctl), hipMemcpyHostToDevice);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (bestH) *bestH = e.members[0]->pcg.H;
+    if (bestRemap) *bestRemap = e.members[0]->pcg.remap;
+    if (bestMs) *bestMs = best;
+    return 0;
 }
 
 }  // namespace
@@ -1015,6 +1081,14 @@ int frehd_dev_set_pcg_tiling(frehd_dev_t *d, int32_t rows_per_tile, int32_t xcd_
     return upload_member_args(d);
 }
 
+int frehd_dev_autotune(frehd_dev_t *d, int32_t iters, int32_t *rows_per_tile, int32_t *xcd_remap,
+                       double *ms_per_iteration)
+{
+    if (!d) return fail(FREHD_EINVAL, "handle is NULL");
+    if (d->batch) return fail(FREHD_ESTATE, "the handle is in a batch: tune the batch");
+    return autotune_group(d->self, iters, rows_per_tile, xcd_remap, ms_per_iteration);
+}
+
 // ---- batches ----------------------------------------------------------------
 int frehd_batch_create(int32_t capacity, frehd_batch_t **out)
 {
@@ -1111,6 +1185,17 @@ int frehd_batch_step(frehd_batch_t *b, int32_t nsteps)
     }
     if (halted) return fail(FREHD_ECFL, "%d batch member(s) halted: max CFL exceeded 20", halted);
     return 0;
+}
+
+int frehd_batch_autotune(frehd_batch_t *b, int32_t iters, int32_t *rows_per_tile, int32_t *xcd_remap,
+                         double *ms_per_iteration)
+{
+    if (!b) return fail(FREHD_EINVAL, "batch is NULL");
+    int rc = autotune_group(b->e, iters, rows_per_tile, xcd_remap, ms_per_iteration);
+    // the members' own groups carry the new tiling too
+    for (auto *d : b->e.members)
+        if (!rc) rc = upload_member_args(d);
+    return rc;
 }
 
 int frehd_batch_enable_timing(frehd_batch_t *b, int32_t on)

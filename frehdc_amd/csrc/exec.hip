@@ -250,6 +250,51 @@ int solve_mixed_finish(Exec &e, int s)
 
 }  // namespace
 
+// Average duration (ms) of one solver iteration of the group under its
+// current tiling: `iters` iterations on the members' current systems, timed
+// with events on the group's stream (FP64: the k_pcg_dir + k_pcg_res pair;
+// mixed: the inner k_cg1 launches after k_refine(0) launches, the
+// refinement sweeps spread over the inner iterations in the proportion of
+// the last solve).  The caller makes the iterations run (zero tolerance)
+// and restores the control blocks.
+int exec_time_solver(Exec &e, int iters, double *ms)
+{
+    Timing local;
+    local.on = 1;
+    struct Swap {
+        Exec &e;
+        Timing *saved;
+        ~Swap() { e.tim = saved; }
+    } swap{e, e.tim};
+    e.tim = &local;
+    int rc;
+    if (e.mixed) {
+        for (int q = 0; q < 4; q++)
+            LAUNCH(K_REFINE, k_refine, grid1(e.ext.nbRef, e.count), dim3(256), e.dRef, 0);
+        for (int it = 0; it <= iters; it++)
+            if ((rc = enqueue_inner_iteration(e, it))) return rc;
+    } else {
+        for (int it = 0; it < iters; it++)
+            if ((rc = enqueue_pcg_iteration(e, it))) return rc;
+    }
+    if ((rc = exec_collect_timing(e))) return rc;
+    auto avg = [&](int k) {
+        const KernelTimer &t = local.timers[k];
+        return t.launches ? t.ms / (double)t.launches : 0.0;
+    };
+    if (e.mixed) {
+        // per solve as the last step ran it: its inner iterations and one
+        // refinement sweep more than inner solves
+        int inner = 0;
+        for (int n : e.mixPlan) inner += n;
+        const int sweeps = (int)e.mixPlan.size() + 1;
+        *ms = (std::max(inner, 1) * avg(K_CG32) + sweeps * avg(K_REFINE)) / std::max(inner, 1);
+    } else {
+        *ms = avg(K_PQ) + avg(K_UPD);
+    }
+    return 0;
+}
+
 int exec_collect_timing(Exec &e)
 {
     Timing *t = e.tim;

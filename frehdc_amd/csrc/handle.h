@@ -121,6 +121,8 @@ int exec_step(Exec &e);
 // combineSubArea)
 int exec_init_subgrid(Exec &e);
 int exec_collect_timing(Exec &e);
+// average ms of one solver iteration under the group's current tiling
+int exec_time_solver(Exec &e, int iters, double *ms);
 // upload one handle's argument blocks to its own group arrays
 int upload_member_args(frehd_dev_impl *d);
 

@@ -18,14 +18,6 @@
 // producer in the same fixed order, so the scalars are bitwise identical in
 // every block and from run to run, without atomics.
 //
-// With decouple set (the inner FP32 solve of the mixed path) a row ignores
-// its neighbours that are identity rows, which makes the inner operator
-// symmetric (block diagonal [I 0; 0 K]); the FP64 refinement around it
-// computes the residual with the full operator, so the coupling enters
-// through the next sweep's right-hand side.  Without it, CG's recurrences on
-// the non-symmetric system stall in FP32 when the residual starts at the
-// Dirichlet rows (a warm start right after the tide moves).
-//
 // k_pcg_dir walks a tile of 256 columns x H rows: each wave owns 64
 // adjacent columns and slides down the rows, keeping p of rows j-1, j, j+1
 // in registers, so every array is read once per iteration (east/west
@@ -132,15 +124,13 @@ __global__ void __launch_bounds__(256) k_pcg_dir(const PcgArgs<T> *__restrict__ 
             const bool halo = ih >= 0 && ih < nx;
 
             // row j0-1 (south neighbour of the first row)
-            T pS = 0, cyS = 0, DS = 1;
+            T pS = 0, cyS = 0;
             if (valid && j0 > 0) {
                 const int c = (j0 - 1) * nx + i;
                 const Raw<T> v = load_raw(a, pOld, c, first);
                 pS = direction(jacobi_z(v.r, v.D), v.po, beta, first, betaOne);
                 cyS = a.cY[c];
-                DS = v.D;
             }
-            const bool dec = a.decouple != 0;
             // row j0
             Raw<T> cur = {0, 1, 0}, nxt = {0, 1, 0}, hcur = {0, 1, 0};
             T cxc = 0, cyc = 0, xc = 0;
@@ -173,36 +163,23 @@ __global__ void __launch_bounds__(256) k_pcg_dir(const PcgArgs<T> *__restrict__ 
                 const T ph = halo ? direction(jacobi_z(hcur.r, hcur.D), hcur.po, beta, first, betaOne) : T(0);
                 T pw = __shfl_up(pc, 1, 64), pe = __shfl_down(pc, 1, 64);
                 T cxw = __shfl_up(cxc, 1, 64);
-                T Dw = __shfl_up(cur.D, 1, 64), De = __shfl_down(cur.D, 1, 64);
                 if (lane == 0) {
                     pw = ph;
-                    Dw = hcur.D;
                     cxw = (valid && i > 0) ? a.cX[c - 1] : T(0);
                 }
-                if (lane == 63) {
-                    pe = ph;
-                    De = hcur.D;
-                }
+                if (lane == 63) pe = ph;
                 if (valid) {
                     const T Dc = cur.D;
                     T sum;
                     if (Dc < 0) {
                         sum = -Dc * pc;
-                    } else if (!dec) {
+                    } else {
                         sum = 0;
                         if (i > 0) sum += (-cxw) * pw;
                         if (j > 0) sum += (-cyS) * pS;
                         sum += Dc * pc;
                         if (j < ny - 1) sum += (-cyc) * pn;
                         if (i < nx - 1) sum += (-cxc) * pe;
-                    } else {
-                        // symmetric inner operator: no coupling to identity rows
-                        sum = 0;
-                        if (i > 0 && !(Dw < 0)) sum += (-cxw) * pw;
-                        if (j > 0 && !(DS < 0)) sum += (-cyS) * pS;
-                        sum += Dc * pc;
-                        if (j < ny - 1 && !(nxt.D < 0)) sum += (-cyc) * pn;
-                        if (i < nx - 1 && !(De < 0)) sum += (-cxc) * pe;
                     }
                     pNew[c] = pc;
                     a.q[c] = sum;
@@ -211,7 +188,6 @@ __global__ void __launch_bounds__(256) k_pcg_dir(const PcgArgs<T> *__restrict__ 
                 }
                 pS = pc;
                 cyS = cyc;
-                DS = cur.D;
                 pc = pn;
                 cur = nxt;
                 nxt = nn;
@@ -444,8 +420,8 @@ __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A
 // r.r (r_0 at it = 0, r_{it-1} from it = 2 on); a stopped solve's x is
 // final, and so is the x of a solve the launch pattern cuts short.
 //
-// Tile walk as in k_pcg_dir: a wave owns 64 columns and slides down the
-// rows, ingesting row t (q_{it-1}, r_it, p_it of row t) and finishing row
+// Tile walk as in k_pcg_dir (loads two rows ahead): a wave owns 64 columns
+// and slides down the rows, ingesting row t (q_{it-1}, r_it, p_it of row t) and finishing row
 // t-1 (q_it, the stores, the partials).  A' p_{it-1} at a neighbour needs
 // p_{it-1} two cells away: lanes 0 and 63 also ingest the column beyond
 // the wave (v) from the column after it (o), with the same expressions the
@@ -582,11 +558,11 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
             float pnm = 0.0f, pnc = 0.0f, rnc = 0.0f;
             float hP0 = at(Pin, j0 - 2, v, halo), hP1 = at(Pin, j0 - 1, v, halo);
             float hcYm = at(cYs, j0 - 2, v, halo), hpn = 0.0f, hcVIprev = 0.0f;
-            Cg1Row cur = load(j0 - 1);
+            // loads run two rows ahead of the arithmetic
+            const Cg1Row zero{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            Cg1Row cur = load(j0 - 1), nxt = load(j0);
             for (int t = j0 - 1; t <= j1; t++) {
-                Cg1Row nxt;
-                if (t < j1) nxt = load(t + 1);
-                else nxt = Cg1Row{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                const Cg1Row nxt2 = t + 2 <= j1 ? load(t + 2) : zero;
                 // own column: q_{it-1}, r_it, p_it of row t
                 float pW = __shfl_up(P1, 1, 64), pE = __shfl_down(P1, 1, 64);
                 float cW = __shfl_up(cur.cX, 1, 64);
@@ -644,6 +620,7 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
                 hpn = hpnT;
                 hcVIprev = cur.hcXv;
                 cur = nxt;
+                nxt = nxt2;
             }
         }
     }

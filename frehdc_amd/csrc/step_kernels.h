@@ -140,7 +140,6 @@ struct PcgArgs {
     // gives each XCD a contiguous band of tiles (nbDir a multiple of 8)
     int gx, gy, H, remap;
     int nbDir, nbRes, nbStart;
-    int decouple;             // inner solve: skip couplings to identity rows
     double *partF[2];         // k_cg1: p.q, r.q, q.q, r.r partials (nbDir each),
                               // launch it writes partF[it&1]
 };

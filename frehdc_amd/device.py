@@ -127,6 +127,12 @@ class DeviceScenario:
     def set_pcg_tiling(self, rows_per_tile=0, xcd_remap=False):
         abi.check(self.lib.frehd_dev_set_pcg_tiling(self.h, int(rows_per_tile), 1 if xcd_remap else 0))
 
+    def autotune(self, iters: int = 20) -> dict:
+        """Time the solver under every candidate tiling on the last step's
+        system and keep the fastest (frehd_dev_autotune); the state is left
+        as it was."""
+        return _autotune(self.lib.frehd_dev_autotune, self.h, iters)
+
     def enable_timing(self, on=True):
         abi.check(self.lib.frehd_dev_enable_timing(self.h, 1 if on else 0))
 
@@ -173,6 +179,10 @@ class Batch:
     def try_step(self, n: int = 1) -> int:
         return self.lib.frehd_batch_step(self.h, int(n))
 
+    def autotune(self, iters: int = 20) -> dict:
+        """frehd_batch_autotune: one solver tiling for all members."""
+        return _autotune(self.lib.frehd_batch_autotune, self.h, iters)
+
     def enable_timing(self, on=True):
         abi.check(self.lib.frehd_batch_enable_timing(self.h, 1 if on else 0))
 
@@ -203,3 +213,9 @@ def device_count() -> int:
     n = C.c_int32()
     lib.frehd_device_count(C.byref(n))
     return n.value
+
+
+def _autotune(fn, h, iters):
+    H, remap, ms = C.c_int32(), C.c_int32(), C.c_double()
+    abi.check(fn(h, int(iters), C.byref(H), C.byref(remap), C.byref(ms)))
+    return {"rows_per_tile": H.value, "xcd_remap": remap.value, "ms_per_iteration": ms.value}

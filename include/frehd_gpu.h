@@ -276,6 +276,18 @@ int frehd_dev_download_tables(frehd_dev_t *dev, double *const tables[10]);
  * depend on the shape beyond the order of the solver's dot-product sums. */
 int frehd_dev_set_pcg_tiling(frehd_dev_t *dev, int32_t rows_per_tile, int32_t xcd_remap);
 
+/* Autotuner of that launch shape (no counterpart in the reference): times
+ * `iters` solver iterations (the FP64 PCG pair, or the mixed path's inner
+ * FP32 iteration plus its share of FP64 refinement sweeps, in the
+ * proportion of the last step's solve) under every candidate shape -- rows_per_tile 0, 1, 2, 4,
+ * ... up to min(ny, 256), XCD remap off and on -- on the system of the last
+ * step, applies the fastest and returns it (each output may be NULL).  The
+ * model state, the step counter and the statistics are left as they were.
+ * Call it after at least one step; FREHD_ESTATE for a handle in a batch
+ * (tune the batch). */
+int frehd_dev_autotune(frehd_dev_t *dev, int32_t iters, int32_t *rows_per_tile, int32_t *xcd_remap,
+                       double *ms_per_iteration);
+
 /* ---- batches: many handles advanced by shared launches ---------------------
  * No counterpart in the reference (one scenario per process there).  A batch
  * holds up to `capacity` handles of one device that share useSubgrid and
@@ -294,6 +306,10 @@ int frehd_batch_add(frehd_batch_t *batch, frehd_dev_t *dev);
 int frehd_batch_remove(frehd_batch_t *batch, frehd_dev_t *dev);
 int frehd_batch_size(frehd_batch_t *batch, int32_t *n);
 int frehd_batch_step(frehd_batch_t *batch, int32_t nsteps);
+/* frehd_dev_autotune over the members together (one shape for all: the
+ * launches are shared). */
+int frehd_batch_autotune(frehd_batch_t *batch, int32_t iters, int32_t *rows_per_tile, int32_t *xcd_remap,
+                         double *ms_per_iteration);
 int frehd_batch_enable_timing(frehd_batch_t *batch, int32_t on);
 int frehd_batch_kernel_times(frehd_batch_t *batch, int32_t cap, const char **names,
                              double *ms, int64_t *launches, int32_t *n);
