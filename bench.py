@@ -9,11 +9,17 @@ U(0.02, 0.04) per cell, one scalar (0, boundary value 1), Jacobi PCG to
 reference's cold start), mixed_cold (FP32 operator and inner PCG, FP64
 refinement to the same |b - Ax| < eps |b| test, cold start) and mixed_warm
 (the same, refinement started from the extrapolated surface 2 eta^n -
-eta^(n-1)).  A step is one
-oneCompleteStep + monitorCFL over the whole grid; value = interior cells x
-timed steps / wall time of the timed region, of the last mixed form; the
-eta / u / v errors of each mixed form against fp64 after the same steps are
-reported beside it.
+eta^(n-1)).  Each form runs the solver-tiling autotuner after 3 steps and
+before its warmup.  A step is one oneCompleteStep + monitorCFL over the
+whole grid; value = interior cells x timed steps / wall time of the timed
+region, of the last mixed form; the eta / u / v errors of each mixed form
+against fp64 after the same steps are reported beside it.
+
+Beside it (not the value): the large single domain (4096^2 channel, subgrid
+4 x 4 evaluated on the fly, three scalars, river inflow; fp64 and
+mixed_warm, with the solver roofline of each), the batched ensemble (64 x
+2048^2) and the serving loop (48 requests), and the reference's CPU step on
+a bounded sample.
 
 Prints one JSON line (rank 0).  --gpus above 1 is refused: the build runs on
 one GPU.
@@ -84,12 +90,15 @@ def cpu_baseline(n, r, budget_s):
                       f"Jacobi PCG eps 1e-10, FP64, uniform Manning n, {dt:.1f} s on 1 host core"}
 
 
-def run_form(scen, steps, warmup, **over):
-    """Time one precision form: warmup steps, then `steps` timed steps (the
-    call returns after the stream is synchronised), then 4 steps with
-    per-kernel HIP event timing.  Returns the numbers and the final state."""
+def run_form(scen, steps, warmup, tune=True, **over):
+    """Time one precision form: 3 steps, the solver-tiling autotuner, warmup
+    steps, then `steps` timed steps (the call returns after the stream is
+    synchronised), then 4 steps with per-kernel HIP event timing.  Returns
+    the numbers and the final state."""
     from frehdc_amd import device
     dev = device.DeviceScenario(scen, **over)
+    dev.step(3)
+    tiling = dev.autotune(20) if tune else None
     dev.step(warmup)
     it0 = dev.stats()["pcgIterTotal"]
     t0 = time.perf_counter()
@@ -105,7 +114,7 @@ def run_form(scen, steps, warmup, **over):
     sec = t1 - t0
     return {"sec": sec, "ms_per_step": sec * 1e3 / steps, "value": scen.N * steps / sec,
             "iters": (st["pcgIterTotal"] - it0) / steps, "sweeps": st["refineLast"],
-            "relres": st["relResLast"], "kt": kt, "fields": fields}
+            "relres": st["relResLast"], "kt": kt, "fields": fields, "tiling": tiling}
 
 
 def run_batch(members, steps, warmup, **over):
@@ -116,6 +125,8 @@ def run_batch(members, steps, warmup, **over):
     b = device.Batch(len(devs))
     for d in devs:
         b.add(d)
+    b.try_step(2)
+    tiling = b.autotune(10)
     b.try_step(warmup)
     t0 = time.perf_counter()
     rc = b.try_step(steps)
@@ -133,7 +144,7 @@ def run_batch(members, steps, warmup, **over):
         d.close()
     cells = sum(sc.N for sc in members)
     return {"sec": t1 - t0, "value": cells * steps / (t1 - t0), "ms_per_step": (t1 - t0) * 1e3 / steps,
-            "surf": surf, "iters": iters, "kt": kt, "halted": halted}
+            "surf": surf, "iters": iters, "kt": kt, "halted": halted, "tiling": tiling}
 
 
 def batched_ensemble(n_members, size, steps, warmup):
@@ -145,7 +156,7 @@ def batched_ensemble(n_members, size, steps, warmup):
     import numpy as np
     t0 = time.perf_counter()
     base = cases.fbm_surface(size, size, 0.7, np.random.default_rng(5), -1.0, 1.5)
-    Nt = warmup + steps + 8
+    Nt = warmup + steps + 12
     members = [cases.ensemble_member(m, n=size, Nt=Nt, base=base) for m in range(n_members)]
     log(f"bench: batch of {n_members} x {size}^2 built in {time.perf_counter() - t0:.1f} s")
     out = {}
@@ -156,7 +167,7 @@ def batched_ensemble(n_members, size, steps, warmup):
         log(f"kernel times batch {f} (ms/launch, launches):",
             json.dumps({k: [round(v[0], 4), v[1]] for k, v in r["kt"].items()}))
         o = {"cell_steps_per_s": r["value"], "ms_per_step": r["ms_per_step"], "pcg_iterations_last": r["iters"],
-             "halted_members": r["halted"]}
+             "halted_members": r["halted"], "tiling": r["tiling"]}
         if ref is None:
             ref = r["surf"]
         else:
@@ -199,6 +210,51 @@ def traffic_of(kernels, size, subgrid):
     return int(sum(k[n]["bytes"] for n in kernels))
 
 
+def solver_roofline(kt, N, mixed):
+    """Roofline of one solver iteration from event-timed kernel averages:
+    FP64, the k_pcg_dir + k_pcg_res pair (64 B per cell); mixed, one working
+    k_cg1 launch of the inner FP32 solve (32 B per cell)."""
+    ks, per_cell = (["k_cg1_f32"], PCG_ITER_BYTES_F32) if mixed else (["k_pcg_dir", "k_pcg_res"], PCG_ITER_BYTES)
+    t_ms = sum(kt.get(k, (0, 0))[0] for k in ks)
+    alg = per_cell * N
+    achieved = alg / (t_ms * 1e-3) / 1e9 if t_ms > 0 else 0.0
+    return ks, {"bound": "hbm", "kernel": f"solver iteration ({' + '.join(ks)})", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "bytes_per_launch": alg, "ms_per_iteration": round(t_ms, 5)}
+
+
+def large_domain(size, r, steps, warmup):
+    """The large single domain on one GPU: channel size^2 (dx = 1 m), subgrid
+    r x r evaluated on the fly from the pixels, three scalars, tide 0.5 m on
+    YP and 50 m^3/s river inflow on row 0; fp64 and mixed_warm timed, the
+    solver roofline of each, the mixed form's errors against fp64."""
+    from frehdc_amd import abi, cases
+    import numpy as np
+    t0 = time.perf_counter()
+    sc = cases.channel_subgrid(n=size, r=r, Nt=warmup + steps + 20)
+    log(f"bench: large domain {sc.name} (Ne={sc.cfg['Ne']}) built in {time.perf_counter() - t0:.1f} s")
+    out, res = {}, {}
+    for f, over in [("fp64", {}), ("mixed_warm", {"precision": abi.PREC_MIXED, "warmStart": 2})]:
+        r_ = run_form(sc, steps, warmup, **over)
+        res[f] = r_
+        _, roof = solver_roofline(r_["kt"], sc.N, f != "fp64")
+        out[f] = {"cell_steps_per_s": r_["value"], "ms_per_step": r_["ms_per_step"],
+                  "pcg_iterations_per_step": round(r_["iters"], 2), "refinement_sweeps": r_["sweeps"],
+                  "rel_residual_last": r_["relres"], "tiling": r_["tiling"], "solver_roofline": roof}
+        log(f"bench: large {f}: {r_['ms_per_step']:.2f} ms/step, {r_['iters']:.1f} iterations/step, "
+            f"solver {roof['achieved']:.0f} GB/s ({roof['frac']:.3f} of peak)")
+    N = sc.N
+    for k, name in (("surf", "max_abs_eta_err_m"), ("uuXP", "max_abs_u_err"), ("vvYP", "max_abs_v_err")):
+        out["mixed_warm"][name] = float(np.max(np.abs(res["mixed_warm"]["fields"][k][:N] - res["fp64"]["fields"][k][:N])))
+    out["mixed_warm"]["wet_mask_mismatches"] = int(np.count_nonzero(
+        (res["mixed_warm"]["fields"]["depth"][:N] > 0) != (res["fp64"]["fields"]["depth"][:N] > 0)))
+    return {"workload": f"channel {size}x{size}, dx=1 m, subgrid {r}x{r} on the fly (Ne={sc.cfg['Ne']}), dt=0.5 s, "
+                        f"fBm bed (Hurst 0.8) [-1.5, 1] m + meandering channel 2 m deep, 20 cells wide, tide 0.5 m "
+                        f"on YP, river 50 m^3/s on row 0, 3 scalars, Jacobi PCG eps=1e-10, {steps} timed steps "
+                        f"after {warmup}",
+            "cells": N, "forms": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +273,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-count", type=int, default=48)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--large-size", type=int, default=4096)
+    ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--no-large", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -226,7 +285,7 @@ def main():
     from frehdc_amd import abi, cases
     import numpy as np
 
-    total = args.warmup + args.steps + 8
+    total = args.warmup + args.steps + 12
 
     t0 = time.perf_counter()
     if args.subgrid > 0:
@@ -257,7 +316,7 @@ def main():
     for f, r in res.items():
         o = {"cell_steps_per_s": r["value"], "ms_per_step": r["ms_per_step"],
              "pcg_iterations_per_step": round(r["iters"], 2), "refinement_sweeps": r["sweeps"],
-             "rel_residual_last": r["relres"]}
+             "rel_residual_last": r["relres"], "tiling": r["tiling"]}
         if f != "fp64" and "fp64" in res:
             o["max_abs_eta_err_m"] = err(f, "surf")
             o["max_abs_u_err"] = err(f, "uuXP")
@@ -273,21 +332,16 @@ def main():
     # on the handle's stream: FP64, the two kernels of one PCG iteration;
     # mixed, the one launch of an inner FP32 iteration (working launches:
     # the set-up and stop-test launches are timed apart)
-    kt = hr["kt"]
-    if head == "fp64":
-        ks, per_cell, dtype = ["k_pcg_dir", "k_pcg_res"], PCG_ITER_BYTES, "f64"
-    else:
-        ks, per_cell, dtype = ["k_cg1_f32"], PCG_ITER_BYTES_F32, "f32+f64"
-    t_iter_ms = sum(kt.get(k, (0, 0))[0] for k in ks)
-    alg_bytes = per_cell * N
-    achieved = alg_bytes / (t_iter_ms * 1e-3) / 1e9 if t_iter_ms > 0 else 0.0
+    dtype = "f64" if head == "fp64" else "f32+f64"
+    ks, roof = solver_roofline(hr["kt"], N, head != "fp64")
     traffic = traffic_of(ks, args.size, args.subgrid)
-    roof = {"bound": "hbm", "kernel": f"solver iteration ({' + '.join(ks)})",
-            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "bytes_per_launch": alg_bytes, "ms_per_iteration": round(t_iter_ms, 5)}
+    roof["traffic"] = traffic
     if traffic:
-        roof["traffic_GBps"] = round(traffic / (t_iter_ms * 1e-3) / 1e9, 1)
+        roof["traffic_GBps"] = round(traffic / (roof["ms_per_iteration"] * 1e-3) / 1e9, 1)
+
+    large = None
+    if not args.no_large and args.large_size > 0:
+        large = large_domain(args.large_size, 4, args.large_steps, 5)
 
     batched = None
     if not args.no_batch and args.batch_members > 0:
@@ -311,6 +365,7 @@ def main():
                                f"one scalar, Jacobi PCG eps=1e-10, form {head}", "cells": N,
                    "form": head, "forms": forms_out},
         "roofline": roof,
+        "large": large,
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": served,

@@ -222,19 +222,21 @@ struct Carve {
 // enough that every block can re-reduce all block partials cheaply
 inline size_t pcg_blocks(size_t N) { return std::min<size_t>(1024, (N + 255) / 256); }
 
-// k_pcg_dir tiling: tiles of 256 columns x H rows, H chosen so that about
-// 1024 tiles cover the grid (16 waves per CU on 256 CUs, and ~1024 partials
-// for k_pcg_res to re-reduce); the XCD remap pads the tile count to a
-// multiple of 8
+// Solver tiling: tiles of 256 columns x H rows.  The default H gives about
+// `target` tiles: 1024 for the FP64 PCG (16 waves per CU on 256 CUs; one
+// partial per tile for k_pcg_res to re-reduce), 512 for the mixed path,
+// whose k_cg1 writes four partials per tile and re-reduces all of them in
+// every tile of the next launch, a cost that grows with the square of the
+// tile count.  The XCD remap pads the tile count to a multiple of 8.
 struct DirTiling {
     int gx, gy, H, remap, nb;
 };
 
-inline DirTiling dir_tiling(int nx, int ny, int H, int remap)
+inline DirTiling dir_tiling(int nx, int ny, int H, int remap, int target = 1024)
 {
     DirTiling t;
     t.gx = (nx + 255) / 256;
-    if (H <= 0) H = std::max(1, (int)((int64_t)t.gx * ny / 1024));
+    if (H <= 0) H = std::max(1, (int)(((int64_t)t.gx * ny + target - 1) / target));
     t.H = H;
     t.gy = (ny + H - 1) / H;
     t.remap = remap;
@@ -636,7 +638,7 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     p.nx = d->g.nx;
     p.ny = d->g.ny;
     p.N = d->g.N;
-    const DirTiling t = dir_tiling(p.nx, p.ny, H, remap);
+    const DirTiling t = dir_tiling(p.nx, p.ny, H, remap, d->cfg.precision == FREHD_PREC_MIXED ? 512 : 1024);
     p.gx = t.gx;
     p.gy = t.gy;
     p.H = t.H;

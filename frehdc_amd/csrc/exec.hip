@@ -386,6 +386,8 @@ void exec_release_graphs(Exec &e)
     e.heads.clear();
     if (e.tail) hipGraphExecDestroy(e.tail);
     e.tail = nullptr;
+    e.stableSteps = 0;
+    e.seen.clear();
 }
 
 // capture what `enqueue` launches on the group's stream into an executable
@@ -404,6 +406,15 @@ static int capture(Exec &e, F &&enqueue, hipGraphExec_t *out)
     err = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (err != hipSuccess) return fail(FREHD_EHIP, "graph instantiation failed: %s", hipGetErrorString(err));
+    return 0;
+}
+
+static int exec_step_eager(Exec &e)
+{
+    for (int ph = 0; ph < 5; ph++) {
+        int rc = exec_phase(e, ph);
+        if (rc) return rc;
+    }
     return 0;
 }
 
@@ -427,6 +438,17 @@ static int exec_step_graphs(Exec &e)
         if (h.first == key) head = h.second;
     int k = 0, rc;
     if (!head) {
+        // a pattern is captured the second time it comes up; the first time
+        // the step runs eagerly
+        int *count = nullptr;
+        for (auto &sk : e.seen)
+            if (sk.first == key) count = &sk.second;
+        if (!count) {
+            if (e.seen.size() >= 64) e.seen.clear();
+            e.seen.push_back({key, 0});
+            count = &e.seen.back().second;
+        }
+        if (++*count < 2) return exec_step_eager(e);
         if (e.heads.size() >= 16) exec_release_graphs(e);
         rc = capture(e, [&] {
             const dim3 b2(64, 4), g2((unsigned)e.ext.gx2, (unsigned)e.ext.gy2, (unsigned)n);
@@ -472,12 +494,11 @@ static int exec_step_graphs(Exec &e)
 
 int exec_step(Exec &e)
 {
-    if (e.useGraphs && !(e.tim && e.tim->on) && e.count > 0) return exec_step_graphs(e);
-    for (int ph = 0; ph < 5; ph++) {
-        int rc = exec_phase(e, ph);
-        if (rc) return rc;
-    }
-    return 0;
+    // graphs once the group has stepped unchanged for a few steps (a group
+    // whose members change every few steps would spend more on captures
+    // than the graphs save)
+    if (e.useGraphs && !(e.tim && e.tim->on) && e.count > 0 && ++e.stableSteps > 3) return exec_step_graphs(e);
+    return exec_step_eager(e);
 }
 
 int exec_init_subgrid(Exec &e)

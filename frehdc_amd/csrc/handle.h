@@ -85,6 +85,10 @@ struct Exec {
     int useGraphs = 1;
     std::vector<std::pair<std::vector<int>, hipGraphExec_t>> heads;
     hipGraphExec_t tail = nullptr;
+    // capture only what repeats: steps since the last refresh, and how often
+    // each launch pattern without a graph has come up
+    int stableSteps = 0;
+    std::vector<std::pair<std::vector<int>, int>> seen;
 };
 
 // drop the captured steps of a group (after its members or their arguments

@@ -267,11 +267,22 @@ __global__ void __launch_bounds__(256) k_pcg_res(const PcgArgs<T> *__restrict__ 
 // neighbours), so its right-hand side is lifted: a row next to an identity
 // row t gets + G * d_t, with d_t = r_t / (-D_t) the identity row's exact
 // correction; the decoupled system then has the coupled system's solution.
-// It is also Jacobi-scaled, S A S y = S (r + lift), d = S y: unit diagonal,
-// so the inner CG needs no preconditioner and reads no diagonal.  Writes
+// It is also Jacobi-scaled, S A S y = S (r + lift), d = S y with S about
+// |D|^-1/2: unit diagonal up to the FP32 rounding of S, so the inner CG
+// needs no preconditioner and reads no diagonal.  Writes
 // x_s, r32 = (float) S (r + lift), at s = 0 the scaled FP32 coefficients,
 // and the partials b.b (s = 0), r.r (the FP64 true residual) and r32.r32
 // (the inner solve's first stop test).
+// The scaling S of the inner system, per cell from its diagonal: about
+// |D|^-1/2 (one FP32 reciprocal square root).  Any positive diagonal S
+// gives the same solution; what matters is that the operator, the
+// right-hand side and the correction use the same S of a cell, which this
+// one function guarantees.
+__device__ inline double diag_scale(double D)
+{
+    return (double)rsqrtf((float)fabs(D));
+}
+
 __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A, int s)
 {
     const RefineArgs &a = A[blockIdx.z];
@@ -295,8 +306,8 @@ __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A
         const int ih = lane == 0 ? i0 - 1 : (lane == 63 ? i0 + 64 : -1);
         const bool halo = ih >= 0 && ih < nx;
         // x_s at a cell from its diagonal: the inner solve returns the
-        // scaled correction, d = |D|^-1/2 y
-        auto xs = [&](int c, double Dv) { return s > 0 ? xin[c] + (double)a.d[c] / sqrt(fabs(Dv)) : xin[c]; };
+        // scaled correction, d = S y
+        auto xs = [&](int c, double Dv) { return s > 0 ? xin[c] + (double)a.d[c] * diag_scale(Dv) : xin[c]; };
         // rows j-1 (S), j (C), j+1 (N): x_s, diagonal and right-hand side
         double xS = 0, DS = 1, bS = 0, cyS = 0;
         if (valid && j0 > 0) {
@@ -369,15 +380,16 @@ __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A
                     if (i < nx - 1 && De < 0) lift += cxc * ((be + De * xe) / -De);
                 }
                 const double r = bc - sum;
-                // the inner right-hand side, scaled: |D|^-1/2 (r + lift)
-                const float r32 = (float)((r + lift) / sqrt(fabs(Dc)));
+                // the inner right-hand side, scaled: S (r + lift)
+                const double Sc = diag_scale(Dc);
+                const float r32 = (float)((r + lift) * Sc);
                 if (s > 0) xout[c] = xc;
                 a.r32[c] = r32;
                 if (s == 0) {
                     // the scaled inner operator: c / sqrt(D_c D_nb), zero on
                     // faces that touch an identity row or the domain edge
-                    a.cXs[c] = (i < nx - 1 && Dc > 0 && De > 0) ? (float)(cxc / sqrt(Dc * De)) : 0.0f;
-                    a.cYs[c] = (j < ny - 1 && Dc > 0 && Dn > 0) ? (float)(cyc / sqrt(Dc * Dn)) : 0.0f;
+                    a.cXs[c] = (i < nx - 1 && Dc > 0 && De > 0) ? (float)(cxc * Sc * diag_scale(De)) : 0.0f;
+                    a.cYs[c] = (j < ny - 1 && Dc > 0 && Dn > 0) ? (float)(cyc * Sc * diag_scale(Dn)) : 0.0f;
                 }
                 part[0] += (double)r32 * (double)r32;
                 part[1] += (double)r32 * (double)r32;
