@@ -272,8 +272,9 @@ size_t layout(frehd_dev *d, char *base)
     for (auto **p : ngArr) *p = cv.take<double>(Ng);
     a.sStride = (int64_t)((Ng + 31) & ~size_t(31));
     a.S = cv.take<double>((size_t)a.sStride * ns);
+    a.S2 = cv.take<double>((size_t)a.sStride * ns);
     a.Sm = cv.take<double>((size_t)a.sStride * ns);
-    double **nArr[] = {&a.uuYP, &a.vvXP, &a.depthA, &a.depthB, &a.cellV, &a.Snew, &a.cX, &a.cY,
+    double **nArr[] = {&a.uuYP, &a.vvXP, &a.depthA, &a.depthB, &a.cellV, &a.cX, &a.cY,
         &a.D, &a.b, &a.x, &a.r, &a.q, &a.pA, &a.pB};
     for (auto **p : nArr) *p = cv.take<double>(N);
     a.manning = nullptr;
@@ -500,7 +501,7 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     d->self.stream = d->stream;
     d->self.tim = &d->timing;
     HIPCHK(hipHostMalloc((void **)&d->self.hFlags, sizeof(int32_t) * F_STRIDE, hipHostMallocDefault));
-    d->parity = 0;
+    d->midStep = 0;
     d->tt = 0;
     // forcing
     const StepArgs &a = d->args;
@@ -543,7 +544,13 @@ int upload_state(frehd_dev *d, const frehd_fields &s)
     }
     const int ns = d->args.p.nScalar;
     for (int q = 0; q < ns; q++) {
-        if (s.S) { int rc = upload(d, s.S + (size_t)q * Ng, a.S + (size_t)q * a.sStride, Ng * sizeof(double)); if (rc) return rc; }
+        // both concentration buffers: the corners, which no step writes,
+        // then agree whichever buffer is current
+        if (s.S) {
+            int rc = upload(d, s.S + (size_t)q * Ng, a.S + (size_t)q * a.sStride, Ng * sizeof(double));
+            if (!rc) rc = upload(d, s.S + (size_t)q * Ng, a.S2 + (size_t)q * a.sStride, Ng * sizeof(double));
+            if (rc) return rc;
+        }
         if (s.Sm) { int rc = upload(d, s.Sm + (size_t)q * N, a.Sm + (size_t)q * a.sStride, N * sizeof(double)); if (rc) return rc; }
     }
     if (a.p.useSubgrid == 1) {
@@ -555,7 +562,7 @@ int upload_state(frehd_dev *d, const frehd_fields &s)
         }
         if (s.ind) { int rc = upload(d, s.ind, a.ind, Ng * sizeof(int)); if (rc) return rc; }
     }
-    d->parity = 0;
+    d->midStep = 0;
     HIPCHK(hipStreamSynchronize(d->stream));
     return 0;
 }
@@ -916,8 +923,15 @@ int frehd_dev_download(frehd_dev_t *d, frehd_fields *o)
     HIPCHK(hipStreamSynchronize(d->stream));
     const size_t N = d->g.N, Ng = d->g.Ng;
     StepArgs &a = d->args;
-    // the current depth is the buffer the last step wrote
-    const double *depthCur = d->parity ? a.depthB : a.depthA;
+    // the buffers holding the current depth and concentration: the device's
+    // start-of-step buffers, or, between the phases of a step that is not
+    // halted, the new ones once updateData (depth) or scalarTransport (S)
+    // has run
+    Ctl c;
+    COPYW(d->stream, &c, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
+    const bool newDepth = !c.halted && d->midStep >= 2, newS = !c.halted && d->midStep >= 3;
+    const double *depthCur = (c.parity ^ newDepth) ? a.depthB : a.depthA;
+    const double *Scur = (c.sParity ^ newS) ? a.S2 : a.S;
     struct { double *dst; const double *src; size_t n; } tab[] = {
         {o->surf, a.surf, Ng}, {o->surfOld, a.surfOld, Ng}, {o->uuXP, a.uuXP, Ng}, {o->vvYP, a.vvYP, Ng},
         {o->uuYP, a.uuYP, N}, {o->vvXP, a.vvXP, N}, {o->depth, depthCur, N}, {o->depthXP, a.depthXP, Ng},
@@ -937,7 +951,7 @@ int frehd_dev_download(frehd_dev_t *d, frehd_fields *o)
     }
     const int ns = std::max(1, a.p.nScalar);
     for (int q = 0; q < ns; q++) {
-        if (o->S) COPYW(d->stream, o->S + (size_t)q * Ng, a.S + (size_t)q * a.sStride, Ng * sizeof(double), hipMemcpyDeviceToHost);
+        if (o->S) COPYW(d->stream, o->S + (size_t)q * Ng, Scur + (size_t)q * a.sStride, Ng * sizeof(double), hipMemcpyDeviceToHost);
         if (o->Sm) COPYW(d->stream, o->Sm + (size_t)q * N, a.Sm + (size_t)q * a.sStride, N * sizeof(double), hipMemcpyDeviceToHost);
     }
     return 0;

@@ -10,8 +10,8 @@
 namespace frehd {
 
 const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble", "k_pcg_dir",
-    "k_pcg_res", "k_surface", "k_limiter", "k_faces", "k_flux", "k_tidal_velocity",
-    "k_finish", "k_scalar", "k_scalar_bc", "k_step_end", "k_combine", "k_sub_ghosts",
+    "k_pcg_res", "k_surface", "k_limiter", "k_faces", "k_tidal_velocity",
+    "k_finish", "k_scalar", "k_step_end", "k_combine",
     "k_subarea_init", "k_cg1_f32", "k_cg1_f32_setup", "k_cg1_f32_stop", "k_refine", "k_refine_test", "k_gather_flags"};
 
 void Timing::reset()
@@ -85,6 +85,13 @@ int launch(Exec &e, KernelId id, F &&fn)
 
 inline dim3 grid1(int nb, int n) { return dim3((unsigned)std::max(nb, 1), 1, (unsigned)n); }
 
+// keep the solver counts of the last 8 solves
+void remember(Exec &e, std::vector<int> counts)
+{
+    if (e.recent.size() >= 8) e.recent.erase(e.recent.begin());
+    e.recent.push_back(std::move(counts));
+}
+
 int enqueue_gather(Exec &e)
 {
     LAUNCH(K_GATHER, k_gather_flags, dim3((unsigned)((e.count + 63) / 64)), dim3(64), e.dArgs, e.count, e.dFlags);
@@ -145,7 +152,9 @@ int solve_fp64_finish(Exec &e, int it)
     int last = 1;
     for (int m = 0; m < e.count; m++)
         if (!flags(e, m)[F_HALTED]) last = std::max(last, (int)flags(e, m)[F_ITER]);
-    e.lastIters = last;
+    remember(e, {last});
+    e.lastIters = 1;
+    for (auto &r : e.recent) e.lastIters = std::max(e.lastIters, r[0]);
     return 0;
 }
 
@@ -231,8 +240,9 @@ int solve_mixed_finish(Exec &e, int s)
         if ((rc = enqueue_refine(e, ++s))) return rc;
         if ((rc = gather(e))) return rc;
     }
-    // pattern for the next step: per sweep, the most inner iterations any
-    // member used, two more where the pattern cut a solve short
+    // this solve's counts: per sweep, the most inner iterations any member
+    // used, two more where the pattern cut a solve short; the next pattern
+    // covers the last steps' counts sweep by sweep
     int sweeps = 0;
     for (int m = 0; m < e.count; m++)
         if (!flags(e, m)[F_HALTED]) sweeps = std::max(sweeps, (int)flags(e, m)[F_REFDONE]);
@@ -244,7 +254,12 @@ int solve_mixed_finish(Exec &e, int s)
         if (k < (int)e.mixPlan.size() && n >= e.mixPlan[k]) n = e.mixPlan[k] + 2;
         plan.push_back(n);
     }
-    e.mixPlan = plan;
+    remember(e, plan);
+    e.mixPlan.clear();
+    for (auto &r : e.recent) {
+        if (r.size() > e.mixPlan.size()) e.mixPlan.resize(r.size(), 0);
+        for (size_t k = 0; k < r.size(); k++) e.mixPlan[k] = std::max(e.mixPlan[k], r[k]);
+    }
     return 0;
 }
 
@@ -323,7 +338,18 @@ int enqueue_assemble(Exec &e)
     return 0;
 }
 
+static int exec_phase_launch(Exec &e, int phase);
+
 int exec_phase(Exec &e, int phase)
+{
+    int rc = exec_phase_launch(e, phase);
+    if (rc) return rc;
+    // where the members are inside the step (downloads between phases)
+    for (auto *m : e.members) m->midStep = phase == 4 ? 0 : phase;
+    return 0;
+}
+
+static int exec_phase_launch(Exec &e, int phase)
 {
     const int n = e.count;
     if (n == 0) return 0;
@@ -346,29 +372,22 @@ int exec_phase(Exec &e, int phase)
     case 2: { // updateData (+ the CFL numbers of monitorCFL)
         if (e.sub) {
             LAUNCH(K_LIM, k_limiter_sub, g2, b2, e.dArgs);
-            LAUNCH(K_COMBINE, k_combine, grid1(x.nbN, n), b1, e.dArgs);
-            LAUNCH(K_SUBGHOST, k_sub_ghosts, grid1(x.nbGhost, n), b1, e.dArgs);
+            LAUNCH(K_COMBINE, k_combine, grid1(x.nbN + x.nbGhost, n), b1, e.dArgs);
             LAUNCH(K_FACES, k_faces_sub, grid1(x.nb1, n), b1, e.dArgs);
-            LAUNCH(K_FLUX, k_flux_sub, grid1(x.nb1, n), b1, e.dArgs);
             if (x.nbTide > 0) LAUNCH(K_TIDEV, k_tidal_velocity_sub, grid1(x.nbTide, n), b1, e.dArgs);
             LAUNCH(K_FINISH, k_finish_sub, grid1(x.nb1, n), b1, e.dArgs);
         } else {
             LAUNCH(K_LIM, k_limiter, g2, b2, e.dArgs);
             LAUNCH(K_FACES, k_faces, grid1(x.nb1, n), b1, e.dArgs);
-            LAUNCH(K_FLUX, k_flux, grid1(x.nb1, n), b1, e.dArgs);
             if (x.nbTide > 0) LAUNCH(K_TIDEV, k_tidal_velocity, grid1(x.nbTide, n), b1, e.dArgs);
             LAUNCH(K_FINISH, k_finish, grid1(x.nb1, n), b1, e.dArgs);
         }
-        // the new depth is now the current one for downloads; the device
-        // flips its own parity at the end of the step
-        for (auto *m : e.members) m->parity ^= 1;
         return 0;
     }
     case 3:   // scalarTransport (members without scalars skip inside)
         for (int q = 0; q < e.nScalarMax; q++) {
             if (e.sub) LAUNCH(K_SCALAR, k_scalar_sub, g2, b2, e.dArgs, q);
             else LAUNCH(K_SCALAR, k_scalar, g2, b2, e.dArgs, q);
-            LAUNCH(K_SCALARBC, k_scalar_bc, grid1(x.nb1, n), b1, e.dArgs, q);
         }
         return 0;
     case 4:   // monitorCFL's stop test
@@ -478,17 +497,11 @@ static int exec_step_graphs(Exec &e)
             return 0;
         }, &e.tail);
         if (rc) return rc;
-        // the capture ran the host bookkeeping of phases 2 and 4 once
-        for (auto *m : e.members) {
-            m->parity ^= 1;
-            m->tt -= 1;
-        }
+        // the capture ran the host bookkeeping of phase 4 once
+        for (auto *m : e.members) m->tt -= 1;
     }
     HIPCHK(hipGraphLaunch(e.tail, e.stream));
-    for (auto *m : e.members) {
-        m->parity ^= 1;
-        m->tt += 1;
-    }
+    for (auto *m : e.members) m->tt += 1;
     return 0;
 }
 
@@ -506,8 +519,7 @@ int exec_init_subgrid(Exec &e)
     const Extents &x = e.ext;
     const int n = e.count;
     LAUNCH(K_SUBINIT, k_subarea_init, grid1(x.nbN, n), dim3(256), e.dArgs);
-    LAUNCH(K_COMBINE, k_combine, grid1(x.nbN, n), dim3(256), e.dArgs);
-    LAUNCH(K_SUBGHOST, k_sub_ghosts, grid1(x.nbGhost, n), dim3(256), e.dArgs);
+    LAUNCH(K_COMBINE, k_combine, grid1(x.nbN + x.nbGhost, n), dim3(256), e.dArgs);
     HIPCHK(hipStreamSynchronize(e.stream));
     return 0;
 }

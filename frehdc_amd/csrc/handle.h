@@ -31,8 +31,8 @@ int fail(int code, const char *fmt, ...);
                                                    hipGetErrorString(e_));                \
     } while (0)
 
-enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_FLUX, K_TIDEV,
-                K_FINISH, K_SCALAR, K_SCALARBC, K_END, K_COMBINE, K_SUBGHOST, K_SUBINIT,
+enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_TIDEV,
+                K_FINISH, K_SCALAR, K_END, K_COMBINE, K_SUBINIT,
                 K_CG32, K_CG32_SET, K_CG32_STOP, K_REFINE, K_RTEST, K_GATHER, K_COUNT };
 extern const char *kKernelNames[K_COUNT];
 
@@ -75,7 +75,12 @@ struct Exec {
     Extents ext{};
     int sub = 0, mixed = 0, nScalarMax = 0, maxIter = 0, maxRefine = 0;
     int lastIters = 16;
-    std::vector<int> mixPlan;           // inner iterations per sweep of the last mixed solve
+    std::vector<int> mixPlan;           // inner iterations per sweep of the next mixed solve's pattern
+    // the last steps' solver counts (FP64 iterations, or inner iterations
+    // per sweep): the next pattern covers their maximum, since a launch that
+    // finds its solve converged costs a few microseconds and a pattern that
+    // falls short costs extra host synchronisations
+    std::vector<std::vector<int>> recent;
     Timing *tim = nullptr;
     std::vector<frehd_dev_impl *> members;
     // captured steps (hipGraph): the step's head (predictor, assembly, the
@@ -107,7 +112,7 @@ struct frehd_dev_impl {
     PcgArgs<double> pcg;                // FP64 solve
     PcgArgs<float> pcg32;               // mixed precision: inner FP32 solve
     RefineArgs ref;                     // mixed precision: FP64 refinement
-    int parity;                         // host view: buffer holding the current depth
+    int midStep;                        // last phase run of an unfinished step (0: at a step boundary)
     int tt;                             // host view of the step counter
     Exec self;                          // the handle as a group of one
     Timing timing;

@@ -46,6 +46,7 @@ struct Ctl {
     int halted;               // set when max CFL > 20
     int haltStep;
     int parity;               // depth buffer of the step's start: 0 -> depthA
+    int sParity;              // scalar buffer of the step's start: 0 -> S
     double lastMaxCFLx, lastMaxCFLy;
     double Vloss[4];
     PcgScalars pcg;
@@ -67,11 +68,14 @@ struct StepArgs {
     // Ng arrays
     double *surf, *surfOld, *uuXP, *vvYP, *depthXP, *depthYP, *EnXP, *EnYP, *DragXP, *DragYP;
     double *CDXP, *CDYP, *Fuu, *Fvv, *CFLx, *CFLy, *wtfXP, *wtfYP, *bottomZ, *bottomXP, *bottomYP;
-    double *S, *Sm;           // nScalar arrays with stride sStride
+    // nScalar arrays with stride sStride; the concentration is
+    // double-buffered (S, S2; Ctl.sParity selects the step's start), the
+    // scalar mass Sm is not
+    double *S, *S2, *Sm;
     int64_t sStride;
     // interior arrays; the depth is double-buffered: Ctl.parity selects the
     // buffer holding the depth at the start of the step
-    double *uuYP, *vvXP, *depthA, *depthB, *cellV, *Snew, *manning;
+    double *uuYP, *vvXP, *depthA, *depthB, *cellV, *manning;
     uint8_t *rmv, *flags;     // flags: bit0 tideP cell, bit1 tideM cell, bits 2-7 inflow count
     // operator and solver vectors
     double *cX, *cY, *D, *b, *x, *r, *q, *pA, *pB;
@@ -174,6 +178,8 @@ __global__ void k_refine_test(const RefineArgs *__restrict__ A, int s);
 
 __device__ inline const double *depth_cur(const StepArgs &a) { return a.ctl->parity ? a.depthB : a.depthA; }
 __device__ inline double *depth_nxt(const StepArgs &a) { return a.ctl->parity ? a.depthA : a.depthB; }
+__device__ inline const double *s_cur(const StepArgs &a) { return a.ctl->sParity ? a.S2 : a.S; }
+__device__ inline double *s_nxt(const StepArgs &a) { return a.ctl->sParity ? a.S : a.S2; }
 
 enum SubTable { T_V, T_Z, T_VXP, T_VXM, T_VYP, T_VYM, T_NP, T_NM, T_OP, T_OM };
 
@@ -182,11 +188,9 @@ __global__ void k_assemble(const StepArgs *__restrict__ A);
 __global__ void k_surface(const StepArgs *__restrict__ A);
 __global__ void k_limiter(const StepArgs *__restrict__ A);
 __global__ void k_faces(const StepArgs *__restrict__ A);
-__global__ void k_flux(const StepArgs *__restrict__ A);
 __global__ void k_tidal_velocity(const StepArgs *__restrict__ A);
 __global__ void k_finish(const StepArgs *__restrict__ A);
 __global__ void k_scalar(const StepArgs *__restrict__ A, int q);
-__global__ void k_scalar_bc(const StepArgs *__restrict__ A, int q);
 __global__ void k_step_end(const StepArgs *__restrict__ A);
 // subgrid variants
 __global__ void k_predictor_sub(const StepArgs *__restrict__ A);
@@ -194,9 +198,7 @@ __global__ void k_assemble_sub(const StepArgs *__restrict__ A);
 __global__ void k_subarea_init(const StepArgs *__restrict__ A);
 __global__ void k_limiter_sub(const StepArgs *__restrict__ A);
 __global__ void k_combine(const StepArgs *__restrict__ A);
-__global__ void k_sub_ghosts(const StepArgs *__restrict__ A);
 __global__ void k_faces_sub(const StepArgs *__restrict__ A);
-__global__ void k_flux_sub(const StepArgs *__restrict__ A);
 __global__ void k_tidal_velocity_sub(const StepArgs *__restrict__ A);
 __global__ void k_finish_sub(const StepArgs *__restrict__ A);
 __global__ void k_scalar_sub(const StepArgs *__restrict__ A, int q);

@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(256) k_limiter(const StepArgs *__restrict__ A)
         const double cellV = p.dx * p.dy * depth_cur(a)[c];
         a.cellV[c] = cellV;
         for (int q = 0; q < p.nScalar; q++)
-            a.Sm[(size_t)q * a.sStride + c] = a.S[(size_t)q * a.sStride + c] * cellV;
+            a.Sm[(size_t)q * a.sStride + c] = s_cur(a)[(size_t)q * a.sStride + c] * cellV;
         // updateDepth
         const double d = h - a.bottomZ[c];
         depth_nxt(a)[c] = d < 0 ? 0 : d;
@@ -444,17 +444,71 @@ __global__ void __launch_bounds__(256) k_faces(const StepArgs *__restrict__ A)
     if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blockIdx.x] = loss[0];
 }
 
+// updateVelocity + waterfallCorrection of interior cell c's east and north
+// faces, given their clipped depths; a pure function of the step's inputs,
+// so a ghost face can recompute the face it mirrors
+struct FaceVel {
+    double u, v, fX, fY, loss;
+};
+
+__device__ inline FaceVel face_vel(const StepArgs &a, int c, double dX, double dY)
+{
+    const Grid &g = a.g;
+    const P &p = a.p;
+    const int i = c % g.nx, j = c / g.nx;
+    const int e = nbE(g, c, i, j), n = nbN(g, c, i, j);
+    const double hc = a.surf[c];
+    const double bXc = a.bottomXP[c], bYc = a.bottomYP[c];
+    FaceVel r{0, 0, 0, 0, 0};
+    // waterfall flags of this cell's east and north faces
+    if (hc < bXc - 0.0 && dX >= p.wtfh) r.fX = -1;
+    if (e < g.N && a.surf[e] < bXc - 0.0 && dX >= p.wtfh) r.fX = 1;
+    if (hc < bYc - 0.0 && dY >= p.wtfh) r.fY = -1;
+    if (n < g.N && a.surf[n] < bYc - 0.0 && dY >= p.wtfh) r.fY = 1;
+    // updateVelocity
+    double uc = a.EnXP[c] - (p.g * p.dt / p.dx) * (a.surf[e] - hc);
+    double vc = a.EnYP[c] - (p.g * p.dt / p.dy) * (a.surf[n] - hc);
+    if (dX == 0) uc = 0;
+    if (dY == 0) vc = 0;
+    const bool rmvC = a.rmv[c] == 1;
+    if (rmvC || (e < g.N && a.rmv[e] == 1)) uc = 0;
+    if (rmvC || (n < g.N && a.rmv[n] == 1)) vc = 0;
+    // waterfallCorrection (Cw = 0.7, CFLmax = 0.24)
+    if (r.fX != 0) {
+        const double vel0 = uc;
+        uc = r.fX * 0.7 * sqrt(2 * p.g * dX);
+        if (uc > 0.24 * p.dx / p.dt) uc = 0.24 * p.dx / p.dt;
+        r.loss -= (vel0 - uc) * p.dt * dX * p.dy;
+    }
+    if (r.fY != 0) {
+        const double vel0 = vc;
+        vc = r.fY * 0.7 * sqrt(2 * p.g * dY);
+        if (vc > 0.24 * p.dy / p.dt) vc = 0.24 * p.dy / p.dt;
+        r.loss -= (vel0 - vc) * p.dt * dY * p.dx;
+    }
+    r.u = uc;
+    r.v = vc;
+    return r;
+}
+
+__device__ inline double clipped(double d) { return d < 0 ? 0 : d; }
+
+// one Ng index: its face depths, velocities and flow rates
+// (computeFaceFlowRate, nsfunctions.c:350-358).  A ghost face takes the
+// velocity of the face it mirrors (enforceVelocityBC, nsfunctions.c:289-314),
+// recomputed here from the same inputs, so the flow rates follow in the same
+// kernel.  The rmvCFL pass also zeroes the iM-ghost face of a left-column
+// cell and the jM-ghost face of a bottom-row cell; enforceVelocityBC
+// overwrites those ghosts right after, so nothing is kept of it.
 __device__ inline void faces_cell(const StepArgs &a, int k, double &loss)
 {
     const Grid &g = a.g;
     const P &p = a.p;
-    double dX = face_depth_x(a, k);
-    double dY = face_depth_y(a, k);
-    // iP ghosts of depthYP copy the iPbd value before its clip; clip applies to all
-    if (dX < 0) dX = 0;
-    if (dY < 0) dY = 0;
+    const double dX = clipped(face_depth_x(a, k));
+    const double dY = clipped(face_depth_y(a, k));
     a.depthXP[k] = dX;
     a.depthYP[k] = dY;
+    double u, v;
     if (k >= g.N) {
         // sticky ghost waterfall flags from the XM / YM tests of the adjacent
         // boundary cell (never reset by the reference)
@@ -466,70 +520,19 @@ __device__ inline void faces_cell(const StepArgs &a, int k, double &loss)
             const int jj = r - 2 * g.nx - g.ny;
             if (a.surf[jj * g.nx] < a.bottomXP[k] - 0.0 && dX >= p.wtfh) a.wtfXP[k] = 1;
         }
-        return;
-    }
-    const int c = k, i = c % g.nx, j = c / g.nx;
-    const int e = nbE(g, c, i, j), n = nbN(g, c, i, j);
-    const double hc = a.surf[c];
-    const double bXc = a.bottomXP[c], bYc = a.bottomYP[c];
-    // waterfall flags of this cell's east and north faces
-    double fX = 0, fY = 0;
-    if (hc < bXc - 0.0 && dX >= p.wtfh) fX = -1;
-    if (e < g.N && a.surf[e] < bXc - 0.0 && dX >= p.wtfh) fX = 1;
-    if (hc < bYc - 0.0 && dY >= p.wtfh) fY = -1;
-    if (n < g.N && a.surf[n] < bYc - 0.0 && dY >= p.wtfh) fY = 1;
-    // updateVelocity
-    double uc = a.EnXP[c] - (p.g * p.dt / p.dx) * (a.surf[e] - hc);
-    double vc = a.EnYP[c] - (p.g * p.dt / p.dy) * (a.surf[n] - hc);
-    if (dX == 0) uc = 0;
-    if (dY == 0) vc = 0;
-    const bool rmvC = a.rmv[c] == 1;
-    if (rmvC || (e < g.N && a.rmv[e] == 1)) uc = 0;
-    if (rmvC || (n < g.N && a.rmv[n] == 1)) vc = 0;
-    // waterfallCorrection (Cw = 0.7, CFLmax = 0.24)
-    if (fX != 0) {
-        const double vel0 = uc;
-        uc = fX * 0.7 * sqrt(2 * p.g * dX);
-        if (uc > 0.24 * p.dx / p.dt) uc = 0.24 * p.dx / p.dt;
-        loss -= (vel0 - uc) * p.dt * dX * p.dy;
-    }
-    if (fY != 0) {
-        const double vel0 = vc;
-        vc = fY * 0.7 * sqrt(2 * p.g * dY);
-        if (vc > 0.24 * p.dy / p.dt) vc = 0.24 * p.dy / p.dt;
-        loss -= (vel0 - vc) * p.dt * dY * p.dx;
-    }
-    a.uuXP[c] = uc;
-    a.vvYP[c] = vc;
-    // the rmvCFL pass also zeroes the iM-ghost face of a left-column cell and
-    // the jM-ghost face of a bottom-row cell; those ghosts are overwritten by
-    // enforceVelocityBC right after, so nothing is stored for them here
-}
-
-// ---------------------------------------------------------------------------
-// enforceVelocityBC + computeFaceFlowRate over Ng (nsfunctions.c:289-314,
-// 350-358).
-__global__ void __launch_bounds__(256) k_flux(const StepArgs *__restrict__ A)
-{
-    const StepArgs &a = A[blockIdx.z];
-    if (blockIdx.x >= a.nb1) return;
-    if (a.ctl->halted) return;
-    const Grid g = a.g;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= g.Ng) return;
-    const P &p = a.p;
-    double u, v;
-    if (k < g.N) {
-        u = a.uuXP[k];
-        v = a.vvYP[k];
+        const int su = ghost_source(g, k, true), sv = ghost_source(g, k, false);
+        u = face_vel(a, su, clipped(face_depth_x(a, su)), clipped(face_depth_y(a, su))).u;
+        v = face_vel(a, sv, clipped(face_depth_x(a, sv)), clipped(face_depth_y(a, sv))).v;
     } else {
-        u = a.uuXP[ghost_source(g, k, true)];
-        v = a.vvYP[ghost_source(g, k, false)];
-        a.uuXP[k] = u;
-        a.vvYP[k] = v;
+        const FaceVel f = face_vel(a, k, dX, dY);
+        u = f.u;
+        v = f.v;
+        loss += f.loss;
     }
-    a.Fuu[k] = u * p.dy * a.depthXP[k];
-    a.Fvv[k] = v * p.dx * a.depthYP[k];
+    a.uuXP[k] = u;
+    a.vvYP[k] = v;
+    a.Fuu[k] = u * p.dy * dX;
+    a.Fvv[k] = v * p.dx * dY;
 }
 
 // adjustTidalVelocity (nsfunctions.c:1034-1078): one thread per tidal cell.
@@ -627,8 +630,11 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgs *__restrict__ A)
 // ---------------------------------------------------------------------------
 // Scalar transport (scalar.c:30-56, upwind): scalarAdvection +
 // scalarDiffusion + updateScalar's connectivity limiter, inflow mass and
-// concentration, and the out-of-range reset (scalar.c:68-97, 259-416).
-// The new concentration goes to Snew (neighbours read the old S).
+// concentration, and the out-of-range reset (scalar.c:68-97, 259-416), then
+// enforceScalarBC (scalar.c:419-471).  Neighbours read the step's start
+// buffer, the new concentration goes to the other one: the interior with
+// the tidal Dirichlet values and zero where dry, and the ghosts mirroring a
+// boundary cell (corners untouched) with that cell's value before those.
 template <bool SUB>
 __device__ inline void scalar_body(const StepArgs &a, int q)
 {
@@ -641,7 +647,7 @@ __device__ inline void scalar_body(const StepArgs &a, int q)
     const int tt = cur_tt(a);
     const int c = j * g.nx + i;
     const int e = nbE(g, c, i, j), w = nbW(g, c, i, j), n = nbN(g, c, i, j), s = nbS(g, c, i, j);
-    const double *S = a.S + (size_t)q * a.sStride;
+    const double *S = s_cur(a) + (size_t)q * a.sStride;
     double *Sm = a.Sm + (size_t)q * a.sStride;
     const double *Fu = a.Fuu, *Fv = a.Fvv;
     const double dt = p.dt, dx = p.dx, dy = p.dy;
@@ -696,7 +702,16 @@ __device__ inline void scalar_body(const StepArgs &a, int q)
             depth_nxt(a)[c] = 0;
         }
     }
-    a.Snew[c] = Sv;
+    double *Snx = s_nxt(a) + (size_t)q * a.sStride;
+    if (j == g.ny - 1) Snx[gjP(g, i)] = Sv;
+    if (j == 0) Snx[gjM(g, i)] = Sv;
+    if (i == g.nx - 1) Snx[giP(g, j)] = Sv;
+    if (i == 0) Snx[giM(g, j)] = Sv;
+    const uint8_t fl = a.flags[c];
+    if ((fl & 1) && p.bcType != 2) Sv = p.useConstTidePS == 1 ? p.tidePS : a.tidalPS[tt];
+    if ((fl & 2) && p.bcType != 1) Sv = p.useConstTideMS == 1 ? p.tideMS : a.tidalMS[tt];
+    if (depth_nxt(a)[c] <= 0) Sv = 0;
+    Snx[c] = Sv;
 }
 
 __global__ void __launch_bounds__(256) k_scalar(const StepArgs *__restrict__ A, int q)
@@ -712,31 +727,6 @@ __global__ void __launch_bounds__(256) k_scalar_sub(const StepArgs *__restrict__
     scalar_body<true>(a, q);
 }
 
-// enforceScalarBC (scalar.c:419-471) over the interior and the four ghost
-// blocks (corners untouched).
-__global__ void __launch_bounds__(256) k_scalar_bc(const StepArgs *__restrict__ A, int q)
-{
-    const StepArgs &a = A[blockIdx.z];
-    if (blockIdx.x >= a.nb1) return;
-    if (a.ctl->halted || q >= a.p.nScalar) return;
-    const Grid g = a.g;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= g.Ng - 4) return;
-    const P &p = a.p;
-    const int tt = cur_tt(a);
-    double *S = a.S + (size_t)q * a.sStride;
-    if (k >= g.N) {
-        S[k] = a.Snew[ghost_source(g, k, false)];
-        return;
-    }
-    double v = a.Snew[k];
-    const uint8_t fl = a.flags[k];
-    if ((fl & 1) && p.bcType != 2) v = p.useConstTidePS == 1 ? p.tidePS : a.tidalPS[tt];
-    if ((fl & 2) && p.bcType != 1) v = p.useConstTideMS == 1 ? p.tideMS : a.tidalMS[tt];
-    if (depth_nxt(a)[k] <= 0) v = 0;
-    S[k] = v;
-}
-
 // Step end: volume-loss and CFL reductions (block partials in block order),
 // and the max CFL test of monitorCFL (nsfunctions.c:1226-1231), which halts
 // the handle instead of exiting the process.  One block of 1024 threads.
@@ -746,21 +736,35 @@ __global__ void __launch_bounds__(1024) k_step_end(const StepArgs *__restrict__ 
     __shared__ double lds[16 * 4];
     Ctl *c = a.ctl;
     if (c->halted) return;
+    // the loads of a thread's partials are issued together (4 per thread
+    // per array up to 4096 blocks), then summed in block order
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < a.nb2; b += blockDim.x) {
-        v[0] += a.vpart[b];
-        v[1] += a.vpart[a.maxBlocks + b];
-    }
-    for (int b = threadIdx.x; b < a.nb1; b += blockDim.x) {
-        v[2] += a.vpart[2 * a.maxBlocks + b];
-        v[3] += a.vpart[3 * a.maxBlocks + b];
+    double mx = 0.0, my = 0.0;
+    const int t = threadIdx.x, B = blockDim.x, mb = a.maxBlocks;
+    for (int b0 = 0; b0 < a.nb1 || b0 < a.nb2; b0 += 4 * B) {
+        double l0[4], l1[4], l2[4], l3[4], c0[4], c1[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int b = b0 + u * B + t;
+            const bool in2 = b < a.nb2, in1 = b < a.nb1;
+            l0[u] = in2 ? a.vpart[b] : 0.0;
+            l1[u] = in2 ? a.vpart[mb + b] : 0.0;
+            l2[u] = in1 ? a.vpart[2 * mb + b] : 0.0;
+            l3[u] = in1 ? a.vpart[3 * mb + b] : 0.0;
+            c0[u] = in1 ? a.cflpart[b] : 0.0;
+            c1[u] = in1 ? a.cflpart[mb + b] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            v[0] += l0[u];
+            v[1] += l1[u];
+            v[2] += l2[u];
+            v[3] += l3[u];
+            mx = fmax(mx, c0[u]);
+            my = fmax(my, c1[u]);
+        }
     }
     block_sum<4>(v, lds);
-    double mx = 0.0, my = 0.0;
-    for (int b = threadIdx.x; b < a.nb1; b += blockDim.x) {
-        mx = fmax(mx, a.cflpart[b]);
-        my = fmax(my, a.cflpart[a.maxBlocks + b]);
-    }
     mx = block_max(mx, lds);
     my = block_max(my, lds);
     if (threadIdx.x != 0) return;
@@ -772,6 +776,7 @@ __global__ void __launch_bounds__(1024) k_step_end(const StepArgs *__restrict__ 
     c->relResLast = c->pcg.bNorm > 0 ? c->pcg.rNorm / c->pcg.bNorm : 1.0;
     c->steps += 1;
     c->parity ^= 1;     // the new depth becomes the next step's start
+    if (a.p.nScalar > 0) c->sParity ^= 1;   // and the new concentration
     if (fabs(mx) > 20 || fabs(my) > 20) {
         c->halted = 1;
         c->haltStep = c->tt;
@@ -903,7 +908,7 @@ __global__ void __launch_bounds__(256) k_limiter_sub(const StepArgs *__restrict_
         const double cellV = a.V[c];   // the previous step's subgrid volume
         a.cellV[c] = cellV;
         for (int q = 0; q < p.nScalar; q++)
-            a.Sm[(size_t)q * a.sStride + c] = a.S[(size_t)q * a.sStride + c] * cellV;
+            a.Sm[(size_t)q * a.sStride + c] = s_cur(a)[(size_t)q * a.sStride + c] * cellV;
         subarea_cell(a, c, i, j, h, depth_cur(a)[c]);
         const double d = h - a.bottomZ[c];
         depth_nxt(a)[c] = d < 0 ? 0 : d;
@@ -911,19 +916,19 @@ __global__ void __launch_bounds__(256) k_limiter_sub(const StepArgs *__restrict_
     store_vloss2(a, loss, lds);
 }
 
-// combineSubArea for the interior (subgrid.c:254-462 and the clip loop
-// :466-478), plus the interior faces the boundary loop closes: Nx of the
-// right column and Oy of the non-tidal top row.  useminV = 1 changes volumes
-// only where Nmin > 0, which stays 0 with useminA = 0.
-__global__ void __launch_bounds__(256) k_combine(const StepArgs *__restrict__ A)
+// combineSubArea (subgrid.c:254-462 and the clip loop :466-478) of interior
+// cell c, plus the interior faces the boundary loop closes: Nx of the right
+// column and Oy of the non-tidal top row.  useminV = 1 changes volumes only
+// where Nmin > 0, which stays 0 with useminA = 0.  A pure function of
+// computeSubArea's outputs, so a ghost can recompute the cell it mirrors.
+struct Combined {
+    double Nx, Oy, Vx, Vy, Zx, Zy, V, Z;
+};
+
+__device__ inline Combined combine_cell(const StepArgs &a, int c)
 {
-    const StepArgs &a = A[blockIdx.z];
-    if (blockIdx.x >= a.nbN) return;
-    if (a.ctl->halted) return;
-    const Grid g = a.g;
+    const Grid &g = a.g;
     const P &p = a.p;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= g.N) return;
     const int i = c % g.nx, j = c / g.nx, N = g.N;
     const int e = nbE(g, c, i, j), n = nbN(g, c, i, j);
     const double dxdy = p.dx * p.dy;
@@ -969,33 +974,43 @@ __global__ void __launch_bounds__(256) k_combine(const StepArgs *__restrict__ A)
     if (Oy < Amin) Oy = 0.0;
     if (i == g.nx - 1) Nx = 0;                                    // subgrid.c:486
     if (j == g.ny - 1 && p.bcType != 2 && !(a.flags[c] & 1)) Oy = 0;   // :508-524
-    a.Nx[c] = Nx; a.Oy[c] = Oy; a.Vx[c] = Vx; a.Vy[c] = Vy;
-    a.Zx[c] = Zx; a.Zy[c] = Zy; a.V[c] = V; a.Z[c] = Z;
+    return Combined{Nx, Oy, Vx, Vy, Zx, Zy, V, Z};
 }
 
-// combineSubArea's ghost copies (subgrid.c:480-506) over the 2nx + 2ny
-// ghosts.  The iM/iP ghosts of the top row take Oy before the top-row
-// closure, i.e. the clipped Op of the corner cell.
-__global__ void __launch_bounds__(256) k_sub_ghosts(const StepArgs *__restrict__ A)
+// combineSubArea over the interior (blocks [0, nbN)) and its ghost copies
+// (subgrid.c:480-506) over the 2nx + 2ny ghosts (the next nbGhost blocks),
+// each ghost recomputing the cell it mirrors.  The iM/iP ghosts of the top
+// row take Oy before the top-row closure, i.e. the clipped Op of the corner
+// cell.
+__global__ void __launch_bounds__(256) k_combine(const StepArgs *__restrict__ A)
 {
     const StepArgs &a = A[blockIdx.z];
-    if (blockIdx.x >= a.nbGhost) return;
+    if (blockIdx.x >= a.nbN + a.nbGhost) return;
     if (a.ctl->halted) return;
     const Grid g = a.g;
-    const int k = g.N + blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int)blockIdx.x < a.nbN) {
+        const int c = blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= g.N) return;
+        const Combined v = combine_cell(a, c);
+        a.Nx[c] = v.Nx; a.Oy[c] = v.Oy; a.Vx[c] = v.Vx; a.Vy[c] = v.Vy;
+        a.Zx[c] = v.Zx; a.Zy[c] = v.Zy; a.V[c] = v.V; a.Z[c] = v.Z;
+        return;
+    }
+    const int k = g.N + (blockIdx.x - a.nbN) * blockDim.x + threadIdx.x;
     if (k >= g.Ng - 4) return;
     const int src = ghost_source(g, k, false);
     const int r = k - g.N;
-    a.V[k] = a.V[src]; a.Z[k] = a.Z[src];
-    a.Vx[k] = a.Vx[src]; a.Vy[k] = a.Vy[src];
-    a.Zx[k] = a.Zx[src]; a.Zy[k] = a.Zy[src];
+    const Combined v = combine_cell(a, src);
+    a.V[k] = v.V; a.Z[k] = v.Z;
+    a.Vx[k] = v.Vx; a.Vy[k] = v.Vy;
+    a.Zx[k] = v.Zx; a.Zy[k] = v.Zy;
     if (r < 2 * g.nx) {                      // jP / jM ghosts
-        a.Nx[k] = a.Nx[src];
-        a.Oy[k] = r < g.nx ? a.Oy[src] : 0.0;
+        a.Nx[k] = v.Nx;
+        a.Oy[k] = r < g.nx ? v.Oy : 0.0;
     } else {                                 // iP / iM ghosts
         const int jj = (r - 2 * g.nx) % g.ny;
         a.Nx[k] = 0.0;
-        double oy = a.Oy[src];
+        double oy = v.Oy;
         if (jj == g.ny - 1) { oy = a.Op[src]; if (oy < 0.002) oy = 0.0; }
         a.Oy[k] = oy;
     }
@@ -1006,40 +1021,23 @@ __global__ void __launch_bounds__(256) k_sub_ghosts(const StepArgs *__restrict__
 // for the faces of every Ng index.  Interior flags are recomputed every step
 // (the Sub variant resets them before detecting); the ghost flags written by
 // the XM/YM tests of boundary cells stay set, as in the reference.
-__device__ inline void faces_sub_cell(const StepArgs &a, int k, double &loss)
+// updateVelocitySub + waterfallCorrectionSub of interior cell c's east and
+// north faces (with the waterfall flags); a pure function of the step's
+// inputs, so a ghost face can recompute the face it mirrors
+__device__ inline FaceVel face_vel_sub(const StepArgs &a, int c)
 {
     const Grid &g = a.g;
     const P &p = a.p;
-    double dX = face_depth_x(a, k);
-    double dY = face_depth_y(a, k);
-    if (dX < 0) dX = 0;
-    if (dY < 0) dY = 0;
-    a.depthXP[k] = dX;
-    a.depthYP[k] = dY;
-    if (k >= g.N) {
-        int r = k - g.N;
-        if (r >= g.nx && r < 2 * g.nx) {             // jM ghost: YM test of cell (i, 0)
-            const int i = r - g.nx;
-            const double bot = a.bottomYP[k];
-            if (a.surf[i] < bot - 0.0 && a.surf[k] - bot >= p.wtfh) a.wtfYP[k] = 1;
-        } else if (r >= 2 * g.nx + g.ny && r < 2 * g.nx + 2 * g.ny) {   // iM ghost
-            const int jj = r - 2 * g.nx - g.ny;
-            const double bot = a.bottomXP[k];
-            if (a.surf[jj * g.nx] < bot - 0.0 && a.surf[k] - bot >= p.wtfh) a.wtfXP[k] = 1;
-        }
-        return;
-    }
-    const int c = k, i = c % g.nx, j = c / g.nx;
+    const int i = c % g.nx, j = c / g.nx;
     const int e = nbE(g, c, i, j), n = nbN(g, c, i, j);
     const double hc = a.surf[c], he = a.surf[e], hn = a.surf[n];
     const double bXc = a.bottomXP[c], bYc = a.bottomYP[c];
-    double fX = 0, fY = 0;
+    FaceVel r{0, 0, 0, 0, 0};
+    double &fX = r.fX, &fY = r.fY;
     if (hc < bXc - 0.0 && he - bXc >= p.wtfh) fX = -1;
     if (e < g.N && he < bXc - 0.0 && hc - bXc >= p.wtfh) fX = 1;
     if (hc < bYc - 0.0 && hn - bYc >= p.wtfh) fY = -1;
     if (n < g.N && hn < bYc - 0.0 && hc - bYc >= p.wtfh) fY = 1;
-    a.wtfXP[c] = fX;
-    a.wtfYP[c] = fY;
     const double Nxc = a.Nx[c], Oyc = a.Oy[c], Vxc = a.Vx[c], Vyc = a.Vy[c];
     const double effHx = Vxc > 0 ? Nxc / Vxc : 0;
     const double effHy = Vyc > 0 ? Oyc / Vyc : 0;
@@ -1064,16 +1062,56 @@ __device__ inline void faces_sub_cell(const StepArgs &a, int k, double &loss)
         const double vel0 = uc;
         uc = fX * Cw * sqrt(2 * p.g * Nxc / p.dy);
         if (fabs(uc) > CFLmax * p.dx / p.dt) uc = fX * CFLmax * p.dx / p.dt;
-        loss -= (vel0 - uc) * p.dt * Nxc;
+        r.loss -= (vel0 - uc) * p.dt * Nxc;
     }
     if (fY != 0) {
         const double vel0 = vc;
         vc = fY * Cw * sqrt(2 * p.g * Oyc / p.dx);
         if (fabs(vc) > CFLmax * p.dy / p.dt) vc = fY * CFLmax * p.dy / p.dt;
-        loss -= (vel0 - vc) * p.dt * Oyc;
+        r.loss -= (vel0 - vc) * p.dt * Oyc;
     }
-    a.uuXP[c] = uc;
-    a.vvYP[c] = vc;
+    r.u = uc;
+    r.v = vc;
+    return r;
+}
+
+// one Ng index: face depths, velocities, waterfall flags and flow rates
+// (computeFaceFlowRateSub, subgrid.c:1190-1212); a ghost face takes the
+// velocity of the face it mirrors (enforceVelocityBC), recomputed here.
+__device__ inline void faces_sub_cell(const StepArgs &a, int k, double &loss)
+{
+    const Grid &g = a.g;
+    const P &p = a.p;
+    const double dX = clipped(face_depth_x(a, k));
+    const double dY = clipped(face_depth_y(a, k));
+    a.depthXP[k] = dX;
+    a.depthYP[k] = dY;
+    double u, v;
+    if (k >= g.N) {
+        int r = k - g.N;
+        if (r >= g.nx && r < 2 * g.nx) {             // jM ghost: YM test of cell (i, 0)
+            const int i = r - g.nx;
+            const double bot = a.bottomYP[k];
+            if (a.surf[i] < bot - 0.0 && a.surf[k] - bot >= p.wtfh) a.wtfYP[k] = 1;
+        } else if (r >= 2 * g.nx + g.ny && r < 2 * g.nx + 2 * g.ny) {   // iM ghost
+            const int jj = r - 2 * g.nx - g.ny;
+            const double bot = a.bottomXP[k];
+            if (a.surf[jj * g.nx] < bot - 0.0 && a.surf[k] - bot >= p.wtfh) a.wtfXP[k] = 1;
+        }
+        u = face_vel_sub(a, ghost_source(g, k, true)).u;
+        v = face_vel_sub(a, ghost_source(g, k, false)).v;
+    } else {
+        const FaceVel f = face_vel_sub(a, k);
+        a.wtfXP[k] = f.fX;
+        a.wtfYP[k] = f.fY;
+        u = f.u;
+        v = f.v;
+        loss += f.loss;
+    }
+    a.uuXP[k] = u;
+    a.vvYP[k] = v;
+    a.Fuu[k] = u * a.Nx[k];
+    a.Fvv[k] = v * a.Oy[k];
 }
 
 __global__ void __launch_bounds__(256) k_faces_sub(const StepArgs *__restrict__ A)
@@ -1087,29 +1125,6 @@ __global__ void __launch_bounds__(256) k_faces_sub(const StepArgs *__restrict__ 
     if (k < a.g.Ng) faces_sub_cell(a, k, loss[0]);
     block_sum<1>(loss, lds);
     if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blockIdx.x] = loss[0];
-}
-
-// enforceVelocityBC + computeFaceFlowRateSub over Ng (subgrid.c:1190-1212)
-__global__ void __launch_bounds__(256) k_flux_sub(const StepArgs *__restrict__ A)
-{
-    const StepArgs &a = A[blockIdx.z];
-    if (blockIdx.x >= a.nb1) return;
-    if (a.ctl->halted) return;
-    const Grid g = a.g;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= g.Ng) return;
-    double u, v;
-    if (k < g.N) {
-        u = a.uuXP[k];
-        v = a.vvYP[k];
-    } else {
-        u = a.uuXP[ghost_source(g, k, true)];
-        v = a.vvYP[ghost_source(g, k, false)];
-        a.uuXP[k] = u;
-        a.vvYP[k] = v;
-    }
-    a.Fuu[k] = u * a.Nx[k];
-    a.Fvv[k] = v * a.Oy[k];
 }
 
 // adjustTidalVelocitySub (subgrid.c:1291-1332)
