@@ -247,6 +247,17 @@ inline DirTiling dir_tiling(int nx, int ny, int H, int remap, int target = 1024)
 
 inline size_t max_dir_blocks(int nx, int ny) { return (size_t)((nx + 255) / 256) * ny + 8; }
 
+// the group's flag array: pinned, coherent host memory mapped for the
+// device, which the flag gather writes directly
+static hipError_t alloc_flags(Exec &e, size_t members)
+{
+    const size_t bytes = members * F_STRIDE * sizeof(int32_t);
+    hipError_t err = hipHostMalloc((void **)&e.hFlags, bytes, hipHostMallocMapped | hipHostMallocCoherent);
+    if (err != hipSuccess) return err;
+    memset(e.hFlags, 0, bytes);
+    return hipHostGetDevicePointer((void **)&e.hFlagsDev, e.hFlags, 0);
+}
+
 // fine pixels per coarse cell side of the on-the-fly subgrid, 0 when subR
 // is not a whole number
 static int fine_r(const frehd_config &k)
@@ -310,7 +321,6 @@ size_t layout(frehd_dev *d, char *base)
     d->self.dPcg = cv.take<PcgArgs<double>>(1);
     d->self.dPcg32 = cv.take<PcgArgs<float>>(1);
     d->self.dRef = cv.take<RefineArgs>(1);
-    d->self.dFlags = cv.take<int32_t>(F_STRIDE);
     a.vpart = cv.take<double>(4 * maxb);
     a.cflpart = cv.take<double>(2 * maxb);
     a.tideP = cv.take<double>(Nt1);
@@ -500,7 +510,7 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     d->self.device = d->device;
     d->self.stream = d->stream;
     d->self.tim = &d->timing;
-    HIPCHK(hipHostMalloc((void **)&d->self.hFlags, sizeof(int32_t) * F_STRIDE, hipHostMallocDefault));
+    HIPCHK(alloc_flags(d->self, 1));
     d->midStep = 0;
     d->tt = 0;
     // forcing
@@ -674,7 +684,12 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     f.gx = t.gx; f.gy = t.gy; f.H = t.H; f.remap = t.remap;
     f.nbDir = t.nb;
     f.nbRes = p.nbRes;
-    f.nbStart = t.nb;
+    // the refinement walks its own tiles (about 1024: it has the bytes of a
+    // stencil sweep and four partials per tile, re-reduced by one block and
+    // by the inner solve's set-up launch only); the set-up launch reduces
+    // its partials
+    const DirTiling tr = dir_tiling(p.nx, p.ny, 0, remap, 1024);
+    f.nbStart = tr.nb;
     RefineArgs &r = d->ref;
     r.cX = a.cX; r.cY = a.cY; r.D = a.D; r.b = a.b;
     r.xbuf[0] = a.x;
@@ -689,8 +704,8 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     r.epsInner = d->cfg.epsInner;
     r.maxRefine = d->cfg.maxRefine;
     r.nx = p.nx; r.ny = p.ny; r.N = p.N;
-    r.gx = t.gx; r.gy = t.gy; r.H = t.H; r.remap = t.remap;
-    r.nb = t.nb;
+    r.gx = tr.gx; r.gy = tr.gy; r.H = tr.H; r.remap = tr.remap;
+    r.nb = tr.nb;
 }
 
 // Init's subgrid part (initialize.c:103-124): level index at round(Ne/2)
@@ -1122,8 +1137,7 @@ int frehd_batch_create(int32_t capacity, frehd_batch_t **out)
         hipMalloc(&e.dPcg, n * sizeof(PcgArgs<double>)) != hipSuccess ||
         hipMalloc(&e.dPcg32, n * sizeof(PcgArgs<float>)) != hipSuccess ||
         hipMalloc(&e.dRef, n * sizeof(RefineArgs)) != hipSuccess ||
-        hipMalloc(&e.dFlags, n * F_STRIDE * sizeof(int32_t)) != hipSuccess ||
-        hipHostMalloc((void **)&e.hFlags, n * F_STRIDE * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+        alloc_flags(e, n) != hipSuccess) {
         frehd_batch_destroy(b);
         return fail(FREHD_ENOMEM, "batch allocation failed");
     }
@@ -1241,7 +1255,6 @@ int frehd_batch_destroy(frehd_batch_t *b)
     if (e.dPcg) hipFree(e.dPcg);
     if (e.dPcg32) hipFree(e.dPcg32);
     if (e.dRef) hipFree(e.dRef);
-    if (e.dFlags) hipFree(e.dFlags);
     if (e.hFlags) hipHostFree(e.hFlags);
     if (e.stream) hipStreamDestroy(e.stream);
     delete b;

@@ -3,6 +3,7 @@
 // refinement loop, per-kernel timing.  Every launch covers all members of
 // the group (grid z = member); see handle.h.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 
 #include "handle.h"
@@ -27,12 +28,14 @@ Timing::~Timing()
 }
 
 // the solver flags of every member, gathered into one array so that one copy
-// brings them to the host
-__global__ void k_gather_flags(const StepArgs *__restrict__ A, int count, int32_t *out)
+// brings them to the host; a member whose solve has not finished is marked
+// open, so the kernels after the solve stand down until it has
+__global__ void k_gather_flags(const StepArgs *__restrict__ A, int count, int mixed, int32_t *out)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= count) return;
-    const Ctl *c = A[m].ctl;
+    Ctl *c = A[m].ctl;
+    c->open = !c->halted && (mixed ? c->ref.doneAt : c->pcg.doneAt) < 0;
     int32_t *o = out + (size_t)m * F_STRIDE;
     o[F_DONE] = c->pcg.doneAt;
     o[F_HALTED] = c->halted;
@@ -41,8 +44,9 @@ __global__ void k_gather_flags(const StepArgs *__restrict__ A, int count, int32_
     o[F_INDONE] = c->pcgIn.doneAt;
     o[F_INITER] = c->pcgIn.iter;
     o[F_TT] = c->tt;
-    o[F_PAD] = 0;
     for (int k = 0; k < 8; k++) o[F_INNER0 + k] = c->ref.innerIters[k];
+    __threadfence_system();
+    o[F_SEQ] = 1;
 }
 
 namespace {
@@ -94,9 +98,44 @@ void remember(Exec &e, std::vector<int> counts)
 
 int enqueue_gather(Exec &e)
 {
-    LAUNCH(K_GATHER, k_gather_flags, dim3((unsigned)((e.count + 63) / 64)), dim3(64), e.dArgs, e.count, e.dFlags);
-    HIPCHK(hipMemcpyAsync(e.hFlags, e.dFlags, sizeof(int32_t) * F_STRIDE * e.count, hipMemcpyDeviceToHost, e.stream));
+    LAUNCH(K_GATHER, k_gather_flags, dim3((unsigned)((e.count + 63) / 64)), dim3(64), e.dArgs, e.count, e.mixed,
+           e.hFlagsDev);
     return 0;
+}
+
+// wait until the gather of every member has written its flags (the host
+// cleared F_SEQ before the launch), checking the stream for errors now and
+// then; the stream keeps running meanwhile
+int wait_flags(Exec &e)
+{
+    volatile int32_t *f = e.hFlags;
+    for (long spin = 0;; spin++) {
+        bool all = true;
+        for (int m = 0; m < e.count && all; m++)
+            if (f[(size_t)m * F_STRIDE + F_SEQ] == 0) all = false;
+        if (all) break;
+        if ((spin & 1023) == 1023) {
+            const hipError_t q = hipStreamQuery(e.stream);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail(FREHD_EHIP, "step failed: %s", hipGetErrorString(q));
+            if (q == hipSuccess) {
+                // the stream drained: the flags are there unless the gather
+                // did not run, which is an error
+                for (int m = 0; m < e.count; m++)
+                    if (f[(size_t)m * F_STRIDE + F_SEQ] == 0) return fail(FREHD_EHIP, "solver flags missing");
+                break;
+            }
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return 0;
+}
+
+void clear_flags(Exec &e)
+{
+    volatile int32_t *f = e.hFlags;
+    for (int m = 0; m < e.count; m++) f[(size_t)m * F_STRIDE + F_SEQ] = 0;
+    std::atomic_thread_fence(std::memory_order_seq_cst);
 }
 
 int gather(Exec &e)
@@ -437,11 +476,28 @@ static int exec_step_eager(Exec &e)
     return 0;
 }
 
-// One step as two graph launches and one synchronisation: the head
-// (predictor, assembly, the solver's launch pattern of the last step, the
-// flag gather) and, once the flags show every member's solve finished (after
-// eager extra iterations where needed), the tail.  The same kernels with the
-// same arguments as the eager path, so the results are identical.
+// enqueue the kernels after the solve (surface, update, scalars, CFL test)
+static int enqueue_tail(Exec &e)
+{
+    LAUNCH(K_SURF, k_surface, grid1(e.ext.nb1, e.count), dim3(256), e.dArgs);
+    for (int ph = 2; ph <= 4; ph++) {
+        int r = exec_phase(e, ph);
+        if (r) return r;
+    }
+    return 0;
+}
+
+// One step as one graph launch per launch pattern: the predictor, the
+// assembly, the solver's pattern, the flag gather -- which writes the flags
+// straight to host memory -- and the kernels after the solve, which stand
+// down for a member the gather marked open.  The host polls the flags while
+// the rest of the graph runs and, normally, returns, so the next step's
+// graph is queued behind this one and the GPU does not idle between steps.
+// If a solve is still open, the host waits for the graph, finishes the solve
+// with eager iterations (their gather closes it) and launches the tail
+// (captured on its own) for that member; members whose tail ran stand down.
+// The same kernels with the same arguments as the eager path, so the results
+// are identical.
 static int exec_step_graphs(Exec &e)
 {
     const int n = e.count;
@@ -452,11 +508,11 @@ static int exec_step_graphs(Exec &e)
     } else {
         key = {0, std::max(1, e.lastIters)};
     }
-    hipGraphExec_t head = nullptr;
+    hipGraphExec_t step = nullptr;
     for (auto &h : e.heads)
-        if (h.first == key) head = h.second;
+        if (h.first == key) step = h.second;
     int k = 0, rc;
-    if (!head) {
+    if (!step) {
         // a pattern is captured the second time it comes up; the first time
         // the step runs eagerly
         int *count = nullptr;
@@ -476,31 +532,34 @@ static int exec_step_graphs(Exec &e)
             int r = enqueue_assemble(e);
             if (!r) r = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
             if (!r) r = enqueue_gather(e);
+            if (!r) r = enqueue_tail(e);
             return r;
-        }, &head);
-        if (rc) return rc;
-        e.heads.push_back({key, head});
-    } else {
-        k = e.mixed ? (int)e.mixPlan.size() : std::max(1, e.lastIters);
-    }
-    HIPCHK(hipGraphLaunch(head, e.stream));
-    HIPCHK(hipStreamSynchronize(e.stream));
-    rc = e.mixed ? solve_mixed_finish(e, k) : solve_fp64_finish(e, k);
-    if (rc) return rc;
-    if (!e.tail) {
-        rc = capture(e, [&] {
-            LAUNCH(K_SURF, k_surface, grid1(e.ext.nb1, n), dim3(256), e.dArgs);
-            for (int ph = 2; ph <= 4; ph++) {
-                int r = exec_phase(e, ph);
-                if (r) return r;
-            }
-            return 0;
-        }, &e.tail);
+        }, &step);
         if (rc) return rc;
         // the capture ran the host bookkeeping of phase 4 once
         for (auto *m : e.members) m->tt -= 1;
+        e.heads.push_back({key, step});
+    } else {
+        k = e.mixed ? (int)e.mixPlan.size() : std::max(1, e.lastIters);
     }
-    HIPCHK(hipGraphLaunch(e.tail, e.stream));
+    clear_flags(e);
+    HIPCHK(hipGraphLaunch(step, e.stream));
+    if ((rc = wait_flags(e))) return rc;
+    bool open = false;
+    for (int m = 0; m < n; m++)
+        if (!flags(e, m)[F_HALTED] && flags(e, m)[e.mixed ? F_REFDONE : F_DONE] < 0) open = true;
+    // finish the open solves (behind the stood-down tail) and record the
+    // counts for the next pattern
+    rc = e.mixed ? solve_mixed_finish(e, k) : solve_fp64_finish(e, k);
+    if (rc) return rc;
+    if (open) {
+        if (!e.tail) {
+            rc = capture(e, [&] { return enqueue_tail(e); }, &e.tail);
+            if (rc) return rc;
+            for (auto *m : e.members) m->tt -= 1;
+        }
+        HIPCHK(hipGraphLaunch(e.tail, e.stream));
+    }
     for (auto *m : e.members) m->tt += 1;
     return 0;
 }

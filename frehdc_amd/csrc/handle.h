@@ -59,7 +59,9 @@ struct Extents {
 };
 
 // flags gathered from every member's control block after a solver chunk
-enum { F_DONE, F_HALTED, F_ITER, F_REFDONE, F_INDONE, F_INITER, F_TT, F_PAD, F_INNER0, F_STRIDE = 16 };
+// F_SEQ: written last (after a system-scope fence), so a nonzero value tells
+// the host the member's flags are complete
+enum { F_DONE, F_HALTED, F_ITER, F_REFDONE, F_INDONE, F_INITER, F_TT, F_SEQ, F_INNER0, F_STRIDE = 16 };
 
 struct frehd_dev_impl;
 
@@ -71,7 +73,9 @@ struct Exec {
     PcgArgs<double> *dPcg = nullptr;
     PcgArgs<float> *dPcg32 = nullptr;
     RefineArgs *dRef = nullptr;
-    int32_t *dFlags = nullptr, *hFlags = nullptr;   // F_STRIDE ints per member
+    // F_STRIDE ints per member in pinned, coherent host memory that the
+    // flag gather writes directly (hFlagsDev: its device address)
+    int32_t *hFlags = nullptr, *hFlagsDev = nullptr;
     Extents ext{};
     int sub = 0, mixed = 0, nScalarMax = 0, maxIter = 0, maxRefine = 0;
     int lastIters = 16;
@@ -83,10 +87,10 @@ struct Exec {
     std::vector<std::vector<int>> recent;
     Timing *tim = nullptr;
     std::vector<frehd_dev_impl *> members;
-    // captured steps (hipGraph): the step's head (predictor, assembly, the
-    // solver's launch pattern, flag gather) per pattern, and its tail
-    // (surface update, updateData, scalars, CFL test); rebuilt whenever the
-    // members' argument blocks change
+    // captured steps (hipGraph): a whole step per solver launch pattern
+    // (predictor, assembly, the pattern, the flag gather, the kernels after
+    // the solve), and those last kernels alone, for a step whose solve the
+    // pattern left open; rebuilt whenever the members' argument blocks change
     int useGraphs = 1;
     std::vector<std::pair<std::vector<int>, hipGraphExec_t>> heads;
     hipGraphExec_t tail = nullptr;

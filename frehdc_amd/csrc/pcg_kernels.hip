@@ -289,7 +289,8 @@ __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A
     if ((int)blockIdx.x >= a.nb) return;
     __shared__ double lds[16 * 4];
     Ctl *ctl = a.ctl;
-    if (ctl->halted || (ctl->ref.doneAt >= 0 && ctl->ref.doneAt < s)) return;
+    const int halted = ctl->halted, refDone = ctl->ref.doneAt;
+    if (halted || (refDone >= 0 && refDone < s)) return;
     const double *xin = s == 0 ? a.xbuf[0] : a.xbuf[(s + 1) & 1];   // x_0 or x_{s-1}
     double *xout = a.xbuf[s & 1];
     int lb = blockIdx.x;
@@ -471,10 +472,10 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
     if ((int)blockIdx.x >= a.nbDir) return;
     __shared__ double lds[16 * 4];
     PcgScalars *sc = a.sc;
-    if (*a.halted) return;
-    if (a.outerDone && *a.outerDone >= 0) return;
-    const int doneAt = sc->doneAt;
-    if (doneAt >= 0 && doneAt < it) return;
+    // the three stop flags in one round trip (most launches of a pattern
+    // past convergence end here)
+    const int halted = *a.halted, outer = *a.outerDone, doneAt = sc->doneAt;
+    if (halted || outer >= 0 || (doneAt >= 0 && doneAt < it)) return;
 
     const double eps = a.epsDyn ? *a.epsDyn : a.eps;
     double alphaD = 0.0, betaD = 0.0;

@@ -47,6 +47,10 @@ struct Ctl {
     int haltStep;
     int parity;               // depth buffer of the step's start: 0 -> depthA
     int sParity;              // scalar buffer of the step's start: 0 -> S
+    int open;                 // the step's solve has not finished: the kernels
+                              // after it stand down (set by the flag gather)
+    int stepDone;             // the step's kernels after the solve have run
+                              // (set by the step end, cleared at the step's begin)
     double lastMaxCFLx, lastMaxCFLy;
     double Vloss[4];
     PcgScalars pcg;
@@ -178,6 +182,10 @@ __global__ void k_refine_test(const RefineArgs *__restrict__ A, int s);
 
 __device__ inline const double *depth_cur(const StepArgs &a) { return a.ctl->parity ? a.depthB : a.depthA; }
 __device__ inline double *depth_nxt(const StepArgs &a) { return a.ctl->parity ? a.depthA : a.depthB; }
+// the kernels after the solve run only for a member that is not halted,
+// whose solve has finished and whose step they have not completed yet (a
+// group launches them again when one member's solve needed more iterations)
+__device__ inline bool tail_off(const StepArgs &a) { return a.ctl->halted || a.ctl->open || a.ctl->stepDone; }
 __device__ inline const double *s_cur(const StepArgs &a) { return a.ctl->sParity ? a.S2 : a.S; }
 __device__ inline double *s_nxt(const StepArgs &a) { return a.ctl->sParity ? a.S : a.S2; }
 

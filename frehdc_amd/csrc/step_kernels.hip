@@ -22,6 +22,7 @@ __device__ inline void step_begin(const StepArgs &a)
     Ctl *c = a.ctl;
     if (c->halted) return;
     c->tt += 1;
+    c->stepDone = 0;
     c->pcg.iter = 0;
     c->pcg.doneAt = -1;
     c->ref.doneAt = -1;
@@ -249,7 +250,7 @@ __global__ void __launch_bounds__(256) k_surface(const StepArgs *__restrict__ A)
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
-    if (a.ctl->halted) return;
+    if (tail_off(a)) return;
     const Grid g = a.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double loss[1] = {0.0};
@@ -339,7 +340,7 @@ __global__ void __launch_bounds__(256) k_limiter(const StepArgs *__restrict__ A)
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
     __shared__ double lds[16 * 2];
-    if (a.ctl->halted) return;
+    if (tail_off(a)) return;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -436,7 +437,7 @@ __global__ void __launch_bounds__(256) k_faces(const StepArgs *__restrict__ A)
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
-    if (a.ctl->halted) return;
+    if (tail_off(a)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double loss[1] = {0.0};
     if (k < a.g.Ng) faces_cell(a, k, loss[0]);
@@ -541,7 +542,7 @@ __global__ void k_tidal_velocity(const StepArgs *__restrict__ A)
 {
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.nbTide) return;
-    if (a.ctl->halted) return;
+    if (tail_off(a)) return;
     const Grid g = a.g;
     const P &p = a.p;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -573,7 +574,7 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgs *__restrict__ A)
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
-    if (a.ctl->halted) return;
+    if (tail_off(a)) return;
     const Grid g = a.g;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const P &p = a.p;
@@ -638,7 +639,7 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgs *__restrict__ A)
 template <bool SUB>
 __device__ inline void scalar_body(const StepArgs &a, int q)
 {
-    if (a.ctl->halted || q >= a.p.nScalar) return;
+    if (tail_off(a) || q >= a.p.nScalar) return;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -735,7 +736,7 @@ __global__ void __launch_bounds__(1024) k_step_end(const StepArgs *__restrict__ 
     const StepArgs &a = A[blockIdx.z];
     __shared__ double lds[16 * 4];
     Ctl *c = a.ctl;
-    if (c->halted) return;
+    if (tail_off(a)) return;
     // the loads of a thread's partials are issued together (4 per thread
     // per array up to 4096 blocks), then summed in block order
     double v[4] = {0.0, 0.0, 0.0, 0.0};
@@ -777,6 +778,7 @@ __global__ void __launch_bounds__(1024) k_step_end(const StepArgs *__restrict__ 
     c->steps += 1;
     c->parity ^= 1;     // the new depth becomes the next step's start
     if (a.p.nScalar > 0) c->sParity ^= 1;   // and the new concentration
+    c->stepDone = 1;
     if (fabs(mx) > 20 || fabs(my) > 20) {
         c->halted = 1;
         c->haltStep = c->tt;
@@ -893,7 +895,7 @@ __global__ void __launch_bounds__(256) k_limiter_sub(const StepArgs *__restrict_
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
     __shared__ double lds[16 * 2];
-    if (a.ctl->halted) return;
+    if (tail_off(a)) return;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -986,7 +988,7 @@ __global__ void __launch_bounds__(256) k_combine(const StepArgs *__restrict__ A)
 {
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.nbN + a.nbGhost) return;
-    if (a.ctl->halted) return;
+    if (tail_off(a)) return;
     const Grid g = a.g;
     if ((int)blockIdx.x < a.nbN) {
         const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1119,7 +1121,7 @@ __global__ void __launch_bounds__(256) k_faces_sub(const StepArgs *__restrict__ 
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
-    if (a.ctl->halted) return;
+    if (tail_off(a)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double loss[1] = {0.0};
     if (k < a.g.Ng) faces_sub_cell(a, k, loss[0]);
@@ -1132,7 +1134,7 @@ __global__ void k_tidal_velocity_sub(const StepArgs *__restrict__ A)
 {
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.nbTide) return;
-    if (a.ctl->halted) return;
+    if (tail_off(a)) return;
     const Grid g = a.g;
     const P &p = a.p;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1161,7 +1163,7 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgs *__restrict__
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
-    if (a.ctl->halted) return;
+    if (tail_off(a)) return;
     const Grid g = a.g;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const P &p = a.p;
