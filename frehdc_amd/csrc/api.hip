@@ -379,8 +379,8 @@ int check_config(const frehd_config *k)
         return fail(FREHD_EINVAL, "mixed precision needs 0 < epsInner < 1 and maxRefine >= 1");
     if (k->precision == FREHD_PREC_MIXED && (k->warmStart < 0 || k->warmStart > 2))
         return fail(FREHD_EINVAL, "warmStart must be 0, 1 or 2");
-    if (k->useScalar == 1 && k->scalarAdv != 1)
-        return fail(FREHD_EINVAL, "only first-order upwind scalar advection (scalarAdv = 1) is supported");
+    if (k->useScalar == 1 && k->scalarAdv != 1 && k->scalarAdv != 2)
+        return fail(FREHD_EINVAL, "scalarAdv must be 1 (first-order upwind) or 2 (TVD superbee)");
     if (k->bcType < 1 || k->bcType > 3) return fail(FREHD_EINVAL, "bcType must be 1, 2 or 3");
     if (k->tideLocLengthP < 0 || k->tideLocLengthM < 0 || k->inflowLocLength < 0)
         return fail(FREHD_EINVAL, "negative location count");
@@ -432,6 +432,7 @@ void fill_params(P &p, const frehd_config &k)
     p.inflowLocLength = k.inflowLocLength; p.useScalar = k.useScalar;
     p.nScalar = k.useScalar == 1 ? std::max(1, k.nScalar) : 0;
     p.useConstTidePS = k.useConstTidePS; p.useConstTideMS = k.useConstTideMS; p.maxIter = k.maxIter;
+    p.scalarAdv = k.scalarAdv;
     p.useSubgrid = k.useSubgrid; p.dsurf = k.dsurf; p.subR = k.subR; p.Ne = k.Ne; p.surfmin = k.surfmin;
     p.warmStart = k.precision == FREHD_PREC_MIXED ? k.warmStart : 0;
 }

@@ -30,6 +30,13 @@ __host__ __device__ inline int nbE(const Grid &g, int c, int i, int j) { return 
 __host__ __device__ inline int nbW(const Grid &g, int c, int i, int j) { return i == 0 ? giM(g, j) : c - 1; }
 __host__ __device__ inline int nbN(const Grid &g, int c, int i, int j) { return j == g.ny - 1 ? gjP(g, i) : c + g.nx; }
 __host__ __device__ inline int nbS(const Grid &g, int c, int i, int j) { return j == 0 ? gjM(g, i) : c - g.nx; }
+// two cells away (createMaps' iPPjc, iMMjc, icjPP, icjMM, map.c:68-98): the
+// ghost beyond the edge where the second cell falls outside; icjMM of row 1,
+// which map.c:69-74 never sets, is the jM ghost likewise
+__host__ __device__ inline int nbEE(const Grid &g, int c, int i, int j) { return i >= g.nx - 2 ? giP(g, j) : c + 2; }
+__host__ __device__ inline int nbWW(const Grid &g, int c, int i, int j) { return i <= 1 ? giM(g, j) : c - 2; }
+__host__ __device__ inline int nbNN(const Grid &g, int c, int i, int j) { return j >= g.ny - 2 ? gjP(g, i) : c + 2 * g.nx; }
+__host__ __device__ inline int nbSS(const Grid &g, int c, int i, int j) { return j <= 1 ? gjM(g, i) : c - 2 * g.nx; }
 
 // Interior cell whose value a ghost (or corner) index mirrors under the
 // reference's Neumann ghost fills (enforceFreeSurfaceBC nsfunctions.c:317-334,

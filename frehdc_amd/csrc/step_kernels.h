@@ -16,6 +16,7 @@ struct P {
     double eps, tidePS, tideMS, inflowS, Kx, Ky, offset;
     int CDnotN, useThinLayer, bcType, tideLocLengthP, tideLocLengthM, inflowLocLength;
     int useScalar, nScalar, useConstTidePS, useConstTideMS, maxIter, useSubgrid;
+    int scalarAdv;            // 1 first-order upwind, 2 TVD superbee
     // subgrid
     double dsurf, subR, surfmin;   // surfmin: raw elevation of level 0
     int Ne;

@@ -629,13 +629,46 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgs *__restrict__ A)
 }
 
 // ---------------------------------------------------------------------------
-// Scalar transport (scalar.c:30-56, upwind): scalarAdvection +
+// Scalar transport (scalar.c:30-56, upwind or TVD): scalarAdvection +
 // scalarDiffusion + updateScalar's connectivity limiter, inflow mass and
 // concentration, and the out-of-range reset (scalar.c:68-97, 259-416), then
 // enforceScalarBC (scalar.c:419-471).  Neighbours read the step's start
 // buffer, the new concentration goes to the other one: the interior with
 // the tidal Dirichlet values and zero where dry, and the ghosts mirroring a
 // boundary cell (corners untouched) with that cell's value before those.
+// superbee limiter of the TVD scheme (scalar.c:113-117)
+__device__ inline double superbee(double r)
+{
+    double p1, p2, phi;
+    if (2 * r > 1) p1 = 1; else p1 = 2 * r;
+    if (r > 2) p2 = 2; else p2 = r;
+    if (p1 > 0) phi = p1; else phi = 0;
+    if (p2 > phi) phi = p2;
+    return phi;
+}
+
+// face value of the TVD scheme (scalar.c:103-245) for the face between cell
+// L and the next cell H (x or y): F the flow rate at the face, FL the one at
+// the face on L's far side, FH at H's far side; SLL, SHH the cells one
+// further out; Cu the face's Courant number.  Upwind from L when F > 0
+// (first order if the flow at L's far side is not also positive), else from H.
+__device__ inline double tvd_face(double SL, double SH, double SLL, double SHH, double F, double FL, double FH,
+                                  double Cu)
+{
+    const double dSmin = 0.00001;
+    double r;
+    if (F > 0) {
+        if (FL <= 0) return SL;
+        if (fabs(SH - SL) < dSmin) r = 0;
+        else r = (SL - SLL) / (SH - SL);
+        return SL + 0.5 * superbee(r) * (SH - SL) * Cu * (1 - Cu);
+    }
+    if (FH >= 0) return SH;
+    if (fabs(SH - SL) < dSmin) r = 0;
+    else r = (SH - SHH) / (SL - SH);
+    return SH + 0.5 * superbee(r) * (SL - SH) * Cu * (1 - Cu);
+}
+
 template <bool SUB>
 __device__ inline void scalar_body(const StepArgs &a, int q)
 {
@@ -655,10 +688,22 @@ __device__ inline void scalar_body(const StepArgs &a, int q)
     const double Sc = S[c], Se = S[e], Sw = S[w], Sn = S[n], Ss = S[s];
     double sm = Sm[c];
     {
-        const double Sd = Fu[c] > 0 ? Sc : Se;
-        const double Su = Fu[w] > 0 ? Sw : Sc;
-        const double Sr = Fv[c] > 0 ? Sc : Sn;
-        const double Sl = Fv[s] > 0 ? Ss : Sc;
+        double Sd, Su, Sr, Sl;
+        if (p.scalarAdv == 2) {
+            // TVD superbee (scalar.c:99-255)
+            const int ee = nbEE(g, c, i, j), ww = nbWW(g, c, i, j), nn = nbNN(g, c, i, j), ss = nbSS(g, c, i, j);
+            const double *u = a.uuXP, *v = a.vvYP;
+            Sd = tvd_face(Sc, Se, Sw, S[ee], Fu[c], Fu[w], Fu[e], u[c] * dt / dx);
+            Su = tvd_face(Sw, Sc, S[ww], Se, Fu[w], Fu[ww], Fu[c], u[w] * dt / dx);
+            Sr = tvd_face(Sc, Sn, Ss, S[nn], Fv[c], Fv[s], Fv[n], v[c] * dt / dy);
+            // the YM face's Courant number divides by dx (scalar.c:222)
+            Sl = tvd_face(Ss, Sc, S[ss], Sn, Fv[s], Fv[ss], Fv[c], v[s] * dt / dx);
+        } else {
+            Sd = Fu[c] > 0 ? Sc : Se;
+            Su = Fu[w] > 0 ? Sw : Sc;
+            Sr = Fv[c] > 0 ? Sc : Sn;
+            Sl = Fv[s] > 0 ? Ss : Sc;
+        }
         sm = sm + dt * (-Fu[c] * Sd + Fu[w] * Su - Fv[c] * Sr + Fv[s] * Sl);
     }
     if (SUB)   // scalarDiffusionSub, subgrid.c:1552-1567

@@ -80,7 +80,7 @@ typedef struct frehd_config {
     int32_t tideLocLengthM;  /* number of YM tidal cells                          */
     int32_t inflowLocLength; /* number of inflow cells                            */
     int32_t useScalar;       /* scalar transport on/off                           */
-    int32_t scalarAdv;       /* 1: first-order upwind (the only device scheme)    */
+    int32_t scalarAdv;       /* 1: first-order upwind, 2: TVD superbee            */
     int32_t useConstTidePS, useConstTideMS;
     int32_t nScalar;         /* scalars carried (>= 1 when useScalar)             */
     double  tidePS, tideMS;  /* constant tidal scalar values                      */

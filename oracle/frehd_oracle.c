@@ -77,6 +77,13 @@ static inline int nbP(const fo_state *s, int c) { int i = ci(s, c); return i == 
 static inline int nbM(const fo_state *s, int c) { int i = ci(s, c); return i == 0 ? giM(s, cj(s, c)) : c - 1; }
 static inline int nbN(const fo_state *s, int c) { int j = cj(s, c); return j == s->ny - 1 ? gjP(s, ci(s, c)) : c + s->nx; }
 static inline int nbS(const fo_state *s, int c) { int j = cj(s, c); return j == 0 ? gjM(s, ci(s, c)) : c - s->nx; }
+/* two cells away (createMaps' iPPjc, iMMjc, icjPP, icjMM, map.c:68-98): the
+   ghost beyond the edge where the second cell falls outside; icjMM of row 1,
+   which map.c:69-74 never sets, is the jM ghost likewise */
+static inline int nbPP(const fo_state *s, int c) { int i = ci(s, c); return i >= s->nx - 2 ? giP(s, cj(s, c)) : c + 2; }
+static inline int nbMM(const fo_state *s, int c) { int i = ci(s, c); return i <= 1 ? giM(s, cj(s, c)) : c - 2; }
+static inline int nbNN(const fo_state *s, int c) { int j = cj(s, c); return j >= s->ny - 2 ? gjP(s, ci(s, c)) : c + 2 * s->nx; }
+static inline int nbSS(const fo_state *s, int c) { int j = cj(s, c); return j <= 1 ? gjM(s, ci(s, c)) : c - 2 * s->nx; }
 /* corners, map.c:101-104 */
 static inline int cornerIMJP(const fo_state *s) { return s->Ng - 4; }
 static inline int cornerIPJP(const fo_state *s) { return s->Ng - 3; }
@@ -1019,6 +1026,38 @@ static int is_inflow_cell(const fo_state *s, int c)
     return 0;
 }
 
+/* superbee limiter of the TVD scheme (scalar.c:113-117) */
+static double superbee(double r)
+{
+    double p1, p2, phi;
+    if (2 * r > 1) p1 = 1; else p1 = 2 * r;
+    if (r > 2) p2 = 2; else p2 = r;
+    if (p1 > 0) phi = p1; else phi = 0;
+    if (p2 > phi) phi = p2;
+    return phi;
+}
+
+/* face value of the TVD scheme (scalar.c:103-245) for the face between cell
+   L and the next cell H (x or y): F the flow rate at the face, FL the one
+   at the face on L's far side, FH at H's far side; SLL, SHH the cells one
+   further out; Cu the face's Courant number.  Upwind from L when F > 0 (first
+   order if the flow at L's far side is not also positive), else from H. */
+static double tvd_face(double SL, double SH, double SLL, double SHH, double F, double FL, double FH, double Cu)
+{
+    const double dSmin = 0.00001;
+    double r;
+    if (F > 0) {
+        if (FL <= 0) return SL;
+        if (fabs(SH - SL) < dSmin) r = 0;
+        else r = (SL - SLL) / (SH - SL);
+        return SL + 0.5 * superbee(r) * (SH - SL) * Cu * (1 - Cu);
+    }
+    if (FH >= 0) return SH;
+    if (fabs(SH - SL) < dSmin) r = 0;
+    else r = (SH - SHH) / (SL - SH);
+    return SH + 0.5 * superbee(r) * (SL - SH) * Cu * (1 - Cu);
+}
+
 static void scalar_pass(fo_state *s, double *S, double *Sm)
 {
     const frehd_config *k = &s->cfg;
@@ -1026,13 +1065,23 @@ static void scalar_pass(fo_state *s, double *S, double *Sm)
     const double dx = k->dx, dy = k->dy, dt = k->dt;
     const double *Fu = s->Fuu, *Fv = s->Fvv, *ho = s->surfOld, *zb = s->bottomZ;
 
-    /* scalarAdvection, scalar.c:68-97 */
+    /* scalarAdvection, scalar.c:68-97 (upwind) and :99-255 (TVD superbee) */
     for (int c = 0; c < N; c++) {
         int e = nbP(s, c), w = nbM(s, c), n = nbN(s, c), so = nbS(s, c);
-        double Sd = Fu[c] > 0 ? S[c] : S[e];
-        double Su = Fu[w] > 0 ? S[w] : S[c];
-        double Sr = Fv[c] > 0 ? S[c] : S[n];
-        double Sl = Fv[so] > 0 ? S[so] : S[c];
+        double Sd, Su, Sr, Sl;
+        if (k->scalarAdv == 2) {
+            const double *u = s->uuXP, *v = s->vvYP;
+            Sd = tvd_face(S[c], S[e], S[w], S[nbPP(s, c)], Fu[c], Fu[w], Fu[e], u[c] * dt / dx);
+            Su = tvd_face(S[w], S[c], S[nbMM(s, c)], S[e], Fu[w], Fu[nbMM(s, c)], Fu[c], u[w] * dt / dx);
+            Sr = tvd_face(S[c], S[n], S[so], S[nbNN(s, c)], Fv[c], Fv[so], Fv[n], v[c] * dt / dy);
+            /* the YM face's Courant number divides by dx (scalar.c:222) */
+            Sl = tvd_face(S[so], S[c], S[nbSS(s, c)], S[n], Fv[so], Fv[nbSS(s, c)], Fv[c], v[so] * dt / dx);
+        } else {
+            Sd = Fu[c] > 0 ? S[c] : S[e];
+            Su = Fu[w] > 0 ? S[w] : S[c];
+            Sr = Fv[c] > 0 ? S[c] : S[n];
+            Sl = Fv[so] > 0 ? S[so] : S[c];
+        }
         Sm[c] = Sm[c] + dt * (-Fu[c] * Sd + Fu[w] * Su - Fv[c] * Sr + Fv[so] * Sl);
     }
     if (s->sub) {

@@ -15,7 +15,9 @@
  *     checkConservation and inflow_rank are set to 0;
  *   - the grid maps are built here with the same index formulas as
  *     createMaps (map.c:15-128) but with icjMM sized correctly (the reference
- *     writes past its end, map.c:69-74);
+ *     writes past its end, map.c:69-74) and defined on row 1, which those
+ *     loops never set: two cells down there is the jM ghost, as iMMjc is the
+ *     iM ghost at i = 1;
  *   - every Data/Bath/BC array is allocated with zeroed guard cells before and
  *     after it, so the reference's out-of-range reads (velocityInterp's
  *     uuXP[-1] and vvYP[ii-nx]; limiterCFL's depth[ghost]) read 0.0;
@@ -103,7 +105,7 @@ static void build_maps(Maps *m, int nx, int ny)
     for (int c = 0; c < N; c++) {
         int i = c % nx, j = c / nx;
         m->icjPP[c] = j >= ny - 1 ? m->icjP[c] : m->icjP[m->icjP[c]];
-        m->icjMM[c] = j <= 0 ? m->icjM[c] : (j == 1 ? m->icjM[c] : m->icjM[m->icjM[c]]);
+        m->icjMM[c] = j <= 0 ? m->icjM[c] : m->icjM[m->icjM[c]];
         m->iPPjc[c] = i == nx - 1 ? m->iPjc[c] : m->iPjc[m->iPjc[c]];
         m->iMMjc[c] = i == 0 ? m->iMjc[c] : m->iMjc[m->iMjc[c]];
     }

@@ -36,8 +36,16 @@ def fast_marsh(n=48, steps=400):
     return sc
 
 
+def tvd_marsh():
+    """The wet/dry marsh with TVD superbee scalar advection (scalarAdv = 2)."""
+    sc = fast_marsh()
+    sc.cfg["scalarAdv"] = 2
+    sc.name += "_tvd"
+    return sc
+
+
 def cases_small():
-    return [cases.basin(nx=40, ny=36, Nt=200), fast_marsh()]
+    return [cases.basin(nx=40, ny=36, Nt=200), fast_marsh(), tvd_marsh()]
 
 
 def compare(dev_fields, ref_fields, names, tag):

@@ -25,9 +25,16 @@ def oracle_state(o, sc):
     return out
 
 
+def tvd_sub():
+    sc = cases.marsh_subgrid(n=28, r=4, Nt=300, period=1200.0)
+    sc.cfg["scalarAdv"] = 2
+    sc.name += "_tvd"
+    return sc
+
+
 def sub_cases():
     return [cases.marsh_subgrid(n=32, r=4, Nt=300, period=1200.0),
-            cases.marsh_subgrid(n=24, r=8, Nt=300, period=1200.0, manning_cell=False)]
+            cases.marsh_subgrid(n=24, r=8, Nt=300, period=1200.0, manning_cell=False), tvd_sub()]
 
 
 @pytest.fixture(scope="module")

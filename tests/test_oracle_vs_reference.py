@@ -35,6 +35,10 @@ CASES = [
     ("basin_ssor", lambda: cases.basin(nx=30, ny=26, Nt=40), {"precond": 1}),
     ("marsh_jacobi", marsh_fast, {}),
     ("basin_constCD_thin", lambda: cases.basin(nx=28, ny=24, Nt=40), {"CDnotN": 1, "hD": 2.6}),
+    # TVD superbee scalar advection (scalar.c:99-255)
+    ("basin_tvd", lambda: cases.basin(nx=30, ny=26, Nt=40), {"scalarAdv": 2}),
+    ("marsh_tvd", marsh_fast, {"scalarAdv": 2}),
+    ("marsh_subgrid4_tvd", marsh_sub, {"scalarAdv": 2}),
 ]
 
 
