@@ -90,6 +90,9 @@ def cpu_baseline(n, r, budget_s):
                       f"Jacobi PCG eps 1e-10, FP64, uniform Manning n, {dt:.1f} s on 1 host core"}
 
 
+TILING = None     # (H, remap) given on the command line instead of the autotuner
+
+
 def run_form(scen, steps, warmup, tune=True, **over):
     """Time one precision form: 3 steps, the solver-tiling autotuner, warmup
     steps, then `steps` timed steps (the call returns after the stream is
@@ -98,7 +101,11 @@ def run_form(scen, steps, warmup, tune=True, **over):
     from frehdc_amd import device
     dev = device.DeviceScenario(scen, **over)
     dev.step(3)
-    tiling = dev.autotune(20) if tune else None
+    if TILING:
+        dev.set_pcg_tiling(*TILING)
+        tiling = {"rows_per_tile": TILING[0], "xcd_remap": TILING[1]}
+    else:
+        tiling = dev.autotune(20) if tune else None
     dev.step(warmup)
     it0 = dev.stats()["pcgIterTotal"]
     t0 = time.perf_counter()
@@ -196,12 +203,12 @@ def serving_run(count, subgrid):
             "forms": out}
 
 
-def traffic_of(kernels, size, subgrid):
+def traffic_of(kernels, workload):
     """HBM bytes per launch of the kernels, from the committed counter
-    summary of the default workload (profiles/traffic_marsh1024_sub8.json);
-    None for any other workload or when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "traffic_marsh1024_sub8.json")
-    if size != 1024 or subgrid != 8 or not os.path.exists(path):
+    summary of the workload (profiles/traffic_<workload>.json, e.g.
+    marsh1024_sub8, channel4096_sub4); None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
         return None
     with open(path) as f:
         k = json.load(f)["kernels"]
@@ -237,7 +244,8 @@ def large_domain(size, r, steps, warmup):
     for f, over in [("fp64", {}), ("mixed_warm", {"precision": abi.PREC_MIXED, "warmStart": 2})]:
         r_ = run_form(sc, steps, warmup, **over)
         res[f] = r_
-        _, roof = solver_roofline(r_["kt"], sc.N, f != "fp64")
+        ks, roof = solver_roofline(r_["kt"], sc.N, f != "fp64")
+        roof["traffic"] = traffic_of(ks, f"channel{size}_sub{r}")
         out[f] = {"cell_steps_per_s": r_["value"], "ms_per_step": r_["ms_per_step"],
                   "pcg_iterations_per_step": round(r_["iters"], 2), "refinement_sweeps": r_["sweeps"],
                   "rel_residual_last": r_["relres"], "tiling": r_["tiling"], "solver_roofline": roof}
@@ -276,7 +284,12 @@ def main():
     ap.add_argument("--large-size", type=int, default=4096)
     ap.add_argument("--large-steps", type=int, default=10)
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--tiling", default="", help="H:remap solver tiling for the single-domain forms instead of "
+                                                 "the autotuner (profiled runs: no trial launches)")
     args = ap.parse_args()
+    global TILING
+    if args.tiling:
+        TILING = tuple(int(x) for x in args.tiling.split(":"))
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(this build has no multi-GPU path)", file=sys.stderr)
@@ -334,7 +347,7 @@ def main():
     # the set-up and stop-test launches are timed apart)
     dtype = "f64" if head == "fp64" else "f32+f64"
     ks, roof = solver_roofline(hr["kt"], N, head != "fp64")
-    traffic = traffic_of(ks, args.size, args.subgrid)
+    traffic = traffic_of(ks, f"marsh{args.size}_sub{args.subgrid}")
     roof["traffic"] = traffic
     if traffic:
         roof["traffic_GBps"] = round(traffic / (roof["ms_per_iteration"] * 1e-3) / 1e9, 1)

@@ -6,6 +6,8 @@
 #include <atomic>
 #include <cstring>
 
+#include <hip/hip_ext.h>
+
 #include "handle.h"
 
 namespace frehd {
@@ -13,7 +15,8 @@ namespace frehd {
 const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble", "k_pcg_dir",
     "k_pcg_res", "k_surface", "k_limiter", "k_faces", "k_tidal_velocity",
     "k_finish", "k_scalar", "k_step_end", "k_combine",
-    "k_subarea_init", "k_cg1_f32", "k_cg1_f32_setup", "k_cg1_f32_stop", "k_refine", "k_refine_test", "k_gather_flags"};
+    "k_subarea_init", "k_cg1_f32", "k_cg1_f32_setup", "k_cg1_f32_stop", "k_refine", "k_refine_test", "k_gather_flags",
+    "k_pcg_dir_stop"};
 
 void Timing::reset()
 {
@@ -61,6 +64,10 @@ hipEvent_t next_event(Timing &t)
     return t.pool[t.next++];
 }
 
+// With timing on, the kernel is launched with a start and a stop event that
+// the runtime stamps from the dispatch itself (hipExtLaunchKernel), so the
+// elapsed time is the kernel's execution, as a kernel trace reports it --
+// not the gap between two event records around the launch.
 template <class F>
 int launch(Exec &e, KernelId id, F &&fn)
 {
@@ -69,22 +76,21 @@ int launch(Exec &e, KernelId id, F &&fn)
     if (t && t->on) {
         e0 = next_event(*t);
         e1 = next_event(*t);
-        hipEventRecord(e0, e.stream);
     }
-    fn();
+    fn(e0, e1);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return fail(FREHD_EHIP, "launch of %s failed: %s", kKernelNames[id], hipGetErrorString(err));
-    if (t && t->on) {
-        hipEventRecord(e1, e.stream);
-        t->pending.push_back({id, {e0, e1}});
-    }
+    if (e0) t->pending.push_back({id, e.tagKind, e.tagS, e.tagIt, e0, e1});
     return 0;
 }
 
-#define LAUNCH(id, kern, grid, block, ...)                                                            \
-    do {                                                                                             \
-        int rc_ = launch(e, id, [&] { hipLaunchKernelGGL(kern, grid, block, 0, e.stream, __VA_ARGS__); }); \
-        if (rc_) return rc_;                                                                         \
+#define LAUNCH(id, kern, grid, block, ...)                                                             \
+    do {                                                                                              \
+        int rc_ = launch(e, id, [&](hipEvent_t a_, hipEvent_t b_) {                                   \
+            if (a_) hipExtLaunchKernelGGL(kern, grid, block, 0, e.stream, a_, b_, 0, __VA_ARGS__);    \
+            else hipLaunchKernelGGL(kern, grid, block, 0, e.stream, __VA_ARGS__);                     \
+        });                                                                                           \
+        if (rc_) return rc_;                                                                          \
     } while (0)
 
 inline dim3 grid1(int nb, int n) { return dim3((unsigned)std::max(nb, 1), 1, (unsigned)n); }
@@ -148,21 +154,65 @@ int gather(Exec &e)
 
 inline const int32_t *flags(const Exec &e, int m) { return e.hFlags + (size_t)m * F_STRIDE; }
 
+struct Tag {
+    Exec &e;
+    Tag(Exec &ex, int kind, int s, int it) : e(ex) { e.tagKind = kind; e.tagS = s; e.tagIt = it; }
+    ~Tag() { e.tagKind = 0; }
+};
+
 int enqueue_pcg_iteration(Exec &e, int it)
 {
+    Tag tag(e, 1, 0, it);
     LAUNCH(K_PQ, k_pcg_dir<double>, grid1(e.ext.nbDir, e.count), dim3(256), e.dPcg, it);
     LAUNCH(K_UPD, k_pcg_res<double>, grid1(e.ext.nbRes, e.count), dim3(256), e.dPcg, it);
     return 0;
 }
 
-// FP64 PCG.  The step enqueues the previous step's iteration count (one
-// iteration while timing, so no idle launch enters the averages) and gathers
-// the flags; solve_fp64_finish then adds small chunks until every member
-// reports convergence (or a halt).
+// After a solve, the timed solver launches by what they did (see
+// TimedLaunch): FP64 iteration it works below the slowest member's stop
+// iteration D, k_pcg_dir(D) applies the last x update and stops; inner
+// iteration it after sweep s works up to the most iterations any member
+// used there, the next launch only stops; a sweep works up to the slowest
+// member's converged sweep.
+void classify_timed(Exec &e)
+{
+    Timing *t = e.tim;
+    if (!t || !t->on) return;
+    int D = 0, R = 0;
+    for (int m = 0; m < e.count; m++) {
+        if (flags(e, m)[F_HALTED]) continue;
+        D = std::max(D, (int)flags(e, m)[F_DONE]);
+        R = std::max(R, (int)flags(e, m)[F_REFDONE]);
+    }
+    auto inner = [&](int s) {
+        int n = -1;
+        for (int m = 0; m < e.count; m++)
+            if (!flags(e, m)[F_HALTED] && flags(e, m)[F_REFDONE] > s && s < 8)
+                n = std::max(n, (int)flags(e, m)[F_INNER0 + s]);
+        return n;   // -1: no member solved after sweep s
+    };
+    for (auto &r : t->pending) {
+        if (r.kind == 1) {
+            if (r.id == K_PQ) r.id = r.it < D ? K_PQ : (r.it == D ? K_PQ_STOP : K_IDLE);
+            else r.id = r.it < D ? K_UPD : K_IDLE;
+        } else if (r.kind == 2) {
+            const int n = inner(r.s);
+            if (n < 0) r.id = K_IDLE;
+            else if (r.it == 0) r.id = K_CG32_SET;
+            else r.id = r.it <= n ? K_CG32 : (r.it == n + 1 ? K_CG32_STOP : K_IDLE);
+        } else if (r.kind == 3) {
+            if (r.s > R) r.id = K_IDLE;
+        }
+        r.kind = 0;
+    }
+}
+
+// FP64 PCG.  The step enqueues the iteration count that covers the last
+// solves and gathers the flags; solve_fp64_finish then adds small chunks
+// until every member reports convergence (or a halt).
 int solve_fp64_pattern(Exec &e, int *it)
 {
-    const bool timing = e.tim && e.tim->on;
-    const int n = timing ? 1 : std::max(1, e.lastIters);
+    const int n = std::max(1, e.lastIters);
     for (int q = 0; q < n; q++) {
         int rc = enqueue_pcg_iteration(e, q);
         if (rc) return rc;
@@ -173,21 +223,20 @@ int solve_fp64_pattern(Exec &e, int *it)
 
 int solve_fp64_finish(Exec &e, int it)
 {
-    const bool timing = e.tim && e.tim->on;
     for (;;) {
         bool all = true;
         for (int m = 0; m < e.count; m++)
             if (flags(e, m)[F_DONE] < 0 && !flags(e, m)[F_HALTED]) all = false;
         if (all) break;
         if (it > e.maxIter + 8) return fail(FREHD_EHIP, "PCG did not report completion");
-        const int chunk = timing ? 1 : 4;
-        for (int q = 0; q < chunk; q++, it++) {
+        for (int q = 0; q < 4; q++, it++) {
             int rc = enqueue_pcg_iteration(e, it);
             if (rc) return rc;
         }
         int rc = gather(e);
         if (rc) return rc;
     }
+    classify_timed(e);
     int last = 1;
     for (int m = 0; m < e.count; m++)
         if (!flags(e, m)[F_HALTED]) last = std::max(last, (int)flags(e, m)[F_ITER]);
@@ -197,9 +246,11 @@ int solve_fp64_finish(Exec &e, int it)
     return 0;
 }
 
-// launch it of the inner FP32 solve (iteration it; 0 sets up)
-int enqueue_inner_iteration(Exec &e, int it)
+// launch it of the inner FP32 solve after refinement sweep s (iteration
+// it; 0 sets up)
+int enqueue_inner_iteration(Exec &e, int s, int it)
 {
+    Tag tag(e, 2, s, it);
     if (it == 0) LAUNCH(K_CG32_SET, k_cg1<true>, grid1(e.ext.nbDir, e.count), dim3(256), e.dPcg32, it);
     else LAUNCH(K_CG32, k_cg1<false>, grid1(e.ext.nbDir, e.count), dim3(256), e.dPcg32, it);
     return 0;
@@ -207,6 +258,7 @@ int enqueue_inner_iteration(Exec &e, int it)
 
 int enqueue_refine(Exec &e, int s)
 {
+    Tag tag(e, 3, s, 0);
     LAUNCH(K_REFINE, k_refine, grid1(e.ext.nbRef, e.count), dim3(256), e.dRef, s);
     LAUNCH(K_RTEST, k_refine_test, dim3(1, 1, (unsigned)e.count), dim3(256), e.dRef, s);
     return 0;
@@ -215,10 +267,10 @@ int enqueue_refine(Exec &e, int s)
 // one inner solve of n iterations: the set-up launch, n iterations and the
 // launch whose stop test sees the residual of iteration n (launches past a
 // member's convergence exit at once)
-int enqueue_inner_solve(Exec &e, int n)
+int enqueue_inner_solve(Exec &e, int s, int n)
 {
     for (int it = 0; it <= n + 1; it++) {
-        int rc = enqueue_inner_iteration(e, it);
+        int rc = enqueue_inner_iteration(e, s, it);
         if (rc) return rc;
     }
     return 0;
@@ -236,11 +288,8 @@ int solve_mixed_pattern(Exec &e, int *sweeps)
 {
     int rc, s = 0;
     if ((rc = enqueue_refine(e, 0))) return rc;
-    // while timing, every launch goes through the one-at-a-time loop of
-    // solve_mixed_finish, so that the averages hold working launches only
-    const bool timing = e.tim && e.tim->on;
-    for (size_t k = 0; k < e.mixPlan.size() && !timing; k++) {
-        if ((rc = enqueue_inner_solve(e, e.mixPlan[k]))) return rc;
+    for (size_t k = 0; k < e.mixPlan.size(); k++) {
+        if ((rc = enqueue_inner_solve(e, s, e.mixPlan[k]))) return rc;
         if ((rc = enqueue_refine(e, ++s))) return rc;
     }
     *sweeps = s;
@@ -256,29 +305,23 @@ int solve_mixed_finish(Exec &e, int s)
             if (outer_open(m)) return true;
         return false;
     };
-    const bool timing = e.tim && e.tim->on;
     while (any_open()) {
         if (s >= e.maxRefine) return fail(FREHD_EHIP, "refinement did not report completion");
         int it = 0;
         for (;;) {
-            const int chunk = timing ? 1 : 8;
-            for (int q = 0; q < chunk; q++, it++)
-                if ((rc = enqueue_inner_iteration(e, it))) return rc;
-            const size_t rec = timing ? e.tim->pending.size() - 1 : 0;
+            for (int q = 0; q < 8; q++, it++)
+                if ((rc = enqueue_inner_iteration(e, s, it))) return rc;
             if ((rc = gather(e))) return rc;
-            bool all = true, stopped = true;
-            for (int m = 0; m < e.count; m++) {
+            bool all = true;
+            for (int m = 0; m < e.count; m++)
                 if (outer_open(m) && flags(e, m)[F_INDONE] < 0) all = false;
-                if (outer_open(m) && flags(e, m)[F_INDONE] != it - 1) stopped = false;
-            }
-            // a launch that only ran the stop test is timed apart
-            if (timing && it - 1 > 0 && stopped) e.tim->pending[rec].first = K_CG32_STOP;
             if (all) break;
             if (it > e.maxIter + 8) return fail(FREHD_EHIP, "inner PCG did not report completion");
         }
         if ((rc = enqueue_refine(e, ++s))) return rc;
         if ((rc = gather(e))) return rc;
     }
+    classify_timed(e);
     // this solve's counts: per sweep, the most inner iterations any member
     // used, two more where the pattern cut a solve short; the next pattern
     // covers the last steps' counts sweep by sweep
@@ -326,7 +369,7 @@ int exec_time_solver(Exec &e, int iters, double *ms)
         for (int q = 0; q < 4; q++)
             LAUNCH(K_REFINE, k_refine, grid1(e.ext.nbRef, e.count), dim3(256), e.dRef, 0);
         for (int it = 0; it <= iters; it++)
-            if ((rc = enqueue_inner_iteration(e, it))) return rc;
+            if ((rc = enqueue_inner_iteration(e, 0, it))) return rc;
     } else {
         for (int it = 0; it < iters; it++)
             if ((rc = enqueue_pcg_iteration(e, it))) return rc;
@@ -359,10 +402,11 @@ int exec_collect_timing(Exec &e)
         for (int q = 0; q < K_COUNT; q++) t->timers[q] = KernelTimer{kKernelNames[q], 0.0, 0};
     }
     for (auto &pe : t->pending) {
+        if (pe.id == K_IDLE) continue;
         float ms = 0;
-        hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
-        t->timers[pe.first].ms += ms;
-        t->timers[pe.first].launches += 1;
+        hipEventElapsedTime(&ms, pe.e0, pe.e1);
+        t->timers[pe.id].ms += ms;
+        t->timers[pe.id].launches += 1;
     }
     t->pending.clear();
     t->next = 0;

@@ -33,7 +33,8 @@ int fail(int code, const char *fmt, ...);
 
 enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_TIDEV,
                 K_FINISH, K_SCALAR, K_END, K_COMBINE, K_SUBINIT,
-                K_CG32, K_CG32_SET, K_CG32_STOP, K_REFINE, K_RTEST, K_GATHER, K_COUNT };
+                K_CG32, K_CG32_SET, K_CG32_STOP, K_REFINE, K_RTEST, K_GATHER, K_PQ_STOP, K_COUNT,
+                K_IDLE = -1 };
 extern const char *kKernelNames[K_COUNT];
 
 struct KernelTimer {
@@ -42,11 +43,22 @@ struct KernelTimer {
     int64_t launches;
 };
 
+// a timed launch; solver launches carry their place in the solve (kind
+// 1: FP64 iteration it, 2: inner iteration it after refinement sweep s,
+// 3: refinement sweep s), classified once the flags show where each solve
+// stopped: working launches keep their kernel, the one that only ran the
+// stop test moves to its _stop entry, launches past convergence are dropped
+struct TimedLaunch {
+    int id;
+    int kind, s, it;
+    hipEvent_t e0, e1;
+};
+
 // per-kernel event timing on the group's stream
 struct Timing {
     int on = 0;
     std::vector<KernelTimer> timers;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> pool;
     size_t next = 0;
     void reset();
@@ -94,6 +106,8 @@ struct Exec {
     int useGraphs = 1;
     std::vector<std::pair<std::vector<int>, hipGraphExec_t>> heads;
     hipGraphExec_t tail = nullptr;
+    // place in the solve of the launches being enqueued (see TimedLaunch)
+    int tagKind = 0, tagS = 0, tagIt = 0;
     // capture only what repeats: steps since the last refresh, and how often
     // each launch pattern without a graph has come up
     int stableSteps = 0;

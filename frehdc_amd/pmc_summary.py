@@ -3,12 +3,14 @@
 counter reads for a device-to-device copy of known size (frehdc_amd.copy_probe
 run under the same counter).
 
-    python -m frehdc_amd.pmc_summary --fetch <bench FETCH csv> --write <bench WRITE csv>
+    python -m frehdc_amd.pmc_summary --fetch <FETCH csv> [...] --write <WRITE csv> [...]
         --fetch-copy <copy FETCH csv> --write-copy <copy WRITE csv> --copy-mib 1024
         [--json out.json] [--cmd "..."]
 
 Dispatches that move less than 10 % of the kernel's largest dispatch are
 left out of the means (solver launches past convergence exit at once).
+Several --fetch / --write files (one process per precision form, say) are
+pooled.
 """
 from __future__ import annotations
 
@@ -18,11 +20,12 @@ import csv
 import json
 
 
-def per_dispatch(path):
+def per_dispatch(paths):
     out = collections.defaultdict(list)
-    with open(path, newline="") as f:
-        for r in csv.DictReader(f):
-            out[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    for path in ([paths] if isinstance(paths, str) else paths):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                out[r["Kernel_Name"]].append(float(r["Counter_Value"]))
     return out
 
 
@@ -35,6 +38,7 @@ def copy_factor(path, mib):
 
 def short(name):
     n = name.split("(")[0].replace("void ", "").replace("frehd::", "")
+    n = n.replace("k_cg1<false>", "k_cg1_f32").replace("k_cg1<true>", "k_cg1_f32_setup")
     return n.replace("<double>", "").replace("<float>", "_f32")
 
 
@@ -59,8 +63,8 @@ def summarize(fetch, write, ffac, wfac):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
+    ap.add_argument("--fetch", required=True, nargs="+")
+    ap.add_argument("--write", required=True, nargs="+")
     ap.add_argument("--fetch-copy", required=True)
     ap.add_argument("--write-copy", required=True)
     ap.add_argument("--copy-mib", type=float, default=1024)

@@ -3,7 +3,9 @@ kernel, launches, mean/min/max duration, total.  Reads either the rocpd
 SQLite output (``*_results.db``) or the CSV kernel trace
 (``--output-format csv`` -> ``*_kernel_trace.csv``).
 
-    python -m frehdc_amd.rocpd_summary <results.db | kernel_trace.csv> [--cmd "..."] [--active-us T]
+    python -m frehdc_amd.rocpd_summary <results.db | kernel_trace.csv> [more ...] [--cmd "..."] [--active-us T]
+
+Several traces (one process per precision form, say) are pooled.
 
 ``--active-us T`` adds a second table over the launches longer than T us
 only: the PCG's chunked launches include iterations that exit at once after
@@ -50,11 +52,14 @@ def table(d, min_ns=0):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("path")
+    ap.add_argument("paths", nargs="+")
     ap.add_argument("--cmd", default="")
     ap.add_argument("--active-us", type=float, default=0.0)
     a = ap.parse_args()
-    d = durations(a.path)
+    d = collections.defaultdict(list)
+    for p in a.paths:
+        for k, v in durations(p).items():
+            d[k].extend(v)
     if a.cmd:
         print(f"command: {a.cmd}")
     print("all launches")
