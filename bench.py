@@ -90,7 +90,7 @@ def cpu_baseline(n, r, budget_s):
                       f"Jacobi PCG eps 1e-10, FP64, uniform Manning n, {dt:.1f} s on 1 host core"}
 
 
-TILING = None     # (H, remap) given on the command line instead of the autotuner
+TILING = None     # (H, remap, cols per lane) given on the command line instead of the autotuner
 
 
 def run_form(scen, steps, warmup, tune=True, **over):
@@ -102,8 +102,10 @@ def run_form(scen, steps, warmup, tune=True, **over):
     dev = device.DeviceScenario(scen, **over)
     dev.step(3)
     if TILING:
-        dev.set_pcg_tiling(*TILING)
-        tiling = {"rows_per_tile": TILING[0], "xcd_remap": TILING[1]}
+        from frehdc_amd import abi
+        lanes = TILING[2] if over.get("precision") == abi.PREC_MIXED else 1
+        dev.set_pcg_tiling(TILING[0], TILING[1], lanes)
+        tiling = {"rows_per_tile": TILING[0], "xcd_remap": TILING[1], "cols_per_lane": lanes}
     else:
         tiling = dev.autotune(20) if tune else None
     dev.step(warmup)
@@ -284,12 +286,14 @@ def main():
     ap.add_argument("--large-size", type=int, default=4096)
     ap.add_argument("--large-steps", type=int, default=10)
     ap.add_argument("--no-large", action="store_true")
-    ap.add_argument("--tiling", default="", help="H:remap solver tiling for the single-domain forms instead of "
-                                                 "the autotuner (profiled runs: no trial launches)")
+    ap.add_argument("--tiling", default="", help="H:remap[:cols_per_lane] solver tiling for the single-domain "
+                                                 "forms instead of the autotuner (profiled runs: no trial "
+                                                 "launches; cols_per_lane applies to the mixed forms)")
     args = ap.parse_args()
     global TILING
     if args.tiling:
         TILING = tuple(int(x) for x in args.tiling.split(":"))
+        TILING = TILING + (1,) * (3 - len(TILING))
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(this build has no multi-GPU path)", file=sys.stderr)

@@ -157,13 +157,13 @@ def load_library(path: str | None = None):
         "frehd_dev_phase": ([V, C.c_int32], C.c_int),
         "frehd_dev_destroy": ([V], C.c_int),
         "frehd_dev_footprint": ([C.POINTER(FrehdConfig), C.POINTER(C.c_int64)], C.c_int),
-        "frehd_dev_autotune": ([V, C.c_int32, c_int32_p, c_int32_p, C.POINTER(C.c_double)], C.c_int),
-        "frehd_batch_autotune": ([V, C.c_int32, c_int32_p, c_int32_p, C.POINTER(C.c_double)], C.c_int),
+        "frehd_dev_autotune": ([V, C.c_int32, c_int32_p, c_int32_p, c_int32_p, C.POINTER(C.c_double)], C.c_int),
+        "frehd_batch_autotune": ([V, C.c_int32, c_int32_p, c_int32_p, c_int32_p, C.POINTER(C.c_double)], C.c_int),
         "frehd_dev_enable_timing": ([V, C.c_int32], C.c_int),
         "frehd_dev_kernel_times": ([V, C.c_int32, C.POINTER(C.c_char_p), c_double_p,
                                     C.POINTER(C.c_int64), c_int32_p], C.c_int),
         "frehd_abi_sizes": ([C.POINTER(C.c_int64)], C.c_int),
-        "frehd_dev_set_pcg_tiling": ([V, C.c_int32, C.c_int32], C.c_int),
+        "frehd_dev_set_pcg_tiling": ([V, C.c_int32, C.c_int32, C.c_int32], C.c_int),
         "frehd_dev_download_tables": ([V, C.POINTER(c_double_p)], C.c_int),
         "frehd_batch_create": ([C.c_int32, C.POINTER(V)], C.c_int),
         "frehd_batch_add": ([V, V], C.c_int),

@@ -222,20 +222,21 @@ struct Carve {
 // enough that every block can re-reduce all block partials cheaply
 inline size_t pcg_blocks(size_t N) { return std::min<size_t>(1024, (N + 255) / 256); }
 
-// Solver tiling: tiles of 256 columns x H rows.  The default H gives about
-// `target` tiles: 1024 for the FP64 PCG (16 waves per CU on 256 CUs; one
-// partial per tile for k_pcg_res to re-reduce), 512 for the mixed path,
-// whose k_cg1 writes four partials per tile and re-reduces all of them in
-// every tile of the next launch, a cost that grows with the square of the
-// tile count.  The XCD remap pads the tile count to a multiple of 8.
+// Solver tiling: tiles of 256 W columns x H rows (W adjacent columns per
+// lane, the inner k_cg1 only).  The default H gives about `target` tiles:
+// 1024 for the FP64 PCG (16 waves per CU on 256 CUs; one partial per tile
+// for k_pcg_res to re-reduce), 512 for the mixed path, whose k_cg1 writes
+// four partials per tile and re-reduces all of them in every tile of the
+// next launch, a cost that grows with the square of the tile count.  The
+// XCD remap pads the tile count to a multiple of 8.
 struct DirTiling {
     int gx, gy, H, remap, nb;
 };
 
-inline DirTiling dir_tiling(int nx, int ny, int H, int remap, int target = 1024)
+inline DirTiling dir_tiling(int nx, int ny, int H, int remap, int target = 1024, int W = 1)
 {
     DirTiling t;
-    t.gx = (nx + 255) / 256;
+    t.gx = (nx + 256 * W - 1) / (256 * W);
     if (H <= 0) H = std::max(1, (int)(((int64_t)t.gx * ny + target - 1) / target));
     t.H = H;
     t.gy = (ny + H - 1) / H;
@@ -452,7 +453,7 @@ int upload_series(frehd_dev *d, const double *src, const double *dst)
     return 0;
 }
 
-void set_pcg_args(frehd_dev *d, int H, int remap);
+void set_pcg_args(frehd_dev *d, int H, int remap, int lanes = 1);
 
 // this handle's launch extents (see StepArgs)
 void fill_extents(frehd_dev *d)
@@ -635,8 +636,11 @@ int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
     return rc;
 }
 
-void set_pcg_args(frehd_dev *d, int H, int remap)
+void set_pcg_args(frehd_dev *d, int H, int remap, int lanes)
 {
+    d->tileH = H;
+    d->tileRemap = remap;
+    d->tileLanes = lanes;
     const StepArgs &a = d->args;
     PcgArgs<double> &p = d->pcg;
     p.cX = a.cX;
@@ -661,6 +665,7 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     p.gy = t.gy;
     p.H = t.H;
     p.remap = t.remap;
+    p.lanes = 1;
     p.nbDir = t.nb;
     p.nbRes = (int)pcg_blocks(p.N);
     p.nbStart = p.nbRes;
@@ -682,8 +687,10 @@ void set_pcg_args(frehd_dev *d, int H, int remap)
     f.eps = d->cfg.epsInner;
     f.maxIter = p.maxIter;
     f.nx = p.nx; f.ny = p.ny; f.N = p.N;
-    f.gx = t.gx; f.gy = t.gy; f.H = t.H; f.remap = t.remap;
-    f.nbDir = t.nb;
+    const DirTiling tc = dir_tiling(p.nx, p.ny, H, remap, 512, lanes);
+    f.gx = tc.gx; f.gy = tc.gy; f.H = tc.H; f.remap = tc.remap;
+    f.lanes = lanes;
+    f.nbDir = tc.nb;
     f.nbRes = p.nbRes;
     // the refinement walks its own tiles (about 1024: it has the bytes of a
     // stencil sweep and four partials per tile, re-reduced by one block and
@@ -721,12 +728,14 @@ int init_subgrid_device(frehd_dev *d)
 
 // Solver tiling of a group chosen by timing: every candidate (rows per tile
 // 1, 2, 4, ..., up to the smallest member height, and the default; XCD remap
-// off and on) runs `iters` solver iterations on the members' current
-// systems with a zero tolerance, and the fastest is applied to every member.
+// off and on; for the mixed path's inner solve also 2 and 4 columns per
+// lane where every member's nx allows it) runs `iters` solver iterations on
+// the members' current systems with a zero tolerance, and the fastest is
+// applied to every member.
 // Control blocks (statistics, step counter, flags) are restored, and the
 // solver's work arrays are rewritten at the next step's assembly, so the
 // model state is untouched.
-int autotune_group(Exec &e, int iters, int32_t *bestH, int32_t *bestRemap, double *bestMs)
+int autotune_group(Exec &e, int iters, int32_t *bestH, int32_t *bestRemap, int32_t *bestLanes, double *bestMs)
 {
     if (e.count == 0) return fail(FREHD_ESTATE, "nothing to tune");
     if (iters < 1 || iters > 1000) return fail(FREHD_EINVAL, "iters %d outside [1, 1000]", iters);
@@ -740,8 +749,16 @@ int autotune_group(Exec &e, int iters, int32_t *bestH, int32_t *bestRemap, doubl
     }
     std::vector<int> Hs = {0};
     for (int H = 1; H <= nyMin && H <= 256; H *= 2) Hs.push_back(H);
-    int rc = 0, bH = 0, bR = 0;
+    std::vector<int> Ws = {1};
+    for (int W : {2, 4}) {
+        bool ok = e.mixed != 0;
+        for (auto *d : e.members)
+            if (d->g.nx % W) ok = false;
+        if (ok) Ws.push_back(W);
+    }
+    int rc = 0, bH = 0, bR = 0, bW = 1;
     double best = -1.0;
+    for (int W : Ws) {
     for (int H : Hs) {
         for (int remap = 0; remap < 2 && !rc; remap++) {
             bool ok = true;
@@ -749,7 +766,7 @@ int autotune_group(Exec &e, int iters, int32_t *bestH, int32_t *bestRemap, doubl
                 if ((size_t)dir_tiling(d->g.nx, d->g.ny, H, remap).nb > max_dir_blocks(d->g.nx, d->g.ny)) ok = false;
             if (!ok) continue;
             for (auto *d : e.members) {
-                set_pcg_args(static_cast<frehd_dev *>(d), H, remap);
+                set_pcg_args(static_cast<frehd_dev *>(d), H, remap, W);
                 d->pcg.eps = 0.0;                     // the trial iterations must all run
                 d->pcg.maxIter = 1 << 30;
                 d->pcg32.maxIter = 1 << 30;
@@ -770,19 +787,23 @@ int autotune_group(Exec &e, int iters, int32_t *bestH, int32_t *bestRemap, doubl
                 best = ms;
                 bH = H;
                 bR = remap;
+                bW = W;
             }
         }
         if (rc) break;
     }
+        if (rc) break;
+    }
     // the winner (or, after an error, the default), then the saved control blocks
-    for (auto *d : e.members) set_pcg_args(static_cast<frehd_dev *>(d), rc ? 0 : bH, rc ? 0 : bR);
+    for (auto *d : e.members) set_pcg_args(static_cast<frehd_dev *>(d), rc ? 0 : bH, rc ? 0 : bR, rc ? 1 : bW);
     int rc2 = exec_refresh(e);
     for (int m = 0; m < e.count && !rc2; m++)
         rc2 = copy_wait(e.stream, e.members[m]->args.ctl, &saved[m], sizeof(Ctl), hipMemcpyHostToDevice);
     if (rc) return rc;
     if (rc2) return rc2;
-    if (bestH) *bestH = e.members[0]->pcg.H;
+    if (bestH) *bestH = e.mixed ? e.members[0]->pcg32.H : e.members[0]->pcg.H;
     if (bestRemap) *bestRemap = e.members[0]->pcg.remap;
+    if (bestLanes) *bestLanes = e.mixed ? e.members[0]->pcg32.lanes : 1;
     if (bestMs) *bestMs = best;
     return 0;
 }
@@ -1100,25 +1121,30 @@ int frehd_dev_download_tables(frehd_dev_t *d, double *const tables[10])
     return rc;
 }
 
-int frehd_dev_set_pcg_tiling(frehd_dev_t *d, int32_t rows_per_tile, int32_t xcd_remap)
+int frehd_dev_set_pcg_tiling(frehd_dev_t *d, int32_t rows_per_tile, int32_t xcd_remap, int32_t cols_per_lane)
 {
     if (!d) return fail(FREHD_EINVAL, "handle is NULL");
     if (rows_per_tile < 0 || rows_per_tile > d->g.ny)
         return fail(FREHD_EINVAL, "rows_per_tile %d out of range [0, %d]", rows_per_tile, d->g.ny);
+    if (cols_per_lane != 1 && cols_per_lane != 2 && cols_per_lane != 4)
+        return fail(FREHD_EINVAL, "cols_per_lane must be 1, 2 or 4");
+    if (cols_per_lane > 1 && (d->cfg.precision != FREHD_PREC_MIXED || d->g.nx % cols_per_lane))
+        return fail(FREHD_EINVAL, "cols_per_lane %d needs the mixed precision path and nx a multiple of it",
+                    cols_per_lane);
     if ((size_t)dir_tiling(d->g.nx, d->g.ny, rows_per_tile, xcd_remap).nb > max_dir_blocks(d->g.nx, d->g.ny))
         return fail(FREHD_EINVAL, "too many tiles");
     HIPCHK(hipSetDevice(d->device));
     HIPCHK(hipStreamSynchronize(d->stream));
-    set_pcg_args(d, rows_per_tile, xcd_remap ? 1 : 0);
+    set_pcg_args(d, rows_per_tile, xcd_remap ? 1 : 0, cols_per_lane);
     return upload_member_args(d);
 }
 
 int frehd_dev_autotune(frehd_dev_t *d, int32_t iters, int32_t *rows_per_tile, int32_t *xcd_remap,
-                       double *ms_per_iteration)
+                       int32_t *cols_per_lane, double *ms_per_iteration)
 {
     if (!d) return fail(FREHD_EINVAL, "handle is NULL");
     if (d->batch) return fail(FREHD_ESTATE, "the handle is in a batch: tune the batch");
-    return autotune_group(d->self, iters, rows_per_tile, xcd_remap, ms_per_iteration);
+    return autotune_group(d->self, iters, rows_per_tile, xcd_remap, cols_per_lane, ms_per_iteration);
 }
 
 // ---- batches ----------------------------------------------------------------
@@ -1155,6 +1181,30 @@ int frehd_batch_add(frehd_batch_t *b, frehd_dev_t *d)
     if ((int)b->e.members.size() >= b->e.capacity) return fail(FREHD_EBUDGET, "batch full (%d)", b->e.capacity);
     HIPCHK(hipSetDevice(d->device));
     HIPCHK(hipStreamSynchronize(d->stream));
+    // the inner solve's launches are shared: the newcomer takes the
+    // members' columns per lane, or everyone goes back to one if its nx
+    // does not allow that
+    if (!b->e.members.empty() && d->cfg.precision == FREHD_PREC_MIXED) {
+        const int L = b->e.members[0]->pcg32.lanes;
+        if (d->pcg32.lanes != L) {
+            int rc = 0;
+            if (d->g.nx % L == 0) {
+                set_pcg_args(d, d->tileH, d->tileRemap, L);
+                rc = upload_member_args(d);
+            } else {
+                for (auto *m : b->e.members) {
+                    frehd_dev *md = static_cast<frehd_dev *>(m);
+                    set_pcg_args(md, md->tileH, md->tileRemap, 1);
+                    if (!rc) rc = upload_member_args(md);
+                }
+                if (!rc && d->pcg32.lanes != 1) {
+                    set_pcg_args(d, d->tileH, d->tileRemap, 1);
+                    rc = upload_member_args(d);
+                }
+            }
+            if (rc) return rc;
+        }
+    }
     b->e.members.push_back(d);
     int rc = exec_refresh(b->e);
     if (rc) {
@@ -1219,10 +1269,10 @@ int frehd_batch_step(frehd_batch_t *b, int32_t nsteps)
 }
 
 int frehd_batch_autotune(frehd_batch_t *b, int32_t iters, int32_t *rows_per_tile, int32_t *xcd_remap,
-                         double *ms_per_iteration)
+                         int32_t *cols_per_lane, double *ms_per_iteration)
 {
     if (!b) return fail(FREHD_EINVAL, "batch is NULL");
-    int rc = autotune_group(b->e, iters, rows_per_tile, xcd_remap, ms_per_iteration);
+    int rc = autotune_group(b->e, iters, rows_per_tile, xcd_remap, cols_per_lane, ms_per_iteration);
     // the members' own groups carry the new tiling too
     for (auto *d : b->e.members)
         if (!rc) rc = upload_member_args(d);

@@ -251,8 +251,16 @@ int solve_fp64_finish(Exec &e, int it)
 int enqueue_inner_iteration(Exec &e, int s, int it)
 {
     Tag tag(e, 2, s, it);
-    if (it == 0) LAUNCH(K_CG32_SET, k_cg1<true>, grid1(e.ext.nbDir, e.count), dim3(256), e.dPcg32, it);
-    else LAUNCH(K_CG32, k_cg1<false>, grid1(e.ext.nbDir, e.count), dim3(256), e.dPcg32, it);
+    const dim3 g = grid1(e.ext.nbDir, e.count), b(256);
+    switch ((it == 0 ? 8 : 0) + e.ext.lanes) {
+    case 9: LAUNCH(K_CG32_SET, (k_cg1<true, 1>), g, b, e.dPcg32, it); break;
+    case 10: LAUNCH(K_CG32_SET, (k_cg1<true, 2>), g, b, e.dPcg32, it); break;
+    case 12: LAUNCH(K_CG32_SET, (k_cg1<true, 4>), g, b, e.dPcg32, it); break;
+    case 1: LAUNCH(K_CG32, (k_cg1<false, 1>), g, b, e.dPcg32, it); break;
+    case 2: LAUNCH(K_CG32, (k_cg1<false, 2>), g, b, e.dPcg32, it); break;
+    case 4: LAUNCH(K_CG32, (k_cg1<false, 4>), g, b, e.dPcg32, it); break;
+    default: return fail(FREHD_EINVAL, "columns per lane %d", e.ext.lanes);
+    }
     return 0;
 }
 
@@ -638,8 +646,10 @@ Extents member_extents(const frehd_dev_impl *d)
     x.nbGhost = a.nbGhost;
     x.nbTide = a.nbTide;
     x.nbRes = a.nbRes;
-    x.nbDir = d->pcg.nbDir;
-    x.nbRef = d->cfg.precision == FREHD_PREC_MIXED ? d->ref.nb : 0;
+    const bool mixed = d->cfg.precision == FREHD_PREC_MIXED;
+    x.nbDir = mixed ? d->pcg32.nbDir : d->pcg.nbDir;
+    x.nbRef = mixed ? d->ref.nb : 0;
+    x.lanes = mixed ? d->pcg32.lanes : 1;
     return x;
 }
 
@@ -681,6 +691,8 @@ int exec_refresh(Exec &e)
         } else if ((d->cfg.useSubgrid == 1) != (bool)e.sub || (d->cfg.precision == FREHD_PREC_MIXED) != (bool)e.mixed) {
             return fail(FREHD_EINVAL, "a group's members must share useSubgrid and precision");
         }
+        if (m > 0 && member_extents(d).lanes != e.ext.lanes)
+            return fail(FREHD_EINVAL, "a group's members must share the inner solve's columns per lane");
         merge(e, d, m == 0);
         HIPCHK(hipMemcpyAsync(e.dArgs + m, &d->args, sizeof(StepArgs), hipMemcpyHostToDevice, e.stream));
         HIPCHK(hipMemcpyAsync(e.dPcg + m, &d->pcg, sizeof(PcgArgs<double>), hipMemcpyHostToDevice, e.stream));

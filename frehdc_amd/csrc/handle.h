@@ -68,6 +68,7 @@ struct Timing {
 // launch extents of a group: the maximum over its members
 struct Extents {
     int gx2, gy2, nb1, nbN, nbGhost, nbTide, nbRes, nbDir, nbRef;
+    int lanes;      // columns per lane of the inner k_cg1 (shared by the group)
 };
 
 // flags gathered from every member's control block after a solver chunk
@@ -130,6 +131,7 @@ struct frehd_dev_impl {
     PcgArgs<double> pcg;                // FP64 solve
     PcgArgs<float> pcg32;               // mixed precision: inner FP32 solve
     RefineArgs ref;                     // mixed precision: FP64 refinement
+    int tileH, tileRemap, tileLanes;    // solver tiling as requested (tileH 0: automatic)
     int midStep;                        // last phase run of an unfinished step (0: at a step boundary)
     int tt;                             // host view of the step counter
     Exec self;                          // the handle as a group of one

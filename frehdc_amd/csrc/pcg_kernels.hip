@@ -433,18 +433,52 @@ __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A
 // r.r (r_0 at it = 0, r_{it-1} from it = 2 on); a stopped solve's x is
 // final, and so is the x of a solve the launch pattern cuts short.
 //
-// Tile walk as in k_pcg_dir (loads two rows ahead): a wave owns 64 columns
-// and slides down the rows, ingesting row t (q_{it-1}, r_it, p_it of row t) and finishing row
-// t-1 (q_it, the stores, the partials).  A' p_{it-1} at a neighbour needs
+// Tile walk as in k_pcg_dir (loads two rows ahead), W adjacent columns
+// per lane (W = 1, 2, 4: one float, float2 or float4 access per array and
+// row; a wave owns 64 W columns, a block 256 W) and slides down the rows,
+// ingesting row t (q_{it-1}, r_it, p_it of row t) and finishing row t-1
+// (q_it, the stores, the partials).  A' p_{it-1} at a neighbour needs
 // p_{it-1} two cells away: lanes 0 and 63 also ingest the column beyond
 // the wave (v) from the column after it (o), with the same expressions the
-// owner of v uses, so both obtain bitwise the same p_it there.
+// owner of v uses, so both obtain bitwise the same p_it there.  Every cell's
+// values do not depend on W; the partial sums' order does.
 namespace {
 
+template <int W>
 struct Cg1Row {
-    float P2, R, cX, cY, X;                 // own column: p(t+1), r(t), cXs(t), cYs(t), x(t-1)
+    float P2[W], R[W], cX[W], cY[W], X[W];  // own columns: p(t+1), r(t), cXs(t), cYs(t), x(t-1)
     float hP2, hR, hcXv, hcY, hPo, hcXo;    // lanes 0 / 63: column v and its outer neighbour o
 };
+
+template <int W> struct VecOf;
+template <> struct VecOf<1> { typedef float T; };
+template <> struct VecOf<2> { typedef float2 T; };
+template <> struct VecOf<4> { typedef float4 T; };
+
+__device__ inline void unpack(float v, float (&o)[1]) { o[0] = v; }
+__device__ inline void unpack(float2 v, float (&o)[2]) { o[0] = v.x; o[1] = v.y; }
+__device__ inline void unpack(float4 v, float (&o)[4]) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
+__device__ inline float pack(const float (&o)[1]) { return o[0]; }
+__device__ inline float2 pack(const float (&o)[2]) { return make_float2(o[0], o[1]); }
+__device__ inline float4 pack(const float (&o)[4]) { return make_float4(o[0], o[1], o[2], o[3]); }
+
+// row `row`, columns i .. i+W-1 (i a multiple of W, nx too), or zeros
+template <int W>
+__device__ inline void load_cols(const float *p, int row, int i, int nx, int ny, bool ok, float (&o)[W])
+{
+    if (ok && row >= 0 && row < ny) {
+        unpack(*reinterpret_cast<const typename VecOf<W>::T *>(p + (size_t)row * nx + i), o);
+    } else {
+#pragma unroll
+        for (int k = 0; k < W; k++) o[k] = 0.0f;
+    }
+}
+
+template <int W>
+__device__ inline void store_cols(float *p, size_t c, const float (&v)[W])
+{
+    *reinterpret_cast<typename VecOf<W>::T *>(p + c) = pack(v);
+}
 
 // A' p at one cell, the terms in the owner's order (W, E, S, N)
 __device__ inline float apply_scaled(float pc, float cW, float pW, float cE, float pE, float cS, float pS, float cN,
@@ -464,7 +498,7 @@ __device__ inline void advance(float R, float q, float P, float al, float be, fl
 
 }  // namespace
 
-template <bool First>
+template <bool First, int W>
 __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ A, int it)
 {
     it = First ? 0 : it;
@@ -525,9 +559,9 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
     if (by < a.gy && (!stop || first)) {
         const int nx = a.nx, ny = a.ny;
         const int lane = threadIdx.x & 63;
-        const int i0 = bx * 256 + (threadIdx.x & ~63);
-        const int i = i0 + lane;
-        const bool valid = i < nx;
+        const int i0 = bx * 256 * W + (threadIdx.x >> 6) * 64 * W;   // the wave's first column
+        const int i = i0 + lane * W;
+        const bool valid = i < nx;                                   // nx is a multiple of W
         const int j0 = by * a.H, j1 = min(j0 + a.H, ny);
         // buffers: r and p of iteration it-1 (p_0 = r_0) and of it
         float *const rBuf[2] = {a.r, a.q};
@@ -538,56 +572,67 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
         float *X = a.x;
         if (stop) {
             // stopped at it = 0: the correction is zero
+            float z[W];
+#pragma unroll
+            for (int k = 0; k < W; k++) z[k] = 0.0f;
             if (valid)
-                for (int j = j0; j < j1; j++) X[j * nx + i] = 0.0f;
+                for (int j = j0; j < j1; j++) store_cols<W>(X, (size_t)j * nx + i, z);
         } else {
             const float *cXs = a.cX, *cYs = a.cY;
             // halo lanes: v = the column beyond the wave, o = the one after it
-            const int v = lane == 0 ? i0 - 1 : (lane == 63 ? i0 + 64 : -1);
+            const int v = lane == 0 ? i0 - 1 : (lane == 63 ? i0 + 64 * W : -1);
             const bool halo = v >= 0 && v < nx;
             const int o = lane == 0 ? v - 1 : v + 1;
             const bool oin = halo && o >= 0 && o < nx;
             auto at = [&](const float *p, int row, int col, bool ok) -> float {
-                return (ok && row >= 0 && row < ny) ? p[row * nx + col] : 0.0f;
+                return (ok && row >= 0 && row < ny) ? p[(size_t)row * nx + col] : 0.0f;
             };
-            auto load = [&](int t) {
-                Cg1Row L;
-                L.P2 = at(Pin, t + 1, i, valid);
-                L.R = at(Rin, t, i, valid);
-                L.cX = at(cXs, t, i, valid);
-                L.cY = at(cYs, t, i, valid);
-                L.X = (!first && t - 1 >= j0) ? at(X, t - 1, i, valid) : 0.0f;
+            auto load = [&](int t, Cg1Row<W> &L) {
+                load_cols<W>(Pin, t + 1, i, nx, ny, valid, L.P2);
+                load_cols<W>(Rin, t, i, nx, ny, valid, L.R);
+                load_cols<W>(cXs, t, i, nx, ny, valid, L.cX);
+                load_cols<W>(cYs, t, i, nx, ny, valid, L.cY);
+                load_cols<W>(X, t - 1, i, nx, ny, valid && !first && t - 1 >= j0, L.X);
                 L.hP2 = at(Pin, t + 1, v, halo);
                 L.hR = at(Rin, t, v, halo);
                 L.hcXv = at(cXs, t, v, halo);
                 L.hcY = at(cYs, t, v, halo);
                 L.hPo = at(Pin, t, o, oin);
                 L.hcXo = lane == 0 ? at(cXs, t, o, oin) : 0.0f;
-                return L;
             };
             // window at the start: rows j0-2 and j0-1 of p, the south face of row j0-1
-            float P0 = at(Pin, j0 - 2, i, valid), P1 = at(Pin, j0 - 1, i, valid);
-            float cYm = at(cYs, j0 - 2, i, valid), cYmm = 0.0f, cXc = 0.0f;
-            float pnm = 0.0f, pnc = 0.0f, rnc = 0.0f;
+            float P0[W], P1[W], cYm[W], cYmm[W], cXc[W], pnm[W], pnc[W], rnc[W];
+            load_cols<W>(Pin, j0 - 2, i, nx, ny, valid, P0);
+            load_cols<W>(Pin, j0 - 1, i, nx, ny, valid, P1);
+            load_cols<W>(cYs, j0 - 2, i, nx, ny, valid, cYm);
+#pragma unroll
+            for (int k = 0; k < W; k++) cYmm[k] = cXc[k] = pnm[k] = pnc[k] = rnc[k] = 0.0f;
             float hP0 = at(Pin, j0 - 2, v, halo), hP1 = at(Pin, j0 - 1, v, halo);
             float hcYm = at(cYs, j0 - 2, v, halo), hpn = 0.0f, hcVIprev = 0.0f;
             // loads run two rows ahead of the arithmetic
-            const Cg1Row zero{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            Cg1Row cur = load(j0 - 1), nxt = load(j0);
+            Cg1Row<W> cur, nxt, nxt2;
+            load(j0 - 1, cur);
+            load(j0, nxt);
             for (int t = j0 - 1; t <= j1; t++) {
-                const Cg1Row nxt2 = t + 2 <= j1 ? load(t + 2) : zero;
-                // own column: q_{it-1}, r_it, p_it of row t
-                float pW = __shfl_up(P1, 1, 64), pE = __shfl_down(P1, 1, 64);
-                float cW = __shfl_up(cur.cX, 1, 64);
-                if (lane == 0) { pW = hP1; cW = cur.hcXv; }
-                if (lane == 63) pE = hP1;
-                float rn, pn;
-                if (first) {
-                    rn = cur.R;
-                    pn = cur.R;
-                } else {
-                    const float q = apply_scaled(P1, cW, pW, cur.cX, pE, cYm, P0, cur.cY, cur.P2);
-                    advance(cur.R, q, P1, al, be, rn, pn);
+                if (t + 2 <= j1) load(t + 2, nxt2);
+                // own columns: q_{it-1}, r_it, p_it of row t
+                float pWl = __shfl_up(P1[W - 1], 1, 64), pEl = __shfl_down(P1[0], 1, 64);
+                float cWl = __shfl_up(cur.cX[W - 1], 1, 64);
+                if (lane == 0) { pWl = hP1; cWl = cur.hcXv; }
+                if (lane == 63) pEl = hP1;
+                float rn[W], pn[W];
+#pragma unroll
+                for (int k = 0; k < W; k++) {
+                    if (first) {
+                        rn[k] = cur.R[k];
+                        pn[k] = cur.R[k];
+                    } else {
+                        const float pW = k == 0 ? pWl : P1[k - 1];
+                        const float pE = k == W - 1 ? pEl : P1[k + 1];
+                        const float cW = k == 0 ? cWl : cur.cX[k - 1];
+                        const float q = apply_scaled(P1[k], cW, pW, cur.cX[k], pE, cYm[k], P0[k], cur.cY[k], cur.P2[k]);
+                        advance(cur.R[k], q, P1[k], al, be, rn[k], pn[k]);
+                    }
                 }
                 // halo column v of row t, the owner's expressions
                 float hpnT = 0.0f;
@@ -596,38 +641,48 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
                         hpnT = cur.hR;
                     } else {
                         const float hq = lane == 0
-                            ? apply_scaled(hP1, cur.hcXo, cur.hPo, cur.hcXv, P1, hcYm, hP0, cur.hcY, cur.hP2)
-                            : apply_scaled(hP1, cur.cX, P1, cur.hcXv, cur.hPo, hcYm, hP0, cur.hcY, cur.hP2);
+                            ? apply_scaled(hP1, cur.hcXo, cur.hPo, cur.hcXv, P1[0], hcYm, hP0, cur.hcY, cur.hP2)
+                            : apply_scaled(hP1, cur.cX[W - 1], P1[W - 1], cur.hcXv, cur.hPo, hcYm, hP0, cur.hcY,
+                                           cur.hP2);
                         float hr;
                         advance(cur.hR, hq, hP1, al, be, hr, hpnT);
                     }
                 }
                 // finish row j = t-1: q_it = A' p_it, stores, partials
                 const int j = t - 1;
-                float qW = __shfl_up(pnc, 1, 64), qE = __shfl_down(pnc, 1, 64);
-                float cWj = __shfl_up(cXc, 1, 64);
-                if (lane == 0) { qW = hpn; cWj = hcVIprev; }
-                if (lane == 63) qE = hpn;
+                float qWl = __shfl_up(pnc[W - 1], 1, 64), qEl = __shfl_down(pnc[0], 1, 64);
+                float cWjl = __shfl_up(cXc[W - 1], 1, 64);
+                if (lane == 0) { qWl = hpn; cWjl = hcVIprev; }
+                if (lane == 63) qEl = hpn;
                 if (valid && j >= j0) {
-                    const float qn = apply_scaled(pnc, cWj, qW, cXc, qE, cYmm, pnm, cYm, pn);
-                    const int c = j * nx + i;
-                    if (!first) {
-                        Rout[c] = rnc;
-                        Pout[c] = pnc;
-                        X[c] = cur.X + al * P0;
-                    } else {
-                        X[c] = 0.0f;
+                    const size_t c = (size_t)j * nx + i;
+                    float xs[W];
+#pragma unroll
+                    for (int k = 0; k < W; k++) {
+                        const float qW = k == 0 ? qWl : pnc[k - 1];
+                        const float qE = k == W - 1 ? qEl : pnc[k + 1];
+                        const float cWj = k == 0 ? cWjl : cXc[k - 1];
+                        const float qn = apply_scaled(pnc[k], cWj, qW, cXc[k], qE, cYmm[k], pnm[k], cYm[k], pn[k]);
+                        xs[k] = first ? 0.0f : cur.X[k] + al * P0[k];
+                        part[0] += (double)pnc[k] * (double)qn;
+                        part[1] += (double)rnc[k] * (double)qn;
+                        part[2] += (double)qn * (double)qn;
+                        part[3] += (double)rnc[k] * (double)rnc[k];
                     }
-                    part[0] += (double)pnc * (double)qn;
-                    part[1] += (double)rnc * (double)qn;
-                    part[2] += (double)qn * (double)qn;
-                    part[3] += (double)rnc * (double)rnc;
+                    if (!first) {
+                        store_cols<W>(Rout, c, rnc);
+                        store_cols<W>(Pout, c, pnc);
+                    }
+                    store_cols<W>(X, c, xs);
                 }
                 // slide
-                P0 = P1; P1 = cur.P2;
-                cYmm = cYm; cYm = cur.cY;
-                cXc = cur.cX;
-                pnm = pnc; pnc = pn; rnc = rn;
+#pragma unroll
+                for (int k = 0; k < W; k++) {
+                    P0[k] = P1[k]; P1[k] = cur.P2[k];
+                    cYmm[k] = cYm[k]; cYm[k] = cur.cY[k];
+                    cXc[k] = cur.cX[k];
+                    pnm[k] = pnc[k]; pnc[k] = pn[k]; rnc[k] = rn[k];
+                }
                 hP0 = hP1; hP1 = cur.hP2;
                 hcYm = cur.hcY;
                 hpn = hpnT;
@@ -681,7 +736,11 @@ __global__ void __launch_bounds__(256) k_refine_test(const RefineArgs *__restric
 
 template __global__ void k_pcg_dir<double>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_pcg_res<double>(const PcgArgs<double> *__restrict__, int);
-template __global__ void k_cg1<true>(const PcgArgs<float> *__restrict__, int);
-template __global__ void k_cg1<false>(const PcgArgs<float> *__restrict__, int);
+template __global__ void k_cg1<true, 1>(const PcgArgs<float> *__restrict__, int);
+template __global__ void k_cg1<false, 1>(const PcgArgs<float> *__restrict__, int);
+template __global__ void k_cg1<true, 2>(const PcgArgs<float> *__restrict__, int);
+template __global__ void k_cg1<false, 2>(const PcgArgs<float> *__restrict__, int);
+template __global__ void k_cg1<true, 4>(const PcgArgs<float> *__restrict__, int);
+template __global__ void k_cg1<false, 4>(const PcgArgs<float> *__restrict__, int);
 
 }  // namespace frehd

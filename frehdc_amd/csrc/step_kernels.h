@@ -145,9 +145,10 @@ struct PcgArgs {
     const double *epsDyn;     // inner solve: tolerance set by the refinement test
     double eps;
     int maxIter, nx, ny, N;
-    // k_pcg_dir tiling: gx x gy tiles of 256 columns x H rows; remap != 0
-    // gives each XCD a contiguous band of tiles (nbDir a multiple of 8)
-    int gx, gy, H, remap;
+    // k_pcg_dir / k_cg1 tiling: gx x gy tiles of 256 lanes columns x H rows
+    // (lanes: adjacent columns per lane, k_cg1 only; 1 for k_pcg_dir); remap
+    // != 0 gives each XCD a contiguous band of tiles (nbDir a multiple of 8)
+    int gx, gy, H, remap, lanes;
     int nbDir, nbRes, nbStart;
     double *partF[2];         // k_cg1: p.q, r.q, q.q, r.r partials (nbDir each),
                               // launch it writes partF[it&1]
@@ -156,9 +157,9 @@ struct PcgArgs {
 template <typename T> __global__ void k_pcg_dir(const PcgArgs<T> *__restrict__ A, int it);
 template <typename T> __global__ void k_pcg_res(const PcgArgs<T> *__restrict__ A, int it);
 // the inner FP32 solve of the mixed path, one launch per iteration (First:
-// the set-up launch, it = 0): cX, cY hold the scaled coefficients, r and q
-// the two residual buffers, D unused
-template <bool First> __global__ void k_cg1(const PcgArgs<float> *__restrict__ A, int it);
+// the set-up launch, it = 0; W = lanes columns per lane): cX, cY hold the
+// scaled coefficients, r and q the two residual buffers, D unused
+template <bool First, int W> __global__ void k_cg1(const PcgArgs<float> *__restrict__ A, int it);
 
 // FP64 iterative refinement around the FP32 solve (mixed precision): sweep
 // s forms x_s = x_{s-1} + d (d: the inner solve's correction) and the true

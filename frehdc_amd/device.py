@@ -124,8 +124,9 @@ class DeviceScenario:
         abi.check(self.lib.frehd_dev_download_tables(self.h, ptrs))
         return out
 
-    def set_pcg_tiling(self, rows_per_tile=0, xcd_remap=False):
-        abi.check(self.lib.frehd_dev_set_pcg_tiling(self.h, int(rows_per_tile), 1 if xcd_remap else 0))
+    def set_pcg_tiling(self, rows_per_tile=0, xcd_remap=False, cols_per_lane=1):
+        abi.check(self.lib.frehd_dev_set_pcg_tiling(self.h, int(rows_per_tile), 1 if xcd_remap else 0,
+                                                    int(cols_per_lane)))
 
     def autotune(self, iters: int = 20) -> dict:
         """Time the solver under every candidate tiling on the last step's
@@ -216,6 +217,7 @@ def device_count() -> int:
 
 
 def _autotune(fn, h, iters):
-    H, remap, ms = C.c_int32(), C.c_int32(), C.c_double()
-    abi.check(fn(h, int(iters), C.byref(H), C.byref(remap), C.byref(ms)))
-    return {"rows_per_tile": H.value, "xcd_remap": remap.value, "ms_per_iteration": ms.value}
+    H, remap, lanes, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+    abi.check(fn(h, int(iters), C.byref(H), C.byref(remap), C.byref(lanes), C.byref(ms)))
+    return {"rows_per_tile": H.value, "xcd_remap": remap.value, "cols_per_lane": lanes.value,
+            "ms_per_iteration": ms.value}

@@ -18,6 +18,7 @@ import argparse
 import collections
 import csv
 import json
+import re
 
 
 def per_dispatch(paths):
@@ -38,7 +39,8 @@ def copy_factor(path, mib):
 
 def short(name):
     n = name.split("(")[0].replace("void ", "").replace("frehd::", "")
-    n = n.replace("k_cg1<false>", "k_cg1_f32").replace("k_cg1<true>", "k_cg1_f32_setup")
+    n = re.sub(r"k_cg1<false(, \d)?>", "k_cg1_f32", n)
+    n = re.sub(r"k_cg1<true(, \d)?>", "k_cg1_f32_setup", n)
     return n.replace("<double>", "").replace("<float>", "_f32")
 
 

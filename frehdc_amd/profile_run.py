@@ -25,8 +25,8 @@ def main():
     ap.add_argument("--forms", default="fp64,mixed_warm")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--tiling", default="",
-                    help="H:remap solver tiling instead of the autotuner (whose trial launches would enter a "
-                         "profile's averages): one for every form, or one per form, comma-separated")
+                    help="H:remap[:cols_per_lane] solver tiling instead of the autotuner (whose trial launches "
+                         "would enter a profile's averages): one for every form, or one per form, comma-separated")
     a = ap.parse_args()
     n = a.size or (1024 if a.workload == "marsh" else 4096)
     t0 = time.perf_counter()
@@ -44,9 +44,11 @@ def main():
     for q, f in enumerate(forms):
         d = DeviceScenario(sc, **FORMS[f])
         if tilings:
-            H, remap = (int(x) for x in tilings[q].split(":"))
-            d.set_pcg_tiling(H, remap)
-            tiling = {"rows_per_tile": H, "xcd_remap": remap}
+            H, remap, lanes = ([int(x) for x in tilings[q].split(":")] + [1])[:3]
+            if f == "fp64":
+                lanes = 1
+            d.set_pcg_tiling(H, remap, lanes)
+            tiling = {"rows_per_tile": H, "xcd_remap": remap, "cols_per_lane": lanes}
             d.step(3)
         else:
             d.step(3)

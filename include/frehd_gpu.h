@@ -270,23 +270,29 @@ int frehd_dev_kernel_times(frehd_dev_t *dev, int32_t cap, const char **names,
 int frehd_dev_download_tables(frehd_dev_t *dev, double *const tables[10]);
 
 /* Launch shape of the solver's direction kernel (no counterpart in the
- * reference; a tuning knob for the autotuner): tiles of 256 columns x
- * rows_per_tile rows (0 = automatic, about 1024 tiles), and xcd_remap != 0
- * to give each of the 8 XCDs a contiguous band of tiles.  Results do not
- * depend on the shape beyond the order of the solver's dot-product sums. */
-int frehd_dev_set_pcg_tiling(frehd_dev_t *dev, int32_t rows_per_tile, int32_t xcd_remap);
+ * reference; a tuning knob for the autotuner): tiles of 256 x cols_per_lane
+ * columns x rows_per_tile rows (0 = automatic: about 1024 tiles, 512 for
+ * the mixed path's inner solve), and xcd_remap != 0 to give each of the 8
+ * XCDs a contiguous band of tiles.  cols_per_lane (1, 2 or 4: adjacent
+ * columns each lane loads as one vector) applies to the mixed path's inner
+ * FP32 solve and needs nx to be a multiple of it; the FP64 PCG takes 1.
+ * Results do not depend on the shape beyond the order of the solver's
+ * dot-product sums.  A handle joining a batch takes the members'
+ * cols_per_lane (or, if its nx does not allow it, the batch goes back to 1). */
+int frehd_dev_set_pcg_tiling(frehd_dev_t *dev, int32_t rows_per_tile, int32_t xcd_remap, int32_t cols_per_lane);
 
 /* Autotuner of that launch shape (no counterpart in the reference): times
  * `iters` solver iterations (the FP64 PCG pair, or the mixed path's inner
  * FP32 iteration plus its share of FP64 refinement sweeps, in the
- * proportion of the last step's solve) under every candidate shape -- rows_per_tile 0, 1, 2, 4,
- * ... up to min(ny, 256), XCD remap off and on -- on the system of the last
- * step, applies the fastest and returns it (each output may be NULL).  The
- * model state, the step counter and the statistics are left as they were.
- * Call it after at least one step; FREHD_ESTATE for a handle in a batch
- * (tune the batch). */
+ * proportion of the last step's solve) under every candidate shape --
+ * rows_per_tile 0, 1, 2, 4, ... up to min(ny, 256), XCD remap off and on,
+ * cols_per_lane 1, 2, 4 (mixed path, where nx allows) -- on the system of
+ * the last step, applies the fastest and returns it (each output may be
+ * NULL; rows_per_tile as resolved).  The model state, the step counter and
+ * the statistics are left as they were.  Call it after at least one step;
+ * FREHD_ESTATE for a handle in a batch (tune the batch). */
 int frehd_dev_autotune(frehd_dev_t *dev, int32_t iters, int32_t *rows_per_tile, int32_t *xcd_remap,
-                       double *ms_per_iteration);
+                       int32_t *cols_per_lane, double *ms_per_iteration);
 
 /* ---- batches: many handles advanced by shared launches ---------------------
  * No counterpart in the reference (one scenario per process there).  A batch
@@ -309,7 +315,7 @@ int frehd_batch_step(frehd_batch_t *batch, int32_t nsteps);
 /* frehd_dev_autotune over the members together (one shape for all: the
  * launches are shared). */
 int frehd_batch_autotune(frehd_batch_t *batch, int32_t iters, int32_t *rows_per_tile, int32_t *xcd_remap,
-                         double *ms_per_iteration);
+                         int32_t *cols_per_lane, double *ms_per_iteration);
 int frehd_batch_enable_timing(frehd_batch_t *batch, int32_t on);
 int frehd_batch_kernel_times(frehd_batch_t *batch, int32_t cap, const char **names,
                              double *ms, int64_t *launches, int32_t *n);

@@ -36,6 +36,7 @@ def test_autotune_keeps_state(make, prec):
         assert np.array_equal(f0[k], f1[k]), k
     assert s0 == s1
     assert best["rows_per_tile"] >= 1 and best["xcd_remap"] in (0, 1) and best["ms_per_iteration"] > 0
+    assert best["cols_per_lane"] in ((1, 2, 4) if prec == abi.PREC_MIXED else (1,))
     a.step(10)
     b.step(10)
     fa, fb = a.download(["surf", "uuXP", "vvYP"]), b.download(["surf", "uuXP", "vvYP"])
@@ -64,6 +65,44 @@ def test_batch_autotune():
             assert np.array_equal(f0[k], f1[k]), k
         assert s0 == s1
     assert best["rows_per_tile"] <= 40
+    assert best["cols_per_lane"] in (1, 2)          # nx = 130 is not a multiple of 4
     b.step(3)
     for d in devs:
         assert d.stats()["steps"] == 6
+
+
+def test_batch_lanes_follow_members():
+    """A handle joining a batch takes the members' columns per lane, or the
+    batch goes back to one column per lane when the newcomer's nx does not
+    allow it; either way the batched run equals the solo runs (the same
+    tiling there) within the solver tolerance."""
+    print("case batch lanes: mixed basins 96x64 (lanes 4), 96x48, 130x40", flush=True)
+    scs = [cases.basin(nx=96, ny=64, Nt=40), cases.basin(nx=96, ny=48, Nt=40, seed=4),
+           cases.basin(nx=130, ny=40, Nt=40, seed=3)]
+    devs = [DeviceScenario(s, precision=abi.PREC_MIXED) for s in scs]
+    devs[0].set_pcg_tiling(0, 0, 4)
+    solo = []
+    for s, lanes, n in zip(scs, (4, 4, 1), (6, 6, 4)):
+        d = DeviceScenario(s, precision=abi.PREC_MIXED)
+        d.set_pcg_tiling(0, 0, lanes)
+        d.step(n)
+        solo.append(d.download(["surf", "uuXP"]))
+        d.close()
+    b = Batch(3)
+    b.add(devs[0])
+    b.add(devs[1])                  # takes 4
+    b.step(2)
+    b.add(devs[2])                  # nx = 130: everyone back to 1
+    b.step(2)
+    b.remove(devs[2])
+    devs[2].step(2)
+    b.step(2)
+    got = [d.download(["surf", "uuXP"]) for d in devs]
+    steps = [d.stats()["steps"] for d in devs]
+    b.close()
+    for d in devs:
+        d.close()
+    assert steps == [6, 6, 4]
+    for g, r, s in zip(got, solo, scs):
+        for k in g:
+            assert np.max(np.abs(g[k][: s.N] - r[k][: s.N])) <= 1e-8, (s.name, k)

@@ -98,13 +98,14 @@ def test_mixed_within_fp64_tolerance_sensitivity(warm):
         assert e_mix[k] <= max(1e-4, 30 * e_tight[k]), k
 
 
-@pytest.mark.parametrize("H,remap", [(0, 0), (1, 0), (3, 1), (70, 0)])
-def test_inner_solver_tilings(H, remap):
-    """The inner FP32 CG walks tiles of 256 columns x H rows; a 300 x 70 grid
-    (two column tiles, a width that is not a multiple of the 64-lane wave)
-    under several row tilings must converge like the FP64 CG and give the
-    FP64 path's surface."""
-    print(f"case basin300x70 mixed cold, tiling H={H} remap={remap}", flush=True)
+@pytest.mark.parametrize("H,remap,lanes", [(0, 0, 1), (1, 0, 1), (3, 1, 1), (70, 0, 1), (0, 0, 4), (1, 0, 2),
+                                           (3, 1, 4), (70, 0, 2), (5, 1, 2)])
+def test_inner_solver_tilings(H, remap, lanes):
+    """The inner FP32 CG walks tiles of 256 lanes columns x H rows; a 300 x 70
+    grid (two column tiles at one column per lane, a width that is not a
+    multiple of the wave's columns) under several tilings must converge like
+    the FP64 CG and give the FP64 path's surface."""
+    print(f"case basin300x70 mixed cold, tiling H={H} remap={remap} lanes={lanes}", flush=True)
     sc = cases.basin(nx=300, ny=70, Nt=60)
     ref = DeviceScenario(sc)
     ref.step(20)
@@ -112,7 +113,7 @@ def test_inner_solver_tilings(H, remap):
     it64 = ref.stats()["pcgIterTotal"] / 20
     ref.close()
     dev = DeviceScenario(sc, precision=abi.PREC_MIXED, warmStart=0)
-    dev.set_pcg_tiling(H, remap)
+    dev.set_pcg_tiling(H, remap, lanes)
     dev.step(20)
     st = dev.stats()
     got = dev.download(["surf"])
