@@ -35,7 +35,8 @@ class FrehdConfig(C.Structure):
         ("epsInner", C.c_double), ("maxRefine", C.c_int32),
         ("useSubgrid", C.c_int32), ("Ne", C.c_int32), ("warmStart", C.c_int32),
         ("surfmin", C.c_double), ("dsurf", C.c_double), ("subR", C.c_double),
-        ("subgridOnTheFly", C.c_int32), ("pad2", C.c_int32),
+        ("subgridOnTheFly", C.c_int32), ("useWind", C.c_int32),
+        ("northAngle", C.c_double), ("rhoa", C.c_double), ("Cw", C.c_double), ("CwT", C.c_double),
     ]
 
 
@@ -44,6 +45,7 @@ class FrehdForcing(C.Structure):
         ("tideLocP", c_int32_p), ("tideLocM", c_int32_p), ("inflowLoc", c_int32_p),
         ("tideP", c_double_p), ("tideM", c_double_p), ("inflow", c_double_p),
         ("tidalPS", c_double_p), ("tidalMS", c_double_p), ("manningCell", c_double_p),
+        ("windspd", c_double_p), ("winddir", c_double_p),
     ]
 
 

@@ -105,6 +105,9 @@ class Case:
             manningCell=None, surf0=ic.surf0, U0=ic.U0, V0=ic.V0, S0=ic.S0, S0cell=_arr(ic.S0cell, N),
             name="case", surf0cell=_arr(ic.surf0cell, N), U0cell=_arr(ic.U0cell, N),
             V0cell=_arr(ic.V0cell, N))
+        if k.useWind == 1:
+            sc.windspd = _arr(f.windspd, Nt1)
+            sc.winddir = _arr(f.winddir, Nt1)
         if k.useSubgrid == 1 and ic.sub:
             s = ic.sub.contents
             sg = {"edgeX": _arr(s.edgeX, N), "edgeY": _arr(s.edgeY, N)}

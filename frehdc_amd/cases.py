@@ -57,6 +57,8 @@ class Scenario:
     U0cell: np.ndarray | None = None
     V0cell: np.ndarray | None = None
     subgrid: dict | None = None        # edgeX, edgeY and level tables (subgrid.py)
+    windspd: np.ndarray | None = None  # (Nt+1,) with cfg useWind = 1
+    winddir: np.ndarray | None = None
 
     @property
     def nx(self):
@@ -94,6 +96,8 @@ class Scenario:
         f.tidalPS = abi.dptr(self.tidalPS)
         f.tidalMS = abi.dptr(self.tidalMS)
         f.manningCell = abi.dptr(self.manningCell)
+        f.windspd = abi.dptr(self.windspd)
+        f.winddir = abi.dptr(self.winddir)
         return f
 
     def c_ic(self) -> abi.FrehdIC:
@@ -127,7 +131,7 @@ def default_config(nx, ny, dx, dt, Nt, **kw) -> dict:
         tideLocLengthM=0, inflowLocLength=0, useScalar=1, scalarAdv=1, useConstTidePS=1,
         useConstTideMS=1, nScalar=1, tidePS=0.0, tideMS=0.0, inflowS=0.0, Kx=1e-4, Ky=1e-4,
         offset=0.0, epsInner=1e-5, maxRefine=8, warmStart=0, useSubgrid=0, Ne=0, surfmin=0.0, dsurf=0.01,
-        subR=1.0, subgridOnTheFly=0,
+        subR=1.0, subgridOnTheFly=0, useWind=0, northAngle=0.0, rhoa=1.225, Cw=1.3e-3, CwT=1.0,
     )
     cfg.update(kw)
     return cfg
@@ -326,13 +330,15 @@ def ensemble_member(m, n=2048, dx=1.0, dt=0.5, Nt=1000, seed=5, hurst=0.7, base=
 
 
 def serving_requests(count=512, rate=20.0, seed=6, lo=64, hi=1024, dx=2.0, dt=0.5,
-                     steps_lo=200, steps_hi=2000, subgrid_r=0):
+                     steps_lo=200, steps_hi=2000, subgrid_r=0, on_the_fly=False):
     """Poisson arrivals of scenarios with log-uniform sizes in [lo^2, hi^2]
     (rounded to multiples of 64 per side), fBm bed with Hurst ~ U(0.6, 0.9) in
     [-1.5, +1] m (an always-wet inlet at the tidal rows), tide ~ U(0.1, 0.7) m,
     Manning ~ U(0.015, 0.04), 1-3 scalars, U(200, 2000) steps; with
     subgrid_r > 0 the bed is drawn at r x r pixels per cell and the tables
-    are generated on the device.  Returns [(arrival_s, steps, side, builder)]
+    are generated on the device (or, on_the_fly, the device keeps the pixels
+    and evaluates the levels at each lookup: r^2 doubles per cell instead
+    of 10 per level).  Returns [(arrival_s, steps, side, builder)]
     where builder() makes the Scenario on demand."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, size=count))
@@ -356,10 +362,25 @@ def serving_requests(count=512, rate=20.0, seed=6, lo=64, hi=1024, dx=2.0, dt=0.
             sc = _finish(cfg, coarse, tide_series(steps, dt, amp),
                          scalar0=np.zeros(nsc * side * side), name=f"req{side}")
             if subgrid_r > 0:
-                sc = with_subgrid(sc, bed, r, -1.5, 1.0, 0.01, device_tables=True)
+                sc = with_subgrid(sc, bed, r, -1.5, 1.0, 0.01, device_tables=True, on_the_fly=on_the_fly)
             return sc
         out.append((float(t[q]), steps, side, build))
     return out
+
+
+def with_wind(scen: Scenario, speed: float = 8.0, gust: float = 3.0, direction: float = 30.0,
+              veer: float = 40.0, northAngle: float = 10.0, period: float = 600.0) -> Scenario:
+    """Switch on the wind stress (windTerm, nsfunctions.c:609-648): speed
+    speed + gust sin(2 pi t / period) m/s from direction + veer sin(pi t /
+    period) degrees east of north, per step (bc->windspd / winddir, Nt+1
+    samples); Config.rhoa, Cw, CwT keep the scenario's values."""
+    Nt, dt = scen.cfg["Nt"], scen.cfg["dt"]
+    t = np.arange(Nt + 1, dtype=np.float64) * dt
+    scen.windspd = np.ascontiguousarray(speed + gust * np.sin(2.0 * math.pi * t / period))
+    scen.winddir = np.ascontiguousarray(direction + veer * np.sin(math.pi * t / period))
+    scen.cfg.update(useWind=1, northAngle=float(northAngle))
+    scen.name = scen.name + "_wind"
+    return scen
 
 
 def with_subgrid(scen: Scenario, fine: np.ndarray, r: int, surfmin: float, surfmax: float,

@@ -329,6 +329,13 @@ size_t layout(frehd_dev *d, char *base)
     a.inflow = cv.take<double>(Nt1);
     a.tidalPS = cv.take<double>(Nt1);
     a.tidalMS = cv.take<double>(Nt1);
+    if (k.useWind == 1) {
+        a.windSpd = cv.take<double>(Nt1);
+        a.windCos = cv.take<double>(Nt1);
+        a.windSin = cv.take<double>(Nt1);
+    } else {
+        a.windSpd = a.windCos = a.windSin = nullptr;
+    }
     a.tideLocP = cv.take<int>(std::max(1, k.tideLocLengthP));
     a.tideLocM = cv.take<int>(std::max(1, k.tideLocLengthM));
     if (k.useSubgrid == 1) {
@@ -376,6 +383,7 @@ int check_config(const frehd_config *k)
                                   "(SSOR's triangular sweeps are sequential)");
     if (k->precision != FREHD_PREC_FP64 && k->precision != FREHD_PREC_MIXED)
         return fail(FREHD_EINVAL, "unknown precision %d", k->precision);
+    if (k->useWind != 0 && k->useWind != 1) return fail(FREHD_EINVAL, "useWind must be 0 or 1");
     if (k->precision == FREHD_PREC_MIXED && !(k->epsInner > 0 && k->epsInner < 1 && k->maxRefine >= 1))
         return fail(FREHD_EINVAL, "mixed precision needs 0 < epsInner < 1 and maxRefine >= 1");
     if (k->precision == FREHD_PREC_MIXED && (k->warmStart < 0 || k->warmStart > 2))
@@ -434,6 +442,8 @@ void fill_params(P &p, const frehd_config &k)
     p.nScalar = k.useScalar == 1 ? std::max(1, k.nScalar) : 0;
     p.useConstTidePS = k.useConstTidePS; p.useConstTideMS = k.useConstTideMS; p.maxIter = k.maxIter;
     p.scalarAdv = k.scalarAdv;
+    p.useWind = k.useWind;
+    p.rhoa = k.rhoa; p.Cw = k.Cw; p.CwT = k.CwT;
     p.useSubgrid = k.useSubgrid; p.dsurf = k.dsurf; p.subR = k.subR; p.Ne = k.Ne; p.surfmin = k.surfmin;
     p.warmStart = k.precision == FREHD_PREC_MIXED ? k.warmStart : 0;
 }
@@ -481,6 +491,7 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
         return fail(FREHD_EINVAL, "tideLocM/tideM missing");
     if (cfg->inflowLocLength > 0 && !(f->inflowLoc && f->inflow))
         return fail(FREHD_EINVAL, "inflowLoc/inflow missing");
+    if (cfg->useWind == 1 && !(f->windspd && f->winddir)) return fail(FREHD_EINVAL, "windspd/winddir missing");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FREHD_EHIP, "no HIP device visible");
     frehd_dev *d = new frehd_dev();
@@ -501,6 +512,8 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     HIPCHK(hipMemsetAsync(d->arena, 0, bytes, d->stream));
     size_t used = layout(d, d->arena);
     if (f->manningCell) layout_manning(d, d->arena, used);
+    const int firstStep = 1;   // Ctl.ttNext at tt = 0
+    COPYW(d->stream, &d->args.ctl->ttNext, &firstStep, sizeof(int), hipMemcpyHostToDevice);
     // no surface seen yet by the subgrid lookup (all-ones bits: a NaN no
     // surface equals)
     if (cfg->useSubgrid == 1)
@@ -522,6 +535,23 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     if ((rc = upload_series(d, f->inflow, a.inflow))) return rc;
     if ((rc = upload_series(d, f->tidalPS, a.tidalPS))) return rc;
     if ((rc = upload_series(d, f->tidalMS, a.tidalMS))) return rc;
+    if (cfg->useWind == 1) {
+        // windTerm (nsfunctions.c:615-617): omega = (winddir + northAngle) pi / 180
+        // with the reference's pi, and the host's cos / sin of it
+        const size_t n = (size_t)cfg->Nt + 1;
+        std::vector<double> cw(n), sw(n);
+        for (size_t t = 0; t < n; t++) {
+            const double pi = 3.1415926;
+            const double phi = f->winddir[t] + cfg->northAngle;
+            const double omega = phi * pi / 180.0;
+            cw[t] = cos(omega);
+            sw[t] = sin(omega);
+        }
+        if ((rc = upload_series(d, f->windspd, a.windSpd))) return rc;
+        if ((rc = upload(d, cw.data(), (void *)a.windCos, n * sizeof(double)))) return rc;
+        if ((rc = upload(d, sw.data(), (void *)a.windSin, n * sizeof(double)))) return rc;
+        HIPCHK(hipStreamSynchronize(d->stream));   // the host vectors go out of scope
+    }
     if (cfg->tideLocLengthP > 0 && f->tideLocP)
         if ((rc = upload(d, f->tideLocP, (void *)a.tideLocP, sizeof(int) * cfg->tideLocLengthP))) return rc;
     if (cfg->tideLocLengthM > 0 && f->tideLocM)
@@ -1031,6 +1061,8 @@ int frehd_dev_set_step_index(frehd_dev_t *d, int32_t tt)
     HIPCHK(hipSetDevice(d->device));
     HIPCHK(hipStreamSynchronize(d->stream));
     COPYW(d->stream, &d->args.ctl->tt, &tt, sizeof(int), hipMemcpyHostToDevice);
+    const int next = tt + 1;
+    COPYW(d->stream, &d->args.ctl->ttNext, &next, sizeof(int), hipMemcpyHostToDevice);
     return 0;
 }
 

@@ -29,7 +29,7 @@ struct frehd_case {
     int useConstSurf0, useConstU0, useConstV0, useConstInitS, useminV, useCellEdge;
     double initSurf, initU, initV, initS;
     std::vector<int32_t> tideLocP, tideLocM, inflowLoc;
-    std::vector<double> tideP, tideM, inflow, tidalPS, tidalMS;
+    std::vector<double> tideP, tideM, inflow, tidalPS, tidalMS, windspd, winddir;
     std::vector<double> allZ, allEdgeX, allEdgeY;
     std::vector<double> surf0, U0, V0, S0;
     std::vector<double> tab[14];   // V Z Vxp Vxm Vyp Vym Np Nm Op Om minVxp minVxm minVyp minVym
@@ -213,7 +213,9 @@ int frehd_case_load(const char *data_dat, const char *base_dir, frehd_case_t **o
     const int tideFirstM = (int)a[24], tideNM = (int)a[25];
     k.inflowLocLength = (int)a[26];
     const int i0 = (int)a[27], i1 = (int)a[28], j0 = (int)a[29], j1 = (int)a[30], inflowN = (int)a[31];
-    const int useWind = (int)a[32];
+    k.useWind = (int)a[32];
+    k.northAngle = a[33];
+    const int windspdN = (int)a[34], winddirN = (int)a[35];
     k.useScalar = (int)a[36];
     k.scalarAdv = (int)a[37];
     k.inflowS = a[39];
@@ -249,6 +251,9 @@ int frehd_case_load(const char *data_dat, const char *base_dir, frehd_case_t **o
     k.useThinLayer = (int)a[74];
     k.hD = a[75];
     k.CDmax = a[76];
+    k.CwT = a[77];
+    k.rhoa = a[78];
+    k.Cw = a[79];
     k.Kx = a[80];
     k.Ky = a[81];
     k.minDepth = a[82];
@@ -271,7 +276,6 @@ int frehd_case_load(const char *data_dat, const char *base_dir, frehd_case_t **o
         return rc;
     };
     if (NX < 1 || NY < 1) return bad(FREHD_EINVAL, "NX and NY must be positive");
-    if (useWind == 1) return bad(FREHD_EINVAL, "wind forcing is not supported by the device step");
     if (k.useSubgrid == 1 && (useSubDrag != 0 || staggeredV != 1 || useminA != 0))
         return bad(FREHD_EINVAL, "subgrid needs useSubDrag = 0, staggeredV = 1, useminA = 0");
     if (k.useSubgrid == 0 && c->useCellEdge == 1)
@@ -308,6 +312,8 @@ int frehd_case_load(const char *data_dat, const char *base_dir, frehd_case_t **o
     if (k.bcType != 2 && (rc = read_bc(c, "tideP.dat", tideNP, c->tideP))) { delete c; return rc; }
     if (k.bcType != 1 && (rc = read_bc(c, "tideM.dat", tideNM, c->tideM))) { delete c; return rc; }
     if ((rc = read_bc(c, "inflow.dat", inflowN, c->inflow))) { delete c; return rc; }
+    if (k.useWind == 1 && (rc = read_bc(c, "windspd.dat", windspdN, c->windspd))) { delete c; return rc; }
+    if (k.useWind == 1 && (rc = read_bc(c, "winddir.dat", winddirN, c->winddir))) { delete c; return rc; }
     if (k.useConstTidePS == 0 && (rc = read_bc(c, "salinityBC.dat", tidalPSN, c->tidalPS))) { delete c; return rc; }
     if (k.useConstTideMS == 0 && (rc = read_bc(c, "salinityBC.dat", tidalMSN, c->tidalMS))) { delete c; return rc; }
 
@@ -357,6 +363,8 @@ int frehd_case_inputs(const frehd_case_t *c, frehd_config *cfg, frehd_forcing *f
         fo->tidalPS = c->tidalPS.empty() ? nullptr : c->tidalPS.data();
         fo->tidalMS = c->tidalMS.empty() ? nullptr : c->tidalMS.data();
         fo->manningCell = nullptr;
+        fo->windspd = c->windspd.empty() ? nullptr : c->windspd.data();
+        fo->winddir = c->winddir.empty() ? nullptr : c->winddir.data();
     }
     if (ic) {
         memset(ic, 0, sizeof *ic);

@@ -490,8 +490,12 @@ static int exec_phase_launch(Exec &e, int phase)
     }
 }
 
+// A step returns while the tail of its graph still runs (the next step's
+// graph queues behind it), so the stream drains before any executable
+// graph is destroyed: the runtime frees a graph's argument buffers with it.
 void exec_release_graphs(Exec &e)
 {
+    if ((!e.heads.empty() || e.tail) && e.stream) hipStreamSynchronize(e.stream);
     for (auto &h : e.heads) hipGraphExecDestroy(h.second);
     e.heads.clear();
     if (e.tail) hipGraphExecDestroy(e.tail);

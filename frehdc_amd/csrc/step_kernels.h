@@ -17,6 +17,8 @@ struct P {
     int CDnotN, useThinLayer, bcType, tideLocLengthP, tideLocLengthM, inflowLocLength;
     int useScalar, nScalar, useConstTidePS, useConstTideMS, maxIter, useSubgrid;
     int scalarAdv;            // 1 first-order upwind, 2 TVD superbee
+    int useWind;
+    double rhoa, Cw, CwT;     // wind stress (windTerm)
     // subgrid
     double dsurf, subR, surfmin;   // surfmin: raw elevation of level 0
     int Ne;
@@ -52,6 +54,9 @@ struct Ctl {
                               // after it stand down (set by the flag gather)
     int stepDone;             // the step's kernels after the solve have run
                               // (set by the step end, cleared at the step's begin)
+    int ttNext;               // tt + 1 at every step boundary: the step the next
+                              // predictor computes (its first thread advances tt
+                              // while the other blocks run, so they read this)
     double lastMaxCFLx, lastMaxCFLy;
     double Vloss[4];
     PcgScalars pcg;
@@ -99,6 +104,9 @@ struct StepArgs {
     int gx2, gy2, nbN, nbGhost, nbTide, nbRes;
     // forcing (Nt+1 samples) and locations
     const double *tideP, *tideM, *inflow, *tidalPS, *tidalMS;
+    // wind per step: speed and the cosine / sine of its direction (computed
+    // on the host with the reference's expression and libm)
+    const double *windSpd, *windCos, *windSin;
     const int *tideLocP, *tideLocM;
     // subgrid: state (Ng), half-cell values (N), level search index (Ng),
     // face bottoms with offset (N), level elevations (Ne), level tables

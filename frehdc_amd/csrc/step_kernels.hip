@@ -15,8 +15,9 @@ __device__ inline int cur_tt(const StepArgs &a) { return a.ctl->tt; }
 
 // ---------------------------------------------------------------------------
 // Step begin: tt += 1 (SolveAll's loop counter, nssolve.c:61) and the solver
-// state reset; done by the predictor's first thread (the predictor reads
-// neither), so a step costs no launch for it.
+// state reset; done by the predictor's first thread (the rest of the
+// predictor reads the step index from ttNext, not tt), so a step costs no
+// launch for it.
 __device__ inline void step_begin(const StepArgs &a)
 {
     Ctl *c = a.ctl;
@@ -119,6 +120,21 @@ __device__ inline void predictor_body(const StepArgs &a)
     }
     EnX = EnX + dt * dfx;
     EnY = EnY + dt * dfy;
+
+    // windTerm (nsfunctions.c:609-648; the plain term on the subgrid path
+    // too, nssolve.c:131): stress rhoa Cw |W - u.e|^2 along the wind
+    // direction, decaying on thin faces, over the face depth
+    if (p.useWind == 1) {
+        const int tt = a.ctl->ttNext;
+        const double cw = a.windCos[tt], sw = a.windSin[tt], spd = a.windSpd[tt];
+        const double rel = spd - uc * cw - vc * sw;
+        const double tau = p.rhoa * p.Cw * rel * rel;
+        const double dX = a.depthXP[c], dY = a.depthYP[c];
+        const double tX = (dX < p.hD && p.useThinLayer == 1) ? tau * exp(p.CwT * (dX - p.hD) / p.hD) : tau;
+        const double tY = (dY < p.hD && p.useThinLayer == 1) ? tau * exp(p.CwT * (dY - p.hD) / p.hD) : tau;
+        if (dX > 0) EnX = EnX + dt * tX * cw / (dX * 1000.0);
+        if (dY > 0) EnY = EnY - dt * tY * sw / (dY * 1000.0);
+    }
 
     const double velx = sqrt(uc * uc + vx * vx);
     const double vely = sqrt(uy * uy + vc * vc);
@@ -824,6 +840,7 @@ __global__ void __launch_bounds__(1024) k_step_end(const StepArgs *__restrict__ 
     c->parity ^= 1;     // the new depth becomes the next step's start
     if (a.p.nScalar > 0) c->sParity ^= 1;   // and the new concentration
     c->stepDone = 1;
+    c->ttNext = c->tt + 1;
     if (fabs(mx) > 20 || fabs(my) > 20) {
         c->halted = 1;
         c->haltStep = c->tt;

@@ -17,6 +17,10 @@ in shared launches (continuous batching), and leave when their run is done.
   between rounds, so new work joins running batches.
 * Reported: per-request latency (completion - arrival) p50/p99, queue wait,
   aggregate cell-steps/s over the makespan, peak resident bytes and members.
+* Subgrid requests keep their pixels on the device and evaluate the two
+  levels bracketing the surface at each lookup (--tables: level tables in
+  HBM instead, 20 KB per cell at 250 levels, which the budget admits only a
+  few dozen of at a time).
 
     python -m frehdc_amd.serving --count 512 --rate 20 --subgrid 4 --forms fp64,mixed_warm
 """
@@ -59,12 +63,13 @@ def footprint(scen, **over) -> int:
 
 class Server:
     def __init__(self, budget_bytes=200e9, buckets=(128, 256, 512, 1024, 2048, 4096), quantum=16,
-                 capacity=128, collect=None, **over):
+                 capacity=128, collect=None, progress=0.0, **over):
         self.budget = float(budget_bytes)
         self.edges = list(buckets)
         self.quantum = int(quantum)
         self.capacity = int(capacity)
         self.collect = collect          # field names to download at completion, or None
+        self.progress = float(progress)  # seconds between progress lines on stdout (0: none)
         self.over = over                # config overrides for every request (precision, ...)
         self.batches = {}
         self.members = collections.defaultdict(list)
@@ -92,7 +97,9 @@ class Server:
         nxt = 0
         active = 0
         rounds = 0
+        done_count = 0
         t0 = time.perf_counter()
+        next_report = self.progress
 
         def now():
             return time.perf_counter() - t0
@@ -151,6 +158,11 @@ class Server:
                     mem.remove(r)
                     used -= r.footprint
                     active -= 1
+                    done_count += 1
+            if self.progress > 0 and now() >= next_report:
+                print(f"# t={now():.0f}s served {done_count}/{len(reqs)}, resident {active}, queued {len(queue)}, "
+                      f"arrived {nxt}, resident bytes {used / 1e9:.1f} GB", flush=True)
+                next_report += self.progress
         self.pool.shutdown()
         for b in self.batches.values():
             b.close()
@@ -181,15 +193,16 @@ def _step_in_thread(batch, k):
     return rc, (batch.lib.frehd_last_error().decode() if rc else "")
 
 
-def make_requests(count, rate, seed=6, lo=64, hi=1024, steps_lo=200, steps_hi=2000, subgrid_r=0, workers=1):
+def make_requests(count, rate, seed=6, lo=64, hi=1024, steps_lo=200, steps_hi=2000, subgrid_r=0, workers=1,
+                  on_the_fly=False):
     """Build the request scenarios up front (their inputs exist before they
     arrive); `workers` > 1 builds them in a process pool."""
     spec = cases.serving_requests(count=count, rate=rate, seed=seed, lo=lo, hi=hi, steps_lo=steps_lo,
-                                  steps_hi=steps_hi, subgrid_r=subgrid_r)
+                                  steps_hi=steps_hi, subgrid_r=subgrid_r, on_the_fly=on_the_fly)
     if workers > 1:
         from concurrent.futures import ProcessPoolExecutor
         with ProcessPoolExecutor(max_workers=workers) as ex:
-            scens = list(ex.map(_build, [(count, rate, seed, lo, hi, steps_lo, steps_hi, subgrid_r, q)
+            scens = list(ex.map(_build, [(count, rate, seed, lo, hi, steps_lo, steps_hi, subgrid_r, on_the_fly, q)
                                          for q in range(count)]))
     else:
         scens = [b() for (_, _, _, b) in spec]
@@ -197,9 +210,9 @@ def make_requests(count, rate, seed=6, lo=64, hi=1024, steps_lo=200, steps_hi=20
 
 
 def _build(args):
-    count, rate, seed, lo, hi, steps_lo, steps_hi, subgrid_r, q = args
+    count, rate, seed, lo, hi, steps_lo, steps_hi, subgrid_r, on_the_fly, q = args
     spec = cases.serving_requests(count=count, rate=rate, seed=seed, lo=lo, hi=hi, steps_lo=steps_lo,
-                                  steps_hi=steps_hi, subgrid_r=subgrid_r)
+                                  steps_hi=steps_hi, subgrid_r=subgrid_r, on_the_fly=on_the_fly)
     return spec[q][3]()
 
 
@@ -216,17 +229,21 @@ def main():
     ap.add_argument("--quantum", type=int, default=16)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--forms", default="fp64,mixed_warm")
+    ap.add_argument("--progress", type=float, default=15.0, help="seconds between progress lines (0: none)")
+    ap.add_argument("--tables", action="store_true",
+                    help="subgrid level tables in HBM (20 KB per cell at 250 levels) instead of evaluating the "
+                         "levels from the pixels at each lookup")
     a = ap.parse_args()
     t0 = time.perf_counter()
     reqs = make_requests(a.count, a.rate, lo=a.lo, hi=a.hi, steps_lo=a.steps_lo, steps_hi=a.steps_hi,
-                         subgrid_r=a.subgrid, workers=a.workers)
+                         subgrid_r=a.subgrid, workers=a.workers, on_the_fly=not a.tables)
     print(f"# built {len(reqs)} requests in {time.perf_counter() - t0:.1f} s", flush=True)
     forms = {"fp64": {}, "mixed_warm": {"precision": abi.PREC_MIXED, "warmStart": 2},
              "mixed_cold": {"precision": abi.PREC_MIXED, "warmStart": 0}}
     out = {}
     for f in a.forms.split(","):
         fresh = [Request(rid=r.rid, arrival=r.arrival, steps=r.steps, scen=r.scen) for r in reqs]
-        st = Server(budget_bytes=a.budget_gb * 1e9, quantum=a.quantum, **forms[f]).run(fresh)
+        st = Server(budget_bytes=a.budget_gb * 1e9, quantum=a.quantum, progress=a.progress, **forms[f]).run(fresh)
         out[f] = st
         print(json.dumps({f: st}), flush=True)
     print(json.dumps({"serving": out, "count": a.count, "rate": a.rate, "subgrid": a.subgrid}), flush=True)

@@ -105,7 +105,13 @@ typedef struct frehd_config {
      * (4096^2 at r = 4 and 450 levels: 600 GB of tables, 2 GB of pixels).
      * No counterpart in the reference, which reads tables from files. */
     int32_t subgridOnTheFly;
-    int32_t pad2;
+    /* wind stress (windTerm, nsfunctions.c:609-648; Config.useWind,
+     * northAngle, rhoa, Cw, CwT): the series are frehd_forcing.windspd /
+     * winddir; with useThinLayer the stress decays as exp(CwT (h - hD) / hD)
+     * on faces shallower than hD */
+    int32_t useWind;
+    double  northAngle;      /* added to the wind direction [deg]                 */
+    double  rhoa, Cw, CwT;   /* air density, wind drag coefficient, thin-layer rate */
 } frehd_config;
 
 /* ---- forcing ----------------------------------------------------------------
@@ -119,6 +125,8 @@ typedef struct frehd_forcing {
     const double  *tideP, *tideM, *inflow;
     const double  *tidalPS, *tidalMS;   /* used when useConstTide?S == 0          */
     const double  *manningCell;
+    const double  *windspd, *winddir;   /* useWind == 1: speed [m/s], direction
+                                           from north [deg] (bc->windspd/winddir) */
 } frehd_forcing;
 
 /* ---- subgrid tables (initAllSubVar, subgrid.c:2112-2284) --------------------

@@ -5,9 +5,10 @@
  * library, never part of it.  Build: oracle/Makefile (gcc -O2
  * -ffp-contract=off, so no fused multiply-add changes a rounding).
  *
- * Scope: the plain-grid path (useSubgrid = 0) with first-order upwind scalar
- * transport, Manning or constant drag, optional thin-layer drag, no wind, no
- * evaporation/rain, one strip (np = 1, irank = 0 = nrank - 1).
+ * Scope: the plain-grid and subgrid paths with first-order upwind or TVD
+ * superbee scalar transport, Manning or constant drag, optional thin-layer
+ * drag and wind stress, no evaporation/rain (their flags are never set by
+ * the reference's reader), one strip (np = 1, irank = 0 = nrank - 1).
  *
  * Order of floating-point operations follows the reference expression by
  * expression (C evaluates a*b*c as (a*b)*c and a+b+c as (a+b)+c), so the
@@ -44,7 +45,7 @@ struct fo_state {
     int tt;
     /* forcing (owned copies) */
     int32_t *tideLocP, *tideLocM, *inflowLoc;
-    double *tideP, *tideM, *inflow, *tidalPS, *tidalMS, *manningCell;
+    double *tideP, *tideM, *inflow, *tidalPS, *tidalMS, *manningCell, *windspd, *winddir;
     /* Ng arrays */
     double *surf, *surfOld, *uuXP, *vvYP, *Fuu, *Fvv, *depthXP, *depthYP;
     double *EnXP, *EnYP, *DragXP, *DragYP, *CDXP, *CDYP, *wtfXP, *wtfYP;
@@ -505,7 +506,26 @@ static void phase_source(fo_state *s)
         s->EnYP[c] = s->EnYP[c] + dt * dfy;
     }
 
-    /* windTerm: useWind = 0 (nsfunctions.c:609-648 not entered) */
+    /* windTerm, nsfunctions.c:609-648 (the plain term on the subgrid path too,
+       nssolve.c:131), with the reference's pi and water density */
+    if (k->useWind == 1) {
+        const double pi = 3.1415926, rho = 1000;
+        const double phi = s->winddir[s->tt] + k->northAngle;
+        const double omega = phi * pi / 180.0;
+        for (int c = 0; c < N; c++) {
+            double tau = k->rhoa * k->Cw *
+                         (s->windspd[s->tt] - u[c] * cos(omega) - v[c] * sin(omega)) *
+                         (s->windspd[s->tt] - u[c] * cos(omega) - v[c] * sin(omega));
+            double tauXP = (s->depthXP[c] < k->hD && k->useThinLayer == 1)
+                               ? tau * exp(k->CwT * (s->depthXP[c] - k->hD) / k->hD) : tau;
+            double tauYP = (s->depthYP[c] < k->hD && k->useThinLayer == 1)
+                               ? tau * exp(k->CwT * (s->depthYP[c] - k->hD) / k->hD) : tau;
+            if (s->depthXP[c] > 0)
+                s->EnXP[c] = s->EnXP[c] + dt * tauXP * cos(omega) / (s->depthXP[c] * rho);
+            if (s->depthYP[c] > 0)
+                s->EnYP[c] = s->EnYP[c] - dt * tauYP * sin(omega) / (s->depthYP[c] * rho);
+        }
+    }
 
     /* dragTerm, nsfunctions.c:651-688 / dragTermSub (staggeredV = 1),
        subgrid.c:857-927 */
@@ -1231,6 +1251,10 @@ fo_state *fo_create(const frehd_config *cfg, const frehd_forcing *f, const frehd
     s->tidalPS = (double *)dup_arr(f->tidalPS, ser);
     s->tidalMS = (double *)dup_arr(f->tidalMS, ser);
     s->manningCell = (double *)dup_arr(f->manningCell, sizeof(double) * (size_t)N);
+    s->windspd = (double *)dup_arr(cfg->useWind == 1 ? f->windspd : NULL, ser);
+    s->winddir = (double *)dup_arr(cfg->useWind == 1 ? f->winddir : NULL, ser);
+    if (!s->windspd) s->windspd = zalloc(cfg->Nt + 1);
+    if (!s->winddir) s->winddir = zalloc(cfg->Nt + 1);
     if (!s->inflow) s->inflow = zalloc(cfg->Nt + 1);
     if (!s->tideP) s->tideP = zalloc(cfg->Nt + 1);
     if (!s->tideM) s->tideM = zalloc(cfg->Nt + 1);
@@ -1326,7 +1350,7 @@ void fo_destroy(fo_state *s)
         s->wtfYP, s->CFLx, s->CFLy, s->advX, s->advY, s->bottomZ, s->bottomXP, s->bottomYP,
         s->uuYP, s->vvXP, s->depth, s->cellV, s->rmvCFL, s->GnCt, s->GnXP, s->GnXM, s->GnYP,
         s->GnYM, s->z, s->S, s->Sm, s->tideP, s->tideM, s->inflow, s->tidalPS, s->tidalMS,
-        s->manningCell};
+        s->manningCell, s->windspd, s->winddir};
     for (size_t q = 0; q < sizeof(arrs) / sizeof(arrs[0]); q++) free(arrs[q]);
     double *subs[] = {s->allSurf, s->edgeX, s->edgeY, s->tV, s->tZ, s->tVxp, s->tVxm, s->tVyp,
         s->tVym, s->tNp, s->tNm, s->tOp, s->tOm, s->tminVxp, s->tminVxm, s->tminVyp, s->tminVym,

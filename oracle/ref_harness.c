@@ -137,7 +137,8 @@ static void fill_config(Config *s, const frehd_config *c)
     s->useConstTidePS = c->useConstTidePS; s->useConstTideMS = c->useConstTideMS;
     s->tidePS = c->tidePS; s->tideMS = c->tideMS; s->inflowS = c->inflowS;
     s->Kx = c->Kx; s->Ky = c->Ky;
-    s->useSubgrid = c->useSubgrid; s->useMPI = 0; s->np = 1; s->useWind = 0;
+    s->useSubgrid = c->useSubgrid; s->useMPI = 0; s->np = 1; s->useWind = c->useWind;
+    s->northAngle = c->northAngle; s->rhoa = c->rhoa; s->Cw = c->Cw; s->CwT = c->CwT;
     s->staggeredV = 1; s->useminA = 0; s->useminV = 1; s->useSubDrag = 0; s->beta = 0;
     s->dsurf = c->dsurf; s->surfmin = c->surfmin; s->surfmax = c->surfmin + c->Ne * c->dsurf;
     s->subR = c->subR; s->dxf = c->dx / c->subR; s->dyf = c->dy / c->subR;
@@ -219,6 +220,8 @@ void *ref_create(const frehd_config *c, const frehd_forcing *f, const frehd_ic *
         if (f->inflow) bc->inflow[t] = f->inflow[t];
         if (f->tidalPS) bc->tidalPS[t] = f->tidalPS[t];
         if (f->tidalMS) bc->tidalMS[t] = f->tidalMS[t];
+        if (c->useWind == 1 && f->windspd) bc->windspd[t] = f->windspd[t];
+        if (c->useWind == 1 && f->winddir) bc->winddir[t] = f->winddir[t];
     }
 
     h->sub = (Sub *)calloc(1, sizeof(Sub));

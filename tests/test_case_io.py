@@ -173,3 +173,56 @@ def test_loaded_deck_runs_like_reference(lib_built, oracle_built):
         a, b = o.field(n).reshape(-1), r.field(n).reshape(-1)
         m = min(len(a), len(b))
         assert np.array_equal(a[:m].view(np.int64), b[:m].view(np.int64)), n
+
+
+def wind_deck(tmp_path):
+    """The sample deck with the wind switched on (north angle 15 deg) and
+    windspd.dat / winddir.dat in the tide files' format (copies of tideP.dat
+    with the direction scaled, 721 samples each as data.dat says)."""
+    deck = tmp_path / "wdeck"
+    shutil.copytree(DECK, deck)
+    out = []
+    for ln in (deck / "data.dat").read_text().splitlines():
+        if ln.strip().endswith("# If wind boundary condition exists"):
+            ln = "1;" + ln[2:]
+        elif ln.strip().endswith("# North angle"):
+            ln = "15;" + ln[3:]
+        out.append(ln)
+    (deck / "data.dat").write_text("\n".join(out) + "\n")
+    shutil.copy(deck / "input" / "tideP.dat", deck / "input" / "windspd.dat")
+    rows = [ln.split() for ln in (deck / "input" / "tideP.dat").read_text().splitlines() if ln.strip()]
+    (deck / "input" / "winddir.dat").write_text("".join(f"{t} {float(v) * 900.0:.6f}\n" for t, v in rows))
+    return deck
+
+
+def test_wind_deck_settings_and_series(lib_built, tmp_path):
+    """Wind settings come from data.dat entries 32-35 and 77-79
+    (configuration.c:63-66, 117-119); the wind series go through the same
+    reader as the tides (readBC, initialize.c:362-366), so a copy of tideP.dat
+    read as windspd.dat gives tideP's series bit for bit."""
+    deck = wind_deck(tmp_path)
+    c = Case(str(deck / "data.dat"), str(deck))
+    k = c.cfg
+    assert (k.useWind, k.northAngle, k.CwT, k.rhoa, k.Cw) == (1, 15.0, 10.0, 1.25, 0.0013)
+    sc = c.scenario()
+    assert np.array_equal(sc.windspd.view(np.int64), sc.tideP.view(np.int64))
+    # the direction file holds 900 x tide to three more digits: the same
+    # interpolation of values scaled by 900 (to the file's rounding)
+    assert np.allclose(sc.winddir, 900.0 * sc.tideP, rtol=1e-6, atol=1e-6)
+    assert np.ptp(sc.winddir) > 10.0
+
+
+@pytest.mark.skipif(not have_ref(), reason="reference checkout / oracle/_ref not present")
+def test_wind_deck_runs_like_reference(lib_built, oracle_built, tmp_path):
+    import pyoracle
+    deck = wind_deck(tmp_path)
+    c = Case(str(deck / "data.dat"), str(deck))
+    sc = c.scenario()
+    o = pyoracle.Oracle(sc)
+    r = pyoracle.Reference(sc)
+    for _ in range(30):
+        assert o.step(1) == 0 and r.step(1) == 0
+    for n in ("surf", "uuXP", "vvYP", "EnXP", "EnYP", "S", "depth"):
+        a, b = o.field(n).reshape(-1), r.field(n).reshape(-1)
+        m = min(len(a), len(b))
+        assert np.array_equal(a[:m].view(np.int64), b[:m].view(np.int64)), n

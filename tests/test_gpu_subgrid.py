@@ -32,9 +32,17 @@ def tvd_sub():
     return sc
 
 
+def wind_sub():
+    """Subgrid marsh under wind (windTerm also runs on the subgrid path); no
+    thin layer, so the predictor is bit-exact (no exp)."""
+    sc = cases.with_wind(cases.marsh_subgrid(n=28, r=4, Nt=300, period=1200.0), speed=12.0)
+    sc.cfg["useThinLayer"] = 0
+    return sc
+
+
 def sub_cases():
     return [cases.marsh_subgrid(n=32, r=4, Nt=300, period=1200.0),
-            cases.marsh_subgrid(n=24, r=8, Nt=300, period=1200.0, manning_cell=False), tvd_sub()]
+            cases.marsh_subgrid(n=24, r=8, Nt=300, period=1200.0, manning_cell=False), tvd_sub(), wind_sub()]
 
 
 @pytest.fixture(scope="module")

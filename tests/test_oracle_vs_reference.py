@@ -39,6 +39,11 @@ CASES = [
     ("basin_tvd", lambda: cases.basin(nx=30, ny=26, Nt=40), {"scalarAdv": 2}),
     ("marsh_tvd", marsh_fast, {"scalarAdv": 2}),
     ("marsh_subgrid4_tvd", marsh_sub, {"scalarAdv": 2}),
+    # wind stress (nsfunctions.c:609-648), with the thin-layer decay on faces
+    # shallower than hD
+    ("basin_wind", lambda: cases.with_wind(cases.basin(nx=30, ny=26, Nt=40)), {}),
+    ("marsh_wind", lambda: cases.with_wind(marsh_fast()), {"hD": 0.3}),
+    ("marsh_subgrid4_wind", lambda: cases.with_wind(marsh_sub()), {}),
 ]
 
 
