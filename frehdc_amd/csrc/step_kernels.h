@@ -129,9 +129,13 @@ struct StepArgs {
     FineBed bed;
 };
 
-__host__ __device__ inline size_t tab_index(size_t c, int k, int Ne, int T)
+// value T of cell c at level k: levels outermost, the ten values of a cell
+// adjacent, cells in order within a level.  A wave's cells sit at about the
+// same level (the water surface is smooth), so a lookup's two levels are
+// two contiguous runs of 80 bytes per cell.
+__host__ __device__ inline size_t tab_index(size_t c, int k, size_t N, int T)
 {
-    return (c * (size_t)Ne + (size_t)k) * 10 + (size_t)T;
+    return ((size_t)k * N + c) * 10 + (size_t)T;
 }
 
 // One Jacobi-PCG solve over device vectors of type T (double for the FP64

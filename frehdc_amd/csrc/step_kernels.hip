@@ -880,7 +880,6 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
     const int N = g.N, Ne = p.Ne;
     const double dsurf = p.dsurf;
     const double *as = a.allSurf;
-    (void)N;
     int ind = a.ind[c];
     while (fabs(as[ind] - h) > 0.5 * dsurf + dsurf / 100) {
         if (h > as[ind]) ind += 1;
@@ -896,7 +895,7 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
     // evaluated from the cell's pixels
     double t1[10], t2[10];
     if (a.tabs) {
-        const double *p1 = a.tabs + tab_index(c, j1, Ne, 0), *p2 = a.tabs + tab_index(c, j2, Ne, 0);
+        const double *p1 = a.tabs + tab_index(c, j1, N, 0), *p2 = a.tabs + tab_index(c, j2, N, 0);
 #pragma unroll
         for (int q = 0; q < 10; q++) {
             t1[q] = p1[q];

@@ -1,7 +1,7 @@
 // subgrid_tables.hip -- the subgrid level tables generated on the device from
 // a fine bathymetry of r x r pixels per coarse cell (the values of each
 // level: cell_levels in subgrid_eval.h).  Tables are written interleaved,
-// tabs[(c*Ne + k)*10 + T] (the device layout of the step kernels),
+// tabs[(k*N + c)*10 + T] (tab_index, the device layout of the step kernels),
 // c = j*nx + i.  One thread per coarse cell walks a chunk of levels; its
 // pixels stay in L1/L2 across levels.
 //
@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(256) k_subgrid_tables(TableGenArgs a)
         const double h[1] = {level_raw(a.surfmin, a.dsurf, k)};
         double v[1][10];
         cell_levels<1>(bed, i, j, h, v);
-        double *o = a.tabs + tab_index(c, k, a.Ne, 0);
+        double *o = a.tabs + tab_index(c, k, (size_t)a.nx * a.ny, 0);
 #pragma unroll
         for (int t = 0; t < 10; t++) o[t] = v[0][t];
     }
@@ -46,13 +46,11 @@ __global__ void __launch_bounds__(256) k_tables_reorder(double *tabs, double *lm
 {
     const size_t total = (size_t)N * Ne;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
-        // q walks the interleaved order (cell-major), so its stores are the
-        // near-contiguous side
-        const size_t c = q / Ne;
-        const int k = (int)(q - c * Ne);
-        const size_t l = (size_t)k * N + c;
-        if (toInterleaved) tabs[tab_index(c, k, Ne, T)] = lm[l];
-        else lm[l] = tabs[tab_index(c, k, Ne, T)];
+        // q = k*N + c on both sides
+        const int k = (int)(q / N);
+        const size_t c = q - (size_t)k * N;
+        if (toInterleaved) tabs[tab_index(c, k, N, T)] = lm[q];
+        else lm[q] = tabs[tab_index(c, k, N, T)];
     }
 }
 
