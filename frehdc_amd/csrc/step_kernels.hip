@@ -730,14 +730,18 @@ __device__ inline void scalar_body(const StepArgs &a, int q)
                         (p.Ky * dx * a.depthYP[c] / dy) * (Sn - Sc) - (p.Ky * dx * a.depthYP[s] / dy) * (Sc - Ss));
     // connectivity with the previous-step surface (scalar.c:292-341); the
     // subgrid tests face areas instead of face depths (subgrid.c:1588-1630)
+    // (every operand loaded up front, so the loads overlap instead of
+    // waiting on each test)
     const double *ho = a.surfOld, *zb = a.bottomZ;
     const double *fx = SUB ? a.Nx : a.depthXP, *fy = SUB ? a.Oy : a.depthYP;
+    const double fxc = fx[c], fxw = fx[w], fyc = fy[c], fys = fy[s];
+    const double dE = ho[e] - zb[e], dW = ho[w] - zb[w], dN = ho[n] - zb[n], dS = ho[s] - zb[s];
     double vals[4];
     int m = 0;
-    if (fx[c] > 0 && ho[e] - zb[e] > 0) vals[m++] = Se;
-    if (fx[w] > 0 && ho[w] - zb[w] > 0) vals[m++] = Sw;
-    if (n <= g.N && fy[c] > 0 && ho[n] - zb[n] > 0) vals[m++] = Sn;
-    if (s <= g.N && fy[s] > 0 && ho[s] - zb[s] > 0) vals[m++] = Ss;
+    if (fxc > 0 && dE > 0) vals[m++] = Se;
+    if (fxw > 0 && dW > 0) vals[m++] = Sw;
+    if (n <= g.N && fyc > 0 && dN > 0) vals[m++] = Sn;
+    if (s <= g.N && fys > 0 && dS > 0) vals[m++] = Ss;
     double smax = 0, smin = 0;
     if (m) {
         smax = vals[0];
@@ -1111,15 +1115,20 @@ __device__ inline FaceVel face_vel_sub(const StepArgs &a, int c)
     if (Oyc == 0) vc = 0;
     // no outflow from a dry cell: the face's own cell stops positive flow,
     // the cell beyond stops negative flow
+    // (the neighbours' depth and rmv flags loaded up front, not behind the
+    // velocity tests; depth and rmv hold interior cells only)
     const double *dn = depth_nxt(a);
-    const bool dryC = dn[c] == 0;
+    const bool eIn = e < g.N, nIn = n < g.N;
+    const double dC = dn[c], dE = eIn ? dn[e] : 1.0, dN = nIn ? dn[n] : 1.0;
+    const uint8_t rC = a.rmv[c], rE = eIn ? a.rmv[e] : 0, rN = nIn ? a.rmv[n] : 0;
+    const bool dryC = dC == 0;
     if (dryC && uc > 0) uc = 0;
-    if (e < g.N && dn[e] == 0 && uc < 0) uc = 0;
+    if (eIn && dE == 0 && uc < 0) uc = 0;
     if (dryC && vc > 0) vc = 0;
-    if (n < g.N && dn[n] == 0 && vc < 0) vc = 0;
-    const bool rmvC = a.rmv[c] == 1;
-    if (rmvC || (e < g.N && a.rmv[e] == 1)) uc = 0;
-    if (rmvC || (n < g.N && a.rmv[n] == 1)) vc = 0;
+    if (nIn && dN == 0 && vc < 0) vc = 0;
+    const bool rmvC = rC == 1;
+    if (rmvC || (eIn && rE == 1)) uc = 0;
+    if (rmvC || (nIn && rN == 1)) vc = 0;
     const double CFLmax = 0.24 / p.subR, Cw = 0.7;
     if (fX != 0) {
         const double vel0 = uc;
