@@ -110,7 +110,7 @@ class Scenario:
         ic.V0cell = abi.dptr(self.V0cell)
         if self.subgrid is not None:
             sg = abi.FrehdSubgrid()
-            for k in ("edgeX", "edgeY", *abi.SUB_TABLES):
+            for k in ("edgeX", "edgeY", *abi.SUB_TABLES, *abi.DRAG_TABLES):
                 a = self.subgrid.get(k)
                 setattr(sg, k, abi.dptr(a) if a is not None else None)
             if "fine" in self.subgrid:
@@ -132,6 +132,7 @@ def default_config(nx, ny, dx, dt, Nt, **kw) -> dict:
         useConstTideMS=1, nScalar=1, tidePS=0.0, tideMS=0.0, inflowS=0.0, Kx=1e-4, Ky=1e-4,
         offset=0.0, epsInner=1e-5, maxRefine=8, warmStart=0, useSubgrid=0, Ne=0, surfmin=0.0, dsurf=0.01,
         subR=1.0, subgridOnTheFly=0, useWind=0, northAngle=0.0, rhoa=1.225, Cw=1.3e-3, CwT=1.0,
+        useSubDrag=0,
     )
     cfg.update(kw)
     return cfg
@@ -380,6 +381,34 @@ def with_wind(scen: Scenario, speed: float = 8.0, gust: float = 3.0, direction: 
     scen.winddir = np.ascontiguousarray(direction + veer * np.sin(math.pi * t / period))
     scen.cfg.update(useWind=1, northAngle=float(northAngle))
     scen.name = scen.name + "_wind"
+    return scen
+
+
+def with_subdrag(scen: Scenario, model: int, seed: int = 11) -> Scenario:
+    """Switch a table-based subgrid scenario to a subgrid drag model
+    (updateCDSub, subgrid.c:1481-1519): 2 (Volp 2013) with YhX/YhY tables, 3
+    or 4 (Casas 2010) with a Cn table.  The tables are synthetic but shaped
+    like the real ones, level-major [k*N + c]: Yh falls with the water depth
+    above the level (a depth factor that the CD cap meets on shallow levels),
+    Cn is a per-cell roughness in [0.015, 0.05] that grows on dry levels."""
+    sg, k = scen.subgrid, scen.cfg
+    assert sg is not None and "V" in sg, "needs level tables (not device tables)"
+    N, Ne = scen.N, k["Ne"]
+    rng = np.random.default_rng(seed)
+    lev = (k["surfmin"] + k["dsurf"] * np.arange(Ne))[:, None]
+    depth = np.maximum(lev - scen.bottomZ[None, :], 0.0)
+    if model == 2:
+        for name in ("YhX", "YhY"):
+            f = rng.uniform(0.8, 1.25, size=N)[None, :]
+            yh = np.where(depth > 0, f / np.cbrt(np.maximum(depth, 0.05)), 0.0)
+            sg[name] = np.ascontiguousarray(yh.reshape(-1))
+    elif model in (3, 4):
+        base = rng.uniform(0.015, 0.04, size=N)[None, :]
+        sg["Cn"] = np.ascontiguousarray(np.where(depth > 0, base, base + 0.01).reshape(-1))
+    else:
+        raise ValueError("useSubDrag 2, 3 or 4")
+    k["useSubDrag"] = int(model)
+    scen.name += f"_subdrag{model}"
     return scen
 
 

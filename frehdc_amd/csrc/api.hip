@@ -358,6 +358,17 @@ size_t layout(frehd_dev *d, char *base)
         } else {
             a.tabs = cv.take<double>((size_t)k.Ne * N * 10);
         }
+        a.allYhX = a.allYhY = a.allCn = nullptr;
+        a.YhX = a.YhY = a.Cn = nullptr;
+        if (k.useSubDrag == 2) {
+            a.allYhX = cv.take<double>((size_t)k.Ne * N);
+            a.allYhY = cv.take<double>((size_t)k.Ne * N);
+            a.YhX = cv.take<double>(N);
+            a.YhY = cv.take<double>(N);
+        } else if (k.useSubDrag == 3 || k.useSubDrag == 4) {
+            a.allCn = cv.take<double>((size_t)k.Ne * N);
+            a.Cn = cv.take<double>(N);
+        }
     }
     return cv.off + 256;
 }
@@ -384,6 +395,12 @@ int check_config(const frehd_config *k)
     if (k->precision != FREHD_PREC_FP64 && k->precision != FREHD_PREC_MIXED)
         return fail(FREHD_EINVAL, "unknown precision %d", k->precision);
     if (k->useWind != 0 && k->useWind != 1) return fail(FREHD_EINVAL, "useWind must be 0 or 1");
+    if (k->useSubDrag != 0 && k->useSubDrag != 2 && k->useSubDrag != 3 && k->useSubDrag != 4)
+        return fail(FREHD_EINVAL, "useSubDrag must be 0, 2, 3 or 4 (1 reads curvature tables the reference "
+                                  "never fills)");
+    if (k->useSubDrag != 0 && (k->useSubgrid != 1 || k->subgridOnTheFly != 0))
+        return fail(FREHD_EINVAL, "useSubDrag needs the subgrid with level tables (useSubgrid = 1, "
+                                  "subgridOnTheFly = 0)");
     if (k->precision == FREHD_PREC_MIXED && !(k->epsInner > 0 && k->epsInner < 1 && k->maxRefine >= 1))
         return fail(FREHD_EINVAL, "mixed precision needs 0 < epsInner < 1 and maxRefine >= 1");
     if (k->precision == FREHD_PREC_MIXED && (k->warmStart < 0 || k->warmStart > 2))
@@ -443,6 +460,7 @@ void fill_params(P &p, const frehd_config &k)
     p.useConstTidePS = k.useConstTidePS; p.useConstTideMS = k.useConstTideMS; p.maxIter = k.maxIter;
     p.scalarAdv = k.scalarAdv;
     p.useWind = k.useWind;
+    p.useSubDrag = k.useSubgrid == 1 ? k.useSubDrag : 0;
     p.rhoa = k.rhoa; p.Cw = k.Cw; p.CwT = k.CwT;
     p.useSubgrid = k.useSubgrid; p.dsurf = k.dsurf; p.subR = k.subR; p.Ne = k.Ne; p.surfmin = k.surfmin;
     p.warmStart = k.precision == FREHD_PREC_MIXED ? k.warmStart : 0;
@@ -632,6 +650,14 @@ int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
     if ((rc = upload(d, ex.data(), (void *)a.edgeX, N * sizeof(double)))) return rc;
     if ((rc = upload(d, ey.data(), (void *)a.edgeY, N * sizeof(double)))) return rc;
     if ((rc = upload(d, lev.data(), (void *)a.allSurf, k.Ne * sizeof(double)))) return rc;
+    if (k.useSubDrag == 2) {
+        if (!sg->YhX || !sg->YhY) return fail(FREHD_EINVAL, "useSubDrag = 2 needs the YhX and YhY tables");
+        if ((rc = upload(d, sg->YhX, (void *)a.allYhX, tn * sizeof(double)))) return rc;
+        if ((rc = upload(d, sg->YhY, (void *)a.allYhY, tn * sizeof(double)))) return rc;
+    } else if (k.useSubDrag == 3 || k.useSubDrag == 4) {
+        if (!sg->Cn) return fail(FREHD_EINVAL, "useSubDrag = %d needs the Cn table", k.useSubDrag);
+        if ((rc = upload(d, sg->Cn, (void *)a.allCn, tn * sizeof(double)))) return rc;
+    }
     if (onTheFly) {
         // the argument block went to the device before the pixel size was
         // known: send it again

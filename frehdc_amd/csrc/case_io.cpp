@@ -33,6 +33,7 @@ struct frehd_case {
     std::vector<double> allZ, allEdgeX, allEdgeY;
     std::vector<double> surf0, U0, V0, S0;
     std::vector<double> tab[14];   // V Z Vxp Vxm Vyp Vym Np Nm Op Om minVxp minVxm minVyp minVym
+    std::vector<double> drag[3];   // YhX YhY (useSubDrag 2), Cn (3, 4)
     frehd_subgrid sub;
 };
 
@@ -234,7 +235,7 @@ int frehd_case_load(const char *data_dat, const char *base_dir, frehd_case_t **o
     c->useConstV0 = (int)a[52];
     c->useConstInitS = (int)a[53];
     k.useSubgrid = (int)a[54];
-    const int useSubDrag = (int)a[55];
+    k.useSubDrag = (int)a[55];
     const double dxf = a[56];
     const double surfmax = a[58];
     k.surfmin = a[59];
@@ -276,8 +277,11 @@ int frehd_case_load(const char *data_dat, const char *base_dir, frehd_case_t **o
         return rc;
     };
     if (NX < 1 || NY < 1) return bad(FREHD_EINVAL, "NX and NY must be positive");
-    if (k.useSubgrid == 1 && (useSubDrag != 0 || staggeredV != 1 || useminA != 0))
-        return bad(FREHD_EINVAL, "subgrid needs useSubDrag = 0, staggeredV = 1, useminA = 0");
+    if (k.useSubgrid == 1 && (staggeredV != 1 || useminA != 0))
+        return bad(FREHD_EINVAL, "subgrid needs staggeredV = 1, useminA = 0");
+    if (k.useSubgrid == 1 && k.useSubDrag != 0 && k.useSubDrag != 2 && k.useSubDrag != 3 && k.useSubDrag != 4)
+        return bad(FREHD_EINVAL, "useSubDrag must be 0, 2, 3 or 4");
+    if (k.useSubgrid != 1) k.useSubDrag = 0;
     if (k.useSubgrid == 0 && c->useCellEdge == 1)
         return bad(FREHD_EINVAL, "cell-edge bathymetry on the plain grid is not supported");
     if (c->info.OutItvl < 1) return bad(FREHD_EINVAL, "output interval shorter than one step");
@@ -343,6 +347,18 @@ int frehd_case_load(const char *data_dat, const char *base_dir, frehd_case_t **o
         const double **dst[14] = {&s.V, &s.Z, &s.Vxp, &s.Vxm, &s.Vyp, &s.Vym, &s.Np, &s.Nm, &s.Op, &s.Om,
                                   &s.minVxp, &s.minVxm, &s.minVyp, &s.minVym};
         for (int t = 0; t < ntab; t++) *dst[t] = c->tab[t].data();
+        // drag-model tables (subgrid.c:2206-2225)
+        const char *dnames[3] = {"YhX.dat", "YhY.dat", "Cn.dat"};
+        const double **ddst[3] = {&s.YhX, &s.YhY, &s.Cn};
+        for (int t = 0; t < 3; t++) {
+            const bool need = k.useSubDrag == 2 ? t < 2 : ((k.useSubDrag == 3 || k.useSubDrag == 4) && t == 2);
+            if (!need) continue;
+            if ((rc = read_values(join(c->inputFolder, c->subgridFolder + dnames[t]), N * (size_t)k.Ne, c->drag[t]))) {
+                delete c;
+                return rc;
+            }
+            *ddst[t] = c->drag[t].data();
+        }
     }
     *out = c;
     return 0;

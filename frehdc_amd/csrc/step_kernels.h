@@ -19,6 +19,7 @@ struct P {
     int scalarAdv;            // 1 first-order upwind, 2 TVD superbee
     int useWind;
     double rhoa, Cw, CwT;     // wind stress (windTerm)
+    int useSubDrag;           // 0 Manning h = V/Z, 2 Volp (Yh), 3/4 Casas (Cn)
     // subgrid
     double dsurf, subR, surfmin;   // surfmin: raw elevation of level 0
     int Ne;
@@ -124,6 +125,10 @@ struct StepArgs {
     int *ind;
     const double *edgeX, *edgeY, *allSurf;
     const double *tabs;
+    // drag-model tables (level-major [k*N + c]) and the values at the lower
+    // bracketing level of each cell (useSubDrag 2: Yh; 3, 4: Cn)
+    const double *allYhX, *allYhY, *allCn;
+    double *YhX, *YhY, *Cn;
     // instead of tabs (NULL): the fine bathymetry, from which a lookup
     // evaluates its two levels (cell_levels, bitwise the table values)
     FineBed bed;

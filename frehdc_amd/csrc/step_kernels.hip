@@ -894,6 +894,13 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
     a.ind[c] = ind;
     const int j1 = ind;
     const int j2 = j1 == 0 ? 0 : (j1 == Ne - 1 ? Ne - 1 : (h > as[j1] ? j1 + 1 : j1 - 1));
+    // drag-model values at the lower level (subgrid.c:121-127)
+    if (p.useSubDrag == 2) {
+        a.YhX[c] = a.allYhX[(size_t)j1 * N + c];
+        a.YhY[c] = a.allYhY[(size_t)j1 * N + c];
+    } else if (p.useSubDrag == 3 || p.useSubDrag == 4) {
+        a.Cn[c] = a.allCn[(size_t)j1 * N + c];
+    }
     const double s1 = as[j1], s2 = as[j2];
     // the two levels' 10 values each: adjacent in the interleaved tables, or
     // evaluated from the cell's pixels
@@ -1225,7 +1232,8 @@ __global__ void k_tidal_velocity_sub(const StepArgs *__restrict__ A)
     }
 }
 
-// velocityInterp + volumeByFluxSub + updateCDSub (useSubDrag = 0: h = V/Z)
+// velocityInterp + volumeByFluxSub + updateCDSub (useSubDrag 0: h = V/Z;
+// 2: Volp; 3, 4: Casas)
 // + thinLayerDrag + CFL numbers (nsfunctions.c:337-347, subgrid.c:1227-1263,
 // 1520-1535, nsfunctions.c:691-727, 1192-1232)
 __global__ void __launch_bounds__(256) k_finish_sub(const StepArgs *__restrict__ A)
@@ -1255,7 +1263,26 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgs *__restrict__
         if (V < 0) V = 0;
         a.cellV[c] = V;
 
-        if (p.CDnotN == 0 && a.V[c] > 0) {
+        if (p.CDnotN == 0 && p.useSubDrag == 2) {
+            // Volp 2013 (subgrid.c:1481-1506): g n^2 Yh, capped, 0 where Yh <= 0
+            const double n = a.manning ? a.manning[c] : p.manningN;
+            const double coeff = p.g * n * n;
+            const double yx = a.YhX[c], yy = a.YhY[c];
+            double cx = 0.0, cy = 0.0;
+            if (yx > 0) { cx = coeff * yx; if (cx > p.CDmax) cx = p.CDmax; }
+            if (yy > 0) { cy = coeff * yy; if (cy > p.CDmax) cy = p.CDmax; }
+            a.CDXP[c] = cx;
+            a.CDYP[c] = cy;
+        } else if (p.CDnotN == 0 && (p.useSubDrag == 3 || p.useSubDrag == 4)) {
+            // Casas 2010 (subgrid.c:1508-1519): g Cn^2 / h^(1/3), h = V/(dx dy)
+            if (a.V[c] > 0) {
+                const double hh = a.V[c] / (p.dx * p.dy);
+                const double cn = a.Cn[c];
+                const double cd = p.g * cn * cn / pow(hh, 1.0 / 3.0);
+                a.CDXP[c] = cd;
+                a.CDYP[c] = cd;
+            }
+        } else if (p.CDnotN == 0 && a.V[c] > 0) {
             const double n = a.manning ? a.manning[c] : p.manningN;
             const double coeff = p.g * n * n;
             const double hh = a.V[c] / a.Z[c];

@@ -112,6 +112,15 @@ typedef struct frehd_config {
     int32_t useWind;
     double  northAngle;      /* added to the wind direction [deg]                 */
     double  rhoa, Cw, CwT;   /* air density, wind drag coefficient, thin-layer rate */
+    /* subgrid drag model (Config.useSubDrag, updateCDSub subgrid.c:1446-1535;
+     * applies with useSubgrid = 1 and CDnotN = 0): 0 Manning with h = V/Z;
+     * 2 Volp 2013, CD = g n^2 Yh capped at CDmax (frehd_subgrid.YhX/YhY);
+     * 3, 4 Casas 2010, CD = g Cn^2 / (V/(dx dy))^(1/3) (frehd_subgrid.Cn).
+     * Each takes its table's value at the lower bracketing level.  1 (the
+     * curvature model) reads Sub.CvX/CvY, which the reference never fills,
+     * and is rejected; the models need tables (subgridOnTheFly = 0). */
+    int32_t useSubDrag;
+    int32_t pad3;
 } frehd_config;
 
 /* ---- forcing ----------------------------------------------------------------
@@ -148,6 +157,10 @@ typedef struct frehd_subgrid {
     const double *fine;
     int32_t fineR, pad;
     double dxf;
+    /* drag-model tables, level-major like the others (YhX.dat, YhY.dat, Cn.dat;
+     * initAllSubVar subgrid.c:2206-2225): YhX/YhY with useSubDrag = 2, Cn with
+     * 3 or 4, NULL otherwise */
+    const double *YhX, *YhY, *Cn;
 } frehd_subgrid;
 
 /* ---- initial condition (the inputs of Init, initialize.c:40-155) ------------

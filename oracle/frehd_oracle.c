@@ -62,6 +62,7 @@ struct fo_state {
     double *allSurf, *edgeX, *edgeY;
     double *tV, *tZ, *tVxp, *tVxm, *tVyp, *tVym, *tNp, *tNm, *tOp, *tOm;
     double *tminVxp, *tminVxm, *tminVyp, *tminVym;
+    double *tYhX, *tYhY, *tCn, *YhX, *YhY, *Cn;   /* useSubDrag 2 / 3, 4 */
     double *V, *Z, *Vxp, *Vyp, *Vxm, *Vym, *Np, *Nm, *Op, *Om, *Vx, *Vy, *Zx, *Zy, *Nx, *Oy;
     double *minVxp, *minVxm, *minVyp, *minVym, *minVx, *minVy, *Nmin, *Omin, *Nmax, *Omax;
     int *ind;
@@ -315,6 +316,9 @@ static void compute_sub_area(fo_state *s)
         else j2 = h > s->allSurf[j1] ? j1 + 1 : j1 - 1;
         const size_t k1 = (size_t)N * j1 + c, k2 = (size_t)N * j2 + c;
         const double s1 = s->allSurf[j1], s2 = s->allSurf[j2];
+        /* drag-model values at the lower level, subgrid.c:121-127 */
+        if (k->useSubDrag == 2) { s->YhX[c] = s->tYhX[k1]; s->YhY[c] = s->tYhY[k1]; }
+        if (k->useSubDrag == 4 || k->useSubDrag == 3) s->Cn[c] = s->tCn[k1];
         s->V[c] = interp_surf(h, s1, s->tV[k1], s2, s->tV[k2]);
         s->Z[c] = interp_surf(h, s1, s->tZ[k1], s2, s->tZ[k2]);
         s->Vxp[c] = interp_surf(h, s1, s->tVxp[k1], s2, s->tVxp[k2]);
@@ -1017,9 +1021,34 @@ static void phase_update(fo_state *s)
         if (s->cellV[c] < 0) s->cellV[c] = 0;
     }
 
-    /* updateCD, nsfunctions.c:1170-1189 / updateCDSub (useSubDrag = 0),
-       subgrid.c:1520-1535 with h = V/Z */
-    if (k->CDnotN == 0) {
+    /* updateCD, nsfunctions.c:1170-1189 / updateCDSub, subgrid.c:1446-1535:
+       useSubDrag 2 (Volp 2013, :1481-1506), 3 and 4 (Casas 2010, :1508-1519),
+       otherwise Manning with h = V/Z (:1520-1535) */
+    if (k->CDnotN == 0 && s->sub && k->useSubDrag == 2) {
+        for (int c = 0; c < N; c++) {
+            double coeff = manning_coeff(s, c);
+            if (s->YhX[c] > 0) {
+                s->CDXP[c] = coeff * s->YhX[c];
+                if (s->CDXP[c] > k->CDmax) s->CDXP[c] = k->CDmax;
+            } else {
+                s->CDXP[c] = 0;
+            }
+            if (s->YhY[c] > 0) {
+                s->CDYP[c] = coeff * s->YhY[c];
+                if (s->CDYP[c] > k->CDmax) s->CDYP[c] = k->CDmax;
+            } else {
+                s->CDYP[c] = 0;
+            }
+        }
+    } else if (k->CDnotN == 0 && s->sub && (k->useSubDrag == 4 || k->useSubDrag == 3)) {
+        for (int c = 0; c < N; c++) {
+            if (s->V[c] > 0) {
+                double h = s->V[c] / (k->dx * k->dy);
+                s->CDXP[c] = k->g * s->Cn[c] * s->Cn[c] / pow(h, 1.0 / 3.0);
+                s->CDYP[c] = k->g * s->Cn[c] * s->Cn[c] / pow(h, 1.0 / 3.0);
+            }
+        }
+    } else if (k->CDnotN == 0) {
         for (int c = 0; c < N; c++) {
             if (s->sub) {
                 if (s->V[c] > 0) {
@@ -1319,6 +1348,12 @@ fo_state *fo_create(const frehd_config *cfg, const frehd_forcing *f, const frehd
             *tabs[q] = (double *)calloc(tn, sizeof(double));
             if (src[q]) memcpy(*tabs[q], src[q], sizeof(double) * tn);
         }
+        s->tYhX = (double *)dup_arr(cfg->useSubDrag == 2 ? sg->YhX : NULL, sizeof(double) * tn);
+        s->tYhY = (double *)dup_arr(cfg->useSubDrag == 2 ? sg->YhY : NULL, sizeof(double) * tn);
+        s->tCn = (double *)dup_arr(cfg->useSubDrag >= 3 ? sg->Cn : NULL, sizeof(double) * tn);
+        s->YhX = zalloc(Ng);
+        s->YhY = zalloc(Ng);
+        s->Cn = zalloc(Ng);
         double **subArr[] = {&s->V, &s->Z, &s->Vxp, &s->Vyp, &s->Vxm, &s->Vym, &s->Np, &s->Nm,
                              &s->Op, &s->Om, &s->Vx, &s->Vy, &s->Zx, &s->Zy, &s->Nx, &s->Oy,
                              &s->minVxp, &s->minVxm, &s->minVyp, &s->minVym, &s->minVx,
@@ -1356,7 +1391,7 @@ void fo_destroy(fo_state *s)
         s->tVym, s->tNp, s->tNm, s->tOp, s->tOm, s->tminVxp, s->tminVxm, s->tminVyp, s->tminVym,
         s->V, s->Z, s->Vxp, s->Vyp, s->Vxm, s->Vym, s->Np, s->Nm, s->Op, s->Om, s->Vx, s->Vy,
         s->Zx, s->Zy, s->Nx, s->Oy, s->minVxp, s->minVxm, s->minVyp, s->minVym, s->minVx,
-        s->minVy, s->Nmin, s->Omin, s->Nmax, s->Omax};
+        s->minVy, s->Nmin, s->Omin, s->Nmax, s->Omax, s->tYhX, s->tYhY, s->tCn, s->YhX, s->YhY, s->Cn};
     for (size_t q = 0; q < sizeof(subs) / sizeof(subs[0]); q++) free(subs[q]);
     free(s->ind);
     free(s->tideLocP); free(s->tideLocM); free(s->inflowLoc);

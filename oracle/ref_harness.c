@@ -139,7 +139,7 @@ static void fill_config(Config *s, const frehd_config *c)
     s->Kx = c->Kx; s->Ky = c->Ky;
     s->useSubgrid = c->useSubgrid; s->useMPI = 0; s->np = 1; s->useWind = c->useWind;
     s->northAngle = c->northAngle; s->rhoa = c->rhoa; s->Cw = c->Cw; s->CwT = c->CwT;
-    s->staggeredV = 1; s->useminA = 0; s->useminV = 1; s->useSubDrag = 0; s->beta = 0;
+    s->staggeredV = 1; s->useminA = 0; s->useminV = 1; s->useSubDrag = c->useSubDrag; s->beta = 0;
     s->dsurf = c->dsurf; s->surfmin = c->surfmin; s->surfmax = c->surfmin + c->Ne * c->dsurf;
     s->subR = c->subR; s->dxf = c->dx / c->subR; s->dyf = c->dy / c->subR;
     s->dA = s->dxf * s->dyf; s->z0 = c->hD;
@@ -265,6 +265,16 @@ void *ref_create(const frehd_config *c, const frehd_forcing *f, const frehd_ic *
             *tabs[q] = padded(h, (size_t)Ne * N, pad);
             if (src[q]) memcpy(*tabs[q], src[q], sizeof(double) * (size_t)Ne * N);
         }
+        /* drag-model tables (subgrid.c:2206-2225) and their per-cell values */
+        double **dtabs[] = {&sb->allYhX, &sb->allYhY, &sb->allCn};
+        const double *dsrc[] = {sg->YhX, sg->YhY, sg->Cn};
+        for (int q = 0; q < 3; q++) {
+            *dtabs[q] = padded(h, (size_t)Ne * N, pad);
+            if (dsrc[q]) memcpy(*dtabs[q], dsrc[q], sizeof(double) * (size_t)Ne * N);
+        }
+        sb->YhX = padded(h, Ng, pad);
+        sb->YhY = padded(h, Ng, pad);
+        sb->Cn = padded(h, Ng, pad);
         double **arr[] = {&sb->V, &sb->Z, &sb->Vxp, &sb->Vyp, &sb->Vxm, &sb->Vym, &sb->Np, &sb->Op,
             &sb->Nm, &sb->Om, &sb->Vx, &sb->Vy, &sb->Nx, &sb->Oy, &sb->Zx, &sb->Zy, &sb->Nmin,
             &sb->Omin, &sb->Nmax, &sb->Omax, &sb->minVyp, &sb->minVxp, &sb->minVym, &sb->minVxm,
@@ -483,6 +493,34 @@ int ref_io_subtables(const char *inputFolder, const char *subgridFolder, int NX,
                        sub->allNm, sub->allOp, sub->allOm, sub->allminVxp, sub->allminVxm, sub->allminVyp,
                        sub->allminVym};
     for (int t = 0; t < 14; t++)
+        if (out[t] && src[t]) memcpy(out[t], src[t], sizeof(double) * NX * NY * sub->Ne);
+    free(s);
+    return 0;
+}
+
+/* initAllSubVar's drag-model tables (subgrid.c:2206-2225): YhX, YhY with
+ * useSubDrag = 2, Cn with 3 or 4, level-major */
+int ref_io_dragtables(const char *inputFolder, const char *subgridFolder, int NX, int NY, double surfmin,
+                      double surfmax, double dsurf, double offset, int useSubDrag, double *out[3], int *Ne)
+{
+    Config *s = io_config(inputFolder, subgridFolder, NX, NY);
+    s->useSubgrid = 1;
+    s->surfmin = surfmin;
+    s->surfmax = surfmax;
+    s->dsurf = dsurf;
+    s->useminV = 0;
+    s->useminA = 0;
+    s->useSubDrag = useSubDrag;
+    Bath b;
+    memset(&b, 0, sizeof b);
+    double off = offset;
+    b.offset = &off;
+    Sub *sub = NULL;
+    initAllSubVar(&sub, &b, s, 0);
+    *Ne = sub->Ne;
+    double *src[3] = {useSubDrag == 2 ? sub->allYhX : NULL, useSubDrag == 2 ? sub->allYhY : NULL,
+                      useSubDrag >= 3 ? sub->allCn : NULL};
+    for (int t = 0; t < 3; t++)
         if (out[t] && src[t]) memcpy(out[t], src[t], sizeof(double) * NX * NY * sub->Ne);
     free(s);
     return 0;

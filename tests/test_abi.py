@@ -58,6 +58,21 @@ def test_error_paths_need_no_device(lib):
     cfg = abi.FrehdConfig(nx=64, ny=64, Nt=10, dx=1.0, dy=1.0, dt=1.0, bcType=1, precond=1)
     assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == abi.FREHD_EINVAL
     assert b"Jacobi" in lib.frehd_last_error()
+    # subgrid drag: model 1 reads curvature tables the reference never fills;
+    # the models need level tables, not the on-the-fly evaluation
+    sub = dict(nx=64, ny=64, Nt=10, dx=1.0, dy=1.0, dt=1.0, bcType=1, useSubgrid=1, Ne=10, dsurf=0.01,
+               subR=2.0)
+    cfg = abi.FrehdConfig(useSubDrag=1, **sub)
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == abi.FREHD_EINVAL
+    assert b"useSubDrag" in lib.frehd_last_error()
+    cfg = abi.FrehdConfig(useSubDrag=2, subgridOnTheFly=1, **sub)
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == abi.FREHD_EINVAL
+    cfg = abi.FrehdConfig(useSubDrag=2, **sub)
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
+    base = b.value
+    cfg = abi.FrehdConfig(useSubDrag=0, **sub)
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
+    assert base - b.value >= 2 * 10 * 64 * 64 * 8     # the two Yh tables
 
 
 def test_footprint_scales(lib):

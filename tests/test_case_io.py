@@ -226,3 +226,62 @@ def test_wind_deck_runs_like_reference(lib_built, oracle_built, tmp_path):
         a, b = o.field(n).reshape(-1), r.field(n).reshape(-1)
         m = min(len(a), len(b))
         assert np.array_equal(a[:m].view(np.int64), b[:m].view(np.int64)), n
+
+
+def subgrid_deck(tmp_path, subdrag=0):
+    """The reference's own deck with its shipped subgrid data (input/subdata/,
+    50 levels) switched on, optionally with a subgrid drag model."""
+    deck = tmp_path / f"sdeck{subdrag}"
+    deck.mkdir()
+    os.symlink(os.path.join(REFDIR, "input"), deck / "input")
+    out = []
+    for ln in open(os.path.join(REFDIR, "data.dat")).read().splitlines():
+        if ln.strip().endswith("# If subgrid model is used"):
+            ln = "1;" + ln[2:]
+        if ln.strip().endswith("# If effective volume is used"):
+            ln = "1;" + ln[2:]
+        if subdrag and ln.strip().endswith("# The subgrid drag model used"):
+            ln = f"{subdrag};" + ln[2:]
+        out.append(ln)
+    (deck / "data.dat").write_text("\n".join(out) + "\n")
+    return deck
+
+
+@pytest.mark.skipif(not have_ref(), reason="reference checkout / oracle/_ref not present")
+def test_volp_tables_equal_reference(lib_built, tmp_path):
+    """useSubDrag = 2 on the shipped subgrid data: our loader's YhX / YhY
+    tables equal initAllSubVar's (subgrid.c:2206-2216), and the loaded deck
+    runs through the reference's own step functions like the C restatement,
+    bit for bit."""
+    import ctypes as C
+    import pyoracle
+    deck = subgrid_deck(tmp_path, 2)
+    c = Case(str(deck / "data.dat"), str(deck))
+    k = c.cfg
+    assert k.useSubgrid == 1 and k.useSubDrag == 2 and k.Ne == 50
+    N = k.nx * k.ny
+    sc = c.scenario()
+    lib = C.CDLL(pyoracle.REF_LIB)
+    P = C.POINTER(C.c_double)
+    outs = [np.zeros(N * 50) for _ in range(3)]
+    arr = (P * 3)(*[o.ctypes.data_as(P) for o in outs])
+    ne = C.c_int()
+    cwd = os.getcwd()
+    os.chdir(deck)
+    try:
+        lib.ref_io_dragtables(b"input/", b"subdata/", k.nx, k.ny, C.c_double(0.25), C.c_double(0.75),
+                              C.c_double(0.01), C.c_double(k.offset), 2, arr, C.byref(ne))
+    finally:
+        os.chdir(cwd)
+    assert ne.value == 50
+    for t, name in enumerate(("YhX", "YhY")):
+        assert np.array_equal(sc.subgrid[name].view(np.int64), outs[t].view(np.int64)), name
+    assert np.any(sc.subgrid["YhX"] > 0)
+    o = pyoracle.Oracle(sc)
+    r = pyoracle.Reference(sc)
+    for _ in range(20):
+        assert o.step(1) == 0 and r.step(1) == 0
+    for n in ("surf", "uuXP", "vvYP", "CDXP", "CDYP", "depth"):
+        a, b = o.field(n).reshape(-1), r.field(n).reshape(-1)
+        m = min(len(a), len(b))
+        assert np.array_equal(a[:m].view(np.int64), b[:m].view(np.int64)), n

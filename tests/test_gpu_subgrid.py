@@ -40,9 +40,22 @@ def wind_sub():
     return sc
 
 
+def volp_sub():
+    """Volp 2013 subgrid drag (useSubDrag = 2) with the CD cap active."""
+    sc = cases.with_subdrag(cases.marsh_subgrid(n=28, r=4, Nt=300, period=1200.0), 2)
+    sc.cfg["CDmax"] = 0.02
+    return sc
+
+
+def casas_sub():
+    """Casas 2010 subgrid drag (useSubDrag = 4)."""
+    return cases.with_subdrag(cases.marsh_subgrid(n=28, r=4, Nt=300, period=1200.0, manning_cell=False), 4)
+
+
 def sub_cases():
     return [cases.marsh_subgrid(n=32, r=4, Nt=300, period=1200.0),
-            cases.marsh_subgrid(n=24, r=8, Nt=300, period=1200.0, manning_cell=False), tvd_sub(), wind_sub()]
+            cases.marsh_subgrid(n=24, r=8, Nt=300, period=1200.0, manning_cell=False), tvd_sub(), wind_sub(),
+            volp_sub(), casas_sub()]
 
 
 @pytest.fixture(scope="module")

@@ -44,6 +44,11 @@ CASES = [
     ("basin_wind", lambda: cases.with_wind(cases.basin(nx=30, ny=26, Nt=40)), {}),
     ("marsh_wind", lambda: cases.with_wind(marsh_fast()), {"hD": 0.3}),
     ("marsh_subgrid4_wind", lambda: cases.with_wind(marsh_sub()), {}),
+    # subgrid drag models (updateCDSub, subgrid.c:1481-1519): Volp 2013 with
+    # the CD cap active on shallow levels, Casas 2010 (both model numbers)
+    ("marsh_subgrid4_volp", lambda: cases.with_subdrag(marsh_sub(), 2), {"CDmax": 0.02}),
+    ("marsh_subgrid4_casas3", lambda: cases.with_subdrag(marsh_sub(), 3), {}),
+    ("marsh_subgrid4_casas4", lambda: cases.with_subdrag(marsh_sub(), 4), {"useThinLayer": 0}),
 ]
 
 
