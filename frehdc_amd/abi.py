@@ -37,7 +37,8 @@ class FrehdConfig(C.Structure):
         ("surfmin", C.c_double), ("dsurf", C.c_double), ("subR", C.c_double),
         ("subgridOnTheFly", C.c_int32), ("useWind", C.c_int32),
         ("northAngle", C.c_double), ("rhoa", C.c_double), ("Cw", C.c_double), ("CwT", C.c_double),
-        ("useSubDrag", C.c_int32), ("pad3", C.c_int32),
+        ("useSubDrag", C.c_int32), ("useminA", C.c_int32), ("useminV", C.c_int32), ("pad3", C.c_int32),
+        ("beta", C.c_double),
     ]
 
 
@@ -53,12 +54,13 @@ class FrehdForcing(C.Structure):
 SUB_TABLES = ["V", "Z", "Vxp", "Vxm", "Vyp", "Vym", "Np", "Nm", "Op", "Om",
               "minVxp", "minVxm", "minVyp", "minVym"]
 DRAG_TABLES = ["YhX", "YhY", "Cn"]
+MINA_TABLES = ["Nmin", "Nmax", "Omin", "Omax"]
 
 
 class FrehdSubgrid(C.Structure):
     _fields_ = ([("edgeX", c_double_p), ("edgeY", c_double_p)] + [(n, c_double_p) for n in SUB_TABLES]
                 + [("fine", c_double_p), ("fineR", C.c_int32), ("pad", C.c_int32), ("dxf", C.c_double)]
-                + [(n, c_double_p) for n in DRAG_TABLES])
+                + [(n, c_double_p) for n in DRAG_TABLES] + [(n, c_double_p) for n in MINA_TABLES])
 
 
 class FrehdIC(C.Structure):

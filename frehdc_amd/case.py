@@ -111,7 +111,7 @@ class Case:
         if k.useSubgrid == 1 and ic.sub:
             s = ic.sub.contents
             sg = {"edgeX": _arr(s.edgeX, N), "edgeY": _arr(s.edgeY, N)}
-            for t in abi.SUB_TABLES + abi.DRAG_TABLES:
+            for t in abi.SUB_TABLES + abi.DRAG_TABLES + abi.MINA_TABLES:
                 p = getattr(s, t)
                 if p:
                     sg[t] = _arr(p, N * k.Ne)

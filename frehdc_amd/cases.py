@@ -110,7 +110,7 @@ class Scenario:
         ic.V0cell = abi.dptr(self.V0cell)
         if self.subgrid is not None:
             sg = abi.FrehdSubgrid()
-            for k in ("edgeX", "edgeY", *abi.SUB_TABLES, *abi.DRAG_TABLES):
+            for k in ("edgeX", "edgeY", *abi.SUB_TABLES, *abi.DRAG_TABLES, *abi.MINA_TABLES):
                 a = self.subgrid.get(k)
                 setattr(sg, k, abi.dptr(a) if a is not None else None)
             if "fine" in self.subgrid:
@@ -132,7 +132,7 @@ def default_config(nx, ny, dx, dt, Nt, **kw) -> dict:
         useConstTideMS=1, nScalar=1, tidePS=0.0, tideMS=0.0, inflowS=0.0, Kx=1e-4, Ky=1e-4,
         offset=0.0, epsInner=1e-5, maxRefine=8, warmStart=0, useSubgrid=0, Ne=0, surfmin=0.0, dsurf=0.01,
         subR=1.0, subgridOnTheFly=0, useWind=0, northAngle=0.0, rhoa=1.225, Cw=1.3e-3, CwT=1.0,
-        useSubDrag=0,
+        useSubDrag=0, useminA=0, useminV=1, beta=0.0,
     )
     cfg.update(kw)
     return cfg
@@ -409,6 +409,31 @@ def with_subdrag(scen: Scenario, model: int, seed: int = 11) -> Scenario:
         raise ValueError("useSubDrag 2, 3 or 4")
     k["useSubDrag"] = int(model)
     scen.name += f"_subdrag{model}"
+    return scen
+
+
+def with_minarea(scen: Scenario, beta: float = 0.5, useminV: int = 1, seed: int = 13) -> Scenario:
+    """Switch on the minimum face areas (useminA = 1, Config.beta) and the
+    effective volumes (useminV) of combineSubArea (subgrid.c:360-414) on a
+    table-based subgrid scenario.  The tables are synthetic, shaped like the
+    real ones (level-major): Nmax/Omax a per-cell fraction in [0.6, 1.1] of the
+    face-area tables Np/Op, Nmin/Omin a fraction in [0.2, 0.7] of Nmax/Omax,
+    the minimum half-cell volumes a fraction in [0.5, 0.9] of Vxp.. Vym."""
+    sg, k = scen.subgrid, scen.cfg
+    assert sg is not None and "V" in sg, "needs level tables (not device tables)"
+    N, Ne = scen.N, k["Ne"]
+    rng = np.random.default_rng(seed)
+
+    def frac(lo, hi):
+        return np.tile(rng.uniform(lo, hi, size=N), Ne)
+
+    for mx, mn, src in (("Nmax", "Nmin", "Np"), ("Omax", "Omin", "Op")):
+        sg[mx] = np.ascontiguousarray(sg[src] * frac(0.6, 1.1))
+        sg[mn] = np.ascontiguousarray(sg[mx] * frac(0.2, 0.7))
+    for m, src in (("minVxp", "Vxp"), ("minVxm", "Vxm"), ("minVyp", "Vyp"), ("minVym", "Vym")):
+        sg[m] = np.ascontiguousarray(sg[src] * frac(0.5, 0.9))
+    k.update(useminA=1, useminV=int(useminV), beta=float(beta))
+    scen.name += f"_minA{useminV}"
     return scen
 
 

@@ -268,6 +268,14 @@ static int fine_r(const frehd_config &k)
     return (int)r;
 }
 
+// the combine reads minimum areas / volumes: useminA = 1, or useminV = 2
+// (with useminA = 0 and useminV = 1 the reference's Nmin stays 0 and the
+// volume adjustment never fires)
+static bool min_tables(const frehd_config &k)
+{
+    return k.useSubgrid == 1 && (k.useminA == 1 || k.useminV == 2);
+}
+
 // lay out every device array; with base == nullptr only the size is computed
 size_t layout(frehd_dev *d, char *base)
 {
@@ -360,6 +368,17 @@ size_t layout(frehd_dev *d, char *base)
         }
         a.allYhX = a.allYhY = a.allCn = nullptr;
         a.YhX = a.YhY = a.Cn = nullptr;
+        const double **mt[8] = {&a.allNmin, &a.allNmax, &a.allOmin, &a.allOmax,
+                                &a.allMinVxp, &a.allMinVxm, &a.allMinVyp, &a.allMinVym};
+        double **mc[8] = {&a.Nmin, &a.Nmax, &a.Omin, &a.Omax, &a.minVxp, &a.minVxm, &a.minVyp, &a.minVym};
+        for (int t = 0; t < 8; t++) { *mt[t] = nullptr; *mc[t] = nullptr; }
+        if (min_tables(k))
+            for (int t = 0; t < 8; t++) {
+                if ((t < 4 && k.useminA == 1) || (t >= 4 && k.useminV != 0)) {
+                    *mt[t] = cv.take<double>((size_t)k.Ne * N);
+                    *mc[t] = cv.take<double>(N);
+                }
+            }
         if (k.useSubDrag == 2) {
             a.allYhX = cv.take<double>((size_t)k.Ne * N);
             a.allYhY = cv.take<double>((size_t)k.Ne * N);
@@ -398,6 +417,10 @@ int check_config(const frehd_config *k)
     if (k->useSubDrag != 0 && k->useSubDrag != 2 && k->useSubDrag != 3 && k->useSubDrag != 4)
         return fail(FREHD_EINVAL, "useSubDrag must be 0, 2, 3 or 4 (1 reads curvature tables the reference "
                                   "never fills)");
+    if (k->useminA != 0 && k->useminA != 1) return fail(FREHD_EINVAL, "useminA must be 0 or 1");
+    if (k->useminV < 0 || k->useminV > 2) return fail(FREHD_EINVAL, "useminV must be 0, 1 or 2");
+    if (min_tables(*k) && k->subgridOnTheFly != 0)
+        return fail(FREHD_EINVAL, "useminA = 1 / useminV = 2 need level tables (subgridOnTheFly = 0)");
     if (k->useSubDrag != 0 && (k->useSubgrid != 1 || k->subgridOnTheFly != 0))
         return fail(FREHD_EINVAL, "useSubDrag needs the subgrid with level tables (useSubgrid = 1, "
                                   "subgridOnTheFly = 0)");
@@ -461,6 +484,10 @@ void fill_params(P &p, const frehd_config &k)
     p.scalarAdv = k.scalarAdv;
     p.useWind = k.useWind;
     p.useSubDrag = k.useSubgrid == 1 ? k.useSubDrag : 0;
+    p.useminA = k.useminA;
+    p.useminV = k.useminV;
+    p.minTabs = min_tables(k) ? 1 : 0;
+    p.beta = k.beta;
     p.rhoa = k.rhoa; p.Cw = k.Cw; p.CwT = k.CwT;
     p.useSubgrid = k.useSubgrid; p.dsurf = k.dsurf; p.subR = k.subR; p.Ne = k.Ne; p.surfmin = k.surfmin;
     p.warmStart = k.precision == FREHD_PREC_MIXED ? k.warmStart : 0;
@@ -657,6 +684,17 @@ int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
     } else if (k.useSubDrag == 3 || k.useSubDrag == 4) {
         if (!sg->Cn) return fail(FREHD_EINVAL, "useSubDrag = %d needs the Cn table", k.useSubDrag);
         if ((rc = upload(d, sg->Cn, (void *)a.allCn, tn * sizeof(double)))) return rc;
+    }
+    if (min_tables(k)) {
+        const double *src[8] = {sg->Nmin, sg->Nmax, sg->Omin, sg->Omax, sg->minVxp, sg->minVxm, sg->minVyp, sg->minVym};
+        const double *dst[8] = {a.allNmin, a.allNmax, a.allOmin, a.allOmax,
+                                a.allMinVxp, a.allMinVxm, a.allMinVyp, a.allMinVym};
+        for (int t = 0; t < 8; t++) {
+            if (!dst[t]) continue;
+            if (!src[t]) return fail(FREHD_EINVAL, "useminA / useminV need the %s table",
+                                     t < 4 ? "Nmin/Nmax/Omin/Omax" : "minV*");
+            if ((rc = upload(d, src[t], (void *)dst[t], tn * sizeof(double)))) return rc;
+        }
     }
     if (onTheFly) {
         // the argument block went to the device before the pixel size was

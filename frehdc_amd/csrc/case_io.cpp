@@ -34,6 +34,7 @@ struct frehd_case {
     std::vector<double> surf0, U0, V0, S0;
     std::vector<double> tab[14];   // V Z Vxp Vxm Vyp Vym Np Nm Op Om minVxp minVxm minVyp minVym
     std::vector<double> drag[3];   // YhX YhY (useSubDrag 2), Cn (3, 4)
+    std::vector<double> mina[4];   // Nmin Nmax Omin Omax (useminA 1)
     frehd_subgrid sub;
 };
 
@@ -240,8 +241,11 @@ int frehd_case_load(const char *data_dat, const char *base_dir, frehd_case_t **o
     const double surfmax = a[58];
     k.surfmin = a[59];
     k.dsurf = a[60];
-    const int staggeredV = (int)a[61], useminA = (int)a[62];
-    c->useminV = (int)a[64];
+    const int staggeredV = (int)a[61];
+    k.useminA = (int)a[62];
+    k.beta = a[63];
+    k.useminV = (int)a[64];
+    c->useminV = k.useminV;
     k.g = a[67];
     k.NUx = a[68];
     k.NUy = a[69];
@@ -277,8 +281,7 @@ int frehd_case_load(const char *data_dat, const char *base_dir, frehd_case_t **o
         return rc;
     };
     if (NX < 1 || NY < 1) return bad(FREHD_EINVAL, "NX and NY must be positive");
-    if (k.useSubgrid == 1 && (staggeredV != 1 || useminA != 0))
-        return bad(FREHD_EINVAL, "subgrid needs staggeredV = 1, useminA = 0");
+    if (k.useSubgrid == 1 && staggeredV != 1) return bad(FREHD_EINVAL, "subgrid needs staggeredV = 1");
     if (k.useSubgrid == 1 && k.useSubDrag != 0 && k.useSubDrag != 2 && k.useSubDrag != 3 && k.useSubDrag != 4)
         return bad(FREHD_EINVAL, "useSubDrag must be 0, 2, 3 or 4");
     if (k.useSubgrid != 1) k.useSubDrag = 0;
@@ -358,6 +361,16 @@ int frehd_case_load(const char *data_dat, const char *base_dir, frehd_case_t **o
                 return rc;
             }
             *ddst[t] = c->drag[t].data();
+        }
+        // minimum / maximum face areas (subgrid.c:2160-2190)
+        const char *mnames[4] = {"Nmin.dat", "Nmax.dat", "Omin.dat", "Omax.dat"};
+        const double **mdst[4] = {&s.Nmin, &s.Nmax, &s.Omin, &s.Omax};
+        for (int t = 0; t < 4 && k.useminA == 1; t++) {
+            if ((rc = read_values(join(c->inputFolder, c->subgridFolder + mnames[t]), N * (size_t)k.Ne, c->mina[t]))) {
+                delete c;
+                return rc;
+            }
+            *mdst[t] = c->mina[t].data();
         }
     }
     *out = c;

@@ -20,6 +20,9 @@ struct P {
     int useWind;
     double rhoa, Cw, CwT;     // wind stress (windTerm)
     int useSubDrag;           // 0 Manning h = V/Z, 2 Volp (Yh), 3/4 Casas (Cn)
+    int useminA, useminV;     // minimum face areas / effective volumes
+    int minTabs;              // the minimum-area and minV tables are on the device
+    double beta;
     // subgrid
     double dsurf, subR, surfmin;   // surfmin: raw elevation of level 0
     int Ne;
@@ -129,6 +132,10 @@ struct StepArgs {
     // bracketing level of each cell (useSubDrag 2: Yh; 3, 4: Cn)
     const double *allYhX, *allYhY, *allCn;
     double *YhX, *YhY, *Cn;
+    // minimum / maximum face areas and minimum half-cell volumes (level-major
+    // tables, per-cell values of the step) when the combine uses them
+    const double *allNmin, *allNmax, *allOmin, *allOmax, *allMinVxp, *allMinVxm, *allMinVyp, *allMinVym;
+    double *Nmin, *Nmax, *Omin, *Omax, *minVxp, *minVxm, *minVyp, *minVym;
     // instead of tabs (NULL): the fine bathymetry, from which a lookup
     // evaluates its two levels (cell_levels, bitwise the table values)
     FineBed bed;

@@ -928,6 +928,23 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
     double vyp = interp_surf(h, s1, t1[T_VYP], s2, t2[T_VYP]);
     double vxm = interp_surf(h, s1, t1[T_VXM], s2, t2[T_VXM]);
     double vym = interp_surf(h, s1, t1[T_VYM], s2, t2[T_VYM]);
+    if (p.minTabs) {
+        // minimum half-cell volumes and minimum / maximum face areas
+        // (subgrid.c:143-150, 210-220), level tables of their own
+        const size_t k1 = (size_t)j1 * N + c, k2 = (size_t)j2 * N + c;
+        if (p.useminV != 0) {
+            a.minVxp[c] = interp_surf(h, s1, a.allMinVxp[k1], s2, a.allMinVxp[k2]);
+            a.minVyp[c] = interp_surf(h, s1, a.allMinVyp[k1], s2, a.allMinVyp[k2]);
+            a.minVxm[c] = interp_surf(h, s1, a.allMinVxm[k1], s2, a.allMinVxm[k2]);
+            a.minVym[c] = interp_surf(h, s1, a.allMinVym[k1], s2, a.allMinVym[k2]);
+        }
+        if (p.useminA != 0) {
+            a.Nmin[c] = interp_surf(h, s1, a.allNmin[k1], s2, a.allNmin[k2]);
+            a.Omin[c] = interp_surf(h, s1, a.allOmin[k1], s2, a.allOmin[k2]);
+            a.Nmax[c] = interp_surf(h, s1, a.allNmax[k1], s2, a.allNmax[k2]);
+            a.Omax[c] = interp_surf(h, s1, a.allOmax[k1], s2, a.allOmax[k2]);
+        }
+    }
     const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
     const double wd[4] = {a.edgeX[c], w < N ? a.edgeX[w] : 0.0, a.edgeY[c], s < N ? a.edgeY[s] : 0.0};
     const int tid[4] = {T_NP, T_NM, T_OP, T_OM};
@@ -993,11 +1010,69 @@ __global__ void __launch_bounds__(256) k_limiter_sub(const StepArgs *__restrict_
 // combineSubArea (subgrid.c:254-462 and the clip loop :466-478) of interior
 // cell c, plus the interior faces the boundary loop closes: Nx of the right
 // column and Oy of the non-tidal top row.  useminV = 1 changes volumes only
-// where Nmin > 0, which stays 0 with useminA = 0.  A pure function of
-// computeSubArea's outputs, so a ghost can recompute the cell it mirrors.
+// where Nmin > 0, which stays 0 with useminA = 0 (min_adjust runs when the
+// minimum tables are on the device).  A pure function of computeSubArea's
+// outputs, so a ghost can recompute the cell it mirrors.
 struct Combined {
     double Nx, Oy, Vx, Vy, Zx, Zy, V, Z;
+    double OyOpen;            // Oy before the top row's closure
 };
+
+// effective-volume factor of combineSubArea (subgrid.c:377-403): r the
+// ratio of the minimum to the maximum face area
+__device__ inline double volume_lambda(double r, double cap)
+{
+    double lambda;
+    if (r > 0 && r < 1) {
+        lambda = -5.01 * pow(1 - r, 2) + 6.35 * (1 - r) - 0.6;
+        if (1 - r > 0.67) lambda = cap;
+        else if (lambda < 0) lambda = 0.0;
+    } else {
+        lambda = 0.0;
+    }
+    return lambda;
+}
+
+// useminA / useminV adjustments of combineSubArea (subgrid.c:360-414), after
+// the waterfall halving and before the small-value clip; the reference's
+// local beta is 0, so useminA takes its first branch (pull Nx, Oy above
+// Nmax, Omax back by Config.beta)
+__device__ inline void min_adjust(const StepArgs &a, int c, int e, int n, double &Nx, double &Oy, double &Vx,
+                                  double &Vy)
+{
+    const P &p = a.p;
+    const int N = a.g.N;
+    const double epsi = 0.0001;
+    double minVx = 0.0, minVy = 0.0;
+    if (p.useminV != 0) {
+        minVx = e < N ? a.minVxp[c] + a.minVxm[e] : 2 * a.minVxp[c];
+        minVy = n < N ? a.minVyp[c] + a.minVym[n] : 2 * a.minVyp[c];
+    }
+    double Nmin = 0.0, Nmax = 0.0, Omin = 0.0, Omax = 0.0;
+    if (p.useminA == 1) {
+        Nmin = a.Nmin[c]; Nmax = a.Nmax[c]; Omin = a.Omin[c]; Omax = a.Omax[c];
+        if (Nx - Nmax > epsi && Nmax > 0) Nx = Nx - p.beta * (Nx - Nmax);
+        if (Oy - Omax > epsi && Omax > 0) Oy = Oy - p.beta * (Oy - Omax);
+    }
+    if (p.useminV == 1) {
+        const double Vfloor = 1.0;
+        if (Nmin > 0 && Vx > minVx && minVx > 0) {
+            const double r = Nmin / Nmax;
+            const double lambda = volume_lambda(r, 1.35);
+            Vx = Vx - (1 - r) * lambda * Vx;
+            if (Vx < Vfloor) Vx = Vfloor;
+        }
+        if (Omin > 0 && Vy > minVy && minVy > 0) {
+            const double r = Omin / Omax;
+            const double lambda = volume_lambda(r, 1.4);
+            Vy = Vy - (1 - r) * lambda * Vy;
+            if (Vy < Vfloor) Vy = Vfloor;
+        }
+    } else if (p.useminV == 2) {
+        Vx = minVx;
+        Vy = minVy;
+    }
+}
 
 __device__ inline Combined combine_cell(const StepArgs &a, int c)
 {
@@ -1036,6 +1111,7 @@ __device__ inline Combined combine_cell(const StepArgs &a, int c)
         Vy = 2 * a.Vyp[c];
         Zy = 2 * Zc;
     }
+    if (p.minTabs) min_adjust(a, c, e, n, Nx, Oy, Vx, Vy);
     const double Vmin = 0.002, Amin = 0.002, Azmin = 0.002;
     double V = a.Vraw[c], Z = Zc;
     if (Vx < Vmin) Vx = 0.0;
@@ -1046,16 +1122,17 @@ __device__ inline Combined combine_cell(const StepArgs &a, int c)
     if (Zy < Amin) Zy = 0.0;
     if (Nx < Amin) Nx = 0.0;
     if (Oy < Amin) Oy = 0.0;
+    const double OyOpen = Oy;
     if (i == g.nx - 1) Nx = 0;                                    // subgrid.c:486
     if (j == g.ny - 1 && p.bcType != 2 && !(a.flags[c] & 1)) Oy = 0;   // :508-524
-    return Combined{Nx, Oy, Vx, Vy, Zx, Zy, V, Z};
+    return Combined{Nx, Oy, Vx, Vy, Zx, Zy, V, Z, OyOpen};
 }
 
 // combineSubArea over the interior (blocks [0, nbN)) and its ghost copies
 // (subgrid.c:480-506) over the 2nx + 2ny ghosts (the next nbGhost blocks),
 // each ghost recomputing the cell it mirrors.  The iM/iP ghosts of the top
-// row take Oy before the top-row closure, i.e. the clipped Op of the corner
-// cell.
+// row take the corner cell's Oy before the top-row closure (the closure
+// runs after the ghost copies, subgrid.c:508-524).
 __global__ void __launch_bounds__(256) k_combine(const StepArgs *__restrict__ A)
 {
     const StepArgs &a = A[blockIdx.z];
@@ -1084,9 +1161,7 @@ __global__ void __launch_bounds__(256) k_combine(const StepArgs *__restrict__ A)
     } else {                                 // iP / iM ghosts
         const int jj = (r - 2 * g.nx) % g.ny;
         a.Nx[k] = 0.0;
-        double oy = v.Oy;
-        if (jj == g.ny - 1) { oy = a.Op[src]; if (oy < 0.002) oy = 0.0; }
-        a.Oy[k] = oy;
+        a.Oy[k] = jj == g.ny - 1 ? v.OyOpen : v.Oy;
     }
 }
 

@@ -120,7 +120,17 @@ typedef struct frehd_config {
      * curvature model) reads Sub.CvX/CvY, which the reference never fills,
      * and is rejected; the models need tables (subgridOnTheFly = 0). */
     int32_t useSubDrag;
+    /* minimum face areas and effective volumes (Config.useminA, beta,
+     * useminV; combineSubArea subgrid.c:360-414): useminA = 1 pulls a face
+     * area above its level's Nmax/Omax back by beta; useminV = 1 reduces a
+     * face volume where the minimum face area Nmin/Omin is positive, 2 takes
+     * the minimum volumes.  They need the Nmin/Nmax/Omin/Omax tables
+     * (useminA) and the minV* tables (useminA = 1 with useminV = 1, or
+     * useminV = 2); with useminA = 0 and useminV = 0 or 1 nothing changes
+     * (Nmin stays 0). */
+    int32_t useminA, useminV;
     int32_t pad3;
+    double  beta;
 } frehd_config;
 
 /* ---- forcing ----------------------------------------------------------------
@@ -141,10 +151,11 @@ typedef struct frehd_forcing {
 /* ---- subgrid tables (initAllSubVar, subgrid.c:2112-2284) --------------------
  * Level-major tables: value of cell ii at level k is t[k*N + ii] (N interior
  * cells, Ne levels).  edgeX/edgeY are the raw (offset-free) bottom
- * elevations of the east and north faces (bottomXP.dat / bottomYP.dat).  Only
- * staggeredV = 1, useminA = 0, useSubDrag = 0 are supported; with useminA = 0
- * the minV* tables never change a result (they are read but inert, as in the
- * reference with useminV = 1) and may be NULL. */
+ * elevations of the east and north faces (bottomXP.dat / bottomYP.dat).
+ * staggeredV = 1 (the reference's only live form).  The minV* tables are
+ * needed with useminA = 1 and useminV = 1, or useminV = 2; otherwise they
+ * never change a result and may be NULL.  The drag-model and minimum-area
+ * tables are needed only by the options that use them (frehd_config). */
 typedef struct frehd_subgrid {
     const double *edgeX, *edgeY;
     const double *V, *Z, *Vxp, *Vxm, *Vyp, *Vym, *Np, *Nm, *Op, *Om;
@@ -161,6 +172,9 @@ typedef struct frehd_subgrid {
      * initAllSubVar subgrid.c:2206-2225): YhX/YhY with useSubDrag = 2, Cn with
      * 3 or 4, NULL otherwise */
     const double *YhX, *YhY, *Cn;
+    /* minimum / maximum face-area tables (Nmin.dat ...; initAllSubVar
+     * subgrid.c:2160-2190), level-major, with useminA = 1 */
+    const double *Nmin, *Nmax, *Omin, *Omax;
 } frehd_subgrid;
 
 /* ---- initial condition (the inputs of Init, initialize.c:40-155) ------------

@@ -63,6 +63,7 @@ struct fo_state {
     double *tV, *tZ, *tVxp, *tVxm, *tVyp, *tVym, *tNp, *tNm, *tOp, *tOm;
     double *tminVxp, *tminVxm, *tminVyp, *tminVym;
     double *tYhX, *tYhY, *tCn, *YhX, *YhY, *Cn;   /* useSubDrag 2 / 3, 4 */
+    double *tNmin, *tNmax, *tOmin, *tOmax;          /* useminA = 1 */
     double *V, *Z, *Vxp, *Vyp, *Vxm, *Vym, *Np, *Nm, *Op, *Om, *Vx, *Vy, *Zx, *Zy, *Nx, *Oy;
     double *minVxp, *minVxm, *minVyp, *minVym, *minVx, *minVy, *Nmin, *Omin, *Nmax, *Omax;
     int *ind;
@@ -329,6 +330,13 @@ static void compute_sub_area(fo_state *s)
         s->minVyp[c] = interp_surf(h, s1, s->tminVyp[k1], s2, s->tminVyp[k2]);
         s->minVxm[c] = interp_surf(h, s1, s->tminVxm[k1], s2, s->tminVxm[k2]);
         s->minVym[c] = interp_surf(h, s1, s->tminVym[k1], s2, s->tminVym[k2]);
+        /* Nmin and Omin, subgrid.c:210-220 */
+        if (k->useminA != 0) {
+            s->Nmin[c] = interp_surf(h, s1, s->tNmin[k1], s2, s->tNmin[k2]);
+            s->Omin[c] = interp_surf(h, s1, s->tOmin[k1], s2, s->tOmin[k2]);
+            s->Nmax[c] = interp_surf(h, s1, s->tNmax[k1], s2, s->tNmax[k2]);
+            s->Omax[c] = interp_surf(h, s1, s->tOmax[k1], s2, s->tOmax[k2]);
+        }
         const int w = nbM(s, c), so = nbS(s, c);
         const double wd[4] = {s->edgeX[c], w < N ? s->edgeX[w] : 0.0, s->edgeY[c], so < N ? s->edgeY[so] : 0.0};
         const double *tab[4] = {s->tNp, s->tNm, s->tOp, s->tOm};
@@ -349,8 +357,22 @@ static void compute_sub_area(fo_state *s)
     }
 }
 
-/* combineSubArea, subgrid.c:254-526 (useminA = 0, useminV = 1 whose
-   adjustment needs Nmin > 0 and is therefore inactive) */
+/* the effective-volume factor of combineSubArea, subgrid.c:377-403 */
+static double volume_lambda(double r, double cap)
+{
+    double lambda;
+    if (r > 0 && r < 1) {
+        lambda = -5.01 * pow(1 - r, 2) + 6.35 * (1 - r) - 0.6;
+        if (1 - r > 0.67) lambda = cap;
+        else if (lambda < 0) lambda = 0.0;
+    } else {
+        lambda = 0.0;
+    }
+    return lambda;
+}
+
+/* combineSubArea, subgrid.c:254-526 (useminV = 1's adjustment needs
+   Nmin > 0, i.e. useminA = 1) */
 static void combine_sub_area(fo_state *s)
 {
     const frehd_config *k = &s->cfg;
@@ -367,7 +389,7 @@ static void combine_sub_area(fo_state *s)
             else s->Vx[c] = 2.0 * (s->Vxp[c] + s->Vxm[e]);
             s->Zx[c] = s->Z[c];
             if (s->Vx[c] > 0 && s->Zx[c] <= 0) s->Zx[c] = dxdy;
-            s->minVx[c] = s->minVxp[c] + s->minVxm[e];
+            if (k->useminV != 0) s->minVx[c] = s->minVxp[c] + s->minVxm[e];
             if (s->surf[c] < s->bottomXP[c]) { s->Vx[c] = s->Vxm[e]; s->Zx[c] = 0.5 * s->Z[c]; }
             if (s->surf[e] < s->bottomXP[c]) { s->Vx[c] = s->Vxp[c]; s->Zx[c] = 0.5 * s->Z[c]; }
         } else {
@@ -380,13 +402,41 @@ static void combine_sub_area(fo_state *s)
             else s->Vy[c] = 2.0 * (s->Vyp[c] + s->Vym[n]);
             s->Zy[c] = s->Z[c];
             if (s->Vy[c] > 0 && s->Zy[c] <= 0) s->Zy[c] = dxdy;
-            s->minVy[c] = s->minVyp[c] + s->minVym[n];
+            if (k->useminV != 0) s->minVy[c] = s->minVyp[c] + s->minVym[n];
             if (s->surf[c] < s->bottomYP[c]) { s->Vy[c] = s->Vym[n]; s->Zy[c] = 0.5 * s->Z[c]; }
             if (s->surf[n] < s->bottomYP[c]) { s->Vy[c] = s->Vyp[c]; s->Zy[c] = 0.5 * s->Z[c]; }
         } else {
             s->Vy[c] = 2 * s->Vyp[c];
             s->Zy[c] = 2 * s->Z[c];
             s->minVy[c] = 2 * s->minVyp[c];
+        }
+        /* useminA (the reference's local beta = 0 selects the first branch),
+           subgrid.c:360-373 */
+        const double epsi = 0.0001;
+        if (k->useminA == 1) {
+            if ((s->Nx[c] - s->Nmax[c] > epsi) && (s->Nmax[c] > 0))
+                s->Nx[c] = s->Nx[c] - k->beta * (s->Nx[c] - s->Nmax[c]);
+            if ((s->Oy[c] - s->Omax[c] > epsi) && (s->Omax[c] > 0))
+                s->Oy[c] = s->Oy[c] - k->beta * (s->Oy[c] - s->Omax[c]);
+        }
+        /* useminV, subgrid.c:375-414 */
+        if (k->useminV == 1) {
+            const double Vfloor = 1.0;
+            if ((s->Nmin[c] > 0) && (s->Vx[c] > s->minVx[c]) && (s->minVx[c] > 0)) {
+                double r = s->Nmin[c] / s->Nmax[c];
+                double lambda = volume_lambda(r, 1.35);
+                s->Vx[c] = s->Vx[c] - (1 - r) * lambda * s->Vx[c];
+                if (s->Vx[c] < Vfloor) s->Vx[c] = Vfloor;
+            }
+            if ((s->Omin[c] > 0) && (s->Vy[c] > s->minVy[c]) && (s->minVy[c] > 0)) {
+                double r = s->Omin[c] / s->Omax[c];
+                double lambda = volume_lambda(r, 1.4);
+                s->Vy[c] = s->Vy[c] - (1 - r) * lambda * s->Vy[c];
+                if (s->Vy[c] < Vfloor) s->Vy[c] = Vfloor;
+            }
+        } else if (k->useminV == 2) {
+            s->Vx[c] = s->minVx[c];
+            s->Vy[c] = s->minVy[c];
         }
     }
     const double Vmin = 0.002, Amin = 0.002, Azmin = 0.002;
@@ -1351,6 +1401,10 @@ fo_state *fo_create(const frehd_config *cfg, const frehd_forcing *f, const frehd
         s->tYhX = (double *)dup_arr(cfg->useSubDrag == 2 ? sg->YhX : NULL, sizeof(double) * tn);
         s->tYhY = (double *)dup_arr(cfg->useSubDrag == 2 ? sg->YhY : NULL, sizeof(double) * tn);
         s->tCn = (double *)dup_arr(cfg->useSubDrag >= 3 ? sg->Cn : NULL, sizeof(double) * tn);
+        s->tNmin = (double *)dup_arr(cfg->useminA == 1 ? sg->Nmin : NULL, sizeof(double) * tn);
+        s->tNmax = (double *)dup_arr(cfg->useminA == 1 ? sg->Nmax : NULL, sizeof(double) * tn);
+        s->tOmin = (double *)dup_arr(cfg->useminA == 1 ? sg->Omin : NULL, sizeof(double) * tn);
+        s->tOmax = (double *)dup_arr(cfg->useminA == 1 ? sg->Omax : NULL, sizeof(double) * tn);
         s->YhX = zalloc(Ng);
         s->YhY = zalloc(Ng);
         s->Cn = zalloc(Ng);
@@ -1391,7 +1445,8 @@ void fo_destroy(fo_state *s)
         s->tVym, s->tNp, s->tNm, s->tOp, s->tOm, s->tminVxp, s->tminVxm, s->tminVyp, s->tminVym,
         s->V, s->Z, s->Vxp, s->Vyp, s->Vxm, s->Vym, s->Np, s->Nm, s->Op, s->Om, s->Vx, s->Vy,
         s->Zx, s->Zy, s->Nx, s->Oy, s->minVxp, s->minVxm, s->minVyp, s->minVym, s->minVx,
-        s->minVy, s->Nmin, s->Omin, s->Nmax, s->Omax, s->tYhX, s->tYhY, s->tCn, s->YhX, s->YhY, s->Cn};
+        s->minVy, s->Nmin, s->Omin, s->Nmax, s->Omax, s->tYhX, s->tYhY, s->tCn, s->YhX, s->YhY, s->Cn,
+        s->tNmin, s->tNmax, s->tOmin, s->tOmax};
     for (size_t q = 0; q < sizeof(subs) / sizeof(subs[0]); q++) free(subs[q]);
     free(s->ind);
     free(s->tideLocP); free(s->tideLocM); free(s->inflowLoc);

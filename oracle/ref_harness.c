@@ -139,7 +139,8 @@ static void fill_config(Config *s, const frehd_config *c)
     s->Kx = c->Kx; s->Ky = c->Ky;
     s->useSubgrid = c->useSubgrid; s->useMPI = 0; s->np = 1; s->useWind = c->useWind;
     s->northAngle = c->northAngle; s->rhoa = c->rhoa; s->Cw = c->Cw; s->CwT = c->CwT;
-    s->staggeredV = 1; s->useminA = 0; s->useminV = 1; s->useSubDrag = c->useSubDrag; s->beta = 0;
+    s->staggeredV = 1; s->useminA = c->useminA; s->useminV = c->useminV; s->useSubDrag = c->useSubDrag;
+    s->beta = c->beta;
     s->dsurf = c->dsurf; s->surfmin = c->surfmin; s->surfmax = c->surfmin + c->Ne * c->dsurf;
     s->subR = c->subR; s->dxf = c->dx / c->subR; s->dyf = c->dy / c->subR;
     s->dA = s->dxf * s->dyf; s->z0 = c->hD;
@@ -266,9 +267,10 @@ void *ref_create(const frehd_config *c, const frehd_forcing *f, const frehd_ic *
             if (src[q]) memcpy(*tabs[q], src[q], sizeof(double) * (size_t)Ne * N);
         }
         /* drag-model tables (subgrid.c:2206-2225) and their per-cell values */
-        double **dtabs[] = {&sb->allYhX, &sb->allYhY, &sb->allCn};
-        const double *dsrc[] = {sg->YhX, sg->YhY, sg->Cn};
-        for (int q = 0; q < 3; q++) {
+        double **dtabs[] = {&sb->allYhX, &sb->allYhY, &sb->allCn, &sb->allNmin, &sb->allNmax, &sb->allOmin,
+                            &sb->allOmax};
+        const double *dsrc[] = {sg->YhX, sg->YhY, sg->Cn, sg->Nmin, sg->Nmax, sg->Omin, sg->Omax};
+        for (int q = 0; q < 7; q++) {
             *dtabs[q] = padded(h, (size_t)Ne * N, pad);
             if (dsrc[q]) memcpy(*dtabs[q], dsrc[q], sizeof(double) * (size_t)Ne * N);
         }
@@ -498,10 +500,12 @@ int ref_io_subtables(const char *inputFolder, const char *subgridFolder, int NX,
     return 0;
 }
 
-/* initAllSubVar's drag-model tables (subgrid.c:2206-2225): YhX, YhY with
- * useSubDrag = 2, Cn with 3 or 4, level-major */
-int ref_io_dragtables(const char *inputFolder, const char *subgridFolder, int NX, int NY, double surfmin,
-                      double surfmax, double dsurf, double offset, int useSubDrag, double *out[3], int *Ne)
+/* initAllSubVar's optional tables, level-major: YhX, YhY with useSubDrag =
+ * 2, Cn with 3 or 4 (subgrid.c:2206-2225); Nmin, Nmax, Omin, Omax with
+ * useminA = 1 (:2160-2190) */
+int ref_io_extratables(const char *inputFolder, const char *subgridFolder, int NX, int NY, double surfmin,
+                       double surfmax, double dsurf, double offset, int useSubDrag, int useminA, double *out[7],
+                       int *Ne)
 {
     Config *s = io_config(inputFolder, subgridFolder, NX, NY);
     s->useSubgrid = 1;
@@ -509,7 +513,7 @@ int ref_io_dragtables(const char *inputFolder, const char *subgridFolder, int NX
     s->surfmax = surfmax;
     s->dsurf = dsurf;
     s->useminV = 0;
-    s->useminA = 0;
+    s->useminA = useminA;
     s->useSubDrag = useSubDrag;
     Bath b;
     memset(&b, 0, sizeof b);
@@ -518,9 +522,10 @@ int ref_io_dragtables(const char *inputFolder, const char *subgridFolder, int NX
     Sub *sub = NULL;
     initAllSubVar(&sub, &b, s, 0);
     *Ne = sub->Ne;
-    double *src[3] = {useSubDrag == 2 ? sub->allYhX : NULL, useSubDrag == 2 ? sub->allYhY : NULL,
-                      useSubDrag >= 3 ? sub->allCn : NULL};
-    for (int t = 0; t < 3; t++)
+    double *src[7] = {useSubDrag == 2 ? sub->allYhX : NULL, useSubDrag == 2 ? sub->allYhY : NULL,
+                      useSubDrag >= 3 ? sub->allCn : NULL, useminA ? sub->allNmin : NULL,
+                      useminA ? sub->allNmax : NULL, useminA ? sub->allOmin : NULL, useminA ? sub->allOmax : NULL};
+    for (int t = 0; t < 7; t++)
         if (out[t] && src[t]) memcpy(out[t], src[t], sizeof(double) * NX * NY * sub->Ne);
     free(s);
     return 0;

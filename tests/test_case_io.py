@@ -228,10 +228,11 @@ def test_wind_deck_runs_like_reference(lib_built, oracle_built, tmp_path):
         assert np.array_equal(a[:m].view(np.int64), b[:m].view(np.int64)), n
 
 
-def subgrid_deck(tmp_path, subdrag=0):
+def subgrid_deck(tmp_path, subdrag=0, minA=None):
     """The reference's own deck with its shipped subgrid data (input/subdata/,
-    50 levels) switched on, optionally with a subgrid drag model."""
-    deck = tmp_path / f"sdeck{subdrag}"
+    50 levels) switched on, optionally with a subgrid drag model or the
+    minimum face areas (minA = beta)."""
+    deck = tmp_path / f"sdeck{subdrag}_{minA}"
     deck.mkdir()
     os.symlink(os.path.join(REFDIR, "input"), deck / "input")
     out = []
@@ -242,6 +243,10 @@ def subgrid_deck(tmp_path, subdrag=0):
             ln = "1;" + ln[2:]
         if subdrag and ln.strip().endswith("# The subgrid drag model used"):
             ln = f"{subdrag};" + ln[2:]
+        if minA is not None and ln.strip().endswith("# If minimum face area is used"):
+            ln = "1;" + ln[2:]
+        if minA is not None and ln.strip().endswith("# Face area correction factor"):
+            ln = f"{minA};" + ln[len(ln.split(';')[0]) + 1:]
         out.append(ln)
     (deck / "data.dat").write_text("\n".join(out) + "\n")
     return deck
@@ -263,14 +268,14 @@ def test_volp_tables_equal_reference(lib_built, tmp_path):
     sc = c.scenario()
     lib = C.CDLL(pyoracle.REF_LIB)
     P = C.POINTER(C.c_double)
-    outs = [np.zeros(N * 50) for _ in range(3)]
-    arr = (P * 3)(*[o.ctypes.data_as(P) for o in outs])
+    outs = [np.zeros(N * 50) for _ in range(7)]
+    arr = (P * 7)(*[o.ctypes.data_as(P) for o in outs])
     ne = C.c_int()
     cwd = os.getcwd()
     os.chdir(deck)
     try:
-        lib.ref_io_dragtables(b"input/", b"subdata/", k.nx, k.ny, C.c_double(0.25), C.c_double(0.75),
-                              C.c_double(0.01), C.c_double(k.offset), 2, arr, C.byref(ne))
+        lib.ref_io_extratables(b"input/", b"subdata/", k.nx, k.ny, C.c_double(0.25), C.c_double(0.75),
+                               C.c_double(0.01), C.c_double(k.offset), 2, 0, arr, C.byref(ne))
     finally:
         os.chdir(cwd)
     assert ne.value == 50
@@ -285,3 +290,45 @@ def test_volp_tables_equal_reference(lib_built, tmp_path):
         a, b = o.field(n).reshape(-1), r.field(n).reshape(-1)
         m = min(len(a), len(b))
         assert np.array_equal(a[:m].view(np.int64), b[:m].view(np.int64)), n
+
+
+@pytest.mark.skipif(not have_ref(), reason="reference checkout / oracle/_ref not present")
+def test_minarea_tables_and_run_equal_reference(lib_built, tmp_path):
+    """useminA = 1 (beta 0.5) with useminV = 1 on the shipped subgrid data:
+    our loader's Nmin/Nmax/Omin/Omax tables equal initAllSubVar's
+    (subgrid.c:2160-2190), and the deck runs through the reference's own step
+    functions like the C restatement, bit for bit."""
+    import ctypes as C
+    import pyoracle
+    deck = subgrid_deck(tmp_path, 0, 0.5)
+    c = Case(str(deck / "data.dat"), str(deck))
+    k = c.cfg
+    assert (k.useSubgrid, k.useminA, k.useminV, k.beta) == (1, 1, 1, 0.5)
+    N = k.nx * k.ny
+    sc = c.scenario()
+    lib = C.CDLL(pyoracle.REF_LIB)
+    P = C.POINTER(C.c_double)
+    outs = [np.zeros(N * 50) for _ in range(7)]
+    arr = (P * 7)(*[o.ctypes.data_as(P) for o in outs])
+    ne = C.c_int()
+    cwd = os.getcwd()
+    os.chdir(deck)
+    try:
+        lib.ref_io_extratables(b"input/", b"subdata/", k.nx, k.ny, C.c_double(0.25), C.c_double(0.75),
+                               C.c_double(0.01), C.c_double(k.offset), 0, 1, arr, C.byref(ne))
+    finally:
+        os.chdir(cwd)
+    for t, name in enumerate(abi.MINA_TABLES):
+        assert np.array_equal(sc.subgrid[name].view(np.int64), outs[3 + t].view(np.int64)), name
+    assert np.any(sc.subgrid["Nmin"] > 0)
+    o = pyoracle.Oracle(sc)
+    r = pyoracle.Reference(sc)
+    for _ in range(20):
+        assert o.step(1) == 0 and r.step(1) == 0
+    for n in ("surf", "uuXP", "vvYP", "depth"):
+        a, b = o.field(n).reshape(-1), r.field(n).reshape(-1)
+        m = min(len(a), len(b))
+        assert np.array_equal(a[:m].view(np.int64), b[:m].view(np.int64)), n
+    for n in ("Vx", "Vy", "Nx", "Oy"):
+        a, b = o.raw(n), r.raw(n)
+        assert np.array_equal(a[:N].view(np.int64), b[:N].view(np.int64)), n

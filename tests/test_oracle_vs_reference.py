@@ -49,6 +49,11 @@ CASES = [
     ("marsh_subgrid4_volp", lambda: cases.with_subdrag(marsh_sub(), 2), {"CDmax": 0.02}),
     ("marsh_subgrid4_casas3", lambda: cases.with_subdrag(marsh_sub(), 3), {}),
     ("marsh_subgrid4_casas4", lambda: cases.with_subdrag(marsh_sub(), 4), {"useThinLayer": 0}),
+    # minimum face areas and effective volumes (combineSubArea, subgrid.c:360-414)
+    ("marsh_subgrid4_minA_V1", lambda: cases.with_minarea(marsh_sub(), 0.5, 1), {}),
+    ("marsh_subgrid4_minA_V0", lambda: cases.with_minarea(marsh_sub(), 0.8, 0), {}),
+    ("marsh_subgrid4_minA_V2", lambda: cases.with_minarea(marsh_sub(), 0.3, 2), {}),
+    ("marsh_subgrid4_minV2", lambda: cases.with_minarea(marsh_sub(), 0.0, 2), {"useminA": 0}),
 ]
 
 
