@@ -613,13 +613,16 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
 }
 
 
-int upload_state(frehd_dev *d, const frehd_fields &s)
+// state arrays into the buffers the next step starts from (depthParity,
+// sParity: the double-buffered depth and concentration buffers in use)
+int upload_state(frehd_dev *d, const frehd_fields &s, int depthParity = 0, int sParity = 0)
 {
     const size_t N = d->g.N, Ng = d->g.Ng;
     StepArgs &a = d->args;
+    double *depthCur = depthParity ? a.depthB : a.depthA;
     struct { const double *src; double *dst; size_t n; } ng[] = {
         {s.surf, a.surf, Ng}, {s.surfOld, a.surfOld, Ng}, {s.uuXP, a.uuXP, Ng}, {s.vvYP, a.vvYP, Ng},
-        {s.uuYP, a.uuYP, N}, {s.vvXP, a.vvXP, N}, {s.depth, a.depthA, N}, {s.depthXP, a.depthXP, Ng},
+        {s.uuYP, a.uuYP, N}, {s.vvXP, a.vvXP, N}, {s.depth, depthCur, N}, {s.depthXP, a.depthXP, Ng},
         {s.depthYP, a.depthYP, Ng}, {s.EnXP, a.EnXP, Ng}, {s.EnYP, a.EnYP, Ng}, {s.DragXP, a.DragXP, Ng},
         {s.DragYP, a.DragYP, Ng}, {s.CDXP, a.CDXP, Ng}, {s.CDYP, a.CDYP, Ng}, {s.Fuu, a.Fuu, Ng},
         {s.Fvv, a.Fvv, Ng}, {s.cellV, a.cellV, N}, {s.CFLx, a.CFLx, Ng}, {s.CFLy, a.CFLy, Ng},
@@ -632,10 +635,11 @@ int upload_state(frehd_dev *d, const frehd_fields &s)
     const int ns = d->args.p.nScalar;
     for (int q = 0; q < ns; q++) {
         // both concentration buffers: the corners, which no step writes,
-        // then agree whichever buffer is current
+        // then agree whichever buffer is current (the current one last)
         if (s.S) {
-            int rc = upload(d, s.S + (size_t)q * Ng, a.S + (size_t)q * a.sStride, Ng * sizeof(double));
-            if (!rc) rc = upload(d, s.S + (size_t)q * Ng, a.S2 + (size_t)q * a.sStride, Ng * sizeof(double));
+            double *cur = sParity ? a.S2 : a.S, *oth = sParity ? a.S : a.S2;
+            int rc = upload(d, s.S + (size_t)q * Ng, oth + (size_t)q * a.sStride, Ng * sizeof(double));
+            if (!rc) rc = upload(d, s.S + (size_t)q * Ng, cur + (size_t)q * a.sStride, Ng * sizeof(double));
             if (rc) return rc;
         }
         if (s.Sm) { int rc = upload(d, s.Sm + (size_t)q * N, a.Sm + (size_t)q * a.sStride, N * sizeof(double)); if (rc) return rc; }
@@ -1106,6 +1110,18 @@ int frehd_dev_stats(frehd_dev_t *d, frehd_stats *out)
     out->halted = c.halted;
     out->haltStep = c.haltStep;
     return 0;
+}
+
+int frehd_dev_upload(frehd_dev_t *d, const frehd_fields *in)
+{
+    if (!d || !in) return fail(FREHD_EINVAL, "NULL argument");
+    if (d->batch) return fail(FREHD_ESTATE, "the handle is in a batch");
+    if (d->midStep) return fail(FREHD_ESTATE, "a step is half done (phase %d ran last)", d->midStep);
+    HIPCHK(hipSetDevice(d->device));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    Ctl c;
+    COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
+    return upload_state(d, *in, c.parity, c.sParity);
 }
 
 int frehd_dev_step_index(frehd_dev_t *d, int32_t *tt)

@@ -97,6 +97,21 @@ class DeviceScenario:
     def field(self, name):
         return self.download([name])[name]
 
+    def upload(self, fields: dict):
+        """frehd_dev_upload: replace the named state arrays (the layout of
+        download(); S as nScalar x Ng) at a step boundary."""
+        flds = abi.FrehdFields()
+        keep = []
+        for n, a in fields.items():
+            if n == "ind":
+                a = np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+                flds.ind = abi.iptr(a)
+            else:
+                a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                setattr(flds, n, abi.dptr(a))
+            keep.append(a)
+        abi.check(self.lib.frehd_dev_upload(self.h, C.byref(flds)))
+
     def stats(self) -> dict:
         st = abi.FrehdStats()
         abi.check(self.lib.frehd_dev_stats(self.h, C.byref(st)))

@@ -281,6 +281,16 @@ int frehd_dev_stats(frehd_dev_t *dev, frehd_stats *out);
 /* Current step counter (number of steps taken). */
 int frehd_dev_step_index(frehd_dev_t *dev, int32_t *tt);
 
+/* Replace state arrays of the handle with host arrays: every non-NULL
+ * pointer of *in (the frehd_fields layout of frehd_dev_download) is copied
+ * into the buffers the next step starts from, as frehd_dev_create_from_state
+ * does at creation.  For callers that change the state on the host between
+ * steps -- the oneCompleteStep shim with the reference's signature
+ * (nssolve.c:110-118), whose caller runs monitorCFL and DataOutput on the
+ * host.  At a step boundary only (FREHD_ESTATE after a partial step, or for
+ * a handle in a batch). */
+int frehd_dev_upload(frehd_dev_t *dev, const frehd_fields *in);
+
 /* Set the step counter, e.g. to resume from a state saved at step tt; the
  * next step taken is tt + 1. */
 int frehd_dev_set_step_index(frehd_dev_t *dev, int32_t tt);
