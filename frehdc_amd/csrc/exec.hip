@@ -475,10 +475,10 @@ static int exec_phase_launch(Exec &e, int phase)
         }
         return 0;
     }
-    case 3:   // scalarTransport (members without scalars skip inside)
-        for (int q = 0; q < e.nScalarMax; q++) {
-            if (e.sub) LAUNCH(K_SCALAR, k_scalar_sub, g2, b2, e.dArgs, q);
-            else LAUNCH(K_SCALAR, k_scalar, g2, b2, e.dArgs, q);
+    case 3:   // scalarTransport, every scalar in one launch (members without scalars skip inside)
+        if (e.nScalarMax > 0) {
+            if (e.sub) LAUNCH(K_SCALAR, k_scalar_sub, g2, b2, e.dArgs);
+            else LAUNCH(K_SCALAR, k_scalar, g2, b2, e.dArgs);
         }
         return 0;
     case 4:   // monitorCFL's stop test

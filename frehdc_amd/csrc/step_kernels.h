@@ -224,7 +224,7 @@ __global__ void k_limiter(const StepArgs *__restrict__ A);
 __global__ void k_faces(const StepArgs *__restrict__ A);
 __global__ void k_tidal_velocity(const StepArgs *__restrict__ A);
 __global__ void k_finish(const StepArgs *__restrict__ A);
-__global__ void k_scalar(const StepArgs *__restrict__ A, int q);
+__global__ void k_scalar(const StepArgs *__restrict__ A);
 __global__ void k_step_end(const StepArgs *__restrict__ A);
 // subgrid variants
 __global__ void k_predictor_sub(const StepArgs *__restrict__ A);
@@ -235,6 +235,6 @@ __global__ void k_combine(const StepArgs *__restrict__ A);
 __global__ void k_faces_sub(const StepArgs *__restrict__ A);
 __global__ void k_tidal_velocity_sub(const StepArgs *__restrict__ A);
 __global__ void k_finish_sub(const StepArgs *__restrict__ A);
-__global__ void k_scalar_sub(const StepArgs *__restrict__ A, int q);
+__global__ void k_scalar_sub(const StepArgs *__restrict__ A);
 
 }  // namespace frehd

@@ -686,9 +686,9 @@ __device__ inline double tvd_face(double SL, double SH, double SLL, double SHH, 
 }
 
 template <bool SUB>
-__device__ inline void scalar_body(const StepArgs &a, int q)
+__device__ inline void scalar_body(const StepArgs &a)
 {
-    if (tail_off(a) || q >= a.p.nScalar) return;
+    if (tail_off(a) || a.p.nScalar < 1) return;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -697,100 +697,108 @@ __device__ inline void scalar_body(const StepArgs &a, int q)
     const int tt = cur_tt(a);
     const int c = j * g.nx + i;
     const int e = nbE(g, c, i, j), w = nbW(g, c, i, j), n = nbN(g, c, i, j), s = nbS(g, c, i, j);
-    const double *S = s_cur(a) + (size_t)q * a.sStride;
-    double *Sm = a.Sm + (size_t)q * a.sStride;
     const double *Fu = a.Fuu, *Fv = a.Fvv;
     const double dt = p.dt, dx = p.dx, dy = p.dy;
-    const double Sc = S[c], Se = S[e], Sw = S[w], Sn = S[n], Ss = S[s];
-    double sm = Sm[c];
-    {
-        double Sd, Su, Sr, Sl;
-        if (p.scalarAdv == 2) {
-            // TVD superbee (scalar.c:99-255)
-            const int ee = nbEE(g, c, i, j), ww = nbWW(g, c, i, j), nn = nbNN(g, c, i, j), ss = nbSS(g, c, i, j);
-            const double *u = a.uuXP, *v = a.vvYP;
-            Sd = tvd_face(Sc, Se, Sw, S[ee], Fu[c], Fu[w], Fu[e], u[c] * dt / dx);
-            Su = tvd_face(Sw, Sc, S[ww], Se, Fu[w], Fu[ww], Fu[c], u[w] * dt / dx);
-            Sr = tvd_face(Sc, Sn, Ss, S[nn], Fv[c], Fv[s], Fv[n], v[c] * dt / dy);
-            // the YM face's Courant number divides by dx (scalar.c:222)
-            Sl = tvd_face(Ss, Sc, S[ss], Sn, Fv[s], Fv[ss], Fv[c], v[s] * dt / dx);
-        } else {
-            Sd = Fu[c] > 0 ? Sc : Se;
-            Su = Fu[w] > 0 ? Sw : Sc;
-            Sr = Fv[c] > 0 ? Sc : Sn;
-            Sl = Fv[s] > 0 ? Ss : Sc;
-        }
-        sm = sm + dt * (-Fu[c] * Sd + Fu[w] * Su - Fv[c] * Sr + Fv[s] * Sl);
-    }
-    if (SUB)   // scalarDiffusionSub, subgrid.c:1552-1567
-        sm = sm + dt * ((p.Kx * a.Nx[c] / dx) * (Se - Sc) - (p.Kx * a.Nx[w] / dx) * (Sc - Sw) +
-                        (p.Ky * a.Oy[c] / dy) * (Sn - Sc) - (p.Ky * a.Oy[s] / dy) * (Sc - Ss));
-    else
-        sm = sm + dt * ((p.Kx * dy * a.depthXP[c] / dx) * (Se - Sc) - (p.Kx * dy * a.depthXP[w] / dx) * (Sc - Sw) +
-                        (p.Ky * dx * a.depthYP[c] / dy) * (Sn - Sc) - (p.Ky * dx * a.depthYP[s] / dy) * (Sc - Ss));
-    // connectivity with the previous-step surface (scalar.c:292-341); the
-    // subgrid tests face areas instead of face depths (subgrid.c:1588-1630)
-    // (every operand loaded up front, so the loads overlap instead of
-    // waiting on each test)
+    // geometry of the cell, the same for every scalar: flow rates, face
+    // areas or depths, the connectivity with the previous-step surface
+    // (scalar.c:292-341; the subgrid tests face areas, subgrid.c:1588-1630)
+    const double Fuc = Fu[c], Fuw = Fu[w], Fvc = Fv[c], Fvs = Fv[s];
     const double *ho = a.surfOld, *zb = a.bottomZ;
     const double *fx = SUB ? a.Nx : a.depthXP, *fy = SUB ? a.Oy : a.depthYP;
     const double fxc = fx[c], fxw = fx[w], fyc = fy[c], fys = fy[s];
-    const double dE = ho[e] - zb[e], dW = ho[w] - zb[w], dN = ho[n] - zb[n], dS = ho[s] - zb[s];
-    double vals[4];
-    int m = 0;
-    if (fxc > 0 && dE > 0) vals[m++] = Se;
-    if (fxw > 0 && dW > 0) vals[m++] = Sw;
-    if (n <= g.N && fyc > 0 && dN > 0) vals[m++] = Sn;
-    if (s <= g.N && fys > 0 && dS > 0) vals[m++] = Ss;
-    double smax = 0, smin = 0;
-    if (m) {
-        smax = vals[0];
-        smin = vals[0];
-        for (int t = 1; t < m; t++) {
-            if (vals[t] > smax) smax = vals[t];
-            if (vals[t] < smin) smin = vals[t];
-        }
-    }
+    const bool cE = fxc > 0 && ho[e] - zb[e] > 0;
+    const bool cW = fxw > 0 && ho[w] - zb[w] > 0;
+    const bool cN = n <= g.N && fyc > 0 && ho[n] - zb[n] > 0;
+    const bool cS = s <= g.N && fys > 0 && ho[s] - zb[s] > 0;
     // the subgrid path adds inflow only when bcType != 3 (subgrid.c:1660)
     const int ninflow = (SUB && p.bcType == 3) ? 0 : (a.flags[c] >> 2);
-    for (int t = 0; t < ninflow; t++) sm = sm + p.inflowS * a.inflow[tt] * dt / p.inflowLocLength;
-    Sm[c] = sm;
     const double V = a.cellV[c];
-    double Sv = V > 0 ? sm / V : 0;
-    if (m != 0 && Sv > smax && ninflow == 0) Sv = smax;
-    else if (m != 0 && Sv < smin && ninflow == 0) Sv = smin;
-    if (Sv > 100 || Sv < 0) {
-        const bool bottomRow = (j == 0);
-        const bool topRowNotFirst = (j == g.ny - 1 && i > 0);
-        if (!bottomRow && !topRowNotFirst) {
-            Sv = 0;
-            a.surf[c] = a.surf[c] - depth_nxt(a)[c];
-            depth_nxt(a)[c] = 0;
-        }
-    }
-    double *Snx = s_nxt(a) + (size_t)q * a.sStride;
-    if (j == g.ny - 1) Snx[gjP(g, i)] = Sv;
-    if (j == 0) Snx[gjM(g, i)] = Sv;
-    if (i == g.nx - 1) Snx[giP(g, j)] = Sv;
-    if (i == 0) Snx[giM(g, j)] = Sv;
     const uint8_t fl = a.flags[c];
-    if ((fl & 1) && p.bcType != 2) Sv = p.useConstTidePS == 1 ? p.tidePS : a.tidalPS[tt];
-    if ((fl & 2) && p.bcType != 1) Sv = p.useConstTideMS == 1 ? p.tideMS : a.tidalMS[tt];
-    if (depth_nxt(a)[c] <= 0) Sv = 0;
-    Snx[c] = Sv;
+    const bool bottomRow = (j == 0), topRowNotFirst = (j == g.ny - 1 && i > 0);
+    // each scalar in turn, as separate passes would run them: a pass that
+    // resets an out-of-range cell lowers that cell's surface and depth for
+    // the passes after it
+    for (int q = 0; q < p.nScalar; q++) {
+        const double *S = s_cur(a) + (size_t)q * a.sStride;
+        double *Sm = a.Sm + (size_t)q * a.sStride;
+        const double Sc = S[c], Se = S[e], Sw = S[w], Sn = S[n], Ss = S[s];
+        double sm = Sm[c];
+        {
+            double Sd, Su, Sr, Sl;
+            if (p.scalarAdv == 2) {
+                // TVD superbee (scalar.c:99-255)
+                const int ee = nbEE(g, c, i, j), ww = nbWW(g, c, i, j), nn = nbNN(g, c, i, j), ss = nbSS(g, c, i, j);
+                const double *u = a.uuXP, *v = a.vvYP;
+                Sd = tvd_face(Sc, Se, Sw, S[ee], Fuc, Fuw, Fu[e], u[c] * dt / dx);
+                Su = tvd_face(Sw, Sc, S[ww], Se, Fuw, Fu[ww], Fuc, u[w] * dt / dx);
+                Sr = tvd_face(Sc, Sn, Ss, S[nn], Fvc, Fvs, Fv[n], v[c] * dt / dy);
+                // the YM face's Courant number divides by dx (scalar.c:222)
+                Sl = tvd_face(Ss, Sc, S[ss], Sn, Fvs, Fv[ss], Fvc, v[s] * dt / dx);
+            } else {
+                Sd = Fuc > 0 ? Sc : Se;
+                Su = Fuw > 0 ? Sw : Sc;
+                Sr = Fvc > 0 ? Sc : Sn;
+                Sl = Fvs > 0 ? Ss : Sc;
+            }
+            sm = sm + dt * (-Fuc * Sd + Fuw * Su - Fvc * Sr + Fvs * Sl);
+        }
+        if (SUB)   // scalarDiffusionSub, subgrid.c:1552-1567
+            sm = sm + dt * ((p.Kx * fxc / dx) * (Se - Sc) - (p.Kx * fxw / dx) * (Sc - Sw) +
+                            (p.Ky * fyc / dy) * (Sn - Sc) - (p.Ky * fys / dy) * (Sc - Ss));
+        else
+            sm = sm + dt * ((p.Kx * dy * fxc / dx) * (Se - Sc) - (p.Kx * dy * fxw / dx) * (Sc - Sw) +
+                            (p.Ky * dx * fyc / dy) * (Sn - Sc) - (p.Ky * dx * fys / dy) * (Sc - Ss));
+        double vals[4];
+        int m = 0;
+        if (cE) vals[m++] = Se;
+        if (cW) vals[m++] = Sw;
+        if (cN) vals[m++] = Sn;
+        if (cS) vals[m++] = Ss;
+        double smax = 0, smin = 0;
+        if (m) {
+            smax = vals[0];
+            smin = vals[0];
+            for (int t = 1; t < m; t++) {
+                if (vals[t] > smax) smax = vals[t];
+                if (vals[t] < smin) smin = vals[t];
+            }
+        }
+        for (int t = 0; t < ninflow; t++) sm = sm + p.inflowS * a.inflow[tt] * dt / p.inflowLocLength;
+        Sm[c] = sm;
+        double Sv = V > 0 ? sm / V : 0;
+        if (m != 0 && Sv > smax && ninflow == 0) Sv = smax;
+        else if (m != 0 && Sv < smin && ninflow == 0) Sv = smin;
+        if (Sv > 100 || Sv < 0) {
+            if (!bottomRow && !topRowNotFirst) {
+                Sv = 0;
+                a.surf[c] = a.surf[c] - depth_nxt(a)[c];
+                depth_nxt(a)[c] = 0;
+            }
+        }
+        double *Snx = s_nxt(a) + (size_t)q * a.sStride;
+        if (j == g.ny - 1) Snx[gjP(g, i)] = Sv;
+        if (j == 0) Snx[gjM(g, i)] = Sv;
+        if (i == g.nx - 1) Snx[giP(g, j)] = Sv;
+        if (i == 0) Snx[giM(g, j)] = Sv;
+        if ((fl & 1) && p.bcType != 2) Sv = p.useConstTidePS == 1 ? p.tidePS : a.tidalPS[tt];
+        if ((fl & 2) && p.bcType != 1) Sv = p.useConstTideMS == 1 ? p.tideMS : a.tidalMS[tt];
+        if (depth_nxt(a)[c] <= 0) Sv = 0;
+        Snx[c] = Sv;
+    }
 }
 
-__global__ void __launch_bounds__(256) k_scalar(const StepArgs *__restrict__ A, int q)
+// every scalar of a member in one launch (the geometry is read once)
+__global__ void __launch_bounds__(256) k_scalar(const StepArgs *__restrict__ A)
 {
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
-    scalar_body<false>(a, q);
+    scalar_body<false>(a);
 }
-__global__ void __launch_bounds__(256) k_scalar_sub(const StepArgs *__restrict__ A, int q)
+__global__ void __launch_bounds__(256) k_scalar_sub(const StepArgs *__restrict__ A)
 {
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
-    scalar_body<true>(a, q);
+    scalar_body<true>(a);
 }
 
 // Step end: volume-loss and CFL reductions (block partials in block order),

@@ -2,10 +2,11 @@
 admitted against an HBM budget, stay resident on the device and are advanced
 in shared launches (continuous batching), and leave when their run is done.
 
-* Size buckets: requests are grouped by grid side (<= 128, 256, 512, 1024,
-  ...); each bucket is one device batch (frehd_batch_*), so a launch covers
-  every resident scenario of similar size and the surplus blocks of the
-  smaller ones stay bounded.
+* Size buckets: requests are grouped by grid side (<= 128, 256, 384, ...,
+  1024, 2048, 4096); each bucket is one device batch (frehd_batch_*), so a
+  launch covers every resident scenario of similar size and the surplus
+  blocks of the smaller ones stay bounded (finer buckets measured no
+  faster: the launches of the concurrent buckets overlap on the device).
 * Admission: first come, first served, while the device footprint of the
   resident scenarios (frehd_dev_footprint) stays within the budget and the
   bucket has a free slot; a request that does not fit waits at the head of
@@ -62,7 +63,7 @@ def footprint(scen, **over) -> int:
 
 
 class Server:
-    def __init__(self, budget_bytes=200e9, buckets=(128, 256, 512, 1024, 2048, 4096), quantum=16,
+    def __init__(self, budget_bytes=200e9, buckets=(128, 256, 384, 512, 640, 768, 896, 1024, 2048, 4096), quantum=16,
                  capacity=128, collect=None, progress=0.0, **over):
         self.budget = float(budget_bytes)
         self.edges = list(buckets)
@@ -230,6 +231,8 @@ def main():
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--forms", default="fp64,mixed_warm")
     ap.add_argument("--progress", type=float, default=15.0, help="seconds between progress lines (0: none)")
+    ap.add_argument("--buckets", default="128,256,384,512,640,768,896,1024,2048,4096",
+                    help="bucket edges (grid side), comma-separated")
     ap.add_argument("--tables", action="store_true",
                     help="subgrid level tables in HBM (20 KB per cell at 250 levels) instead of evaluating the "
                          "levels from the pixels at each lookup")
@@ -243,7 +246,8 @@ def main():
     out = {}
     for f in a.forms.split(","):
         fresh = [Request(rid=r.rid, arrival=r.arrival, steps=r.steps, scen=r.scen) for r in reqs]
-        st = Server(budget_bytes=a.budget_gb * 1e9, quantum=a.quantum, progress=a.progress, **forms[f]).run(fresh)
+        st = Server(budget_bytes=a.budget_gb * 1e9, quantum=a.quantum, progress=a.progress,
+                    buckets=tuple(int(x) for x in a.buckets.split(",")), **forms[f]).run(fresh)
         out[f] = st
         print(json.dumps({f: st}), flush=True)
     print(json.dumps({"serving": out, "count": a.count, "rate": a.rate, "subgrid": a.subgrid}), flush=True)

@@ -62,8 +62,25 @@ def wind_marsh():
     return sc
 
 
+def three_scalar_marsh():
+    """The wet/dry marsh carrying three scalars in one pass: the boundary
+    tracer, a plume, and a patch above 100 that updateScalar's out-of-range
+    reset (scalar.c:392-402) removes, lowering those cells' surface and depth
+    for the scalars after it."""
+    sc = fast_marsh()
+    n = sc.nx
+    sc.cfg["nScalar"] = 3
+    yy, xx = np.mgrid[0:n, 0:n]
+    plume = np.exp(-((xx - n / 2) ** 2 + (yy - n / 2) ** 2) / (2 * (n / 8) ** 2))
+    patch = np.where((abs(xx - n / 3) < 3) & (abs(yy - n / 2) < 3), 100.5, 5.0)
+    sc.S0cell = np.ascontiguousarray(np.concatenate([np.zeros(n * n), plume.reshape(-1), patch.reshape(-1)]))
+    sc.name += "_3scalars"
+    return sc
+
+
 def cases_small():
-    return [cases.basin(nx=40, ny=36, Nt=200), fast_marsh(), tvd_marsh(), wind_basin(), wind_marsh()]
+    return [cases.basin(nx=40, ny=36, Nt=200), fast_marsh(), tvd_marsh(), wind_basin(), wind_marsh(),
+            three_scalar_marsh()]
 
 
 def compare(dev_fields, ref_fields, names, tag, ulp=ULP_FIELDS, rel=None):
