@@ -14,8 +14,8 @@
  * expression (C evaluates a*b*c as (a*b)*c and a+b+c as (a+b)+c), so the
  * result matches the reference bit for bit; tests/test_oracle_vs_reference.py
  * checks that against the reference's own functions compiled into
- * oracle/_ref.  Where the reference reads outside an array (SURVEY-listed
- * quirks), the value read is defined here as 0.0 and the reference harness
+ * oracle/_ref.  Where the reference reads outside an array, the value read
+ * is defined here as 0.0 and the reference harness
  * pads every array with zeros so both agree:
  *   - velocityInterp reads uuXP[ii-1] at ii = 0 and vvYP[ii-nx] for ii < nx
  *     (nsfunctions.c:342-345);

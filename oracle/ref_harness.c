@@ -7,8 +7,8 @@
  * into oracle/_ref/libfrehd_ref.so.  Nothing of the reference is copied into
  * this repository; the built library stays out of git (.gitignore: *.so).
  *
- * The harness replaces only what cannot run as shipped (survey of the
- * reference's main in FREHD.c:66-118 and Init in initialize.c:40-155):
+ * The harness replaces only what cannot run as shipped (the reference's
+ * main in FREHD.c:66-118 and Init in initialize.c:40-155):
  *   - Config is filled directly instead of parsed from data.dat
  *     (ReadUserSettings/dateNum read an unterminated buffer, utilities.c:114);
  *     the never-assigned flags useEvap, useRain, useSubStep,

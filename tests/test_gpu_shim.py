@@ -1,5 +1,5 @@
-"""The oneCompleteStep shim (the reference's signature, nssolve.c:110-118;
-INTEGRATION.md shows the C code): per step the host state goes to the
+"""The oneCompleteStep shim (the reference's signature, nssolve.c:110-118,
+forwarding to the device library): per step the host state goes to the
 device (frehd_dev_upload), the four phases of oneCompleteStep and the
 device's monitorCFL with the step's end (phase 4) run there, the state comes
 back, and the host runs monitorCFL (nsfunctions.c:1192-1232, CFL = |u| dt/dx
