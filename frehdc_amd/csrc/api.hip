@@ -424,6 +424,9 @@ int check_config(const frehd_config *k)
     if (k->useSubDrag != 0 && (k->useSubgrid != 1 || k->subgridOnTheFly != 0))
         return fail(FREHD_EINVAL, "useSubDrag needs the subgrid with level tables (useSubgrid = 1, "
                                   "subgridOnTheFly = 0)");
+    if (k->precision == FREHD_PREC_MIXED && (int64_t)k->nx * k->ny * 4 >= (int64_t)1 << 31)
+        return fail(FREHD_EINVAL, "mixed precision addresses its FP32 vectors with 32-bit byte offsets "
+                                  "(nx*ny < 2^29)");
     if (k->precision == FREHD_PREC_MIXED && !(k->epsInner > 0 && k->epsInner < 1 && k->maxRefine >= 1))
         return fail(FREHD_EINVAL, "mixed precision needs 0 < epsInner < 1 and maxRefine >= 1");
     if (k->precision == FREHD_PREC_MIXED && (k->warmStart < 0 || k->warmStart > 2))

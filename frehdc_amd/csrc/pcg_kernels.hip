@@ -450,34 +450,74 @@ struct Cg1Row {
     float hP2, hR, hcXv, hcY, hPo, hcXo;    // lanes 0 / 63: column v and its outer neighbour o
 };
 
+// clang vector types (not HIP's float2 / float4 classes, whose member-wise
+// copies go through generic references and lose the global address space)
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef float vf4 __attribute__((ext_vector_type(4)));
 template <int W> struct VecOf;
 template <> struct VecOf<1> { typedef float T; };
-template <> struct VecOf<2> { typedef float2 T; };
-template <> struct VecOf<4> { typedef float4 T; };
+template <> struct VecOf<2> { typedef vf2 T; };
+template <> struct VecOf<4> { typedef vf4 T; };
 
 __device__ inline void unpack(float v, float (&o)[1]) { o[0] = v; }
-__device__ inline void unpack(float2 v, float (&o)[2]) { o[0] = v.x; o[1] = v.y; }
-__device__ inline void unpack(float4 v, float (&o)[4]) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
+__device__ inline void unpack(vf2 v, float (&o)[2]) { o[0] = v.x; o[1] = v.y; }
+__device__ inline void unpack(vf4 v, float (&o)[4]) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
 __device__ inline float pack(const float (&o)[1]) { return o[0]; }
-__device__ inline float2 pack(const float (&o)[2]) { return make_float2(o[0], o[1]); }
-__device__ inline float4 pack(const float (&o)[4]) { return make_float4(o[0], o[1], o[2], o[3]); }
+__device__ inline vf2 pack(const float (&o)[2]) { return vf2{o[0], o[1]}; }
+__device__ inline vf4 pack(const float (&o)[4]) { return vf4{o[0], o[1], o[2], o[3]}; }
+
+// Buffer loads of the N-float vectors.  The resource covers the array, and
+// an offset beyond it reads zeros: a row above or below the grid or a
+// column beyond it loads as 0 with no branch.  (A branch around the load
+// makes its registers a merge with the zero path, and the copy that merge
+// needs waits for the load at once, so the rows loaded ahead would not stay
+// in flight.)  Byte offsets are 32-bit: N < 2^29, checked at create.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr uint32_t OFF_NONE = 0x80000000u;
+
+__device__ inline Rsrc rsrc_of(const float *p, int n)
+{
+    // the pointers are the same in every lane (argument-block fields picked by
+    // the iteration number); say so, or the compiler wraps each load in a
+    // loop over the distinct resources of the wave
+    const uint64_t v = (uint64_t)p;
+    const uint64_t u = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    // word 3: 32-bit data format, the raw-buffer setting of the gfx9 family
+    return __builtin_amdgcn_make_buffer_rsrc((void *)u, (short)0, __builtin_amdgcn_readfirstlane(n * 4), 0x00020000);
+}
+
+__device__ inline float bload1(Rsrc r, uint32_t off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ inline void bload(Rsrc r, uint32_t off, float (&o)[1]) { o[0] = bload1(r, off); }
+__device__ inline void bload(Rsrc r, uint32_t off, float (&o)[2])
+{
+    unpack(__builtin_bit_cast(vf2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0)), o);
+}
+__device__ inline void bload(Rsrc r, uint32_t off, float (&o)[4])
+{
+    unpack(__builtin_bit_cast(vf4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0)), o);
+}
+
+// byte offset of (row, col), or OFF_NONE outside the grid or when !ok
+__device__ inline uint32_t off_of(int row, int col, int nx, int ny, bool ok)
+{
+    return (ok && row >= 0 && row < ny) ? (uint32_t)(row * nx + col) * 4u : OFF_NONE;
+}
 
 // row `row`, columns i .. i+W-1 (i a multiple of W, nx too), or zeros
 template <int W>
-__device__ inline void load_cols(const float *p, int row, int i, int nx, int ny, bool ok, float (&o)[W])
+__device__ inline void load_cols(Rsrc p, int row, int i, int nx, int ny, bool ok, float (&o)[W])
 {
-    if (ok && row >= 0 && row < ny) {
-        unpack(*reinterpret_cast<const typename VecOf<W>::T *>(p + (size_t)row * nx + i), o);
-    } else {
-#pragma unroll
-        for (int k = 0; k < W; k++) o[k] = 0.0f;
-    }
+    bload(p, off_of(row, i, nx, ny, ok), o);
 }
 
 template <int W>
 __device__ inline void store_cols(float *p, size_t c, const float (&v)[W])
 {
-    *reinterpret_cast<typename VecOf<W>::T *>(p + c) = pack(v);
+    gst(reinterpret_cast<typename VecOf<W>::T *>(p + c), pack(v));
 }
 
 // A' p at one cell, the terms in the owner's order (W, E, S, N)
@@ -578,42 +618,45 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
             if (valid)
                 for (int j = j0; j < j1; j++) store_cols<W>(X, (size_t)j * nx + i, z);
         } else {
-            const float *cXs = a.cX, *cYs = a.cY;
+            const Rsrc cXs = rsrc_of(a.cX, a.N), cYs = rsrc_of(a.cY, a.N);
             // halo lanes: v = the column beyond the wave, o = the one after it
             const int v = lane == 0 ? i0 - 1 : (lane == 63 ? i0 + 64 * W : -1);
             const bool halo = v >= 0 && v < nx;
             const int o = lane == 0 ? v - 1 : v + 1;
             const bool oin = halo && o >= 0 && o < nx;
-            auto at = [&](const float *p, int row, int col, bool ok) -> float {
-                return (ok && row >= 0 && row < ny) ? p[(size_t)row * nx + col] : 0.0f;
+            const Rsrc rIn = rsrc_of(Rin, a.N), pIn = rsrc_of(Pin, a.N), xR = rsrc_of(X, a.N);
+            auto at = [&](Rsrc p, int row, int col, bool ok) -> float {
+                return bload1(p, off_of(row, col, nx, ny, ok));
             };
             auto load = [&](int t, Cg1Row<W> &L) {
-                load_cols<W>(Pin, t + 1, i, nx, ny, valid, L.P2);
-                load_cols<W>(Rin, t, i, nx, ny, valid, L.R);
+                load_cols<W>(pIn, t + 1, i, nx, ny, valid, L.P2);
+                load_cols<W>(rIn, t, i, nx, ny, valid, L.R);
                 load_cols<W>(cXs, t, i, nx, ny, valid, L.cX);
                 load_cols<W>(cYs, t, i, nx, ny, valid, L.cY);
-                load_cols<W>(X, t - 1, i, nx, ny, valid && !first && t - 1 >= j0, L.X);
-                L.hP2 = at(Pin, t + 1, v, halo);
-                L.hR = at(Rin, t, v, halo);
+                load_cols<W>(xR, t - 1, i, nx, ny, valid && !first && t - 1 >= j0, L.X);
+                L.hP2 = at(pIn, t + 1, v, halo);
+                L.hR = at(rIn, t, v, halo);
                 L.hcXv = at(cXs, t, v, halo);
                 L.hcY = at(cYs, t, v, halo);
-                L.hPo = at(Pin, t, o, oin);
-                L.hcXo = lane == 0 ? at(cXs, t, o, oin) : 0.0f;
+                L.hPo = at(pIn, t, o, oin);
+                L.hcXo = at(cXs, t, o, oin && lane == 0);
             };
             // window at the start: rows j0-2 and j0-1 of p, the south face of row j0-1
             float P0[W], P1[W], cYm[W], cYmm[W], cXc[W], pnm[W], pnc[W], rnc[W];
-            load_cols<W>(Pin, j0 - 2, i, nx, ny, valid, P0);
-            load_cols<W>(Pin, j0 - 1, i, nx, ny, valid, P1);
+            load_cols<W>(pIn, j0 - 2, i, nx, ny, valid, P0);
+            load_cols<W>(pIn, j0 - 1, i, nx, ny, valid, P1);
             load_cols<W>(cYs, j0 - 2, i, nx, ny, valid, cYm);
 #pragma unroll
             for (int k = 0; k < W; k++) cYmm[k] = cXc[k] = pnm[k] = pnc[k] = rnc[k] = 0.0f;
-            float hP0 = at(Pin, j0 - 2, v, halo), hP1 = at(Pin, j0 - 1, v, halo);
+            float hP0 = at(pIn, j0 - 2, v, halo), hP1 = at(pIn, j0 - 1, v, halo);
             float hcYm = at(cYs, j0 - 2, v, halo), hpn = 0.0f, hcVIprev = 0.0f;
-            // loads run two rows ahead of the arithmetic
-            Cg1Row<W> cur, nxt, nxt2;
-            load(j0 - 1, cur);
-            load(j0, nxt);
-            for (int t = j0 - 1; t <= j1; t++) {
+            // loads run two rows ahead of the arithmetic, into three row
+            // buffers that rotate by name (the loop body is instanced three
+            // times): a copy of a buffer still being loaded would wait for it
+            Cg1Row<W> b0, b1, b2;
+            load(j0 - 1, b0);
+            load(j0, b1);
+            auto row = [&](int t, const Cg1Row<W> &cur, Cg1Row<W> &nxt2) {
                 if (t + 2 <= j1) load(t + 2, nxt2);
                 // own columns: q_{it-1}, r_it, p_it of row t
                 float pWl = __shfl_up(P1[W - 1], 1, 64), pEl = __shfl_down(P1[0], 1, 64);
@@ -634,20 +677,20 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
                         advance(cur.R[k], q, P1[k], al, be, rn[k], pn[k]);
                     }
                 }
-                // halo column v of row t, the owner's expressions
-                float hpnT = 0.0f;
-                if (halo) {
-                    if (first) {
-                        hpnT = cur.hR;
-                    } else {
-                        const float hq = lane == 0
-                            ? apply_scaled(hP1, cur.hcXo, cur.hPo, cur.hcXv, P1[0], hcYm, hP0, cur.hcY, cur.hP2)
-                            : apply_scaled(hP1, cur.cX[W - 1], P1[W - 1], cur.hcXv, cur.hPo, hcYm, hP0, cur.hcY,
-                                           cur.hP2);
-                        float hr;
-                        advance(cur.hR, hq, hP1, al, be, hr, hpnT);
-                    }
+                // halo column v of row t, the owner's expressions, evaluated
+                // in every lane and kept by lanes 0 and 63 (the other lanes'
+                // halo loads read zeros): no divergent branch in the row walk
+                float hpnT;
+                if (first) {
+                    hpnT = cur.hR;
+                } else {
+                    const float hq0 = apply_scaled(hP1, cur.hcXo, cur.hPo, cur.hcXv, P1[0], hcYm, hP0, cur.hcY, cur.hP2);
+                    const float hq63 = apply_scaled(hP1, cur.cX[W - 1], P1[W - 1], cur.hcXv, cur.hPo, hcYm, hP0,
+                                                    cur.hcY, cur.hP2);
+                    float hr;
+                    advance(cur.hR, lane == 0 ? hq0 : hq63, hP1, al, be, hr, hpnT);
                 }
+                hpnT = halo ? hpnT : 0.0f;
                 // finish row j = t-1: q_it = A' p_it, stores, partials
                 const int j = t - 1;
                 float qWl = __shfl_up(pnc[W - 1], 1, 64), qEl = __shfl_down(pnc[0], 1, 64);
@@ -687,8 +730,13 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
                 hcYm = cur.hcY;
                 hpn = hpnT;
                 hcVIprev = cur.hcXv;
-                cur = nxt;
-                nxt = nxt2;
+            };
+            for (int t = j0 - 1; t <= j1; t += 3) {
+                row(t, b0, b2);
+                if (t + 1 > j1) break;
+                row(t + 1, b1, b0);
+                if (t + 2 > j1) break;
+                row(t + 2, b2, b1);
             }
         }
     }

@@ -150,6 +150,23 @@ __host__ __device__ inline size_t tab_index(size_t c, int k, size_t N, int T)
     return ((size_t)k * N + c) * 10 + (size_t)T;
 }
 
+// Loads and stores through the global address space.  The kernels read their
+// array pointers out of argument blocks in memory, so the compiler sees
+// generic pointers and emits flat instructions; a flat access also counts
+// against the LDS / scalar wait counter, so the wait before each cross-lane
+// shuffle (ds_bpermute) would drain every load issued ahead of it.
+#define FREHD_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ inline T gld(const T *p)
+{
+    return *(const FREHD_GLOBAL T *)p;
+}
+template <typename T>
+__device__ inline void gst(T *p, const T &v)
+{
+    *(FREHD_GLOBAL T *)p = v;
+}
+
 // One Jacobi-PCG solve over device vectors of type T (double for the FP64
 // path, float for the inner solve of the mixed-precision path).  Operator
 // row c (= j*nx + i): W -cX[c-1], S -cY[c-nx], C D[c], N -cY[c], E -cX[c];
