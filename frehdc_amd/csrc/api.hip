@@ -2,6 +2,9 @@
 // creation (Init restated on the host), device layout, upload, download,
 // statistics, and groups of handles stepped together (frehd_batch_t).  The
 // kernel sequencing itself is in exec.hip.
+// host code only (no kernels): the argument blocks' pointers stay plain in
+// both compilation passes
+#define FREHD_HOST_SOURCE
 #include <hip/hip_runtime.h>
 
 #include <math.h>

@@ -10,6 +10,21 @@
 
 namespace frehd {
 
+// Pointer to device memory in the argument blocks.  The kernels read these
+// pointers out of memory, so with plain pointers the compiler cannot tell
+// global memory from LDS or scratch and emits flat instructions, and a flat
+// access also counts against the wait counter of LDS and scalar loads: every
+// scalar load of another pointer (and every cross-lane shuffle) would first
+// drain all loads in flight.  Typed for the global address space in the
+// device pass, the accesses become global loads and stores; the host pass
+// sees plain pointers of the same size, and so does a source holding only
+// host code (FREHD_HOST_SOURCE), which fills the blocks.
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(FREHD_HOST_SOURCE)
+template <class T> using gptr = __attribute__((address_space(1))) T *;
+#else
+template <class T> using gptr = T *;
+#endif
+
 // scalar settings (the Config fields the step reads)
 struct P {
     double dx, dy, dt, g, NUx, NUy, CDx, CDy, manningN, hD, CDmax, minDepth, wtfh, CFLl, CFLh;
@@ -78,28 +93,28 @@ struct Ctl {
 struct StepArgs {
     Grid g;
     P p;
-    Ctl *ctl;
+    gptr<Ctl> ctl;
     // Ng arrays
-    double *surf, *surfOld, *uuXP, *vvYP, *depthXP, *depthYP, *EnXP, *EnYP, *DragXP, *DragYP;
-    double *CDXP, *CDYP, *Fuu, *Fvv, *CFLx, *CFLy, *wtfXP, *wtfYP, *bottomZ, *bottomXP, *bottomYP;
+    gptr<double> surf, surfOld, uuXP, vvYP, depthXP, depthYP, EnXP, EnYP, DragXP, DragYP;
+    gptr<double> CDXP, CDYP, Fuu, Fvv, CFLx, CFLy, wtfXP, wtfYP, bottomZ, bottomXP, bottomYP;
     // nScalar arrays with stride sStride; the concentration is
     // double-buffered (S, S2; Ctl.sParity selects the step's start), the
     // scalar mass Sm is not
-    double *S, *S2, *Sm;
+    gptr<double> S, S2, Sm;
     int64_t sStride;
     // interior arrays; the depth is double-buffered: Ctl.parity selects the
     // buffer holding the depth at the start of the step
-    double *uuYP, *vvXP, *depthA, *depthB, *cellV, *manning;
-    uint8_t *rmv, *flags;     // flags: bit0 tideP cell, bit1 tideM cell, bits 2-7 inflow count
+    gptr<double> uuYP, vvXP, depthA, depthB, cellV, manning;
+    gptr<uint8_t> rmv, flags;      // flags: bit0 tideP cell, bit1 tideM cell, bits 2-7 inflow count
     // operator and solver vectors
-    double *cX, *cY, *D, *b, *x, *r, *q, *pA, *pB;
-    float *cX32, *cY32;       // mixed precision only (else NULL): the scaled FP32
-                              // operator of the inner solve, written by k_refine
-    double *xB;                // mixed precision: second refinement x buffer
+    gptr<double> cX, cY, D, b, x, r, q, pA, pB;
+    gptr<float> cX32, cY32;        // mixed precision only (else NULL): the scaled FP32
+                                   // operator of the inner solve, written by k_refine
+    gptr<double> xB;               // mixed precision: second refinement x buffer
     // per-block partial sums: solver (partA: r.r, r.z; partB: p.q), volume
     // losses (vpart: 4 slots) and CFL maxima (cflpart: 2 slots), each slot
     // maxBlocks long; nb1 = blocks of an Ng sweep, nb2 = blocks of a 2-D sweep
-    double *partA, *partB, *vpart, *cflpart;
+    gptr<double> partA, partB, vpart, cflpart;
     int maxBlocks, nb1, nb2, pad3;
     // this member's launch extents (a batched launch is sized for its
     // largest member; blocks past these exit at once): 2-D stencil grid of
@@ -107,35 +122,35 @@ struct StepArgs {
     // the grid-stride solver grid
     int gx2, gy2, nbN, nbGhost, nbTide, nbRes;
     // forcing (Nt+1 samples) and locations
-    const double *tideP, *tideM, *inflow, *tidalPS, *tidalMS;
+    gptr<const double> tideP, tideM, inflow, tidalPS, tidalMS;
     // wind per step: speed and the cosine / sine of its direction (computed
     // on the host with the reference's expression and libm)
-    const double *windSpd, *windCos, *windSin;
-    const int *tideLocP, *tideLocM;
+    gptr<const double> windSpd, windCos, windSin;
+    gptr<const int> tideLocP, tideLocM;
     // subgrid: state (Ng), half-cell values (N), level search index (Ng),
     // face bottoms with offset (N), level elevations (Ne), level tables
     // V Z Vxp Vxm Vyp Vym Np Nm Op Om interleaved per cell and level:
     // tabs[(c*Ne + k)*10 + T], so the two levels a cell interpolates between
     // are 160 contiguous bytes (the reference's files are level-major,
     // [k*N + c], and are reordered on upload)
-    double *V, *Z, *Vx, *Vy, *Zx, *Zy, *Nx, *Oy;
-    double *Vxp, *Vxm, *Vyp, *Vym, *Np, *Nm, *Op, *Om;
+    gptr<double> V, Z, Vx, Vy, Zx, Zy, Nx, Oy;
+    gptr<double> Vxp, Vxm, Vyp, Vym, Np, Nm, Op, Om;
     // computeSubArea's own V and Z (before combineSubArea clips them in
     // place), and the inputs it last saw per cell (surface, dry-start flag):
     // a cell whose inputs are bitwise unchanged keeps its outputs
-    double *Vraw, *Zraw, *hSub;
-    uint8_t *dSub;
-    int *ind;
-    const double *edgeX, *edgeY, *allSurf;
-    const double *tabs;
+    gptr<double> Vraw, Zraw, hSub;
+    gptr<uint8_t> dSub;
+    gptr<int> ind;
+    gptr<const double> edgeX, edgeY, allSurf;
+    gptr<const double> tabs;
     // drag-model tables (level-major [k*N + c]) and the values at the lower
     // bracketing level of each cell (useSubDrag 2: Yh; 3, 4: Cn)
-    const double *allYhX, *allYhY, *allCn;
-    double *YhX, *YhY, *Cn;
+    gptr<const double> allYhX, allYhY, allCn;
+    gptr<double> YhX, YhY, Cn;
     // minimum / maximum face areas and minimum half-cell volumes (level-major
     // tables, per-cell values of the step) when the combine uses them
-    const double *allNmin, *allNmax, *allOmin, *allOmax, *allMinVxp, *allMinVxm, *allMinVyp, *allMinVym;
-    double *Nmin, *Nmax, *Omin, *Omax, *minVxp, *minVxm, *minVyp, *minVym;
+    gptr<const double> allNmin, allNmax, allOmin, allOmax, allMinVxp, allMinVxm, allMinVyp, allMinVym;
+    gptr<double> Nmin, Nmax, Omin, Omax, minVxp, minVxm, minVyp, minVym;
     // instead of tabs (NULL): the fine bathymetry, from which a lookup
     // evaluates its two levels (cell_levels, bitwise the table values)
     FineBed bed;
@@ -150,11 +165,8 @@ __host__ __device__ inline size_t tab_index(size_t c, int k, size_t N, int T)
     return ((size_t)k * N + c) * 10 + (size_t)T;
 }
 
-// Loads and stores through the global address space.  The kernels read their
-// array pointers out of argument blocks in memory, so the compiler sees
-// generic pointers and emits flat instructions; a flat access also counts
-// against the LDS / scalar wait counter, so the wait before each cross-lane
-// shuffle (ds_bpermute) would drain every load issued ahead of it.
+// Loads and stores through the global address space from a plain pointer
+// (the same reason as gptr).
 #define FREHD_GLOBAL __attribute__((address_space(1)))
 template <typename T>
 __device__ inline T gld(const T *p)
@@ -174,16 +186,16 @@ __device__ inline void gst(T *p, const T &v)
 // dry subgrid cells).
 template <typename T>
 struct PcgArgs {
-    const T *cX, *cY, *D;
-    T *r, *q, *x;
-    T *pBuf[2];               // iteration it reads p from pBuf[it&1], writes pBuf[(it+1)&1]
-    double *partA;            // r.r, r.z partials: of the start residual (nbStart
-                              // blocks) at iteration 0, of k_pcg_res (nbRes) after
-    double *partB;            // p.q partials of k_pcg_dir (nbDir blocks)
-    PcgScalars *sc;
-    const int *halted;
-    const int *outerDone;     // inner solve: the refinement has converged (>= 0)
-    const double *epsDyn;     // inner solve: tolerance set by the refinement test
+    gptr<const T> cX, cY, D;
+    gptr<T> r, q, x;
+    gptr<T> pBuf[2];               // iteration it reads p from pBuf[it&1], writes pBuf[(it+1)&1]
+    gptr<double> partA;            // r.r, r.z partials: of the start residual (nbStart
+                                   // blocks) at iteration 0, of k_pcg_res (nbRes) after
+    gptr<double> partB;            // p.q partials of k_pcg_dir (nbDir blocks)
+    gptr<PcgScalars> sc;
+    gptr<const int> halted;
+    gptr<const int> outerDone;     // inner solve: the refinement has converged (>= 0)
+    gptr<const double> epsDyn;     // inner solve: tolerance set by the refinement test
     double eps;
     int maxIter, nx, ny, N;
     // k_pcg_dir / k_cg1 tiling: gx x gy tiles of 256 lanes columns x H rows
@@ -191,8 +203,8 @@ struct PcgArgs {
     // != 0 gives each XCD a contiguous band of tiles (nbDir a multiple of 8)
     int gx, gy, H, remap, lanes;
     int nbDir, nbRes, nbStart;
-    double *partF[2];         // k_cg1: p.q, r.q, q.q, r.r partials (nbDir each),
-                              // launch it writes partF[it&1]
+    gptr<double> partF[2];         // k_cg1: p.q, r.q, q.q, r.r partials (nbDir each),
+                                   // launch it writes partF[it&1]
 };
 
 template <typename T> __global__ void k_pcg_dir(const PcgArgs<T> *__restrict__ A, int it);
@@ -207,14 +219,14 @@ template <bool First, int W> __global__ void k_cg1(const PcgArgs<float> *__restr
 // residual r = b - A x_s with the FP64 operator; k_refine_test decides
 // whether the solve has converged and sets up the next inner solve.
 struct RefineArgs {
-    const double *cX, *cY, *D, *b;
-    double *xbuf[2];          // x_s lives in xbuf[s & 1]
-    const float *d;           // inner correction
-    float *r32;               // scaled residual, rounded: the next inner right-hand side
-    float *cXs, *cYs;         // scaled FP32 coefficients of the inner solve (written at s = 0)
-    double *partA;            // r32.r32 partials (inner start)
-    double *partC;            // b.b, r.r partials (FP64)
-    Ctl *ctl;
+    gptr<const double> cX, cY, D, b;
+    gptr<double> xbuf[2];          // x_s lives in xbuf[s & 1]
+    gptr<const float> d;           // inner correction
+    gptr<float> r32;               // scaled residual, rounded: the next inner right-hand side
+    gptr<float> cXs, cYs;          // scaled FP32 coefficients of the inner solve (written at s = 0)
+    gptr<double> partA;            // r32.r32 partials (inner start)
+    gptr<double> partC;            // b.b, r.r partials (FP64)
+    gptr<Ctl> ctl;
     double eps, epsInner;
     int maxRefine, nx, ny, N;
     int gx, gy, H, remap, nb, pad;
@@ -223,14 +235,14 @@ struct RefineArgs {
 __global__ void k_refine(const RefineArgs *__restrict__ A, int s);
 __global__ void k_refine_test(const RefineArgs *__restrict__ A, int s);
 
-__device__ inline const double *depth_cur(const StepArgs &a) { return a.ctl->parity ? a.depthB : a.depthA; }
-__device__ inline double *depth_nxt(const StepArgs &a) { return a.ctl->parity ? a.depthA : a.depthB; }
+__device__ inline gptr<const double> depth_cur(const StepArgs &a) { return a.ctl->parity ? a.depthB : a.depthA; }
+__device__ inline gptr<double> depth_nxt(const StepArgs &a) { return a.ctl->parity ? a.depthA : a.depthB; }
 // the kernels after the solve run only for a member that is not halted,
 // whose solve has finished and whose step they have not completed yet (a
 // group launches them again when one member's solve needed more iterations)
 __device__ inline bool tail_off(const StepArgs &a) { return a.ctl->halted || a.ctl->open || a.ctl->stepDone; }
-__device__ inline const double *s_cur(const StepArgs &a) { return a.ctl->sParity ? a.S2 : a.S; }
-__device__ inline double *s_nxt(const StepArgs &a) { return a.ctl->sParity ? a.S : a.S2; }
+__device__ inline gptr<const double> s_cur(const StepArgs &a) { return a.ctl->sParity ? a.S2 : a.S; }
+__device__ inline gptr<double> s_nxt(const StepArgs &a) { return a.ctl->sParity ? a.S : a.S2; }
 
 enum SubTable { T_V, T_Z, T_VXP, T_VXM, T_VYP, T_VYM, T_NP, T_NM, T_OP, T_OM };
 
