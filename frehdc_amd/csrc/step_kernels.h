@@ -10,21 +10,6 @@
 
 namespace frehd {
 
-// Pointer to device memory in the argument blocks.  The kernels read these
-// pointers out of memory, so with plain pointers the compiler cannot tell
-// global memory from LDS or scratch and emits flat instructions, and a flat
-// access also counts against the wait counter of LDS and scalar loads: every
-// scalar load of another pointer (and every cross-lane shuffle) would first
-// drain all loads in flight.  Typed for the global address space in the
-// device pass, the accesses become global loads and stores; the host pass
-// sees plain pointers of the same size, and so does a source holding only
-// host code (FREHD_HOST_SOURCE), which fills the blocks.
-#if defined(__HIP_DEVICE_COMPILE__) && !defined(FREHD_HOST_SOURCE)
-template <class T> using gptr = __attribute__((address_space(1))) T *;
-#else
-template <class T> using gptr = T *;
-#endif
-
 // scalar settings (the Config fields the step reads)
 struct P {
     double dx, dy, dt, g, NUx, NUy, CDx, CDy, manningN, hD, CDmax, minDepth, wtfh, CFLl, CFLh;

@@ -874,17 +874,49 @@ __device__ inline double interp_surf(double x, double x1, double y1, double x2, 
     return (y2 - y1) * (x - x2) / (x2 - x1) + y2;
 }
 
+// The inputs of cell c's lookup that do not depend on its new surface,
+// loaded together before the surface is known: the lookup's chain of
+// dependent memory round trips is then this one, the level walk (cached)
+// and the table reads.  edgeX/edgeY of a ghost neighbour read 0 (the
+// reference reads past its N-array); the face bottoms of a ghost neighbour
+// are not read.
+struct SubPre {
+    double hSub, eXc, eXw, eYc, eYs, bXc, bYc, bXw, bYs;
+    int ind;
+    uint8_t dSub;
+};
+
+__device__ inline SubPre subarea_pre(const StepArgs &a, int c, int i, int j)
+{
+    const Grid &g = a.g;
+    const int N = g.N;
+    const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
+    SubPre q;
+    q.hSub = a.hSub[c];
+    q.dSub = a.dSub[c];
+    q.ind = a.ind[c];
+    q.eXc = a.edgeX[c];
+    q.eXw = w < N ? a.edgeX[w] : 0.0;
+    q.eYc = a.edgeY[c];
+    q.eYs = s < N ? a.edgeY[s] : 0.0;
+    q.bXc = a.bottomXP[c];
+    q.bYc = a.bottomYP[c];
+    q.bXw = w < N ? a.bottomXP[w] : 0.0;
+    q.bYs = s < N ? a.bottomYP[s] : 0.0;
+    return q;
+}
+
 // computeSubArea for cell c (subgrid.c:55-251): walk the level index to the
 // surface, interpolate the half-cell volumes and face areas between the two
-// bracketing levels, zero what is dry.  edgeX/edgeY of a ghost neighbour
-// read 0 (the reference reads past its N-array).
-__device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, double h, double depthOld)
+// bracketing levels, zero what is dry.
+__device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, double h, double depthOld,
+                                    const SubPre &q)
 {
     // the outputs are a function of (h, depthOld == 0) and constant inputs,
     // and the level walk from the last index with the same h stops at once:
     // bitwise-equal inputs give bitwise-equal outputs, so skip the lookup
     const uint8_t dry = depthOld == 0;
-    if (__double_as_longlong(a.hSub[c]) == __double_as_longlong(h) && a.dSub[c] == dry) return;
+    if (__double_as_longlong(q.hSub) == __double_as_longlong(h) && q.dSub == dry) return;
     a.hSub[c] = h;
     a.dSub[c] = dry;
     const Grid &g = a.g;
@@ -892,7 +924,7 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
     const int N = g.N, Ne = p.Ne;
     const double dsurf = p.dsurf;
     const double *as = a.allSurf;
-    int ind = a.ind[c];
+    int ind = q.ind;
     while (fabs(as[ind] - h) > 0.5 * dsurf + dsurf / 100) {
         if (h > as[ind]) ind += 1;
         else ind -= 1;
@@ -954,7 +986,7 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
         }
     }
     const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
-    const double wd[4] = {a.edgeX[c], w < N ? a.edgeX[w] : 0.0, a.edgeY[c], s < N ? a.edgeY[s] : 0.0};
+    const double wd[4] = {q.eXc, q.eXw, q.eYc, q.eYs};
     const int tid[4] = {T_NP, T_NM, T_OP, T_OM};
     double f[4];
 #pragma unroll
@@ -966,10 +998,10 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
         }
     }
     if (depthOld == 0) { vxp = 0; vyp = 0; vxm = 0; vym = 0; }
-    if (h - a.bottomXP[c] <= 0) f[0] = 0;
-    if (h - a.bottomYP[c] <= 0) f[2] = 0;
-    if (w < N && h - a.bottomXP[w] <= 0) f[1] = 0;
-    if (s < N && h - a.bottomYP[s] <= 0) f[3] = 0;
+    if (h - q.bXc <= 0) f[0] = 0;
+    if (h - q.bYc <= 0) f[2] = 0;
+    if (w < N && h - q.bXw <= 0) f[1] = 0;
+    if (s < N && h - q.bYs <= 0) f[3] = 0;
     a.Vxp[c] = vxp; a.Vyp[c] = vyp; a.Vxm[c] = vxm; a.Vym[c] = vym;
     a.Np[c] = f[0]; a.Nm[c] = f[1]; a.Op[c] = f[2]; a.Om[c] = f[3];
 }
@@ -982,7 +1014,8 @@ __global__ void __launch_bounds__(256) k_subarea_init(const StepArgs *__restrict
     const Grid g = a.g;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
-    subarea_cell(a, c, c % g.nx, c / g.nx, a.surf[c], depth_cur(a)[c]);
+    const int i = c % g.nx, j = c / g.nx;
+    subarea_cell(a, c, i, j, a.surf[c], depth_cur(a)[c], subarea_pre(a, c, i, j));
 }
 
 // limiterCFL + computeVolumeSub + computeSubArea + updateDepth
@@ -1000,17 +1033,31 @@ __global__ void __launch_bounds__(256) k_limiter_sub(const StepArgs *__restrict_
     if (i < g.nx && j < g.ny) {
         const P &p = a.p;
         const int c = j * g.nx + i;
+        // every load that does not depend on the new surface first, so that
+        // one round trip covers them (a wave waits at the first use of each
+        // load, and a load after a store to another array cannot be moved
+        // above it)
+        const SubPre pre = subarea_pre(a, c, i, j);
+        const double cellV = a.V[c];   // the previous step's subgrid volume
+        const double zb = a.bottomZ[c];
+        const double dOld = depth_cur(a)[c];
+        constexpr int NS = 4;          // scalars loaded up front
+        const size_t ss = a.sStride;
+        double sv[NS];
+#pragma unroll
+        for (int q = 0; q < NS; q++) sv[q] = q < p.nScalar ? s_cur(a)[q * ss + c] : 0.0;
         uint8_t rmv;
         const double h = limiter_cell(a, c, i, j, loss, rmv);
+        const double d = h - zb;
         a.surf[c] = h;
         a.rmv[c] = rmv;
-        const double cellV = a.V[c];   // the previous step's subgrid volume
         a.cellV[c] = cellV;
-        for (int q = 0; q < p.nScalar; q++)
-            a.Sm[(size_t)q * a.sStride + c] = s_cur(a)[(size_t)q * a.sStride + c] * cellV;
-        subarea_cell(a, c, i, j, h, depth_cur(a)[c]);
-        const double d = h - a.bottomZ[c];
+#pragma unroll
+        for (int q = 0; q < NS; q++)
+            if (q < p.nScalar) a.Sm[q * ss + c] = sv[q] * cellV;
+        for (int q = NS; q < p.nScalar; q++) a.Sm[q * ss + c] = s_cur(a)[q * ss + c] * cellV;
         depth_nxt(a)[c] = d < 0 ? 0 : d;
+        subarea_cell(a, c, i, j, h, dOld, pre);
     }
     store_vloss2(a, loss, lds);
 }

@@ -25,6 +25,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "frehd_dev.h"
+
 namespace frehd {
 
 // raw elevation of level k, two roundings (no fused multiply-add)
@@ -36,7 +38,7 @@ __host__ __device__ inline double level_raw(double surfmin, double dsurf, int k)
 }
 
 struct FineBed {
-    const double *z;   // (ny*r) rows of (nx*r) raw elevations, x fastest
+    gptr<const double> z;   // (ny*r) rows of (nx*r) raw elevations, x fastest
     int nx, ny, r;
     double dxf;
 };

@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(256) k_subgrid_tables(TableGenArgs a)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= a.nx || j >= a.ny) return;
-    const FineBed bed{a.fine, a.nx, a.ny, a.r, a.dxf};
+    const FineBed bed{(gptr<const double>)a.fine, a.nx, a.ny, a.r, a.dxf};
     const size_t c = (size_t)j * a.nx + i;
     for (int k = a.k0; k < a.k1; k++) {
         const double h[1] = {level_raw(a.surfmin, a.dsurf, k)};
