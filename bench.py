@@ -43,7 +43,50 @@ PCG_ITER_BYTES_F32 = 4 * 8
 # predictor + assembly and update + one scalar (11 + 12 array accesses + 3 for
 # the solver start), frozen per cell-step
 STEP_BYTES_FP64 = 8 * 26
+# each scalar beyond the first: read and write its concentration
+SCALAR_BYTES = 8 * 2
+# one FP64 refinement sweep of the mixed form: residual R {x, b, cX, cY}
+# (FP64) + W r (FP32), correction R {x, d} + W x
+REFINE_SWEEP_BYTES = 8 * 6 + 4 * 2
 HBM_PEAK_GBS = 8000.0
+
+
+def step_bytes(cfg, mixed, k, m):
+    """Algorithmic HBM bytes of one cell-step: every array each phase needs,
+    read or written once per phase (stencil halos free).  Subgrid adds the
+    two bracketing levels of T tables, the level index and the eight face /
+    volume arrays written by the update and read by the predictor: T = 10,
+    + 4 minimum face areas when useminA, + 4 minimum volumes when they can
+    change a result (useminV = 2, or useminV = 1 with useminA: without the
+    minimum areas the volume adjustment never fires, and the reference's
+    unconditional reads of those tables are not counted).  k solver
+    iterations per step (FP64 64 B, inner FP32 32 B per cell) and m
+    refinement sweeps.  The predictor and update arrays are counted at FP64
+    in both forms: the mixed form keeps them in FP64."""
+    b = STEP_BYTES_FP64 + SCALAR_BYTES * max(cfg.get("nScalar", 1) - 1, 0)
+    if cfg.get("useSubgrid", 0):
+        minA = cfg.get("useminA", 0) == 1
+        minV = cfg.get("useminV", 0) == 2 or (cfg.get("useminV", 0) == 1 and minA)
+        T = 10 + (4 if minA else 0) + (4 if minV else 0)
+        b += 8 * (2 * T + 16) + 8
+    if mixed:
+        b += PCG_ITER_BYTES_F32 * k + REFINE_SWEEP_BYTES * m
+    else:
+        b += PCG_ITER_BYTES * k
+    return b
+
+
+def step_roofline(cfg, N, r, mixed):
+    """The whole step against HBM bandwidth: algorithmic bytes per cell-step
+    (step_bytes, with the measured iterations and the refinement sweeps of
+    the kernel-timed steps) x cells / the timed ms per step."""
+    m = r["kt"].get("k_refine", (0, 0))[1] / 4.0 if mixed else 0.0
+    per_cell = step_bytes(cfg, mixed, r["iters"], m)
+    achieved = per_cell * N / (r["ms_per_step"] * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "whole step (all kernels)", "bytes_per_cell_step": round(per_cell, 1),
+            "pcg_iterations_per_step": round(r["iters"], 2), "refinement_sweeps_per_step": m,
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4)}
 
 
 def log(*a):
@@ -250,7 +293,8 @@ def large_domain(size, r, steps, warmup):
         roof["traffic"] = traffic_of(ks, f"channel{size}_sub{r}")
         out[f] = {"cell_steps_per_s": r_["value"], "ms_per_step": r_["ms_per_step"],
                   "pcg_iterations_per_step": round(r_["iters"], 2), "refinement_sweeps": r_["sweeps"],
-                  "rel_residual_last": r_["relres"], "tiling": r_["tiling"], "solver_roofline": roof}
+                  "rel_residual_last": r_["relres"], "tiling": r_["tiling"], "solver_roofline": roof,
+                  "step_roofline": step_roofline(sc.cfg, sc.N, r_, f != "fp64")}
         log(f"bench: large {f}: {r_['ms_per_step']:.2f} ms/step, {r_['iters']:.1f} iterations/step, "
             f"solver {roof['achieved']:.0f} GB/s ({roof['frac']:.3f} of peak)")
     N = sc.N
@@ -333,7 +377,8 @@ def main():
     for f, r in res.items():
         o = {"cell_steps_per_s": r["value"], "ms_per_step": r["ms_per_step"],
              "pcg_iterations_per_step": round(r["iters"], 2), "refinement_sweeps": r["sweeps"],
-             "rel_residual_last": r["relres"], "tiling": r["tiling"]}
+             "rel_residual_last": r["relres"], "tiling": r["tiling"],
+             "step_roofline": step_roofline(scen.cfg, N, r, f != "fp64")}
         if f != "fp64" and "fp64" in res:
             o["max_abs_eta_err_m"] = err(f, "surf")
             o["max_abs_u_err"] = err(f, "uuXP")
