@@ -463,8 +463,7 @@ static int exec_phase_launch(Exec &e, int phase)
     case 2: { // updateData (+ the CFL numbers of monitorCFL)
         if (e.sub) {
             LAUNCH(K_LIM, k_limiter_sub, g2, b2, e.dArgs);
-            LAUNCH(K_COMBINE, k_combine, grid1(x.nbN + x.nbGhost, n), b1, e.dArgs);
-            LAUNCH(K_FACES, k_faces_sub, grid1(x.nb1, n), b1, e.dArgs);
+            LAUNCH(K_FACES, k_combine_faces, grid1(x.nb1, n), b1, e.dArgs);
             if (x.nbTide > 0) LAUNCH(K_TIDEV, k_tidal_velocity_sub, grid1(x.nbTide, n), b1, e.dArgs);
             LAUNCH(K_FINISH, k_finish_sub, grid1(x.nb1, n), b1, e.dArgs);
         } else {

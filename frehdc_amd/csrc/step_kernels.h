@@ -246,7 +246,7 @@ __global__ void k_assemble_sub(const StepArgs *__restrict__ A);
 __global__ void k_subarea_init(const StepArgs *__restrict__ A);
 __global__ void k_limiter_sub(const StepArgs *__restrict__ A);
 __global__ void k_combine(const StepArgs *__restrict__ A);
-__global__ void k_faces_sub(const StepArgs *__restrict__ A);
+__global__ void k_combine_faces(const StepArgs *__restrict__ A);
 __global__ void k_tidal_velocity_sub(const StepArgs *__restrict__ A);
 __global__ void k_finish_sub(const StepArgs *__restrict__ A);
 __global__ void k_scalar_sub(const StepArgs *__restrict__ A);

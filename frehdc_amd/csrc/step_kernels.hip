@@ -1226,9 +1226,10 @@ __global__ void __launch_bounds__(256) k_combine(const StepArgs *__restrict__ A)
 // (the Sub variant resets them before detecting); the ghost flags written by
 // the XM/YM tests of boundary cells stay set, as in the reference.
 // updateVelocitySub + waterfallCorrectionSub of interior cell c's east and
-// north faces (with the waterfall flags); a pure function of the step's
-// inputs, so a ghost face can recompute the face it mirrors
-__device__ inline FaceVel face_vel_sub(const StepArgs &a, int c)
+// north faces (with the waterfall flags), given the cell's combined face
+// values v; a pure function of the step's inputs, so a ghost face can
+// recompute the face it mirrors
+__device__ inline FaceVel face_vel_sub(const StepArgs &a, int c, const Combined &cv)
 {
     const Grid &g = a.g;
     const P &p = a.p;
@@ -1242,7 +1243,7 @@ __device__ inline FaceVel face_vel_sub(const StepArgs &a, int c)
     if (e < g.N && he < bXc - 0.0 && hc - bXc >= p.wtfh) fX = 1;
     if (hc < bYc - 0.0 && hn - bYc >= p.wtfh) fY = -1;
     if (n < g.N && hn < bYc - 0.0 && hc - bYc >= p.wtfh) fY = 1;
-    const double Nxc = a.Nx[c], Oyc = a.Oy[c], Vxc = a.Vx[c], Vyc = a.Vy[c];
+    const double Nxc = cv.Nx, Oyc = cv.Oy, Vxc = cv.Vx, Vyc = cv.Vy;
     const double effHx = Vxc > 0 ? Nxc / Vxc : 0;
     const double effHy = Vyc > 0 ? Oyc / Vyc : 0;
     const double divider = 1.0;
@@ -1284,20 +1285,57 @@ __device__ inline FaceVel face_vel_sub(const StepArgs &a, int c)
     return r;
 }
 
-// one Ng index: face depths, velocities, waterfall flags and flow rates
-// (computeFaceFlowRateSub, subgrid.c:1190-1212); a ghost face takes the
-// velocity of the face it mirrors (enforceVelocityBC), recomputed here.
-__device__ inline void faces_sub_cell(const StepArgs &a, int k, double &loss)
+// combineSubArea of one Ng index (an interior cell, or a ghost recomputing
+// the cell it mirrors; the four corner ghosts are not written), then its
+// face depths, velocities, waterfall flags and flow rates
+// (computeFaceFlowRateSub, subgrid.c:1190-1212).  The face velocities of a
+// cell need only that cell's combined values, so one pass does both; a
+// ghost face takes the velocity of the face it mirrors (enforceVelocityBC),
+// recomputed here from its source cell's combined values.
+__device__ inline void combine_faces_cell(const StepArgs &a, int k, double &loss)
 {
     const Grid &g = a.g;
     const P &p = a.p;
     const double dX = clipped(face_depth_x(a, k));
     const double dY = clipped(face_depth_y(a, k));
-    a.depthXP[k] = dX;
-    a.depthYP[k] = dY;
-    double u, v;
-    if (k >= g.N) {
-        int r = k - g.N;
+    double u, v, NxK, OyK;
+    if (k < g.N) {
+        const Combined cv = combine_cell(a, k);
+        a.Nx[k] = cv.Nx; a.Oy[k] = cv.Oy; a.Vx[k] = cv.Vx; a.Vy[k] = cv.Vy;
+        a.Zx[k] = cv.Zx; a.Zy[k] = cv.Zy; a.V[k] = cv.V; a.Z[k] = cv.Z;
+        const FaceVel f = face_vel_sub(a, k, cv);
+        a.wtfXP[k] = f.fX;
+        a.wtfYP[k] = f.fY;
+        u = f.u;
+        v = f.v;
+        NxK = cv.Nx;
+        OyK = cv.Oy;
+        loss += f.loss;
+    } else {
+        const int r = k - g.N;
+        const int sv = ghost_source(g, k, false), su = ghost_source(g, k, true);
+        const Combined cvv = combine_cell(a, sv);
+        if (k < g.Ng - 4) {
+            // combineSubArea's ghost copies (subgrid.c:480-506); the iM / iP
+            // ghosts of the top row take the corner cell's Oy before the
+            // top-row closure
+            a.V[k] = cvv.V; a.Z[k] = cvv.Z;
+            a.Vx[k] = cvv.Vx; a.Vy[k] = cvv.Vy;
+            a.Zx[k] = cvv.Zx; a.Zy[k] = cvv.Zy;
+            if (r < 2 * g.nx) {                      // jP / jM ghosts
+                NxK = cvv.Nx;
+                OyK = r < g.nx ? cvv.Oy : 0.0;
+            } else {                                 // iP / iM ghosts
+                const int jj = (r - 2 * g.nx) % g.ny;
+                NxK = 0.0;
+                OyK = jj == g.ny - 1 ? cvv.OyOpen : cvv.Oy;
+            }
+            a.Nx[k] = NxK;
+            a.Oy[k] = OyK;
+        } else {
+            NxK = a.Nx[k];
+            OyK = a.Oy[k];
+        }
         if (r >= g.nx && r < 2 * g.nx) {             // jM ghost: YM test of cell (i, 0)
             const int i = r - g.nx;
             const double bot = a.bottomYP[k];
@@ -1307,23 +1345,24 @@ __device__ inline void faces_sub_cell(const StepArgs &a, int k, double &loss)
             const double bot = a.bottomXP[k];
             if (a.surf[jj * g.nx] < bot - 0.0 && a.surf[k] - bot >= p.wtfh) a.wtfXP[k] = 1;
         }
-        u = face_vel_sub(a, ghost_source(g, k, true)).u;
-        v = face_vel_sub(a, ghost_source(g, k, false)).v;
-    } else {
-        const FaceVel f = face_vel_sub(a, k);
-        a.wtfXP[k] = f.fX;
-        a.wtfYP[k] = f.fY;
-        u = f.u;
-        v = f.v;
-        loss += f.loss;
+        u = face_vel_sub(a, su, su == sv ? cvv : combine_cell(a, su)).u;
+        v = face_vel_sub(a, sv, cvv).v;
     }
+    a.depthXP[k] = dX;
+    a.depthYP[k] = dY;
     a.uuXP[k] = u;
     a.vvYP[k] = v;
-    a.Fuu[k] = u * a.Nx[k];
-    a.Fvv[k] = v * a.Oy[k];
+    a.Fuu[k] = u * NxK;
+    a.Fvv[k] = v * OyK;
 }
 
-__global__ void __launch_bounds__(256) k_faces_sub(const StepArgs *__restrict__ A)
+// combineSubArea + updateAllFaceDepth (plain, nssolve.c:230) +
+// detectWaterfallLocationSub + updateVelocitySub + waterfallCorrectionSub
+// (subgrid.c:254-526, 1335-1444, 1084-1187) for the faces of every Ng index.
+// Interior flags are recomputed every step (the Sub variant resets them
+// before detecting); the ghost flags written by the XM/YM tests of boundary
+// cells stay set, as in the reference.
+__global__ void __launch_bounds__(256) k_combine_faces(const StepArgs *__restrict__ A)
 {
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
@@ -1331,7 +1370,7 @@ __global__ void __launch_bounds__(256) k_faces_sub(const StepArgs *__restrict__ 
     if (tail_off(a)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double loss[1] = {0.0};
-    if (k < a.g.Ng) faces_sub_cell(a, k, loss[0]);
+    if (k < a.g.Ng) combine_faces_cell(a, k, loss[0]);
     block_sum<1>(loss, lds);
     if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blockIdx.x] = loss[0];
 }
