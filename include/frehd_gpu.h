@@ -50,7 +50,9 @@ extern "C" {
 
 #define FREHD_PREC_FP64  0       /* every array and operation in FP64             */
 #define FREHD_PREC_MIXED 1       /* FP32 stencils / PCG, FP64 refinement, FP64
-                                    dot products, eta and volume accumulation      */
+                                    dot products, eta and volume accumulation;
+                                    nx*ny < 2^29 (32-bit byte offsets into the
+                                    FP32 vectors), else FREHD_EINVAL              */
 
 /* ---- configuration ----------------------------------------------------------
  * Scalar fields of the reference Config (FrehdC_v42/configuration.h:4-111) that

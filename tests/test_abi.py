@@ -101,3 +101,16 @@ def test_on_the_fly_footprint():
     abi.check(lib.frehd_dev_footprint(C.byref(cf), C.byref(f)))
     N = sc.N
     assert t.value - f.value == pytest.approx(8 * N * (10 * sc.cfg["Ne"] - 16), abs=4096)
+
+
+def test_mixed_precision_size_limit(lib):
+    # the inner FP32 solve addresses its vectors with 32-bit byte offsets
+    b = C.c_int64()
+    mixed = dict(Nt=10, dx=1.0, dy=1.0, dt=1.0, bcType=1, precision=abi.PREC_MIXED, epsInner=1e-6, maxRefine=8)
+    cfg = abi.FrehdConfig(nx=16384, ny=32768, **mixed)          # exactly 2^29 cells
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == abi.FREHD_EINVAL
+    assert b"2^29" in lib.frehd_last_error()
+    cfg = abi.FrehdConfig(nx=16384, ny=32767, **mixed)
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
+    cfg = abi.FrehdConfig(nx=16384, ny=32768, **dict(mixed, precision=abi.PREC_FP64))
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
