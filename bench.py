@@ -400,6 +400,14 @@ def main():
     roof["traffic"] = traffic
     if traffic:
         roof["traffic_GBps"] = round(traffic / (roof["ms_per_iteration"] * 1e-3) / 1e9, 1)
+    # where the step's time goes in the kernel-timed steps: the solver
+    # iteration's share, and the kernel with the largest share (whose bytes
+    # have no per-kernel frozen count; the whole step's is step_roofline)
+    shares = {k: v[0] * v[1] / 4.0 / hr["ms_per_step"] for k, v in hr["kt"].items()}
+    roof["step_share"] = round(sum(shares.get(k, 0.0) for k in ks), 3)
+    top = max(shares, key=shares.get)
+    roof["largest_kernel"] = {"kernel": top, "ms_per_launch": round(hr["kt"][top][0], 5),
+                              "step_share": round(shares[top], 3)}
 
     large = None
     if not args.no_large and args.large_size > 0:
