@@ -317,16 +317,17 @@ def channel_subgrid(n=4096, r=4, dx=1.0, dt=0.5, Nt=500, seed=4, hurst=0.8, amp=
     return sc
 
 
-def ensemble_member(m, n=2048, dx=1.0, dt=0.5, Nt=1000, seed=5, hurst=0.7, base=None):
+def ensemble_member(m, n=2048, dx=1.0, dt=0.5, Nt=1000, seed=5, hurst=0.7, base=None, period=TIDE_PERIOD):
     """Ensemble member m: shared fBm base plus a seeded N(0, 0.05) perturbation,
-    Manning n ~ U(0.015, 0.04) and tide amplitude ~ U(0.3, 0.8) per member."""
+    Manning n ~ U(0.015, 0.04) and tide amplitude ~ U(0.3, 0.8) per member
+    (tidal period `period`)."""
     if base is None:
         base = fbm_surface(n, n, hurst, np.random.default_rng(seed), -1.0, 1.5)
     r = np.random.default_rng(100 + m)
     bed = base + r.normal(0.0, 0.05, size=base.shape)
     bed[-INLET_ROWS:, :] = np.minimum(bed[-INLET_ROWS:, :], INLET_BED)   # always-wet tidal inlet
     cfg = default_config(n, n, dx, dt, Nt, manningN=float(r.uniform(0.015, 0.04)), eps=1e-10)
-    return _finish(cfg, bed, tide_series(Nt, dt, float(r.uniform(0.3, 0.8))),
+    return _finish(cfg, bed, tide_series(Nt, dt, float(r.uniform(0.3, 0.8)), period=period),
                    scalar0=np.zeros(n * n), name=f"member{m}")
 
 
@@ -483,6 +484,22 @@ def fine_basin(nx, ny, r, dx=10.0, seed=1, bumps=20):
     for a, b in zip(cx, cy):
         bed = bed + 0.3 * np.exp(-((x - a) ** 2 + (y - b) ** 2) / (2 * 30.0 ** 2))
     return bed + rng.normal(0.0, 0.05, size=bed.shape)
+
+
+def basin_subgrid(nx=128, ny=128, r=4, dx=10.0, dt=1.0, Nt=3600, seed=1, amp=0.2, dsurf=0.01,
+                  device_tables=False, on_the_fly=False):
+    """The basin workload on the subgrid model: r x r pixels per cell of
+    fine_basin's bed (slope, bumps, pixel roughness), levels from 0.5 m
+    below the deepest pixel to 0.5 m above the highest tide every dsurf;
+    staggeredV = 1, useminA = 0, useminV = 1, useSubDrag = 0 (the
+    scenario defaults)."""
+    sc = basin(nx=nx, ny=ny, dx=dx, dt=dt, Nt=Nt, seed=seed, amp=amp)
+    fine = fine_basin(nx, ny, r, dx=dx, seed=seed)
+    lo = round(math.floor((float(fine.min()) - 0.5) / dsurf) * dsurf, 9)
+    hi = round(math.ceil((amp + 0.5) / dsurf) * dsurf, 9)
+    sc = with_subgrid(sc, fine, r, lo, hi, dsurf, device_tables=device_tables, on_the_fly=on_the_fly)
+    sc.name = f"basin{nx}x{ny}_sub{r}"
+    return sc
 
 
 def fine_marsh(n, r, seed=2, hurst=0.7, lo=-1.0, hi=1.5):

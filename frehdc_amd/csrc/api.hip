@@ -658,6 +658,10 @@ int upload_state(frehd_dev *d, const frehd_fields &s, int depthParity = 0, int s
             if (rc) return rc;
         }
         if (s.ind) { int rc = upload(d, s.ind, a.ind, Ng * sizeof(int)); if (rc) return rc; }
+        // the lookup's skip cache holds results computed from the previous
+        // bottoms and level index: forget every cell (all-ones bits, a NaN
+        // no surface equals)
+        HIPCHK(hipMemsetAsync(a.hSub, 0xFF, sizeof(double) * N, d->stream));
     }
     d->midStep = 0;
     HIPCHK(hipStreamSynchronize(d->stream));
@@ -914,6 +918,19 @@ int autotune_group(Exec &e, int iters, int32_t *bestH, int32_t *bestRemap, int32
 
 }  // namespace
 
+// FREHD_ESTATE (with a message) when `call` is not valid in the handle's
+// current place inside a step or while it belongs to a batch
+static int state_check(const frehd_dev *d, StepCall call, int phase = 0)
+{
+    const int rc = step_call_check(d->midStep, d->batch != nullptr, call, phase);
+    if (rc == FREHD_EINVAL) return fail(rc, "phase %d out of range [0, 4]", phase);
+    if (rc == 0) return 0;
+    if (d->batch) return fail(rc, "the handle is in a batch: drive the batch");
+    if (call == CALL_PHASE)
+        return fail(rc, "phase %d again: %d phase(s) of the current step have run", phase, d->midStep);
+    return fail(rc, "a step is half done (%d of its 5 phases ran): finish it with frehd_dev_phase", d->midStep);
+}
+
 // ---------------------------------------------------------------------------
 extern "C" {
 
@@ -1024,6 +1041,7 @@ int frehd_dev_step(frehd_dev_t *d, int32_t nsteps)
 {
     if (!d) return fail(FREHD_EINVAL, "handle is NULL");
     if (nsteps < 0) return fail(FREHD_EINVAL, "nsteps < 0");
+    if (int rc = state_check(d, CALL_STEP)) return rc;
     HIPCHK(hipSetDevice(d->device));
     Ctl c;
     COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
@@ -1050,6 +1068,7 @@ int frehd_dev_step(frehd_dev_t *d, int32_t nsteps)
 int frehd_dev_phase(frehd_dev_t *d, int32_t phase)
 {
     if (!d) return fail(FREHD_EINVAL, "handle is NULL");
+    if (int rc = state_check(d, CALL_PHASE, phase)) return rc;
     HIPCHK(hipSetDevice(d->device));
     int rc = exec_phase(d->self, phase);
     if (rc) return rc;
@@ -1070,7 +1089,8 @@ int frehd_dev_download(frehd_dev_t *d, frehd_fields *o)
     // has run
     Ctl c;
     COPYW(d->stream, &c, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
-    const bool newDepth = !c.halted && d->midStep >= 2, newS = !c.halted && d->midStep >= 3;
+    const bool newDepth = !c.halted && step_state_new_depth(d->midStep);
+    const bool newS = !c.halted && step_state_new_scalar(d->midStep);
     const double *depthCur = (c.parity ^ newDepth) ? a.depthB : a.depthA;
     const double *Scur = (c.sParity ^ newS) ? a.S2 : a.S;
     struct { double *dst; const double *src; size_t n; } tab[] = {
@@ -1121,8 +1141,7 @@ int frehd_dev_stats(frehd_dev_t *d, frehd_stats *out)
 int frehd_dev_upload(frehd_dev_t *d, const frehd_fields *in)
 {
     if (!d || !in) return fail(FREHD_EINVAL, "NULL argument");
-    if (d->batch) return fail(FREHD_ESTATE, "the handle is in a batch");
-    if (d->midStep) return fail(FREHD_ESTATE, "a step is half done (phase %d ran last)", d->midStep);
+    if (int rc = state_check(d, CALL_UPLOAD)) return rc;
     HIPCHK(hipSetDevice(d->device));
     HIPCHK(hipStreamSynchronize(d->stream));
     Ctl c;
@@ -1144,6 +1163,7 @@ int frehd_dev_set_step_index(frehd_dev_t *d, int32_t tt)
 {
     if (!d) return fail(FREHD_EINVAL, "handle is NULL");
     if (tt < 0 || tt > d->cfg.Nt) return fail(FREHD_EINVAL, "step index %d outside [0, Nt]", tt);
+    if (int rc = state_check(d, CALL_SET_INDEX)) return rc;
     HIPCHK(hipSetDevice(d->device));
     HIPCHK(hipStreamSynchronize(d->stream));
     COPYW(d->stream, &d->args.ctl->tt, &tt, sizeof(int), hipMemcpyHostToDevice);
@@ -1251,6 +1271,7 @@ int frehd_dev_set_pcg_tiling(frehd_dev_t *d, int32_t rows_per_tile, int32_t xcd_
                     cols_per_lane);
     if ((size_t)dir_tiling(d->g.nx, d->g.ny, rows_per_tile, xcd_remap).nb > max_dir_blocks(d->g.nx, d->g.ny))
         return fail(FREHD_EINVAL, "too many tiles");
+    if (int rc = state_check(d, CALL_TILING)) return rc;
     HIPCHK(hipSetDevice(d->device));
     HIPCHK(hipStreamSynchronize(d->stream));
     set_pcg_args(d, rows_per_tile, xcd_remap ? 1 : 0, cols_per_lane);
@@ -1261,7 +1282,7 @@ int frehd_dev_autotune(frehd_dev_t *d, int32_t iters, int32_t *rows_per_tile, in
                        int32_t *cols_per_lane, double *ms_per_iteration)
 {
     if (!d) return fail(FREHD_EINVAL, "handle is NULL");
-    if (d->batch) return fail(FREHD_ESTATE, "the handle is in a batch: tune the batch");
+    if (int rc = state_check(d, CALL_AUTOTUNE)) return rc;
     return autotune_group(d->self, iters, rows_per_tile, xcd_remap, cols_per_lane, ms_per_iteration);
 }
 
@@ -1295,6 +1316,7 @@ int frehd_batch_add(frehd_batch_t *b, frehd_dev_t *d)
 {
     if (!b || !d) return fail(FREHD_EINVAL, "NULL argument");
     if (d->batch) return fail(FREHD_ESTATE, "the handle is already in a batch");
+    if (d->midStep) return fail(FREHD_ESTATE, "a step of the handle is half done (%d of its 5 phases ran)", d->midStep);
     if (d->device != b->e.device) return fail(FREHD_EINVAL, "the handle lives on another device");
     if ((int)b->e.members.size() >= b->e.capacity) return fail(FREHD_EBUDGET, "batch full (%d)", b->e.capacity);
     HIPCHK(hipSetDevice(d->device));

@@ -436,7 +436,7 @@ int exec_phase(Exec &e, int phase)
     int rc = exec_phase_launch(e, phase);
     if (rc) return rc;
     // where the members are inside the step (downloads between phases)
-    for (auto *m : e.members) m->midStep = phase == 4 ? 0 : phase;
+    for (auto *m : e.members) m->midStep = step_state_after(phase);
     return 0;
 }
 

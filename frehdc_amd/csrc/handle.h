@@ -19,6 +19,7 @@
 
 #include "frehd_gpu.h"
 #include "step_kernels.h"
+#include "step_state.h"
 
 namespace frehd {
 
@@ -132,7 +133,7 @@ struct frehd_dev_impl {
     PcgArgs<float> pcg32;               // mixed precision: inner FP32 solve
     RefineArgs ref;                     // mixed precision: FP64 refinement
     int tileH, tileRemap, tileLanes;    // solver tiling as requested (tileH 0: automatic)
-    int midStep;                        // last phase run of an unfinished step (0: at a step boundary)
+    int midStep;                        // phases run of the step in progress (step_state.h; 0: at a step boundary)
     int tt;                             // host view of the step counter
     Exec self;                          // the handle as a group of one
     Timing timing;

@@ -115,6 +115,19 @@ class _CpuModel:
             raise KeyError(name)
         return np.ctypeslib.as_array(p, shape=(ln.value,)).copy()
 
+    def poke(self, name, values):
+        """Overwrite the model's own array `name` (its first len(values)
+        entries) with `values`: the models keep their state in the arrays
+        the field accessor points to."""
+        ln = C.c_int64()
+        p = getattr(self.lib, self.PREFIX + "field")(self.h, name.encode(), C.byref(ln))
+        if not p:
+            raise KeyError(name)
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if v.size > ln.value:
+            raise ValueError(f"{name}: {v.size} values for {ln.value} slots")
+        np.ctypeslib.as_array(p, shape=(ln.value,))[: v.size] = v
+
     @property
     def tt(self):
         return getattr(self.lib, self.PREFIX + "tt")(self.h)
