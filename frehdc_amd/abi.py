@@ -157,6 +157,7 @@ def load_library(path: str | None = None):
                                          C.POINTER(FrehdSubgrid), C.POINTER(FrehdFields),
                                          C.POINTER(V)], C.c_int),
         "frehd_dev_step": ([V, C.c_int32], C.c_int),
+        "frehd_dev_step_noalloc": ([V, C.c_int32], C.c_int),
         "frehd_dev_download": ([V, C.POINTER(FrehdFields)], C.c_int),
         "frehd_dev_stats": ([V, C.POINTER(FrehdStats)], C.c_int),
         "frehd_dev_step_index": ([V, c_int32_p], C.c_int),

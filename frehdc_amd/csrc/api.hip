@@ -279,6 +279,34 @@ static bool min_tables(const frehd_config &k)
     return k.useSubgrid == 1 && (k.useminA == 1 || k.useminV == 2);
 }
 
+}  // namespace
+
+namespace frehd {
+PartialSizes partial_sizes(const frehd_config &k)
+{
+    const size_t N = (size_t)k.nx * k.ny, Ng = (size_t)(k.nx + 2) * (k.ny + 2);
+    const size_t nb2d = (size_t)((k.nx + 63) / 64) * ((k.ny + 3) / 4);
+    const size_t nb1d = (Ng + 255) / 256;
+    const size_t G = pcg_blocks(N);
+    const size_t maxb = std::max(std::max(nb2d, nb1d), G);
+    const size_t maxDir = max_dir_blocks(k.nx, k.ny);
+    PartialSizes z;
+    z.maxBlocks = maxb;
+    z.nb1 = nb1d;
+    z.nb2 = nb2d;
+    z.partA = 2 * std::max(G, maxDir);
+    z.partB = maxDir;
+    z.vpart = 4 * maxb;
+    z.cflpart = 2 * maxb;
+    const bool mixed = k.precision == FREHD_PREC_MIXED;
+    z.partF = mixed ? 4 * maxDir : 0;
+    z.partC = mixed ? 2 * maxDir : 0;
+    return z;
+}
+}  // namespace frehd
+
+namespace {
+
 // lay out every device array; with base == nullptr only the size is computed
 size_t layout(frehd_dev *d, char *base)
 {
@@ -303,16 +331,12 @@ size_t layout(frehd_dev *d, char *base)
     a.manning = nullptr;
     a.rmv = cv.take<uint8_t>(N);
     a.flags = cv.take<uint8_t>(N);
-    const size_t nb2d = (size_t)((k.nx + 63) / 64) * ((k.ny + 3) / 4);
-    const size_t nb1d = (Ng + 255) / 256;
-    const size_t G = pcg_blocks(N);
-    const size_t maxb = std::max(std::max(nb2d, nb1d), G);
-    a.maxBlocks = (int)maxb;
-    a.nb1 = (int)nb1d;
-    a.nb2 = (int)nb2d;
-    const size_t maxDir = max_dir_blocks(k.nx, k.ny);
-    a.partA = cv.take<double>(2 * std::max(G, maxDir));
-    a.partB = cv.take<double>(maxDir);
+    const PartialSizes ps = partial_sizes(k);
+    a.maxBlocks = (int)ps.maxBlocks;
+    a.nb1 = (int)ps.nb1;
+    a.nb2 = (int)ps.nb2;
+    a.partA = cv.take<double>(ps.partA);
+    a.partB = cv.take<double>(ps.partB);
     a.cX32 = a.cY32 = nullptr;
     a.xB = nullptr;
     if (k.precision == FREHD_PREC_MIXED) {
@@ -321,10 +345,10 @@ size_t layout(frehd_dev *d, char *base)
         float **fArr[] = {&a.cX32, &a.cY32, &d->pcg32.r, &d->pcg32.q, &d->pcg32.pBuf[0],
                           &d->pcg32.pBuf[1], &d->pcg32.x};
         for (auto **p : fArr) *p = cv.take<float>(N);
-        d->pcg32.partF[0] = cv.take<double>(4 * maxDir);
-        d->pcg32.partF[1] = cv.take<double>(4 * maxDir);
+        d->pcg32.partF[0] = cv.take<double>(ps.partF);
+        d->pcg32.partF[1] = cv.take<double>(ps.partF);
         a.xB = cv.take<double>(N);
-        d->ref.partC = cv.take<double>(2 * maxDir);
+        d->ref.partC = cv.take<double>(ps.partC);
     }
     // the handle as a group of one: device copies of its argument blocks
     // and the flag gather buffer
@@ -333,8 +357,14 @@ size_t layout(frehd_dev *d, char *base)
     d->self.dPcg = cv.take<PcgArgs<double>>(1);
     d->self.dPcg32 = cv.take<PcgArgs<float>>(1);
     d->self.dRef = cv.take<RefineArgs>(1);
-    a.vpart = cv.take<double>(4 * maxb);
-    a.cflpart = cv.take<double>(2 * maxb);
+    // the argument blocks of frehd_dev_step_noalloc (partial sums in the
+    // library's static scratch array instead of these)
+    d->naArgs = cv.take<StepArgs>(1);
+    d->naPcg = cv.take<PcgArgs<double>>(1);
+    d->naPcg32 = cv.take<PcgArgs<float>>(1);
+    d->naRef = cv.take<RefineArgs>(1);
+    a.vpart = cv.take<double>(ps.vpart);
+    a.cflpart = cv.take<double>(ps.cflpart);
     a.tideP = cv.take<double>(Nt1);
     a.tideM = cv.take<double>(Nt1);
     a.inflow = cv.take<double>(Nt1);

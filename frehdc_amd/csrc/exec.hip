@@ -96,10 +96,13 @@ int launch(Exec &e, KernelId id, F &&fn)
 inline dim3 grid1(int nb, int n) { return dim3((unsigned)std::max(nb, 1), 1, (unsigned)n); }
 
 // keep the solver counts of the last 8 solves
-void remember(Exec &e, std::vector<int> counts)
+void remember(Exec &e, const SolveCounts &counts)
 {
-    if (e.recent.size() >= 8) e.recent.erase(e.recent.begin());
-    e.recent.push_back(std::move(counts));
+    if (e.nRecent == 8) {
+        for (int q = 1; q < 8; q++) e.recent[q - 1] = e.recent[q];
+        e.nRecent = 7;
+    }
+    e.recent[e.nRecent++] = counts;
 }
 
 int enqueue_gather(Exec &e)
@@ -240,9 +243,12 @@ int solve_fp64_finish(Exec &e, int it)
     int last = 1;
     for (int m = 0; m < e.count; m++)
         if (!flags(e, m)[F_HALTED]) last = std::max(last, (int)flags(e, m)[F_ITER]);
-    remember(e, {last});
+    SolveCounts c;
+    c.n = 1;
+    c.v[0] = last;
+    remember(e, c);
     e.lastIters = 1;
-    for (auto &r : e.recent) e.lastIters = std::max(e.lastIters, r[0]);
+    for (int q = 0; q < e.nRecent; q++) e.lastIters = std::max(e.lastIters, e.recent[q].v[0]);
     return 0;
 }
 
@@ -296,8 +302,8 @@ int solve_mixed_pattern(Exec &e, int *sweeps)
 {
     int rc, s = 0;
     if ((rc = enqueue_refine(e, 0))) return rc;
-    for (size_t k = 0; k < e.mixPlan.size(); k++) {
-        if ((rc = enqueue_inner_solve(e, s, e.mixPlan[k]))) return rc;
+    for (int k = 0; k < e.mixPlan.n; k++) {
+        if ((rc = enqueue_inner_solve(e, s, e.mixPlan.v[k]))) return rc;
         if ((rc = enqueue_refine(e, ++s))) return rc;
     }
     *sweeps = s;
@@ -336,19 +342,20 @@ int solve_mixed_finish(Exec &e, int s)
     int sweeps = 0;
     for (int m = 0; m < e.count; m++)
         if (!flags(e, m)[F_HALTED]) sweeps = std::max(sweeps, (int)flags(e, m)[F_REFDONE]);
-    std::vector<int> plan;
+    SolveCounts plan;
     for (int k = 0; k < sweeps && k < 8; k++) {
         int n = 1;
         for (int m = 0; m < e.count; m++)
             if (!flags(e, m)[F_HALTED] && flags(e, m)[F_REFDONE] > k) n = std::max(n, (int)flags(e, m)[F_INNER0 + k]);
-        if (k < (int)e.mixPlan.size() && n >= e.mixPlan[k]) n = e.mixPlan[k] + 2;
-        plan.push_back(n);
+        if (k < e.mixPlan.n && n >= e.mixPlan.v[k]) n = e.mixPlan.v[k] + 2;
+        plan.v[plan.n++] = n;
     }
     remember(e, plan);
-    e.mixPlan.clear();
-    for (auto &r : e.recent) {
-        if (r.size() > e.mixPlan.size()) e.mixPlan.resize(r.size(), 0);
-        for (size_t k = 0; k < r.size(); k++) e.mixPlan[k] = std::max(e.mixPlan[k], r[k]);
+    e.mixPlan = SolveCounts();
+    for (int q = 0; q < e.nRecent; q++) {
+        const SolveCounts &r = e.recent[q];
+        e.mixPlan.n = std::max(e.mixPlan.n, r.n);
+        for (int k = 0; k < r.n; k++) e.mixPlan.v[k] = std::max(e.mixPlan.v[k], r.v[k]);
     }
     return 0;
 }
@@ -391,8 +398,8 @@ int exec_time_solver(Exec &e, int iters, double *ms)
         // per solve as the last step ran it: its inner iterations and one
         // refinement sweep more than inner solves
         int inner = 0;
-        for (int n : e.mixPlan) inner += n;
-        const int sweeps = (int)e.mixPlan.size() + 1;
+        for (int k = 0; k < e.mixPlan.n; k++) inner += e.mixPlan.v[k];
+        const int sweeps = e.mixPlan.n + 1;
         *ms = (std::max(inner, 1) * avg(K_CG32) + sweeps * avg(K_REFINE)) / std::max(inner, 1);
     } else {
         *ms = avg(K_PQ) + avg(K_UPD);
@@ -559,7 +566,7 @@ static int exec_step_graphs(Exec &e)
     std::vector<int> key;
     if (e.mixed) {
         key.push_back(1);
-        key.insert(key.end(), e.mixPlan.begin(), e.mixPlan.end());
+        key.insert(key.end(), e.mixPlan.v, e.mixPlan.v + e.mixPlan.n);
     } else {
         key = {0, std::max(1, e.lastIters)};
     }
@@ -595,7 +602,7 @@ static int exec_step_graphs(Exec &e)
         for (auto *m : e.members) m->tt -= 1;
         e.heads.push_back({key, step});
     } else {
-        k = e.mixed ? (int)e.mixPlan.size() : std::max(1, e.lastIters);
+        k = e.mixed ? e.mixPlan.n : std::max(1, e.lastIters);
     }
     clear_flags(e);
     HIPCHK(hipGraphLaunch(step, e.stream));

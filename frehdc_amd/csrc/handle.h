@@ -79,6 +79,14 @@ enum { F_DONE, F_HALTED, F_ITER, F_REFDONE, F_INDONE, F_INITER, F_TT, F_SEQ, F_I
 
 struct frehd_dev_impl;
 
+// solver counts of one solve: FP64 iterations (n = 1, v[0]), or inner
+// iterations per refinement sweep (n sweeps, at most 8); fixed storage, so
+// the step's bookkeeping allocates nothing
+struct SolveCounts {
+    int n = 0;
+    int v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
 struct Exec {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -93,12 +101,13 @@ struct Exec {
     Extents ext{};
     int sub = 0, mixed = 0, nScalarMax = 0, maxIter = 0, maxRefine = 0;
     int lastIters = 16;
-    std::vector<int> mixPlan;           // inner iterations per sweep of the next mixed solve's pattern
+    SolveCounts mixPlan;                // inner iterations per sweep of the next mixed solve's pattern
     // the last steps' solver counts (FP64 iterations, or inner iterations
     // per sweep): the next pattern covers their maximum, since a launch that
     // finds its solve converged costs a few microseconds and a pattern that
     // falls short costs extra host synchronisations
-    std::vector<std::vector<int>> recent;
+    SolveCounts recent[8];
+    int nRecent = 0;
     Timing *tim = nullptr;
     std::vector<frehd_dev_impl *> members;
     // captured steps (hipGraph): a whole step per solver launch pattern
@@ -120,6 +129,15 @@ struct Exec {
 // change, and before it is destroyed)
 void exec_release_graphs(Exec &e);
 
+// element counts of a handle's per-block partial-sum buffers (solver:
+// partA, partB, and for mixed precision partF twice and partC; volume
+// losses vpart, CFL maxima cflpart) and the block counts they are sized by
+struct PartialSizes {
+    size_t partA, partB, vpart, cflpart, partF, partC;
+    size_t maxBlocks, nb1, nb2;
+};
+PartialSizes partial_sizes(const frehd_config &k);
+
 struct frehd_dev_impl {
     frehd_config cfg;
     Grid g;
@@ -136,6 +154,17 @@ struct frehd_dev_impl {
     int midStep;                        // phases run of the step in progress (step_state.h; 0: at a step boundary)
     int tt;                             // host view of the step counter
     Exec self;                          // the handle as a group of one
+    // frehd_dev_step_noalloc: device argument blocks (carved at create)
+    // whose partial-sum pointers lead into the static scratch array, and
+    // their host images, which stay alive while the copies are in flight
+    StepArgs *naArgs;
+    PcgArgs<double> *naPcg;
+    PcgArgs<float> *naPcg32;
+    RefineArgs *naRef;
+    StepArgs naArgsH;
+    PcgArgs<double> naPcgH;
+    PcgArgs<float> naPcg32H;
+    RefineArgs naRefH;
     Timing timing;
     void *batch;                        // the batch holding the handle, or NULL
 };

@@ -293,6 +293,19 @@ int frehd_dev_step_index(frehd_dev_t *dev, int32_t *tt);
  * a handle in a batch). */
 int frehd_dev_upload(frehd_dev_t *dev, const frehd_fields *in);
 
+/* Whole steps as frehd_dev_step, allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops
+ * and harnesses that must not call hipMalloc).  The kernels' per-block
+ * partial sums go to a static device array of the library instead of the
+ * handle's buffers -- 16 MiB, enough for grids up to about 4096 x 4096 cells
+ * (FREHD_EINVAL beyond) -- and the kernels are launched one at a time (no
+ * graph is instantiated), untimed.  Results are bit for bit those of
+ * frehd_dev_step; the two may be mixed on one handle.  The array is shared
+ * by every call on a device and keeps the last call's contents (each call
+ * clears what it uses first): calls that overlap on one device race.
+ * FREHD_ESTATE mid-step or for a handle in a batch. */
+int frehd_dev_step_noalloc(frehd_dev_t *dev, int32_t nsteps);
+
 /* Set the step counter, e.g. to resume from a state saved at step tt; the
  * next step taken is tt + 1. */
 int frehd_dev_set_step_index(frehd_dev_t *dev, int32_t tt);
