@@ -1,4 +1,6 @@
-"""Build the device library libfrehd_gpu.so in-tree with hipcc for gfx950."""
+"""Build the device library libfrehd_gpu.so in-tree with hipcc for gfx950, and
+the reference-signature entry points beside it (seam/libfrehd_seam.so) where
+the FrehdC headers are present."""
 from __future__ import annotations
 
 import os
@@ -35,7 +37,29 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libfrehd_gpu.so")
     os.replace(tmp, OUT)
+    build_seam(verbose)
     return OUT
+
+
+SEAM_DIR = os.path.join(HERE, "seam")
+SEAM_LIB = os.path.join(SEAM_DIR, "libfrehd_seam.so")
+REF_HEADERS = os.environ.get("FREHD_REFERENCE", "/root/reference") + "/FrehdC_v42"
+
+
+def build_seam(verbose: bool = False) -> str | None:
+    """The reference-signature entry points (seam/frehd_seam.c), compiled
+    against the FrehdC headers where a checkout is present; elsewhere the
+    library built next to one travels prebuilt."""
+    if not os.path.isfile(os.path.join(REF_HEADERS, "nssolve.h")):
+        return None
+    cmd = ["make", "-C", SEAM_DIR, "-B", "-s", f"REFROOT={os.path.dirname(REF_HEADERS)}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building libfrehd_seam.so failed")
+    return SEAM_LIB
 
 
 if __name__ == "__main__":

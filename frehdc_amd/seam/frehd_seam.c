@@ -1,0 +1,227 @@
+/*
+ * frehd_seam.c -- FrehdC v4.2's time-step entry points with the reference's
+ * own signatures (include/frehd_seam.h), forwarding to the device library
+ * (include/frehd_gpu.h).
+ *
+ * Built against the reference's headers by frehdc_amd/seam/Makefile (from
+ * where they lie in a FrehdC checkout; nothing of them is copied here), so
+ * the compiler checks oneCompleteStep against nssolve.h:10-11.  In a FrehdC
+ * build this file replaces oneCompleteStep in nssolve.c and adds
+ * SolveAllOnDevice beside SolveAll; DataOutput / SubOutput (fileio.c) come
+ * from the program it is linked into.
+ *
+ * Error convention of the reference: a failure prints a message and exits
+ * (as monitorCFL does at CFL > 20, nsfunctions.c:1226-1231).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "frehd_seam.h"
+#include "fileio.h"
+#include "frehd_gpu.h"
+
+/* a device handle and the forcing series it was made from (Nt + 1 samples) */
+typedef struct seam {
+    frehd_dev_t *dev;
+    double *series[7];       /* tideP tideM inflow tidalPS tidalMS windspd winddir */
+} seam;
+
+static seam g_step;          /* oneCompleteStep's handle, kept between calls */
+
+static void die(const char *what)
+{
+    printf("frehd seam: %s failed: %s\n", what, frehd_last_error());
+    exit(EXIT_FAILURE);
+}
+
+/* The Config fields the device step reads (configuration.h:4-111), with the
+ * reference's names; one strip, one scalar, FP64, Jacobi-preconditioned CG to
+ * the reference's tolerance (SSORPrecond's triangular sweeps, nsfunctions.c:1016,
+ * are sequential). */
+static void seam_config(const Config *s, const Bath *b, const Sub *sb, frehd_config *c)
+{
+    memset(c, 0, sizeof *c);
+    c->nx = s->nx; c->ny = s->ny; c->Nt = s->Nt; c->maxIter = s->maxIter;
+    c->dx = s->dx; c->dy = s->dy; c->dt = s->dt; c->g = s->g;
+    c->NUx = s->NUx; c->NUy = s->NUy; c->CDx = s->CDx; c->CDy = s->CDy;
+    c->manningN = s->manningN; c->CDnotN = s->CDnotN; c->useThinLayer = s->useThinLayer;
+    c->hD = s->hD; c->CDmax = s->CDmax; c->minDepth = s->minDepth; c->wtfh = s->wtfh;
+    c->CFLl = s->CFLl; c->CFLh = s->CFLh; c->eps = s->eps;
+    c->precond = FREHD_PRECOND_JACOBI;
+    c->precision = FREHD_PREC_FP64;
+    c->bcType = s->bcType; c->tideLocLengthP = s->tideLocLengthP; c->tideLocLengthM = s->tideLocLengthM;
+    c->inflowLocLength = s->inflowLocLength;
+    c->useScalar = s->useScalar; c->scalarAdv = s->scalarAdv;
+    c->useConstTidePS = s->useConstTidePS; c->useConstTideMS = s->useConstTideMS;
+    c->nScalar = 1;
+    c->tidePS = s->tidePS; c->tideMS = s->tideMS; c->inflowS = s->inflowS; c->Kx = s->Kx; c->Ky = s->Ky;
+    c->offset = b->offset[0];
+    c->epsInner = 1e-5; c->maxRefine = 8;           /* (mixed precision only) */
+    c->useSubgrid = s->useSubgrid != 0;
+    if (c->useSubgrid) {
+        c->Ne = sb->Ne; c->surfmin = s->surfmin; c->dsurf = s->dsurf; c->subR = s->subR;
+    }
+    c->useWind = s->useWind; c->northAngle = s->northAngle; c->rhoa = s->rhoa; c->Cw = s->Cw; c->CwT = s->CwT;
+    c->useSubDrag = s->useSubDrag; c->useminA = s->useminA; c->useminV = s->useminV; c->beta = s->beta;
+}
+
+/* The reference allocates its boundary series with Nt samples
+ * (initialize.c:330-342) while step Nt reads sample Nt (nsfunctions.c:386),
+ * one past them; the device takes Nt + 1 samples, so sample Nt repeats the
+ * last one the reference holds. */
+static double *series_copy(const double *src, int Nt)
+{
+    double *d = (double *)calloc((size_t)Nt + 1, sizeof(double));
+    if (!d) return NULL;
+    if (src && Nt > 0) {
+        memcpy(d, src, sizeof(double) * (size_t)Nt);
+        d[Nt] = src[Nt - 1];
+    }
+    return d;
+}
+
+/* the Data / Sub / Bath arrays in the device library's field layout */
+static void seam_fields(Data *d, Sub *sb, Bath *b, frehd_fields *st)
+{
+    memset(st, 0, sizeof *st);
+    st->surf = d->surf; st->surfOld = d->surfOld; st->uuXP = d->uuXP; st->vvYP = d->vvYP;
+    st->uuYP = d->uuYP; st->vvXP = d->vvXP; st->depth = d->depth; st->depthXP = d->depthXP;
+    st->depthYP = d->depthYP; st->EnXP = d->EnXP; st->EnYP = d->EnYP; st->DragXP = d->DragXP;
+    st->DragYP = d->DragYP; st->CDXP = d->CDXP; st->CDYP = d->CDYP; st->Fuu = d->Fuu; st->Fvv = d->Fvv;
+    st->cellV = d->cellV; st->S = d->S; st->Sm = d->Sm; st->CFLx = d->CFLx; st->CFLy = d->CFLy;
+    st->wtfXP = d->wtfXP; st->wtfYP = d->wtfYP;
+    st->bottomZ = b->bottomZ; st->bottomXP = b->bottomXP; st->bottomYP = b->bottomYP;
+    if (sb) {
+        st->V = sb->V; st->Z = sb->Z; st->Vx = sb->Vx; st->Vy = sb->Vy;
+        st->Zx = sb->Zx; st->Zy = sb->Zy; st->Nx = sb->Nx; st->Oy = sb->Oy; st->ind = sb->ind;
+    }
+}
+
+/* a handle from the caller's state (the reference's Init output) and its
+ * subgrid tables (initAllSubVar, subgrid.c:2112-2284) */
+static int seam_open(seam *sm, Data *d, Sub *sb, BC *bc, Bath *b, Config *s, frehd_fields *st)
+{
+    frehd_config c;
+    seam_config(s, b, sb, &c);
+    const double *src[7] = {bc->tideP, bc->tideM, bc->inflow, bc->tidalPS, bc->tidalMS,
+                            s->useWind == 1 ? bc->windspd : NULL, s->useWind == 1 ? bc->winddir : NULL};
+    for (int q = 0; q < 7; q++) {
+        sm->series[q] = series_copy(src[q], s->Nt);
+        if (!sm->series[q]) return FREHD_ENOMEM;
+    }
+    frehd_forcing f;
+    memset(&f, 0, sizeof f);
+    f.tideLocP = s->tideLocP; f.tideLocM = s->tideLocM; f.inflowLoc = s->inflowLoc;
+    f.tideP = sm->series[0]; f.tideM = sm->series[1]; f.inflow = sm->series[2];
+    f.tidalPS = sm->series[3]; f.tidalMS = sm->series[4];
+    f.manningCell = NULL;
+    if (s->useWind == 1) { f.windspd = sm->series[5]; f.winddir = sm->series[6]; }
+    frehd_subgrid sg;
+    memset(&sg, 0, sizeof sg);
+    if (c.useSubgrid) {
+        /* only the tables an option reads: the reference leaves the others unset */
+        const int minV = (s->useminA == 1 && s->useminV == 1) || s->useminV == 2;
+        sg.edgeX = b->allEdgeX; sg.edgeY = b->allEdgeY;
+        sg.V = sb->allV; sg.Z = sb->allZ; sg.Vxp = sb->allVxp; sg.Vxm = sb->allVxm;
+        sg.Vyp = sb->allVyp; sg.Vym = sb->allVym; sg.Np = sb->allNp; sg.Nm = sb->allNm;
+        sg.Op = sb->allOp; sg.Om = sb->allOm;
+        if (minV) {
+            sg.minVxp = sb->allminVxp; sg.minVxm = sb->allminVxm;
+            sg.minVyp = sb->allminVyp; sg.minVym = sb->allminVym;
+        }
+        if (s->useSubDrag == 2) { sg.YhX = sb->allYhX; sg.YhY = sb->allYhY; }
+        if (s->useSubDrag == 3 || s->useSubDrag == 4) sg.Cn = sb->allCn;
+        if (s->useminA == 1) { sg.Nmin = sb->allNmin; sg.Nmax = sb->allNmax; sg.Omin = sb->allOmin; sg.Omax = sb->allOmax; }
+    }
+    return frehd_dev_create_from_state(&c, &f, c.useSubgrid ? &sg : NULL, st, &sm->dev);
+}
+
+static void seam_close(seam *sm)
+{
+    if (sm->dev) frehd_dev_destroy(sm->dev);
+    sm->dev = NULL;
+    for (int q = 0; q < 7; q++) { free(sm->series[q]); sm->series[q] = NULL; }
+}
+
+void frehd_seam_release(void) { seam_close(&g_step); }
+
+/* oneCompleteStep (nssolve.c:110-118): solveSourceTerm, solveFreeSurface,
+ * updateData and scalarTransport of step tt on the device.  The state the
+ * caller holds goes up (its monitorCFL has just written CFLx/CFLy, its
+ * DataOutput may have touched surf), the step runs, the state comes back. */
+void oneCompleteStep(Data **data, Sub **sub, Maps *map, BC *bc, Bath *bath, Config *setting, int tt, QMatrix A,
+                     Vector x, Vector z, int irank, int nrank)
+{
+    (void)map; (void)A; (void)x; (void)z; (void)irank;
+    if (nrank != 1) { printf("frehd seam: one strip only (nrank = %d)\n", nrank); exit(EXIT_FAILURE); }
+    frehd_fields st;
+    seam_fields(*data, setting->useSubgrid ? *sub : NULL, bath, &st);
+    if (!g_step.dev) {
+        if (seam_open(&g_step, *data, setting->useSubgrid ? *sub : NULL, bc, bath, setting, &st)) die("handle");
+    } else if (frehd_dev_upload(g_step.dev, &st)) {
+        die("state upload");
+    }
+    if (frehd_dev_set_step_index(g_step.dev, tt - 1)) die("step index");
+    const int rc = frehd_dev_step(g_step.dev, 1);
+    /* max CFL > 20: the caller's monitorCFL stops the run right after */
+    if (rc && rc != FREHD_ECFL) die("step");
+    if (frehd_dev_download(g_step.dev, &st)) die("state download");
+}
+
+/* SolveAll (nssolve.c:48-106) with the state resident on the device: steps
+ * run back to back and the state comes to the host only for DataOutput /
+ * SubOutput every OutItvl steps (and at the end).  monitorCFL's stop at
+ * CFL > 20 is the device's (the state of that step is written out and the
+ * program exits, as nsfunctions.c:1226-1231 does); its per-cell CFL > 2
+ * warnings become one line per output interval. */
+void SolveAllOnDevice(Data **data, Sub **sub, Maps *map, BC *bc, Bath *bath, Config *setting, int irank, int nrank)
+{
+    (void)map;
+    const int root = 0;
+    if (nrank != 1) { printf("frehd seam: one strip only (nrank = %d)\n", nrank); exit(EXIT_FAILURE); }
+    Sub *sb = setting->useSubgrid ? *sub : NULL;
+    DataOutput(data, bath, setting, 0, root, irank);
+    if (setting->useSubgrid != 0) SubOutput(sub, setting, 0, root, irank);
+    frehd_fields st;
+    seam_fields(*data, sb, bath, &st);
+    seam sm;
+    memset(&sm, 0, sizeof sm);
+    if (seam_open(&sm, *data, sb, bc, bath, setting, &st)) die("handle");
+    if (irank == 0) printf("Ready for time stepping ...\n");
+    const int itv = setting->OutItvl > 0 ? setting->OutItvl : setting->Nt + 1;
+    for (int tt = 1; tt <= setting->Nt;) {
+        int next = ((tt + itv - 1) / itv) * itv;          /* the next output step */
+        if (next > setting->Nt) next = setting->Nt;
+        const int rc = frehd_dev_step(sm.dev, next - tt + 1);
+        if (rc == FREHD_ECFL) {
+            int at = next;
+            frehd_dev_step_index(sm.dev, &at);
+            if (frehd_dev_download(sm.dev, &st)) die("state download");
+            DataOutput(data, bath, setting, at, root, irank);
+            printf("WARNING: Max CFL exceeds 20, program forced to quit!\n\n");
+            seam_close(&sm);
+            exit(EXIT_FAILURE);
+        }
+        if (rc) die("step");
+        frehd_stats stats;
+        if (frehd_dev_stats(sm.dev, &stats)) die("statistics");
+        if (stats.maxCFLx > 2 || stats.maxCFLy > 2)
+            printf("WARNING: Max CFL is %lf in x, %lf in y at step %d, rank %d!\n", stats.maxCFLx, stats.maxCFLy,
+                   next, irank);
+        if (next % itv == 0) {
+            if (frehd_dev_download(sm.dev, &st)) die("state download");
+            DataOutput(data, bath, setting, next, root, irank);
+            if (setting->useSubgrid != 0) SubOutput(sub, setting, next, root, irank);
+            /* DataOutput subtracts and adds back the offset of surf in place
+               (fileio.c:65-66, 120-121); the reference steps on from that */
+            if (frehd_dev_upload(sm.dev, &st)) die("state upload");
+        }
+        if (irank == 0) printf("---------- Step %d executed on the device ----------\n", next);
+        tt = next + 1;
+    }
+    if (frehd_dev_download(sm.dev, &st)) die("state download");
+    seam_close(&sm);
+    if (irank == 0) printf("Time stepping completed...\n");
+}

@@ -287,10 +287,11 @@ int frehd_dev_step_index(frehd_dev_t *dev, int32_t *tt);
  * pointer of *in (the frehd_fields layout of frehd_dev_download) is copied
  * into the buffers the next step starts from, as frehd_dev_create_from_state
  * does at creation.  For callers that change the state on the host between
- * steps -- the oneCompleteStep shim with the reference's signature
- * (nssolve.c:110-118), whose caller runs monitorCFL and DataOutput on the
- * host.  At a step boundary only (FREHD_ESTATE after a partial step, or for
- * a handle in a batch). */
+ * steps -- oneCompleteStep with the reference's signature (frehd_seam.h;
+ * nssolve.c:110-118), whose caller runs monitorCFL and DataOutput on the
+ * host.  The subgrid lookup then starts afresh for every cell.  At a step
+ * boundary only (FREHD_ESTATE after a partial step, or for a handle in a
+ * batch). */
 int frehd_dev_upload(frehd_dev_t *dev, const frehd_fields *in);
 
 /* Whole steps as frehd_dev_step, allocating nothing.  No counterpart in the

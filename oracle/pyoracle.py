@@ -180,6 +180,22 @@ class Reference(_CpuModel):
             return a.reshape(1, -1)
         return a
 
+    def seam_steps(self, step_fn, nsteps):
+        """SolveAll's loop body around `step_fn` (a C function with the
+        oneCompleteStep signature) and the reference's monitorCFL."""
+        f = self.lib.ref_seam_steps
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        f.restype = C.c_int
+        return f(self.h, C.cast(step_fn, C.c_void_p), int(nsteps))
+
+    def seam_solve_all(self, all_fn):
+        """`all_fn` (a C function with SolveAll's signature) called as
+        SolveAll is, on this model's state."""
+        f = self.lib.ref_seam_solve_all
+        f.argtypes = [C.c_void_p, C.c_void_p]
+        f.restype = C.c_int
+        return f(self.h, C.cast(all_fn, C.c_void_p))
+
     def z(self):
         """RHS in cell order (the reference stores it in the solver's
         transposed order, z[k] for cell (k % ny)*nx + k/ny)."""

@@ -282,6 +282,9 @@ void *ref_create(const frehd_config *c, const frehd_forcing *f, const frehd_ic *
             &sb->Omin, &sb->Nmax, &sb->Omax, &sb->minVyp, &sb->minVxp, &sb->minVym, &sb->minVxm,
             &sb->minVy, &sb->minVx};
         for (size_t q = 0; q < sizeof(arr) / sizeof(arr[0]); q++) *arr[q] = padded(h, Ng, pad);
+        /* SubOutput's gather buffers (initAllSubVar allocates them, N2CI each) */
+        double **outs[] = {&sb->allNx, &sb->allOy, &sb->allVx, &sb->allVy, &sb->allVo, &sb->allZo};
+        for (size_t q = 0; q < sizeof(outs) / sizeof(outs[0]); q++) *outs[q] = padded(h, N, pad);
         sb->ind = ialloc(Ng);
         for (int q = 0; q < Ng; q++) sb->ind[q] = round(Ne / 2);
         computeSubArea(&h->sub, h->data, b, h->map, s);
@@ -364,6 +367,43 @@ int ref_step(void *hv)
     if (ref_phase(hv, 0)) return FREHD_EINVAL;
     for (int p = 1; p < 4; p++) ref_phase(hv, p);
     return ref_phase(hv, 4);
+}
+
+/* The reference's callers of the step seam, around given implementations
+ * of its entry points (the device library's frehdc_amd/seam/frehd_seam.c):
+ * SolveAll's loop body (nssolve.c:61-104 without the output) calling
+ * `step_fn` with the oneCompleteStep signature (nssolve.h:10-11) and then
+ * the reference's monitorCFL; and a SolveAll replacement called as SolveAll
+ * is (FREHD.c:105).  Step numbers continue from the handle's. */
+typedef void (*seam_step_fn)(Data **, Sub **, Maps *, BC *, Bath *, Config *, int, QMatrix, Vector, Vector, int,
+                             int);
+typedef void (*seam_all_fn)(Data **, Sub **, Maps *, BC *, Bath *, Config *, int, int);
+
+int ref_seam_steps(void *hv, void *step_fn, int nsteps)
+{
+    ref_handle *h = (ref_handle *)hv;
+    Config *s = h->setting;
+    for (int t = 0; t < nsteps; t++) {
+        if (h->tt + 1 > s->Nt) return FREHD_EINVAL;
+        h->tt++;
+        QMatrix A; Vector z, x;
+        Q_Constr(&A, "A", s->N2ci, False, Rowws, Normal, True);
+        V_Constr(&z, "z", s->N2ci, Normal, True);
+        V_Constr(&x, "x", s->N2ci, Normal, True);
+        ((seam_step_fn)step_fn)(&h->data, &h->sub, h->map, h->bc, h->bath, s, h->tt, A, x, z, 0, 1);
+        Q_Destr(&A); V_Destr(&x); V_Destr(&z);
+        int rc = ref_phase(hv, 4);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int ref_seam_solve_all(void *hv, void *all_fn)
+{
+    ref_handle *h = (ref_handle *)hv;
+    ((seam_all_fn)all_fn)(&h->data, &h->sub, h->map, h->bc, h->bath, h->setting, 0, 1);
+    h->tt = h->setting->Nt;
+    return 0;
 }
 
 int ref_run(void *hv, int nsteps)
