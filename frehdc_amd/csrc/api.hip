@@ -328,6 +328,8 @@ size_t layout(frehd_dev *d, char *base)
     double **nArr[] = {&a.uuYP, &a.vvXP, &a.depthA, &a.depthB, &a.cellV, &a.cX, &a.cY,
         &a.D, &a.b, &a.x, &a.r, &a.q, &a.pA, &a.pB};
     for (auto **p : nArr) *p = cv.take<double>(N);
+    // SSOR: the preconditioned residual z
+    d->pcg.z = k.precond == FREHD_PRECOND_SSOR ? cv.take<double>(N) : nullptr;
     a.manning = nullptr;
     a.rmv = cv.take<uint8_t>(N);
     a.flags = cv.take<uint8_t>(N);
@@ -441,9 +443,11 @@ int check_config(const frehd_config *k)
         return fail(FREHD_EINVAL, "grid too large for 32-bit cell indices");
     if (k->Nt < 1) return fail(FREHD_EINVAL, "Nt must be >= 1");
     if (!(k->dx > 0 && k->dy > 0 && k->dt > 0)) return fail(FREHD_EINVAL, "dx, dy, dt must be > 0");
-    if (k->precond != FREHD_PRECOND_JACOBI)
-        return fail(FREHD_EINVAL, "the device solver supports only the Jacobi preconditioner "
-                                  "(SSOR's triangular sweeps are sequential)");
+    if (k->precond != FREHD_PRECOND_JACOBI && k->precond != FREHD_PRECOND_SSOR)
+        return fail(FREHD_EINVAL, "unknown preconditioner %d", k->precond);
+    if (k->precond == FREHD_PRECOND_SSOR && k->precision != FREHD_PREC_FP64)
+        return fail(FREHD_EINVAL, "SSOR preconditions the FP64 solve only (the mixed path's inner CG is "
+                                  "diagonally scaled)");
     if (k->precision != FREHD_PREC_FP64 && k->precision != FREHD_PREC_MIXED)
         return fail(FREHD_EINVAL, "unknown precision %d", k->precision);
     if (k->useWind != 0 && k->useWind != 1) return fail(FREHD_EINVAL, "useWind must be 0 or 1");
@@ -545,6 +549,19 @@ int upload_series(frehd_dev *d, const double *src, const double *dst)
 }
 
 void set_pcg_args(frehd_dev *d, int H, int remap, int lanes = 1);
+
+// cells of k_ssor's vector that fit the device's LDS of one workgroup (a
+// few hundred bytes stay for the kernel's own reductions)
+int ssor_lds_cells(int device)
+{
+    int bytes = 0;
+    if (hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        bytes = 65536;
+    bytes = std::min(bytes, 160 * 1024) - 512;
+    if (bytes <= 0) return 0;
+    ssor_allow_lds(bytes);
+    return bytes / (int)sizeof(double);
+}
 
 // this handle's launch extents (see StepArgs)
 void fill_extents(frehd_dev *d)
@@ -809,6 +826,8 @@ void set_pcg_args(frehd_dev *d, int H, int remap, int lanes)
     p.nbStart = p.nbRes;
     p.outerDone = nullptr;
     p.epsDyn = nullptr;
+    p.ssor = d->cfg.precond == FREHD_PRECOND_SSOR ? 1 : 0;
+    p.ldsCells = ssor_lds_cells(d->device);
     if (d->cfg.precision != FREHD_PREC_MIXED) return;
     // inner FP32 solve: same tiling, FP32 operator and vectors (r, q, p, x =
     // the correction d were carved by layout()), started by k_refine

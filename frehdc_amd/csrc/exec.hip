@@ -16,7 +16,7 @@ const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble"
     "k_pcg_res", "k_surface", "k_limiter", "k_faces", "k_tidal_velocity",
     "k_finish", "k_scalar", "k_step_end", "k_combine",
     "k_subarea_init", "k_cg1_f32", "k_cg1_f32_setup", "k_cg1_f32_stop", "k_refine", "k_refine_test", "k_gather_flags",
-    "k_pcg_dir_stop"};
+    "k_pcg_dir_stop", "k_ssor"};
 
 void Timing::reset()
 {
@@ -84,14 +84,15 @@ int launch(Exec &e, KernelId id, F &&fn)
     return 0;
 }
 
-#define LAUNCH(id, kern, grid, block, ...)                                                             \
+#define LAUNCH_SHM(id, kern, grid, block, shm, ...)                                                    \
     do {                                                                                              \
         int rc_ = launch(e, id, [&](hipEvent_t a_, hipEvent_t b_) {                                   \
-            if (a_) hipExtLaunchKernelGGL(kern, grid, block, 0, e.stream, a_, b_, 0, __VA_ARGS__);    \
-            else hipLaunchKernelGGL(kern, grid, block, 0, e.stream, __VA_ARGS__);                     \
+            if (a_) hipExtLaunchKernelGGL(kern, grid, block, shm, e.stream, a_, b_, 0, __VA_ARGS__);  \
+            else hipLaunchKernelGGL(kern, grid, block, shm, e.stream, __VA_ARGS__);                   \
         });                                                                                           \
         if (rc_) return rc_;                                                                          \
     } while (0)
+#define LAUNCH(id, kern, grid, block, ...) LAUNCH_SHM(id, kern, grid, block, 0, __VA_ARGS__)
 
 inline dim3 grid1(int nb, int n) { return dim3((unsigned)std::max(nb, 1), 1, (unsigned)n); }
 
@@ -166,6 +167,12 @@ struct Tag {
 int enqueue_pcg_iteration(Exec &e, int it)
 {
     Tag tag(e, 1, 0, it);
+    if (e.ext.ssorN > 0) {
+        // z = M^-1 r of the members preconditioned with SSOR (one workgroup each)
+        const dim3 g(1, 1, (unsigned)e.count), b(1024);
+        if (e.ext.ssorLds) LAUNCH_SHM(K_SSOR, k_ssor<true>, g, b, (size_t)e.ext.ssorN * sizeof(double), e.dPcg, it);
+        else LAUNCH(K_SSOR, k_ssor<false>, g, b, e.dPcg, it);
+    }
     LAUNCH(K_PQ, k_pcg_dir<double>, grid1(e.ext.nbDir, e.count), dim3(256), e.dPcg, it);
     LAUNCH(K_UPD, k_pcg_res<double>, grid1(e.ext.nbRes, e.count), dim3(256), e.dPcg, it);
     return 0;
@@ -196,7 +203,8 @@ void classify_timed(Exec &e)
     };
     for (auto &r : t->pending) {
         if (r.kind == 1) {
-            if (r.id == K_PQ) r.id = r.it < D ? K_PQ : (r.it == D ? K_PQ_STOP : K_IDLE);
+            if (r.id == K_SSOR) r.id = r.it <= D ? K_SSOR : K_IDLE;
+            else if (r.id == K_PQ) r.id = r.it < D ? K_PQ : (r.it == D ? K_PQ_STOP : K_IDLE);
             else r.id = r.it < D ? K_UPD : K_IDLE;
         } else if (r.kind == 2) {
             const int n = inner(r.s);
@@ -635,6 +643,13 @@ int exec_step(Exec &e)
     return exec_step_eager(e);
 }
 
+// lets k_ssor<true> take `bytes` of dynamic LDS (beyond the default cap)
+void ssor_allow_lds(int bytes)
+{
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_ssor<true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
 int exec_init_subgrid(Exec &e)
 {
     const Extents &x = e.ext;
@@ -660,6 +675,8 @@ Extents member_extents(const frehd_dev_impl *d)
     x.nbDir = mixed ? d->pcg32.nbDir : d->pcg.nbDir;
     x.nbRef = mixed ? d->ref.nb : 0;
     x.lanes = mixed ? d->pcg32.lanes : 1;
+    x.ssorN = d->pcg.ssor ? d->g.N : 0;
+    x.ssorLds = !d->pcg.ssor || d->g.N <= d->pcg.ldsCells;
     return x;
 }
 
@@ -683,6 +700,8 @@ static void merge(Exec &e, const frehd_dev_impl *d, bool first)
     x.nbRes = std::max(x.nbRes, m.nbRes);
     x.nbDir = std::max(x.nbDir, m.nbDir);
     x.nbRef = std::max(x.nbRef, m.nbRef);
+    x.ssorN = std::max(x.ssorN, m.ssorN);
+    x.ssorLds = x.ssorLds && m.ssorLds;
     e.maxIter = std::max(e.maxIter, d->cfg.maxIter);
     e.maxRefine = std::max(e.maxRefine, d->cfg.maxRefine);
     e.nScalarMax = std::max(e.nScalarMax, d->args.p.nScalar);

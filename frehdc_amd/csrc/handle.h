@@ -34,7 +34,7 @@ int fail(int code, const char *fmt, ...);
 
 enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_TIDEV,
                 K_FINISH, K_SCALAR, K_END, K_COMBINE, K_SUBINIT,
-                K_CG32, K_CG32_SET, K_CG32_STOP, K_REFINE, K_RTEST, K_GATHER, K_PQ_STOP, K_COUNT,
+                K_CG32, K_CG32_SET, K_CG32_STOP, K_REFINE, K_RTEST, K_GATHER, K_PQ_STOP, K_SSOR, K_COUNT,
                 K_IDLE = -1 };
 extern const char *kKernelNames[K_COUNT];
 
@@ -70,6 +70,8 @@ struct Timing {
 struct Extents {
     int gx2, gy2, nb1, nbN, nbGhost, nbTide, nbRes, nbDir, nbRef;
     int lanes;      // columns per lane of the inner k_cg1 (shared by the group)
+    int ssorN;      // largest N of the members preconditioned with SSOR (0: none)
+    int ssorLds;    // every SSOR member fits k_ssor's LDS vector
 };
 
 // flags gathered from every member's control block after a solver chunk
@@ -184,6 +186,8 @@ int exec_collect_timing(Exec &e);
 int exec_time_solver(Exec &e, int iters, double *ms);
 // upload one handle's argument blocks to its own group arrays
 int upload_member_args(frehd_dev_impl *d);
+// lets the SSOR kernel's LDS variant take `bytes` of dynamic LDS
+void ssor_allow_lds(int bytes);
 
 }  // namespace frehd
 

@@ -41,7 +41,8 @@ __device__ inline T direction(T z, T po, T beta, int first, int betaOne)
     return betaOne ? z + po : z + beta * po;
 }
 
-// raw values of one cell needed to form its new direction
+// raw values of one cell needed to form its new direction: the residual
+// (Jacobi: z = r / |D| is formed from it) or, with SSOR, z itself
 template <typename T>
 struct Raw {
     T r, D, po;
@@ -51,10 +52,16 @@ template <typename T>
 __device__ inline Raw<T> load_raw(const PcgArgs<T> &a, const T *pOld, int c, int first)
 {
     Raw<T> v;
-    v.r = a.r[c];
+    v.r = a.ssor ? a.z[c] : a.r[c];
     v.D = a.D[c];
     v.po = first ? T(0) : pOld[c];
     return v;
+}
+
+template <typename T>
+__device__ inline T precond_z(const PcgArgs<T> &a, const Raw<T> &v)
+{
+    return a.ssor ? v.r : jacobi_z(v.r, v.D);
 }
 
 template <typename T>
@@ -73,7 +80,8 @@ __global__ void __launch_bounds__(256) k_pcg_dir(const PcgArgs<T> *__restrict__ 
 
     double rs[2];
     reduce_partials<2>(a.partA, it == 0 ? a.nbStart : a.nbRes, rs, lds);
-    const double rr = rs[0], rz = rs[1];
+    // r.z: of the Jacobi z from the partials, or k_ssor's sum
+    const double rr = rs[0], rz = a.ssor ? sc->rhoHist[it & 1] : rs[1];
     const double rNorm = sqrt(rr);
     const double bNorm = it == 0 ? rNorm : sc->bNorm;
     const double eps = a.epsDyn ? *a.epsDyn : a.eps;
@@ -83,7 +91,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(const PcgArgs<T> *__restrict__ 
         if (it == 0) { sc->bb = rr; sc->bNorm = bNorm; }
         sc->rr = rr;
         sc->rNorm = rNorm;
-        sc->rhoHist[it & 1] = rz;
+        if (!a.ssor) sc->rhoHist[it & 1] = rz;
         sc->iter = it;
         if (stop) sc->doneAt = it;
     }
@@ -128,7 +136,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(const PcgArgs<T> *__restrict__ 
             if (valid && j0 > 0) {
                 const int c = (j0 - 1) * nx + i;
                 const Raw<T> v = load_raw(a, pOld, c, first);
-                pS = direction(jacobi_z(v.r, v.D), v.po, beta, first, betaOne);
+                pS = direction(precond_z(a, v), v.po, beta, first, betaOne);
                 cyS = a.cY[c];
             }
             // row j0
@@ -143,7 +151,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(const PcgArgs<T> *__restrict__ 
                 if (j0 + 1 < ny) nxt = load_raw(a, pOld, c + nx, first);
             }
             if (halo) hcur = load_raw(a, pOld, j0 * nx + ih, first);
-            T pc = direction(jacobi_z(cur.r, cur.D), cur.po, beta, first, betaOne);
+            T pc = direction(precond_z(a, cur), cur.po, beta, first, betaOne);
 
             for (int j = j0; j < j1; j++) {
                 const int c = j * nx + i;
@@ -158,9 +166,9 @@ __global__ void __launch_bounds__(256) k_pcg_dir(const PcgArgs<T> *__restrict__ 
                 }
                 if (halo && j + 1 < j1) hn = load_raw(a, pOld, c + nx - i + ih, first);
 
-                const T pn = (valid && j + 1 < ny) ? direction(jacobi_z(nxt.r, nxt.D), nxt.po, beta, first, betaOne)
+                const T pn = (valid && j + 1 < ny) ? direction(precond_z(a, nxt), nxt.po, beta, first, betaOne)
                                                    : T(0);
-                const T ph = halo ? direction(jacobi_z(hcur.r, hcur.D), hcur.po, beta, first, betaOne) : T(0);
+                const T ph = halo ? direction(precond_z(a, hcur), hcur.po, beta, first, betaOne) : T(0);
                 T pw = __shfl_up(pc, 1, 64), pe = __shfl_down(pc, 1, 64);
                 T cxw = __shfl_up(cxc, 1, 64);
                 if (lane == 0) {
@@ -200,6 +208,78 @@ __global__ void __launch_bounds__(256) k_pcg_dir(const PcgArgs<T> *__restrict__ 
     }
     block_sum<1>(part, lds);
     if (!stop) store_partials<1>(a.partB, a.nbDir, {part[0]});
+}
+
+// ---------------------------------------------------------------------------
+// SSORPrecond with omega = 1 (precond.c:48-73): z = (D+U)^-1 D (D+L)^-1 r,
+// the rows in the solver's transposed order k = i*ny + j (cell j*nx + i).
+// The forward solve with D+L (MulInv_QV, operats.c:1136-1160) takes row k
+// after rows k-ny (west) and k-1 (south); the backward solve with D+U
+// (operats.c:1081-1105) after k+ny (east) and k+1 (north).  Every cell of
+// an anti-diagonal i + j = d depends only on the diagonal before it, so the
+// sweeps run diagonal by diagonal with a barrier in between, each row
+// evaluated with the reference's own expression and order (west then south;
+// east then north; the diagonal product of Mul_QV in between), so z is the
+// reference's bit for bit.  Identity rows (D < 0: tidal Dirichlet rows,
+// isolated dry subgrid cells) have zero off-diagonals, as the reference's
+// adjusted rows do.  One workgroup per member: nx + ny - 1 dependent stages
+// per sweep.  The sweep's vector lives in LDS when it fits, else in z.
+template <class V>
+__device__ inline void ssor_sweeps(const PcgArgs<double> &a, V t)
+{
+    const int nx = a.nx, ny = a.ny, nd = nx + ny - 1;
+    for (int d = 0; d < nd; d++) {
+        const int ilo = max(0, d - (ny - 1)), ihi = min(d, nx - 1);
+        for (int i = ilo + (int)threadIdx.x; i <= ihi; i += blockDim.x) {
+            const int j = d - i, c = j * nx + i;
+            const double D = a.D[c];
+            const bool id = D < 0;
+            double sum = 0.0;
+            if (i > 0) sum -= (-(id ? 0.0 : a.cX[c - 1])) * t[c - 1];
+            if (j > 0) sum -= (-(id ? 0.0 : a.cY[c - nx])) * t[c - nx];
+            sum += a.r[c];
+            t[c] = sum * (1.0 / fabs(D));
+        }
+        __syncthreads();
+    }
+    for (int d = nd - 1; d >= 0; d--) {
+        const int ilo = max(0, d - (ny - 1)), ihi = min(d, nx - 1);
+        for (int i = ilo + (int)threadIdx.x; i <= ihi; i += blockDim.x) {
+            const int j = d - i, c = j * nx + i;
+            const double D = a.D[c];
+            const bool id = D < 0;
+            double sum = 0.0;
+            if (i < nx - 1) sum -= (-(id ? 0.0 : a.cX[c])) * t[c + 1];
+            if (j < ny - 1) sum -= (-(id ? 0.0 : a.cY[c])) * t[c + nx];
+            double m = 0.0;
+            m += fabs(D) * t[c];
+            sum += m;
+            t[c] = sum * (1.0 / fabs(D));
+        }
+        __syncthreads();
+    }
+}
+
+template <bool LDS>
+__global__ void __launch_bounds__(1024) k_ssor(const PcgArgs<double> *__restrict__ A, int it)
+{
+    const PcgArgs<double> &a = A[blockIdx.z];
+    extern __shared__ double zl[];
+    __shared__ double lds[16];
+    if (!a.ssor || *a.halted) return;
+    PcgScalars *sc = a.sc;
+    const int doneAt = sc->doneAt;
+    if (doneAt >= 0 && doneAt < it) return;
+    if (LDS) ssor_sweeps(a, zl);
+    else ssor_sweeps(a, a.z);
+    double part[1] = {0.0};
+    for (int c = threadIdx.x; c < a.N; c += blockDim.x) {
+        const double z = LDS ? zl[c] : (double)a.z[c];
+        if (LDS) a.z[c] = z;
+        part[0] += a.r[c] * z;
+    }
+    block_sum<1>(part, lds);
+    if (threadIdx.x == 0) sc->rhoHist[it & 1] = part[0];
 }
 
 template <typename T>
@@ -784,6 +864,8 @@ __global__ void __launch_bounds__(256) k_refine_test(const RefineArgs *__restric
 
 template __global__ void k_pcg_dir<double>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_pcg_res<double>(const PcgArgs<double> *__restrict__, int);
+template __global__ void k_ssor<true>(const PcgArgs<double> *__restrict__, int);
+template __global__ void k_ssor<false>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_cg1<true, 1>(const PcgArgs<float> *__restrict__, int);
 template __global__ void k_cg1<false, 1>(const PcgArgs<float> *__restrict__, int);
 template __global__ void k_cg1<true, 2>(const PcgArgs<float> *__restrict__, int);

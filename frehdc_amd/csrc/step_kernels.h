@@ -190,10 +190,18 @@ struct PcgArgs {
     int nbDir, nbRes, nbStart;
     gptr<double> partF[2];         // k_cg1: p.q, r.q, q.q, r.r partials (nbDir each),
                                    // launch it writes partF[it&1]
+    // SSOR preconditioning (FP64 only): k_ssor writes z = M^-1 r here and
+    // r.z into sc->rhoHist[it & 1]; k_pcg_dir then reads z instead of
+    // forming r / |D|
+    gptr<T> z;
+    int ssor, ldsCells;            // ldsCells: k_ssor keeps z in LDS when N <= this
 };
 
 template <typename T> __global__ void k_pcg_dir(const PcgArgs<T> *__restrict__ A, int it);
 template <typename T> __global__ void k_pcg_res(const PcgArgs<T> *__restrict__ A, int it);
+// SSORPrecond with omega = 1 for iteration it, one workgroup per member
+// (LDS: the sweep's vector in dynamic LDS, else in z itself)
+template <bool LDS> __global__ void k_ssor(const PcgArgs<double> *__restrict__ A, int it);
 // the inner FP32 solve of the mixed path, one launch per iteration (First:
 // the set-up launch, it = 0; W = lanes columns per lane): cX, cY hold the
 // scaled coefficients, r and q the two residual buffers, D unused

@@ -37,9 +37,9 @@ static void die(const char *what)
 }
 
 /* The Config fields the device step reads (configuration.h:4-111), with the
- * reference's names; one strip, one scalar, FP64, Jacobi-preconditioned CG to
- * the reference's tolerance (SSORPrecond's triangular sweeps, nsfunctions.c:1016,
- * are sequential). */
+ * reference's names; one strip, one scalar, FP64, the reference's
+ * SSOR-preconditioned CG (nsfunctions.c:1016), or Jacobi with
+ * FREHD_SEAM_PRECOND=jacobi. */
 static void seam_config(const Config *s, const Bath *b, const Sub *sb, frehd_config *c)
 {
     memset(c, 0, sizeof *c);
@@ -49,7 +49,8 @@ static void seam_config(const Config *s, const Bath *b, const Sub *sb, frehd_con
     c->manningN = s->manningN; c->CDnotN = s->CDnotN; c->useThinLayer = s->useThinLayer;
     c->hD = s->hD; c->CDmax = s->CDmax; c->minDepth = s->minDepth; c->wtfh = s->wtfh;
     c->CFLl = s->CFLl; c->CFLh = s->CFLh; c->eps = s->eps;
-    c->precond = FREHD_PRECOND_JACOBI;
+    const char *pc = getenv("FREHD_SEAM_PRECOND");
+    c->precond = (pc && strcmp(pc, "jacobi") == 0) ? FREHD_PRECOND_JACOBI : FREHD_PRECOND_SSOR;
     c->precision = FREHD_PREC_FP64;
     c->bcType = s->bcType; c->tideLocLengthP = s->tideLocLengthP; c->tideLocLengthM = s->tideLocLengthM;
     c->inflowLocLength = s->inflowLocLength;

@@ -45,8 +45,12 @@ extern "C" {
 
 /* ---- solver and precision selectors --------------------------------------- */
 #define FREHD_PRECOND_JACOBI 0   /* laspack/precond.c:26-46 (JacobiPrecond)       */
-#define FREHD_PRECOND_SSOR   1   /* laspack/precond.c:48-73; sequential, so the
-                                    device path rejects it (the CPU oracle has it) */
+#define FREHD_PRECOND_SSOR   1   /* laspack/precond.c:48-73 with omega = 1 (the
+                                    reference's hard-wired choice, nsfunctions.c:1016):
+                                    the triangular sweeps run diagonal by diagonal in
+                                    one workgroup per scenario (nx + ny - 1 dependent
+                                    stages), bit for bit the reference's vector;
+                                    FP64 only                                        */
 
 #define FREHD_PREC_FP64  0       /* every array and operation in FP64             */
 #define FREHD_PREC_MIXED 1       /* FP32 stencils / PCG, FP64 refinement, FP64

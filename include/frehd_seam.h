@@ -23,8 +23,10 @@
  *                     call starts a new one from the host state).
  *
  * One strip (np = 1: irank = 0, nrank = 1), one scalar, the reference's
- * Manning n; the device solves with Jacobi-preconditioned CG to the
- * reference's tolerance (its SSOR sweeps are sequential).
+ * Manning n, and the reference's solve: CG with SSOR (omega = 1,
+ * nsfunctions.c:1016).  The environment variable FREHD_SEAM_PRECOND=jacobi
+ * selects Jacobi-preconditioned CG instead (same tolerance; the device's SSOR
+ * sweeps are nx + ny - 1 dependent stages, slow on large grids).
  */
 #ifndef FREHD_SEAM_H
 #define FREHD_SEAM_H

@@ -55,9 +55,19 @@ def test_error_paths_need_no_device(lib):
     cfg = abi.FrehdConfig(nx=2, ny=2, Nt=1, dx=1.0, dy=1.0, dt=1.0)
     b = C.c_int64()
     assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == abi.FREHD_EINVAL
+    # SSOR preconditions the FP64 solve (one more vector: z); not the mixed path
     cfg = abi.FrehdConfig(nx=64, ny=64, Nt=10, dx=1.0, dy=1.0, dt=1.0, bcType=1, precond=1)
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
+    with_z = b.value
+    cfg = abi.FrehdConfig(nx=64, ny=64, Nt=10, dx=1.0, dy=1.0, dt=1.0, bcType=1, precond=0)
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
+    assert with_z - b.value >= 64 * 64 * 8
+    cfg = abi.FrehdConfig(nx=64, ny=64, Nt=10, dx=1.0, dy=1.0, dt=1.0, bcType=1, precond=1, precision=1,
+                          epsInner=1e-5, maxRefine=4)
     assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == abi.FREHD_EINVAL
-    assert b"Jacobi" in lib.frehd_last_error()
+    assert b"SSOR" in lib.frehd_last_error()
+    cfg = abi.FrehdConfig(nx=64, ny=64, Nt=10, dx=1.0, dy=1.0, dt=1.0, bcType=1, precond=2)
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == abi.FREHD_EINVAL
     # subgrid drag: model 1 reads curvature tables the reference never fills;
     # the models need level tables, not the on-the-fly evaluation
     sub = dict(nx=64, ny=64, Nt=10, dx=1.0, dy=1.0, dt=1.0, bcType=1, useSubgrid=1, Ne=10, dsurf=0.01,

@@ -11,7 +11,8 @@ code, on the reference's own Init state (oracle/_ref):
   resident run too, after DataOutput's in-place offset round trip of the
   surface at step 0 (fileio.c:65-66, 120-121) and with the last forcing
   sample repeated (the reference's series hold Nt samples, step Nt reads one
-  more)."""
+  more).
+Both solve as the reference does, with SSOR-preconditioned CG."""
 import ctypes as C
 import dataclasses
 
@@ -90,7 +91,7 @@ def test_one_complete_step_seam(name):
         assert r.seam_steps(lib.oneCompleteStep, 60) == 0
     finally:
         lib.frehd_seam_release()
-    dev = DeviceScenario(sc, from_state=init)
+    dev = DeviceScenario(sc, from_state=init, precond=abi.PRECOND_SSOR)
     dev.step(60)
     assert_same(r, dev, sub, name)
     dev.close()
@@ -116,7 +117,7 @@ def test_solve_all_on_device(name):
         a = getattr(sc, k).copy()
         a[nt] = a[nt - 1]
         setattr(last, k, a)
-    dev = DeviceScenario(last, from_state=init, Nt=nt)
+    dev = DeviceScenario(last, from_state=init, Nt=nt, precond=abi.PRECOND_SSOR)
     dev.step(nt)
     assert_same(r, dev, sub, name)
     dev.close()
