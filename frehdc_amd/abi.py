@@ -97,6 +97,20 @@ class FrehdStats(C.Structure):
     ]
 
 
+MAX_SCALARS = 16
+
+
+class FrehdBudget(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64), ("nScalar", C.c_int32), ("on", C.c_int32), ("volume", C.c_double),
+        ("volumeResidual", C.c_double), ("volumeResidualMax", C.c_double), ("volumeBoundary", C.c_double),
+        ("volumeInflow", C.c_double), ("volumeClip", C.c_double), ("volumeMoved", C.c_double),
+        ("mass", C.c_double * MAX_SCALARS), ("massResidual", C.c_double * MAX_SCALARS),
+        ("massResidualMax", C.c_double * MAX_SCALARS), ("massBoundary", C.c_double * MAX_SCALARS),
+        ("massInflow", C.c_double * MAX_SCALARS), ("massMoved", C.c_double * MAX_SCALARS),
+    ]
+
+
 FREHD_OK = 0
 FREHD_EINVAL = -1
 FREHD_EHIP = -2
@@ -160,6 +174,8 @@ def load_library(path: str | None = None):
         "frehd_dev_step_noalloc": ([V, C.c_int32], C.c_int),
         "frehd_dev_download": ([V, C.POINTER(FrehdFields)], C.c_int),
         "frehd_dev_stats": ([V, C.POINTER(FrehdStats)], C.c_int),
+        "frehd_dev_set_budget": ([V, C.c_int32], C.c_int),
+        "frehd_dev_budget": ([V, C.POINTER(FrehdBudget)], C.c_int),
         "frehd_dev_step_index": ([V, c_int32_p], C.c_int),
         "frehd_dev_set_step_index": ([V, C.c_int32], C.c_int),
         "frehd_dev_phase": ([V, C.c_int32], C.c_int),

@@ -301,6 +301,7 @@ PartialSizes partial_sizes(const frehd_config &k)
     const bool mixed = k.precision == FREHD_PREC_MIXED;
     z.partF = mixed ? 4 * maxDir : 0;
     z.partC = mixed ? 2 * maxDir : 0;
+    z.bpart = (size_t)budget_slots(k.useScalar == 1 ? std::max(1, k.nScalar) : 0) * maxb;
     return z;
 }
 }  // namespace frehd
@@ -367,6 +368,7 @@ size_t layout(frehd_dev *d, char *base)
     d->naRef = cv.take<RefineArgs>(1);
     a.vpart = cv.take<double>(ps.vpart);
     a.cflpart = cv.take<double>(ps.cflpart);
+    a.bpart = cv.take<double>(ps.bpart);
     a.tideP = cv.take<double>(Nt1);
     a.tideM = cv.take<double>(Nt1);
     a.inflow = cv.take<double>(Nt1);
@@ -473,7 +475,8 @@ int check_config(const frehd_config *k)
     if (k->bcType < 1 || k->bcType > 3) return fail(FREHD_EINVAL, "bcType must be 1, 2 or 3");
     if (k->tideLocLengthP < 0 || k->tideLocLengthM < 0 || k->inflowLocLength < 0)
         return fail(FREHD_EINVAL, "negative location count");
-    if (k->useScalar == 1 && k->nScalar > 16) return fail(FREHD_EINVAL, "at most 16 scalars");
+    if (k->useScalar == 1 && k->nScalar > FREHD_MAX_SCALARS)
+        return fail(FREHD_EINVAL, "at most %d scalars", FREHD_MAX_SCALARS);
     if (k->useSubgrid != 0 && k->useSubgrid != 1) return fail(FREHD_EINVAL, "useSubgrid must be 0 or 1");
     if (k->useSubgrid == 1 && !(k->Ne >= 1 && k->dsurf > 0 && k->subR > 0))
         return fail(FREHD_EINVAL, "subgrid needs Ne >= 1, dsurf > 0, subR > 0");
@@ -995,13 +998,14 @@ int frehd_device_count(int32_t *count)
     return 0;
 }
 
-int frehd_abi_sizes(int64_t sizes[5])
+int frehd_abi_sizes(int64_t sizes[6])
 {
     sizes[0] = sizeof(frehd_config);
     sizes[1] = sizeof(frehd_forcing);
     sizes[2] = sizeof(frehd_ic);
     sizes[3] = sizeof(frehd_fields);
     sizes[4] = sizeof(frehd_stats);
+    sizes[5] = sizeof(frehd_budget);
     return 0;
 }
 
@@ -1184,6 +1188,49 @@ int frehd_dev_stats(frehd_dev_t *d, frehd_stats *out)
     for (int q = 0; q < 4; q++) out->Vloss[q] = c.Vloss[q];
     out->halted = c.halted;
     out->haltStep = c.haltStep;
+    return 0;
+}
+
+int frehd_dev_set_budget(frehd_dev_t *d, int32_t on)
+{
+    if (!d) return fail(FREHD_EINVAL, "handle is NULL");
+    if (int rc = state_check(d, CALL_TILING)) return rc;
+    HIPCHK(hipSetDevice(d->device));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    Budget zero;
+    memset(&zero, 0, sizeof zero);
+    COPYW(d->stream, &d->args.ctl->budget, &zero, sizeof zero, hipMemcpyHostToDevice);
+    d->args.p.budget = on ? 1 : 0;
+    return upload_member_args(d);
+}
+
+int frehd_dev_budget(frehd_dev_t *d, frehd_budget *out)
+{
+    if (!d || !out) return fail(FREHD_EINVAL, "NULL argument");
+    HIPCHK(hipSetDevice(d->device));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    Ctl c;
+    COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
+    memset(out, 0, sizeof *out);
+    const Budget &b = c.budget;
+    out->steps = b.steps;
+    out->nScalar = d->args.p.nScalar;
+    out->on = d->args.p.budget;
+    out->volume = b.vol;
+    out->volumeResidual = b.volRes;
+    out->volumeResidualMax = b.volResMax;
+    out->volumeBoundary = b.volBnd;
+    out->volumeInflow = b.volIn;
+    out->volumeClip = b.volClip;
+    out->volumeMoved = b.volMoved;
+    for (int q = 0; q < d->args.p.nScalar && q < FREHD_MAX_SCALARS; q++) {
+        out->mass[q] = b.mass[q];
+        out->massResidual[q] = b.massRes[q];
+        out->massResidualMax[q] = b.massResMax[q];
+        out->massBoundary[q] = b.massBnd[q];
+        out->massInflow[q] = b.massIn[q];
+        out->massMoved[q] = b.massMoved[q];
+    }
     return 0;
 }
 

@@ -146,6 +146,55 @@ __device__ inline double block_max(double v, double *lds)
     return v;
 }
 
+// ---- compensated (double-double) sums for the conservation budgets -------
+// hi + lo carries a sum to about twice FP64's precision (Knuth's two-sum;
+// the file is compiled without contraction, so no FMA alters the error terms)
+struct DD {
+    double hi, lo;
+};
+
+__host__ __device__ inline DD dd_two_sum(double a, double b)
+{
+    const double s = a + b;
+    const double bb = s - a;
+    return DD{s, (a - (s - bb)) + (b - bb)};
+}
+
+__host__ __device__ inline DD dd_add(DD x, DD y)
+{
+    const DD s = dd_two_sum(x.hi, y.hi);
+    const double e = s.lo + (x.lo + y.lo);
+    const double h = s.hi + e;
+    return DD{h, e - (h - s.hi)};
+}
+
+__host__ __device__ inline DD dd_add(DD x, double y) { return dd_add(x, DD{y, 0.0}); }
+
+// block sum of a DD in a fixed order; valid in thread 0.  lds: 32 doubles.
+__device__ inline DD block_dd_sum(DD v, double *lds)
+{
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+    const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const DD o{__shfl_xor(v.hi, off, 64), __shfl_xor(v.lo, off, 64)};
+        v = dd_add(v, o);
+    }
+    if (lane == 0) {
+        lds[2 * wid] = v.hi;
+        lds[2 * wid + 1] = v.lo;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        DD s{0.0, 0.0};
+        for (int w = 0; w < nw; w++) s = dd_add(s, DD{lds[2 * w], lds[2 * w + 1]});
+        v = s;
+    }
+    __syncthreads();
+    return v;
+}
+
 // LASpack's IsZero / IsOne tests (elcmp.h:31-32): |a| < 10 DBL_MIN and
 // |a - 1| < 10 DBL_EPSILON
 #define LAS_IS_ZERO(a) (fabs(a) < 10.0 * 2.2250738585072014e-308)

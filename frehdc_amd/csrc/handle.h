@@ -135,7 +135,7 @@ void exec_release_graphs(Exec &e);
 // partA, partB, and for mixed precision partF twice and partC; volume
 // losses vpart, CFL maxima cflpart) and the block counts they are sized by
 struct PartialSizes {
-    size_t partA, partB, vpart, cflpart, partF, partC;
+    size_t partA, partB, vpart, cflpart, partF, partC, bpart;
     size_t maxBlocks, nb1, nb2;
 };
 PartialSizes partial_sizes(const frehd_config &k);

@@ -78,6 +78,7 @@ size_t point_into_scratch(frehd_dev_impl *d, char *base)
     a.partB = take(ps.partB);
     a.vpart = take(ps.vpart);
     a.cflpart = take(ps.cflpart);
+    a.bpart = take(ps.bpart);
     PcgArgs<double> &p = d->naPcgH;
     p = d->pcg;
     p.partA = a.partA;

@@ -27,7 +27,27 @@ struct P {
     double dsurf, subR, surfmin;   // surfmin: raw elevation of level 0
     int Ne;
     int warmStart;            // mixed precision: refinement starts from eta^n instead of 0
+    int budget;               // account the conservation budgets (frehd_dev_set_budget)
 };
+
+// Flux-form conservation budgets, accumulated over the steps since they
+// were switched on: per step, the change of the domain's volume (of each
+// scalar's mass) minus what crossed the domain boundary, the inflow and --
+// for the volume -- what the clip of negative cell volumes added; sums in
+// double-double per step (the residual), then in FP64 over steps.
+constexpr int kMaxScalars = 16;
+struct Budget {
+    long long steps;
+    double vol, volRes, volResMax, volBnd, volIn, volClip, volMoved;
+    double mass[kMaxScalars], massRes[kMaxScalars], massResMax[kMaxScalars];
+    double massBnd[kMaxScalars], massIn[kMaxScalars], massMoved[kMaxScalars];
+};
+// per-block partial slots of the budgets (bpart[slot * maxBlocks + block]):
+// volumeByFlux's 7 (residual hi, lo, volume, boundary, inflow, clip, sum of
+// the cells' |change|), then 6 per scalar (residual hi, lo, mass, boundary,
+// inflow, sum of |change|)
+constexpr int kBudgetVolSlots = 7, kBudgetScalarSlots = 6;
+__host__ __device__ inline int budget_slots(int nScalar) { return kBudgetVolSlots + kBudgetScalarSlots * nScalar; }
 
 struct PcgScalars {
     double bb, bNorm, rr, rNorm, pq, alpha;
@@ -73,6 +93,7 @@ struct Ctl {
     PcgScalars pcgIn;         // inner FP32 solve of the mixed-precision path
     int refineLast;
     int pad1;
+    Budget budget;
 };
 
 struct StepArgs {
@@ -100,6 +121,7 @@ struct StepArgs {
     // losses (vpart: 4 slots) and CFL maxima (cflpart: 2 slots), each slot
     // maxBlocks long; nb1 = blocks of an Ng sweep, nb2 = blocks of a 2-D sweep
     gptr<double> partA, partB, vpart, cflpart;
+    gptr<double> bpart;            // conservation-budget partials (budget_slots x maxBlocks)
     int maxBlocks, nb1, nb2, pad3;
     // this member's launch extents (a batched launch is sized for its
     // largest member; blocks past these exit at once): 2-D stencil grid of

@@ -581,6 +581,68 @@ __global__ void k_tidal_velocity(const StepArgs *__restrict__ A)
 }
 
 // ---------------------------------------------------------------------------
+// volumeByFlux (nsfunctions.c:373-409; volumeByFluxSub subgrid.c:1227-1265,
+// whose inflow needs bcType != 3) of cell c, and, with the budgets on, its
+// share of the flux-form volume budget: the change of the cell's volume
+// minus the flow through its faces on the domain boundary, the inflow and
+// the volume the clip of a negative value added.  Interior faces cancel
+// between their two cells, so the sum over the domain is zero up to the
+// rounding of the cells' updates.
+struct VolBudget {
+    DD res;
+    double vol, bnd, in, clip, moved;
+};
+
+__device__ inline void volume_update(const StepArgs &a, int c, int i, int j, int tt, bool inflowOn, VolBudget &b)
+{
+    const Grid &g = a.g;
+    const P &p = a.p;
+    const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
+    const double Vold = a.cellV[c];
+    double V = Vold + p.dt * (a.Fuu[w] - a.Fuu[c] + a.Fvv[s] - a.Fvv[c]);
+    const int ninflow = inflowOn ? (a.flags[c] >> 2) : 0;
+    for (int q = 0; q < ninflow; q++) V = V + a.inflow[tt] * p.dt;
+    double clip = 0.0;
+    if (V < 0) {
+        clip = -V;
+        V = 0;
+    }
+    a.cellV[c] = V;
+    if (p.budget) {
+        double bf = 0.0;
+        if (i == 0) bf += a.Fuu[w];
+        if (i == g.nx - 1) bf -= a.Fuu[c];
+        if (j == 0) bf += a.Fvv[s];
+        if (j == g.ny - 1) bf -= a.Fvv[c];
+        const double bnd = p.dt * bf, in = ninflow * (a.inflow[tt] * p.dt);
+        DD r = dd_two_sum(V, -Vold);
+        r = dd_add(r, -bnd);
+        r = dd_add(r, -in);
+        r = dd_add(r, -clip);
+        b.res = dd_add(b.res, r);
+        b.vol += V;
+        b.bnd += bnd;
+        b.in += in;
+        b.clip += clip;
+        b.moved += fabs(V - Vold);
+    }
+}
+
+// the block's volume-budget partials (budget slots 0..6)
+__device__ inline void store_vol_budget(const StepArgs &a, VolBudget &b, double *lds)
+{
+    const DD r = block_dd_sum(b.res, lds);
+    double v[5] = {b.vol, b.bnd, b.in, b.clip, b.moved};
+    block_sum<5>(v, lds);
+    if (threadIdx.x == 0) {
+        const int mb = a.maxBlocks, k = blockIdx.x;
+        a.bpart[0 * mb + k] = r.hi;
+        a.bpart[1 * mb + k] = r.lo;
+        for (int q = 0; q < 5; q++) a.bpart[(2 + q) * mb + k] = v[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // velocityInterp + volumeByFlux + updateCD + thinLayerDrag + monitorCFL
 // (nsfunctions.c:337-347, 373-409, 1170-1189, 691-727, 1192-1232).
 // velocityInterp uses flat-index neighbours (row wrap, ghost reads); indices
@@ -589,12 +651,13 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgs *__restrict__ A)
 {
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
-    __shared__ double lds[16];
+    __shared__ double lds[16 * 5];
     if (tail_off(a)) return;
     const Grid g = a.g;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const P &p = a.p;
     double cflx = 0, cfly = 0;
+    VolBudget vb{{0.0, 0.0}, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (c < g.N) {
         const int tt = cur_tt(a);
         const int nx = g.nx, i = c % nx, j = c / nx;
@@ -606,11 +669,7 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgs *__restrict__ A)
         a.uuYP[c] = 0.25 * (u[c] + um1 + u[c + nx] + um1n);
         a.vvXP[c] = 0.25 * (v[c] + v[c + 1] + vmn + vmn1);
 
-        double V = a.cellV[c] + p.dt * (a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] + a.Fvv[nbS(g, c, i, j)] - a.Fvv[c]);
-        const int ninflow = a.flags[c] >> 2;
-        for (int q = 0; q < ninflow; q++) V = V + a.inflow[tt] * p.dt;
-        if (V < 0) V = 0;
-        a.cellV[c] = V;
+        volume_update(a, c, i, j, tt, true, vb);
 
         const double d = depth_nxt(a)[c];
         if (p.CDnotN == 0 && d > 0) {
@@ -642,6 +701,7 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgs *__restrict__ A)
         a.cflpart[blockIdx.x] = mx;
         a.cflpart[a.maxBlocks + blockIdx.x] = my;
     }
+    if (p.budget) store_vol_budget(a, vb, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -685,46 +745,67 @@ __device__ inline double tvd_face(double SL, double SH, double SLL, double SHH, 
     return SH + 0.5 * superbee(r) * (SL - SH) * Cu * (1 - Cu);
 }
 
+// One scalar's share of the flux-form mass budget of a cell: the change of
+// its mass minus the advective and diffusive fluxes through its faces on the
+// domain boundary and the inflow mass (the interior faces cancel between
+// their two cells).
+struct MassBudget {
+    DD res;
+    double mass, bnd, in, moved;
+};
+
 template <bool SUB>
-__device__ inline void scalar_body(const StepArgs &a)
+__device__ inline void scalar_body(const StepArgs &a, double *lds)
 {
     if (tail_off(a) || a.p.nScalar < 1) return;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= g.nx || j >= g.ny) return;
+    // every thread of the block stays to the end (the budget's block sums);
+    // the cells outside the grid do nothing
+    const bool valid = i < g.nx && j < g.ny;
     const P &p = a.p;
     const int tt = cur_tt(a);
-    const int c = j * g.nx + i;
-    const int e = nbE(g, c, i, j), w = nbW(g, c, i, j), n = nbN(g, c, i, j), s = nbS(g, c, i, j);
+    const int c = valid ? j * g.nx + i : 0;
+    const int e = valid ? nbE(g, c, i, j) : 0, w = valid ? nbW(g, c, i, j) : 0;
+    const int n = valid ? nbN(g, c, i, j) : 0, s = valid ? nbS(g, c, i, j) : 0;
     const double *Fu = a.Fuu, *Fv = a.Fvv;
     const double dt = p.dt, dx = p.dx, dy = p.dy;
     // geometry of the cell, the same for every scalar: flow rates, face
     // areas or depths, the connectivity with the previous-step surface
     // (scalar.c:292-341; the subgrid tests face areas, subgrid.c:1588-1630)
-    const double Fuc = Fu[c], Fuw = Fu[w], Fvc = Fv[c], Fvs = Fv[s];
     const double *ho = a.surfOld, *zb = a.bottomZ;
     const double *fx = SUB ? a.Nx : a.depthXP, *fy = SUB ? a.Oy : a.depthYP;
-    const double fxc = fx[c], fxw = fx[w], fyc = fy[c], fys = fy[s];
-    const bool cE = fxc > 0 && ho[e] - zb[e] > 0;
-    const bool cW = fxw > 0 && ho[w] - zb[w] > 0;
-    const bool cN = n <= g.N && fyc > 0 && ho[n] - zb[n] > 0;
-    const bool cS = s <= g.N && fys > 0 && ho[s] - zb[s] > 0;
-    // the subgrid path adds inflow only when bcType != 3 (subgrid.c:1660)
-    const int ninflow = (SUB && p.bcType == 3) ? 0 : (a.flags[c] >> 2);
-    const double V = a.cellV[c];
-    const uint8_t fl = a.flags[c];
+    double Fuc = 0, Fuw = 0, Fvc = 0, Fvs = 0, fxc = 0, fxw = 0, fyc = 0, fys = 0, V = 0;
+    bool cE = false, cW = false, cN = false, cS = false;
+    int ninflow = 0;
+    uint8_t fl = 0;
+    if (valid) {
+        Fuc = Fu[c]; Fuw = Fu[w]; Fvc = Fv[c]; Fvs = Fv[s];
+        fxc = fx[c]; fxw = fx[w]; fyc = fy[c]; fys = fy[s];
+        cE = fxc > 0 && ho[e] - zb[e] > 0;
+        cW = fxw > 0 && ho[w] - zb[w] > 0;
+        cN = n <= g.N && fyc > 0 && ho[n] - zb[n] > 0;
+        cS = s <= g.N && fys > 0 && ho[s] - zb[s] > 0;
+        // the subgrid path adds inflow only when bcType != 3 (subgrid.c:1660)
+        ninflow = (SUB && p.bcType == 3) ? 0 : (a.flags[c] >> 2);
+        V = a.cellV[c];
+        fl = a.flags[c];
+    }
     const bool bottomRow = (j == 0), topRowNotFirst = (j == g.ny - 1 && i > 0);
     // each scalar in turn, as separate passes would run them: a pass that
     // resets an out-of-range cell lowers that cell's surface and depth for
     // the passes after it
     for (int q = 0; q < p.nScalar; q++) {
+        MassBudget mb{{0.0, 0.0}, 0.0, 0.0, 0.0, 0.0};
+        if (valid) {
         const double *S = s_cur(a) + (size_t)q * a.sStride;
         double *Sm = a.Sm + (size_t)q * a.sStride;
         const double Sc = S[c], Se = S[e], Sw = S[w], Sn = S[n], Ss = S[s];
-        double sm = Sm[c];
+        const double smOld = Sm[c];
+        double sm = smOld;
+        double Sd, Su, Sr, Sl;
         {
-            double Sd, Su, Sr, Sl;
             if (p.scalarAdv == 2) {
                 // TVD superbee (scalar.c:99-255)
                 const int ee = nbEE(g, c, i, j), ww = nbWW(g, c, i, j), nn = nbNN(g, c, i, j), ss = nbSS(g, c, i, j);
@@ -742,12 +823,13 @@ __device__ inline void scalar_body(const StepArgs &a)
             }
             sm = sm + dt * (-Fuc * Sd + Fuw * Su - Fvc * Sr + Fvs * Sl);
         }
-        if (SUB)   // scalarDiffusionSub, subgrid.c:1552-1567
-            sm = sm + dt * ((p.Kx * fxc / dx) * (Se - Sc) - (p.Kx * fxw / dx) * (Sc - Sw) +
-                            (p.Ky * fyc / dy) * (Sn - Sc) - (p.Ky * fys / dy) * (Sc - Ss));
-        else
-            sm = sm + dt * ((p.Kx * dy * fxc / dx) * (Se - Sc) - (p.Kx * dy * fxw / dx) * (Sc - Sw) +
-                            (p.Ky * dx * fyc / dy) * (Sn - Sc) - (p.Ky * dx * fys / dy) * (Sc - Ss));
+        // diffusion coefficients of the four faces
+        const double kE = SUB ? p.Kx * fxc / dx : p.Kx * dy * fxc / dx;
+        const double kW = SUB ? p.Kx * fxw / dx : p.Kx * dy * fxw / dx;
+        const double kN = SUB ? p.Ky * fyc / dy : p.Ky * dx * fyc / dy;
+        const double kS = SUB ? p.Ky * fys / dy : p.Ky * dx * fys / dy;
+        // scalarDiffusionSub, subgrid.c:1552-1567 / scalarDiffusion, scalar.c:259-274
+        sm = sm + dt * (kE * (Se - Sc) - kW * (Sc - Sw) + kN * (Sn - Sc) - kS * (Sc - Ss));
         double vals[4];
         int m = 0;
         if (cE) vals[m++] = Se;
@@ -765,6 +847,23 @@ __device__ inline void scalar_body(const StepArgs &a)
         }
         for (int t = 0; t < ninflow; t++) sm = sm + p.inflowS * a.inflow[tt] * dt / p.inflowLocLength;
         Sm[c] = sm;
+        if (p.budget) {
+            double bf = 0.0;
+            if (i == 0) bf += Fuw * Su - kW * (Sc - Sw);
+            if (i == g.nx - 1) bf += -Fuc * Sd + kE * (Se - Sc);
+            if (j == 0) bf += Fvs * Sl - kS * (Sc - Ss);
+            if (j == g.ny - 1) bf += -Fvc * Sr + kN * (Sn - Sc);
+            const double bnd = dt * bf;
+            const double in = ninflow ? ninflow * (p.inflowS * a.inflow[tt] * dt / p.inflowLocLength) : 0.0;
+            DD r = dd_two_sum(sm, -smOld);
+            r = dd_add(r, -bnd);
+            r = dd_add(r, -in);
+            mb.res = r;
+            mb.mass = sm;
+            mb.bnd = bnd;
+            mb.in = in;
+            mb.moved = fabs(sm - smOld);
+        }
         double Sv = V > 0 ? sm / V : 0;
         if (m != 0 && Sv > smax && ninflow == 0) Sv = smax;
         else if (m != 0 && Sv < smin && ninflow == 0) Sv = smin;
@@ -784,6 +883,20 @@ __device__ inline void scalar_body(const StepArgs &a)
         if ((fl & 2) && p.bcType != 1) Sv = p.useConstTideMS == 1 ? p.tideMS : a.tidalMS[tt];
         if (depth_nxt(a)[c] <= 0) Sv = 0;
         Snx[c] = Sv;
+        }
+        if (p.budget) {
+            // the block's partials of this scalar (slots 6 + 5q ...)
+            const DD r = block_dd_sum(mb.res, lds);
+            double v[4] = {mb.mass, mb.bnd, mb.in, mb.moved};
+            block_sum<4>(v, lds);
+            if (threadIdx.x == 0 && threadIdx.y == 0) {
+                const int mbk = a.maxBlocks, k = blockIdx.x + a.gx2 * blockIdx.y;
+                const int s0 = kBudgetVolSlots + kBudgetScalarSlots * q;
+                a.bpart[(size_t)(s0 + 0) * mbk + k] = r.hi;
+                a.bpart[(size_t)(s0 + 1) * mbk + k] = r.lo;
+                for (int u = 0; u < 4; u++) a.bpart[(size_t)(s0 + 2 + u) * mbk + k] = v[u];
+            }
+        }
     }
 }
 
@@ -792,13 +905,66 @@ __global__ void __launch_bounds__(256) k_scalar(const StepArgs *__restrict__ A)
 {
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
-    scalar_body<false>(a);
+    __shared__ double lds[16 * 4];
+    scalar_body<false>(a, lds);
 }
 __global__ void __launch_bounds__(256) k_scalar_sub(const StepArgs *__restrict__ A)
 {
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
-    scalar_body<true>(a);
+    __shared__ double lds[16 * 4];
+    scalar_body<true>(a, lds);
+}
+
+// the conservation budgets of the step: the blocks' partials summed in
+// block order (double-double for the residuals), the step's residuals and
+// totals added to the member's accumulators (thread 0)
+__device__ inline void step_end_budget(const StepArgs &a)
+{
+    __shared__ double lds[16 * 5];
+    const int t = threadIdx.x, nt = blockDim.x, mb = a.maxBlocks;
+    Budget &bu = a.ctl->budget;
+    {
+        DD r{0.0, 0.0};
+        double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int b = t; b < a.nb1; b += nt) {
+            r = dd_add(r, DD{a.bpart[b], a.bpart[mb + b]});
+            for (int q = 0; q < 5; q++) v[q] += a.bpart[(size_t)(2 + q) * mb + b];
+        }
+        r = block_dd_sum(r, lds);
+        block_sum<5>(v, lds);
+        if (t == 0) {
+            const double res = r.hi + r.lo;
+            bu.vol = v[0];
+            bu.volRes += res;
+            bu.volResMax = fmax(bu.volResMax, fabs(res));
+            bu.volBnd += v[1];
+            bu.volIn += v[2];
+            bu.volClip += v[3];
+            bu.volMoved += v[4];
+            bu.steps += 1;
+        }
+    }
+    for (int q = 0; q < a.p.nScalar; q++) {
+        const int s0 = kBudgetVolSlots + kBudgetScalarSlots * q;
+        DD r{0.0, 0.0};
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int b = t; b < a.nb2; b += nt) {
+            r = dd_add(r, DD{a.bpart[(size_t)s0 * mb + b], a.bpart[(size_t)(s0 + 1) * mb + b]});
+            for (int u = 0; u < 4; u++) v[u] += a.bpart[(size_t)(s0 + 2 + u) * mb + b];
+        }
+        r = block_dd_sum(r, lds);
+        block_sum<4>(v, lds);
+        if (t == 0) {
+            const double res = r.hi + r.lo;
+            bu.mass[q] = v[0];
+            bu.massRes[q] += res;
+            bu.massResMax[q] = fmax(bu.massResMax[q], fabs(res));
+            bu.massBnd[q] += v[1];
+            bu.massIn[q] += v[2];
+            bu.massMoved[q] += v[3];
+        }
+    }
 }
 
 // Step end: volume-loss and CFL reductions (block partials in block order),
@@ -841,6 +1007,7 @@ __global__ void __launch_bounds__(1024) k_step_end(const StepArgs *__restrict__ 
     block_sum<4>(v, lds);
     mx = block_max(mx, lds);
     my = block_max(my, lds);
+    if (a.p.budget) step_end_budget(a);
     if (threadIdx.x != 0) return;
     for (int q = 0; q < 4; q++) c->Vloss[q] += v[q];
     c->lastMaxCFLx = mx;
@@ -1409,12 +1576,13 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgs *__restrict__
 {
     const StepArgs &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
-    __shared__ double lds[16];
+    __shared__ double lds[16 * 5];
     if (tail_off(a)) return;
     const Grid g = a.g;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const P &p = a.p;
     double cflx = 0, cfly = 0;
+    VolBudget vb{{0.0, 0.0}, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (c < g.N) {
         const int tt = cur_tt(a);
         const int nx = g.nx, i = c % nx, j = c / nx;
@@ -1426,11 +1594,7 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgs *__restrict__
         a.uuYP[c] = 0.25 * (u[c] + um1 + u[c + nx] + um1n);
         a.vvXP[c] = 0.25 * (v[c] + v[c + 1] + vmn + vmn1);
 
-        double V = a.cellV[c] + p.dt * (a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] + a.Fvv[nbS(g, c, i, j)] - a.Fvv[c]);
-        const int ninflow = p.bcType != 3 ? (a.flags[c] >> 2) : 0;
-        for (int q = 0; q < ninflow; q++) V = V + a.inflow[tt] * p.dt;
-        if (V < 0) V = 0;
-        a.cellV[c] = V;
+        volume_update(a, c, i, j, tt, p.bcType != 3, vb);
 
         if (p.CDnotN == 0 && p.useSubDrag == 2) {
             // Volp 2013 (subgrid.c:1481-1506): g n^2 Yh, capped, 0 where Yh <= 0
@@ -1481,6 +1645,7 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgs *__restrict__
         a.cflpart[blockIdx.x] = mx;
         a.cflpart[a.maxBlocks + blockIdx.x] = my;
     }
+    if (p.budget) store_vol_budget(a, vb, lds);
 }
 
 // matrixSourceTermSub + matrixCoeffSub (useSubgrid = 1, staggeredV = 1) +

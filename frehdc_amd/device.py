@@ -120,6 +120,35 @@ class DeviceScenario:
                     maxCFLy=st.maxCFLy, Vloss=list(st.Vloss), halted=bool(st.halted),
                     haltStep=st.haltStep)
 
+    def set_budget(self, on=True):
+        """Switch the conservation budgets on (cleared) or off
+        (frehd_dev_set_budget)."""
+        abi.check(self.lib.frehd_dev_set_budget(self.h, 1 if on else 0))
+
+    def budget(self) -> dict:
+        """The conservation budgets (frehd_dev_budget): volume and, per
+        scalar, mass residuals of the flux-form updates, and relative
+        measures (largest step residual over the total, summed residual
+        over the total)."""
+        b = abi.FrehdBudget()
+        abi.check(self.lib.frehd_dev_budget(self.h, C.byref(b)))
+        ns = b.nScalar
+        out = dict(steps=b.steps, on=bool(b.on), volume=b.volume, volumeResidual=b.volumeResidual,
+                   volumeResidualMax=b.volumeResidualMax, volumeBoundary=b.volumeBoundary,
+                   volumeInflow=b.volumeInflow, volumeClip=b.volumeClip, volumeMoved=b.volumeMoved,
+                   mass=list(b.mass[:ns]), massResidual=list(b.massResidual[:ns]),
+                   massResidualMax=list(b.massResidualMax[:ns]), massBoundary=list(b.massBoundary[:ns]),
+                   massInflow=list(b.massInflow[:ns]), massMoved=list(b.massMoved[:ns]))
+        out["volumeRelResidual"] = abs(b.volumeResidual) / b.volume if b.volume > 0 else 0.0
+        out["volumeRelResidualMax"] = b.volumeResidualMax / b.volume if b.volume > 0 else 0.0
+        # a scalar's mass may start at zero (the boundary brings it in): its
+        # scale is the larger of the total and the mass the faces moved
+        scale = [max(abs(m), abs(mv) / max(b.steps, 1), 1e-300) for m, mv in
+                 zip(out["mass"], out["massMoved"])]
+        out["massRelResidual"] = [abs(r) / s for r, s in zip(out["massResidual"], scale)]
+        out["massRelResidualMax"] = [r / s for r, s in zip(out["massResidualMax"], scale)]
+        return out
+
     @property
     def tt(self):
         v = C.c_int32()

@@ -232,6 +232,39 @@ typedef struct frehd_stats {
     int32_t haltStep;         /* the step at which it stopped                      */
 } frehd_stats;
 
+/* ---- conservation budgets (frehd_dev_set_budget / frehd_dev_budget) --------
+ * Flux form, accumulated over the steps since the budgets were switched on:
+ * per step, the change of the domain's total cell volume (volumeByFlux,
+ * nsfunctions.c:373-409; subgrid.c:1227-1265) minus the flow through the
+ * domain boundary's faces, the inflow and what the clip of negative cell
+ * volumes added; per scalar, the change of its total mass over the
+ * advection-diffusion update (scalar.c:68-274, with the inflow mass,
+ * scalar.c:346-356) minus the advective and diffusive fluxes through the
+ * boundary's faces and the inflow mass.  Interior faces cancel between
+ * their cells, so the residuals measure only the rounding of the updates:
+ * each step's is summed in double-double, the steps' in FP64.  The
+ * concentration limiter and the out-of-range reset that follow
+ * (scalar.c:357-412) are outside the flux form and outside these sums. */
+#define FREHD_MAX_SCALARS 16
+typedef struct frehd_budget {
+    int64_t steps;              /* steps accounted                                */
+    int32_t nScalar, on;
+    double volume;              /* total cell volume after the last step [m^3]    */
+    double volumeResidual;      /* sum of the steps' residuals [m^3]              */
+    double volumeResidualMax;   /* largest |residual| of one step [m^3]           */
+    double volumeBoundary;      /* volume in through the boundary faces [m^3]     */
+    double volumeInflow;        /* inflow volume [m^3]                            */
+    double volumeClip;          /* volume the clip of negative cells added [m^3]  */
+    double volumeMoved;         /* sum over steps and cells of |volume change|:
+                                   the water the faces moved [m^3]               */
+    double mass[FREHD_MAX_SCALARS];            /* total Sm after the last step     */
+    double massResidual[FREHD_MAX_SCALARS];
+    double massResidualMax[FREHD_MAX_SCALARS];
+    double massBoundary[FREHD_MAX_SCALARS];    /* mass in through the boundary     */
+    double massInflow[FREHD_MAX_SCALARS];
+    double massMoved[FREHD_MAX_SCALARS];       /* sum of |mass change|             */
+} frehd_budget;
+
 typedef struct frehd_dev frehd_dev_t;
 
 /* Last error message on the calling thread ("" when none). */
@@ -243,9 +276,9 @@ const char *frehd_version(void);
 /* Number of visible devices (0 when none). */
 int frehd_device_count(int32_t *count);
 
-/* sizeof of frehd_config, frehd_forcing, frehd_ic, frehd_fields, frehd_stats
- * as compiled into the library (binding checks). */
-int frehd_abi_sizes(int64_t sizes[5]);
+/* sizeof of frehd_config, frehd_forcing, frehd_ic, frehd_fields, frehd_stats,
+ * frehd_budget as compiled into the library (binding checks). */
+int frehd_abi_sizes(int64_t sizes[6]);
 
 /* Create a device handle from raw inputs and run the Init sequence
  * (initialize.c:40-155: bathymetry ghosts, face elevations, initial fields,
@@ -283,6 +316,14 @@ int frehd_dev_download(frehd_dev_t *dev, frehd_fields *out);
 
 /* Statistics of the handle. */
 int frehd_dev_stats(frehd_dev_t *dev, frehd_stats *out);
+
+/* Switch the conservation budgets on (on != 0) or off and clear them: the
+ * steps after it are accounted (frehd_budget).  No counterpart in the
+ * reference (its checkConservation flag is never assigned, configuration.c).
+ * At a step boundary, on a handle not in a batch (set it before adding the
+ * handle); costs a few per-block partial sums per step. */
+int frehd_dev_set_budget(frehd_dev_t *dev, int32_t on);
+int frehd_dev_budget(frehd_dev_t *dev, frehd_budget *out);
 
 /* Current step counter (number of steps taken). */
 int frehd_dev_step_index(frehd_dev_t *dev, int32_t *tt);

@@ -41,10 +41,10 @@ def test_exports_every_declared_function(lib):
 
 
 def test_struct_sizes_match(lib):
-    sizes = (C.c_int64 * 5)()
+    sizes = (C.c_int64 * 6)()
     assert lib.frehd_abi_sizes(sizes) == 0
     ours = [C.sizeof(abi.FrehdConfig), C.sizeof(abi.FrehdForcing), C.sizeof(abi.FrehdIC),
-            C.sizeof(abi.FrehdFields), C.sizeof(abi.FrehdStats)]
+            C.sizeof(abi.FrehdFields), C.sizeof(abi.FrehdStats), C.sizeof(abi.FrehdBudget)]
     assert list(sizes) == ours
 
 
