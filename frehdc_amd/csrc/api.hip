@@ -201,9 +201,9 @@ static int copy_wait(hipStream_t st, void *dst, const void *src, size_t bytes, h
     } while (0)
 
 namespace frehd {
-int launch_subgrid_tables(const double *fineDev, double *tabs, int nx, int ny, int r, int Ne,
-                          double surfmin, double dsurf, double dxf, hipStream_t stream);
-int launch_tables_reorder(double *tabs, double *lm, int N, int Ne, int T, int toInterleaved, hipStream_t stream);
+int launch_subgrid_tables(const double *fineDev, void *tabs, int nx, int ny, int r, int Ne, double surfmin,
+                          double dsurf, double dxf, int f32, hipStream_t stream);
+int launch_tables_reorder(void *tabs, double *lm, int N, int Ne, int T, int toInterleaved, int f32, hipStream_t stream);
 }
 
 namespace {
@@ -308,6 +308,9 @@ PartialSizes partial_sizes(const frehd_config &k)
 
 namespace {
 
+// the explicit state is stored in FP32 by the mixed form
+inline bool state_f32(const frehd_config &k) { return k.precision == FREHD_PREC_MIXED; }
+
 // lay out every device array; with base == nullptr only the size is computed
 size_t layout(frehd_dev *d, char *base)
 {
@@ -317,17 +320,23 @@ size_t layout(frehd_dev *d, char *base)
     const size_t Nt1 = (size_t)k.Nt + 1;
     Carve cv{base, 0};
     StepArgs &a = d->args;
+    // the explicit state in the handle's state precision (FP32 for the mixed
+    // form; the pointers are typed double on the host either way)
+    const bool f32 = state_f32(k);
+    auto takeR = [&](size_t n) { return f32 ? reinterpret_cast<double *>(cv.take<float>(n)) : cv.take<double>(n); };
     a.ctl = cv.take<Ctl>(1);
-    double **ngArr[] = {&a.surf, &a.surfOld, &a.uuXP, &a.vvYP, &a.depthXP, &a.depthYP, &a.EnXP,
-        &a.EnYP, &a.DragXP, &a.DragYP, &a.CDXP, &a.CDYP, &a.Fuu, &a.Fvv, &a.CFLx, &a.CFLy,
-        &a.wtfXP, &a.wtfYP, &a.bottomZ, &a.bottomXP, &a.bottomYP};
+    double **ngArr[] = {&a.surf, &a.surfOld, &a.bottomZ, &a.bottomXP, &a.bottomYP};
     for (auto **p : ngArr) *p = cv.take<double>(Ng);
+    double **ngR[] = {&a.uuXP, &a.vvYP, &a.depthXP, &a.depthYP, &a.EnXP, &a.EnYP, &a.DragXP, &a.DragYP,
+        &a.CDXP, &a.CDYP, &a.Fuu, &a.Fvv, &a.CFLx, &a.CFLy, &a.wtfXP, &a.wtfYP};
+    for (auto **p : ngR) *p = takeR(Ng);
     a.sStride = (int64_t)((Ng + 31) & ~size_t(31));
-    a.S = cv.take<double>((size_t)a.sStride * ns);
-    a.S2 = cv.take<double>((size_t)a.sStride * ns);
+    a.S = takeR((size_t)a.sStride * ns);
+    a.S2 = takeR((size_t)a.sStride * ns);
     a.Sm = cv.take<double>((size_t)a.sStride * ns);
-    double **nArr[] = {&a.uuYP, &a.vvXP, &a.depthA, &a.depthB, &a.cellV, &a.cX, &a.cY,
-        &a.D, &a.b, &a.x, &a.r, &a.q, &a.pA, &a.pB};
+    a.uuYP = takeR(N);
+    a.vvXP = takeR(N);
+    double **nArr[] = {&a.depthA, &a.depthB, &a.cellV, &a.cX, &a.cY, &a.D, &a.b, &a.x, &a.r, &a.q, &a.pA, &a.pB};
     for (auto **p : nArr) *p = cv.take<double>(N);
     // SSOR: the preconditioned residual z
     d->pcg.z = k.precond == FREHD_PRECOND_SSOR ? cv.take<double>(N) : nullptr;
@@ -385,10 +394,10 @@ size_t layout(frehd_dev *d, char *base)
     a.tideLocM = cv.take<int>(std::max(1, k.tideLocLengthM));
     if (k.useSubgrid == 1) {
         double **ngs[] = {&a.V, &a.Z, &a.Vx, &a.Vy, &a.Zx, &a.Zy, &a.Nx, &a.Oy};
-        for (auto **p : ngs) *p = cv.take<double>(Ng);
-        double **ns2[] = {&a.Vxp, &a.Vxm, &a.Vyp, &a.Vym, &a.Np, &a.Nm, &a.Op, &a.Om, &a.Vraw, &a.Zraw,
-                          &a.hSub};
-        for (auto **p : ns2) *p = cv.take<double>(N);
+        for (auto **p : ngs) *p = takeR(Ng);
+        double **ns2[] = {&a.Vxp, &a.Vxm, &a.Vyp, &a.Vym, &a.Np, &a.Nm, &a.Op, &a.Om, &a.Vraw, &a.Zraw};
+        for (auto **p : ns2) *p = takeR(N);
+        a.hSub = cv.take<double>(N);
         a.dSub = cv.take<uint8_t>(N);
         a.ind = cv.take<int>(Ng);
         a.edgeX = cv.take<double>(N);
@@ -401,7 +410,7 @@ size_t layout(frehd_dev *d, char *base)
             a.bed.r = r;
             a.bed.z = cv.take<double>(N * (size_t)r * r);
         } else {
-            a.tabs = cv.take<double>((size_t)k.Ne * N * 10);
+            a.tabs = takeR((size_t)k.Ne * N * 10);
         }
         a.allYhX = a.allYhY = a.allCn = nullptr;
         a.YhX = a.YhY = a.Cn = nullptr;
@@ -669,6 +678,33 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
 }
 
 
+// n host doubles into a device array of the handle's state precision
+// (rounded to FP32 for the mixed form; `keep` holds the rounded copies until
+// the copies have completed)
+int upload_r(frehd_dev *d, const double *src, double *dst, size_t n, std::vector<std::vector<float>> &keep)
+{
+    if (!src) return 0;
+    if (!state_f32(d->cfg)) return upload(d, src, dst, n * sizeof(double));
+    keep.emplace_back(n);
+    std::vector<float> &f = keep.back();
+    for (size_t i = 0; i < n; i++) f[i] = (float)src[i];
+    return upload(d, f.data(), dst, n * sizeof(float));
+}
+
+// a device array of the handle's state precision into n host doubles
+int download_r(frehd_dev *d, double *dst, const double *src, size_t n)
+{
+    if (!dst) return 0;
+    if (!state_f32(d->cfg)) {
+        COPYW(d->stream, dst, src, n * sizeof(double), hipMemcpyDeviceToHost);
+        return 0;
+    }
+    std::vector<float> f(n);
+    COPYW(d->stream, f.data(), src, n * sizeof(float), hipMemcpyDeviceToHost);
+    for (size_t i = 0; i < n; i++) dst[i] = f[i];
+    return 0;
+}
+
 // state arrays into the buffers the next step starts from (depthParity,
 // sParity: the double-buffered depth and concentration buffers in use)
 int upload_state(frehd_dev *d, const frehd_fields &s, int depthParity = 0, int sParity = 0)
@@ -676,26 +712,32 @@ int upload_state(frehd_dev *d, const frehd_fields &s, int depthParity = 0, int s
     const size_t N = d->g.N, Ng = d->g.Ng;
     StepArgs &a = d->args;
     double *depthCur = depthParity ? a.depthB : a.depthA;
-    struct { const double *src; double *dst; size_t n; } ng[] = {
-        {s.surf, a.surf, Ng}, {s.surfOld, a.surfOld, Ng}, {s.uuXP, a.uuXP, Ng}, {s.vvYP, a.vvYP, Ng},
-        {s.uuYP, a.uuYP, N}, {s.vvXP, a.vvXP, N}, {s.depth, depthCur, N}, {s.depthXP, a.depthXP, Ng},
-        {s.depthYP, a.depthYP, Ng}, {s.EnXP, a.EnXP, Ng}, {s.EnYP, a.EnYP, Ng}, {s.DragXP, a.DragXP, Ng},
-        {s.DragYP, a.DragYP, Ng}, {s.CDXP, a.CDXP, Ng}, {s.CDYP, a.CDYP, Ng}, {s.Fuu, a.Fuu, Ng},
-        {s.Fvv, a.Fvv, Ng}, {s.cellV, a.cellV, N}, {s.CFLx, a.CFLx, Ng}, {s.CFLy, a.CFLy, Ng},
-        {s.wtfXP, a.wtfXP, Ng}, {s.wtfYP, a.wtfYP, Ng}, {s.bottomZ, a.bottomZ, Ng},
-        {s.bottomXP, a.bottomXP, Ng}, {s.bottomYP, a.bottomYP, Ng}};
+    std::vector<std::vector<float>> keep;
+    // r: stored in the state precision
+    struct { const double *src; double *dst; size_t n; bool r; } ng[] = {
+        {s.surf, a.surf, Ng, false}, {s.surfOld, a.surfOld, Ng, false}, {s.uuXP, a.uuXP, Ng, true},
+        {s.vvYP, a.vvYP, Ng, true}, {s.uuYP, a.uuYP, N, true}, {s.vvXP, a.vvXP, N, true},
+        {s.depth, depthCur, N, false}, {s.depthXP, a.depthXP, Ng, true}, {s.depthYP, a.depthYP, Ng, true},
+        {s.EnXP, a.EnXP, Ng, true}, {s.EnYP, a.EnYP, Ng, true}, {s.DragXP, a.DragXP, Ng, true},
+        {s.DragYP, a.DragYP, Ng, true}, {s.CDXP, a.CDXP, Ng, true}, {s.CDYP, a.CDYP, Ng, true},
+        {s.Fuu, a.Fuu, Ng, true}, {s.Fvv, a.Fvv, Ng, true}, {s.cellV, a.cellV, N, false},
+        {s.CFLx, a.CFLx, Ng, true}, {s.CFLy, a.CFLy, Ng, true}, {s.wtfXP, a.wtfXP, Ng, true},
+        {s.wtfYP, a.wtfYP, Ng, true}, {s.bottomZ, a.bottomZ, Ng, false}, {s.bottomXP, a.bottomXP, Ng, false},
+        {s.bottomYP, a.bottomYP, Ng, false}};
     for (auto &e : ng) {
-        int rc = upload(d, e.src, e.dst, e.n * sizeof(double));
+        int rc = e.r ? upload_r(d, e.src, e.dst, e.n, keep) : upload(d, e.src, e.dst, e.n * sizeof(double));
         if (rc) return rc;
     }
     const int ns = d->args.p.nScalar;
+    const size_t rw = state_f32(d->cfg) ? sizeof(float) : sizeof(double);
     for (int q = 0; q < ns; q++) {
         // both concentration buffers: the corners, which no step writes,
         // then agree whichever buffer is current (the current one last)
         if (s.S) {
-            double *cur = sParity ? a.S2 : a.S, *oth = sParity ? a.S : a.S2;
-            int rc = upload(d, s.S + (size_t)q * Ng, oth + (size_t)q * a.sStride, Ng * sizeof(double));
-            if (!rc) rc = upload(d, s.S + (size_t)q * Ng, cur + (size_t)q * a.sStride, Ng * sizeof(double));
+            char *cur = reinterpret_cast<char *>(sParity ? a.S2 : a.S), *oth = reinterpret_cast<char *>(sParity ? a.S : a.S2);
+            const size_t off = (size_t)q * a.sStride * rw;
+            int rc = upload_r(d, s.S + (size_t)q * Ng, reinterpret_cast<double *>(oth + off), Ng, keep);
+            if (!rc) rc = upload_r(d, s.S + (size_t)q * Ng, reinterpret_cast<double *>(cur + off), Ng, keep);
             if (rc) return rc;
         }
         if (s.Sm) { int rc = upload(d, s.Sm + (size_t)q * N, a.Sm + (size_t)q * a.sStride, N * sizeof(double)); if (rc) return rc; }
@@ -704,7 +746,7 @@ int upload_state(frehd_dev *d, const frehd_fields &s, int depthParity = 0, int s
         struct { const double *src; double *dst; } sb[] = {{s.V, a.V}, {s.Z, a.Z}, {s.Vx, a.Vx},
             {s.Vy, a.Vy}, {s.Zx, a.Zx}, {s.Zy, a.Zy}, {s.Nx, a.Nx}, {s.Oy, a.Oy}};
         for (auto &e : sb) {
-            int rc = upload(d, e.src, e.dst, Ng * sizeof(double));
+            int rc = upload_r(d, e.src, e.dst, Ng, keep);
             if (rc) return rc;
         }
         if (s.ind) { int rc = upload(d, s.ind, a.ind, Ng * sizeof(int)); if (rc) return rc; }
@@ -774,7 +816,7 @@ int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
             return fail(FREHD_ENOMEM, "hipMalloc of the fine bathymetry (%zu bytes) failed", nf * sizeof(double));
         rc = upload(d, sg->fine, fine, nf * sizeof(double));
         if (!rc && launch_subgrid_tables(fine, const_cast<double *>(a.tabs), k.nx, k.ny, sg->fineR, k.Ne,
-                                         k.surfmin, k.dsurf, sg->dxf, d->stream))
+                                         k.surfmin, k.dsurf, sg->dxf, state_f32(k), d->stream))
             rc = fail(FREHD_EHIP, "table generator launch failed");
         hipStreamSynchronize(d->stream);
         hipFree(fine);
@@ -786,7 +828,8 @@ int upload_subgrid(frehd_dev *d, const frehd_subgrid *sg)
         return fail(FREHD_ENOMEM, "hipMalloc of the table staging buffer (%zu bytes) failed", tn * sizeof(double));
     for (int t = 0; t < 10 && !rc; t++) {
         rc = upload(d, tabs[t], stage, tn * sizeof(double));
-        if (!rc && launch_tables_reorder(const_cast<double *>(a.tabs), stage, (int)N, k.Ne, t, 1, d->stream))
+        if (!rc && launch_tables_reorder(const_cast<double *>(a.tabs), stage, (int)N, k.Ne, t, 1, state_f32(k),
+                                         d->stream))
             rc = fail(FREHD_EHIP, "table reorder launch failed");
     }
     hipStreamSynchronize(d->stream);
@@ -1145,27 +1188,44 @@ int frehd_dev_download(frehd_dev_t *d, frehd_fields *o)
     const bool newDepth = !c.halted && step_state_new_depth(d->midStep);
     const bool newS = !c.halted && step_state_new_scalar(d->midStep);
     const double *depthCur = (c.parity ^ newDepth) ? a.depthB : a.depthA;
-    const double *Scur = (c.sParity ^ newS) ? a.S2 : a.S;
-    struct { double *dst; const double *src; size_t n; } tab[] = {
-        {o->surf, a.surf, Ng}, {o->surfOld, a.surfOld, Ng}, {o->uuXP, a.uuXP, Ng}, {o->vvYP, a.vvYP, Ng},
-        {o->uuYP, a.uuYP, N}, {o->vvXP, a.vvXP, N}, {o->depth, depthCur, N}, {o->depthXP, a.depthXP, Ng},
-        {o->depthYP, a.depthYP, Ng}, {o->EnXP, a.EnXP, Ng}, {o->EnYP, a.EnYP, Ng}, {o->DragXP, a.DragXP, Ng},
-        {o->DragYP, a.DragYP, Ng}, {o->CDXP, a.CDXP, Ng}, {o->CDYP, a.CDYP, Ng}, {o->Fuu, a.Fuu, Ng},
-        {o->Fvv, a.Fvv, Ng}, {o->cellV, a.cellV, N}, {o->CFLx, a.CFLx, Ng}, {o->CFLy, a.CFLy, Ng},
-        {o->wtfXP, a.wtfXP, Ng}, {o->wtfYP, a.wtfYP, Ng}, {o->bottomZ, a.bottomZ, Ng},
-        {o->bottomXP, a.bottomXP, Ng}, {o->bottomYP, a.bottomYP, Ng}};
-    for (auto &e : tab)
-        if (e.dst) COPYW(d->stream, e.dst, e.src, e.n * sizeof(double), hipMemcpyDeviceToHost);
+    const char *Scur = reinterpret_cast<const char *>((c.sParity ^ newS) ? a.S2 : a.S);
+    // r: stored in the state precision
+    struct { double *dst; const double *src; size_t n; bool r; } tab[] = {
+        {o->surf, a.surf, Ng, false}, {o->surfOld, a.surfOld, Ng, false}, {o->uuXP, a.uuXP, Ng, true},
+        {o->vvYP, a.vvYP, Ng, true}, {o->uuYP, a.uuYP, N, true}, {o->vvXP, a.vvXP, N, true},
+        {o->depth, depthCur, N, false}, {o->depthXP, a.depthXP, Ng, true}, {o->depthYP, a.depthYP, Ng, true},
+        {o->EnXP, a.EnXP, Ng, true}, {o->EnYP, a.EnYP, Ng, true}, {o->DragXP, a.DragXP, Ng, true},
+        {o->DragYP, a.DragYP, Ng, true}, {o->CDXP, a.CDXP, Ng, true}, {o->CDYP, a.CDYP, Ng, true},
+        {o->Fuu, a.Fuu, Ng, true}, {o->Fvv, a.Fvv, Ng, true}, {o->cellV, a.cellV, N, false},
+        {o->CFLx, a.CFLx, Ng, true}, {o->CFLy, a.CFLy, Ng, true}, {o->wtfXP, a.wtfXP, Ng, true},
+        {o->wtfYP, a.wtfYP, Ng, true}, {o->bottomZ, a.bottomZ, Ng, false}, {o->bottomXP, a.bottomXP, Ng, false},
+        {o->bottomYP, a.bottomYP, Ng, false}};
+    for (auto &e : tab) {
+        if (!e.dst) continue;
+        if (e.r) {
+            int rc = download_r(d, e.dst, e.src, e.n);
+            if (rc) return rc;
+        } else {
+            COPYW(d->stream, e.dst, e.src, e.n * sizeof(double), hipMemcpyDeviceToHost);
+        }
+    }
     if (a.p.useSubgrid == 1) {
         struct { double *dst; const double *src; } sb[] = {{o->V, a.V}, {o->Z, a.Z}, {o->Vx, a.Vx},
             {o->Vy, a.Vy}, {o->Zx, a.Zx}, {o->Zy, a.Zy}, {o->Nx, a.Nx}, {o->Oy, a.Oy}};
-        for (auto &e : sb)
-            if (e.dst) COPYW(d->stream, e.dst, e.src, Ng * sizeof(double), hipMemcpyDeviceToHost);
+        for (auto &e : sb) {
+            int rc = download_r(d, e.dst, e.src, Ng);
+            if (rc) return rc;
+        }
         if (o->ind) COPYW(d->stream, o->ind, a.ind, Ng * sizeof(int), hipMemcpyDeviceToHost);
     }
     const int ns = std::max(1, a.p.nScalar);
+    const size_t rw = state_f32(d->cfg) ? sizeof(float) : sizeof(double);
     for (int q = 0; q < ns; q++) {
-        if (o->S) COPYW(d->stream, o->S + (size_t)q * Ng, Scur + (size_t)q * a.sStride, Ng * sizeof(double), hipMemcpyDeviceToHost);
+        if (o->S) {
+            int rc = download_r(d, o->S + (size_t)q * Ng,
+                                reinterpret_cast<const double *>(Scur + (size_t)q * a.sStride * rw), Ng);
+            if (rc) return rc;
+        }
         if (o->Sm) COPYW(d->stream, o->Sm + (size_t)q * N, a.Sm + (size_t)q * a.sStride, N * sizeof(double), hipMemcpyDeviceToHost);
     }
     return 0;
@@ -1321,18 +1381,21 @@ int frehd_dev_download_tables(frehd_dev_t *d, double *const tables[10])
     HIPCHK(hipStreamSynchronize(d->stream));
     const size_t tn = (size_t)d->cfg.Ne * d->g.N;
     const StepArgs &a = d->args;
-    // on the fly: generate the interleaved tables from the pixels first
-    double *gen = nullptr;
+    // on the fly: generate the interleaved tables from the pixels first, in
+    // the precision the handle's lookups round them to
+    const int f32 = state_f32(d->cfg);
+    const size_t tw = f32 ? sizeof(float) : sizeof(double);
+    void *gen = nullptr;
     if (!a.tabs) {
-        if (hipMalloc(&gen, tn * 10 * sizeof(double)) != hipSuccess)
-            return fail(FREHD_ENOMEM, "hipMalloc of the generated tables (%zu bytes) failed", tn * 10 * sizeof(double));
+        if (hipMalloc(&gen, tn * 10 * tw) != hipSuccess)
+            return fail(FREHD_ENOMEM, "hipMalloc of the generated tables (%zu bytes) failed", tn * 10 * tw);
         if (launch_subgrid_tables(a.bed.z, gen, d->cfg.nx, d->cfg.ny, a.bed.r, d->cfg.Ne, d->cfg.surfmin,
-                                  d->cfg.dsurf, a.bed.dxf, d->stream)) {
+                                  d->cfg.dsurf, a.bed.dxf, f32, d->stream)) {
             hipFree(gen);
             return fail(FREHD_EHIP, "table generator launch failed");
         }
     }
-    const double *src = a.tabs ? a.tabs : gen;
+    const void *src = a.tabs ? static_cast<const void *>(a.tabs) : gen;
     double *stage = nullptr;
     if (hipMalloc(&stage, tn * sizeof(double)) != hipSuccess) {
         hipStreamSynchronize(d->stream);
@@ -1342,8 +1405,7 @@ int frehd_dev_download_tables(frehd_dev_t *d, double *const tables[10])
     int rc = 0;
     for (int t = 0; t < 10 && !rc; t++) {
         if (!tables[t]) continue;
-        if (launch_tables_reorder(const_cast<double *>(src), stage, (int)d->g.N, d->cfg.Ne, t, 0,
-                                  d->stream))
+        if (launch_tables_reorder(const_cast<void *>(src), stage, (int)d->g.N, d->cfg.Ne, t, 0, f32, d->stream))
             rc = fail(FREHD_EHIP, "table reorder launch failed");
         else if (hipMemcpyAsync(tables[t], stage, tn * sizeof(double), hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
                  hipStreamSynchronize(d->stream) != hipSuccess)

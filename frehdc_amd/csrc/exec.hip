@@ -93,6 +93,14 @@ int launch(Exec &e, KernelId id, F &&fn)
         if (rc_) return rc_;                                                                          \
     } while (0)
 #define LAUNCH(id, kern, grid, block, ...) LAUNCH_SHM(id, kern, grid, block, 0, __VA_ARGS__)
+// a step kernel in the group's state precision: FP32 storage for the mixed
+// form, FP64 otherwise (the argument blocks are StepArgs in both; the FP32
+// instantiation reads the same pointers as StepArgsT<float>)
+#define LAUNCH_ST(id, kern, grid, block)                                                               \
+    do {                                                                                              \
+        if (e.f32) LAUNCH(id, kern<float>, grid, block, reinterpret_cast<const StepArgsT<float> *>(e.dArgs)); \
+        else LAUNCH(id, kern<double>, grid, block, e.dArgs);                                          \
+    } while (0)
 
 inline dim3 grid1(int nb, int n) { return dim3((unsigned)std::max(nb, 1), 1, (unsigned)n); }
 
@@ -439,8 +447,8 @@ int exec_collect_timing(Exec &e)
 int enqueue_assemble(Exec &e)
 {
     const int n = e.count;
-    if (e.sub) LAUNCH(K_ASM, k_assemble_sub, grid1(e.ext.nbRes, n), dim3(256), e.dArgs);
-    else LAUNCH(K_ASM, k_assemble, grid1(e.ext.nbRes, n), dim3(256), e.dArgs);
+    if (e.sub) LAUNCH_ST(K_ASM, k_assemble_sub, grid1(e.ext.nbRes, n), dim3(256));
+    else LAUNCH_ST(K_ASM, k_assemble, grid1(e.ext.nbRes, n), dim3(256));
     return 0;
 }
 
@@ -463,8 +471,8 @@ static int exec_phase_launch(Exec &e, int phase)
     const dim3 b2(64, 4), g2((unsigned)x.gx2, (unsigned)x.gy2, (unsigned)n), b1(256);
     switch (phase) {
     case 0:   // solveSourceTerm (up to the RHS, which the assembly computes)
-        if (e.sub) LAUNCH(K_PRED, k_predictor_sub, g2, b2, e.dArgs);
-        else LAUNCH(K_PRED, k_predictor, g2, b2, e.dArgs);
+        if (e.sub) LAUNCH_ST(K_PRED, k_predictor_sub, g2, b2);
+        else LAUNCH_ST(K_PRED, k_predictor, g2, b2);
         return 0;
     case 1: { // solveFreeSurface
         int rc = enqueue_assemble(e), k = 0;
@@ -472,31 +480,41 @@ static int exec_phase_launch(Exec &e, int phase)
         if (!rc) rc = gather(e);
         if (!rc) rc = e.mixed ? solve_mixed_finish(e, k) : solve_fp64_finish(e, k);
         if (rc) return rc;
-        LAUNCH(K_SURF, k_surface, grid1(x.nb1, n), b1, e.dArgs);
+        LAUNCH_ST(K_SURF, k_surface, grid1(x.nb1, n), b1);
         return 0;
     }
     case 2: { // updateData (+ the CFL numbers of monitorCFL)
         if (e.sub) {
-            LAUNCH(K_LIM, k_limiter_sub, g2, b2, e.dArgs);
-            LAUNCH(K_FACES, k_combine_faces, grid1(x.nb1, n), b1, e.dArgs);
-            if (x.nbTide > 0) LAUNCH(K_TIDEV, k_tidal_velocity_sub, grid1(x.nbTide, n), b1, e.dArgs);
-            LAUNCH(K_FINISH, k_finish_sub, grid1(x.nb1, n), b1, e.dArgs);
+            LAUNCH_ST(K_LIM, k_limiter_sub, g2, b2);
+            LAUNCH_ST(K_FACES, k_combine_faces, grid1(x.nb1, n), b1);
+            if (x.nbTide > 0) LAUNCH_ST(K_TIDEV, k_tidal_velocity_sub, grid1(x.nbTide, n), b1);
+            LAUNCH_ST(K_FINISH, k_finish_sub, grid1(x.nb1, n), b1);
         } else {
-            LAUNCH(K_LIM, k_limiter, g2, b2, e.dArgs);
-            LAUNCH(K_FACES, k_faces, grid1(x.nb1, n), b1, e.dArgs);
-            if (x.nbTide > 0) LAUNCH(K_TIDEV, k_tidal_velocity, grid1(x.nbTide, n), b1, e.dArgs);
-            LAUNCH(K_FINISH, k_finish, grid1(x.nb1, n), b1, e.dArgs);
+            LAUNCH_ST(K_LIM, k_limiter, g2, b2);
+            LAUNCH_ST(K_FACES, k_faces, grid1(x.nb1, n), b1);
+            if (x.nbTide > 0) LAUNCH_ST(K_TIDEV, k_tidal_velocity, grid1(x.nbTide, n), b1);
+            LAUNCH_ST(K_FINISH, k_finish, grid1(x.nb1, n), b1);
         }
         return 0;
     }
     case 3:   // scalarTransport, every scalar in one launch (members without scalars skip inside)
         if (e.nScalarMax > 0) {
-            if (e.sub) LAUNCH(K_SCALAR, k_scalar_sub, g2, b2, e.dArgs);
-            else LAUNCH(K_SCALAR, k_scalar, g2, b2, e.dArgs);
+            const auto *a32 = reinterpret_cast<const StepArgsT<float> *>(e.dArgs);
+            const int v = (e.sub ? 4 : 0) + (e.f32 ? 2 : 0) + (e.ext.budget ? 1 : 0);
+            switch (v) {
+            case 0: LAUNCH(K_SCALAR, (k_scalar<double, false>), g2, b2, e.dArgs); break;
+            case 1: LAUNCH(K_SCALAR, (k_scalar<double, true>), g2, b2, e.dArgs); break;
+            case 2: LAUNCH(K_SCALAR, (k_scalar<float, false>), g2, b2, a32); break;
+            case 3: LAUNCH(K_SCALAR, (k_scalar<float, true>), g2, b2, a32); break;
+            case 4: LAUNCH(K_SCALAR, (k_scalar_sub<double, false>), g2, b2, e.dArgs); break;
+            case 5: LAUNCH(K_SCALAR, (k_scalar_sub<double, true>), g2, b2, e.dArgs); break;
+            case 6: LAUNCH(K_SCALAR, (k_scalar_sub<float, false>), g2, b2, a32); break;
+            default: LAUNCH(K_SCALAR, (k_scalar_sub<float, true>), g2, b2, a32); break;
+            }
         }
         return 0;
     case 4:   // monitorCFL's stop test
-        LAUNCH(K_END, k_step_end, dim3(1, 1, n), dim3(1024), e.dArgs);
+        LAUNCH_ST(K_END, k_step_end, dim3(1, 1, n), dim3(1024));
         for (auto *m : e.members) m->tt += 1;
         return 0;
     default:
@@ -549,7 +567,7 @@ static int exec_step_eager(Exec &e)
 // enqueue the kernels after the solve (surface, update, scalars, CFL test)
 static int enqueue_tail(Exec &e)
 {
-    LAUNCH(K_SURF, k_surface, grid1(e.ext.nb1, e.count), dim3(256), e.dArgs);
+    LAUNCH_ST(K_SURF, k_surface, grid1(e.ext.nb1, e.count), dim3(256));
     for (int ph = 2; ph <= 4; ph++) {
         int r = exec_phase(e, ph);
         if (r) return r;
@@ -597,8 +615,8 @@ static int exec_step_graphs(Exec &e)
         if (e.heads.size() >= 16) exec_release_graphs(e);
         rc = capture(e, [&] {
             const dim3 b2(64, 4), g2((unsigned)e.ext.gx2, (unsigned)e.ext.gy2, (unsigned)n);
-            if (e.sub) LAUNCH(K_PRED, k_predictor_sub, g2, b2, e.dArgs);
-            else LAUNCH(K_PRED, k_predictor, g2, b2, e.dArgs);
+            if (e.sub) LAUNCH_ST(K_PRED, k_predictor_sub, g2, b2);
+            else LAUNCH_ST(K_PRED, k_predictor, g2, b2);
             int r = enqueue_assemble(e);
             if (!r) r = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
             if (!r) r = enqueue_gather(e);
@@ -654,8 +672,8 @@ int exec_init_subgrid(Exec &e)
 {
     const Extents &x = e.ext;
     const int n = e.count;
-    LAUNCH(K_SUBINIT, k_subarea_init, grid1(x.nbN, n), dim3(256), e.dArgs);
-    LAUNCH(K_COMBINE, k_combine, grid1(x.nbN + x.nbGhost, n), dim3(256), e.dArgs);
+    LAUNCH_ST(K_SUBINIT, k_subarea_init, grid1(x.nbN, n), dim3(256));
+    LAUNCH_ST(K_COMBINE, k_combine, grid1(x.nbN + x.nbGhost, n), dim3(256));
     HIPCHK(hipStreamSynchronize(e.stream));
     return 0;
 }
@@ -676,6 +694,7 @@ Extents member_extents(const frehd_dev_impl *d)
     x.nbRef = mixed ? d->ref.nb : 0;
     x.lanes = mixed ? d->pcg32.lanes : 1;
     x.ssorN = d->pcg.ssor ? d->g.N : 0;
+    x.budget = d->args.p.budget;
     x.ssorLds = !d->pcg.ssor || d->g.N <= d->pcg.ldsCells;
     return x;
 }
@@ -701,6 +720,7 @@ static void merge(Exec &e, const frehd_dev_impl *d, bool first)
     x.nbDir = std::max(x.nbDir, m.nbDir);
     x.nbRef = std::max(x.nbRef, m.nbRef);
     x.ssorN = std::max(x.ssorN, m.ssorN);
+    x.budget = x.budget || m.budget;
     x.ssorLds = x.ssorLds && m.ssorLds;
     e.maxIter = std::max(e.maxIter, d->cfg.maxIter);
     e.maxRefine = std::max(e.maxRefine, d->cfg.maxRefine);
@@ -717,6 +737,7 @@ int exec_refresh(Exec &e)
         if (m == 0) {
             e.sub = d->cfg.useSubgrid == 1;
             e.mixed = d->cfg.precision == FREHD_PREC_MIXED;
+            e.f32 = e.mixed;
         } else if ((d->cfg.useSubgrid == 1) != (bool)e.sub || (d->cfg.precision == FREHD_PREC_MIXED) != (bool)e.mixed) {
             return fail(FREHD_EINVAL, "a group's members must share useSubgrid and precision");
         }

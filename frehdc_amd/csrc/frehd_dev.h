@@ -26,6 +26,41 @@ template <class T> using gptr = __attribute__((address_space(1))) T *;
 template <class T> using gptr = T *;
 #endif
 
+// An array of the explicit state stored in R and used in FP64.  In the device
+// pass an FP32 array is a pointer wrapper whose elements read widened to
+// double and store rounded to float, so every expression on the state
+// computes in FP64 whatever the storage (the difference of two FP32 flow
+// rates is formed in FP64, as the FP64 form forms it); an FP64 array is the
+// plain global pointer.  On the host either is the plain pointer (same size
+// and alignment), which is what the argument blocks carry.
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(FREHD_HOST_SOURCE)
+template <class R>
+struct WidePtr {
+    gptr<R> p;
+    struct Ref {
+        gptr<R> q;
+        __device__ operator double() const { return (double)*q; }
+        __device__ const Ref &operator=(double v) const
+        {
+            *q = (R)v;
+            return *this;
+        }
+        __device__ const Ref &operator=(const Ref &o) const
+        {
+            *q = *o.q;
+            return *this;
+        }
+    };
+    __device__ Ref operator[](size_t i) const { return Ref{p + i}; }
+    __device__ WidePtr operator+(size_t off) const { return WidePtr{p + off}; }
+};
+template <class R> struct StatePtr { using type = WidePtr<R>; };
+template <> struct StatePtr<double> { using type = gptr<double>; };
+template <class R> using sptr = typename StatePtr<R>::type;
+#else
+template <class R> using sptr = R *;
+#endif
+
 struct Grid {
     int nx, ny, N, Ng;
 };

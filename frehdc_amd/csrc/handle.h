@@ -72,6 +72,7 @@ struct Extents {
     int lanes;      // columns per lane of the inner k_cg1 (shared by the group)
     int ssorN;      // largest N of the members preconditioned with SSOR (0: none)
     int ssorLds;    // every SSOR member fits k_ssor's LDS vector
+    int budget;     // a member accounts its conservation budgets
 };
 
 // flags gathered from every member's control block after a solver chunk
@@ -102,6 +103,7 @@ struct Exec {
     int32_t *hFlags = nullptr, *hFlagsDev = nullptr;
     Extents ext{};
     int sub = 0, mixed = 0, nScalarMax = 0, maxIter = 0, maxRefine = 0;
+    int f32 = 0;                        // the members store the explicit state in FP32 (mixed form)
     int lastIters = 16;
     SolveCounts mixPlan;                // inner iterations per sweep of the next mixed solve's pattern
     // the last steps' solver counts (FP64 iterations, or inner iterations

@@ -96,21 +96,34 @@ struct Ctl {
     Budget budget;
 };
 
-struct StepArgs {
+// The explicit phase's state arrays are stored in R: FP64 for the FP64 form,
+// FP32 for the mixed form (which keeps eta, the bed, the cell depths and
+// volumes, the scalar masses, the solver vectors and the subgrid lookup's
+// surface cache in FP64).  Arithmetic is FP64 in both (sptr): an FP32 value
+// read is widened, a result stored is rounded to R.  The host
+// fills StepArgs (R = double): the pointers of both instantiations have
+// the same size and place, so the device reads the same block as
+// StepArgsT<float> when the handle stores FP32.
+template <class R>
+struct StepArgsT {
     Grid g;
     P p;
     gptr<Ctl> ctl;
     // Ng arrays
-    gptr<double> surf, surfOld, uuXP, vvYP, depthXP, depthYP, EnXP, EnYP, DragXP, DragYP;
-    gptr<double> CDXP, CDYP, Fuu, Fvv, CFLx, CFLy, wtfXP, wtfYP, bottomZ, bottomXP, bottomYP;
+    gptr<double> surf, surfOld;
+    sptr<R> uuXP, vvYP, depthXP, depthYP, EnXP, EnYP, DragXP, DragYP;
+    sptr<R> CDXP, CDYP, Fuu, Fvv, CFLx, CFLy, wtfXP, wtfYP;
+    gptr<double> bottomZ, bottomXP, bottomYP;
     // nScalar arrays with stride sStride; the concentration is
     // double-buffered (S, S2; Ctl.sParity selects the step's start), the
     // scalar mass Sm is not
-    gptr<double> S, S2, Sm;
+    sptr<R> S, S2;
+    gptr<double> Sm;
     int64_t sStride;
     // interior arrays; the depth is double-buffered: Ctl.parity selects the
     // buffer holding the depth at the start of the step
-    gptr<double> uuYP, vvXP, depthA, depthB, cellV, manning;
+    sptr<R> uuYP, vvXP;
+    gptr<double> depthA, depthB, cellV, manning;
     gptr<uint8_t> rmv, flags;      // flags: bit0 tideP cell, bit1 tideM cell, bits 2-7 inflow count
     // operator and solver vectors
     gptr<double> cX, cY, D, b, x, r, q, pA, pB;
@@ -140,16 +153,17 @@ struct StepArgs {
     // tabs[(c*Ne + k)*10 + T], so the two levels a cell interpolates between
     // are 160 contiguous bytes (the reference's files are level-major,
     // [k*N + c], and are reordered on upload)
-    gptr<double> V, Z, Vx, Vy, Zx, Zy, Nx, Oy;
-    gptr<double> Vxp, Vxm, Vyp, Vym, Np, Nm, Op, Om;
+    sptr<R> V, Z, Vx, Vy, Zx, Zy, Nx, Oy;
+    sptr<R> Vxp, Vxm, Vyp, Vym, Np, Nm, Op, Om;
     // computeSubArea's own V and Z (before combineSubArea clips them in
     // place), and the inputs it last saw per cell (surface, dry-start flag):
     // a cell whose inputs are bitwise unchanged keeps its outputs
-    gptr<double> Vraw, Zraw, hSub;
+    sptr<R> Vraw, Zraw;
+    gptr<double> hSub;
     gptr<uint8_t> dSub;
     gptr<int> ind;
     gptr<const double> edgeX, edgeY, allSurf;
-    gptr<const double> tabs;
+    gptr<const R> tabs;            // in the state precision (FP32 for the mixed form)
     // drag-model tables (level-major [k*N + c]) and the values at the lower
     // bracketing level of each cell (useSubDrag 2: Yh; 3, 4: Cn)
     gptr<const double> allYhX, allYhY, allCn;
@@ -162,6 +176,8 @@ struct StepArgs {
     // evaluates its two levels (cell_levels, bitwise the table values)
     FineBed bed;
 };
+using StepArgs = StepArgsT<double>;
+
 
 // value T of cell c at level k: levels outermost, the ten values of a cell
 // adjacent, cells in order within a level.  A wave's cells sit at about the
@@ -250,35 +266,40 @@ struct RefineArgs {
 __global__ void k_refine(const RefineArgs *__restrict__ A, int s);
 __global__ void k_refine_test(const RefineArgs *__restrict__ A, int s);
 
-__device__ inline gptr<const double> depth_cur(const StepArgs &a) { return a.ctl->parity ? a.depthB : a.depthA; }
-__device__ inline gptr<double> depth_nxt(const StepArgs &a) { return a.ctl->parity ? a.depthA : a.depthB; }
+template <class R>
+__device__ inline gptr<const double> depth_cur(const StepArgsT<R> &a) { return a.ctl->parity ? a.depthB : a.depthA; }
+template <class R>
+__device__ inline gptr<double> depth_nxt(const StepArgsT<R> &a) { return a.ctl->parity ? a.depthA : a.depthB; }
 // the kernels after the solve run only for a member that is not halted,
 // whose solve has finished and whose step they have not completed yet (a
 // group launches them again when one member's solve needed more iterations)
-__device__ inline bool tail_off(const StepArgs &a) { return a.ctl->halted || a.ctl->open || a.ctl->stepDone; }
-__device__ inline gptr<const double> s_cur(const StepArgs &a) { return a.ctl->sParity ? a.S2 : a.S; }
-__device__ inline gptr<double> s_nxt(const StepArgs &a) { return a.ctl->sParity ? a.S : a.S2; }
+template <class R>
+__device__ inline bool tail_off(const StepArgsT<R> &a) { return a.ctl->halted || a.ctl->open || a.ctl->stepDone; }
+template <class R>
+__device__ inline sptr<R> s_cur(const StepArgsT<R> &a) { return a.ctl->sParity ? a.S2 : a.S; }
+template <class R>
+__device__ inline sptr<R> s_nxt(const StepArgsT<R> &a) { return a.ctl->sParity ? a.S : a.S2; }
 
 enum SubTable { T_V, T_Z, T_VXP, T_VXM, T_VYP, T_VYM, T_NP, T_NM, T_OP, T_OM };
 
-__global__ void k_predictor(const StepArgs *__restrict__ A);
-__global__ void k_assemble(const StepArgs *__restrict__ A);
-__global__ void k_surface(const StepArgs *__restrict__ A);
-__global__ void k_limiter(const StepArgs *__restrict__ A);
-__global__ void k_faces(const StepArgs *__restrict__ A);
-__global__ void k_tidal_velocity(const StepArgs *__restrict__ A);
-__global__ void k_finish(const StepArgs *__restrict__ A);
-__global__ void k_scalar(const StepArgs *__restrict__ A);
-__global__ void k_step_end(const StepArgs *__restrict__ A);
+template <class R> __global__ void k_predictor(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_assemble(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_surface(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_limiter(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_faces(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_tidal_velocity(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_finish(const StepArgsT<R> *__restrict__ A);
+template <class R, bool BUDGET> __global__ void k_scalar(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_step_end(const StepArgsT<R> *__restrict__ A);
 // subgrid variants
-__global__ void k_predictor_sub(const StepArgs *__restrict__ A);
-__global__ void k_assemble_sub(const StepArgs *__restrict__ A);
-__global__ void k_subarea_init(const StepArgs *__restrict__ A);
-__global__ void k_limiter_sub(const StepArgs *__restrict__ A);
-__global__ void k_combine(const StepArgs *__restrict__ A);
-__global__ void k_combine_faces(const StepArgs *__restrict__ A);
-__global__ void k_tidal_velocity_sub(const StepArgs *__restrict__ A);
-__global__ void k_finish_sub(const StepArgs *__restrict__ A);
-__global__ void k_scalar_sub(const StepArgs *__restrict__ A);
+template <class R> __global__ void k_predictor_sub(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_assemble_sub(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_subarea_init(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_limiter_sub(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_combine(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_combine_faces(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_tidal_velocity_sub(const StepArgsT<R> *__restrict__ A);
+template <class R> __global__ void k_finish_sub(const StepArgsT<R> *__restrict__ A);
+template <class R, bool BUDGET> __global__ void k_scalar_sub(const StepArgsT<R> *__restrict__ A);
 
 }  // namespace frehd

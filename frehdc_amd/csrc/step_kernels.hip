@@ -11,14 +11,16 @@
 
 namespace frehd {
 
-__device__ inline int cur_tt(const StepArgs &a) { return a.ctl->tt; }
+template <class R>
+__device__ inline int cur_tt(const StepArgsT<R> &a) { return a.ctl->tt; }
 
 // ---------------------------------------------------------------------------
 // Step begin: tt += 1 (SolveAll's loop counter, nssolve.c:61) and the solver
 // state reset; done by the predictor's first thread (the rest of the
 // predictor reads the step index from ttNext, not tt), so a step costs no
 // launch for it.
-__device__ inline void step_begin(const StepArgs &a)
+template <class R>
+__device__ inline void step_begin(const StepArgsT<R> &a)
 {
     Ctl *c = a.ctl;
     if (c->halted) return;
@@ -57,8 +59,8 @@ __device__ inline void limit_y(double &aY, double cflYK, double cflXK, double fY
     else if (cflXK > hi) aY = 0;   // tests CFLx, nsfunctions.c:574
 }
 
-template <bool SUB>
-__device__ inline void predictor_body(const StepArgs &a)
+template <class R, bool SUB>
+__device__ inline void predictor_body(const StepArgsT<R> &a)
 {
     if (a.ctl->halted) return;
     const Grid g = a.g;
@@ -69,7 +71,8 @@ __device__ inline void predictor_body(const StepArgs &a)
     const int e = nbE(g, c, i, j), w = nbW(g, c, i, j), n = nbN(g, c, i, j), s = nbS(g, c, i, j);
     const P &p = a.p;
     const double dx = p.dx, dy = p.dy, dt = p.dt;
-    const double *u = a.uuXP, *v = a.vvYP;
+    const auto u = a.uuXP;
+    const auto v = a.vvYP;
     const double uc = u[c], ue = u[e], uw = u[w], un = u[n], us = u[s];
     const double vc = v[c], ve = v[e], vw = v[w], vn = v[n], vs = v[s];
     const double vx = a.vvXP[c], uy = a.uuYP[c];
@@ -163,19 +166,21 @@ __device__ inline void predictor_body(const StepArgs &a)
     if (i == 0) { a.DragYP[giM(g, j)] = DragY; a.EnYP[giM(g, j)] = EnY; }
 }
 
-__global__ void __launch_bounds__(256) k_predictor(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_predictor(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) step_begin(a);
-    predictor_body<false>(a);
+    predictor_body<R, false>(a);
 }
-__global__ void __launch_bounds__(256) k_predictor_sub(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_predictor_sub(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) step_begin(a);
-    predictor_body<true>(a);
+    predictor_body<R, true>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -195,7 +200,8 @@ __device__ inline double face_coef(double coef, double depthF, double dragF)
 
 // stores the operator row and the solver start of cell c, accumulating b.b
 // and r.z (z = r / D) of the thread's cells
-__device__ inline void store_row(const StepArgs &a, int c, double GXP, double GYP, double D, double z,
+template <class R>
+__device__ inline void store_row(const StepArgsT<R> &a, int c, double GXP, double GYP, double D, double z,
                                  double (&part)[3])
 {
     a.cX[c] = GXP;
@@ -217,9 +223,10 @@ __device__ inline void store_row(const StepArgs &a, int c, double GXP, double GY
     part[1] += z * (z * invd);
 }
 
-__global__ void __launch_bounds__(256) k_assemble(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_assemble(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.nbRes) return;
     __shared__ double lds[16 * 3];
     if (a.ctl->halted) return;
@@ -254,16 +261,18 @@ __global__ void __launch_bounds__(256) k_assemble(const StepArgs *__restrict__ A
 }
 
 
-__device__ inline void surface_cell(const StepArgs &a, int k, double &loss);
+template <class R>
+__device__ inline void surface_cell(const StepArgsT<R> &a, int k, double &loss);
 
 // ---------------------------------------------------------------------------
 // Surface update: getFreeSurface + zeroNegSurf + enforceFreeSurfaceBC +
 // enforceTidalBC (nsfunctions.c:1020-1031, 436-449, 317-334, 412-433), over
 // all Ng indices.  A ghost takes the zero-clipped solution of the interior
 // cell it mirrors (the tidal override comes after the ghost fill).
-__global__ void __launch_bounds__(256) k_surface(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_surface(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
     if (tail_off(a)) return;
@@ -275,7 +284,8 @@ __global__ void __launch_bounds__(256) k_surface(const StepArgs *__restrict__ A)
     if (threadIdx.x == 0) a.vpart[2 * a.maxBlocks + blockIdx.x] = loss[0];
 }
 
-__device__ inline void surface_cell(const StepArgs &a, int k, double &loss)
+template <class R>
+__device__ inline void surface_cell(const StepArgsT<R> &a, int k, double &loss)
 {
     const Grid &g = a.g;
     const P &p = a.p;
@@ -310,7 +320,8 @@ __device__ inline void surface_cell(const StepArgs &a, int k, double &loss)
 // depth at a ghost index is 0 (the reference reads past the N-array).
 // limiterCFL for cell c (both sweeps); returns the new surface, sets rmv and
 // adds the Vloss1/Vloss2 contributions
-__device__ inline double limiter_cell(const StepArgs &a, int c, int i, int j, double (&loss)[2], uint8_t &rmv)
+template <class R>
+__device__ inline double limiter_cell(const StepArgsT<R> &a, int c, int i, int j, double (&loss)[2], uint8_t &rmv)
 {
     const Grid &g = a.g;
     const P &p = a.p;
@@ -341,7 +352,8 @@ __device__ inline double limiter_cell(const StepArgs &a, int c, int i, int j, do
     return h;
 }
 
-__device__ inline void store_vloss2(const StepArgs &a, double (&loss)[2], double *lds)
+template <class R>
+__device__ inline void store_vloss2(const StepArgsT<R> &a, double (&loss)[2], double *lds)
 {
     block_sum<2>(loss, lds);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
@@ -351,9 +363,10 @@ __device__ inline void store_vloss2(const StepArgs &a, double (&loss)[2], double
     }
 }
 
-__global__ void __launch_bounds__(256) k_limiter(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_limiter(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
     __shared__ double lds[16 * 2];
     if (tail_off(a)) return;
@@ -388,7 +401,8 @@ __global__ void __launch_bounds__(256) k_limiter(const StepArgs *__restrict__ A)
 // the cell itself holds.  The rmvCFL zeroing of a cell's own and its west /
 // south faces becomes: a face is zeroed when either adjacent interior cell
 // has rmvCFL == 1.
-__device__ inline double face_depth_x(const StepArgs &a, int k)
+template <class R>
+__device__ inline double face_depth_x(const StepArgsT<R> &a, int k)
 {
     // depthXP per updateAllFaceDepth (irank = 0 = nrank - 1), before the
     // final negative clip
@@ -409,7 +423,8 @@ __device__ inline double face_depth_x(const StepArgs &a, int k)
     return 0.0;                                      // iP / iM ghosts, corners
 }
 
-__device__ inline double face_depth_y(const StepArgs &a, int k)
+template <class R>
+__device__ inline double face_depth_y(const StepArgsT<R> &a, int k)
 {
     const Grid &g = a.g;
     const double *h = a.surf, *bY = a.bottomYP;
@@ -446,11 +461,13 @@ __device__ inline double face_depth_y(const StepArgs &a, int k)
     return 0.0;
 }
 
-__device__ inline void faces_cell(const StepArgs &a, int k, double &loss);
+template <class R>
+__device__ inline void faces_cell(const StepArgsT<R> &a, int k, double &loss);
 
-__global__ void __launch_bounds__(256) k_faces(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_faces(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
     if (tail_off(a)) return;
@@ -468,7 +485,8 @@ struct FaceVel {
     double u, v, fX, fY, loss;
 };
 
-__device__ inline FaceVel face_vel(const StepArgs &a, int c, double dX, double dY)
+template <class R>
+__device__ inline FaceVel face_vel(const StepArgsT<R> &a, int c, double dX, double dY)
 {
     const Grid &g = a.g;
     const P &p = a.p;
@@ -517,7 +535,8 @@ __device__ inline double clipped(double d) { return d < 0 ? 0 : d; }
 // kernel.  The rmvCFL pass also zeroes the iM-ghost face of a left-column
 // cell and the jM-ghost face of a bottom-row cell; enforceVelocityBC
 // overwrites those ghosts right after, so nothing is kept of it.
-__device__ inline void faces_cell(const StepArgs &a, int k, double &loss)
+template <class R>
+__device__ inline void faces_cell(const StepArgsT<R> &a, int k, double &loss)
 {
     const Grid &g = a.g;
     const P &p = a.p;
@@ -554,9 +573,10 @@ __device__ inline void faces_cell(const StepArgs &a, int k, double &loss)
 
 // adjustTidalVelocity (nsfunctions.c:1034-1078): one thread per tidal cell.
 // Fvv is not recomputed afterwards (the reference keeps the old flux).
-__global__ void k_tidal_velocity(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void k_tidal_velocity(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.nbTide) return;
     if (tail_off(a)) return;
     const Grid g = a.g;
@@ -593,7 +613,8 @@ struct VolBudget {
     double vol, bnd, in, clip, moved;
 };
 
-__device__ inline void volume_update(const StepArgs &a, int c, int i, int j, int tt, bool inflowOn, VolBudget &b)
+template <class R>
+__device__ inline void volume_update(const StepArgsT<R> &a, int c, int i, int j, int tt, bool inflowOn, VolBudget &b)
 {
     const Grid &g = a.g;
     const P &p = a.p;
@@ -629,7 +650,8 @@ __device__ inline void volume_update(const StepArgs &a, int c, int i, int j, int
 }
 
 // the block's volume-budget partials (budget slots 0..6)
-__device__ inline void store_vol_budget(const StepArgs &a, VolBudget &b, double *lds)
+template <class R>
+__device__ inline void store_vol_budget(const StepArgsT<R> &a, VolBudget &b, double *lds)
 {
     const DD r = block_dd_sum(b.res, lds);
     double v[5] = {b.vol, b.bnd, b.in, b.clip, b.moved};
@@ -647,9 +669,10 @@ __device__ inline void store_vol_budget(const StepArgs &a, VolBudget &b, double 
 // (nsfunctions.c:337-347, 373-409, 1170-1189, 691-727, 1192-1232).
 // velocityInterp uses flat-index neighbours (row wrap, ghost reads); indices
 // below 0 read 0.
-__global__ void __launch_bounds__(256) k_finish(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_finish(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16 * 5];
     if (tail_off(a)) return;
@@ -661,7 +684,8 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgs *__restrict__ A)
     if (c < g.N) {
         const int tt = cur_tt(a);
         const int nx = g.nx, i = c % nx, j = c / nx;
-        const double *u = a.uuXP, *v = a.vvYP;
+        const auto u = a.uuXP;
+        const auto v = a.vvYP;
         const double um1 = c - 1 >= 0 ? u[c - 1] : 0.0;
         const double um1n = c - 1 + nx >= 0 ? u[c - 1 + nx] : 0.0;
         const double vmn = c - nx >= 0 ? v[c - nx] : 0.0;
@@ -754,8 +778,8 @@ struct MassBudget {
     double mass, bnd, in, moved;
 };
 
-template <bool SUB>
-__device__ inline void scalar_body(const StepArgs &a, double *lds)
+template <class R, bool SUB, bool BUDGET>
+__device__ inline void scalar_body(const StepArgsT<R> &a, double *lds)
 {
     if (tail_off(a) || a.p.nScalar < 1) return;
     const Grid g = a.g;
@@ -769,13 +793,15 @@ __device__ inline void scalar_body(const StepArgs &a, double *lds)
     const int c = valid ? j * g.nx + i : 0;
     const int e = valid ? nbE(g, c, i, j) : 0, w = valid ? nbW(g, c, i, j) : 0;
     const int n = valid ? nbN(g, c, i, j) : 0, s = valid ? nbS(g, c, i, j) : 0;
-    const double *Fu = a.Fuu, *Fv = a.Fvv;
+    const auto Fu = a.Fuu;
+    const auto Fv = a.Fvv;
     const double dt = p.dt, dx = p.dx, dy = p.dy;
     // geometry of the cell, the same for every scalar: flow rates, face
     // areas or depths, the connectivity with the previous-step surface
     // (scalar.c:292-341; the subgrid tests face areas, subgrid.c:1588-1630)
     const double *ho = a.surfOld, *zb = a.bottomZ;
-    const double *fx = SUB ? a.Nx : a.depthXP, *fy = SUB ? a.Oy : a.depthYP;
+    const auto fx = SUB ? a.Nx : a.depthXP;
+    const auto fy = SUB ? a.Oy : a.depthYP;
     double Fuc = 0, Fuw = 0, Fvc = 0, Fvs = 0, fxc = 0, fxw = 0, fyc = 0, fys = 0, V = 0;
     bool cE = false, cW = false, cN = false, cS = false;
     int ninflow = 0;
@@ -799,7 +825,7 @@ __device__ inline void scalar_body(const StepArgs &a, double *lds)
     for (int q = 0; q < p.nScalar; q++) {
         MassBudget mb{{0.0, 0.0}, 0.0, 0.0, 0.0, 0.0};
         if (valid) {
-        const double *S = s_cur(a) + (size_t)q * a.sStride;
+        const auto S = s_cur(a) + (size_t)q * a.sStride;
         double *Sm = a.Sm + (size_t)q * a.sStride;
         const double Sc = S[c], Se = S[e], Sw = S[w], Sn = S[n], Ss = S[s];
         const double smOld = Sm[c];
@@ -809,7 +835,8 @@ __device__ inline void scalar_body(const StepArgs &a, double *lds)
             if (p.scalarAdv == 2) {
                 // TVD superbee (scalar.c:99-255)
                 const int ee = nbEE(g, c, i, j), ww = nbWW(g, c, i, j), nn = nbNN(g, c, i, j), ss = nbSS(g, c, i, j);
-                const double *u = a.uuXP, *v = a.vvYP;
+                const auto u = a.uuXP;
+                const auto v = a.vvYP;
                 Sd = tvd_face(Sc, Se, Sw, S[ee], Fuc, Fuw, Fu[e], u[c] * dt / dx);
                 Su = tvd_face(Sw, Sc, S[ww], Se, Fuw, Fu[ww], Fuc, u[w] * dt / dx);
                 Sr = tvd_face(Sc, Sn, Ss, S[nn], Fvc, Fvs, Fv[n], v[c] * dt / dy);
@@ -847,7 +874,7 @@ __device__ inline void scalar_body(const StepArgs &a, double *lds)
         }
         for (int t = 0; t < ninflow; t++) sm = sm + p.inflowS * a.inflow[tt] * dt / p.inflowLocLength;
         Sm[c] = sm;
-        if (p.budget) {
+        if (BUDGET && p.budget) {
             double bf = 0.0;
             if (i == 0) bf += Fuw * Su - kW * (Sc - Sw);
             if (i == g.nx - 1) bf += -Fuc * Sd + kE * (Se - Sc);
@@ -874,7 +901,7 @@ __device__ inline void scalar_body(const StepArgs &a, double *lds)
                 depth_nxt(a)[c] = 0;
             }
         }
-        double *Snx = s_nxt(a) + (size_t)q * a.sStride;
+        const auto Snx = s_nxt(a) + (size_t)q * a.sStride;
         if (j == g.ny - 1) Snx[gjP(g, i)] = Sv;
         if (j == 0) Snx[gjM(g, i)] = Sv;
         if (i == g.nx - 1) Snx[giP(g, j)] = Sv;
@@ -884,8 +911,8 @@ __device__ inline void scalar_body(const StepArgs &a, double *lds)
         if (depth_nxt(a)[c] <= 0) Sv = 0;
         Snx[c] = Sv;
         }
-        if (p.budget) {
-            // the block's partials of this scalar (slots 6 + 5q ...)
+        if (BUDGET && p.budget) {
+            // the block's partials of this scalar (kBudgetVolSlots + kBudgetScalarSlots q ...)
             const DD r = block_dd_sum(mb.res, lds);
             double v[4] = {mb.mass, mb.bnd, mb.in, mb.moved};
             block_sum<4>(v, lds);
@@ -901,25 +928,30 @@ __device__ inline void scalar_body(const StepArgs &a, double *lds)
 }
 
 // every scalar of a member in one launch (the geometry is read once)
-__global__ void __launch_bounds__(256) k_scalar(const StepArgs *__restrict__ A)
+// (the mass budget's block sums compiled only into the variant a group with
+// a budgeting member launches)
+template <class R, bool BUDGET>
+__global__ void __launch_bounds__(256) k_scalar(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
     __shared__ double lds[16 * 4];
-    scalar_body<false>(a, lds);
+    scalar_body<R, false, BUDGET>(a, lds);
 }
-__global__ void __launch_bounds__(256) k_scalar_sub(const StepArgs *__restrict__ A)
+template <class R, bool BUDGET>
+__global__ void __launch_bounds__(256) k_scalar_sub(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
     __shared__ double lds[16 * 4];
-    scalar_body<true>(a, lds);
+    scalar_body<R, true, BUDGET>(a, lds);
 }
 
 // the conservation budgets of the step: the blocks' partials summed in
 // block order (double-double for the residuals), the step's residuals and
 // totals added to the member's accumulators (thread 0)
-__device__ inline void step_end_budget(const StepArgs &a)
+template <class R>
+__device__ inline void step_end_budget(const StepArgsT<R> &a)
 {
     __shared__ double lds[16 * 5];
     const int t = threadIdx.x, nt = blockDim.x, mb = a.maxBlocks;
@@ -970,9 +1002,10 @@ __device__ inline void step_end_budget(const StepArgs &a)
 // Step end: volume-loss and CFL reductions (block partials in block order),
 // and the max CFL test of monitorCFL (nsfunctions.c:1226-1231), which halts
 // the handle instead of exiting the process.  One block of 1024 threads.
-__global__ void __launch_bounds__(1024) k_step_end(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(1024) k_step_end(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     __shared__ double lds[16 * 4];
     Ctl *c = a.ctl;
     if (tail_off(a)) return;
@@ -1053,7 +1086,8 @@ struct SubPre {
     uint8_t dSub;
 };
 
-__device__ inline SubPre subarea_pre(const StepArgs &a, int c, int i, int j)
+template <class R>
+__device__ inline SubPre subarea_pre(const StepArgsT<R> &a, int c, int i, int j)
 {
     const Grid &g = a.g;
     const int N = g.N;
@@ -1076,7 +1110,8 @@ __device__ inline SubPre subarea_pre(const StepArgs &a, int c, int i, int j)
 // computeSubArea for cell c (subgrid.c:55-251): walk the level index to the
 // surface, interpolate the half-cell volumes and face areas between the two
 // bracketing levels, zero what is dry.
-__device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, double h, double depthOld,
+template <class R>
+__device__ inline void subarea_cell(const StepArgsT<R> &a, int c, int i, int j, double h, double depthOld,
                                     const SubPre &q)
 {
     // the outputs are a function of (h, depthOld == 0) and constant inputs,
@@ -1111,9 +1146,11 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
     const double s1 = as[j1], s2 = as[j2];
     // the two levels' 10 values each: adjacent in the interleaved tables, or
     // evaluated from the cell's pixels
+    // (tables in the state precision: evaluated values are rounded to it,
+    // so both hold the same numbers)
     double t1[10], t2[10];
     if (a.tabs) {
-        const double *p1 = a.tabs + tab_index(c, j1, N, 0), *p2 = a.tabs + tab_index(c, j2, N, 0);
+        const auto p1 = a.tabs + tab_index(c, j1, N, 0), p2 = a.tabs + tab_index(c, j2, N, 0);
 #pragma unroll
         for (int q = 0; q < 10; q++) {
             t1[q] = p1[q];
@@ -1125,8 +1162,8 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
         cell_levels<2>(a.bed, i, j, hl, v);
 #pragma unroll
         for (int q = 0; q < 10; q++) {
-            t1[q] = v[0][q];
-            t2[q] = v[1][q];
+            t1[q] = (R)v[0][q];
+            t2[q] = (R)v[1][q];
         }
     }
     a.Vraw[c] = interp_surf(h, s1, t1[T_V], s2, t2[T_V]);
@@ -1174,9 +1211,10 @@ __device__ inline void subarea_cell(const StepArgs &a, int c, int i, int j, doub
 }
 
 // Init: computeSubArea on the initial surface (initialize.c:107)
-__global__ void __launch_bounds__(256) k_subarea_init(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_subarea_init(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.nbN) return;
     const Grid g = a.g;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1187,9 +1225,10 @@ __global__ void __launch_bounds__(256) k_subarea_init(const StepArgs *__restrict
 
 // limiterCFL + computeVolumeSub + computeSubArea + updateDepth
 // (nsfunctions.c:452-495, subgrid.c:1215-1224, 55-251, nsfunctions.c:239-247)
-__global__ void __launch_bounds__(256) k_limiter_sub(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_limiter_sub(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
     __shared__ double lds[16 * 2];
     if (tail_off(a)) return;
@@ -1259,7 +1298,8 @@ __device__ inline double volume_lambda(double r, double cap)
 // the waterfall halving and before the small-value clip; the reference's
 // local beta is 0, so useminA takes its first branch (pull Nx, Oy above
 // Nmax, Omax back by Config.beta)
-__device__ inline void min_adjust(const StepArgs &a, int c, int e, int n, double &Nx, double &Oy, double &Vx,
+template <class R>
+__device__ inline void min_adjust(const StepArgsT<R> &a, int c, int e, int n, double &Nx, double &Oy, double &Vx,
                                   double &Vy)
 {
     const P &p = a.p;
@@ -1296,7 +1336,8 @@ __device__ inline void min_adjust(const StepArgs &a, int c, int e, int n, double
     }
 }
 
-__device__ inline Combined combine_cell(const StepArgs &a, int c)
+template <class R>
+__device__ inline Combined combine_cell(const StepArgsT<R> &a, int c)
 {
     const Grid &g = a.g;
     const P &p = a.p;
@@ -1355,9 +1396,10 @@ __device__ inline Combined combine_cell(const StepArgs &a, int c)
 // each ghost recomputing the cell it mirrors.  The iM/iP ghosts of the top
 // row take the corner cell's Oy before the top-row closure (the closure
 // runs after the ghost copies, subgrid.c:508-524).
-__global__ void __launch_bounds__(256) k_combine(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_combine(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.nbN + a.nbGhost) return;
     if (tail_off(a)) return;
     const Grid g = a.g;
@@ -1396,7 +1438,8 @@ __global__ void __launch_bounds__(256) k_combine(const StepArgs *__restrict__ A)
 // north faces (with the waterfall flags), given the cell's combined face
 // values v; a pure function of the step's inputs, so a ghost face can
 // recompute the face it mirrors
-__device__ inline FaceVel face_vel_sub(const StepArgs &a, int c, const Combined &cv)
+template <class R>
+__device__ inline FaceVel face_vel_sub(const StepArgsT<R> &a, int c, const Combined &cv)
 {
     const Grid &g = a.g;
     const P &p = a.p;
@@ -1459,7 +1502,8 @@ __device__ inline FaceVel face_vel_sub(const StepArgs &a, int c, const Combined 
 // cell need only that cell's combined values, so one pass does both; a
 // ghost face takes the velocity of the face it mirrors (enforceVelocityBC),
 // recomputed here from its source cell's combined values.
-__device__ inline void combine_faces_cell(const StepArgs &a, int k, double &loss)
+template <class R>
+__device__ inline void combine_faces_cell(const StepArgsT<R> &a, int k, double &loss)
 {
     const Grid &g = a.g;
     const P &p = a.p;
@@ -1529,9 +1573,10 @@ __device__ inline void combine_faces_cell(const StepArgs &a, int k, double &loss
 // Interior flags are recomputed every step (the Sub variant resets them
 // before detecting); the ghost flags written by the XM/YM tests of boundary
 // cells stay set, as in the reference.
-__global__ void __launch_bounds__(256) k_combine_faces(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_combine_faces(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
     if (tail_off(a)) return;
@@ -1543,9 +1588,10 @@ __global__ void __launch_bounds__(256) k_combine_faces(const StepArgs *__restric
 }
 
 // adjustTidalVelocitySub (subgrid.c:1291-1332)
-__global__ void k_tidal_velocity_sub(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void k_tidal_velocity_sub(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.nbTide) return;
     if (tail_off(a)) return;
     const Grid g = a.g;
@@ -1572,9 +1618,10 @@ __global__ void k_tidal_velocity_sub(const StepArgs *__restrict__ A)
 // 2: Volp; 3, 4: Casas)
 // + thinLayerDrag + CFL numbers (nsfunctions.c:337-347, subgrid.c:1227-1263,
 // 1520-1535, nsfunctions.c:691-727, 1192-1232)
-__global__ void __launch_bounds__(256) k_finish_sub(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_finish_sub(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16 * 5];
     if (tail_off(a)) return;
@@ -1586,7 +1633,8 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgs *__restrict__
     if (c < g.N) {
         const int tt = cur_tt(a);
         const int nx = g.nx, i = c % nx, j = c / nx;
-        const double *u = a.uuXP, *v = a.vvYP;
+        const auto u = a.uuXP;
+        const auto v = a.vvYP;
         const double um1 = c - 1 >= 0 ? u[c - 1] : 0.0;
         const double um1n = c - 1 + nx >= 0 ? u[c - 1 + nx] : 0.0;
         const double vmn = c - nx >= 0 ? v[c - nx] : 0.0;
@@ -1655,9 +1703,10 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgs *__restrict__
 // N = 0, so the wall adjustment subtracts zeros.  GnCt equal to 0 or dx*dy
 // becomes dx*dy; a dry cell with no velocity on its four faces gets the
 // identity-like row dx*dy * x = dx*dy * eta (flagged by a negative D).
-__global__ void __launch_bounds__(256) k_assemble_sub(const StepArgs *__restrict__ A)
+template <class R>
+__global__ void __launch_bounds__(256) k_assemble_sub(const StepArgsT<R> *__restrict__ A)
 {
-    const StepArgs &a = A[blockIdx.z];
+    const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.nbRes) return;
     __shared__ double lds[16 * 3];
     if (a.ctl->halted) return;
@@ -1698,5 +1747,48 @@ __global__ void __launch_bounds__(256) k_assemble_sub(const StepArgs *__restrict
     block_sum<3>(part, lds);
     store_partials<2>(a.partA, a.nbRes, {part[0], part[1]});
 }
+
+
+// both storage precisions of every step kernel
+template __global__ void k_assemble<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_assemble<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_assemble_sub<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_assemble_sub<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_combine<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_combine<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_combine_faces<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_combine_faces<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_faces<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_faces<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_finish<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_finish<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_finish_sub<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_finish_sub<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_limiter<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_limiter<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_limiter_sub<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_limiter_sub<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_predictor<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_predictor<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_predictor_sub<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_predictor_sub<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_scalar<double, false>(const StepArgsT<double> *__restrict__);
+template __global__ void k_scalar<double, true>(const StepArgsT<double> *__restrict__);
+template __global__ void k_scalar<float, false>(const StepArgsT<float> *__restrict__);
+template __global__ void k_scalar<float, true>(const StepArgsT<float> *__restrict__);
+template __global__ void k_scalar_sub<double, false>(const StepArgsT<double> *__restrict__);
+template __global__ void k_scalar_sub<double, true>(const StepArgsT<double> *__restrict__);
+template __global__ void k_scalar_sub<float, false>(const StepArgsT<float> *__restrict__);
+template __global__ void k_scalar_sub<float, true>(const StepArgsT<float> *__restrict__);
+template __global__ void k_step_end<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_step_end<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_subarea_init<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_subarea_init<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_surface<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_surface<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_tidal_velocity<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_tidal_velocity<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_tidal_velocity_sub<double>(const StepArgsT<double> *__restrict__);
+template __global__ void k_tidal_velocity_sub<float>(const StepArgsT<float> *__restrict__);
 
 }  // namespace frehd

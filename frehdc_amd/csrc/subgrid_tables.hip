@@ -6,7 +6,9 @@
 // pixels stay in L1/L2 across levels.
 //
 // k_tables_reorder moves one table between the reference's level-major
-// layout t[k*N + c] (files, host arrays) and the interleaved one.
+// layout t[k*N + c] (files, host arrays; FP64) and the interleaved one.
+// The interleaved tables are stored in the handle's state precision TT
+// (FP32 for the mixed form: values rounded to nearest).
 #include <hip/hip_runtime.h>
 
 #include "step_kernels.h"
@@ -14,15 +16,17 @@
 
 namespace frehd {
 
+template <class TT>
 struct TableGenArgs {
     const double *fine;       // fy x fx, row-major (x fastest), fx = nx*r
-    double *tabs;             // interleaved V Z Vxp Vxm Vyp Vym Np Nm Op Om
+    TT *tabs;                 // interleaved V Z Vxp Vxm Vyp Vym Np Nm Op Om
     double surfmin, dsurf, dxf;
     int nx, ny, r, Ne;
     int k0, k1;               // level range of this launch
 };
 
-__global__ void __launch_bounds__(256) k_subgrid_tables(TableGenArgs a)
+template <class TT>
+__global__ void __launch_bounds__(256) k_subgrid_tables(TableGenArgs<TT> a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -33,15 +37,16 @@ __global__ void __launch_bounds__(256) k_subgrid_tables(TableGenArgs a)
         const double h[1] = {level_raw(a.surfmin, a.dsurf, k)};
         double v[1][10];
         cell_levels<1>(bed, i, j, h, v);
-        double *o = a.tabs + tab_index(c, k, (size_t)a.nx * a.ny, 0);
+        TT *o = a.tabs + tab_index(c, k, (size_t)a.nx * a.ny, 0);
 #pragma unroll
-        for (int t = 0; t < 10; t++) o[t] = v[0][t];
+        for (int t = 0; t < 10; t++) o[t] = (TT)v[0][t];
     }
 }
 
 // one table between level-major lm[k*N + c] and the interleaved layout;
 // toInterleaved selects the direction
-__global__ void __launch_bounds__(256) k_tables_reorder(double *tabs, double *lm, int N, int Ne, int T,
+template <class TT>
+__global__ void __launch_bounds__(256) k_tables_reorder(TT *tabs, double *lm, int N, int Ne, int T,
                                                         int toInterleaved)
 {
     const size_t total = (size_t)N * Ne;
@@ -49,22 +54,28 @@ __global__ void __launch_bounds__(256) k_tables_reorder(double *tabs, double *lm
         // q = k*N + c on both sides
         const int k = (int)(q / N);
         const size_t c = q - (size_t)k * N;
-        if (toInterleaved) tabs[tab_index(c, k, N, T)] = lm[q];
+        if (toInterleaved) tabs[tab_index(c, k, N, T)] = (TT)lm[q];
         else lm[q] = tabs[tab_index(c, k, N, T)];
     }
 }
 
-int launch_tables_reorder(double *tabs, double *lm, int N, int Ne, int T, int toInterleaved, hipStream_t stream)
+int launch_tables_reorder(void *tabs, double *lm, int N, int Ne, int T, int toInterleaved, int f32, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_tables_reorder, dim3(4096), dim3(256), 0, stream, tabs, lm, N, Ne, T, toInterleaved);
+    if (f32)
+        hipLaunchKernelGGL(k_tables_reorder<float>, dim3(4096), dim3(256), 0, stream, static_cast<float *>(tabs), lm, N,
+                           Ne, T, toInterleaved);
+    else
+        hipLaunchKernelGGL(k_tables_reorder<double>, dim3(4096), dim3(256), 0, stream, static_cast<double *>(tabs), lm,
+                           N, Ne, T, toInterleaved);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // host launcher: levels in chunks so that no launch runs long
-int launch_subgrid_tables(const double *fineDev, double *tabs, int nx, int ny, int r, int Ne,
-                          double surfmin, double dsurf, double dxf, hipStream_t stream)
+template <class TT>
+static int launch_tables(const double *fineDev, TT *tabs, int nx, int ny, int r, int Ne, double surfmin, double dsurf,
+                         double dxf, hipStream_t stream)
 {
-    TableGenArgs a;
+    TableGenArgs<TT> a;
     a.fine = fineDev;
     a.tabs = tabs;
     a.surfmin = surfmin;
@@ -79,10 +90,17 @@ int launch_subgrid_tables(const double *fineDev, double *tabs, int nx, int ny, i
     for (int k0 = 0; k0 < Ne; k0 += chunk) {
         a.k0 = k0;
         a.k1 = k0 + chunk < Ne ? k0 + chunk : Ne;
-        hipLaunchKernelGGL(k_subgrid_tables, g, b, 0, stream, a);
+        hipLaunchKernelGGL(k_subgrid_tables<TT>, g, b, 0, stream, a);
         if (hipGetLastError() != hipSuccess) return -1;
     }
     return 0;
+}
+
+int launch_subgrid_tables(const double *fineDev, void *tabs, int nx, int ny, int r, int Ne, double surfmin,
+                          double dsurf, double dxf, int f32, hipStream_t stream)
+{
+    if (f32) return launch_tables(fineDev, static_cast<float *>(tabs), nx, ny, r, Ne, surfmin, dsurf, dxf, stream);
+    return launch_tables(fineDev, static_cast<double *>(tabs), nx, ny, r, Ne, surfmin, dsurf, dxf, stream);
 }
 
 }  // namespace frehd
