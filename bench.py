@@ -35,14 +35,23 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# algorithmic bytes per interior cell: one PCG iteration reads the two
-# face-coefficient arrays and reads+writes x, r and p (8 accesses), FP64 or
-# FP32 (the inner solve of the mixed form)
-PCG_ITER_BYTES = 8 * 8
+# algorithmic bytes per interior cell of one solver iteration.  FP64 Jacobi
+# PCG: the two face-coefficient arrays and the diagonal (z = r / |D| and the
+# operator's centre term need it every iteration), x, r and p read and
+# written: 9 accesses.  The inner FP32 solve of the mixed form runs on the
+# Jacobi-scaled operator (unit diagonal): cX, cY, x, r, p: 8 accesses.
+PCG_ITER_BYTES = 9 * 8
 PCG_ITER_BYTES_F32 = 4 * 8
-# predictor + assembly and update + one scalar (11 + 12 array accesses + 3 for
-# the solver start), frozen per cell-step
+# the FP64 solve from a nonzero start: r = b - A x_0, R {b, x, D, cX, cY} W r
+PCG_START_BYTES = 6 * 8
+# predictor + assembly and update + one scalar, frozen per cell-step: FP64,
+# 11 + 12 array accesses + 3 for the solver start, 8 B each; the mixed form
+# keeps the explicit state at 4 B except eta, b and x (8 B): predictor R
+# {eta 8, u, v, zb, zbx, zby} W {b 8, cx, cy, Enx, Eny} = 52 B, update R {x 8,
+# eta 8, zb, zbx, zby, Enx, Eny, S} W {eta 8, u, v, S} = 60 B (its solver
+# start is the first refinement sweep, counted with the sweeps)
 STEP_BYTES_FP64 = 8 * 26
+STEP_BYTES_MIXED = 52 + 60
 # each scalar beyond the first: read and write its concentration
 SCALAR_BYTES = 8 * 2
 # one FP64 refinement sweep of the mixed form: residual R {x, b, cX, cY}
@@ -51,37 +60,39 @@ REFINE_SWEEP_BYTES = 8 * 6 + 4 * 2
 HBM_PEAK_GBS = 8000.0
 
 
-def step_bytes(cfg, mixed, k, m):
+def step_bytes(cfg, mixed, k, m, warm=False):
     """Algorithmic HBM bytes of one cell-step: every array each phase needs,
-    read or written once per phase (stencil halos free).  Subgrid adds the
-    two bracketing levels of T tables, the level index and the eight face /
-    volume arrays written by the update and read by the predictor: T = 10,
-    + 4 minimum face areas when useminA, + 4 minimum volumes when they can
-    change a result (useminV = 2, or useminV = 1 with useminA: without the
-    minimum areas the volume adjustment never fires, and the reference's
-    unconditional reads of those tables are not counted).  k solver
-    iterations per step (FP64 64 B, inner FP32 32 B per cell) and m
-    refinement sweeps.  The predictor and update arrays are counted at FP64
-    in both forms: the mixed form keeps them in FP64."""
-    b = STEP_BYTES_FP64 + SCALAR_BYTES * max(cfg.get("nScalar", 1) - 1, 0)
+    read or written once per phase (stencil halos free), at the width the
+    form stores it (mixed: 4 B but for eta, b, x).  Subgrid adds the two
+    bracketing levels of T tables, the level index (4 B, read and written)
+    and the eight face / volume arrays written by the update and read by the
+    predictor: T = 10, + 4 minimum face areas when useminA, + 4 minimum
+    volumes when they can change a result (useminV = 2, or useminV = 1 with
+    useminA: without the minimum areas the volume adjustment never fires,
+    and the reference's unconditional reads of those tables are not
+    counted).  k solver iterations per step (FP64 72 B, inner FP32 32 B per
+    cell), m refinement sweeps, and the FP64 start residual from a nonzero
+    start (warm)."""
+    w = 4 if mixed else 8
+    b = (STEP_BYTES_MIXED if mixed else STEP_BYTES_FP64) + 2 * w * max(cfg.get("nScalar", 1) - 1, 0)
     if cfg.get("useSubgrid", 0):
         minA = cfg.get("useminA", 0) == 1
         minV = cfg.get("useminV", 0) == 2 or (cfg.get("useminV", 0) == 1 and minA)
         T = 10 + (4 if minA else 0) + (4 if minV else 0)
-        b += 8 * (2 * T + 16) + 8
+        b += w * (2 * T + 16) + 8
     if mixed:
         b += PCG_ITER_BYTES_F32 * k + REFINE_SWEEP_BYTES * m
     else:
-        b += PCG_ITER_BYTES * k
+        b += PCG_ITER_BYTES * k + (PCG_START_BYTES if warm else 0)
     return b
 
 
-def step_roofline(cfg, N, r, mixed):
+def step_roofline(cfg, N, r, mixed, warm=False):
     """The whole step against HBM bandwidth: algorithmic bytes per cell-step
     (step_bytes, with the measured iterations and the refinement sweeps of
     the kernel-timed steps) x cells / the timed ms per step."""
     m = r["kt"].get("k_refine", (0, 0))[1] / 4.0 if mixed else 0.0
-    per_cell = step_bytes(cfg, mixed, r["iters"], m)
+    per_cell = step_bytes(cfg, mixed, r["iters"], m, warm)
     achieved = per_cell * N / (r["ms_per_step"] * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": "whole step (all kernels)", "bytes_per_cell_step": round(per_cell, 1),
             "pcg_iterations_per_step": round(r["iters"], 2), "refinement_sweeps_per_step": m,
@@ -146,7 +157,7 @@ def run_form(scen, steps, warmup, tune=True, **over):
     dev.step(3)
     if TILING:
         from frehdc_amd import abi
-        lanes = TILING[2] if over.get("precision") == abi.PREC_MIXED else 1
+        lanes = min(TILING[2], 4 if over.get("precision") == abi.PREC_MIXED else 2)
         dev.set_pcg_tiling(TILING[0], TILING[1], lanes)
         tiling = {"rows_per_tile": TILING[0], "xcd_remap": TILING[1], "cols_per_lane": lanes}
     else:
@@ -264,9 +275,9 @@ def traffic_of(kernels, workload):
 
 def solver_roofline(kt, N, mixed):
     """Roofline of one solver iteration from event-timed kernel averages:
-    FP64, the k_pcg_dir + k_pcg_res pair (64 B per cell); mixed, one working
-    k_cg1 launch of the inner FP32 solve (32 B per cell)."""
-    ks, per_cell = (["k_cg1_f32"], PCG_ITER_BYTES_F32) if mixed else (["k_pcg_dir", "k_pcg_res"], PCG_ITER_BYTES)
+    FP64, one working k_pcg1 launch (72 B per cell); mixed, one working k_cg1
+    launch of the inner FP32 solve (32 B per cell)."""
+    ks, per_cell = (["k_cg1_f32"], PCG_ITER_BYTES_F32) if mixed else (["k_pcg1"], PCG_ITER_BYTES)
     t_ms = sum(kt.get(k, (0, 0))[0] for k in ks)
     alg = per_cell * N
     achieved = alg / (t_ms * 1e-3) / 1e9 if t_ms > 0 else 0.0
@@ -294,7 +305,7 @@ def large_domain(size, r, steps, warmup):
         out[f] = {"cell_steps_per_s": r_["value"], "ms_per_step": r_["ms_per_step"],
                   "pcg_iterations_per_step": round(r_["iters"], 2), "refinement_sweeps": r_["sweeps"],
                   "rel_residual_last": r_["relres"], "tiling": r_["tiling"], "solver_roofline": roof,
-                  "step_roofline": step_roofline(sc.cfg, sc.N, r_, f != "fp64")}
+                  "step_roofline": step_roofline(sc.cfg, sc.N, r_, f != "fp64", f != "fp64")}
         log(f"bench: large {f}: {r_['ms_per_step']:.2f} ms/step, {r_['iters']:.1f} iterations/step, "
             f"solver {roof['achieved']:.0f} GB/s ({roof['frac']:.3f} of peak)")
     N = sc.N
@@ -319,7 +330,7 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp64,mixed_cold,mixed_warm",
+    ap.add_argument("--forms", default="fp64,fp64_warm,mixed_cold,mixed_warm",
                     help="precision forms to time (the headline value is the last mixed form)")
     ap.add_argument("--batch-members", type=int, default=64)
     ap.add_argument("--batch-size", type=int, default=2048)
@@ -332,7 +343,7 @@ def main():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--tiling", default="", help="H:remap[:cols_per_lane] solver tiling for the single-domain "
                                                  "forms instead of the autotuner (profiled runs: no trial "
-                                                 "launches; cols_per_lane applies to the mixed forms)")
+                                                 "launches; cols_per_lane: at most 2 for the FP64 forms)")
     args = ap.parse_args()
     global TILING
     if args.tiling:
@@ -356,7 +367,10 @@ def main():
     log(f"bench: scenario built in {time.perf_counter() - t0:.1f} s")
     N = scen.N
     grid = f"subgrid {args.subgrid}x{args.subgrid} (Ne={scen.cfg['Ne']})" if args.subgrid > 0 else "plain grid"
-    forms = {"fp64": {}, "mixed_cold": {"precision": abi.PREC_MIXED, "warmStart": 0},
+    # fp64: the reference's solve (cold start every step); fp64_warm: the same
+    # FP64 solve from the extrapolated start 2 eta^n - eta^(n-1), so that the
+    # gain of the start and the gain of the precision are reported apart
+    forms = {"fp64": {}, "fp64_warm": {"warmStart": 2}, "mixed_cold": {"precision": abi.PREC_MIXED, "warmStart": 0},
              "mixed_warm": {"precision": abi.PREC_MIXED, "warmStart": 2}}
     names = [f for f in args.forms.split(",") if f]
     res = {}
@@ -378,7 +392,7 @@ def main():
         o = {"cell_steps_per_s": r["value"], "ms_per_step": r["ms_per_step"],
              "pcg_iterations_per_step": round(r["iters"], 2), "refinement_sweeps": r["sweeps"],
              "rel_residual_last": r["relres"], "tiling": r["tiling"],
-             "step_roofline": step_roofline(scen.cfg, N, r, f != "fp64")}
+             "step_roofline": step_roofline(scen.cfg, N, r, f.startswith("mixed"), f.endswith("warm"))}
         if f != "fp64" and "fp64" in res:
             o["max_abs_eta_err_m"] = err(f, "surf")
             o["max_abs_u_err"] = err(f, "uuXP")
@@ -394,8 +408,8 @@ def main():
     # on the handle's stream: FP64, the two kernels of one PCG iteration;
     # mixed, the one launch of an inner FP32 iteration (working launches:
     # the set-up and stop-test launches are timed apart)
-    dtype = "f64" if head == "fp64" else "f32+f64"
-    ks, roof = solver_roofline(hr["kt"], N, head != "fp64")
+    dtype = "f32+f64" if head.startswith("mixed") else "f64"
+    ks, roof = solver_roofline(hr["kt"], N, head.startswith("mixed"))
     traffic = traffic_of(ks, f"marsh{args.size}_sub{args.subgrid}")
     roof["traffic"] = traffic
     if traffic:

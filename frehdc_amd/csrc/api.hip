@@ -226,12 +226,12 @@ struct Carve {
 inline size_t pcg_blocks(size_t N) { return std::min<size_t>(1024, (N + 255) / 256); }
 
 // Solver tiling: tiles of 256 W columns x H rows (W adjacent columns per
-// lane, the inner k_cg1 only).  The default H gives about `target` tiles:
-// 1024 for the FP64 PCG (16 waves per CU on 256 CUs; one partial per tile
-// for k_pcg_res to re-reduce), 512 for the mixed path, whose k_cg1 writes
-// four partials per tile and re-reduces all of them in every tile of the
-// next launch, a cost that grows with the square of the tile count.  The
-// XCD remap pads the tile count to a multiple of 8.
+// lane: k_cg1, k_pcg1).  The default H gives about `target` tiles: 1024 for
+// the SSOR path's k_pcg_dir (16 waves per CU on 256 CUs; one partial per
+// tile for k_pcg_res to re-reduce), 512 for k_cg1 and k_pcg1, which write
+// four or five partials per tile and re-reduce all of them in every tile of
+// the next launch, a cost that grows with the square of the tile count.
+// The XCD remap pads the tile count to a multiple of 8.
 struct DirTiling {
     int gx, gy, H, remap, nb;
 };
@@ -250,6 +250,14 @@ inline DirTiling dir_tiling(int nx, int ny, int H, int remap, int target = 1024,
 }
 
 inline size_t max_dir_blocks(int nx, int ny) { return (size_t)((nx + 255) / 256) * ny + 8; }
+
+// the most adjacent columns per lane the configuration's solver takes:
+// k_cg1 4 (mixed), k_pcg1 2 (FP64 Jacobi), k_pcg_dir 1 (SSOR)
+inline int max_lanes(const frehd_config &k)
+{
+    if (k.precision == FREHD_PREC_MIXED) return 4;
+    return k.precond == FREHD_PRECOND_JACOBI ? 2 : 1;
+}
 
 // the group's flag array: pinned, coherent host memory mapped for the
 // device, which the flag gather writes directly
@@ -294,12 +302,12 @@ PartialSizes partial_sizes(const frehd_config &k)
     z.maxBlocks = maxb;
     z.nb1 = nb1d;
     z.nb2 = nb2d;
-    z.partA = 2 * std::max(G, maxDir);
+    z.partA = 3 * std::max(G, maxDir);
     z.partB = maxDir;
     z.vpart = 4 * maxb;
     z.cflpart = 2 * maxb;
     const bool mixed = k.precision == FREHD_PREC_MIXED;
-    z.partF = mixed ? 4 * maxDir : 0;
+    z.partF = mixed ? 4 * maxDir : (k.precond == FREHD_PRECOND_JACOBI ? 5 * maxDir : 0);
     z.partC = mixed ? 2 * maxDir : 0;
     z.bpart = (size_t)budget_slots(k.useScalar == 1 ? std::max(1, k.nScalar) : 0) * maxb;
     return z;
@@ -340,6 +348,7 @@ size_t layout(frehd_dev *d, char *base)
     for (auto **p : nArr) *p = cv.take<double>(N);
     // SSOR: the preconditioned residual z
     d->pcg.z = k.precond == FREHD_PRECOND_SSOR ? cv.take<double>(N) : nullptr;
+    d->pcg.partF[0] = d->pcg.partF[1] = nullptr;
     a.manning = nullptr;
     a.rmv = cv.take<uint8_t>(N);
     a.flags = cv.take<uint8_t>(N);
@@ -349,6 +358,11 @@ size_t layout(frehd_dev *d, char *base)
     a.nb2 = (int)ps.nb2;
     a.partA = cv.take<double>(ps.partA);
     a.partB = cv.take<double>(ps.partB);
+    if (k.precision == FREHD_PREC_FP64 && ps.partF) {
+        // k_pcg1's partials
+        d->pcg.partF[0] = cv.take<double>(ps.partF);
+        d->pcg.partF[1] = cv.take<double>(ps.partF);
+    }
     a.cX32 = a.cY32 = nullptr;
     a.xB = nullptr;
     if (k.precision == FREHD_PREC_MIXED) {
@@ -477,8 +491,13 @@ int check_config(const frehd_config *k)
                                   "(nx*ny < 2^29)");
     if (k->precision == FREHD_PREC_MIXED && !(k->epsInner > 0 && k->epsInner < 1 && k->maxRefine >= 1))
         return fail(FREHD_EINVAL, "mixed precision needs 0 < epsInner < 1 and maxRefine >= 1");
-    if (k->precision == FREHD_PREC_MIXED && (k->warmStart < 0 || k->warmStart > 2))
-        return fail(FREHD_EINVAL, "warmStart must be 0, 1 or 2");
+    if (k->warmStart < 0 || k->warmStart > 2) return fail(FREHD_EINVAL, "warmStart must be 0, 1 or 2");
+    if (k->warmStart != 0 && k->precond == FREHD_PRECOND_SSOR)
+        return fail(FREHD_EINVAL, "warmStart needs the Jacobi preconditioner (the SSOR path starts from 0, as the "
+                                  "reference's solveMatrix does)");
+    if (k->precision == FREHD_PREC_FP64 && k->precond == FREHD_PRECOND_JACOBI &&
+        (int64_t)k->nx * k->ny * 8 > (int64_t)1 << 31)
+        return fail(FREHD_EINVAL, "the FP64 solver addresses its vectors with 32-bit byte offsets (nx*ny <= 2^28)");
     if (k->useScalar == 1 && k->scalarAdv != 1 && k->scalarAdv != 2)
         return fail(FREHD_EINVAL, "scalarAdv must be 1 (first-order upwind) or 2 (TVD superbee)");
     if (k->bcType < 1 || k->bcType > 3) return fail(FREHD_EINVAL, "bcType must be 1, 2 or 3");
@@ -542,7 +561,7 @@ void fill_params(P &p, const frehd_config &k)
     p.beta = k.beta;
     p.rhoa = k.rhoa; p.Cw = k.Cw; p.CwT = k.CwT;
     p.useSubgrid = k.useSubgrid; p.dsurf = k.dsurf; p.subR = k.subR; p.Ne = k.Ne; p.surfmin = k.surfmin;
-    p.warmStart = k.precision == FREHD_PREC_MIXED ? k.warmStart : 0;
+    p.warmStart = k.warmStart;
 }
 
 int upload(frehd_dev *d, const void *host, void *dev, size_t bytes)
@@ -861,13 +880,19 @@ void set_pcg_args(frehd_dev *d, int H, int remap, int lanes)
     p.nx = d->g.nx;
     p.ny = d->g.ny;
     p.N = d->g.N;
-    const DirTiling t = dir_tiling(p.nx, p.ny, H, remap, d->cfg.precision == FREHD_PREC_MIXED ? 512 : 1024);
+    // FP64: k_pcg1 (Jacobi) takes W columns per lane and writes five
+    // partials per tile, k_pcg_dir (SSOR) one column and one partial
+    const bool mixed = d->cfg.precision == FREHD_PREC_MIXED;
+    const bool jacobi64 = !mixed && d->cfg.precond == FREHD_PRECOND_JACOBI;
+    p.lanes = jacobi64 ? lanes : 1;
+    const DirTiling t = dir_tiling(p.nx, p.ny, H, remap, d->cfg.precond == FREHD_PRECOND_SSOR ? 1024 : 512, p.lanes);
     p.gx = t.gx;
     p.gy = t.gy;
     p.H = t.H;
     p.remap = t.remap;
-    p.lanes = 1;
     p.nbDir = t.nb;
+    p.b = a.b;
+    p.warm = !mixed && d->cfg.warmStart != 0;
     p.nbRes = (int)pcg_blocks(p.N);
     p.nbStart = p.nbRes;
     p.outerDone = nullptr;
@@ -954,9 +979,9 @@ int autotune_group(Exec &e, int iters, int32_t *bestH, int32_t *bestRemap, int32
     for (int H = 1; H <= nyMin && H <= 256; H *= 2) Hs.push_back(H);
     std::vector<int> Ws = {1};
     for (int W : {2, 4}) {
-        bool ok = e.mixed != 0;
+        bool ok = true;
         for (auto *d : e.members)
-            if (d->g.nx % W) ok = false;
+            if (d->g.nx % W || W > max_lanes(d->cfg)) ok = false;
         if (ok) Ws.push_back(W);
     }
     int rc = 0, bH = 0, bR = 0, bW = 1;
@@ -1006,7 +1031,7 @@ int autotune_group(Exec &e, int iters, int32_t *bestH, int32_t *bestRemap, int32
     if (rc2) return rc2;
     if (bestH) *bestH = e.mixed ? e.members[0]->pcg32.H : e.members[0]->pcg.H;
     if (bestRemap) *bestRemap = e.members[0]->pcg.remap;
-    if (bestLanes) *bestLanes = e.mixed ? e.members[0]->pcg32.lanes : 1;
+    if (bestLanes) *bestLanes = e.mixed ? e.members[0]->pcg32.lanes : e.members[0]->pcg.lanes;
     if (bestMs) *bestMs = best;
     return 0;
 }
@@ -1424,9 +1449,9 @@ int frehd_dev_set_pcg_tiling(frehd_dev_t *d, int32_t rows_per_tile, int32_t xcd_
         return fail(FREHD_EINVAL, "rows_per_tile %d out of range [0, %d]", rows_per_tile, d->g.ny);
     if (cols_per_lane != 1 && cols_per_lane != 2 && cols_per_lane != 4)
         return fail(FREHD_EINVAL, "cols_per_lane must be 1, 2 or 4");
-    if (cols_per_lane > 1 && (d->cfg.precision != FREHD_PREC_MIXED || d->g.nx % cols_per_lane))
-        return fail(FREHD_EINVAL, "cols_per_lane %d needs the mixed precision path and nx a multiple of it",
-                    cols_per_lane);
+    if (cols_per_lane > max_lanes(d->cfg) || d->g.nx % cols_per_lane)
+        return fail(FREHD_EINVAL, "cols_per_lane %d: at most %d for this solver (4 mixed, 2 FP64 Jacobi, 1 SSOR), "
+                    "and nx a multiple of it", cols_per_lane, max_lanes(d->cfg));
     if ((size_t)dir_tiling(d->g.nx, d->g.ny, rows_per_tile, xcd_remap).nb > max_dir_blocks(d->g.nx, d->g.ny))
         return fail(FREHD_EINVAL, "too many tiles");
     if (int rc = state_check(d, CALL_TILING)) return rc;
@@ -1479,12 +1504,15 @@ int frehd_batch_add(frehd_batch_t *b, frehd_dev_t *d)
     if ((int)b->e.members.size() >= b->e.capacity) return fail(FREHD_EBUDGET, "batch full (%d)", b->e.capacity);
     HIPCHK(hipSetDevice(d->device));
     HIPCHK(hipStreamSynchronize(d->stream));
-    // the inner solve's launches are shared: the newcomer takes the
-    // members' columns per lane, or everyone goes back to one if its nx
-    // does not allow that
-    if (!b->e.members.empty() && d->cfg.precision == FREHD_PREC_MIXED) {
-        const int L = b->e.members[0]->pcg32.lanes;
-        if (d->pcg32.lanes != L) {
+    // the solver's launches are shared: the newcomer takes the members'
+    // columns per lane, or everyone goes back to one if its nx does not
+    // allow that
+    auto lanes_of = [](const frehd_dev_impl *m) {
+        return m->cfg.precision == FREHD_PREC_MIXED ? m->pcg32.lanes : m->pcg.lanes;
+    };
+    if (!b->e.members.empty() && max_lanes(d->cfg) > 1 && max_lanes(b->e.members[0]->cfg) > 1) {
+        const int L = lanes_of(b->e.members[0]);
+        if (lanes_of(d) != L) {
             int rc = 0;
             if (d->g.nx % L == 0) {
                 set_pcg_args(d, d->tileH, d->tileRemap, L);
@@ -1495,7 +1523,7 @@ int frehd_batch_add(frehd_batch_t *b, frehd_dev_t *d)
                     set_pcg_args(md, md->tileH, md->tileRemap, 1);
                     if (!rc) rc = upload_member_args(md);
                 }
-                if (!rc && d->pcg32.lanes != 1) {
+                if (!rc && lanes_of(d) != 1) {
                     set_pcg_args(d, d->tileH, d->tileRemap, 1);
                     rc = upload_member_args(d);
                 }

@@ -16,7 +16,7 @@ const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble"
     "k_pcg_res", "k_surface", "k_limiter", "k_faces", "k_tidal_velocity",
     "k_finish", "k_scalar", "k_step_end", "k_combine",
     "k_subarea_init", "k_cg1_f32", "k_cg1_f32_setup", "k_cg1_f32_stop", "k_refine", "k_refine_test", "k_gather_flags",
-    "k_pcg_dir_stop", "k_ssor"};
+    "k_pcg_dir_stop", "k_ssor", "k_pcg1", "k_pcg1_setup", "k_pcg1_stop", "k_pcg1_start"};
 
 void Timing::reset()
 {
@@ -172,6 +172,9 @@ struct Tag {
     ~Tag() { e.tagKind = 0; }
 };
 
+// launch it of the FP64 solve: with Jacobi one k_pcg1 (it = 0 sets up,
+// after k_pcg1_start for members starting from a nonzero x); with SSOR
+// (a group's members share the preconditioner) k_ssor, k_pcg_dir, k_pcg_res
 int enqueue_pcg_iteration(Exec &e, int it)
 {
     Tag tag(e, 1, 0, it);
@@ -180,15 +183,31 @@ int enqueue_pcg_iteration(Exec &e, int it)
         const dim3 g(1, 1, (unsigned)e.count), b(1024);
         if (e.ext.ssorLds) LAUNCH_SHM(K_SSOR, k_ssor<true>, g, b, (size_t)e.ext.ssorN * sizeof(double), e.dPcg, it);
         else LAUNCH(K_SSOR, k_ssor<false>, g, b, e.dPcg, it);
+        LAUNCH(K_PQ, k_pcg_dir<double>, grid1(e.ext.nbDir, e.count), dim3(256), e.dPcg, it);
+        LAUNCH(K_UPD, k_pcg_res<double>, grid1(e.ext.nbRes, e.count), dim3(256), e.dPcg, it);
+        return 0;
     }
-    LAUNCH(K_PQ, k_pcg_dir<double>, grid1(e.ext.nbDir, e.count), dim3(256), e.dPcg, it);
-    LAUNCH(K_UPD, k_pcg_res<double>, grid1(e.ext.nbRes, e.count), dim3(256), e.dPcg, it);
+    const dim3 g = grid1(e.ext.nbDir, e.count), b(256);
+    if (it == 0 && e.ext.warm) LAUNCH(K_PCG1_START, k_pcg1_start, grid1(e.ext.nbRes, e.count), b, e.dPcg);
+    switch ((it == 0 ? 8 : 0) + e.ext.lanes) {
+    case 9: LAUNCH(K_PCG1_SET, (k_pcg1<true, 1>), g, b, e.dPcg, it); break;
+    case 10: LAUNCH(K_PCG1_SET, (k_pcg1<true, 2>), g, b, e.dPcg, it); break;
+    case 1: LAUNCH(K_PCG1, (k_pcg1<false, 1>), g, b, e.dPcg, it); break;
+    case 2: LAUNCH(K_PCG1, (k_pcg1<false, 2>), g, b, e.dPcg, it); break;
+    default: return fail(FREHD_EINVAL, "columns per lane %d", e.ext.lanes);
+    }
     return 0;
 }
 
+// launches of an FP64 solve of n iterations, the one that stops included:
+// k_pcg1 launches 0..n+1 (launch L tests the residual of iteration L-1),
+// SSOR iterations 0..n (k_pcg_dir(n) tests the residual of iteration n)
+inline int fp64_launches(const Exec &e, int n) { return e.ext.ssorN > 0 ? n + 1 : n + 2; }
+
 // After a solve, the timed solver launches by what they did (see
-// TimedLaunch): FP64 iteration it works below the slowest member's stop
-// iteration D, k_pcg_dir(D) applies the last x update and stops; inner
+// TimedLaunch): FP64 launch it works below the slowest member's stop
+// launch D, which only tests (k_pcg_dir(D) also applies the last x update
+// of the SSOR path); k_pcg1_start always works; inner
 // iteration it after sweep s works up to the most iterations any member
 // used there, the next launch only stops; a sweep works up to the slowest
 // member's converged sweep.
@@ -213,7 +232,8 @@ void classify_timed(Exec &e)
         if (r.kind == 1) {
             if (r.id == K_SSOR) r.id = r.it <= D ? K_SSOR : K_IDLE;
             else if (r.id == K_PQ) r.id = r.it < D ? K_PQ : (r.it == D ? K_PQ_STOP : K_IDLE);
-            else r.id = r.it < D ? K_UPD : K_IDLE;
+            else if (r.id == K_UPD) r.id = r.it < D ? K_UPD : K_IDLE;
+            else if (r.id == K_PCG1 || r.id == K_PCG1_SET) r.id = r.it < D ? r.id : (r.it == D ? K_PCG1_STOP : K_IDLE);
         } else if (r.kind == 2) {
             const int n = inner(r.s);
             if (n < 0) r.id = K_IDLE;
@@ -231,7 +251,7 @@ void classify_timed(Exec &e)
 // until every member reports convergence (or a halt).
 int solve_fp64_pattern(Exec &e, int *it)
 {
-    const int n = std::max(1, e.lastIters);
+    const int n = fp64_launches(e, std::max(1, e.lastIters));
     for (int q = 0; q < n; q++) {
         int rc = enqueue_pcg_iteration(e, q);
         if (rc) return rc;
@@ -380,7 +400,8 @@ int solve_mixed_finish(Exec &e, int s)
 
 // Average duration (ms) of one solver iteration of the group under its
 // current tiling: `iters` iterations on the members' current systems, timed
-// with events on the group's stream (FP64: the k_pcg_dir + k_pcg_res pair;
+// with events on the group's stream (FP64: one k_pcg1 launch, or the SSOR
+// triple k_ssor + k_pcg_dir + k_pcg_res;
 // mixed: the inner k_cg1 launches after k_refine(0) launches, the
 // refinement sweeps spread over the inner iterations in the proportion of
 // the last solve).  The caller makes the iterations run (zero tolerance)
@@ -402,7 +423,7 @@ int exec_time_solver(Exec &e, int iters, double *ms)
         for (int it = 0; it <= iters; it++)
             if ((rc = enqueue_inner_iteration(e, 0, it))) return rc;
     } else {
-        for (int it = 0; it < iters; it++)
+        for (int it = 0; it <= iters; it++)
             if ((rc = enqueue_pcg_iteration(e, it))) return rc;
     }
     if ((rc = exec_collect_timing(e))) return rc;
@@ -417,8 +438,10 @@ int exec_time_solver(Exec &e, int iters, double *ms)
         for (int k = 0; k < e.mixPlan.n; k++) inner += e.mixPlan.v[k];
         const int sweeps = e.mixPlan.n + 1;
         *ms = (std::max(inner, 1) * avg(K_CG32) + sweeps * avg(K_REFINE)) / std::max(inner, 1);
+    } else if (e.ext.ssorN > 0) {
+        *ms = avg(K_SSOR) + avg(K_PQ) + avg(K_UPD);
     } else {
-        *ms = avg(K_PQ) + avg(K_UPD);
+        *ms = avg(K_PCG1);
     }
     return 0;
 }
@@ -628,7 +651,7 @@ static int exec_step_graphs(Exec &e)
         for (auto *m : e.members) m->tt -= 1;
         e.heads.push_back({key, step});
     } else {
-        k = e.mixed ? e.mixPlan.n : std::max(1, e.lastIters);
+        k = e.mixed ? e.mixPlan.n : fp64_launches(e, std::max(1, e.lastIters));
     }
     clear_flags(e);
     HIPCHK(hipGraphLaunch(step, e.stream));
@@ -692,8 +715,9 @@ Extents member_extents(const frehd_dev_impl *d)
     const bool mixed = d->cfg.precision == FREHD_PREC_MIXED;
     x.nbDir = mixed ? d->pcg32.nbDir : d->pcg.nbDir;
     x.nbRef = mixed ? d->ref.nb : 0;
-    x.lanes = mixed ? d->pcg32.lanes : 1;
+    x.lanes = mixed ? d->pcg32.lanes : d->pcg.lanes;
     x.ssorN = d->pcg.ssor ? d->g.N : 0;
+    x.warm = !mixed && d->pcg.warm;
     x.budget = d->args.p.budget;
     x.ssorLds = !d->pcg.ssor || d->g.N <= d->pcg.ldsCells;
     return x;
@@ -720,6 +744,7 @@ static void merge(Exec &e, const frehd_dev_impl *d, bool first)
     x.nbDir = std::max(x.nbDir, m.nbDir);
     x.nbRef = std::max(x.nbRef, m.nbRef);
     x.ssorN = std::max(x.ssorN, m.ssorN);
+    x.warm = x.warm || m.warm;
     x.budget = x.budget || m.budget;
     x.ssorLds = x.ssorLds && m.ssorLds;
     e.maxIter = std::max(e.maxIter, d->cfg.maxIter);
@@ -741,8 +766,10 @@ int exec_refresh(Exec &e)
         } else if ((d->cfg.useSubgrid == 1) != (bool)e.sub || (d->cfg.precision == FREHD_PREC_MIXED) != (bool)e.mixed) {
             return fail(FREHD_EINVAL, "a group's members must share useSubgrid and precision");
         }
+        if (m > 0 && (d->pcg.ssor != 0) != (e.ext.ssorN > 0))
+            return fail(FREHD_EINVAL, "a group's members must share the preconditioner");
         if (m > 0 && member_extents(d).lanes != e.ext.lanes)
-            return fail(FREHD_EINVAL, "a group's members must share the inner solve's columns per lane");
+            return fail(FREHD_EINVAL, "a group's members must share the solver's columns per lane");
         merge(e, d, m == 0);
         HIPCHK(hipMemcpyAsync(e.dArgs + m, &d->args, sizeof(StepArgs), hipMemcpyHostToDevice, e.stream));
         HIPCHK(hipMemcpyAsync(e.dPcg + m, &d->pcg, sizeof(PcgArgs<double>), hipMemcpyHostToDevice, e.stream));

@@ -34,7 +34,8 @@ int fail(int code, const char *fmt, ...);
 
 enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_TIDEV,
                 K_FINISH, K_SCALAR, K_END, K_COMBINE, K_SUBINIT,
-                K_CG32, K_CG32_SET, K_CG32_STOP, K_REFINE, K_RTEST, K_GATHER, K_PQ_STOP, K_SSOR, K_COUNT,
+                K_CG32, K_CG32_SET, K_CG32_STOP, K_REFINE, K_RTEST, K_GATHER, K_PQ_STOP, K_SSOR,
+                K_PCG1, K_PCG1_SET, K_PCG1_STOP, K_PCG1_START, K_COUNT,
                 K_IDLE = -1 };
 extern const char *kKernelNames[K_COUNT];
 
@@ -69,8 +70,9 @@ struct Timing {
 // launch extents of a group: the maximum over its members
 struct Extents {
     int gx2, gy2, nb1, nbN, nbGhost, nbTide, nbRes, nbDir, nbRef;
-    int lanes;      // columns per lane of the inner k_cg1 (shared by the group)
+    int lanes;      // columns per lane of k_cg1 / k_pcg1 (shared by the group)
     int ssorN;      // largest N of the members preconditioned with SSOR (0: none)
+    int warm;       // an FP64 member starts its solve from a nonzero x
     int ssorLds;    // every SSOR member fits k_ssor's LDS vector
     int budget;     // a member accounts its conservation budgets
 };

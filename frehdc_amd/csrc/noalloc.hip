@@ -83,6 +83,10 @@ size_t point_into_scratch(frehd_dev_impl *d, char *base)
     p = d->pcg;
     p.partA = a.partA;
     p.partB = a.partB;
+    if (d->cfg.precision != FREHD_PREC_MIXED && ps.partF) {
+        p.partF[0] = take(ps.partF);
+        p.partF[1] = take(ps.partF);
+    }
     PcgArgs<float> &f = d->naPcg32H;
     f = d->pcg32;
     RefineArgs &r = d->naRefH;

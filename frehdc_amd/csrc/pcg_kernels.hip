@@ -862,6 +862,358 @@ __global__ void __launch_bounds__(256) k_refine_test(const RefineArgs *__restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 Jacobi PCG in one kernel and one reduction per iteration (the FP64
+// path with JacobiPrecond; SSOR keeps the k_pcg_dir / k_pcg_res pair).
+// CGIter's recurrences (itersolv.c:396-421, z = r / |D|) regrouped so that
+// launch L finishes iteration L-1 and starts iteration L, from r_{L-1},
+// p_{L-1}, x_{L-1} and the block partials of launch L-1 (p.q, z.q, q.M^-1 q,
+// r.r, r.z of iteration L-1, q = A p):
+//   alpha = rho_{L-1} / p.q                 (rho_{L-1} = r.z, summed directly)
+//   q_{L-1} = A p_{L-1}, recomputed here rather than stored
+//   x_L = x_{L-1} + alpha p_{L-1}           r_L = r_{L-1} - alpha q_{L-1}
+//   rho_L = rho_{L-1} - 2 alpha z.q + alpha^2 q.M^-1 q   (r_L.z_L expanded:
+//                                            valid for any A, no symmetry used)
+//   beta = rho_L / rho_{L-1}                p_L = z_L + beta p_{L-1}
+//   q_L = A p_L -> the partials of iteration L
+// LASpack's stop test |r| < eps |b| (RTCResult rtc.c:48-72, at the top of
+// every iteration) on r_{L-1} runs at the start of launch L on the directly
+// summed r_{L-1}.r_{L-1}; x_{L-1} of a stopped solve was written by launch
+// L-1.  The IsOne(alpha) / IsOne(beta) shortcuts of LASpack's scaled vector
+// operations are kept (a factor within 10 DBL_EPSILON of 1 is taken as 1).
+// Launch 0 tests r_0 (partials r.r, r.z, b.b of the assembly, or of
+// k_pcg1_start from a nonzero start) and forms p_0 = z_0 and the partials of
+// q_0.  One launch reads r, p, x, D, cX, cY and writes r, p, x: 72 B per
+// cell.  r and p alternate between two buffers (r_L in r / q by L & 1, p_L
+// in pBuf[L & 1]), because the halo lanes read their neighbours' previous
+// values; x is read and written only at the thread's own cells.
+//
+// Tile walk as in k_cg1: W adjacent columns per lane (W = 1, 2: one double
+// or one 16-byte access per array and row), loads two rows ahead, row t
+// ingested (q_{L-1}, r_L, p_L of row t) while row t-1 is finished (q_L, the
+// stores, the partials); lanes 0 and 63 also evaluate the column v beyond
+// the wave from the column o after it, with the owner's expressions.
+namespace {
+
+typedef double vd2 __attribute__((ext_vector_type(2)));
+
+__device__ inline Rsrc rsrc_of(const double *p, int n)
+{
+    const uint64_t v = (uint64_t)p;
+    const uint64_t u = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)u, (short)0, __builtin_amdgcn_readfirstlane(n * 8), 0x00020000);
+}
+
+__device__ inline double dload1(Rsrc r, uint32_t off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ inline void dload(Rsrc r, uint32_t off, double (&o)[1]) { o[0] = dload1(r, off); }
+__device__ inline void dload(Rsrc r, uint32_t off, double (&o)[2])
+{
+    const vd2 v = __builtin_bit_cast(vd2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    o[0] = v.x;
+    o[1] = v.y;
+}
+
+// byte offset of (row, col) in an N-double vector, or OFF_NONE outside the
+// grid or when !ok (N <= 2^28, checked at create)
+__device__ inline uint32_t doff_of(int row, int col, int nx, int ny, bool ok)
+{
+    return (ok && row >= 0 && row < ny) ? (uint32_t)(row * nx + col) * 8u : OFF_NONE;
+}
+
+template <int W>
+__device__ inline void dload_cols(Rsrc p, int row, int i, int nx, int ny, bool ok, double (&o)[W])
+{
+    dload(p, doff_of(row, i, nx, ny, ok), o);
+}
+
+__device__ inline void dstore_cols(double *p, size_t c, const double (&v)[1]) { gst(p + c, v[0]); }
+__device__ inline void dstore_cols(double *p, size_t c, const double (&v)[2])
+{
+    gst(reinterpret_cast<vd2 *>(p + c), vd2{v[0], v[1]});
+}
+
+// (A p) of one cell, Mul_QV's row order (operats.c:870-888, columns sorted:
+// west, south, centre, north, east); D < 0: an identity row scaled by -D.
+// Neighbours outside the grid come in as zeros.
+__device__ inline double apply_row(double pc, double Dc, double cW, double pW, double cS, double pS, double cN,
+                                   double pN, double cE, double pE)
+{
+#pragma clang fp contract(off)
+    if (Dc < 0) return -Dc * pc;
+    double s = 0.0;
+    s += (-cW) * pW;
+    s += (-cS) * pS;
+    s += Dc * pc;
+    s += (-cN) * pN;
+    s += (-cE) * pE;
+    return s;
+}
+
+// r_it and p_it of a cell from r_{it-1}, q_{it-1}, p_{it-1} (the same
+// rounding in the owner's lane and in a halo lane)
+__device__ inline void advance_jacobi(double R, double q, double P, double inv, double al, double be, double &rn,
+                                      double &pn)
+{
+#pragma clang fp contract(off)
+    rn = R - al * q;
+    pn = rn * inv + be * P;
+}
+
+// Jacobi's 1 / |D|, and 0 where D is 0: lanes past the grid's last column
+// and the row past its last load zeros, and their p must come out 0 (their
+// neighbours multiply it by a face coefficient, which is not 0 on the north
+// edge)
+__device__ inline double inv_diag(double D) { return D != 0.0 ? 1.0 / fabs(D) : 0.0; }
+
+template <int W>
+struct Pcg1Row {
+    double P2[W], R[W], D[W], cX[W], cY[W], X[W];   // own columns: p(t+1), r(t), D(t), cX(t), cY(t), x(t-1)
+    double hP2, hR, hD, hcXv, hcY, hPo, hcXo;       // lanes 0 / 63: column v and its outer neighbour o
+};
+
+}  // namespace
+
+template <bool First, int W>
+__global__ void __launch_bounds__(256) k_pcg1(const PcgArgs<double> *__restrict__ A, int it)
+{
+    it = First ? 0 : it;
+    const PcgArgs<double> &a = A[blockIdx.z];
+    if ((int)blockIdx.x >= a.nbDir) return;
+    __shared__ double lds[16 * 5];
+    PcgScalars *sc = a.sc;
+    const int halted = *a.halted, doneAt = sc->doneAt;
+    if (halted || (doneAt >= 0 && doneAt < it)) return;
+
+    double alphaD = 0.0, betaD = 0.0;
+    bool stop;
+    if (First) {
+        double v[3];
+        reduce_partials<3>(a.partA, a.nbStart, v, lds);
+        const double rNorm = sqrt(v[0]), bNorm = sqrt(v[2]);
+        stop = rNorm < a.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm)) || a.maxIter <= 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->bb = v[2];
+            sc->bNorm = bNorm;
+            sc->rr = v[0];
+            sc->rNorm = rNorm;
+            sc->iter = 0;
+            if (stop) sc->doneAt = 0;
+        }
+    } else {
+        double v[5];
+        reduce_partials<5>(a.partF[(it - 1) & 1], a.nbDir, v, lds);
+        const double pq = v[0], zq = v[1], qMq = v[2], rr = v[3], rz = v[4];
+        const double rNorm = sqrt(rr), bNorm = sc->bNorm;
+        stop = (it >= 2 && (rNorm < a.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm)))) ||
+               it - 1 >= a.maxIter || !(pq > 0);
+        alphaD = rz / pq;
+        if (LAS_IS_ONE(alphaD)) alphaD = 1.0;
+        double rho = rz - 2.0 * alphaD * zq + alphaD * alphaD * qMq;
+        if (!(rho > 0)) rho = 0;               // cancellation at convergence: restart the directions
+        betaD = rho / rz;
+        if (LAS_IS_ONE(betaD)) betaD = 1.0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->rr = rr;
+            sc->rNorm = rNorm;
+            sc->iter = stop ? it - 1 : it;
+            if (stop) {
+                sc->doneAt = it;
+            } else {
+                sc->pq = pq;
+                sc->alpha = alphaD;
+                sc->rhoHist[(it - 1) & 1] = rz;
+                sc->rhoHist[it & 1] = rho;
+            }
+        }
+    }
+    if (stop) return;                          // x_{it-1} is final
+    const double al = alphaD, be = betaD;
+
+    int lb = blockIdx.x;
+    if (a.remap) lb = (lb & 7) * (a.nbDir >> 3) + (lb >> 3);
+    const int bx = lb % a.gx, by = lb / a.gx;
+    double part[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (by < a.gy) {
+        const int nx = a.nx, ny = a.ny;
+        const int lane = threadIdx.x & 63;
+        const int i0 = bx * 256 * W + (threadIdx.x >> 6) * 64 * W;   // the wave's first column
+        const int i = i0 + lane * W;
+        const bool valid = i < nx;                                   // nx is a multiple of W
+        const int j0 = by * a.H, j1 = min(j0 + a.H, ny);
+        double *const rBuf[2] = {a.r, a.q};
+        const double *Rin = rBuf[First ? 0 : (it - 1) & 1];
+        const double *Pin = a.pBuf[(it - 1) & 1];                    // unused at it = 0
+        double *Rout = rBuf[it & 1];
+        double *Pout = a.pBuf[it & 1];
+        double *X = a.x;
+        const Rsrc cXr = rsrc_of(a.cX, a.N), cYr = rsrc_of(a.cY, a.N), Dr = rsrc_of(a.D, a.N);
+        const Rsrc rIn = rsrc_of(Rin, a.N), pIn = rsrc_of(Pin, a.N), xR = rsrc_of(X, a.N);
+        // halo lanes: v = the column beyond the wave, o = the one after it
+        const int v = lane == 0 ? i0 - 1 : (lane == 63 ? i0 + 64 * W : -1);
+        const bool halo = v >= 0 && v < nx;
+        const int o = lane == 0 ? v - 1 : v + 1;
+        const bool oin = halo && o >= 0 && o < nx;
+        auto at = [&](Rsrc p, int row, int col, bool ok) -> double {
+            return dload1(p, doff_of(row, col, nx, ny, ok));
+        };
+        auto load = [&](int t, Pcg1Row<W> &L) {
+            dload_cols<W>(pIn, t + 1, i, nx, ny, valid && !First, L.P2);
+            dload_cols<W>(rIn, t, i, nx, ny, valid, L.R);
+            dload_cols<W>(Dr, t, i, nx, ny, valid, L.D);
+            dload_cols<W>(cXr, t, i, nx, ny, valid, L.cX);
+            dload_cols<W>(cYr, t, i, nx, ny, valid, L.cY);
+            dload_cols<W>(xR, t - 1, i, nx, ny, valid && !First && t - 1 >= j0, L.X);
+            L.hP2 = at(pIn, t + 1, v, halo && !First);
+            L.hR = at(rIn, t, v, halo);
+            L.hD = at(Dr, t, v, halo);
+            L.hcXv = at(cXr, t, v, halo);
+            L.hcY = at(cYr, t, v, halo);
+            L.hPo = at(pIn, t, o, oin && !First);
+            L.hcXo = at(cXr, t, o, oin && lane == 0 && !First);
+        };
+        // window at the start: rows j0-2 and j0-1 of p_{it-1}, the south
+        // face of row j0-1 and the diagonal of row j0-1 come from the first
+        // ingested row
+        double P0[W], P1[W], cYm[W], cYmm[W], cXc[W], Dm[W], pnm[W], pnc[W], rnc[W];
+        dload_cols<W>(pIn, j0 - 2, i, nx, ny, valid && !First, P0);
+        dload_cols<W>(pIn, j0 - 1, i, nx, ny, valid && !First, P1);
+        dload_cols<W>(cYr, j0 - 2, i, nx, ny, valid, cYm);
+#pragma unroll
+        for (int k = 0; k < W; k++) cYmm[k] = cXc[k] = Dm[k] = pnm[k] = pnc[k] = rnc[k] = 0.0;
+        double hP0 = at(pIn, j0 - 2, v, halo && !First), hP1 = at(pIn, j0 - 1, v, halo && !First);
+        double hcYm = at(cYr, j0 - 2, v, halo), hpn = 0.0, hcVIprev = 0.0;
+        Pcg1Row<W> b0, b1, b2;
+        load(j0 - 1, b0);
+        load(j0, b1);
+        auto row = [&](int t, const Pcg1Row<W> &cur, Pcg1Row<W> &nxt2) {
+            if (t + 2 <= j1) load(t + 2, nxt2);
+            // own columns: q_{it-1}, r_it, p_it of row t
+            double pWl = __shfl_up(P1[W - 1], 1, 64), pEl = __shfl_down(P1[0], 1, 64);
+            double cWl = __shfl_up(cur.cX[W - 1], 1, 64);
+            if (lane == 0) { pWl = hP1; cWl = cur.hcXv; }
+            if (lane == 63) pEl = hP1;
+            double rn[W], pn[W];
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                const double inv = inv_diag(cur.D[k]);
+                if (First) {
+                    rn[k] = cur.R[k];
+                    pn[k] = cur.R[k] * inv;
+                } else {
+                    const double pW = k == 0 ? pWl : P1[k - 1];
+                    const double pE = k == W - 1 ? pEl : P1[k + 1];
+                    const double cW = k == 0 ? cWl : cur.cX[k - 1];
+                    const double q = apply_row(P1[k], cur.D[k], cW, pW, cYm[k], P0[k], cur.cY[k], cur.P2[k],
+                                               cur.cX[k], pE);
+                    advance_jacobi(cur.R[k], q, P1[k], inv, al, be, rn[k], pn[k]);
+                }
+            }
+            // halo column v of row t, the owner's expressions, evaluated in
+            // every lane and kept by lanes 0 and 63
+            double hpnT;
+            {
+                const double inv = inv_diag(cur.hD);
+                if (First) {
+                    hpnT = cur.hR * inv;
+                } else {
+                    const double hq0 = apply_row(hP1, cur.hD, cur.hcXo, cur.hPo, hcYm, hP0, cur.hcY, cur.hP2,
+                                                 cur.hcXv, P1[0]);
+                    const double hq63 = apply_row(hP1, cur.hD, cur.cX[W - 1], P1[W - 1], hcYm, hP0, cur.hcY,
+                                                  cur.hP2, cur.hcXv, cur.hPo);
+                    double hr;
+                    advance_jacobi(cur.hR, lane == 0 ? hq0 : hq63, hP1, inv, al, be, hr, hpnT);
+                }
+            }
+            hpnT = halo ? hpnT : 0.0;
+            // finish row j = t-1: q_it = A p_it, stores, partials
+            const int j = t - 1;
+            double qWl = __shfl_up(pnc[W - 1], 1, 64), qEl = __shfl_down(pnc[0], 1, 64);
+            double cWjl = __shfl_up(cXc[W - 1], 1, 64);
+            if (lane == 0) { qWl = hpn; cWjl = hcVIprev; }
+            if (lane == 63) qEl = hpn;
+            if (valid && j >= j0) {
+                const size_t c = (size_t)j * nx + i;
+                double xs[W];
+#pragma unroll
+                for (int k = 0; k < W; k++) {
+                    const double qW = k == 0 ? qWl : pnc[k - 1];
+                    const double qE = k == W - 1 ? qEl : pnc[k + 1];
+                    const double cWj = k == 0 ? cWjl : cXc[k - 1];
+                    const double qn = apply_row(pnc[k], Dm[k], cWj, qW, cYmm[k], pnm[k], cYm[k], pn[k], cXc[k], qE);
+                    const double inv = 1.0 / fabs(Dm[k]);
+                    const double zc = rnc[k] * inv;
+                    xs[k] = cur.X[k] + al * P0[k];
+                    part[0] += pnc[k] * qn;
+                    part[1] += zc * qn;
+                    part[2] += qn * (qn * inv);
+                    part[3] += rnc[k] * rnc[k];
+                    part[4] += rnc[k] * zc;
+                }
+                if (!First) {
+                    dstore_cols(Rout, c, rnc);
+                    dstore_cols(X, c, xs);
+                }
+                dstore_cols(Pout, c, pnc);
+            }
+            // slide
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                P0[k] = P1[k]; P1[k] = cur.P2[k];
+                cYmm[k] = cYm[k]; cYm[k] = cur.cY[k];
+                cXc[k] = cur.cX[k];
+                Dm[k] = cur.D[k];
+                pnm[k] = pnc[k]; pnc[k] = pn[k]; rnc[k] = rn[k];
+            }
+            hP0 = hP1; hP1 = cur.hP2;
+            hcYm = cur.hcY;
+            hpn = hpnT;
+            hcVIprev = cur.hcXv;
+        };
+        for (int t = j0 - 1; t <= j1; t += 3) {
+            row(t, b0, b2);
+            if (t + 1 > j1) break;
+            row(t + 1, b1, b0);
+            if (t + 2 > j1) break;
+            row(t + 2, b2, b1);
+        }
+    }
+    block_sum<5>(part, lds);
+    store_partials<5>(a.partF[it & 1], a.nbDir, {part[0], part[1], part[2], part[3], part[4]});
+}
+
+// The start residual from a nonzero start x_0 (CGIter itersolv.c:388-393:
+// r = b - A x_0), one cell per thread over a grid-stride loop, with the
+// partials r.r, r.z and b.b for k_pcg1's launch 0.
+__global__ void __launch_bounds__(256) k_pcg1_start(const PcgArgs<double> *__restrict__ A)
+{
+    const PcgArgs<double> &a = A[blockIdx.z];
+    if ((int)blockIdx.x >= a.nbStart || !a.warm) return;
+    __shared__ double lds[16 * 3];
+    if (*a.halted) return;
+    const double *b = a.b;
+    double part[3] = {0.0, 0.0, 0.0};
+    const int nx = a.nx, ny = a.ny;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < a.N; c += a.nbStart * blockDim.x) {
+        const int i = c % nx, j = c / nx;
+        const double Dc = a.D[c];
+        const double ax = apply_row(a.x[c], Dc, i > 0 ? a.cX[c - 1] : 0.0, i > 0 ? a.x[c - 1] : 0.0,
+                                    j > 0 ? a.cY[c - nx] : 0.0, j > 0 ? a.x[c - nx] : 0.0, a.cY[c],
+                                    j < ny - 1 ? a.x[c + nx] : 0.0, a.cX[c], i < nx - 1 ? a.x[c + 1] : 0.0);
+        const double bc = b[c];
+        const double r = bc - ax;
+        a.r[c] = r;
+        part[0] += r * r;
+        part[1] += r * (r * (1.0 / fabs(Dc)));
+        part[2] += bc * bc;
+    }
+    block_sum<3>(part, lds);
+    store_partials<3>(a.partA, a.nbStart, {part[0], part[1], part[2]});
+}
+
 template __global__ void k_pcg_dir<double>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_pcg_res<double>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_ssor<true>(const PcgArgs<double> *__restrict__, int);
@@ -872,5 +1224,9 @@ template __global__ void k_cg1<true, 2>(const PcgArgs<float> *__restrict__, int)
 template __global__ void k_cg1<false, 2>(const PcgArgs<float> *__restrict__, int);
 template __global__ void k_cg1<true, 4>(const PcgArgs<float> *__restrict__, int);
 template __global__ void k_cg1<false, 4>(const PcgArgs<float> *__restrict__, int);
+template __global__ void k_pcg1<true, 1>(const PcgArgs<double> *__restrict__, int);
+template __global__ void k_pcg1<false, 1>(const PcgArgs<double> *__restrict__, int);
+template __global__ void k_pcg1<true, 2>(const PcgArgs<double> *__restrict__, int);
+template __global__ void k_pcg1<false, 2>(const PcgArgs<double> *__restrict__, int);
 
 }  // namespace frehd

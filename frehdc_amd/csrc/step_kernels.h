@@ -212,8 +212,8 @@ struct PcgArgs {
     gptr<const T> cX, cY, D;
     gptr<T> r, q, x;
     gptr<T> pBuf[2];               // iteration it reads p from pBuf[it&1], writes pBuf[(it+1)&1]
-    gptr<double> partA;            // r.r, r.z partials: of the start residual (nbStart
-                                   // blocks) at iteration 0, of k_pcg_res (nbRes) after
+    gptr<double> partA;            // r.r, r.z (, b.b) partials: of the start residual
+                                   // (nbStart blocks) at iteration 0, of k_pcg_res (nbRes) after
     gptr<double> partB;            // p.q partials of k_pcg_dir (nbDir blocks)
     gptr<PcgScalars> sc;
     gptr<const int> halted;
@@ -226,13 +226,18 @@ struct PcgArgs {
     // != 0 gives each XCD a contiguous band of tiles (nbDir a multiple of 8)
     int gx, gy, H, remap, lanes;
     int nbDir, nbRes, nbStart;
-    gptr<double> partF[2];         // k_cg1: p.q, r.q, q.q, r.r partials (nbDir each),
-                                   // launch it writes partF[it&1]
+    gptr<double> partF[2];         // k_cg1: p.q, r.q, q.q, r.r partials (nbDir each);
+                                   // k_pcg1: p.q, z.q, q.M^-1 q, r.r, r.z; launch it
+                                   // writes partF[it&1]
     // SSOR preconditioning (FP64 only): k_ssor writes z = M^-1 r here and
     // r.z into sc->rhoHist[it & 1]; k_pcg_dir then reads z instead of
     // forming r / |D|
     gptr<T> z;
     int ssor, ldsCells;            // ldsCells: k_ssor keeps z in LDS when N <= this
+    // FP64 Jacobi from a nonzero start x_0 (warm != 0): k_pcg1_start forms
+    // r = b - A x_0 and the start partials
+    gptr<const T> b;
+    int warm, pad;
 };
 
 template <typename T> __global__ void k_pcg_dir(const PcgArgs<T> *__restrict__ A, int it);
@@ -244,6 +249,12 @@ template <bool LDS> __global__ void k_ssor(const PcgArgs<double> *__restrict__ A
 // the set-up launch, it = 0; W = lanes columns per lane): cX, cY hold the
 // scaled coefficients, r and q the two residual buffers, D unused
 template <bool First, int W> __global__ void k_cg1(const PcgArgs<float> *__restrict__ A, int it);
+// the FP64 Jacobi PCG, one launch per iteration (First: launch 0; W = lanes
+// columns per lane, 1 or 2): partials in partF, r in r / q and p in pBuf
+// alternating by iteration
+template <bool First, int W> __global__ void k_pcg1(const PcgArgs<double> *__restrict__ A, int it);
+// r = b - A x_0 and the start partials (r.r, r.z, b.b) of a member with warm != 0
+__global__ void k_pcg1_start(const PcgArgs<double> *__restrict__ A);
 
 // FP64 iterative refinement around the FP32 solve (mixed precision): sweep
 // s forms x_s = x_{s-1} + d (d: the inner solve's correction) and the true

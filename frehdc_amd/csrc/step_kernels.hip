@@ -198,8 +198,8 @@ __device__ inline double face_coef(double coef, double depthF, double dragF)
     return coef * depthF / (1 + dragF);
 }
 
-// stores the operator row and the solver start of cell c, accumulating b.b
-// and r.z (z = r / D) of the thread's cells
+// stores the operator row and the solver start of cell c, accumulating r.r,
+// r.z (z = r / |D|) and b.b of the thread's cells (r = b from a zero start)
 template <class R>
 __device__ inline void store_row(const StepArgsT<R> &a, int c, double GXP, double GYP, double D, double z,
                                  double (&part)[3])
@@ -208,19 +208,17 @@ __device__ inline void store_row(const StepArgsT<R> &a, int c, double GXP, doubl
     a.cY[c] = GYP;
     a.D[c] = D;
     a.b[c] = z;
-    if (a.cX32) {
-        // mixed precision: the refinement starts from 0 or from the current
-        // surface, forms r and the scaled FP32 operator itself.  Start: 0
-        // (the reference's), the current surface, or its linear
-        // extrapolation in time 2 eta^n - eta^(n-1)
-        a.x[c] = a.p.warmStart == 2 ? 2.0 * a.surf[c] - a.surfOld[c] : (a.p.warmStart ? a.surf[c] : 0.0);
-        return;
-    }
+    // the solver's start: 0 (the reference's), the current surface, or its
+    // linear extrapolation in time 2 eta^n - eta^(n-1).  From a nonzero
+    // start the solver forms r = b - A x itself (the mixed path's refinement,
+    // k_pcg1_start for FP64).
+    a.x[c] = a.p.warmStart == 2 ? 2.0 * a.surf[c] - a.surfOld[c] : (a.p.warmStart ? a.surf[c] : 0.0);
+    if (a.cX32 || a.p.warmStart) return;
     a.r[c] = z;
-    a.x[c] = 0.0;
     const double invd = 1.0 / fabs(D);
     part[0] += z * z;
     part[1] += z * (z * invd);
+    part[2] += z * z;
 }
 
 template <class R>
@@ -232,7 +230,7 @@ __global__ void __launch_bounds__(256) k_assemble(const StepArgsT<R> *__restrict
     if (a.ctl->halted) return;
     const Grid g = a.g;
     const P &p = a.p;
-    double part[3] = {0.0, 0.0, 0.0};   // b.b, r.z
+    double part[3] = {0.0, 0.0, 0.0};   // r.r, r.z, b.b
     const int tt = cur_tt(a);
     const double dt = p.dt;
     const double coefx = p.g * (dt * dt / (p.dx * p.dx));
@@ -257,7 +255,7 @@ __global__ void __launch_bounds__(256) k_assemble(const StepArgsT<R> *__restrict
         store_row(a, c, GXP, GYP, D, z, part);
     }
     block_sum<3>(part, lds);
-    store_partials<2>(a.partA, a.nbRes, {part[0], part[1]});
+    store_partials<3>(a.partA, a.nbRes, {part[0], part[1], part[2]});
 }
 
 
@@ -1745,7 +1743,7 @@ __global__ void __launch_bounds__(256) k_assemble_sub(const StepArgsT<R> *__rest
         store_row(a, c, GXP, GYP, D, z, part);
     }
     block_sum<3>(part, lds);
-    store_partials<2>(a.partA, a.nbRes, {part[0], part[1]});
+    store_partials<3>(a.partA, a.nbRes, {part[0], part[1], part[2]});
 }
 
 

@@ -55,13 +55,15 @@ def test_error_paths_need_no_device(lib):
     cfg = abi.FrehdConfig(nx=2, ny=2, Nt=1, dx=1.0, dy=1.0, dt=1.0)
     b = C.c_int64()
     assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == abi.FREHD_EINVAL
-    # SSOR preconditions the FP64 solve (one more vector: z); not the mixed path
+    # SSOR preconditions the FP64 solve (one more vector: z, but not the five
+    # per-tile partials of the one-kernel Jacobi solve, 2 x 5 x (64 + 8)
+    # doubles here); not the mixed path
     cfg = abi.FrehdConfig(nx=64, ny=64, Nt=10, dx=1.0, dy=1.0, dt=1.0, bcType=1, precond=1)
     assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
     with_z = b.value
     cfg = abi.FrehdConfig(nx=64, ny=64, Nt=10, dx=1.0, dy=1.0, dt=1.0, bcType=1, precond=0)
     assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
-    assert with_z - b.value >= 64 * 64 * 8
+    assert with_z - b.value >= 64 * 64 * 8 - 2 * 5 * 72 * 8 - 2 * 256
     cfg = abi.FrehdConfig(nx=64, ny=64, Nt=10, dx=1.0, dy=1.0, dt=1.0, bcType=1, precond=1, precision=1,
                           epsInner=1e-5, maxRefine=4)
     assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == abi.FREHD_EINVAL
@@ -122,5 +124,13 @@ def test_mixed_precision_size_limit(lib):
     assert b"2^29" in lib.frehd_last_error()
     cfg = abi.FrehdConfig(nx=16384, ny=32767, **mixed)
     assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
-    cfg = abi.FrehdConfig(nx=16384, ny=32768, **dict(mixed, precision=abi.PREC_FP64))
+    # the one-kernel FP64 Jacobi solve does the same with its FP64 vectors
+    # (2^28 cells at most); the SSOR path addresses them directly
+    fp64 = dict(mixed, precision=abi.PREC_FP64)
+    cfg = abi.FrehdConfig(nx=16384, ny=16385, **fp64)
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == abi.FREHD_EINVAL
+    assert b"2^28" in lib.frehd_last_error()
+    cfg = abi.FrehdConfig(nx=16384, ny=16384, **fp64)          # exactly 2^28 cells
+    assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
+    cfg = abi.FrehdConfig(nx=16384, ny=32768, **dict(fp64, precond=abi.PRECOND_SSOR))
     assert lib.frehd_dev_footprint(C.byref(cfg), C.byref(b)) == 0
