@@ -104,14 +104,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n, r, budget_s):
+def cpu_baseline(n, r, budget_s, procs=1):
     """Time the reference's own step functions (oracle/_ref, one host core)
     on a bounded sample of the workload: an n x n marsh from the same
     generator (fBm bed, r x r subgrid, tables generated on the device and
     downloaded, uniform Manning n since the reference has no per-cell n),
-    stepping until about budget_s seconds have passed.  Falls back to the C
-    restatement (same algorithm, bit-identical results) when the reference
-    build is absent."""
+    stepping until about budget_s seconds have passed; with procs > 1, then
+    as many processes at once, each on its own copy of the sample (the host
+    cores' aggregate).  Falls back to the C restatement (same algorithm,
+    bit-identical results) when the reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle
@@ -138,10 +139,64 @@ def cpu_baseline(n, r, budget_s):
         model.step(1)
         steps += 1
     dt = time.perf_counter() - t0
-    val = scen.N * steps / dt
-    return {"value": val, "unit": "cell-steps/s", "cores": 1, "kind": kind,
-            "sample": f"{n}x{n} marsh, subgrid {r}x{r} (Ne={scen.cfg['Ne']}), steps 2-{steps + 1}, "
-                      f"Jacobi PCG eps 1e-10, FP64, uniform Manning n, {dt:.1f} s on 1 host core"}
+    model.close()
+    one = {"value": scen.N * steps / dt, "cores": 1, "steps": steps, "seconds": round(dt, 2)}
+    sample = (f"{n}x{n} marsh, subgrid {r}x{r} (Ne={scen.cfg['Ne']}), Jacobi PCG eps 1e-10, FP64, uniform "
+              f"Manning n")
+    if procs <= 1:
+        return {"value": one["value"], "unit": "cell-steps/s", "cores": 1, "kind": kind,
+                "sample": f"{sample}, steps 2-{steps + 1}, {dt:.1f} s on 1 host core"}
+    many = cpu_processes(scen, kind, procs, budget_s)
+    return {"value": many["value"], "unit": "cell-steps/s", "cores": procs, "kind": kind,
+            "sample": f"{sample}: {procs} single-threaded processes each stepping its own copy of the sample "
+                      f"at once for about {budget_s:.0f} s (no halo exchange, so an upper bound of the "
+                      f"reference's MPI strips on as many cores; host CPUs visible: {os.cpu_count()})",
+            "single_core": one, "per_process_steps": many["steps"]}
+
+
+def _cpu_process(path, kind, budget_s, barrier, q):
+    """One process of the multi-core CPU baseline: its own copy of the
+    sample, stepped for budget_s seconds after every process is ready."""
+    import pickle
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    with open(path, "rb") as f:
+        scen = pickle.load(f)
+    model = pyoracle.Reference(scen) if kind == "reference" else pyoracle.Oracle(scen)
+    model.scen = None
+    del scen
+    model.step(1)
+    barrier.wait()
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and steps < 3000:
+        model.step(1)
+        steps += 1
+    q.put((steps, time.perf_counter() - t0))
+    model.close()
+
+
+def cpu_processes(scen, kind, procs, budget_s):
+    """`procs` processes (spawned: they never touch the GPU) stepping copies
+    of the sample together; aggregate cell-steps/s."""
+    import dataclasses
+    import multiprocessing as mp
+    import pickle
+    import tempfile
+    ctx = mp.get_context("spawn")
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        path = os.path.join(tmp, "sample.pkl")
+        with open(path, "wb") as f:
+            # the dataclass fields only (building a model caches its ctypes
+            # structures on the scenario)
+            pickle.dump(dataclasses.replace(scen), f, protocol=5)
+        barrier, q = ctx.Barrier(procs), ctx.Queue()
+        ps = [ctx.Process(target=_cpu_process, args=(path, kind, budget_s, barrier, q)) for _ in range(procs)]
+        for p_ in ps:
+            p_.start()
+        res = [q.get(timeout=budget_s + 600) for _ in ps]
+        for p_ in ps:
+            p_.join()
+    return {"value": sum(scen.N * st / dt for st, dt in res), "steps": sorted(st for st, _ in res)}
 
 
 TILING = None     # (H, remap, cols per lane) given on the command line instead of the autotuner
@@ -173,11 +228,28 @@ def run_form(scen, steps, warmup, tune=True, **over):
     dev.step(4)
     kt = dev.kernel_times()
     dev.enable_timing(False)
+    budget = measure_budget(dev, 10)
     dev.close()
     sec = t1 - t0
     return {"sec": sec, "ms_per_step": sec * 1e3 / steps, "value": scen.N * steps / sec,
             "iters": (st["pcgIterTotal"] - it0) / steps, "sweeps": st["refineLast"],
-            "relres": st["relResLast"], "kt": kt, "fields": fields, "tiling": tiling}
+            "relres": st["relResLast"], "kt": kt, "fields": fields, "tiling": tiling, "budget": budget}
+
+
+def measure_budget(dev, steps):
+    """Conservation over `steps` more steps with the device budgets on
+    (flux-form volume and scalar-mass closure per step, summed in
+    double-double; accounting changes no result): relative residuals of the
+    sum and of the worst step, and what the faces moved (the check is not
+    vacuous)."""
+    dev.set_budget(True)
+    dev.step(steps)
+    b = dev.budget()
+    dev.set_budget(False)
+    return {"steps": b["steps"], "volume_rel_residual": b["volumeRelResidual"],
+            "volume_rel_residual_max_step": b["volumeRelResidualMax"], "volume_moved_m3": b["volumeMoved"],
+            "mass_rel_residual": b["massRelResidual"], "mass_rel_residual_max_step": b["massRelResidualMax"],
+            "mass_moved": b["massMoved"]}
 
 
 def run_batch(members, steps, warmup, **over):
@@ -197,7 +269,8 @@ def run_batch(members, steps, warmup, **over):
     if rc not in (0, abi.FREHD_ECFL):
         abi.check(rc)
     halted = sum(1 for d in devs if d.stats()["halted"])
-    surf = [d.download(["surf"])["surf"][: sc.N] for d, sc in zip(devs, members)]
+    fields = [{k: v[: sc.N] for k, v in d.download(["surf", "uuXP", "vvYP", "depth"]).items()}
+              for d, sc in zip(devs, members)]
     iters = max(d.stats()["pcgIterLast"] for d in devs)
     b.enable_timing(True)
     b.try_step(2)
@@ -207,7 +280,7 @@ def run_batch(members, steps, warmup, **over):
         d.close()
     cells = sum(sc.N for sc in members)
     return {"sec": t1 - t0, "value": cells * steps / (t1 - t0), "ms_per_step": (t1 - t0) * 1e3 / steps,
-            "surf": surf, "iters": iters, "kt": kt, "halted": halted, "tiling": tiling}
+            "fields": fields, "iters": iters, "kt": kt, "halted": halted, "tiling": tiling}
 
 
 def batched_ensemble(n_members, size, steps, warmup):
@@ -232,9 +305,9 @@ def batched_ensemble(n_members, size, steps, warmup):
         o = {"cell_steps_per_s": r["value"], "ms_per_step": r["ms_per_step"], "pcg_iterations_last": r["iters"],
              "halted_members": r["halted"], "tiling": r["tiling"]}
         if ref is None:
-            ref = r["surf"]
+            ref = r["fields"]
         else:
-            o["max_abs_eta_err_m"] = float(max(np.max(np.abs(a - b)) for a, b in zip(r["surf"], ref)))
+            o.update(field_errors(r["fields"], ref))
         out[f] = o
     return {"workload": f"{n_members} members x {size}x{size}, dx=1 m, dt=0.5 s, plain grid, shared fBm "
                         f"base + N(0, 0.05 m) per member, Manning U(0.015, 0.04), tide U(0.3, 0.8) m, "
@@ -247,16 +320,36 @@ def serving_run(count, subgrid):
     reqs = serving.make_requests(count, 20.0, subgrid_r=subgrid, workers=16)
     log(f"bench: {count} serving requests built in {time.perf_counter() - t0:.1f} s")
     out = {}
+    ref = None
     for f, over in (("fp64", {}), ("mixed_warm", {"precision": abi.PREC_MIXED, "warmStart": 2})):
         fresh = [serving.Request(rid=r.rid, arrival=r.arrival, steps=r.steps, scen=r.scen) for r in reqs]
-        st = serving.Server(budget_bytes=200e9, **over).run(fresh)
+        st = serving.Server(budget_bytes=200e9, collect=["surf", "uuXP", "vvYP", "depth"], **over).run(fresh)
         log(f"bench: serving {f}: p50 {st['latency_p50_s']:.2f} s, p99 {st['latency_p99_s']:.2f} s, "
             f"{st['cell_steps_per_s'] / 1e9:.3f} G cell-steps/s")
+        # every request's final state, against the FP64 form's
+        res = [{k: v[: r.scen.N] for k, v in r.result.items()} for r in sorted(fresh, key=lambda q: q.rid)]
+        if ref is None:
+            ref = res
+        else:
+            st.update(field_errors(res, ref))
         out[f] = st
     return {"workload": f"{count} requests, Poisson 20/s, sides log-uniform 64..1024 (multiples of 64), "
                         f"dx=2 m, subgrid {subgrid}x{subgrid}, fBm Hurst U(0.6, 0.9), tide U(0.1, 0.7) m, "
                         f"Manning U(0.015, 0.04), 1-3 scalars, U(200, 2000) steps, HBM budget 200 GB",
             "forms": out}
+
+
+def field_errors(got, ref):
+    """Largest |eta|, |u|, |v| differences and wet-mask mismatches over a
+    list of scenarios' final fields, against the FP64 form's."""
+    import numpy as np
+    e = {"max_abs_eta_err_m": 0.0, "max_abs_u_err": 0.0, "max_abs_v_err": 0.0, "wet_mask_mismatches": 0}
+    for a, b in zip(got, ref):
+        e["max_abs_eta_err_m"] = max(e["max_abs_eta_err_m"], float(np.max(np.abs(a["surf"] - b["surf"]))))
+        e["max_abs_u_err"] = max(e["max_abs_u_err"], float(np.max(np.abs(a["uuXP"] - b["uuXP"]))))
+        e["max_abs_v_err"] = max(e["max_abs_v_err"], float(np.max(np.abs(a["vvYP"] - b["vvYP"]))))
+        e["wet_mask_mismatches"] += int(np.count_nonzero((a["depth"] > 0) != (b["depth"] > 0)))
+    return e
 
 
 def traffic_of(kernels, workload):
@@ -305,7 +398,8 @@ def large_domain(size, r, steps, warmup):
         out[f] = {"cell_steps_per_s": r_["value"], "ms_per_step": r_["ms_per_step"],
                   "pcg_iterations_per_step": round(r_["iters"], 2), "refinement_sweeps": r_["sweeps"],
                   "rel_residual_last": r_["relres"], "tiling": r_["tiling"], "solver_roofline": roof,
-                  "step_roofline": step_roofline(sc.cfg, sc.N, r_, f != "fp64", f != "fp64")}
+                  "step_roofline": step_roofline(sc.cfg, sc.N, r_, f != "fp64", f != "fp64"),
+                  "budget": r_["budget"]}
         log(f"bench: large {f}: {r_['ms_per_step']:.2f} ms/step, {r_['iters']:.1f} iterations/step, "
             f"solver {roof['achieved']:.0f} GB/s ({roof['frac']:.3f} of peak)")
     N = sc.N
@@ -329,6 +423,8 @@ def main():
     ap.add_argument("--subgrid", type=int, default=8, help="subgrid pixels per cell side (0: plain grid)")
     ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="processes of the multi-core CPU baseline (the host cores the GPU's share has)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp64_warm,mixed_cold,mixed_warm",
                     help="precision forms to time (the headline value is the last mixed form)")
@@ -392,7 +488,8 @@ def main():
         o = {"cell_steps_per_s": r["value"], "ms_per_step": r["ms_per_step"],
              "pcg_iterations_per_step": round(r["iters"], 2), "refinement_sweeps": r["sweeps"],
              "rel_residual_last": r["relres"], "tiling": r["tiling"],
-             "step_roofline": step_roofline(scen.cfg, N, r, f.startswith("mixed"), f.endswith("warm"))}
+             "step_roofline": step_roofline(scen.cfg, N, r, f.startswith("mixed"), f.endswith("warm")),
+             "budget": r["budget"]}
         if f != "fp64" and "fp64" in res:
             o["max_abs_eta_err_m"] = err(f, "surf")
             o["max_abs_u_err"] = err(f, "uuXP")
@@ -436,7 +533,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.cpu_size, max(args.subgrid, 1), args.cpu_seconds)
+        procs = min(args.cpu_procs, len(os.sched_getaffinity(0)))
+        cpu = cpu_baseline(args.cpu_size, max(args.subgrid, 1), args.cpu_seconds, procs)
         log("cpu baseline:", json.dumps(cpu))
 
     out = {

@@ -423,6 +423,14 @@ int exec_time_solver(Exec &e, int iters, double *ms)
         for (int it = 0; it <= iters; it++)
             if ((rc = enqueue_inner_iteration(e, 0, it))) return rc;
     } else {
+        // each trial solves from the step's start residual r = b (earlier
+        // trials have iterated r down to rounding noise, on which the
+        // directions can break down and the launches exit at once); a warm
+        // member's k_pcg1_start recomputes it from b
+        for (auto *d : e.members)
+            if (!d->pcg.warm)
+                HIPCHK(hipMemcpyAsync(d->args.r, d->args.b, (size_t)d->g.N * sizeof(double), hipMemcpyDeviceToDevice,
+                                      e.stream));
         for (int it = 0; it <= iters; it++)
             if ((rc = enqueue_pcg_iteration(e, it))) return rc;
     }

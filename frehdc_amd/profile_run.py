@@ -14,7 +14,7 @@ import time
 from . import abi, cases
 from .device import DeviceScenario
 
-FORMS = {"fp64": {}, "mixed_cold": {"precision": abi.PREC_MIXED, "warmStart": 0},
+FORMS = {"fp64": {}, "fp64_warm": {"warmStart": 2}, "mixed_cold": {"precision": abi.PREC_MIXED, "warmStart": 0},
          "mixed_warm": {"precision": abi.PREC_MIXED, "warmStart": 2}}
 
 
@@ -45,8 +45,8 @@ def main():
         d = DeviceScenario(sc, **FORMS[f])
         if tilings:
             H, remap, lanes = ([int(x) for x in tilings[q].split(":")] + [1])[:3]
-            if f == "fp64":
-                lanes = 1
+            if not f.startswith("mixed"):
+                lanes = min(lanes, 2)          # k_pcg1 takes 1 or 2 columns per lane
             d.set_pcg_tiling(H, remap, lanes)
             tiling = {"rows_per_tile": H, "xcd_remap": remap, "cols_per_lane": lanes}
             d.step(3)

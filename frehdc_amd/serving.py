@@ -201,8 +201,12 @@ def make_requests(count, rate, seed=6, lo=64, hi=1024, steps_lo=200, steps_hi=20
     spec = cases.serving_requests(count=count, rate=rate, seed=seed, lo=lo, hi=hi, steps_lo=steps_lo,
                                   steps_hi=steps_hi, subgrid_r=subgrid_r, on_the_fly=on_the_fly)
     if workers > 1:
+        # spawned workers: a fork of a process whose HIP runtime is up would
+        # carry its threads' state, and the pool is shut down (its processes
+        # joined) before this returns
+        import multiprocessing as mp
         from concurrent.futures import ProcessPoolExecutor
-        with ProcessPoolExecutor(max_workers=workers) as ex:
+        with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
             scens = list(ex.map(_build, [(count, rate, seed, lo, hi, steps_lo, steps_hi, subgrid_r, on_the_fly, q)
                                          for q in range(count)]))
     else:

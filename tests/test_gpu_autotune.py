@@ -48,6 +48,27 @@ def test_autotune_keeps_state(make, prec):
     b.close()
 
 
+@pytest.mark.parametrize("warm", [0, 2], ids=["cold", "warm"])
+def test_autotune_times_working_iterations(warm):
+    # the trial iterations time working launches: the tuned time per
+    # iteration is the event-timed k_pcg1 launch's to within a factor of two
+    # (trials that ran on an exhausted residual exit at once, a few times
+    # faster than a working launch at this size)
+    sc = cases.marsh(n=512, Nt=40)
+    print(f"case autotune marsh512 fp64, warmStart {warm}", flush=True)
+    d = DeviceScenario(sc, warmStart=warm)
+    d.step(3)
+    best = d.autotune(iters=20)
+    d.enable_timing(True)
+    d.step(3)
+    kt = d.kernel_times()
+    d.enable_timing(False)
+    print("  best", best, "k_pcg1", kt.get("k_pcg1"), flush=True)
+    ms = kt["k_pcg1"][0]
+    assert 0.5 * ms <= best["ms_per_iteration"] <= 2.0 * ms
+    d.close()
+
+
 def test_batch_autotune():
     print("case batch autotune, two mixed basins", flush=True)
     scs = [cases.basin(nx=96, ny=64, Nt=40), cases.basin(nx=130, ny=40, Nt=40, seed=3)]
