@@ -149,6 +149,24 @@ class FrehdError(RuntimeError):
         self.code = code
 
 
+class FrehdSubgridSpec(C.Structure):
+    """include/frehd_subgrid.h frehd_subgrid_spec"""
+    _fields_ = [("nxFine", C.c_int32), ("nyFine", C.c_int32), ("r", C.c_int32), ("Ne", C.c_int32),
+                ("dxf", C.c_double), ("surfmin", C.c_double), ("dsurf", C.c_double),
+                ("checkBlock", C.c_int32), ("useminVolume", C.c_int32), ("useYh", C.c_int32),
+                ("useCn", C.c_int32), ("n0", C.c_double)]
+
+
+SUBGRID_TABLES = ("V", "Z", "Vxp", "Vxm", "Vyp", "Vym", "Np", "Nm", "Op", "Om", "Nmin", "Nmax", "Omin", "Omax",
+                  "minVxp", "minVxm", "minVyp", "minVym", "YhX", "YhY", "Cn")
+SUBGRID_GEOMETRY = ("bottom", "bottomXP", "bottomYP")
+
+
+class FrehdSubgridOut(C.Structure):
+    """include/frehd_subgrid.h frehd_subgrid_out"""
+    _fields_ = [(n, c_double_p) for n in SUBGRID_TABLES + SUBGRID_GEOMETRY]
+
+
 def load_library(path: str | None = None):
     """Load libfrehd_gpu.so (built in-tree).  Raises if it is missing: the
     device path has no fallback."""
@@ -199,6 +217,10 @@ def load_library(path: str | None = None):
         "frehd_batch_kernel_times": ([V, C.c_int32, C.POINTER(C.c_char_p), c_double_p,
                                       C.POINTER(C.c_int64), c_int32_p], C.c_int),
         "frehd_batch_destroy": ([V], C.c_int),
+        "frehd_subgrid_generate": ([C.POINTER(C.c_double), C.POINTER(FrehdSubgridSpec),
+                                    C.POINTER(FrehdSubgridOut), C.c_int32], C.c_int),
+        "frehd_subgrid_write_text": ([C.c_char_p, C.POINTER(FrehdSubgridSpec), C.POINTER(FrehdSubgridOut)],
+                                     C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -13,7 +13,7 @@ OUT = os.path.join(HERE, "libfrehd_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["api.hip", "exec.hip", "step_kernels.hip", "pcg_kernels.hip", "subgrid_tables.hip", "noalloc.hip",
-           "case_io.cpp"]
+           "case_io.cpp", "subgrid_gen.cpp"]
 
 # -ffp-contract=off: no fused multiply-add, so every FP64 expression rounds as
 # the reference's (x86-64, no FMA) does.  Packed FP32 instructions

@@ -438,6 +438,39 @@ def with_minarea(scen: Scenario, beta: float = 0.5, useminV: int = 1, seed: int 
     return scen
 
 
+def with_generated_tables(scen: Scenario, fine: np.ndarray, useminA: int = 1, useminV: int = 1,
+                          beta: float = 0.5, subdrag: int = 0, check_block: bool = True) -> Scenario:
+    """Replace a table-based subgrid scenario's tables by the generator's
+    with the reference's block check (frehdc_amd/subgrid.py generate():
+    blocked faces, minimum / maximum face areas, minimum half-cell volumes,
+    and the Volp (subdrag 2) or Casas (3, 4) drag tables), from the pixels
+    `fine` the scenario was built on, and switch on the options that read
+    them (useminA, useminV, Config.beta, useSubDrag)."""
+    from . import subgrid as sgm
+    sg, k = scen.subgrid, scen.cfg
+    assert sg is not None and "V" in sg, "needs level tables (not device tables)"
+    r = int(round(k["subR"]))
+    t = sgm.generate(fine, r, k["dx"] / r, k["surfmin"], k["dsurf"], k["Ne"], check_block=check_block,
+                     min_volume=True, yh=subdrag == 2, cn=subdrag in (3, 4))
+    for name, v in t.items():
+        if name not in ("bottom", "bottomXP", "bottomYP"):
+            sg[name] = np.ascontiguousarray(v)
+    k.update(useminA=int(useminA), useminV=int(useminV), beta=float(beta))
+    if subdrag:
+        k["useSubDrag"] = int(subdrag)
+    scen.name += f"_gen{'B' if check_block else ''}_minA{useminA}V{useminV}" + (f"_drag{subdrag}" if subdrag else "")
+    return scen
+
+
+def marsh_subgrid_generated(n=28, r=4, Nt=300, period=1200.0, useminA=1, useminV=1, beta=0.5, subdrag=0,
+                            manning_cell=True, seed=2):
+    """The subgrid marsh on the generator's block-checked tables (see
+    with_generated_tables)."""
+    sc = marsh_subgrid(n=n, r=r, Nt=Nt, period=period, manning_cell=manning_cell, seed=seed)
+    return with_generated_tables(sc, fine_marsh(n, r, seed=seed), useminA=useminA, useminV=useminV, beta=beta,
+                                 subdrag=subdrag)
+
+
 def with_subgrid(scen: Scenario, fine: np.ndarray, r: int, surfmin: float, surfmax: float,
                  dsurf: float = 0.01, device_tables: bool = False, on_the_fly: bool = False) -> Scenario:
     """Turn a scenario into its subgrid form: `fine` holds the pixel bed

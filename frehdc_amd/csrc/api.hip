@@ -251,6 +251,16 @@ inline DirTiling dir_tiling(int nx, int ny, int H, int remap, int target = 1024,
 
 inline size_t max_dir_blocks(int nx, int ny) { return (size_t)((nx + 255) / 256) * ny + 8; }
 
+// FREHD_GRAPHS=0 in the environment when a handle or batch is created: its
+// steps are launched kernel by kernel, never replayed as graphs (for
+// profilers: under rocprofv3 a process that replays the graphs of a second
+// handle has died in hipGraphLaunch)
+inline int graphs_from_env()
+{
+    const char *v = getenv("FREHD_GRAPHS");
+    return !(v && v[0] == '0');
+}
+
 // the most adjacent columns per lane the configuration's solver takes:
 // k_cg1 4 (mixed), k_pcg1 2 (FP64 Jacobi), k_pcg_dir 1 (SSOR)
 inline int max_lanes(const frehd_config &k)
@@ -654,6 +664,7 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     d->self.device = d->device;
     d->self.stream = d->stream;
     d->self.tim = &d->timing;
+    d->self.useGraphs = graphs_from_env();
     HIPCHK(alloc_flags(d->self, 1));
     d->midStep = 0;
     d->tt = 0;
@@ -1502,6 +1513,7 @@ int frehd_batch_create(int32_t capacity, frehd_batch_t **out)
         return fail(FREHD_ENOMEM, "batch allocation failed");
     }
     e.tim = &b->timing;
+    e.useGraphs = graphs_from_env();
     *out = b;
     return 0;
 }

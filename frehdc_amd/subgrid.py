@@ -98,3 +98,59 @@ def tables(fine: np.ndarray, r: int, dxf: float, surfmin: float, dsurf: float, N
     for k, src in (("minVxp", "Vxp"), ("minVxm", "Vxm"), ("minVyp", "Vyp"), ("minVym", "Vym")):
         out[k] = out[src]
     return {k: np.ascontiguousarray(v.reshape(-1)) for k, v in out.items()}
+
+
+def _spec(fine, r, dxf, surfmin, dsurf, Ne, check_block, min_volume, yh, cn, n0):
+    from . import abi
+    fy, fx = fine.shape
+    return abi.FrehdSubgridSpec(nxFine=fx, nyFine=fy, r=r, Ne=Ne, dxf=dxf, surfmin=surfmin, dsurf=dsurf,
+                                checkBlock=int(check_block), useminVolume=int(min_volume), useYh=int(yh),
+                                useCn=int(cn), n0=n0)
+
+
+def generate(fine: np.ndarray, r: int, dxf: float, surfmin: float, dsurf: float, Ne: int, check_block=True,
+             min_volume=True, yh=False, cn=False, n0=0.03, threads=0):
+    """The reference's table generator with its block check (native host
+    code, frehd_subgrid_generate; preProcessing/script_CreateSubAreaWithBlockV16.m
+    with ComputeFaceAreaWithBlockCheck.m): level tables (level-major, N =
+    nx * ny per level) and the cell / face bottoms.  Returns a dict of flat
+    float64 arrays: the ten tables, Nmin Nmax Omin Omax (check_block),
+    minVxp minVxm minVyp minVym (min_volume), YhX YhY (yh), Cn (cn), bottom
+    bottomXP bottomYP.  `fine` is [y, x]."""
+    from . import abi
+    import ctypes as C
+    lib = abi.load_library()
+    fine = np.ascontiguousarray(fine, dtype=np.float64)
+    fy, fx = fine.shape
+    N = (fy // r) * (fx // r)
+    want = list(abi.SUBGRID_TABLES[:10])
+    if check_block:
+        want += ["Nmin", "Nmax", "Omin", "Omax"]
+    if min_volume:
+        want += ["minVxp", "minVxm", "minVyp", "minVym"]
+    if yh:
+        want += ["YhX", "YhY"]
+    if cn:
+        want += ["Cn"]
+    arrays = {k: np.zeros(N * Ne) for k in want}
+    arrays.update({k: np.zeros(N) for k in abi.SUBGRID_GEOMETRY})
+    out = abi.FrehdSubgridOut(**{k: v.ctypes.data_as(abi.c_double_p) for k, v in arrays.items()})
+    spec = _spec(fine, r, dxf, surfmin, dsurf, Ne, check_block, min_volume, yh, cn, n0)
+    abi.check(lib.frehd_subgrid_generate(fine.ctypes.data_as(abi.c_double_p), C.byref(spec), C.byref(out),
+                                         int(threads)))
+    return arrays
+
+
+def write_text(folder: str, tabs: dict, fine_shape, r: int, dxf: float, surfmin: float, dsurf: float, Ne: int):
+    """Write `tabs` (generate()'s dict) as the text files initAllSubVar reads
+    (frehd_subgrid_write_text: "<folder>/<name>.dat", one "%f" line per
+    value)."""
+    from . import abi
+    import ctypes as C
+    lib = abi.load_library()
+    fy, fx = fine_shape
+    spec = abi.FrehdSubgridSpec(nxFine=fx, nyFine=fy, r=r, Ne=Ne, dxf=dxf, surfmin=surfmin, dsurf=dsurf)
+    keep = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in tabs.items()
+            if k in abi.SUBGRID_TABLES + abi.SUBGRID_GEOMETRY}
+    out = abi.FrehdSubgridOut(**{k: v.ctypes.data_as(abi.c_double_p) for k, v in keep.items()})
+    abi.check(lib.frehd_subgrid_write_text(folder.encode(), C.byref(spec), C.byref(out)))

@@ -1,5 +1,6 @@
 """The device library builds for the CPU-only container and exports exactly
-the C ABI declared in include/frehd_gpu.h; the ctypes mirror matches the
+the C ABI declared in include/frehd_gpu.h, frehd_io.h and frehd_subgrid.h;
+the ctypes mirror matches the
 compiled struct sizes.  No device call is made here."""
 import ctypes as C
 import os
@@ -11,13 +12,18 @@ import pytest
 from frehdc_amd import abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "frehd_gpu.h")
+# the headers whose functions libfrehd_gpu.so implements (frehd_seam.h's are
+# the seam library's)
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("frehd_gpu.h", "frehd_io.h", "frehd_subgrid.h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(frehd_[a-z_0-9]+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(frehd_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
 
 
 @pytest.fixture(scope="module")

@@ -59,10 +59,18 @@ def minarea_sub(useminV):
     return cases.with_minarea(cases.marsh_subgrid(n=28, r=4, Nt=300, period=1200.0), 0.5, useminV)
 
 
+def generated_sub(**kw):
+    """The generator's block-checked tables (minimum areas and volumes, drag
+    tables) from the marsh's own pixels."""
+    return cases.marsh_subgrid_generated(n=28, r=4, Nt=300, period=1200.0, **kw)
+
+
 def sub_cases():
     return [cases.marsh_subgrid(n=32, r=4, Nt=300, period=1200.0),
             cases.marsh_subgrid(n=24, r=8, Nt=300, period=1200.0, manning_cell=False), tvd_sub(), wind_sub(),
-            volp_sub(), casas_sub(), minarea_sub(0), minarea_sub(2)]
+            volp_sub(), casas_sub(), minarea_sub(0), minarea_sub(2), generated_sub(useminV=2),
+            generated_sub(useminA=0, useminV=0, subdrag=2), generated_sub(useminA=0, useminV=0, subdrag=4,
+                                                                        manning_cell=False)]
 
 
 def test_minarea_effective_volume_pow(dev_mod, oracle_built):

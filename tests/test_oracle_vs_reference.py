@@ -25,6 +25,10 @@ def marsh_sub():
     return cases.marsh_subgrid(n=20, r=4, Nt=200, period=1200.0, manning_cell=False)
 
 
+def gen_sub(**kw):
+    return cases.marsh_subgrid_generated(n=20, r=4, Nt=200, period=1200.0, manning_cell=False, **kw)
+
+
 SUB_FIELDS = ["V", "Z", "Vx", "Vy", "Zx", "Zy", "Nx", "Oy", "Vxp", "Vxm", "Vyp", "Vym", "Np",
               "Nm", "Op", "Om", "minVx", "minVy"]
 
@@ -54,6 +58,12 @@ CASES = [
     ("marsh_subgrid4_minA_V0", lambda: cases.with_minarea(marsh_sub(), 0.8, 0), {}),
     ("marsh_subgrid4_minA_V2", lambda: cases.with_minarea(marsh_sub(), 0.3, 2), {}),
     ("marsh_subgrid4_minV2", lambda: cases.with_minarea(marsh_sub(), 0.0, 2), {"useminA": 0}),
+    # the generator's own tables (block check, minimum areas and volumes,
+    # drag tables) in the reference's step
+    ("marsh_subgrid4_gen_minA_V1", lambda: gen_sub(useminV=1), {}),
+    ("marsh_subgrid4_gen_minA_V2", lambda: gen_sub(useminV=2), {}),
+    ("marsh_subgrid4_gen_volp", lambda: gen_sub(useminA=0, useminV=0, subdrag=2), {}),
+    ("marsh_subgrid4_gen_casas", lambda: gen_sub(useminA=0, useminV=0, subdrag=3), {}),
 ]
 
 
