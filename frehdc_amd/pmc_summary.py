@@ -41,6 +41,8 @@ def short(name):
     n = name.split("(")[0].replace("void ", "").replace("frehd::", "")
     n = re.sub(r"k_cg1<false(, \d)?>", "k_cg1_f32", n)
     n = re.sub(r"k_cg1<true(, \d)?>", "k_cg1_f32_setup", n)
+    n = re.sub(r"k_pcg1<false(, \d)?>", "k_pcg1", n)
+    n = re.sub(r"k_pcg1<true(, \d)?>", "k_pcg1_setup", n)
     return n.replace("<double>", "").replace("<float>", "_f32")
 
 
