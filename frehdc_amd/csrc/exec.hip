@@ -15,7 +15,7 @@ namespace frehd {
 const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble", "k_pcg_dir",
     "k_pcg_res", "k_surface", "k_limiter", "k_faces", "k_tidal_velocity",
     "k_finish", "k_scalar", "k_step_end", "k_combine",
-    "k_subarea_init", "k_cg1_f32", "k_cg1_f32_setup", "k_cg1_f32_stop", "k_refine", "k_refine_test", "k_gather_flags",
+    "k_subarea_init", "k_cg1_f32", "k_cg1_f32_setup", "k_cg1_f32_stop", "k_refine", "k_gather_flags",
     "k_pcg_dir_stop", "k_ssor", "k_pcg1", "k_pcg1_setup", "k_pcg1_stop", "k_pcg1_start"};
 
 void Timing::reset()
@@ -30,14 +30,22 @@ Timing::~Timing()
     for (auto e : pool) hipEventDestroy(e);
 }
 
-// the solver flags of every member, gathered into one array so that one copy
-// brings them to the host; a member whose solve has not finished is marked
-// open, so the kernels after the solve stand down until it has
-__global__ void k_gather_flags(const StepArgs *__restrict__ A, int count, int mixed, int32_t *out)
+// the solver flags of every member (one block each), gathered into one
+// array so that one copy brings them to the host; a member whose solve has
+// not finished is marked open, so the kernels after the solve stand down
+// until it has.  testS >= 0: first the outer test of the mixed solve's
+// refinement sweep testS (R: the members' refinement arguments).
+__global__ void __launch_bounds__(256) k_gather_flags(const StepArgs *__restrict__ A, int count, int mixed,
+                                                      int32_t *out, const RefineArgs *__restrict__ R, int testS)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.x;
     if (m >= count) return;
+    __shared__ double lds[16 * 2];
     Ctl *c = A[m].ctl;
+    // (testS is uniform, so every thread takes part in the test's reduction;
+    // thread 0 records it unless the member has halted or converged before)
+    if (testS >= 0) refine_test(R[m], testS, lds, !c->halted && c->ref.doneAt < 0);
+    if (threadIdx.x != 0) return;
     c->open = !c->halted && (mixed ? c->ref.doneAt : c->pcg.doneAt) < 0;
     int32_t *o = out + (size_t)m * F_STRIDE;
     o[F_DONE] = c->pcg.doneAt;
@@ -116,8 +124,10 @@ void remember(Exec &e, const SolveCounts &counts)
 
 int enqueue_gather(Exec &e)
 {
-    LAUNCH(K_GATHER, k_gather_flags, dim3((unsigned)((e.count + 63) / 64)), dim3(64), e.dArgs, e.count, e.mixed,
-           e.hFlagsDev);
+    const int testS = e.mixed ? e.pendTest : -1;
+    e.pendTest = -1;
+    LAUNCH(K_GATHER, k_gather_flags, dim3((unsigned)e.count), dim3(256), e.dArgs, e.count, e.mixed, e.hFlagsDev,
+           e.dRef, testS);
     return 0;
 }
 
@@ -289,28 +299,34 @@ int solve_fp64_finish(Exec &e, int it)
 }
 
 // launch it of the inner FP32 solve after refinement sweep s (iteration
-// it; 0 sets up)
+// it; 0 sets up, and runs the outer test of the sweep before it when one is
+// pending)
 int enqueue_inner_iteration(Exec &e, int s, int it)
 {
     Tag tag(e, 2, s, it);
     const dim3 g = grid1(e.ext.nbDir, e.count), b(256);
+    const int testS = it == 0 ? e.pendTest : -1;
+    if (it == 0) e.pendTest = -1;
+    const RefineArgs *R = e.dRef;
     switch ((it == 0 ? 8 : 0) + e.ext.lanes) {
-    case 9: LAUNCH(K_CG32_SET, (k_cg1<true, 1>), g, b, e.dPcg32, it); break;
-    case 10: LAUNCH(K_CG32_SET, (k_cg1<true, 2>), g, b, e.dPcg32, it); break;
-    case 12: LAUNCH(K_CG32_SET, (k_cg1<true, 4>), g, b, e.dPcg32, it); break;
-    case 1: LAUNCH(K_CG32, (k_cg1<false, 1>), g, b, e.dPcg32, it); break;
-    case 2: LAUNCH(K_CG32, (k_cg1<false, 2>), g, b, e.dPcg32, it); break;
-    case 4: LAUNCH(K_CG32, (k_cg1<false, 4>), g, b, e.dPcg32, it); break;
+    case 9: LAUNCH(K_CG32_SET, (k_cg1<true, 1>), g, b, e.dPcg32, it, R, testS); break;
+    case 10: LAUNCH(K_CG32_SET, (k_cg1<true, 2>), g, b, e.dPcg32, it, R, testS); break;
+    case 12: LAUNCH(K_CG32_SET, (k_cg1<true, 4>), g, b, e.dPcg32, it, R, testS); break;
+    case 1: LAUNCH(K_CG32, (k_cg1<false, 1>), g, b, e.dPcg32, it, R, -1); break;
+    case 2: LAUNCH(K_CG32, (k_cg1<false, 2>), g, b, e.dPcg32, it, R, -1); break;
+    case 4: LAUNCH(K_CG32, (k_cg1<false, 4>), g, b, e.dPcg32, it, R, -1); break;
     default: return fail(FREHD_EINVAL, "columns per lane %d", e.ext.lanes);
     }
     return 0;
 }
 
+// refinement sweep s; its outer test runs in the next launch (the inner
+// solve's set-up or the flag gather), which takes it from pendTest
 int enqueue_refine(Exec &e, int s)
 {
     Tag tag(e, 3, s, 0);
     LAUNCH(K_REFINE, k_refine, grid1(e.ext.nbRef, e.count), dim3(256), e.dRef, s);
-    LAUNCH(K_RTEST, k_refine_test, dim3(1, 1, (unsigned)e.count), dim3(256), e.dRef, s);
+    e.pendTest = s;
     return 0;
 }
 
@@ -337,6 +353,7 @@ int enqueue_inner_solve(Exec &e, int s, int n)
 int solve_mixed_pattern(Exec &e, int *sweeps)
 {
     int rc, s = 0;
+    e.pendTest = -1;
     if ((rc = enqueue_refine(e, 0))) return rc;
     for (int k = 0; k < e.mixPlan.n; k++) {
         if ((rc = enqueue_inner_solve(e, s, e.mixPlan.v[k]))) return rc;
@@ -416,6 +433,7 @@ int exec_time_solver(Exec &e, int iters, double *ms)
         ~Swap() { e.tim = saved; }
     } swap{e, e.tim};
     e.tim = &local;
+    e.pendTest = -1;
     int rc;
     if (e.mixed) {
         for (int q = 0; q < 4; q++)

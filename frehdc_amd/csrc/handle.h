@@ -34,7 +34,7 @@ int fail(int code, const char *fmt, ...);
 
 enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_TIDEV,
                 K_FINISH, K_SCALAR, K_END, K_COMBINE, K_SUBINIT,
-                K_CG32, K_CG32_SET, K_CG32_STOP, K_REFINE, K_RTEST, K_GATHER, K_PQ_STOP, K_SSOR,
+                K_CG32, K_CG32_SET, K_CG32_STOP, K_REFINE, K_GATHER, K_PQ_STOP, K_SSOR,
                 K_PCG1, K_PCG1_SET, K_PCG1_STOP, K_PCG1_START, K_COUNT,
                 K_IDLE = -1 };
 extern const char *kKernelNames[K_COUNT];
@@ -125,6 +125,7 @@ struct Exec {
     hipGraphExec_t tail = nullptr;
     // place in the solve of the launches being enqueued (see TimedLaunch)
     int tagKind = 0, tagS = 0, tagIt = 0;
+    int pendTest = -1;                  // a refinement sweep whose outer test the next launch runs
     // capture only what repeats: steps since the last refresh, and how often
     // each launch pattern without a graph has come up
     int stableSteps = 0;

@@ -619,7 +619,8 @@ __device__ inline void advance(float R, float q, float P, float al, float be, fl
 }  // namespace
 
 template <bool First, int W>
-__global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ A, int it)
+__global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ A, int it,
+                                             const RefineArgs *__restrict__ R, int testS)
 {
     it = First ? 0 : it;
     const PcgArgs<float> &a = A[blockIdx.z];
@@ -628,10 +629,26 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
     PcgScalars *sc = a.sc;
     // the three stop flags in one round trip (most launches of a pattern
     // past convergence end here)
-    const int halted = *a.halted, outer = *a.outerDone, doneAt = sc->doneAt;
+    const int halted = *a.halted;
+    int outer = *a.outerDone, doneAt = sc->doneAt;
+    double eps = a.epsDyn ? *a.epsDyn : a.eps;
+    if (First && testS >= 0) {
+        // the outer test of the sweep before this solve, in every thread of
+        // every block (testS is uniform; the test reads only the sweep's
+        // partials): converged, or this solve's tolerance and a fresh start.
+        // Thread 0 of block 0 records it, unless the member has halted or
+        // converged before.  A thread that read the control block after that
+        // record saw the convergence this test finds, or the unchanged
+        // outer flag; the inner flag it replaces.
+        const RefTest t = refine_test(R[blockIdx.z], testS, lds, blockIdx.x == 0 && !halted && outer < 0);
+        if (t.conv) outer = testS;
+        else {
+            doneAt = -1;
+            eps = t.epsIn;
+        }
+    }
     if (halted || outer >= 0 || (doneAt >= 0 && doneAt < it)) return;
 
-    const double eps = a.epsDyn ? *a.epsDyn : a.eps;
     double alphaD = 0.0, betaD = 0.0;
     bool stop;
     if (it == 0) {
@@ -822,44 +839,6 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
     }
     block_sum<4>(part, lds);
     if (!stop) store_partials<4>(a.partF[it & 1], a.nbDir, {part[0], part[1], part[2], part[3]});
-}
-
-// k_refine_test(s), one block: the outer stop test |b - A x_s| < eps |b|
-// (the reference's criterion, rtc.c:54, on the FP64 residual) or s at the
-// sweep cap; otherwise resets the inner solve with tolerance
-// max(epsInner, eps |b| / (2 |r|)), so the last inner solve stops about when
-// the outer test will pass.
-__global__ void __launch_bounds__(256) k_refine_test(const RefineArgs *__restrict__ A, int s)
-{
-    const RefineArgs &a = A[blockIdx.z];
-    __shared__ double lds[16 * 2];
-    Ctl *ctl = a.ctl;
-    if (ctl->halted || ctl->ref.doneAt >= 0) return;
-    double v[2];
-    reduce_partials<2>(a.partC, a.nb, v, lds);
-    if (threadIdx.x != 0) return;
-    RefineScalars &R = ctl->ref;
-    if (s == 0) R.bNorm = sqrt(v[0]);
-    const double bNorm = R.bNorm, rNorm = sqrt(v[1]);
-    if (s > 0) {
-        const int n = ctl->pcgIn.iter;
-        if (s - 1 < 8) R.innerIters[s - 1] = n;
-        ctl->pcg.iter += n;
-    }
-    R.rNorm = rNorm;
-    R.sweep = s;
-    const bool conv = rNorm < a.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm)) ||
-                      s >= a.maxRefine;
-    if (conv) {
-        R.doneAt = s;
-        ctl->pcg.rNorm = rNorm;
-        ctl->pcg.bNorm = bNorm;
-        ctl->refineLast = s;
-    } else {
-        ctl->pcgIn.doneAt = -1;
-        ctl->pcgIn.iter = 0;
-        R.epsIn = fmax(a.epsInner, 0.5 * a.eps * bNorm / rNorm);
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1218,12 +1197,12 @@ template __global__ void k_pcg_dir<double>(const PcgArgs<double> *__restrict__, 
 template __global__ void k_pcg_res<double>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_ssor<true>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_ssor<false>(const PcgArgs<double> *__restrict__, int);
-template __global__ void k_cg1<true, 1>(const PcgArgs<float> *__restrict__, int);
-template __global__ void k_cg1<false, 1>(const PcgArgs<float> *__restrict__, int);
-template __global__ void k_cg1<true, 2>(const PcgArgs<float> *__restrict__, int);
-template __global__ void k_cg1<false, 2>(const PcgArgs<float> *__restrict__, int);
-template __global__ void k_cg1<true, 4>(const PcgArgs<float> *__restrict__, int);
-template __global__ void k_cg1<false, 4>(const PcgArgs<float> *__restrict__, int);
+template __global__ void k_cg1<true, 1>(const PcgArgs<float> *__restrict__, int, const RefineArgs *__restrict__, int);
+template __global__ void k_cg1<false, 1>(const PcgArgs<float> *__restrict__, int, const RefineArgs *__restrict__, int);
+template __global__ void k_cg1<true, 2>(const PcgArgs<float> *__restrict__, int, const RefineArgs *__restrict__, int);
+template __global__ void k_cg1<false, 2>(const PcgArgs<float> *__restrict__, int, const RefineArgs *__restrict__, int);
+template __global__ void k_cg1<true, 4>(const PcgArgs<float> *__restrict__, int, const RefineArgs *__restrict__, int);
+template __global__ void k_cg1<false, 4>(const PcgArgs<float> *__restrict__, int, const RefineArgs *__restrict__, int);
 template __global__ void k_pcg1<true, 1>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_pcg1<false, 1>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_pcg1<true, 2>(const PcgArgs<double> *__restrict__, int);

@@ -245,10 +245,14 @@ template <typename T> __global__ void k_pcg_res(const PcgArgs<T> *__restrict__ A
 // SSORPrecond with omega = 1 for iteration it, one workgroup per member
 // (LDS: the sweep's vector in dynamic LDS, else in z itself)
 template <bool LDS> __global__ void k_ssor(const PcgArgs<double> *__restrict__ A, int it);
+struct RefineArgs;
 // the inner FP32 solve of the mixed path, one launch per iteration (First:
 // the set-up launch, it = 0; W = lanes columns per lane): cX, cY hold the
-// scaled coefficients, r and q the two residual buffers, D unused
-template <bool First, int W> __global__ void k_cg1(const PcgArgs<float> *__restrict__ A, int it);
+// scaled coefficients, r and q the two residual buffers, D unused.  The
+// set-up launch first runs the outer test of refinement sweep testS when
+// testS >= 0 (R: the members' refinement arguments)
+template <bool First, int W>
+__global__ void k_cg1(const PcgArgs<float> *__restrict__ A, int it, const RefineArgs *__restrict__ R, int testS);
 // the FP64 Jacobi PCG, one launch per iteration (First: launch 0; W = lanes
 // columns per lane, 1 or 2): partials in partF, r in r / q and p in pBuf
 // alternating by iteration
@@ -258,8 +262,9 @@ __global__ void k_pcg1_start(const PcgArgs<double> *__restrict__ A);
 
 // FP64 iterative refinement around the FP32 solve (mixed precision): sweep
 // s forms x_s = x_{s-1} + d (d: the inner solve's correction) and the true
-// residual r = b - A x_s with the FP64 operator; k_refine_test decides
-// whether the solve has converged and sets up the next inner solve.
+// residual r = b - A x_s with the FP64 operator; refine_test (run by the
+// launch after it) decides whether the solve has converged and sets up the
+// next inner solve.
 struct RefineArgs {
     gptr<const double> cX, cY, D, b;
     gptr<double> xbuf[2];          // x_s lives in xbuf[s & 1]
@@ -275,7 +280,53 @@ struct RefineArgs {
 };
 
 __global__ void k_refine(const RefineArgs *__restrict__ A, int s);
-__global__ void k_refine_test(const RefineArgs *__restrict__ A, int s);
+
+// The outer stop test after refinement sweep s: |b - A x_s| < eps |b| (the
+// reference's criterion, rtc.c:54, on the FP64 residual) or s at the sweep
+// cap; otherwise the next inner solve's tolerance max(epsInner,
+// eps |b| / (2 |r|)), so the last inner solve stops about when the outer
+// test will pass.  Every thread of the block takes part (the partials of
+// k_refine(s) are reduced in a fixed order, so every block that runs it
+// obtains the same result); thread 0 of the block with `record` writes it
+// to the control block.  Run by the launch after k_refine(s): the inner
+// solve's set-up launch, or the flag gather.
+struct RefTest {
+    bool conv;
+    double epsIn;
+};
+
+__device__ inline RefTest refine_test(const RefineArgs &a, int s, double *lds /* 16*2 */, bool record)
+{
+    Ctl *ctl = a.ctl;
+    double v[2];
+    reduce_partials<2>(a.partC, a.nb, v, lds);
+    RefineScalars &R = ctl->ref;
+    const double bNorm = s == 0 ? sqrt(v[0]) : R.bNorm, rNorm = sqrt(v[1]);
+    const bool conv = rNorm < a.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm)) ||
+                      s >= a.maxRefine;
+    const double epsIn = fmax(a.epsInner, 0.5 * a.eps * bNorm / rNorm);
+    if (record && threadIdx.x == 0) {
+        if (s == 0) R.bNorm = bNorm;
+        if (s > 0) {
+            const int n = ctl->pcgIn.iter;
+            if (s - 1 < 8) R.innerIters[s - 1] = n;
+            ctl->pcg.iter += n;
+        }
+        R.rNorm = rNorm;
+        R.sweep = s;
+        if (conv) {
+            R.doneAt = s;
+            ctl->pcg.rNorm = rNorm;
+            ctl->pcg.bNorm = bNorm;
+            ctl->refineLast = s;
+        } else {
+            ctl->pcgIn.doneAt = -1;
+            ctl->pcgIn.iter = 0;
+            R.epsIn = epsIn;
+        }
+    }
+    return RefTest{conv, epsIn};
+}
 
 template <class R>
 __device__ inline gptr<const double> depth_cur(const StepArgsT<R> &a) { return a.ctl->parity ? a.depthB : a.depthA; }

@@ -59,15 +59,15 @@ __device__ inline void limit_y(double &aY, double cflYK, double cflXK, double fY
     else if (cflXK > hi) aY = 0;   // tests CFLx, nsfunctions.c:574
 }
 
+// the predictor's results of one cell: En and Drag of its east and north faces
+struct Pred {
+    double EnX, EnY, DragX, DragY;
+};
+
 template <class R, bool SUB>
-__device__ inline void predictor_body(const StepArgsT<R> &a)
+__device__ inline Pred predict(const StepArgsT<R> &a, int c, int i, int j)
 {
-    if (a.ctl->halted) return;
-    const Grid g = a.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= g.nx || j >= g.ny) return;
-    const int c = j * g.nx + i;
+    const Grid &g = a.g;
     const int e = nbE(g, c, i, j), w = nbW(g, c, i, j), n = nbN(g, c, i, j), s = nbS(g, c, i, j);
     const P &p = a.p;
     const double dx = p.dx, dy = p.dy, dt = p.dt;
@@ -156,14 +156,40 @@ __device__ inline void predictor_body(const StepArgsT<R> &a)
     }
     EnX = EnX / (1 + DragX);
     EnY = EnY / (1 + DragY);
+    return Pred{EnX, EnY, DragX, DragY};
+}
 
-    a.EnXP[c] = EnX;
-    a.EnYP[c] = EnY;
-    a.DragXP[c] = DragX;
-    a.DragYP[c] = DragY;
+// the values as the state stores them (rounded to R): what a later kernel
+// reads back
+template <class R>
+__device__ inline Pred stored(const Pred &q)
+{
+    return Pred{(double)(R)q.EnX, (double)(R)q.EnY, (double)(R)q.DragX, (double)(R)q.DragY};
+}
+
+template <class R>
+__device__ inline void store_pred(const StepArgsT<R> &a, int c, int i, int j, const Pred &q)
+{
+    const Grid &g = a.g;
+    a.EnXP[c] = q.EnX;
+    a.EnYP[c] = q.EnY;
+    a.DragXP[c] = q.DragX;
+    a.DragYP[c] = q.DragY;
     // ghost copies (dragTerm :679-686, dragInversion :739-745)
-    if (j == 0) { a.DragXP[gjM(g, i)] = DragX; a.EnXP[gjM(g, i)] = EnX; }
-    if (i == 0) { a.DragYP[giM(g, j)] = DragY; a.EnYP[giM(g, j)] = EnY; }
+    if (j == 0) { a.DragXP[gjM(g, i)] = q.DragX; a.EnXP[gjM(g, i)] = q.EnX; }
+    if (i == 0) { a.DragYP[giM(g, j)] = q.DragY; a.EnYP[giM(g, j)] = q.EnY; }
+}
+
+template <class R, bool SUB>
+__device__ inline void predictor_body(const StepArgsT<R> &a)
+{
+    if (a.ctl->halted) return;
+    const Grid g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny) return;
+    const int c = j * g.nx + i;
+    store_pred(a, c, i, j, predict<R, SUB>(a, c, i, j));
 }
 
 template <class R>
@@ -198,14 +224,20 @@ __device__ inline double face_coef(double coef, double depthF, double dragF)
     return coef * depthF / (1 + dragF);
 }
 
+// one operator row: G of the east and north faces, the centre (D = -1: a
+// Dirichlet row) and the right-hand side
+struct Row {
+    double GXP, GYP, D, z;
+};
+
 // stores the operator row and the solver start of cell c, accumulating r.r,
 // r.z (z = r / |D|) and b.b of the thread's cells (r = b from a zero start)
 template <class R>
-__device__ inline void store_row(const StepArgsT<R> &a, int c, double GXP, double GYP, double D, double z,
-                                 double (&part)[3])
+__device__ inline void store_row(const StepArgsT<R> &a, int c, const Row &w, double (&part)[3])
 {
-    a.cX[c] = GXP;
-    a.cY[c] = GYP;
+    const double D = w.D, z = w.z;
+    a.cX[c] = w.GXP;
+    a.cY[c] = w.GYP;
     a.D[c] = D;
     a.b[c] = z;
     // the solver's start: 0 (the reference's), the current surface, or its
@@ -221,43 +253,39 @@ __device__ inline void store_row(const StepArgsT<R> &a, int c, double GXP, doubl
     part[2] += z * z;
 }
 
+// the predictor's values one row needs: En, Drag of the cell's east / north
+// faces (c) and of its west (w) and south (s) neighbours' faces
+struct RowEn {
+    double EnXc, EnXw, EnYc, EnYs, DXc, DXw, DYc, DYs;
+};
+
+// one row of the plain operator and its right-hand side (the body of the
+// reference's loops, in their order); tt: the step being assembled
 template <class R>
-__global__ void __launch_bounds__(256) k_assemble(const StepArgsT<R> *__restrict__ A)
+__device__ inline Row assemble_row(const StepArgsT<R> &a, int c, int i, int j, int tt, const RowEn &q)
 {
-    const StepArgsT<R> &a = A[blockIdx.z];
-    if (blockIdx.x >= a.nbRes) return;
-    __shared__ double lds[16 * 3];
-    if (a.ctl->halted) return;
-    const Grid g = a.g;
+    const Grid &g = a.g;
     const P &p = a.p;
-    double part[3] = {0.0, 0.0, 0.0};   // r.r, r.z, b.b
-    const int tt = cur_tt(a);
     const double dt = p.dt;
     const double coefx = p.g * (dt * dt / (p.dx * p.dx));
     const double coefy = p.g * (dt * dt / (p.dy * p.dy));
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += a.nbRes * blockDim.x) {
-        const int i = c % g.nx, j = c / g.nx;
-        const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
-        const double GXP = face_coef(coefx, a.depthXP[c], a.DragXP[c]);
-        const double GXM = face_coef(coefx, a.depthXP[w], a.DragXP[w]);
-        const double GYP = face_coef(coefy, a.depthYP[c], a.DragYP[c]);
-        const double GYM = face_coef(coefy, a.depthYP[s], a.DragYP[s]);
-        double D = 1 + GXP + GYP + GXM + GYM;
+    const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
+    const double GXP = face_coef(coefx, a.depthXP[c], q.DXc);
+    const double GXM = face_coef(coefx, a.depthXP[w], q.DXw);
+    const double GYP = face_coef(coefy, a.depthYP[c], q.DYc);
+    const double GYM = face_coef(coefy, a.depthYP[s], q.DYs);
+    double D = 1 + GXP + GYP + GXM + GYM;
 
-        double z = a.surf[c] - (dt / p.dx) * (a.EnXP[c] * a.depthXP[c] - a.EnXP[w] * a.depthXP[w]) -
-                   (dt / p.dy) * (a.EnYP[c] * a.depthYP[c] - a.EnYP[s] * a.depthYP[s]);
-        const uint8_t fl = a.flags[c];
-        const int ninflow = fl >> 2;
-        for (int q = 0; q < ninflow; q++)
-            z = z + (dt / (p.dx * p.dy)) * (a.inflow[tt] / p.inflowLocLength);
-        if ((fl & 1) && p.bcType != 2) { z = a.tideP[tt] + p.offset; D = -1.0; }
-        if ((fl & 2) && p.bcType != 1) { z = a.tideM[tt] + p.offset; D = -1.0; }
-        store_row(a, c, GXP, GYP, D, z, part);
-    }
-    block_sum<3>(part, lds);
-    store_partials<3>(a.partA, a.nbRes, {part[0], part[1], part[2]});
+    double z = a.surf[c] - (dt / p.dx) * (q.EnXc * a.depthXP[c] - q.EnXw * a.depthXP[w]) -
+               (dt / p.dy) * (q.EnYc * a.depthYP[c] - q.EnYs * a.depthYP[s]);
+    const uint8_t fl = a.flags[c];
+    const int ninflow = fl >> 2;
+    for (int k = 0; k < ninflow; k++)
+        z = z + (dt / (p.dx * p.dy)) * (a.inflow[tt] / p.inflowLocLength);
+    if ((fl & 1) && p.bcType != 2) { z = a.tideP[tt] + p.offset; D = -1.0; }
+    if ((fl & 2) && p.bcType != 1) { z = a.tideM[tt] + p.offset; D = -1.0; }
+    return Row{GXP, GYP, D, z};
 }
-
 
 template <class R>
 __device__ inline void surface_cell(const StepArgsT<R> &a, int k, double &loss);
@@ -1702,50 +1730,84 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgsT<R> *__restri
 // becomes dx*dy; a dry cell with no velocity on its four faces gets the
 // identity-like row dx*dy * x = dx*dy * eta (flagged by a negative D).
 template <class R>
-__global__ void __launch_bounds__(256) k_assemble_sub(const StepArgsT<R> *__restrict__ A)
+__device__ inline Row assemble_row_sub(const StepArgsT<R> &a, int c, int i, int j, int tt, const RowEn &q)
 {
-    const StepArgsT<R> &a = A[blockIdx.z];
-    if (blockIdx.x >= a.nbRes) return;
-    __shared__ double lds[16 * 3];
-    if (a.ctl->halted) return;
-    const Grid g = a.g;
+    const Grid &g = a.g;
     const P &p = a.p;
-    double part[3] = {0.0, 0.0, 0.0};
-    const int tt = cur_tt(a);
     const double dt = p.dt, dxdy = p.dx * p.dy;
     const double cx = p.g * (dt * dt), cy = p.g * (dt * dt);
     const double divider = 1.0;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += a.nbRes * blockDim.x) {
-        const int i = c % g.nx, j = c / g.nx;
-        const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
-        const double Nxc = a.Nx[c], Nxw = a.Nx[w], Oyc = a.Oy[c], Oys = a.Oy[s];
-        const double Vxc = a.Vx[c], Vxw = a.Vx[w], Vyc = a.Vy[c], Vys = a.Vy[s];
-        double GXP = 0, GXM = 0, GYP = 0, GYM = 0;
-        if (Vxc > 0) GXP = cx * Nxc * Nxc / Vxc / divider / (1 + a.DragXP[c]);
-        if (Vxw > 0) GXM = cx * Nxw * Nxw / Vxw / divider / (1 + a.DragXP[w]);
-        if (Vyc > 0) GYP = cy * Oyc * Oyc / Vyc / divider / (1 + a.DragYP[c]);
-        if (Vys > 0) GYM = cy * Oys * Oys / Vys / divider / (1 + a.DragYP[s]);
-        double D = a.Z[c] + GXP + GYP + GXM + GYM;
-        const double hc = a.surf[c];
-        double z = hc * a.Z[c] - dt * (a.EnXP[c] * Nxc - a.EnXP[w] * Nxw) -
-                   dt * (a.EnYP[c] * Oyc - a.EnYP[s] * Oys);
-        const uint8_t fl = a.flags[c];
-        const int ninflow = p.bcType != 3 ? (fl >> 2) : 0;
-        for (int q = 0; q < ninflow; q++) z = z + dt * (a.inflow[tt] / p.inflowLocLength);
-        if (z == 0 || z == hc * p.dx * p.dy) z = hc * p.dx * p.dy;
-        if (D == 0.0 || D == dxdy) D = dxdy;
-        if (depth_cur(a)[c] == 0 && a.uuXP[c] == 0 && a.uuXP[w] == 0 && a.vvYP[c] == 0 && a.vvYP[s] == 0) {
-            D = -dxdy;
-            z = hc * p.dx * p.dy;
+    const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
+    const double Nxc = a.Nx[c], Nxw = a.Nx[w], Oyc = a.Oy[c], Oys = a.Oy[s];
+    const double Vxc = a.Vx[c], Vxw = a.Vx[w], Vyc = a.Vy[c], Vys = a.Vy[s];
+    double GXP = 0, GXM = 0, GYP = 0, GYM = 0;
+    if (Vxc > 0) GXP = cx * Nxc * Nxc / Vxc / divider / (1 + q.DXc);
+    if (Vxw > 0) GXM = cx * Nxw * Nxw / Vxw / divider / (1 + q.DXw);
+    if (Vyc > 0) GYP = cy * Oyc * Oyc / Vyc / divider / (1 + q.DYc);
+    if (Vys > 0) GYM = cy * Oys * Oys / Vys / divider / (1 + q.DYs);
+    double D = a.Z[c] + GXP + GYP + GXM + GYM;
+    const double hc = a.surf[c];
+    double z = hc * a.Z[c] - dt * (q.EnXc * Nxc - q.EnXw * Nxw) - dt * (q.EnYc * Oyc - q.EnYs * Oys);
+    const uint8_t fl = a.flags[c];
+    const int ninflow = p.bcType != 3 ? (fl >> 2) : 0;
+    for (int k = 0; k < ninflow; k++) z = z + dt * (a.inflow[tt] / p.inflowLocLength);
+    if (z == 0 || z == hc * p.dx * p.dy) z = hc * p.dx * p.dy;
+    if (D == 0.0 || D == dxdy) D = dxdy;
+    if (depth_cur(a)[c] == 0 && a.uuXP[c] == 0 && a.uuXP[w] == 0 && a.vvYP[c] == 0 && a.vvYP[s] == 0) {
+        D = -dxdy;
+        z = hc * p.dx * p.dy;
+    }
+    if ((fl & 1) && p.bcType != 2) { z = a.tideP[tt] + p.offset; D = -1.0; }
+    if ((fl & 2) && p.bcType != 1) { z = a.tideM[tt] + p.offset; D = -1.0; }
+    return Row{GXP, GYP, D, z};
+}
+
+// the assembly over the cells, grid-stride over nbRes blocks, reading the
+// predictor's En and Drag back (plain grid and subgrid)
+template <class R, bool SUB>
+__device__ inline void assemble_body(const StepArgsT<R> &a, double *lds)
+{
+    double part[3] = {0.0, 0.0, 0.0};   // r.r, r.z, b.b
+    if (!a.ctl->halted) {
+        const Grid g = a.g;
+        const int tt = cur_tt(a);
+        for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += a.nbRes * blockDim.x) {
+            const int i = c % g.nx, j = c / g.nx;
+            const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
+            RowEn q;
+            q.EnXc = a.EnXP[c];
+            q.EnXw = a.EnXP[w];
+            q.EnYc = a.EnYP[c];
+            q.EnYs = a.EnYP[s];
+            q.DXc = a.DragXP[c];
+            q.DXw = a.DragXP[w];
+            q.DYc = a.DragYP[c];
+            q.DYs = a.DragYP[s];
+            const Row r = SUB ? assemble_row_sub(a, c, i, j, tt, q) : assemble_row(a, c, i, j, tt, q);
+            store_row(a, c, r, part);
         }
-        if ((fl & 1) && p.bcType != 2) { z = a.tideP[tt] + p.offset; D = -1.0; }
-        if ((fl & 2) && p.bcType != 1) { z = a.tideM[tt] + p.offset; D = -1.0; }
-        store_row(a, c, GXP, GYP, D, z, part);
     }
     block_sum<3>(part, lds);
     store_partials<3>(a.partA, a.nbRes, {part[0], part[1], part[2]});
 }
 
+template <class R>
+__global__ void __launch_bounds__(256) k_assemble(const StepArgsT<R> *__restrict__ A)
+{
+    const StepArgsT<R> &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nbRes) return;
+    __shared__ double lds[16 * 3];
+    assemble_body<R, false>(a, lds);
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_assemble_sub(const StepArgsT<R> *__restrict__ A)
+{
+    const StepArgsT<R> &a = A[blockIdx.z];
+    if (blockIdx.x >= a.nbRes) return;
+    __shared__ double lds[16 * 3];
+    assemble_body<R, true>(a, lds);
+}
 
 // both storage precisions of every step kernel
 template __global__ void k_assemble<double>(const StepArgsT<double> *__restrict__);
