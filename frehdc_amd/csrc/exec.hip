@@ -56,6 +56,7 @@ __global__ void __launch_bounds__(256) k_gather_flags(const StepArgs *__restrict
     o[F_INITER] = c->pcgIn.iter;
     o[F_TT] = c->tt;
     for (int k = 0; k < 8; k++) o[F_INNER0 + k] = c->ref.innerIters[k];
+    o[F_INCONV] = c->ref.innerConv;
     __threadfence_system();
     o[F_SEQ] = 1;
 }
@@ -390,17 +391,24 @@ int solve_mixed_finish(Exec &e, int s)
     }
     classify_timed(e);
     // this solve's counts: per sweep, the most inner iterations any member
-    // used, two more where the pattern cut a solve short; the next pattern
-    // covers the last steps' counts sweep by sweep
+    // used, two more where the pattern cut a member's inner solve short (it
+    // ended before its own stop test passed) and the member then needed
+    // another sweep; the next pattern covers the last steps' counts sweep by
+    // sweep
     int sweeps = 0;
     for (int m = 0; m < e.count; m++)
         if (!flags(e, m)[F_HALTED]) sweeps = std::max(sweeps, (int)flags(e, m)[F_REFDONE]);
     SolveCounts plan;
     for (int k = 0; k < sweeps && k < 8; k++) {
         int n = 1;
-        for (int m = 0; m < e.count; m++)
-            if (!flags(e, m)[F_HALTED] && flags(e, m)[F_REFDONE] > k) n = std::max(n, (int)flags(e, m)[F_INNER0 + k]);
-        if (k < e.mixPlan.n && n >= e.mixPlan.v[k]) n = e.mixPlan.v[k] + 2;
+        bool cut = false;
+        for (int m = 0; m < e.count; m++) {
+            const int32_t *f = flags(e, m);
+            if (f[F_HALTED] || f[F_REFDONE] <= k) continue;
+            n = std::max(n, (int)f[F_INNER0 + k]);
+            if (!((f[F_INCONV] >> k) & 1) && f[F_REFDONE] > k + 1) cut = true;
+        }
+        if (k < e.mixPlan.n && cut && n >= e.mixPlan.v[k]) n = e.mixPlan.v[k] + 2;
         plan.v[plan.n++] = n;
     }
     remember(e, plan);

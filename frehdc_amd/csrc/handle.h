@@ -80,7 +80,7 @@ struct Extents {
 // flags gathered from every member's control block after a solver chunk
 // F_SEQ: written last (after a system-scope fence), so a nonzero value tells
 // the host the member's flags are complete
-enum { F_DONE, F_HALTED, F_ITER, F_REFDONE, F_INDONE, F_INITER, F_TT, F_SEQ, F_INNER0, F_STRIDE = 16 };
+enum { F_DONE, F_HALTED, F_ITER, F_REFDONE, F_INDONE, F_INITER, F_TT, F_SEQ, F_INNER0, F_INCONV = F_INNER0 + 8, F_STRIDE = 20 };
 
 struct frehd_dev_impl;
 

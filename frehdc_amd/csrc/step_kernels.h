@@ -63,6 +63,7 @@ struct RefineScalars {
     int sweep;                // last refinement sweep tested
     int doneAt;               // sweep whose residual test succeeded, -1 while running
     int innerIters[8];        // inner iterations of sweeps 0..7 of the last solve
+    int innerConv;            // bit s: the inner solve after sweep s met its own stop test
 };
 
 // device control block (one per handle)
@@ -310,6 +311,7 @@ __device__ inline RefTest refine_test(const RefineArgs &a, int s, double *lds /*
         if (s > 0) {
             const int n = ctl->pcgIn.iter;
             if (s - 1 < 8) R.innerIters[s - 1] = n;
+            if (s - 1 < 31 && ctl->pcgIn.doneAt >= 0) R.innerConv |= 1 << (s - 1);
             ctl->pcg.iter += n;
         }
         R.rNorm = rNorm;

@@ -30,6 +30,7 @@ __device__ inline void step_begin(const StepArgsT<R> &a)
     c->pcg.doneAt = -1;
     c->ref.doneAt = -1;
     c->ref.sweep = 0;
+    c->ref.innerConv = 0;
     c->pcgIn.doneAt = -1;
     c->pcgIn.iter = 0;
     c->pcg.rhoHist[0] = c->pcg.rhoHist[1] = 0.0;
@@ -200,8 +201,10 @@ __global__ void __launch_bounds__(256) k_predictor(const StepArgsT<R> *__restric
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) step_begin(a);
     predictor_body<R, false>(a);
 }
+// (at least 6 waves per SIMD: the compiler then keeps it to 72 registers, 7
+// waves, with no spill; by default it takes 86, 5 waves)
 template <class R>
-__global__ void __launch_bounds__(256) k_predictor_sub(const StepArgsT<R> *__restrict__ A)
+__global__ void __launch_bounds__(256, 6) k_predictor_sub(const StepArgsT<R> *__restrict__ A)
 {
     const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
