@@ -202,11 +202,13 @@ def cpu_processes(scen, kind, procs, budget_s):
 TILING = None     # (H, remap, cols per lane) given on the command line instead of the autotuner
 
 
-def run_form(scen, steps, warmup, tune=True, **over):
-    """Time one precision form: 3 steps, the solver-tiling autotuner, warmup
-    steps, then `steps` timed steps (the call returns after the stream is
-    synchronised), then 4 steps with per-kernel HIP event timing.  Returns
-    the numbers and the final state."""
+def run_form(scen, steps, warmup, tune=True, spinup=0, job=0, **over):
+    """Time one precision form: 3 steps, the solver-tiling autotuner,
+    `spinup` steps of the job from rest, warmup steps, then `steps` timed
+    steps (the call returns after the stream is synchronised); with job > 0,
+    then the rest of a `job`-step run, every step after the autotuner timed
+    for the whole job's average; then 4 steps with per-kernel HIP event
+    timing.  Returns the numbers and the final state."""
     from frehdc_amd import device
     dev = device.DeviceScenario(scen, **over)
     dev.step(3)
@@ -217,13 +219,30 @@ def run_form(scen, steps, warmup, tune=True, **over):
         tiling = {"rows_per_tile": TILING[0], "xcd_remap": TILING[1], "cols_per_lane": lanes}
     else:
         tiling = dev.autotune(20) if tune else None
+    jt = 0.0                     # the job's stepping time after the autotuner
+    if spinup > 0:
+        t0 = time.perf_counter()
+        dev.step(spinup)
+        jt += time.perf_counter() - t0
+    t0 = time.perf_counter()
     dev.step(warmup)
+    jt += time.perf_counter() - t0
     it0 = dev.stats()["pcgIterTotal"]
     t0 = time.perf_counter()
     dev.step(steps)
     t1 = time.perf_counter()
+    jt += t1 - t0
     st = dev.stats()
     fields = dev.download(["surf", "uuXP", "vvYP", "depth"])
+    job_out = None
+    rest = job - (3 + spinup + warmup + steps)
+    if job > 0 and rest >= 0:
+        jit0 = dev.stats()["pcgIterTotal"]
+        t2 = time.perf_counter()
+        dev.step(rest)
+        jt += time.perf_counter() - t2
+        n = job - 3
+        job_out = {"steps": n, "ms_per_step": jt * 1e3 / n, "cell_steps_per_s": scen.N * n / jt}
     dev.enable_timing(True)
     dev.step(4)
     kt = dev.kernel_times()
@@ -233,7 +252,8 @@ def run_form(scen, steps, warmup, tune=True, **over):
     sec = t1 - t0
     return {"sec": sec, "ms_per_step": sec * 1e3 / steps, "value": scen.N * steps / sec,
             "iters": (st["pcgIterTotal"] - it0) / steps, "sweeps": st["refineLast"],
-            "relres": st["relResLast"], "kt": kt, "fields": fields, "tiling": tiling, "budget": budget}
+            "relres": st["relResLast"], "kt": kt, "fields": fields, "tiling": tiling, "budget": budget,
+            "job": job_out}
 
 
 def measure_budget(dev, steps):
@@ -426,6 +446,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--spinup", type=int, default=100,
+                    help="steps of the single-domain job from rest before the warmup and the timed steps")
+    ap.add_argument("--job", type=int, default=2000,
+                    help="steps of the single-domain job whose whole-run average is reported per form (0: none)")
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--subgrid", type=int, default=8, help="subgrid pixels per cell side (0: plain grid)")
     ap.add_argument("--cpu-size", type=int, default=256)
@@ -460,13 +484,13 @@ def main():
     from frehdc_amd import abi, cases
     import numpy as np
 
-    total = args.warmup + args.steps + 12
+    total = max(3 + args.spinup + args.warmup + args.steps, args.job) + 16
 
     t0 = time.perf_counter()
     if args.subgrid > 0:
-        scen = cases.marsh_subgrid(n=args.size, r=args.subgrid, Nt=max(total, 2000), device_tables=True)
+        scen = cases.marsh_subgrid(n=args.size, r=args.subgrid, Nt=max(total, 2016), device_tables=True)
     else:
-        scen = cases.marsh(n=args.size, Nt=max(total, 2000))
+        scen = cases.marsh(n=args.size, Nt=max(total, 2016))
     log(f"bench: scenario built in {time.perf_counter() - t0:.1f} s")
     N = scen.N
     grid = f"subgrid {args.subgrid}x{args.subgrid} (Ne={scen.cfg['Ne']})" if args.subgrid > 0 else "plain grid"
@@ -478,8 +502,8 @@ def main():
     names = [f for f in args.forms.split(",") if f]
     res = {}
     for f in names:
-        log(f"bench: marsh {args.size}^2 {f}, warmup {args.warmup}, steps {args.steps}")
-        r = run_form(scen, args.steps, args.warmup, **forms[f])
+        log(f"bench: marsh {args.size}^2 {f}, spin-up {args.spinup}, warmup {args.warmup}, steps {args.steps}")
+        r = run_form(scen, args.steps, args.warmup, spinup=args.spinup, job=args.job, **forms[f])
         log(f"bench: {f}: {r['ms_per_step']:.3f} ms/step, {r['iters']:.1f} PCG iterations/step, "
             f"{r['sweeps']} refinement sweeps")
         log(f"kernel times {f} (ms/launch, launches):",
@@ -494,7 +518,7 @@ def main():
     for f, r in res.items():
         o = {"cell_steps_per_s": r["value"], "ms_per_step": r["ms_per_step"],
              "pcg_iterations_per_step": round(r["iters"], 2), "refinement_sweeps": r["sweeps"],
-             "rel_residual_last": r["relres"], "tiling": r["tiling"],
+             "rel_residual_last": r["relres"], "tiling": r["tiling"], "job": r["job"],
              "step_roofline": step_roofline(scen.cfg, N, r, f.startswith("mixed"), f.endswith("warm")),
              "budget": r["budget"]}
         if f != "fp64" and "fp64" in res:
@@ -503,6 +527,10 @@ def main():
             o["max_abs_v_err"] = err(f, "vvYP")
             o["wet_mask_mismatches"] = int(np.count_nonzero(
                 (r["fields"]["depth"][:N] > 0) != (res["fp64"]["fields"]["depth"][:N] > 0)))
+        if f != "fp64" and "fp64" in res:
+            o["vs_fp64"] = round(r["value"] / res["fp64"]["value"], 3)
+            if r["job"] and res["fp64"]["job"]:
+                o["job_vs_fp64"] = round(r["job"]["cell_steps_per_s"] / res["fp64"]["job"]["cell_steps_per_s"], 3)
         forms_out[f] = o
     mixed = [f for f in names if f.startswith("mixed")]
     head = mixed[-1] if mixed else names[-1]
@@ -551,7 +579,11 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
         "config": {"workload": f"marsh {args.size}x{args.size}, dx=2 m, {grid}, dt=0.5 s, fBm bed "
                                f"[-1, 1.5] m, tide 0.6 m / 44712 s, Manning n U(0.02, 0.04) per cell, "
-                               f"one scalar, Jacobi PCG eps=1e-10, form {head}", "cells": N,
+                               f"one scalar, Jacobi PCG eps=1e-10, form {head}; timed: steps "
+                               f"{4 + args.spinup + args.warmup}-{3 + args.spinup + args.warmup + args.steps} "
+                               f"of a {args.job or 3 + args.spinup + args.warmup + args.steps}-step run from rest "
+                               f"(forms.*.job: the average over all of its steps after the autotuner)",
+                   "cells": N,
                    "form": head, "forms": forms_out},
         "roofline": roof,
         "large": large,
