@@ -544,7 +544,7 @@ def main():
     ks, roof = solver_roofline(hr["kt"], N, head.startswith("mixed"))
     traffic = traffic_of(ks, f"marsh{args.size}_sub{args.subgrid}")
     roof["traffic"] = traffic
-    if traffic:
+    if traffic and roof["ms_per_iteration"] > 0:
         roof["traffic_GBps"] = round(traffic / (roof["ms_per_iteration"] * 1e-3) / 1e9, 1)
     # where the step's time goes in the kernel-timed steps: the solver
     # iteration's share, and the kernel with the largest share (whose bytes

@@ -206,6 +206,9 @@ def load_library(path: str | None = None):
                                     C.POINTER(C.c_int64), c_int32_p], C.c_int),
         "frehd_abi_sizes": ([C.POINTER(C.c_int64)], C.c_int),
         "frehd_dev_set_pcg_tiling": ([V, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+        "frehd_dev_set_fused_solve": ([V, C.c_int32], C.c_int),
+        "frehd_batch_set_fused_solve": ([V, C.c_int32], C.c_int),
+        "frehd_fused_solve_capacity": ([C.c_int32, c_int32_p], C.c_int),
         "frehd_dev_upload": ([V, C.POINTER(FrehdFields)], C.c_int),
         "frehd_dev_download_tables": ([V, C.POINTER(c_double_p)], C.c_int),
         "frehd_batch_create": ([C.c_int32, C.POINTER(V)], C.c_int),

@@ -393,6 +393,15 @@ size_t layout(frehd_dev *d, char *base)
     d->self.dPcg = cv.take<PcgArgs<double>>(1);
     d->self.dPcg32 = cv.take<PcgArgs<float>>(1);
     d->self.dRef = cv.take<RefineArgs>(1);
+    // the fused mixed solve's plan and barrier state (zeroed with the arena)
+    d->self.dSync = nullptr;
+    d->self.dPlan = nullptr;
+    d->self.planCap = 0;
+    if (k.precision == FREHD_PREC_MIXED) {
+        d->self.dSync = cv.take<SolveSync>(1);
+        d->self.dPlan = cv.take<SolveBlock>(kPlanCap);
+        d->self.planCap = kPlanCap;
+    }
     // the argument blocks of frehd_dev_step_noalloc (partial sums in the
     // library's static scratch array instead of these)
     d->naArgs = cv.take<StepArgs>(1);
@@ -1298,6 +1307,41 @@ int frehd_dev_stats(frehd_dev_t *d, frehd_stats *out)
     return 0;
 }
 
+int frehd_dev_set_fused_solve(frehd_dev_t *d, int32_t max_blocks)
+{
+    if (!d) return fail(FREHD_EINVAL, "handle is NULL");
+    if (int rc = state_check(d, CALL_TILING)) return rc;
+    HIPCHK(hipSetDevice(d->device));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    d->self.fusedMax = max_blocks;
+    // the launched solve's pattern history starts afresh
+    d->self.mixPlan = SolveCounts();
+    d->self.nRecent = 0;
+    return upload_member_args(d);
+}
+
+int frehd_batch_set_fused_solve(frehd_batch_t *b, int32_t max_blocks)
+{
+    if (!b) return fail(FREHD_EINVAL, "batch is NULL");
+    HIPCHK(hipSetDevice(b->e.device));
+    HIPCHK(hipStreamSynchronize(b->e.stream));
+    b->e.fusedMax = max_blocks;
+    b->e.mixPlan = SolveCounts();
+    b->e.nRecent = 0;
+    return exec_refresh(b->e);
+}
+
+int frehd_fused_solve_capacity(int32_t cols_per_lane, int32_t *blocks)
+{
+    if (!blocks) return fail(FREHD_EINVAL, "blocks is NULL");
+    if (cols_per_lane != 1 && cols_per_lane != 2 && cols_per_lane != 4)
+        return fail(FREHD_EINVAL, "cols_per_lane %d not 1, 2 or 4", cols_per_lane);
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    *blocks = fused_capacity(dev, cols_per_lane);
+    return 0;
+}
+
 int frehd_dev_set_budget(frehd_dev_t *d, int32_t on)
 {
     if (!d) return fail(FREHD_EINVAL, "handle is NULL");
@@ -1508,10 +1552,15 @@ int frehd_batch_create(int32_t capacity, frehd_batch_t **out)
         hipMalloc(&e.dPcg, n * sizeof(PcgArgs<double>)) != hipSuccess ||
         hipMalloc(&e.dPcg32, n * sizeof(PcgArgs<float>)) != hipSuccess ||
         hipMalloc(&e.dRef, n * sizeof(RefineArgs)) != hipSuccess ||
+        hipMalloc(&e.dSync, sizeof(SolveSync)) != hipSuccess ||
+        hipMalloc(&e.dPlan, kPlanCap * sizeof(SolveBlock)) != hipSuccess ||
         alloc_flags(e, n) != hipSuccess) {
         frehd_batch_destroy(b);
         return fail(FREHD_ENOMEM, "batch allocation failed");
     }
+    e.planCap = kPlanCap;
+    HIPCHK(hipMemsetAsync(e.dSync, 0, sizeof(SolveSync), e.stream));
+    HIPCHK(hipStreamSynchronize(e.stream));
     e.tim = &b->timing;
     e.useGraphs = graphs_from_env();
     *out = b;
@@ -1655,6 +1704,8 @@ int frehd_batch_destroy(frehd_batch_t *b)
     if (e.dPcg) hipFree(e.dPcg);
     if (e.dPcg32) hipFree(e.dPcg32);
     if (e.dRef) hipFree(e.dRef);
+    if (e.dSync) hipFree(e.dSync);
+    if (e.dPlan) hipFree(e.dPlan);
     if (e.hFlags) hipHostFree(e.hFlags);
     if (e.stream) hipStreamDestroy(e.stream);
     delete b;

@@ -16,7 +16,7 @@ const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble"
     "k_pcg_res", "k_surface", "k_limiter", "k_faces", "k_tidal_velocity",
     "k_finish", "k_scalar", "k_step_end", "k_combine",
     "k_subarea_init", "k_cg1_f32", "k_cg1_f32_setup", "k_cg1_f32_stop", "k_refine", "k_gather_flags",
-    "k_pcg_dir_stop", "k_ssor", "k_pcg1", "k_pcg1_setup", "k_pcg1_stop", "k_pcg1_start"};
+    "k_pcg_dir_stop", "k_ssor", "k_pcg1", "k_pcg1_setup", "k_pcg1_stop", "k_pcg1_start", "k_solve_mixed"};
 
 void Timing::reset()
 {
@@ -47,18 +47,7 @@ __global__ void __launch_bounds__(256) k_gather_flags(const StepArgs *__restrict
     if (testS >= 0) refine_test(R[m], testS, lds, !c->halted && c->ref.doneAt < 0);
     if (threadIdx.x != 0) return;
     c->open = !c->halted && (mixed ? c->ref.doneAt : c->pcg.doneAt) < 0;
-    int32_t *o = out + (size_t)m * F_STRIDE;
-    o[F_DONE] = c->pcg.doneAt;
-    o[F_HALTED] = c->halted;
-    o[F_ITER] = c->pcg.iter;
-    o[F_REFDONE] = c->ref.doneAt;
-    o[F_INDONE] = c->pcgIn.doneAt;
-    o[F_INITER] = c->pcgIn.iter;
-    o[F_TT] = c->tt;
-    for (int k = 0; k < 8; k++) o[F_INNER0 + k] = c->ref.innerIters[k];
-    o[F_INCONV] = c->ref.innerConv;
-    __threadfence_system();
-    o[F_SEQ] = 1;
+    write_flags(c, out + (size_t)m * F_STRIDE);
 }
 
 namespace {
@@ -421,6 +410,49 @@ int solve_mixed_finish(Exec &e, int s)
     return 0;
 }
 
+// The whole mixed-precision solve of every member, its surface update and
+// its flags in one persistent launch (k_solve_mixed; the group's plan gives
+// each block its member and share).
+int enqueue_fused_solve(Exec &e)
+{
+    const auto *a32 = reinterpret_cast<const StepArgsT<float> *>(e.dArgs);
+    const dim3 g((unsigned)e.planBlocks), b(256);
+    switch (e.ext.lanes) {
+    case 1: LAUNCH(K_SOLVE, (k_solve_mixed<1>), g, b, e.dRef, e.dPcg32, a32, e.dPlan, e.dSync, e.hFlagsDev, e.spinTicks); break;
+    case 2: LAUNCH(K_SOLVE, (k_solve_mixed<2>), g, b, e.dRef, e.dPcg32, a32, e.dPlan, e.dSync, e.hFlagsDev, e.spinTicks); break;
+    case 4: LAUNCH(K_SOLVE, (k_solve_mixed<4>), g, b, e.dRef, e.dPcg32, a32, e.dPlan, e.dSync, e.hFlagsDev, e.spinTicks); break;
+    default: return fail(FREHD_EINVAL, "columns per lane %d", e.ext.lanes);
+    }
+    return 0;
+}
+
+// A fused solve that did not deliver every member's flags: its barrier
+// timed out (the blocks were not resident together) or it did not finish.
+// The barrier state is cleared for the next launch; the step fails.
+int fused_failed(Exec &e, int rc)
+{
+    (void)hipStreamSynchronize(e.stream);
+    SolveSync st{};
+    (void)hipMemcpy(&st, e.dSync, sizeof(SolveSync), hipMemcpyDeviceToHost);
+    (void)hipMemsetAsync(e.dSync, 0, sizeof(SolveSync), e.stream);
+    (void)hipStreamSynchronize(e.stream);
+    if (st.err)
+        return fail(FREHD_EHIP, "fused solve: a grid barrier timed out (its %d blocks were not resident together; "
+                                "run fewer fused solves at once, or switch it off)", e.planBlocks);
+    return rc ? rc : fail(FREHD_EHIP, "fused solve: a member's solve did not finish");
+}
+
+// after a fused solve: every member's flags arrived and every solve that
+// ran converged
+int fused_check(Exec &e)
+{
+    int rc = wait_flags(e);
+    if (rc) return fused_failed(e, rc);
+    for (int m = 0; m < e.count; m++)
+        if (!flags(e, m)[F_HALTED] && flags(e, m)[F_REFDONE] < 0) return fused_failed(e, 0);
+    return 0;
+}
+
 }  // namespace
 
 // Average duration (ms) of one solver iteration of the group under its
@@ -533,6 +565,12 @@ static int exec_phase_launch(Exec &e, int phase)
         return 0;
     case 1: { // solveFreeSurface
         int rc = enqueue_assemble(e), k = 0;
+        if (!rc && e.planBlocks > 0) {
+            // the fused solve updates the surface too
+            clear_flags(e);
+            if ((rc = enqueue_fused_solve(e))) return rc;
+            return fused_check(e);
+        }
         if (!rc) rc = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
         if (!rc) rc = gather(e);
         if (!rc) rc = e.mixed ? solve_mixed_finish(e, k) : solve_fp64_finish(e, k);
@@ -621,10 +659,11 @@ static int exec_step_eager(Exec &e)
     return 0;
 }
 
-// enqueue the kernels after the solve (surface, update, scalars, CFL test)
+// enqueue the kernels after the solve (surface -- unless the fused solve
+// wrote it --, update, scalars, CFL test)
 static int enqueue_tail(Exec &e)
 {
-    LAUNCH_ST(K_SURF, k_surface, grid1(e.ext.nb1, e.count), dim3(256));
+    if (e.planBlocks == 0) LAUNCH_ST(K_SURF, k_surface, grid1(e.ext.nb1, e.count), dim3(256));
     for (int ph = 2; ph <= 4; ph++) {
         int r = exec_phase(e, ph);
         if (r) return r;
@@ -647,7 +686,9 @@ static int exec_step_graphs(Exec &e)
 {
     const int n = e.count;
     std::vector<int> key;
-    if (e.mixed) {
+    if (e.planBlocks > 0) {
+        key = {2};
+    } else if (e.mixed) {
         key.push_back(1);
         key.insert(key.end(), e.mixPlan.v, e.mixPlan.v + e.mixPlan.n);
     } else {
@@ -675,8 +716,12 @@ static int exec_step_graphs(Exec &e)
             if (e.sub) LAUNCH_ST(K_PRED, k_predictor_sub, g2, b2);
             else LAUNCH_ST(K_PRED, k_predictor, g2, b2);
             int r = enqueue_assemble(e);
-            if (!r) r = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
-            if (!r) r = enqueue_gather(e);
+            if (e.planBlocks > 0) {
+                if (!r) r = enqueue_fused_solve(e);
+            } else {
+                if (!r) r = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
+                if (!r) r = enqueue_gather(e);
+            }
             if (!r) r = enqueue_tail(e);
             return r;
         }, &step);
@@ -689,6 +734,12 @@ static int exec_step_graphs(Exec &e)
     }
     clear_flags(e);
     HIPCHK(hipGraphLaunch(step, e.stream));
+    if (e.planBlocks > 0) {
+        // the fused solve finishes every solve on the device
+        if ((rc = fused_check(e))) return rc;
+        for (auto *m : e.members) m->tt += 1;
+        return 0;
+    }
     if ((rc = wait_flags(e))) return rc;
     bool open = false;
     for (int m = 0; m < n; m++)
@@ -786,6 +837,49 @@ static void merge(Exec &e, const frehd_dev_impl *d, bool first)
     e.nScalarMax = std::max(e.nScalarMax, d->args.p.nScalar);
 }
 
+int fused_capacity(int device, int lanes)
+{
+    const void *k = lanes == 4 ? reinterpret_cast<const void *>(k_solve_mixed<4>)
+                  : lanes == 2 ? reinterpret_cast<const void *>(k_solve_mixed<2>)
+                               : reinterpret_cast<const void *>(k_solve_mixed<1>);
+    int per = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 256, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    return per * cus;
+}
+
+// The fused solve's plan for a mixed-precision group: each member gets
+// blocks in proportion to its inner-solve tiles (one tile each when the
+// device holds a block per tile of every member), within what the device
+// holds at once (and fusedMax); a group with more members than blocks keeps
+// the launched solve.
+static int plan_fused(Exec &e)
+{
+    e.planBlocks = 0;
+    if (!e.mixed || e.fusedMax < 0 || e.count == 0 || !e.dPlan || !e.dSync) return 0;
+    int cap = std::min(fused_capacity(e.device, e.ext.lanes), e.planCap);
+    if (e.fusedMax > 0) cap = std::min(cap, e.fusedMax);
+    if (cap < e.count) return 0;
+    long long total = 0;
+    for (auto *d : e.members) total += d->pcg32.nbDir;
+    e.hPlan.clear();
+    for (int m = 0; m < e.count; m++) {
+        const int nb = e.members[m]->pcg32.nbDir;
+        long long B = total <= cap ? nb : std::max<long long>(1, (long long)nb * (cap - e.count) / total);
+        B = std::min<long long>(B, nb);
+        for (int j = 0; j < (int)B; j++) e.hPlan.push_back(SolveBlock{m, j, (int)B, 0});
+    }
+    if ((int)e.hPlan.size() > cap) return fail(FREHD_EHIP, "fused solve plan of %zu blocks exceeds %d",
+                                               e.hPlan.size(), cap);
+    HIPCHK(hipMemcpyAsync(e.dPlan, e.hPlan.data(), e.hPlan.size() * sizeof(SolveBlock), hipMemcpyHostToDevice,
+                          e.stream));
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e.device) != hipSuccess || khz <= 0) khz = 100000;
+    e.spinTicks = (unsigned long long)khz * 2000ull;   // a barrier waits 2 s at most
+    e.planBlocks = (int)e.hPlan.size();
+    return 0;
+}
+
 int exec_refresh(Exec &e)
 {
     exec_release_graphs(e);
@@ -810,6 +904,7 @@ int exec_refresh(Exec &e)
         HIPCHK(hipMemcpyAsync(e.dPcg32 + m, &d->pcg32, sizeof(PcgArgs<float>), hipMemcpyHostToDevice, e.stream));
         HIPCHK(hipMemcpyAsync(e.dRef + m, &d->ref, sizeof(RefineArgs), hipMemcpyHostToDevice, e.stream));
     }
+    if (int rc = plan_fused(e)) return rc;
     HIPCHK(hipStreamSynchronize(e.stream));
     return 0;
 }

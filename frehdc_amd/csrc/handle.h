@@ -35,7 +35,7 @@ int fail(int code, const char *fmt, ...);
 enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_TIDEV,
                 K_FINISH, K_SCALAR, K_END, K_COMBINE, K_SUBINIT,
                 K_CG32, K_CG32_SET, K_CG32_STOP, K_REFINE, K_GATHER, K_PQ_STOP, K_SSOR,
-                K_PCG1, K_PCG1_SET, K_PCG1_STOP, K_PCG1_START, K_COUNT,
+                K_PCG1, K_PCG1_SET, K_PCG1_STOP, K_PCG1_START, K_SOLVE, K_COUNT,
                 K_IDLE = -1 };
 extern const char *kKernelNames[K_COUNT];
 
@@ -76,11 +76,6 @@ struct Extents {
     int ssorLds;    // every SSOR member fits k_ssor's LDS vector
     int budget;     // a member accounts its conservation budgets
 };
-
-// flags gathered from every member's control block after a solver chunk
-// F_SEQ: written last (after a system-scope fence), so a nonzero value tells
-// the host the member's flags are complete
-enum { F_DONE, F_HALTED, F_ITER, F_REFDONE, F_INDONE, F_INITER, F_TT, F_SEQ, F_INNER0, F_INCONV = F_INNER0 + 8, F_STRIDE = 20 };
 
 struct frehd_dev_impl;
 
@@ -130,7 +125,24 @@ struct Exec {
     // each launch pattern without a graph has come up
     int stableSteps = 0;
     std::vector<std::pair<std::vector<int>, int>> seen;
+    // fused mixed-precision solve (k_solve_mixed): fusedMax < 0 off (the
+    // default), 0 as many blocks as the device holds at once, > 0 at most
+    // that many;
+    // planBlocks = the grid of the current plan (0: the launched solve);
+    // the plan (planCap entries) and the barrier state live in device memory
+    int fusedMax = -1;
+    int planBlocks = 0, planCap = 0;
+    SolveBlock *dPlan = nullptr;
+    SolveSync *dSync = nullptr;
+    std::vector<SolveBlock> hPlan;
+    unsigned long long spinTicks = 0;
 };
+
+// plan entries a group's device plan array holds (blocks of one fused solve)
+constexpr int kPlanCap = 4096;
+// co-resident blocks of the fused solve with `lanes` columns per lane on a
+// device (occupancy x compute units)
+int fused_capacity(int device, int lanes);
 
 // drop the captured steps of a group (after its members or their arguments
 // change, and before it is destroyed)

@@ -363,20 +363,14 @@ __device__ inline double diag_scale(double D)
     return (double)rsqrtf((float)fabs(D));
 }
 
-__global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A, int s)
+// refinement sweep s over tile lb (tile column lb % gx, row lb / gx):
+// x_s, r32 (and at s = 0 the scaled coefficients) of the tile's cells; the
+// tile's sums r32.r32, r32.z32, b.b, r.r added to part
+__device__ inline void refine_tile(const RefineArgs &a, int s, int lb, double (&part)[4])
 {
-    const RefineArgs &a = A[blockIdx.z];
-    if ((int)blockIdx.x >= a.nb) return;
-    __shared__ double lds[16 * 4];
-    Ctl *ctl = a.ctl;
-    const int halted = ctl->halted, refDone = ctl->ref.doneAt;
-    if (halted || (refDone >= 0 && refDone < s)) return;
     const double *xin = s == 0 ? a.xbuf[0] : a.xbuf[(s + 1) & 1];   // x_0 or x_{s-1}
     double *xout = a.xbuf[s & 1];
-    int lb = blockIdx.x;
-    if (a.remap) lb = (lb & 7) * (a.nb >> 3) + (lb >> 3);
     const int bx = lb % a.gx, by = lb / a.gx;
-    double part[4] = {0.0, 0.0, 0.0, 0.0};     // r32.r32, r32.z32, b.b, r.r
     if (by < a.gy) {
         const int nx = a.nx, ny = a.ny;
         const int lane = threadIdx.x & 63;
@@ -482,13 +476,31 @@ __global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A
             xn = xnn; Dn = Dnn; bn = bnn;
         }
     }
+}
+
+// the block-reduced sums of a refinement tile into partial slot `slot`
+// (thread 0)
+__device__ inline void refine_store(const RefineArgs &a, int slot, const double (&part)[4])
+{
+    if (threadIdx.x != 0) return;
+    a.partA[slot] = part[0];
+    a.partA[a.nb + slot] = part[1];
+    a.partC[slot] = part[2];
+    a.partC[a.nb + slot] = part[3];
+}
+
+__global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A, int s)
+{
+    const RefineArgs &a = A[blockIdx.z];
+    if ((int)blockIdx.x >= a.nb) return;
+    __shared__ double lds[16 * 4];
+    Ctl *ctl = a.ctl;
+    const int halted = ctl->halted, refDone = ctl->ref.doneAt;
+    if (halted || (refDone >= 0 && refDone < s)) return;
+    double part[4] = {0.0, 0.0, 0.0, 0.0};     // r32.r32, r32.z32, b.b, r.r
+    refine_tile(a, s, tile_of(blockIdx.x, a.nb, a.remap), part);
     block_sum<4>(part, lds);
-    if (threadIdx.x == 0) {
-        a.partA[blockIdx.x] = part[0];
-        a.partA[a.nb + blockIdx.x] = part[1];
-        a.partC[blockIdx.x] = part[2];
-        a.partC[a.nb + blockIdx.x] = part[3];
-    }
+    refine_store(a, blockIdx.x, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -618,63 +630,49 @@ __device__ inline void advance(float R, float q, float P, float al, float be, fl
 
 }  // namespace
 
-template <bool First, int W>
-__global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ A, int it,
-                                             const RefineArgs *__restrict__ R, int testS)
-{
-    it = First ? 0 : it;
-    const PcgArgs<float> &a = A[blockIdx.z];
-    if ((int)blockIdx.x >= a.nbDir) return;
-    __shared__ double lds[16 * 4];
-    PcgScalars *sc = a.sc;
-    // the three stop flags in one round trip (most launches of a pattern
-    // past convergence end here)
-    const int halted = *a.halted;
-    int outer = *a.outerDone, doneAt = sc->doneAt;
-    double eps = a.epsDyn ? *a.epsDyn : a.eps;
-    if (First && testS >= 0) {
-        // the outer test of the sweep before this solve, in every thread of
-        // every block (testS is uniform; the test reads only the sweep's
-        // partials): converged, or this solve's tolerance and a fresh start.
-        // Thread 0 of block 0 records it, unless the member has halted or
-        // converged before.  A thread that read the control block after that
-        // record saw the convergence this test finds, or the unchanged
-        // outer flag; the inner flag it replaces.
-        const RefTest t = refine_test(R[blockIdx.z], testS, lds, blockIdx.x == 0 && !halted && outer < 0);
-        if (t.conv) outer = testS;
-        else {
-            doneAt = -1;
-            eps = t.epsIn;
-        }
-    }
-    if (halted || outer >= 0 || (doneAt >= 0 && doneAt < it)) return;
-
-    double alphaD = 0.0, betaD = 0.0;
+// The start of inner iteration it: the stop test on the partials of the
+// launch before (r_0's norm at it = 0; p.q, r.q, q.q, r.r of iteration it-1
+// after), alpha and beta.  Every block evaluates it from the same partials
+// in the same order, so all obtain bitwise the same result; thread 0 of the
+// block with `record` writes the solver's scalars.  bNorm: |r_0| of this
+// solve (read at it >= 2).
+struct Cg1Dec {
     bool stop;
+    double alphaD, betaD, rNorm0;
+};
+
+__device__ inline Cg1Dec cg1_decide(const PcgArgs<float> &a, int it, double eps, double bNorm, double *lds,
+                                    bool record)
+{
+    PcgScalars *sc = a.sc;
+    Cg1Dec d{false, 0.0, 0.0, 0.0};
     if (it == 0) {
         double v[1];
         reduce_partials<1>(a.partA, a.nbStart, v, lds);
         const double rNorm = sqrt(v[0]);
-        stop = (LAS_IS_ZERO(rNorm) && LAS_IS_ONE(1.0 + rNorm)) || a.maxIter <= 0;
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        d.rNorm0 = rNorm;
+        d.stop = (LAS_IS_ZERO(rNorm) && LAS_IS_ONE(1.0 + rNorm)) || a.maxIter <= 0;
+        if (record && threadIdx.x == 0) {
             sc->bb = v[0];
             sc->bNorm = rNorm;
             sc->rr = v[0];
             sc->rNorm = rNorm;
             sc->iter = 0;
-            if (stop) sc->doneAt = 0;
+            if (d.stop) sc->doneAt = 0;
         }
     } else {
         double v[4];
         reduce_partials<4>(a.partF[(it - 1) & 1], a.nbDir, v, lds);
         const double pq = v[0], rq = v[1], qq = v[2], rr = v[3];
         const double rNorm = sqrt(rr);
-        stop = (it >= 2 && (rNorm < eps * sc->bNorm || it - 1 >= a.maxIter)) || !(pq > 0);
-        alphaD = rr / pq;
+        const bool stop = (it >= 2 && (rNorm < eps * bNorm || it - 1 >= a.maxIter)) || !(pq > 0);
+        const double alphaD = rr / pq;
         double rho = rr - 2.0 * alphaD * rq + alphaD * alphaD * qq;
         if (!(rho > 0)) rho = 0;               // cancellation at convergence: restart the directions
-        betaD = rho / rr;
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        d.stop = stop;
+        d.alphaD = alphaD;
+        d.betaD = rho / rr;
+        if (record && threadIdx.x == 0) {
             sc->rr = rr;
             sc->rNorm = rNorm;
             sc->iter = stop ? it - 1 : it;
@@ -686,13 +684,17 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
             }
         }
     }
-    const bool first = it == 0;
-    const float al = (float)alphaD, be = (float)betaD;
+    return d;
+}
 
-    int lb = blockIdx.x;
-    if (a.remap) lb = (lb & 7) * (a.nbDir >> 3) + (lb >> 3);
+// inner iteration it over tile lb (see above): the partials of q_it added to
+// part; with stop at it = 0, the tile's correction is zeroed
+template <bool First, int W>
+__device__ inline void cg1_tile(const PcgArgs<float> &a, int it, float al, float be, bool stop, int lb,
+                                double (&part)[4])
+{
+    constexpr bool first = First;   // it == 0
     const int bx = lb % a.gx, by = lb / a.gx;
-    double part[4] = {0.0, 0.0, 0.0, 0.0};
     if (by < a.gy && (!stop || first)) {
         const int nx = a.nx, ny = a.ny;
         const int lane = threadIdx.x & 63;
@@ -837,8 +839,44 @@ __global__ void __launch_bounds__(256) k_cg1(const PcgArgs<float> *__restrict__ 
             }
         }
     }
+}
+
+template <bool First, int W>
+__global__ void __launch_bounds__(256, 3) k_cg1(const PcgArgs<float> *__restrict__ A, int it,
+                                             const RefineArgs *__restrict__ R, int testS)
+{
+    it = First ? 0 : it;
+    const PcgArgs<float> &a = A[blockIdx.z];
+    if ((int)blockIdx.x >= a.nbDir) return;
+    __shared__ double lds[16 * 4];
+    PcgScalars *sc = a.sc;
+    // the three stop flags in one round trip (most launches of a pattern
+    // past convergence end here)
+    const int halted = *a.halted;
+    int outer = *a.outerDone, doneAt = sc->doneAt;
+    double eps = a.epsDyn ? *a.epsDyn : a.eps;
+    if (First && testS >= 0) {
+        // the outer test of the sweep before this solve, in every thread of
+        // every block (testS is uniform; the test reads only the sweep's
+        // partials): converged, or this solve's tolerance and a fresh start.
+        // Thread 0 of block 0 records it, unless the member has halted or
+        // converged before.  A thread that read the control block after that
+        // record saw the convergence this test finds, or the unchanged
+        // outer flag; the inner flag it replaces.
+        const RefTest t = refine_test(R[blockIdx.z], testS, lds, blockIdx.x == 0 && !halted && outer < 0);
+        if (t.conv) outer = testS;
+        else {
+            doneAt = -1;
+            eps = t.epsIn;
+        }
+    }
+    if (halted || outer >= 0 || (doneAt >= 0 && doneAt < it)) return;
+
+    const Cg1Dec d = cg1_decide(a, it, eps, sc->bNorm, lds, blockIdx.x == 0);
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    cg1_tile<First, W>(a, it, (float)d.alphaD, (float)d.betaD, d.stop, tile_of(blockIdx.x, a.nbDir, a.remap), part);
     block_sum<4>(part, lds);
-    if (!stop) store_partials<4>(a.partF[it & 1], a.nbDir, {part[0], part[1], part[2], part[3]});
+    if (!d.stop) store_partials<4>(a.partF[it & 1], a.nbDir, {part[0], part[1], part[2], part[3]});
 }
 
 // ---------------------------------------------------------------------------
@@ -1193,6 +1231,203 @@ __global__ void __launch_bounds__(256) k_pcg1_start(const PcgArgs<double> *__res
     store_partials<3>(a.partA, a.nbStart, {part[0], part[1], part[2]});
 }
 
+// ---------------------------------------------------------------------------
+// The fused mixed-precision solve: every launch of the solve (k_refine,
+// k_cg1, the flag gather, k_surface) as rounds of one persistent kernel.
+//
+// Rounds and barriers.  In a round each block does the work of one launch
+// of the launched form for its member -- refinement sweep s, inner iteration
+// it, or the surface update -- over its share of the member's tiles, each
+// tile's block-reduced sums stored in the partial slot the launched form's
+// block for that tile uses.  A grid-wide barrier follows; then every block
+// takes the decision the next launch would take at its start (the outer
+// test of sweep s, the inner stop test with alpha and beta of the next
+// iteration) from the same partials in the same fixed order, so every block
+// of a member obtains bitwise the same decision and the results are those of
+// the launched form, bit for bit.  The member's block 0 records the solver
+// scalars and, once the solve has converged, writes the member's flags.
+// Nothing is launched past a member's convergence, and the host does not
+// take part in the solve.
+//
+// The barrier: each block's thread 0, after an agent-scope release fence of
+// every thread (the block's stores written back from its XCD's L2), reads
+// the generation, adds its vote (work in this round) and its arrival; the
+// last arrival resets the arrivals and the next round's votes and advances
+// the generation; the others wait for the generation to move (an acquire
+// load, so the round's stores of every block are visible after it).  A wait
+// longer than spinTicks of the device's wall clock sets the timeout flag,
+// and every block leaves at its next barrier: a launch whose blocks were not
+// all resident ends with the solve unfinished instead of waiting forever.
+// The launch leaves the barrier state as it found it (zero arrivals and
+// votes; the generation is only compared).
+namespace {
+
+enum { PH_REFINE, PH_CG, PH_SURF, PH_DONE };
+
+// returns the votes of this round (0: no block had work), or -1 after a timeout
+__device__ inline int grid_barrier(SolveSync *sy, unsigned nblocks, int round, int vote,
+                                   unsigned long long spinTicks, int *sh)
+{
+    // the block's stores are complete in its XCD's L2 after the workgroup
+    // barrier; thread 0's agent-scope release writes them back, its acquire
+    // after the wait invalidates the stale lines once for the block (the
+    // waiting itself polls with coherent relaxed loads, which leave the
+    // caches alone)
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned g = __hip_atomic_load(&sy->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (vote) __hip_atomic_fetch_add(&sy->votes[round & 1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned arrived = __hip_atomic_fetch_add(&sy->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int err = 0;
+        if (arrived == nblocks - 1) {
+            // every block's stores happen before this release of the
+            // generation: acquire the arrivals first
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __hip_atomic_store(&sy->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&sy->votes[(round + 1) & 1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&sy->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const unsigned long long t0 = wall_clock64();
+            while (__hip_atomic_load(&sy->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                if (__hip_atomic_load(&sy->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    err = 1;
+                    break;
+                }
+                if ((unsigned long long)wall_clock64() - t0 > spinTicks) {
+                    __hip_atomic_store(&sy->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    err = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        sh[0] = err ? -1 : (int)__hip_atomic_load(&sy->votes[round & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    return sh[0];
+}
+
+// block `index` of `count`'s share [lo, hi) of n items
+__device__ inline void share(int n, int index, int count, int &lo, int &hi)
+{
+    lo = (int)((long long)n * index / count);
+    hi = (int)((long long)n * (index + 1) / count);
+}
+
+}  // namespace
+
+template <int W>
+__global__ void __launch_bounds__(256) k_solve_mixed(const RefineArgs *__restrict__ R,
+                                                     const PcgArgs<float> *__restrict__ P,
+                                                     const StepArgsT<float> *__restrict__ S,
+                                                     const SolveBlock *__restrict__ plan, SolveSync *sync,
+                                                     int32_t *flagsOut, unsigned long long spinTicks)
+{
+    const SolveBlock pb = plan[blockIdx.x];
+    const int m = pb.member;
+    const RefineArgs &ra = R[m];
+    const PcgArgs<float> &pa = P[m];
+    const StepArgsT<float> &sa = S[m];
+    __shared__ double lds[16 * 4];
+    __shared__ int sh[2];
+    Ctl *ctl = ra.ctl;
+    const bool rec = pb.index == 0;        // the member's recording block
+    const bool halted = ctl->halted != 0;
+    if (rec && threadIdx.x == 0) {
+        // until the surface is written, the kernels after the solve stand
+        // down (a launch that ends at a timeout leaves the member open)
+        ctl->open = halted ? 0 : 1;
+        if (halted) write_flags(ctl, flagsOut + (size_t)m * F_STRIDE);
+    }
+    int ph = halted ? PH_DONE : PH_REFINE;
+    int s = 0, it = 0;
+    double eps = 0.0, bNorm = 0.0;
+    float al = 0.0f, be = 0.0f;
+    bool stop0 = false;                     // the inner solve stopped at its set-up
+    int lo, hi;
+    for (int round = 0;; round++) {
+        if (ph == PH_REFINE) {
+            share(ra.nb, pb.index, pb.count, lo, hi);
+            for (int lb = lo; lb < hi; lb++) {
+                double part[4] = {0.0, 0.0, 0.0, 0.0};
+                refine_tile(ra, s, lb, part);
+                block_sum<4>(part, lds);
+                refine_store(ra, slot_of(lb, ra.nb, ra.remap), part);
+            }
+        } else if (ph == PH_CG) {
+            share(pa.nbDir, pb.index, pb.count, lo, hi);
+            for (int lb = lo; lb < hi; lb++) {
+                double part[4] = {0.0, 0.0, 0.0, 0.0};
+                if (it == 0) cg1_tile<true, W>(pa, it, al, be, stop0, lb, part);
+                else cg1_tile<false, W>(pa, it, al, be, stop0, lb, part);
+                block_sum<4>(part, lds);
+                if (!stop0 && threadIdx.x == 0) {
+                    const int slot = slot_of(lb, pa.nbDir, pa.remap);
+                    double *pf = pa.partF[it & 1];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) pf[(size_t)q * pa.nbDir + slot] = part[q];
+                }
+            }
+        } else if (ph == PH_SURF) {
+            // the surface from the converged sweep's x (x_s in xbuf[s & 1]),
+            // in the chunks of 256 indices of k_surface, whose partials it
+            // writes
+            const gptr<const double> xs = (s & 1) ? sa.xB : sa.x;
+            const int nb1 = (sa.g.Ng + 255) / 256;
+            share(nb1, pb.index, pb.count, lo, hi);
+            for (int ch = lo; ch < hi; ch++) {
+                double loss[1] = {0.0};
+                const int k = ch * 256 + (int)threadIdx.x;
+                if (k < sa.g.Ng) surface_cell(sa, k, loss[0], xs);
+                block_sum<1>(loss, lds);
+                if (threadIdx.x == 0) sa.vpart[2 * sa.maxBlocks + ch] = loss[0];
+            }
+            if (rec && threadIdx.x == 0) {
+                ctl->open = 0;
+                write_flags(ctl, flagsOut + (size_t)m * F_STRIDE);
+            }
+        }
+        const int total = grid_barrier(sync, gridDim.x, round, ph == PH_REFINE || ph == PH_CG, spinTicks, sh);
+        if (total <= 0) break;
+        // the decision the next launch would take at its start
+        if (ph == PH_REFINE) {
+            const RefTest t = refine_test(ra, s, lds, rec);
+            if (t.conv) {
+                ph = PH_SURF;
+            } else {
+                ph = PH_CG;
+                it = 0;
+                eps = t.epsIn;
+                const Cg1Dec d = cg1_decide(pa, 0, eps, 0.0, lds, rec);
+                stop0 = d.stop;
+                bNorm = d.rNorm0;
+                al = be = 0.0f;
+            }
+        } else if (ph == PH_CG) {
+            bool ended = it == 0 && stop0;
+            if (!ended) {
+                const Cg1Dec d = cg1_decide(pa, it + 1, eps, bNorm, lds, rec);
+                if (d.stop) {
+                    ended = true;
+                } else {
+                    it += 1;
+                    al = (float)d.alphaD;
+                    be = (float)d.betaD;
+                }
+            }
+            if (ended) {
+                ph = PH_REFINE;
+                s += 1;
+                stop0 = false;
+            }
+        } else if (ph == PH_SURF) {
+            ph = PH_DONE;
+        }
+    }
+}
+
 template __global__ void k_pcg_dir<double>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_pcg_res<double>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_ssor<true>(const PcgArgs<double> *__restrict__, int);
@@ -1207,5 +1442,12 @@ template __global__ void k_pcg1<true, 1>(const PcgArgs<double> *__restrict__, in
 template __global__ void k_pcg1<false, 1>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_pcg1<true, 2>(const PcgArgs<double> *__restrict__, int);
 template __global__ void k_pcg1<false, 2>(const PcgArgs<double> *__restrict__, int);
+#define FREHD_SOLVE_MIXED(W)                                                                             \
+    template __global__ void k_solve_mixed<W>(const RefineArgs *__restrict__, const PcgArgs<float> *__restrict__, \
+                                              const StepArgsT<float> *__restrict__, const SolveBlock *__restrict__, \
+                                              SolveSync *, int32_t *, unsigned long long);
+FREHD_SOLVE_MIXED(1)
+FREHD_SOLVE_MIXED(2)
+FREHD_SOLVE_MIXED(4)
 
 }  // namespace frehd

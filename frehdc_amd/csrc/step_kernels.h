@@ -241,6 +241,20 @@ struct PcgArgs {
     int warm, pad;
 };
 
+// The tile a solver block walks: with remap, launch block vb takes tile
+// (vb mod 8) * nb/8 + vb/8 (nb a multiple of 8), so the blocks the
+// dispatcher hands to one XCD walk one contiguous band of tiles; block vb
+// writes its partials into slot vb.  slot_of is the inverse: the slot of
+// tile lb's partials.
+__host__ __device__ inline int tile_of(int vb, int nb, int remap)
+{
+    return remap ? (vb & 7) * (nb >> 3) + (vb >> 3) : vb;
+}
+__host__ __device__ inline int slot_of(int lb, int nb, int remap)
+{
+    return remap ? (lb % (nb >> 3)) * 8 + lb / (nb >> 3) : lb;
+}
+
 template <typename T> __global__ void k_pcg_dir(const PcgArgs<T> *__restrict__ A, int it);
 template <typename T> __global__ void k_pcg_res(const PcgArgs<T> *__restrict__ A, int it);
 // SSORPrecond with omega = 1 for iteration it, one workgroup per member
@@ -305,7 +319,10 @@ __device__ inline RefTest refine_test(const RefineArgs &a, int s, double *lds /*
     const double bNorm = s == 0 ? sqrt(v[0]) : R.bNorm, rNorm = sqrt(v[1]);
     const bool conv = rNorm < a.eps * bNorm || (LAS_IS_ZERO(bNorm) && LAS_IS_ONE(1.0 + rNorm)) ||
                       s >= a.maxRefine;
-    const double epsIn = fmax(a.epsInner, 0.5 * a.eps * bNorm / rNorm);
+#ifndef FREHD_INNER_MARGIN
+#define FREHD_INNER_MARGIN 0.5
+#endif
+    const double epsIn = fmax(a.epsInner, FREHD_INNER_MARGIN * a.eps * bNorm / rNorm);
     if (record && threadIdx.x == 0) {
         if (s == 0) R.bNorm = bNorm;
         if (s > 0) {
@@ -330,6 +347,52 @@ __device__ inline RefTest refine_test(const RefineArgs &a, int s, double *lds /*
     return RefTest{conv, epsIn};
 }
 
+// Solver flags of a member, written to pinned host memory after its solve
+// (F_STRIDE ints per member).  F_SEQ is written last, after a system-scope
+// fence, so a nonzero value tells the host the member's flags are complete.
+enum { F_DONE, F_HALTED, F_ITER, F_REFDONE, F_INDONE, F_INITER, F_TT, F_SEQ, F_INNER0, F_INCONV = F_INNER0 + 8,
+       F_STRIDE = 20 };
+
+// the member's flags from its control block (one thread)
+__device__ inline void write_flags(const Ctl *c, int32_t *o)
+{
+    o[F_DONE] = c->pcg.doneAt;
+    o[F_HALTED] = c->halted;
+    o[F_ITER] = c->pcg.iter;
+    o[F_REFDONE] = c->ref.doneAt;
+    o[F_INDONE] = c->pcgIn.doneAt;
+    o[F_INITER] = c->pcgIn.iter;
+    o[F_TT] = c->tt;
+    for (int k = 0; k < 8; k++) o[F_INNER0 + k] = c->ref.innerIters[k];
+    o[F_INCONV] = c->ref.innerConv;
+    __threadfence_system();
+    o[F_SEQ] = 1;
+}
+
+// The fused mixed-precision solve (k_solve_mixed): one persistent launch
+// runs a group's whole free-surface solve -- refinement sweeps, inner
+// iterations, the convergence tests, the surface update and the flags --
+// with grid-wide barriers where separate launches had kernel boundaries.
+// Its blocks must be resident together; the host sizes the grid from the
+// kernel's occupancy.  Each block serves one member (plan entry: the
+// member, the block's index among the member's blocks and their count) and
+// walks its share of the member's tiles in every round.
+struct SolveBlock {
+    int member, index, count, pad;
+};
+// Barrier state in device memory, zero between launches: arrivals at the
+// current barrier, barriers passed, blocks that had work in the round (by
+// round parity), and a timeout flag (a barrier that waited longer than the
+// launch's limit -- blocks not resident together -- ends the launch with
+// the solve unfinished, which the host reports as an error).
+struct SolveSync {
+    unsigned count, gen, votes[2], err, pad[3];
+};
+template <int W>
+__global__ void k_solve_mixed(const RefineArgs *__restrict__ R, const PcgArgs<float> *__restrict__ P,
+                              const StepArgsT<float> *__restrict__ S, const SolveBlock *__restrict__ plan,
+                              SolveSync *sync, int32_t *flagsOut, unsigned long long spinTicks);
+
 template <class R>
 __device__ inline gptr<const double> depth_cur(const StepArgsT<R> &a) { return a.ctl->parity ? a.depthB : a.depthA; }
 template <class R>
@@ -343,6 +406,37 @@ template <class R>
 __device__ inline sptr<R> s_cur(const StepArgsT<R> &a) { return a.ctl->sParity ? a.S2 : a.S; }
 template <class R>
 __device__ inline sptr<R> s_nxt(const StepArgsT<R> &a) { return a.ctl->sParity ? a.S : a.S2; }
+
+// Surface update of one index k of the Ng array: getFreeSurface +
+// zeroNegSurf + enforceFreeSurfaceBC + enforceTidalBC (nsfunctions.c:
+// 1020-1031, 436-449, 317-334, 412-433).  xs: the solution (x, or for the
+// mixed path the refinement buffer of the sweep that converged).  A ghost
+// takes the zero-clipped solution of the interior cell it mirrors (the tidal
+// override comes after the ghost fill); Vloss3 (the clip) into loss.
+template <class R>
+__device__ inline void surface_cell(const StepArgsT<R> &a, int k, double &loss, gptr<const double> xs)
+{
+    const Grid &g = a.g;
+    const P &p = a.p;
+    const int tt = a.ctl->tt;
+    const double old = a.surf[k];
+    a.surfOld[k] = old;
+    const int src = ghost_source(g, k, false);
+    const int cell = src < 0 ? k : src;
+    double h = xs[cell];
+    const double zb = a.bottomZ[cell];
+    if (h < zb) {
+        const double diff = zb - h;
+        h = zb;
+        if (src < 0 && tt > 1) loss -= diff * p.dx * p.dy;
+    }
+    if (src < 0) {
+        const uint8_t fl = a.flags[k];
+        if ((fl & 1) && p.bcType != 2 && a.tideP[tt] + p.offset > zb) h = a.tideP[tt] + p.offset;
+        if ((fl & 2) && p.bcType != 1 && a.tideM[tt] + p.offset > zb) h = a.tideM[tt] + p.offset;
+    }
+    a.surf[k] = h;
+}
 
 enum SubTable { T_V, T_Z, T_VXP, T_VXM, T_VYP, T_VYM, T_NP, T_NM, T_OP, T_OM };
 

@@ -290,9 +290,6 @@ __device__ inline Row assemble_row(const StepArgsT<R> &a, int c, int i, int j, i
     return Row{GXP, GYP, D, z};
 }
 
-template <class R>
-__device__ inline void surface_cell(const StepArgsT<R> &a, int k, double &loss);
-
 // ---------------------------------------------------------------------------
 // Surface update: getFreeSurface + zeroNegSurf + enforceFreeSurfaceBC +
 // enforceTidalBC (nsfunctions.c:1020-1031, 436-449, 317-334, 412-433), over
@@ -308,38 +305,14 @@ __global__ void __launch_bounds__(256) k_surface(const StepArgsT<R> *__restrict_
     const Grid g = a.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double loss[1] = {0.0};
-    if (k < g.Ng) surface_cell(a, k, loss[0]);
+    // the solution: x, or for the mixed path the refinement buffer of the
+    // sweep that converged
+    const gptr<const double> xs = (a.cX32 && (a.ctl->ref.doneAt & 1)) ? a.xB : a.x;
+    if (k < g.Ng) surface_cell(a, k, loss[0], xs);
     block_sum<1>(loss, lds);
     if (threadIdx.x == 0) a.vpart[2 * a.maxBlocks + blockIdx.x] = loss[0];
 }
 
-template <class R>
-__device__ inline void surface_cell(const StepArgsT<R> &a, int k, double &loss)
-{
-    const Grid &g = a.g;
-    const P &p = a.p;
-    const int tt = cur_tt(a);
-    const double old = a.surf[k];
-    a.surfOld[k] = old;
-    const int src = ghost_source(g, k, false);
-    const int cell = src < 0 ? k : src;
-    // the solution: x, or for the mixed path the refinement buffer of the
-    // sweep that converged
-    const double *xs = (a.cX32 && (a.ctl->ref.doneAt & 1)) ? a.xB : a.x;
-    double h = xs[cell];
-    const double zb = a.bottomZ[cell];
-    if (h < zb) {
-        const double diff = zb - h;
-        h = zb;
-        if (src < 0 && tt > 1) loss -= diff * p.dx * p.dy;
-    }
-    if (src < 0) {
-        const uint8_t fl = a.flags[k];
-        if ((fl & 1) && p.bcType != 2 && a.tideP[tt] + p.offset > zb) h = a.tideP[tt] + p.offset;
-        if ((fl & 2) && p.bcType != 1 && a.tideM[tt] + p.offset > zb) h = a.tideM[tt] + p.offset;
-    }
-    a.surf[k] = h;
-}
 
 // ---------------------------------------------------------------------------
 // limiterCFL + computeVolume + updateDepth (nsfunctions.c:452-495, 361-370,

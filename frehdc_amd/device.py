@@ -172,6 +172,12 @@ class DeviceScenario:
         abi.check(self.lib.frehd_dev_set_pcg_tiling(self.h, int(rows_per_tile), 1 if xcd_remap else 0,
                                                     int(cols_per_lane)))
 
+    def set_fused_solve(self, max_blocks: int = 0):
+        """Mixed form: the whole solve as one persistent launch (0: as many
+        blocks as the device holds, > 0: at most that many, < 0: off, the
+        launched solve) -- frehd_dev_set_fused_solve."""
+        abi.check(self.lib.frehd_dev_set_fused_solve(self.h, int(max_blocks)))
+
     def autotune(self, iters: int = 20) -> dict:
         """Time the solver under every candidate tiling on the last step's
         system and keep the fastest (frehd_dev_autotune); the state is left
@@ -224,6 +230,10 @@ class Batch:
     def try_step(self, n: int = 1) -> int:
         return self.lib.frehd_batch_step(self.h, int(n))
 
+    def set_fused_solve(self, max_blocks: int = 0):
+        """frehd_batch_set_fused_solve: one fused solve launch for all members."""
+        abi.check(self.lib.frehd_batch_set_fused_solve(self.h, int(max_blocks)))
+
     def autotune(self, iters: int = 20) -> dict:
         """frehd_batch_autotune: one solver tiling for all members."""
         return _autotune(self.lib.frehd_batch_autotune, self.h, iters)
@@ -251,6 +261,14 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def fused_solve_capacity(cols_per_lane: int = 4) -> int:
+    """Blocks of the fused solve the device holds at once."""
+    import ctypes as C
+    n = C.c_int32()
+    abi.check(abi.load_library().frehd_fused_solve_capacity(int(cols_per_lane), C.byref(n)))
+    return n.value
 
 
 def device_count() -> int:

@@ -400,6 +400,27 @@ int frehd_dev_set_pcg_tiling(frehd_dev_t *dev, int32_t rows_per_tile, int32_t xc
 int frehd_dev_autotune(frehd_dev_t *dev, int32_t iters, int32_t *rows_per_tile, int32_t *xcd_remap,
                        int32_t *cols_per_lane, double *ms_per_iteration);
 
+/* Fused solve of the mixed-precision form (no counterpart in the reference):
+ * the whole free-surface solve of a step -- the FP64 refinement sweeps, the
+ * inner FP32 iterations, their convergence tests, the surface update and
+ * the solver flags -- runs as one persistent launch with grid-wide barriers
+ * instead of one launch per sweep or iteration; results are bitwise those of
+ * the launched solve.  Its blocks must be resident on the device together:
+ * max_blocks < 0 switches it off (the launched solve), 0 (the default)
+ * lets it take as many blocks as the device holds at once, > 0 caps it
+ * (the default is off).  A
+ * caller that steps several handles or batches concurrently (one stream
+ * each) divides frehd_fused_solve_capacity among them; a launch whose blocks
+ * were not resident together ends after a 2 s barrier wait with FREHD_EHIP.
+ * The launched solve's pattern history (launch counts of the last solves)
+ * starts afresh.
+ * No effect on FP64 handles; FREHD_ESTATE for a handle in a batch (set it on
+ * the batch: frehd_batch_set_fused_solve). */
+int frehd_dev_set_fused_solve(frehd_dev_t *dev, int32_t max_blocks);
+/* blocks of the fused solve the current device holds at once, for a solve
+ * with cols_per_lane (1, 2, 4) columns per lane */
+int frehd_fused_solve_capacity(int32_t cols_per_lane, int32_t *blocks);
+
 /* ---- batches: many handles advanced by shared launches ---------------------
  * No counterpart in the reference (one scenario per process there).  A batch
  * holds up to `capacity` handles of one device that share useSubgrid and
@@ -422,6 +443,8 @@ int frehd_batch_step(frehd_batch_t *batch, int32_t nsteps);
  * launches are shared). */
 int frehd_batch_autotune(frehd_batch_t *batch, int32_t iters, int32_t *rows_per_tile, int32_t *xcd_remap,
                          int32_t *cols_per_lane, double *ms_per_iteration);
+/* frehd_dev_set_fused_solve for a batch (one fused launch for all members) */
+int frehd_batch_set_fused_solve(frehd_batch_t *batch, int32_t max_blocks);
 int frehd_batch_enable_timing(frehd_batch_t *batch, int32_t on);
 int frehd_batch_kernel_times(frehd_batch_t *batch, int32_t cap, const char **names,
                              double *ms, int64_t *launches, int32_t *n);
