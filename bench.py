@@ -202,11 +202,15 @@ def cpu_processes(scen, kind, procs, budget_s):
 TILING = None     # (H, remap, cols per lane) given on the command line instead of the autotuner
 
 
+BLOCK_TUNE_STEPS = 2   # steps per candidate of the stencil block-shape tuner (6x that many steps run)
+
+
 def run_form(scen, steps, warmup, tune=True, spinup=0, job=0, **over):
-    """Time one precision form: 3 steps, the solver-tiling autotuner,
-    `spinup` steps of the job from rest, warmup steps, then `steps` timed
+    """Time one precision form: 3 steps, the solver-tiling autotuner, the
+    stencil block-shape tuner (whose steps count as the first of the
+    spin-up), `spinup` steps of the job from rest, warmup steps, then `steps` timed
     steps (the call returns after the stream is synchronised); with job > 0,
-    then the rest of a `job`-step run, every step after the autotuner timed
+    then the rest of a `job`-step run, every step after the autotuners timed
     for the whole job's average; then 4 steps with per-kernel HIP event
     timing.  Returns the numbers and the final state."""
     from frehdc_amd import device
@@ -219,7 +223,14 @@ def run_form(scen, steps, warmup, tune=True, spinup=0, job=0, **over):
         tiling = {"rows_per_tile": TILING[0], "xcd_remap": TILING[1], "cols_per_lane": lanes}
     else:
         tiling = dev.autotune(20) if tune else None
-    jt = 0.0                     # the job's stepping time after the autotuner
+    jt = 0.0                     # the job's stepping time after the autotuners
+    tuned = 0                    # steps the block-shape tuner ran
+    if tune and not TILING and spinup >= 6 * BLOCK_TUNE_STEPS:
+        blocks = dev.autotune_blocks(BLOCK_TUNE_STEPS)
+        tuned = 6 * BLOCK_TUNE_STEPS
+        spinup -= tuned
+        if tiling is not None:
+            tiling = dict(tiling, stencil_block=[blocks["block_x"], blocks["block_y"]])
     if spinup > 0:
         t0 = time.perf_counter()
         dev.step(spinup)
@@ -235,13 +246,13 @@ def run_form(scen, steps, warmup, tune=True, spinup=0, job=0, **over):
     st = dev.stats()
     fields = dev.download(["surf", "uuXP", "vvYP", "depth"])
     job_out = None
-    rest = job - (3 + spinup + warmup + steps)
+    rest = job - (3 + tuned + spinup + warmup + steps)
     if job > 0 and rest >= 0:
         jit0 = dev.stats()["pcgIterTotal"]
         t2 = time.perf_counter()
         dev.step(rest)
         jt += time.perf_counter() - t2
-        n = job - 3
+        n = job - 3 - tuned
         job_out = {"steps": n, "ms_per_step": jt * 1e3 / n, "cell_steps_per_s": scen.N * n / jt}
     dev.enable_timing(True)
     dev.step(4)
@@ -273,7 +284,8 @@ def measure_budget(dev, steps):
 
 
 def run_batch(members, steps, warmup, **over):
-    """One batch of ensemble members (shared launches): warmup steps, then
+    """One batch of ensemble members (shared launches): 2 steps, the solver
+    tiling and stencil block-shape tuners (6 steps), warmup steps, then
     `steps` timed steps, then 2 steps with per-kernel event timing."""
     from frehdc_amd import abi, device
     devs = [device.DeviceScenario(sc, **over) for sc in members]
@@ -282,6 +294,8 @@ def run_batch(members, steps, warmup, **over):
         b.add(d)
     b.try_step(2)
     tiling = b.autotune(10)
+    blocks = b.autotune_blocks(1)          # 6 steps
+    tiling = dict(tiling, stencil_block=[blocks["block_x"], blocks["block_y"]])
     b.try_step(warmup)
     t0 = time.perf_counter()
     rc = b.try_step(steps)
@@ -312,7 +326,7 @@ def batched_ensemble(n_members, size, steps, warmup):
     import numpy as np
     t0 = time.perf_counter()
     base = cases.fbm_surface(size, size, 0.7, np.random.default_rng(5), -1.0, 1.5)
-    Nt = warmup + steps + 12
+    Nt = warmup + steps + 18
     members = [cases.ensemble_member(m, n=size, Nt=Nt, base=base) for m in range(n_members)]
     log(f"bench: batch of {n_members} x {size}^2 built in {time.perf_counter() - t0:.1f} s")
     out = {}

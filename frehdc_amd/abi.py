@@ -207,6 +207,8 @@ def load_library(path: str | None = None):
         "frehd_abi_sizes": ([C.POINTER(C.c_int64)], C.c_int),
         "frehd_dev_set_pcg_tiling": ([V, C.c_int32, C.c_int32, C.c_int32], C.c_int),
         "frehd_dev_set_fused_solve": ([V, C.c_int32], C.c_int),
+        "frehd_dev_autotune_blocks": ([V, C.c_int32, c_int32_p, C.POINTER(C.c_double)], C.c_int),
+        "frehd_batch_autotune_blocks": ([V, C.c_int32, c_int32_p, C.POINTER(C.c_double)], C.c_int),
         "frehd_batch_set_fused_solve": ([V, C.c_int32], C.c_int),
         "frehd_fused_solve_capacity": ([C.c_int32, c_int32_p], C.c_int),
         "frehd_dev_upload": ([V, C.POINTER(FrehdFields)], C.c_int),

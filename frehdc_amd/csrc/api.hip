@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -303,7 +304,11 @@ namespace frehd {
 PartialSizes partial_sizes(const frehd_config &k)
 {
     const size_t N = (size_t)k.nx * k.ny, Ng = (size_t)(k.nx + 2) * (k.ny + 2);
-    const size_t nb2d = (size_t)((k.nx + 63) / 64) * ((k.ny + 3) / 4);
+    // 2-D sweeps: blocks of the largest grid any of the 256-thread shapes
+    // (64 x 4, 128 x 2, 256 x 1) gives
+    size_t nb2d = 0;
+    for (int bx : {64, 128, 256})
+        nb2d = std::max(nb2d, (size_t)((k.nx + bx - 1) / bx) * ((k.ny + 256 / bx - 1) / (256 / bx)));
     const size_t nb1d = (Ng + 255) / 256;
     const size_t G = pcg_blocks(N);
     const size_t maxb = std::max(std::max(nb2d, nb1d), G);
@@ -618,8 +623,10 @@ void fill_extents(frehd_dev *d)
 {
     StepArgs &a = d->args;
     const frehd_config &k = d->cfg;
-    a.gx2 = (k.nx + 63) / 64;
-    a.gy2 = (k.ny + 3) / 4;
+    const int bx = d->blockX, by = 256 / bx;
+    a.gx2 = (k.nx + bx - 1) / bx;
+    a.gy2 = (k.ny + by - 1) / by;
+    a.nb2 = a.gx2 * a.gy2;
     a.nbN = (d->g.N + 255) / 256;
     a.nbGhost = (2 * k.nx + 2 * k.ny + 255) / 256;
     const int nt = (k.bcType != 2 ? k.tideLocLengthP : 0) + (k.bcType != 1 ? k.tideLocLengthM : 0);
@@ -668,6 +675,7 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
         HIPCHK(hipMemsetAsync(d->args.hSub, 0xFF, sizeof(double) * d->g.N, d->stream));
     d->args.g = d->g;
     fill_params(d->args.p, d->cfg);
+    d->blockX = 64;
     fill_extents(d);
     set_pcg_args(d, 0, 0);
     d->self.device = d->device;
@@ -1603,6 +1611,12 @@ int frehd_batch_add(frehd_batch_t *b, frehd_dev_t *d)
             if (rc) return rc;
         }
     }
+    // ... and the members' block shape of the stencil kernels
+    if (!b->e.members.empty() && d->blockX != b->e.members[0]->blockX) {
+        d->blockX = b->e.members[0]->blockX;
+        fill_extents(d);
+        if (int rc = upload_member_args(d)) return rc;
+    }
     b->e.members.push_back(d);
     int rc = exec_refresh(b->e);
     if (rc) {
@@ -1675,6 +1689,94 @@ int frehd_batch_autotune(frehd_batch_t *b, int32_t iters, int32_t *rows_per_tile
     for (auto *d : b->e.members)
         if (!rc) rc = upload_member_args(d);
     return rc;
+}
+
+// Block shape of the 2-D stencil kernels chosen by timing whole steps: each
+// candidate (64 x 4, 128 x 2, 256 x 1 cells) runs `steps` steps twice, in two
+// passes, with per-kernel event timing; the one whose predictor, limiter
+// and scalar kernels take the least time per step wins.  step(k) advances
+// the group k steps.  The shape changes no cell's arithmetic (only the order
+// of the block partial sums of the volume losses and budgets), so the model
+// advances as it would with any shape.
+static int autotune_blocks(Exec &e, std::vector<frehd_dev_impl *> members, int steps, int32_t *bestX, double *bestMs,
+                           const std::function<int(int)> &step)
+{
+    if (members.empty()) return fail(FREHD_ESTATE, "nothing to tune");
+    if (steps < 1 || steps > 1000) return fail(FREHD_EINVAL, "steps %d outside [1, 1000]", steps);
+    for (auto *d : members) {
+        Ctl c;
+        COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
+        if (!c.halted && c.tt + 6 * steps > d->cfg.Nt)
+            return fail(FREHD_EINVAL, "tuning steps to %d beyond the forcing series (Nt = %d)", c.tt + 6 * steps,
+                        d->cfg.Nt);
+    }
+    Timing local;
+    local.on = 1;
+    struct Swap {
+        Exec &e;
+        Timing *saved;
+        ~Swap() { e.tim = saved; }
+    } swap{e, e.tim};
+    e.tim = &local;
+    const int cands[3] = {64, 128, 256};
+    double ms[3] = {-1.0, -1.0, -1.0};
+    int rc = 0;
+    for (int pass = 0; pass < 2 && !rc; pass++)
+        for (int q = 0; q < 3 && !rc; q++) {
+            for (auto *d : members) {
+                frehd_dev *dd = static_cast<frehd_dev *>(d);
+                dd->blockX = cands[q];
+                fill_extents(dd);
+                if ((rc = upload_member_args(dd))) break;
+            }
+            if (!rc && &e != &members[0]->self) rc = exec_refresh(e);
+            if (rc) break;
+            local.reset();
+            if ((rc = step(steps))) break;
+            double t = 0.0;
+            for (int k : {K_PRED, K_LIM, K_SCALAR})
+                if ((size_t)k < local.timers.size()) t += local.timers[k].ms;
+            t /= steps;
+            if (t > 0 && (ms[q] < 0 || t < ms[q])) ms[q] = t;
+        }
+    int best = 0;
+    for (int q = 1; q < 3; q++)
+        if (ms[q] > 0 && ms[q] < ms[best]) best = q;
+    for (auto *d : members) {
+        frehd_dev *dd = static_cast<frehd_dev *>(d);
+        dd->blockX = rc ? 64 : cands[best];
+        fill_extents(dd);
+        int r2 = upload_member_args(dd);
+        if (!rc) rc = r2;
+    }
+    if (!rc && &e != &members[0]->self) rc = exec_refresh(e);
+    if (rc) return rc;
+    if (bestX) *bestX = cands[best];
+    if (bestMs) *bestMs = ms[best];
+    return 0;
+}
+
+int frehd_dev_autotune_blocks(frehd_dev_t *d, int32_t steps, int32_t *block_x, double *ms_per_step)
+{
+    if (!d) return fail(FREHD_EINVAL, "handle is NULL");
+    if (int rc = state_check(d, CALL_AUTOTUNE)) return rc;
+    HIPCHK(hipSetDevice(d->device));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return autotune_blocks(d->self, {d}, steps, block_x, ms_per_step, [&](int k) {
+        int rc = frehd_dev_step(d, k);
+        return rc == FREHD_ECFL ? 0 : rc;
+    });
+}
+
+int frehd_batch_autotune_blocks(frehd_batch_t *b, int32_t steps, int32_t *block_x, double *ms_per_step)
+{
+    if (!b) return fail(FREHD_EINVAL, "batch is NULL");
+    HIPCHK(hipSetDevice(b->e.device));
+    HIPCHK(hipStreamSynchronize(b->e.stream));
+    return autotune_blocks(b->e, b->e.members, steps, block_x, ms_per_step, [&](int k) {
+        int rc = frehd_batch_step(b, k);
+        return rc == FREHD_ECFL ? 0 : rc;
+    });
 }
 
 int frehd_batch_enable_timing(frehd_batch_t *b, int32_t on)

@@ -4,6 +4,8 @@
 // the group (grid z = member); see handle.h.
 #include <algorithm>
 #include <atomic>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include <hip/hip_ext.h>
@@ -557,7 +559,8 @@ static int exec_phase_launch(Exec &e, int phase)
     const int n = e.count;
     if (n == 0) return 0;
     const Extents &x = e.ext;
-    const dim3 b2(64, 4), g2((unsigned)x.gx2, (unsigned)x.gy2, (unsigned)n), b1(256);
+    const dim3 b2((unsigned)x.blockX, (unsigned)(256 / x.blockX)), g2((unsigned)x.gx2, (unsigned)x.gy2, (unsigned)n),
+        b1(256);
     switch (phase) {
     case 0:   // solveSourceTerm (up to the RHS, which the assembly computes)
         if (e.sub) LAUNCH_ST(K_PRED, k_predictor_sub, g2, b2);
@@ -711,8 +714,14 @@ static int exec_step_graphs(Exec &e)
         }
         if (++*count < 2) return exec_step_eager(e);
         if (e.heads.size() >= 16) exec_release_graphs(e);
+        if (getenv("FREHD_GRAPH_LOG")) {
+            fprintf(stderr, "frehd: capturing step graph %zu, pattern", e.heads.size());
+            for (int v : key) fprintf(stderr, " %d", v);
+            fprintf(stderr, " (member 0 at step %d)\n", e.count ? e.members[0]->tt + 1 : -1);
+        }
         rc = capture(e, [&] {
-            const dim3 b2(64, 4), g2((unsigned)e.ext.gx2, (unsigned)e.ext.gy2, (unsigned)n);
+            const dim3 b2((unsigned)e.ext.blockX, (unsigned)(256 / e.ext.blockX)),
+                g2((unsigned)e.ext.gx2, (unsigned)e.ext.gy2, (unsigned)n);
             if (e.sub) LAUNCH_ST(K_PRED, k_predictor_sub, g2, b2);
             else LAUNCH_ST(K_PRED, k_predictor, g2, b2);
             int r = enqueue_assemble(e);
@@ -792,6 +801,7 @@ Extents member_extents(const frehd_dev_impl *d)
     Extents x;
     x.gx2 = a.gx2;
     x.gy2 = a.gy2;
+    x.blockX = d->blockX;
     x.nb1 = a.nb1;
     x.nbN = a.nbN;
     x.nbGhost = a.nbGhost;
@@ -898,6 +908,8 @@ int exec_refresh(Exec &e)
             return fail(FREHD_EINVAL, "a group's members must share the preconditioner");
         if (m > 0 && member_extents(d).lanes != e.ext.lanes)
             return fail(FREHD_EINVAL, "a group's members must share the solver's columns per lane");
+        if (m > 0 && d->blockX != e.ext.blockX)
+            return fail(FREHD_EINVAL, "a group's members must share the stencil kernels' block shape");
         merge(e, d, m == 0);
         HIPCHK(hipMemcpyAsync(e.dArgs + m, &d->args, sizeof(StepArgs), hipMemcpyHostToDevice, e.stream));
         HIPCHK(hipMemcpyAsync(e.dPcg + m, &d->pcg, sizeof(PcgArgs<double>), hipMemcpyHostToDevice, e.stream));

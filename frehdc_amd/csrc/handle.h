@@ -70,6 +70,7 @@ struct Timing {
 // launch extents of a group: the maximum over its members
 struct Extents {
     int gx2, gy2, nb1, nbN, nbGhost, nbTide, nbRes, nbDir, nbRef;
+    int blockX;     // the 2-D stencil kernels' block: blockX x (256 / blockX) (shared by the group)
     int lanes;      // columns per lane of k_cg1 / k_pcg1 (shared by the group)
     int ssorN;      // largest N of the members preconditioned with SSOR (0: none)
     int warm;       // an FP64 member starts its solve from a nonzero x
@@ -170,6 +171,7 @@ struct frehd_dev_impl {
     PcgArgs<float> pcg32;               // mixed precision: inner FP32 solve
     RefineArgs ref;                     // mixed precision: FP64 refinement
     int tileH, tileRemap, tileLanes;    // solver tiling as requested (tileH 0: automatic)
+    int blockX;                         // 2-D stencil kernels: blocks of blockX x (256 / blockX) cells
     int midStep;                        // phases run of the step in progress (step_state.h; 0: at a step boundary)
     int tt;                             // host view of the step counter
     Exec self;                          // the handle as a group of one

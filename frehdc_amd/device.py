@@ -172,6 +172,11 @@ class DeviceScenario:
         abi.check(self.lib.frehd_dev_set_pcg_tiling(self.h, int(rows_per_tile), 1 if xcd_remap else 0,
                                                     int(cols_per_lane)))
 
+    def autotune_blocks(self, steps: int = 2) -> dict:
+        """frehd_dev_autotune_blocks: the stencil kernels' block shape by
+        timing; the model advances by 6 * steps steps."""
+        return _autotune_blocks(self.lib.frehd_dev_autotune_blocks, self.h, steps)
+
     def set_fused_solve(self, max_blocks: int = 0):
         """Mixed form: the whole solve as one persistent launch (0: as many
         blocks as the device holds, > 0: at most that many, < 0: off, the
@@ -230,6 +235,11 @@ class Batch:
     def try_step(self, n: int = 1) -> int:
         return self.lib.frehd_batch_step(self.h, int(n))
 
+    def autotune_blocks(self, steps: int = 2) -> dict:
+        """frehd_batch_autotune_blocks: one stencil block shape for all
+        members; they advance by 6 * steps steps."""
+        return _autotune_blocks(self.lib.frehd_batch_autotune_blocks, self.h, steps)
+
     def set_fused_solve(self, max_blocks: int = 0):
         """frehd_batch_set_fused_solve: one fused solve launch for all members."""
         abi.check(self.lib.frehd_batch_set_fused_solve(self.h, int(max_blocks)))
@@ -261,6 +271,13 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def _autotune_blocks(fn, h, steps):
+    import ctypes as C
+    bx, ms = C.c_int32(), C.c_double()
+    abi.check(fn(h, int(steps), C.byref(bx), C.byref(ms)))
+    return {"block_x": bx.value, "block_y": 256 // bx.value, "ms_per_step": ms.value}
 
 
 def fused_solve_capacity(cols_per_lane: int = 4) -> int:

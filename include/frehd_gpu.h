@@ -421,6 +421,18 @@ int frehd_dev_set_fused_solve(frehd_dev_t *dev, int32_t max_blocks);
  * with cols_per_lane (1, 2, 4) columns per lane */
 int frehd_fused_solve_capacity(int32_t cols_per_lane, int32_t *blocks);
 
+/* Block shape of the 2-D stencil kernels (predictor, limiter, scalar
+ * transport; no counterpart in the reference): each candidate -- blocks of
+ * 64 x 4, 128 x 2 and 256 x 1 cells -- runs `steps` whole steps twice with
+ * per-kernel event timing, and the shape whose three kernels take the least
+ * time per step is kept (block_x: 64, 128 or 256; ms_per_step: their time;
+ * either output may be NULL).  The model ADVANCES by 6 * steps steps: the
+ * shape changes no cell's arithmetic, only the order of the block partial
+ * sums of the volume losses and budgets, so these are ordinary steps of the
+ * run.  FREHD_EINVAL when the forcing series ends before them; FREHD_ESTATE
+ * for a handle in a batch (tune the batch). */
+int frehd_dev_autotune_blocks(frehd_dev_t *dev, int32_t steps, int32_t *block_x, double *ms_per_step);
+
 /* ---- batches: many handles advanced by shared launches ---------------------
  * No counterpart in the reference (one scenario per process there).  A batch
  * holds up to `capacity` handles of one device that share useSubgrid and
@@ -443,6 +455,9 @@ int frehd_batch_step(frehd_batch_t *batch, int32_t nsteps);
  * launches are shared). */
 int frehd_batch_autotune(frehd_batch_t *batch, int32_t iters, int32_t *rows_per_tile, int32_t *xcd_remap,
                          int32_t *cols_per_lane, double *ms_per_iteration);
+/* frehd_dev_autotune_blocks for a batch (one shape for all members; the
+ * members advance by 6 * steps steps) */
+int frehd_batch_autotune_blocks(frehd_batch_t *batch, int32_t steps, int32_t *block_x, double *ms_per_step);
 /* frehd_dev_set_fused_solve for a batch (one fused launch for all members) */
 int frehd_batch_set_fused_solve(frehd_batch_t *batch, int32_t max_blocks);
 int frehd_batch_enable_timing(frehd_batch_t *batch, int32_t on);
