@@ -669,6 +669,9 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     if (f->manningCell) layout_manning(d, d->arena, used);
     const int firstStep = 1;   // Ctl.ttNext at tt = 0
     COPYW(d->stream, &d->args.ctl->ttNext, &firstStep, sizeof(int), hipMemcpyHostToDevice);
+    // the update phase may run first (a parity check entering a step there)
+    const int go = 1;
+    COPYW(d->stream, &d->args.ctl->runTail, &go, sizeof(int), hipMemcpyHostToDevice);
     // no surface seen yet by the subgrid lookup (all-ones bits: a NaN no
     // surface equals)
     if (cfg->useSubgrid == 1)
@@ -1197,6 +1200,62 @@ int frehd_dev_create_from_state(const frehd_config *cfg, const frehd_forcing *f,
     return 0;
 }
 
+}  // extern "C"
+
+namespace frehd {
+
+// the maxima of the last step's CFL numbers, from the velocity update's
+// block partials
+int cfl_maxima(frehd_dev_impl *d, double *mx, double *my)
+{
+    const StepArgs &a = d->args;
+    const size_t nb = (size_t)a.nb1;
+    std::vector<double> v(nb);
+    double m[2] = {0.0, 0.0};
+    for (int q = 0; q < 2; q++) {
+        COPYW(d->stream, v.data(), a.cflpart + (size_t)q * a.maxBlocks, nb * sizeof(double), hipMemcpyDeviceToHost);
+        for (double x : v) m[q] = std::fmax(m[q], x);
+    }
+    *mx = m[0];
+    *my = m[1];
+    return 0;
+}
+
+// limiterCFL's / zeroNegSurf's / the velocity update's volume losses since
+// creation: the per-block accumulators summed in block order
+static int vloss_totals(frehd_dev_impl *d, double out[4])
+{
+    const StepArgs &a = d->args;
+    std::vector<double> v((size_t)a.maxBlocks);
+    for (int q = 0; q < 4; q++) {
+        COPYW(d->stream, v.data(), a.vpart + (size_t)q * a.maxBlocks, v.size() * sizeof(double), hipMemcpyDeviceToHost);
+        double t = 0.0;
+        for (double x : v) t += x;
+        out[q] = t;
+    }
+    return 0;
+}
+
+// the buffers holding the current depth and concentration (1: depthB / S2):
+// between steps the device's start-of-step buffers with the flips the last
+// step left pending; between the phases of a step that is not halted, the
+// new ones once updateData (depth) or scalarTransport (S) has run
+static void current_buffers(const frehd_dev_impl *d, const Ctl &c, int *dp, int *sp)
+{
+    const bool scal = d->args.p.nScalar > 0;
+    if (d->midStep != 0 && !c.halted) {
+        *dp = c.parity ^ (step_state_new_depth(d->midStep) ? 1 : 0);
+        *sp = c.sParity ^ (step_state_new_scalar(d->midStep) ? 1 : 0);
+    } else {
+        *dp = c.parity ^ (c.flipPending ? 1 : 0);
+        *sp = c.sParity ^ (c.flipPending && scal ? 1 : 0);
+    }
+}
+
+}  // namespace frehd
+
+extern "C" {
+
 int frehd_dev_step(frehd_dev_t *d, int32_t nsteps)
 {
     if (!d) return fail(FREHD_EINVAL, "handle is NULL");
@@ -1205,7 +1264,8 @@ int frehd_dev_step(frehd_dev_t *d, int32_t nsteps)
     HIPCHK(hipSetDevice(d->device));
     Ctl c;
     COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
-    if (c.halted) return fail(FREHD_ECFL, "handle halted at step %d: max CFL exceeded 20", c.haltStep);
+    if (c.halted || c.haltPending)
+        return fail(FREHD_ECFL, "handle halted at step %d: max CFL exceeded 20", c.haltStep);
     if (c.tt + nsteps > d->cfg.Nt)
         return fail(FREHD_EINVAL, "stepping to %d exceeds the forcing series (Nt = %d)", c.tt + nsteps, d->cfg.Nt);
     d->tt = c.tt;
@@ -1220,8 +1280,11 @@ int frehd_dev_step(frehd_dev_t *d, int32_t nsteps)
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(d->stream));
     COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
-    if (c.halted) return fail(FREHD_ECFL, "max CFL exceeded 20 at step %d (CFLx %.3g, CFLy %.3g)", c.haltStep,
-                              c.lastMaxCFLx, c.lastMaxCFLy);
+    if (c.halted || c.haltPending) {
+        double mx = 0, my = 0;
+        cfl_maxima(d, &mx, &my);
+        return fail(FREHD_ECFL, "max CFL exceeded 20 at step %d (CFLx %.3g, CFLy %.3g)", c.haltStep, mx, my);
+    }
     return 0;
 }
 
@@ -1249,10 +1312,10 @@ int frehd_dev_download(frehd_dev_t *d, frehd_fields *o)
     // has run
     Ctl c;
     COPYW(d->stream, &c, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
-    const bool newDepth = !c.halted && step_state_new_depth(d->midStep);
-    const bool newS = !c.halted && step_state_new_scalar(d->midStep);
-    const double *depthCur = (c.parity ^ newDepth) ? a.depthB : a.depthA;
-    const char *Scur = reinterpret_cast<const char *>((c.sParity ^ newS) ? a.S2 : a.S);
+    int dp, sp;
+    current_buffers(d, c, &dp, &sp);
+    const double *depthCur = dp ? a.depthB : a.depthA;
+    const char *Scur = reinterpret_cast<const char *>(sp ? a.S2 : a.S);
     // r: stored in the state precision
     struct { double *dst; const double *src; size_t n; bool r; } tab[] = {
         {o->surf, a.surf, Ng, false}, {o->surfOld, a.surfOld, Ng, false}, {o->uuXP, a.uuXP, Ng, true},
@@ -1307,10 +1370,9 @@ int frehd_dev_stats(frehd_dev_t *d, frehd_stats *out)
     out->pcgIterLast = c.pcgIterLast;
     out->refineLast = c.refineLast;
     out->relResLast = c.relResLast;
-    out->maxCFLx = c.lastMaxCFLx;
-    out->maxCFLy = c.lastMaxCFLy;
-    for (int q = 0; q < 4; q++) out->Vloss[q] = c.Vloss[q];
-    out->halted = c.halted;
+    if (int rc = cfl_maxima(d, &out->maxCFLx, &out->maxCFLy)) return rc;
+    if (int rc = vloss_totals(d, out->Vloss)) return rc;
+    out->halted = c.halted || c.haltPending;
     out->haltStep = c.haltStep;
     return 0;
 }
@@ -1401,7 +1463,18 @@ int frehd_dev_upload(frehd_dev_t *d, const frehd_fields *in)
     HIPCHK(hipStreamSynchronize(d->stream));
     Ctl c;
     COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
-    return upload_state(d, *in, c.parity, c.sParity);
+    // the pending buffer flips applied first, so the uploaded state is the
+    // next step's start
+    int dp, sp;
+    current_buffers(d, c, &dp, &sp);
+    if (dp != c.parity || sp != c.sParity || c.flipPending || !c.runTail) {
+        c.parity = dp;
+        c.sParity = sp;
+        c.flipPending = 0;
+        c.runTail = 1;    // as a new handle: the update phase may come next
+        COPYW(d->stream, d->args.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice);
+    }
+    return upload_state(d, *in, dp, sp);
 }
 
 int frehd_dev_step_index(frehd_dev_t *d, int32_t *tt)
@@ -1657,7 +1730,7 @@ int frehd_batch_step(frehd_batch_t *b, int32_t nsteps)
     for (auto *d : e.members) {
         Ctl c;
         COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
-        if (!c.halted && c.tt + nsteps > d->cfg.Nt)
+        if (!(c.halted || c.haltPending) && c.tt + nsteps > d->cfg.Nt)
             return fail(FREHD_EINVAL, "a member steps to %d beyond its forcing series (Nt = %d)", c.tt + nsteps,
                         d->cfg.Nt);
         d->tt = c.tt;
@@ -1674,7 +1747,7 @@ int frehd_batch_step(frehd_batch_t *b, int32_t nsteps)
     for (auto *d : e.members) {
         Ctl c;
         COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
-        halted += c.halted != 0;
+        halted += (c.halted || c.haltPending) ? 1 : 0;
     }
     if (halted) return fail(FREHD_ECFL, "%d batch member(s) halted: max CFL exceeded 20", halted);
     return 0;
@@ -1706,7 +1779,7 @@ static int autotune_blocks(Exec &e, std::vector<frehd_dev_impl *> members, int s
     for (auto *d : members) {
         Ctl c;
         COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
-        if (!c.halted && c.tt + 6 * steps > d->cfg.Nt)
+        if (!(c.halted || c.haltPending) && c.tt + 6 * steps > d->cfg.Nt)
             return fail(FREHD_EINVAL, "tuning steps to %d beyond the forcing series (Nt = %d)", c.tt + 6 * steps,
                         d->cfg.Nt);
     }

@@ -38,7 +38,8 @@ Timing::~Timing()
 // until it has.  testS >= 0: first the outer test of the mixed solve's
 // refinement sweep testS (R: the members' refinement arguments).
 __global__ void __launch_bounds__(256) k_gather_flags(const StepArgs *__restrict__ A, int count, int mixed,
-                                                      int32_t *out, const RefineArgs *__restrict__ R, int testS)
+                                                      int32_t *out, const RefineArgs *__restrict__ R, int testS,
+                                                      int issue)
 {
     const int m = blockIdx.x;
     if (m >= count) return;
@@ -48,7 +49,7 @@ __global__ void __launch_bounds__(256) k_gather_flags(const StepArgs *__restrict
     // thread 0 records it unless the member has halted or converged before)
     if (testS >= 0) refine_test(R[m], testS, lds, !c->halted && c->ref.doneAt < 0);
     if (threadIdx.x != 0) return;
-    c->open = !c->halted && (mixed ? c->ref.doneAt : c->pcg.doneAt) < 0;
+    close_solve(c, mixed != 0, issue != 0);
     write_flags(c, out + (size_t)m * F_STRIDE);
 }
 
@@ -114,12 +115,14 @@ void remember(Exec &e, const SolveCounts &counts)
     e.recent[e.nRecent++] = counts;
 }
 
-int enqueue_gather(Exec &e)
+// issue: the kernels after the solve follow this gather (members whose
+// solve has ended get their go-ahead)
+int enqueue_gather(Exec &e, int issue)
 {
     const int testS = e.mixed ? e.pendTest : -1;
     e.pendTest = -1;
     LAUNCH(K_GATHER, k_gather_flags, dim3((unsigned)e.count), dim3(256), e.dArgs, e.count, e.mixed, e.hFlagsDev,
-           e.dRef, testS);
+           e.dRef, testS, issue);
     return 0;
 }
 
@@ -160,7 +163,7 @@ void clear_flags(Exec &e)
 
 int gather(Exec &e)
 {
-    int rc = enqueue_gather(e);
+    int rc = enqueue_gather(e, 0);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(e.stream));
     return 0;
@@ -577,6 +580,7 @@ static int exec_phase_launch(Exec &e, int phase)
         if (!rc) rc = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
         if (!rc) rc = gather(e);
         if (!rc) rc = e.mixed ? solve_mixed_finish(e, k) : solve_fp64_finish(e, k);
+        if (!rc) rc = enqueue_gather(e, 1);
         if (rc) return rc;
         LAUNCH_ST(K_SURF, k_surface, grid1(x.nb1, n), b1);
         return 0;
@@ -611,8 +615,8 @@ static int exec_phase_launch(Exec &e, int phase)
             }
         }
         return 0;
-    case 4:   // monitorCFL's stop test
-        LAUNCH_ST(K_END, k_step_end, dim3(1, 1, n), dim3(1024));
+    case 4:   // monitorCFL: its stop test ran in the velocity update; the budgets' step end
+        if (e.ext.budget) LAUNCH_ST(K_END, k_step_end, dim3(1, 1, n), dim3(1024));
         for (auto *m : e.members) m->tt += 1;
         return 0;
     default:
@@ -729,7 +733,7 @@ static int exec_step_graphs(Exec &e)
                 if (!r) r = enqueue_fused_solve(e);
             } else {
                 if (!r) r = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
-                if (!r) r = enqueue_gather(e);
+                if (!r) r = enqueue_gather(e, 1);
             }
             if (!r) r = enqueue_tail(e);
             return r;
@@ -758,6 +762,7 @@ static int exec_step_graphs(Exec &e)
     rc = e.mixed ? solve_mixed_finish(e, k) : solve_fp64_finish(e, k);
     if (rc) return rc;
     if (open) {
+        if ((rc = enqueue_gather(e, 1))) return rc;
         if (!e.tail) {
             rc = capture(e, [&] { return enqueue_tail(e); }, &e.tail);
             if (rc) return rc;

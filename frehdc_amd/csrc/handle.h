@@ -205,6 +205,8 @@ int exec_collect_timing(Exec &e);
 int exec_time_solver(Exec &e, int iters, double *ms);
 // upload one handle's argument blocks to its own group arrays
 int upload_member_args(frehd_dev_impl *d);
+// the last step's CFL maxima (from the velocity update's block partials)
+int cfl_maxima(frehd_dev_impl *d, double *mx, double *my);
 // lets the SSOR kernel's LDS variant take `bytes` of dynamic LDS
 void ssor_allow_lds(int bytes);
 

@@ -76,8 +76,8 @@ size_t point_into_scratch(frehd_dev_impl *d, char *base)
     a = d->args;
     a.partA = take(ps.partA);
     a.partB = take(ps.partB);
-    a.vpart = take(ps.vpart);
-    a.cflpart = take(ps.cflpart);
+    // (the volume-loss accumulators and the CFL maxima the statistics read
+    // stay in the handle's own memory)
     a.bpart = take(ps.bpart);
     PcgArgs<double> &p = d->naPcgH;
     p = d->pcg;
@@ -128,7 +128,8 @@ extern "C" int frehd_dev_step_noalloc(frehd_dev_t *d, int32_t nsteps)
     Ctl c;
     HIPCHK(hipMemcpyAsync(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, d->stream));
     HIPCHK(hipStreamSynchronize(d->stream));
-    if (c.halted) return fail(FREHD_ECFL, "handle halted at step %d: max CFL exceeded 20", c.haltStep);
+    if (c.halted || c.haltPending)
+        return fail(FREHD_ECFL, "handle halted at step %d: max CFL exceeded 20", c.haltStep);
     if (c.tt + nsteps > d->cfg.Nt)
         return fail(FREHD_EINVAL, "stepping to %d exceeds the forcing series (Nt = %d)", c.tt + nsteps, d->cfg.Nt);
     if (nsteps == 0) return 0;
@@ -176,8 +177,10 @@ extern "C" int frehd_dev_step_noalloc(frehd_dev_t *d, int32_t nsteps)
     HIPCHK(hipStreamSynchronize(d->stream));
     HIPCHK(hipMemcpyAsync(&c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, d->stream));
     HIPCHK(hipStreamSynchronize(d->stream));
-    if (c.halted)
-        return fail(FREHD_ECFL, "max CFL exceeded 20 at step %d (CFLx %.3g, CFLy %.3g)", c.haltStep, c.lastMaxCFLx,
-                    c.lastMaxCFLy);
+    if (c.halted || c.haltPending) {
+        double mx = 0, my = 0;
+        cfl_maxima(d, &mx, &my);
+        return fail(FREHD_ECFL, "max CFL exceeded 20 at step %d (CFLx %.3g, CFLy %.3g)", c.haltStep, mx, my);
+    }
     return 0;
 }

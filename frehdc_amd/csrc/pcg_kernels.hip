@@ -1336,10 +1336,14 @@ __global__ void __launch_bounds__(256) k_solve_mixed(const RefineArgs *__restric
     const bool rec = pb.index == 0;        // the member's recording block
     const bool halted = ctl->halted != 0;
     if (rec && threadIdx.x == 0) {
-        // until the surface is written, the kernels after the solve stand
-        // down (a launch that ends at a timeout leaves the member open)
+        // until the surface is written, the solve is open (a launch that
+        // ends at a timeout leaves it so, and the kernels after the solve
+        // stand down)
         ctl->open = halted ? 0 : 1;
-        if (halted) write_flags(ctl, flagsOut + (size_t)m * F_STRIDE);
+        if (halted) {
+            close_solve(ctl, true, true);
+            write_flags(ctl, flagsOut + (size_t)m * F_STRIDE);
+        }
     }
     int ph = halted ? PH_DONE : PH_REFINE;
     int s = 0, it = 0;
@@ -1382,10 +1386,10 @@ __global__ void __launch_bounds__(256) k_solve_mixed(const RefineArgs *__restric
                 const int k = ch * 256 + (int)threadIdx.x;
                 if (k < sa.g.Ng) surface_cell(sa, k, loss[0], xs);
                 block_sum<1>(loss, lds);
-                if (threadIdx.x == 0) sa.vpart[2 * sa.maxBlocks + ch] = loss[0];
+                if (threadIdx.x == 0) sa.vpart[2 * sa.maxBlocks + ch] += loss[0];
             }
             if (rec && threadIdx.x == 0) {
-                ctl->open = 0;
+                close_solve(ctl, true, true);
                 write_flags(ctl, flagsOut + (size_t)m * F_STRIDE);
             }
         }

@@ -75,15 +75,18 @@ struct Ctl {
     int haltStep;
     int parity;               // depth buffer of the step's start: 0 -> depthA
     int sParity;              // scalar buffer of the step's start: 0 -> S
-    int open;                 // the step's solve has not finished: the kernels
-                              // after it stand down (set by the flag gather)
-    int stepDone;             // the step's kernels after the solve have run
-                              // (set by the step end, cleared at the step's begin)
+    int open;                 // the step's solve has not finished (set by the flag gather)
+    int runTail;              // the kernels after the solve run for this member
+                              // (set by the gather that first sees its solve done)
+    int tailIssued;           // that gather has run in this step (cleared at the step's begin)
+    int flipPending;          // the update wrote the new depth (and the scalar transport the
+                              // concentrations) into the second buffers: the next step's
+                              // begin flips parity / sParity
+    int haltPending;          // a cell's CFL exceeded 20 in this step (set by the velocity
+                              // update); the next step's begin halts the member
     int ttNext;               // tt + 1 at every step boundary: the step the next
                               // predictor computes (its first thread advances tt
                               // while the other blocks run, so they read this)
-    double lastMaxCFLx, lastMaxCFLy;
-    double Vloss[4];
     PcgScalars pcg;
     int pcgIterLast;
     int pad0;
@@ -401,7 +404,27 @@ __device__ inline gptr<double> depth_nxt(const StepArgsT<R> &a) { return a.ctl->
 // whose solve has finished and whose step they have not completed yet (a
 // group launches them again when one member's solve needed more iterations)
 template <class R>
-__device__ inline bool tail_off(const StepArgsT<R> &a) { return a.ctl->halted || a.ctl->open || a.ctl->stepDone; }
+__device__ inline bool tail_off(const StepArgsT<R> &a) { return a.ctl->halted || !a.ctl->runTail; }
+
+// After a solver launch (one thread): the open flag, and, when the kernels
+// after the solve are about to be launched (issue), the go-ahead for them
+// -- for a member whose solve has ended and whose tail has not run in this
+// step -- with the step's statistics.  (A handle starts with the go-ahead
+// given, so a parity check can enter a step at its update phase.)
+__device__ inline void close_solve(Ctl *c, bool mixed, bool issue)
+{
+    c->open = !c->halted && (mixed ? c->ref.doneAt : c->pcg.doneAt) < 0;
+    if (!issue) return;
+    const int run = !c->halted && !c->open && !c->tailIssued;
+    c->runTail = run;
+    if (!run) return;
+    c->tailIssued = 1;
+    c->pcgIterLast = c->pcg.iter;
+    c->pcgIterTotal += c->pcg.iter;
+    c->relResLast = c->pcg.bNorm > 0 ? c->pcg.rNorm / c->pcg.bNorm : 1.0;
+    c->steps += 1;
+    c->ttNext = c->tt + 1;
+}
 template <class R>
 __device__ inline sptr<R> s_cur(const StepArgsT<R> &a) { return a.ctl->sParity ? a.S2 : a.S; }
 template <class R>

@@ -24,8 +24,21 @@ __device__ inline void step_begin(const StepArgsT<R> &a)
 {
     Ctl *c = a.ctl;
     if (c->halted) return;
+    if (c->haltPending) {
+        // monitorCFL's stop after the last step (nsfunctions.c:1226-1231):
+        // nothing of this step runs; that step's buffer flips stay pending
+        // (the host reads its state through them)
+        c->halted = 1;
+        return;
+    }
+    if (c->flipPending) {
+        c->parity ^= 1;                       // the new depth becomes this step's start
+        if (a.p.nScalar > 0) c->sParity ^= 1; // and the new concentration
+        c->flipPending = 0;
+    }
     c->tt += 1;
-    c->stepDone = 0;
+    c->runTail = 0;
+    c->tailIssued = 0;
     c->pcg.iter = 0;
     c->pcg.doneAt = -1;
     c->ref.doneAt = -1;
@@ -34,6 +47,20 @@ __device__ inline void step_begin(const StepArgsT<R> &a)
     c->pcgIn.doneAt = -1;
     c->pcgIn.iter = 0;
     c->pcg.rhoHist[0] = c->pcg.rhoHist[1] = 0.0;
+}
+
+// monitorCFL's stop test (nsfunctions.c:1226-1231) on one block's maxima of
+// the new CFL numbers: a block whose cells exceed 20 marks the member (the
+// global maximum exceeds 20 exactly when some block's does); the next
+// step's begin halts it, since the reference completes the step and then
+// exits.  Any number of blocks may write the same values.
+template <class R>
+__device__ inline void cfl_stop_test(const StepArgsT<R> &a, double mx, double my)
+{
+    if (fabs(mx) > 20 || fabs(my) > 20) {
+        a.ctl->haltPending = 1;
+        a.ctl->haltStep = a.ctl->tt;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -184,7 +211,7 @@ __device__ inline void store_pred(const StepArgsT<R> &a, int c, int i, int j, co
 template <class R, bool SUB>
 __device__ inline void predictor_body(const StepArgsT<R> &a)
 {
-    if (a.ctl->halted) return;
+    if (a.ctl->halted || a.ctl->haltPending) return;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -310,7 +337,7 @@ __global__ void __launch_bounds__(256) k_surface(const StepArgsT<R> *__restrict_
     const gptr<const double> xs = (a.cX32 && (a.ctl->ref.doneAt & 1)) ? a.xB : a.x;
     if (k < g.Ng) surface_cell(a, k, loss[0], xs);
     block_sum<1>(loss, lds);
-    if (threadIdx.x == 0) a.vpart[2 * a.maxBlocks + blockIdx.x] = loss[0];
+    if (threadIdx.x == 0) a.vpart[2 * a.maxBlocks + blockIdx.x] += loss[0];
 }
 
 
@@ -360,8 +387,8 @@ __device__ inline void store_vloss2(const StepArgsT<R> &a, double (&loss)[2], do
     block_sum<2>(loss, lds);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         const int bid = blockIdx.x + a.gx2 * blockIdx.y;
-        a.vpart[0 * a.maxBlocks + bid] = loss[0];
-        a.vpart[1 * a.maxBlocks + bid] = loss[1];
+        a.vpart[0 * a.maxBlocks + bid] += loss[0];
+        a.vpart[1 * a.maxBlocks + bid] += loss[1];
     }
 }
 
@@ -372,6 +399,8 @@ __global__ void __launch_bounds__(256) k_limiter(const StepArgsT<R> *__restrict_
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
     __shared__ double lds[16 * 2];
     if (tail_off(a)) return;
+    // the new depth goes to the second buffer: the next step starts from it
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) a.ctl->flipPending = 1;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -477,7 +506,7 @@ __global__ void __launch_bounds__(256) k_faces(const StepArgsT<R> *__restrict__ 
     double loss[1] = {0.0};
     if (k < a.g.Ng) faces_cell(a, k, loss[0]);
     block_sum<1>(loss, lds);
-    if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blockIdx.x] = loss[0];
+    if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blockIdx.x] += loss[0];
 }
 
 // updateVelocity + waterfallCorrection of interior cell c's east and north
@@ -726,6 +755,7 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgsT<R> *__restrict__
     if (threadIdx.x == 0) {
         a.cflpart[blockIdx.x] = mx;
         a.cflpart[a.maxBlocks + blockIdx.x] = my;
+        cfl_stop_test(a, mx, my);
     }
     if (p.budget) store_vol_budget(a, vb, lds);
 }
@@ -1001,64 +1031,18 @@ __device__ inline void step_end_budget(const StepArgsT<R> &a)
     }
 }
 
-// Step end: volume-loss and CFL reductions (block partials in block order),
-// and the max CFL test of monitorCFL (nsfunctions.c:1226-1231), which halts
-// the handle instead of exiting the process.  One block of 1024 threads.
+// Step end with the conservation budgets on: the step's budget partials
+// reduced in block order and added to the member's accumulators.  (The
+// volume losses accumulate in per-block slots, the CFL stop test runs in
+// the velocity update and the step's bookkeeping in the flag gather and
+// the next step's begin, so a step without budgets has no such launch.)
+// One block of 1024 threads.
 template <class R>
 __global__ void __launch_bounds__(1024) k_step_end(const StepArgsT<R> *__restrict__ A)
 {
     const StepArgsT<R> &a = A[blockIdx.z];
-    __shared__ double lds[16 * 4];
-    Ctl *c = a.ctl;
-    if (tail_off(a)) return;
-    // the loads of a thread's partials are issued together (4 per thread
-    // per array up to 4096 blocks), then summed in block order
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    double mx = 0.0, my = 0.0;
-    const int t = threadIdx.x, B = blockDim.x, mb = a.maxBlocks;
-    for (int b0 = 0; b0 < a.nb1 || b0 < a.nb2; b0 += 4 * B) {
-        double l0[4], l1[4], l2[4], l3[4], c0[4], c1[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int b = b0 + u * B + t;
-            const bool in2 = b < a.nb2, in1 = b < a.nb1;
-            l0[u] = in2 ? a.vpart[b] : 0.0;
-            l1[u] = in2 ? a.vpart[mb + b] : 0.0;
-            l2[u] = in1 ? a.vpart[2 * mb + b] : 0.0;
-            l3[u] = in1 ? a.vpart[3 * mb + b] : 0.0;
-            c0[u] = in1 ? a.cflpart[b] : 0.0;
-            c1[u] = in1 ? a.cflpart[mb + b] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            v[0] += l0[u];
-            v[1] += l1[u];
-            v[2] += l2[u];
-            v[3] += l3[u];
-            mx = fmax(mx, c0[u]);
-            my = fmax(my, c1[u]);
-        }
-    }
-    block_sum<4>(v, lds);
-    mx = block_max(mx, lds);
-    my = block_max(my, lds);
-    if (a.p.budget) step_end_budget(a);
-    if (threadIdx.x != 0) return;
-    for (int q = 0; q < 4; q++) c->Vloss[q] += v[q];
-    c->lastMaxCFLx = mx;
-    c->lastMaxCFLy = my;
-    c->pcgIterLast = c->pcg.iter;
-    c->pcgIterTotal += c->pcg.iter;
-    c->relResLast = c->pcg.bNorm > 0 ? c->pcg.rNorm / c->pcg.bNorm : 1.0;
-    c->steps += 1;
-    c->parity ^= 1;     // the new depth becomes the next step's start
-    if (a.p.nScalar > 0) c->sParity ^= 1;   // and the new concentration
-    c->stepDone = 1;
-    c->ttNext = c->tt + 1;
-    if (fabs(mx) > 20 || fabs(my) > 20) {
-        c->halted = 1;
-        c->haltStep = c->tt;
-    }
+    if (tail_off(a) || !a.p.budget) return;
+    step_end_budget(a);
 }
 
 }  // namespace frehd
@@ -1234,6 +1218,8 @@ __global__ void __launch_bounds__(256) k_limiter_sub(const StepArgsT<R> *__restr
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
     __shared__ double lds[16 * 2];
     if (tail_off(a)) return;
+    // the new depth goes to the second buffer: the next step starts from it
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) a.ctl->flipPending = 1;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -1403,7 +1389,6 @@ __global__ void __launch_bounds__(256) k_combine(const StepArgsT<R> *__restrict_
 {
     const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.nbN + a.nbGhost) return;
-    if (tail_off(a)) return;
     const Grid g = a.g;
     if ((int)blockIdx.x < a.nbN) {
         const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1586,7 +1571,7 @@ __global__ void __launch_bounds__(256) k_combine_faces(const StepArgsT<R> *__res
     double loss[1] = {0.0};
     if (k < a.g.Ng) combine_faces_cell(a, k, loss[0]);
     block_sum<1>(loss, lds);
-    if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blockIdx.x] = loss[0];
+    if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blockIdx.x] += loss[0];
 }
 
 // adjustTidalVelocitySub (subgrid.c:1291-1332)
@@ -1694,6 +1679,7 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgsT<R> *__restri
     if (threadIdx.x == 0) {
         a.cflpart[blockIdx.x] = mx;
         a.cflpart[a.maxBlocks + blockIdx.x] = my;
+        cfl_stop_test(a, mx, my);
     }
     if (p.budget) store_vol_budget(a, vb, lds);
 }
