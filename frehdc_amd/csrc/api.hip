@@ -348,7 +348,7 @@ size_t layout(frehd_dev *d, char *base)
     const bool f32 = state_f32(k);
     auto takeR = [&](size_t n) { return f32 ? reinterpret_cast<double *>(cv.take<float>(n)) : cv.take<double>(n); };
     a.ctl = cv.take<Ctl>(1);
-    double **ngArr[] = {&a.surf, &a.surfOld, &a.bottomZ, &a.bottomXP, &a.bottomYP};
+    double **ngArr[] = {&a.surfA, &a.surfB, &a.bottomZ, &a.bottomXP, &a.bottomYP};
     for (auto **p : ngArr) *p = cv.take<double>(Ng);
     double **ngR[] = {&a.uuXP, &a.vvYP, &a.depthXP, &a.depthYP, &a.EnXP, &a.EnYP, &a.DragXP, &a.DragYP,
         &a.CDXP, &a.CDYP, &a.Fuu, &a.Fvv, &a.CFLx, &a.CFLy, &a.wtfXP, &a.wtfYP};
@@ -765,7 +765,8 @@ int upload_state(frehd_dev *d, const frehd_fields &s, int depthParity = 0, int s
     std::vector<std::vector<float>> keep;
     // r: stored in the state precision
     struct { const double *src; double *dst; size_t n; bool r; } ng[] = {
-        {s.surf, a.surf, Ng, false}, {s.surfOld, a.surfOld, Ng, false}, {s.uuXP, a.uuXP, Ng, true},
+        {s.surf, depthParity ? a.surfB : a.surfA, Ng, false}, {s.surfOld, depthParity ? a.surfA : a.surfB, Ng, false},
+        {s.uuXP, a.uuXP, Ng, true},
         {s.vvYP, a.vvYP, Ng, true}, {s.uuYP, a.uuYP, N, true}, {s.vvXP, a.vvXP, N, true},
         {s.depth, depthCur, N, false}, {s.depthXP, a.depthXP, Ng, true}, {s.depthYP, a.depthYP, Ng, true},
         {s.EnXP, a.EnXP, Ng, true}, {s.EnYP, a.EnYP, Ng, true}, {s.DragXP, a.DragXP, Ng, true},
@@ -1236,19 +1237,22 @@ static int vloss_totals(frehd_dev_impl *d, double out[4])
     return 0;
 }
 
-// the buffers holding the current depth and concentration (1: depthB / S2):
-// between steps the device's start-of-step buffers with the flips the last
-// step left pending; between the phases of a step that is not halted, the
-// new ones once updateData (depth) or scalarTransport (S) has run
-static void current_buffers(const frehd_dev_impl *d, const Ctl &c, int *dp, int *sp)
+// the buffers holding the current depth, concentration and surface (1:
+// depthB / S2 / surfB, whose partner then holds surfOld): between steps the
+// device's start-of-step buffers with the flips the last step left pending;
+// between the phases of a step that is not halted, the new ones once the
+// solve (surface), updateData (depth) or scalarTransport (S) has run
+static void current_buffers(const frehd_dev_impl *d, const Ctl &c, int *dp, int *sp, int *hp)
 {
     const bool scal = d->args.p.nScalar > 0;
     if (d->midStep != 0 && !c.halted) {
         *dp = c.parity ^ (step_state_new_depth(d->midStep) ? 1 : 0);
         *sp = c.sParity ^ (step_state_new_scalar(d->midStep) ? 1 : 0);
+        *hp = c.parity ^ (step_state_new_surface(d->midStep) ? 1 : 0);
     } else {
         *dp = c.parity ^ (c.flipPending ? 1 : 0);
         *sp = c.sParity ^ (c.flipPending && scal ? 1 : 0);
+        *hp = *dp;
     }
 }
 
@@ -1293,6 +1297,17 @@ int frehd_dev_phase(frehd_dev_t *d, int32_t phase)
     if (!d) return fail(FREHD_EINVAL, "handle is NULL");
     if (int rc = state_check(d, CALL_PHASE, phase)) return rc;
     HIPCHK(hipSetDevice(d->device));
+    if (d->midStep == 0 && phase >= 2) {
+        // entering a step after its solve: the state's surface is the new one
+        // (a state given at a boundary: no flip pending from a step run here)
+        Ctl c;
+        COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
+        if (c.flipPending) return fail(FREHD_ESTATE, "phase %d entered after a whole step: run phases 0 and 1", phase);
+        const int n = d->g.Ng;
+        hipLaunchKernelGGL(k_swap_surface, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, d->stream,
+                           c.parity ? d->args.surfB : d->args.surfA, c.parity ? d->args.surfA : d->args.surfB, n);
+        HIPCHK(hipGetLastError());
+    }
     int rc = exec_phase(d->self, phase);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(d->stream));
@@ -1312,13 +1327,14 @@ int frehd_dev_download(frehd_dev_t *d, frehd_fields *o)
     // has run
     Ctl c;
     COPYW(d->stream, &c, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
-    int dp, sp;
-    current_buffers(d, c, &dp, &sp);
+    int dp, sp, hp;
+    current_buffers(d, c, &dp, &sp, &hp);
     const double *depthCur = dp ? a.depthB : a.depthA;
     const char *Scur = reinterpret_cast<const char *>(sp ? a.S2 : a.S);
     // r: stored in the state precision
     struct { double *dst; const double *src; size_t n; bool r; } tab[] = {
-        {o->surf, a.surf, Ng, false}, {o->surfOld, a.surfOld, Ng, false}, {o->uuXP, a.uuXP, Ng, true},
+        {o->surf, hp ? a.surfB : a.surfA, Ng, false}, {o->surfOld, hp ? a.surfA : a.surfB, Ng, false},
+        {o->uuXP, a.uuXP, Ng, true},
         {o->vvYP, a.vvYP, Ng, true}, {o->uuYP, a.uuYP, N, true}, {o->vvXP, a.vvXP, N, true},
         {o->depth, depthCur, N, false}, {o->depthXP, a.depthXP, Ng, true}, {o->depthYP, a.depthYP, Ng, true},
         {o->EnXP, a.EnXP, Ng, true}, {o->EnYP, a.EnYP, Ng, true}, {o->DragXP, a.DragXP, Ng, true},
@@ -1465,8 +1481,8 @@ int frehd_dev_upload(frehd_dev_t *d, const frehd_fields *in)
     COPYW(d->stream, &c, d->args.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
     // the pending buffer flips applied first, so the uploaded state is the
     // next step's start
-    int dp, sp;
-    current_buffers(d, c, &dp, &sp);
+    int dp, sp, hp;
+    current_buffers(d, c, &dp, &sp, &hp);
     if (dp != c.parity || sp != c.sParity || c.flipPending || !c.runTail) {
         c.parity = dp;
         c.sParity = sp;

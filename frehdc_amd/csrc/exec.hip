@@ -548,6 +548,40 @@ int enqueue_assemble(Exec &e)
 
 static int exec_phase_launch(Exec &e, int phase);
 
+// updateData (+ the CFL numbers of monitorCFL); fusedSurface: the limiter
+// also makes the surface update (no separate surface sweep before it)
+static int enqueue_update(Exec &e, bool fusedSurface)
+{
+    const int n = e.count;
+    const Extents &x = e.ext;
+    const dim3 b2((unsigned)x.blockX, (unsigned)(256 / x.blockX)), g2((unsigned)x.gx2, (unsigned)x.gy2, (unsigned)n),
+        b1(256);
+    const auto *a32 = reinterpret_cast<const StepArgsT<float> *>(e.dArgs);
+    const int v = (e.f32 ? 2 : 0) + (fusedSurface ? 1 : 0);
+    if (e.sub) {
+        switch (v) {
+        case 0: LAUNCH(K_LIM, (k_limiter_sub<double, false>), g2, b2, e.dArgs); break;
+        case 1: LAUNCH(K_LIM, (k_limiter_sub<double, true>), g2, b2, e.dArgs); break;
+        case 2: LAUNCH(K_LIM, (k_limiter_sub<float, false>), g2, b2, a32); break;
+        default: LAUNCH(K_LIM, (k_limiter_sub<float, true>), g2, b2, a32); break;
+        }
+        LAUNCH_ST(K_FACES, k_combine_faces, grid1(x.nb1, n), b1);
+        if (x.nbTide > 0) LAUNCH_ST(K_TIDEV, k_tidal_velocity_sub, grid1(x.nbTide, n), b1);
+        LAUNCH_ST(K_FINISH, k_finish_sub, grid1(x.nb1, n), b1);
+    } else {
+        switch (v) {
+        case 0: LAUNCH(K_LIM, (k_limiter<double, false>), g2, b2, e.dArgs); break;
+        case 1: LAUNCH(K_LIM, (k_limiter<double, true>), g2, b2, e.dArgs); break;
+        case 2: LAUNCH(K_LIM, (k_limiter<float, false>), g2, b2, a32); break;
+        default: LAUNCH(K_LIM, (k_limiter<float, true>), g2, b2, a32); break;
+        }
+        LAUNCH_ST(K_FACES, k_faces, grid1(x.nb1, n), b1);
+        if (x.nbTide > 0) LAUNCH_ST(K_TIDEV, k_tidal_velocity, grid1(x.nbTide, n), b1);
+        LAUNCH_ST(K_FINISH, k_finish, grid1(x.nb1, n), b1);
+    }
+    return 0;
+}
+
 int exec_phase(Exec &e, int phase)
 {
     int rc = exec_phase_launch(e, phase);
@@ -585,20 +619,8 @@ static int exec_phase_launch(Exec &e, int phase)
         LAUNCH_ST(K_SURF, k_surface, grid1(x.nb1, n), b1);
         return 0;
     }
-    case 2: { // updateData (+ the CFL numbers of monitorCFL)
-        if (e.sub) {
-            LAUNCH_ST(K_LIM, k_limiter_sub, g2, b2);
-            LAUNCH_ST(K_FACES, k_combine_faces, grid1(x.nb1, n), b1);
-            if (x.nbTide > 0) LAUNCH_ST(K_TIDEV, k_tidal_velocity_sub, grid1(x.nbTide, n), b1);
-            LAUNCH_ST(K_FINISH, k_finish_sub, grid1(x.nb1, n), b1);
-        } else {
-            LAUNCH_ST(K_LIM, k_limiter, g2, b2);
-            LAUNCH_ST(K_FACES, k_faces, grid1(x.nb1, n), b1);
-            if (x.nbTide > 0) LAUNCH_ST(K_TIDEV, k_tidal_velocity, grid1(x.nbTide, n), b1);
-            LAUNCH_ST(K_FINISH, k_finish, grid1(x.nb1, n), b1);
-        }
-        return 0;
-    }
+    case 2:   // updateData (+ the CFL numbers of monitorCFL)
+        return enqueue_update(e, false);
     case 3:   // scalarTransport, every scalar in one launch (members without scalars skip inside)
         if (e.nScalarMax > 0) {
             const auto *a32 = reinterpret_cast<const StepArgsT<float> *>(e.dArgs);
@@ -666,15 +688,16 @@ static int exec_step_eager(Exec &e)
     return 0;
 }
 
-// enqueue the kernels after the solve (surface -- unless the fused solve
-// wrote it --, update, scalars, CFL test)
+// enqueue the kernels after the solve in a captured step: the update --
+// with the surface update fused into its limiter, unless the fused solve
+// wrote the surface --, the scalars, the step end
 static int enqueue_tail(Exec &e)
 {
-    if (e.planBlocks == 0) LAUNCH_ST(K_SURF, k_surface, grid1(e.ext.nb1, e.count), dim3(256));
-    for (int ph = 2; ph <= 4; ph++) {
-        int r = exec_phase(e, ph);
-        if (r) return r;
-    }
+    int r = enqueue_update(e, e.planBlocks == 0);
+    if (r) return r;
+    for (auto *m : e.members) m->midStep = step_state_after(2);
+    for (int ph = 3; ph <= 4; ph++)
+        if ((r = exec_phase(e, ph))) return r;
     return 0;
 }
 

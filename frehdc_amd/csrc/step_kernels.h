@@ -113,8 +113,9 @@ struct StepArgsT {
     Grid g;
     P p;
     gptr<Ctl> ctl;
-    // Ng arrays
-    gptr<double> surf, surfOld;
+    // Ng arrays; the free surface is double-buffered (sf_start / sf_new:
+    // Ctl.parity selects, as for the depth)
+    gptr<double> surfA, surfB;
     sptr<R> uuXP, vvYP, depthXP, depthYP, EnXP, EnYP, DragXP, DragYP;
     sptr<R> CDXP, CDYP, Fuu, Fvv, CFLx, CFLy, wtfXP, wtfYP;
     gptr<double> bottomZ, bottomXP, bottomYP;
@@ -396,6 +397,17 @@ __global__ void k_solve_mixed(const RefineArgs *__restrict__ R, const PcgArgs<fl
                               const StepArgsT<float> *__restrict__ S, const SolveBlock *__restrict__ plan,
                               SolveSync *sync, int32_t *flagsOut, unsigned long long spinTicks);
 
+// The free-surface buffers.  sf_start holds eta^n through the whole step;
+// sf_new holds eta^(n-1) (the solver's extrapolated start reads it) until
+// the surface update writes eta^(n+1) into it.  So the reference's surf is
+// sf_start before the surface update and sf_new after it, and its surfOld
+// is sf_start after it; the next step's begin flips parity, and sf_new
+// becomes that step's sf_start.
+template <class R>
+__device__ inline gptr<double> sf_start(const StepArgsT<R> &a) { return a.ctl->parity ? a.surfB : a.surfA; }
+template <class R>
+__device__ inline gptr<double> sf_new(const StepArgsT<R> &a) { return a.ctl->parity ? a.surfA : a.surfB; }
+
 template <class R>
 __device__ inline gptr<const double> depth_cur(const StepArgsT<R> &a) { return a.ctl->parity ? a.depthB : a.depthA; }
 template <class R>
@@ -436,15 +448,14 @@ __device__ inline sptr<R> s_nxt(const StepArgsT<R> &a) { return a.ctl->sParity ?
 // mixed path the refinement buffer of the sweep that converged).  A ghost
 // takes the zero-clipped solution of the interior cell it mirrors (the tidal
 // override comes after the ghost fill); Vloss3 (the clip) into loss.
+//
+// The new surface goes to the step's second buffer (sf_new below); the old
+// one stays in the first (the reference's copy into surfOld).
 template <class R>
-__device__ inline void surface_cell(const StepArgsT<R> &a, int k, double &loss, gptr<const double> xs)
+__device__ inline double surface_value(const StepArgsT<R> &a, int k, int src, double &loss, gptr<const double> xs)
 {
-    const Grid &g = a.g;
     const P &p = a.p;
     const int tt = a.ctl->tt;
-    const double old = a.surf[k];
-    a.surfOld[k] = old;
-    const int src = ghost_source(g, k, false);
     const int cell = src < 0 ? k : src;
     double h = xs[cell];
     const double zb = a.bottomZ[cell];
@@ -458,7 +469,14 @@ __device__ inline void surface_cell(const StepArgsT<R> &a, int k, double &loss, 
         if ((fl & 1) && p.bcType != 2 && a.tideP[tt] + p.offset > zb) h = a.tideP[tt] + p.offset;
         if ((fl & 2) && p.bcType != 1 && a.tideM[tt] + p.offset > zb) h = a.tideM[tt] + p.offset;
     }
-    a.surf[k] = h;
+    return h;
+}
+
+template <class R>
+__device__ inline void surface_cell(const StepArgsT<R> &a, int k, double &loss, gptr<const double> xs)
+{
+    const double h = surface_value(a, k, ghost_source(a.g, k, false), loss, xs);
+    (a.ctl->parity ? a.surfA : a.surfB)[k] = h;
 }
 
 enum SubTable { T_V, T_Z, T_VXP, T_VXM, T_VYP, T_VYM, T_NP, T_NM, T_OP, T_OM };
@@ -466,17 +484,18 @@ enum SubTable { T_V, T_Z, T_VXP, T_VXM, T_VYP, T_VYM, T_NP, T_NM, T_OP, T_OM };
 template <class R> __global__ void k_predictor(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_assemble(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_surface(const StepArgsT<R> *__restrict__ A);
-template <class R> __global__ void k_limiter(const StepArgsT<R> *__restrict__ A);
+template <class R, bool FUSED> __global__ void k_limiter(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_faces(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_tidal_velocity(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_finish(const StepArgsT<R> *__restrict__ A);
 template <class R, bool BUDGET> __global__ void k_scalar(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_step_end(const StepArgsT<R> *__restrict__ A);
+__global__ void k_swap_surface(double *a, double *b, int n);
 // subgrid variants
 template <class R> __global__ void k_predictor_sub(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_assemble_sub(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_subarea_init(const StepArgsT<R> *__restrict__ A);
-template <class R> __global__ void k_limiter_sub(const StepArgsT<R> *__restrict__ A);
+template <class R, bool FUSED> __global__ void k_limiter_sub(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_combine(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_combine_faces(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_tidal_velocity_sub(const StepArgsT<R> *__restrict__ A);

@@ -274,7 +274,7 @@ __device__ inline void store_row(const StepArgsT<R> &a, int c, const Row &w, dou
     // linear extrapolation in time 2 eta^n - eta^(n-1).  From a nonzero
     // start the solver forms r = b - A x itself (the mixed path's refinement,
     // k_pcg1_start for FP64).
-    a.x[c] = a.p.warmStart == 2 ? 2.0 * a.surf[c] - a.surfOld[c] : (a.p.warmStart ? a.surf[c] : 0.0);
+    a.x[c] = a.p.warmStart == 2 ? 2.0 * sf_start(a)[c] - sf_new(a)[c] : (a.p.warmStart ? sf_start(a)[c] : 0.0);
     if (a.cX32 || a.p.warmStart) return;
     a.r[c] = z;
     const double invd = 1.0 / fabs(D);
@@ -306,7 +306,7 @@ __device__ inline Row assemble_row(const StepArgsT<R> &a, int c, int i, int j, i
     const double GYM = face_coef(coefy, a.depthYP[s], q.DYs);
     double D = 1 + GXP + GYP + GXM + GYM;
 
-    double z = a.surf[c] - (dt / p.dx) * (q.EnXc * a.depthXP[c] - q.EnXw * a.depthXP[w]) -
+    double z = sf_start(a)[c] - (dt / p.dx) * (q.EnXc * a.depthXP[c] - q.EnXw * a.depthXP[w]) -
                (dt / p.dy) * (q.EnYc * a.depthYP[c] - q.EnYs * a.depthYP[s]);
     const uint8_t fl = a.flags[c];
     const int ninflow = fl >> 2;
@@ -350,13 +350,13 @@ __global__ void __launch_bounds__(256) k_surface(const StepArgsT<R> *__restrict_
 // limiterCFL for cell c (both sweeps); returns the new surface, sets rmv and
 // adds the Vloss1/Vloss2 contributions
 template <class R>
-__device__ inline double limiter_cell(const StepArgsT<R> &a, int c, int i, int j, double (&loss)[2], uint8_t &rmv)
+__device__ inline double limiter_cell(const StepArgsT<R> &a, int c, int i, int j, double h, double (&loss)[2],
+                                      uint8_t &rmv)
 {
     const Grid &g = a.g;
     const P &p = a.p;
-    const double *dOld = depth_cur(a), *ho = a.surfOld;
+    const double *dOld = depth_cur(a), *ho = sf_start(a);
     const double zb = a.bottomZ[c];
-    double h = a.surf[c];
     double diff = h - zb;
     rmv = 0;
     if (dOld[c] <= 0 && diff > 0) {
@@ -381,36 +381,71 @@ __device__ inline double limiter_cell(const StepArgsT<R> &a, int c, int i, int j
     return h;
 }
 
-template <class R>
-__device__ inline void store_vloss2(const StepArgsT<R> &a, double (&loss)[2], double *lds)
+// the volume losses of a 2-D block into its accumulator slots: Vloss1,
+// Vloss2 (limiterCFL) and, where the kernel also made the surface update,
+// Vloss3 (zeroNegSurf)
+template <class R, int NL>
+__device__ inline void store_vloss(const StepArgsT<R> &a, double (&loss)[NL], double *lds)
 {
-    block_sum<2>(loss, lds);
+    block_sum<NL>(loss, lds);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         const int bid = blockIdx.x + a.gx2 * blockIdx.y;
-        a.vpart[0 * a.maxBlocks + bid] += loss[0];
-        a.vpart[1 * a.maxBlocks + bid] += loss[1];
+#pragma unroll
+        for (int q = 0; q < NL; q++) a.vpart[(size_t)q * a.maxBlocks + bid] += loss[q];
     }
 }
 
+// The surface update (surface_cell) of interior cell c made inside the
+// limiter kernel: c's new surface (tidal values, clip loss into loss), and
+// the ghost entries that mirror c -- its row's jP / jM ghost, its column's
+// iP / iM ghost and the corners (ghost_source) -- written here.  The
+// limiter reads the neighbours' old surface from sf_start, so no thread
+// depends on another's new value.
 template <class R>
+__device__ inline double fused_surface(const StepArgsT<R> &a, int c, int i, int j, double &loss)
+{
+    const Grid &g = a.g;
+    const gptr<const double> xs = (a.cX32 && (a.ctl->ref.doneAt & 1)) ? a.xB : a.x;
+    const gptr<double> hn = sf_new(a);
+    const bool top = j == g.ny - 1, bot = j == 0, right = i == g.nx - 1, left = i == 0;
+    if (top || bot || right || left) {
+        double none = 0.0;
+        const double hg = surface_value(a, gjP(g, i), c, none, xs);   // any ghost of c: the clipped x of c
+        if (top) hn[gjP(g, i)] = hg;
+        if (bot) hn[gjM(g, i)] = hg;
+        if (right) hn[giP(g, j)] = hg;
+        if (left) hn[giM(g, j)] = hg;
+        if (top && right) hn[cIPJP(g)] = hg;
+        if (top && left) hn[cIPJM(g)] = hg;
+        if (bot && right) hn[cIMJP(g)] = hg;
+        if (bot && left) hn[cIMJM(g)] = hg;
+    }
+    return surface_value(a, c, -1, loss, xs);
+}
+
+// FUSED: the surface update is made here too (the step's separate surface
+// sweep, k_surface, is left out)
+template <class R, bool FUSED>
 __global__ void __launch_bounds__(256) k_limiter(const StepArgsT<R> *__restrict__ A)
 {
     const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
-    __shared__ double lds[16 * 2];
+    __shared__ double lds[16 * 3];
     if (tail_off(a)) return;
     // the new depth goes to the second buffer: the next step starts from it
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) a.ctl->flipPending = 1;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    double loss[2] = {0.0, 0.0};
+    double loss[3] = {0.0, 0.0, 0.0};
+    double (&lim)[2] = *reinterpret_cast<double (*)[2]>(loss);
     if (i < g.nx && j < g.ny) {
         const P &p = a.p;
         const int c = j * g.nx + i;
         uint8_t rmv;
-        const double h = limiter_cell(a, c, i, j, loss, rmv);
-        a.surf[c] = h;
+        const double hs = FUSED ? fused_surface(a, c, i, j, loss[2]) : sf_new(a)[c];
+        const double h = limiter_cell(a, c, i, j, hs, lim, rmv);
+        sf_new(a)[c] = h;
         a.rmv[c] = rmv;
         // computeVolume: depth from the previous step
         const double cellV = p.dx * p.dy * depth_cur(a)[c];
@@ -421,7 +456,8 @@ __global__ void __launch_bounds__(256) k_limiter(const StepArgsT<R> *__restrict_
         const double d = h - a.bottomZ[c];
         depth_nxt(a)[c] = d < 0 ? 0 : d;
     }
-    store_vloss2(a, loss, lds);
+    if (FUSED) store_vloss<R, 3>(a, loss, lds);
+    else store_vloss<R, 2>(a, lim, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -438,7 +474,7 @@ __device__ inline double face_depth_x(const StepArgsT<R> &a, int k)
     // depthXP per updateAllFaceDepth (irank = 0 = nrank - 1), before the
     // final negative clip
     const Grid &g = a.g;
-    const double *h = a.surf, *bX = a.bottomXP;
+    const double *h = sf_new(a), *bX = a.bottomXP;
     if (k < g.N) {
         const int i = k % g.nx, j = k / g.nx;
         if (i == g.nx - 1) return 0.0;               // iPbd faces zeroed
@@ -458,7 +494,7 @@ template <class R>
 __device__ inline double face_depth_y(const StepArgsT<R> &a, int k)
 {
     const Grid &g = a.g;
-    const double *h = a.surf, *bY = a.bottomYP;
+    const double *h = sf_new(a), *bY = a.bottomYP;
     if (k < g.N) {
         const int i = k % g.nx, j = k / g.nx;
         if (j == g.ny - 1) return 0.0;               // last strip: jPbd zeroed
@@ -523,17 +559,18 @@ __device__ inline FaceVel face_vel(const StepArgsT<R> &a, int c, double dX, doub
     const P &p = a.p;
     const int i = c % g.nx, j = c / g.nx;
     const int e = nbE(g, c, i, j), n = nbN(g, c, i, j);
-    const double hc = a.surf[c];
+    const double *hs = sf_new(a);
+    const double hc = hs[c];
     const double bXc = a.bottomXP[c], bYc = a.bottomYP[c];
     FaceVel r{0, 0, 0, 0, 0};
     // waterfall flags of this cell's east and north faces
     if (hc < bXc - 0.0 && dX >= p.wtfh) r.fX = -1;
-    if (e < g.N && a.surf[e] < bXc - 0.0 && dX >= p.wtfh) r.fX = 1;
+    if (e < g.N && hs[e] < bXc - 0.0 && dX >= p.wtfh) r.fX = 1;
     if (hc < bYc - 0.0 && dY >= p.wtfh) r.fY = -1;
-    if (n < g.N && a.surf[n] < bYc - 0.0 && dY >= p.wtfh) r.fY = 1;
+    if (n < g.N && hs[n] < bYc - 0.0 && dY >= p.wtfh) r.fY = 1;
     // updateVelocity
-    double uc = a.EnXP[c] - (p.g * p.dt / p.dx) * (a.surf[e] - hc);
-    double vc = a.EnYP[c] - (p.g * p.dt / p.dy) * (a.surf[n] - hc);
+    double uc = a.EnXP[c] - (p.g * p.dt / p.dx) * (hs[e] - hc);
+    double vc = a.EnYP[c] - (p.g * p.dt / p.dy) * (hs[n] - hc);
     if (dX == 0) uc = 0;
     if (dY == 0) vc = 0;
     const bool rmvC = a.rmv[c] == 1;
@@ -582,10 +619,10 @@ __device__ inline void faces_cell(const StepArgsT<R> &a, int k, double &loss)
         int r = k - g.N;
         if (r >= g.nx && r < 2 * g.nx) {             // jM ghost: YM face of cell (i, 0)
             const int i = r - g.nx;
-            if (a.surf[i] < a.bottomYP[k] - 0.0 && dY >= p.wtfh) a.wtfYP[k] = 1;
+            if (sf_new(a)[i] < a.bottomYP[k] - 0.0 && dY >= p.wtfh) a.wtfYP[k] = 1;
         } else if (r >= 2 * g.nx + g.ny && r < 2 * g.nx + 2 * g.ny) {   // iM ghost
             const int jj = r - 2 * g.nx - g.ny;
-            if (a.surf[jj * g.nx] < a.bottomXP[k] - 0.0 && dX >= p.wtfh) a.wtfXP[k] = 1;
+            if (sf_new(a)[jj * g.nx] < a.bottomXP[k] - 0.0 && dX >= p.wtfh) a.wtfXP[k] = 1;
         }
         const int su = ghost_source(g, k, true), sv = ghost_source(g, k, false);
         u = face_vel(a, su, clipped(face_depth_x(a, su)), clipped(face_depth_y(a, su))).u;
@@ -619,7 +656,7 @@ __global__ void k_tidal_velocity(const StepArgsT<R> *__restrict__ A)
     const bool isP = q < nP;
     const int c = isP ? a.tideLocP[q] : a.tideLocM[q - nP];
     const int i = c % g.nx, j = c / g.nx;
-    const double Fww = (a.surf[c] - a.surfOld[c]) * p.dx * p.dy / p.dt;
+    const double Fww = (sf_new(a)[c] - sf_start(a)[c]) * p.dx * p.dy / p.dt;
     const double dc = depth_nxt(a)[c];
     if (isP) {
         const double all = a.Fvv[nbS(g, c, i, j)] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww;
@@ -831,7 +868,7 @@ __device__ inline void scalar_body(const StepArgsT<R> &a, double *lds)
     // geometry of the cell, the same for every scalar: flow rates, face
     // areas or depths, the connectivity with the previous-step surface
     // (scalar.c:292-341; the subgrid tests face areas, subgrid.c:1588-1630)
-    const double *ho = a.surfOld, *zb = a.bottomZ;
+    const double *ho = sf_start(a), *zb = a.bottomZ;
     const auto fx = SUB ? a.Nx : a.depthXP;
     const auto fy = SUB ? a.Oy : a.depthYP;
     double Fuc = 0, Fuw = 0, Fvc = 0, Fvs = 0, fxc = 0, fxw = 0, fyc = 0, fys = 0, V = 0;
@@ -929,7 +966,7 @@ __device__ inline void scalar_body(const StepArgsT<R> &a, double *lds)
         if (Sv > 100 || Sv < 0) {
             if (!bottomRow && !topRowNotFirst) {
                 Sv = 0;
-                a.surf[c] = a.surf[c] - depth_nxt(a)[c];
+                sf_new(a)[c] = sf_new(a)[c] - depth_nxt(a)[c];
                 depth_nxt(a)[c] = 0;
             }
         }
@@ -1029,6 +1066,21 @@ __device__ inline void step_end_budget(const StepArgsT<R> &a)
             bu.massMoved[q] += v[3];
         }
     }
+}
+
+// A step entered at its update phase from a state given at a step boundary
+// (a parity check starting from a mid-step state): the given surface is the
+// solve's new one, so the two surface buffers trade places (sf_new takes the
+// new surface, sf_start the old).  (Plain pointers: a kernel parameter typed
+// for the global address space would be named differently in the device
+// pass than in the host pass, and the launch would not find the kernel.)
+__global__ void __launch_bounds__(256) k_swap_surface(double *a, double *b, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double t = a[k];
+    a[k] = b[k];
+    b[k] = t;
 }
 
 // Step end with the conservation budgets on: the step's budget partials
@@ -1206,24 +1258,26 @@ __global__ void __launch_bounds__(256) k_subarea_init(const StepArgsT<R> *__rest
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const int i = c % g.nx, j = c / g.nx;
-    subarea_cell(a, c, i, j, a.surf[c], depth_cur(a)[c], subarea_pre(a, c, i, j));
+    subarea_cell(a, c, i, j, sf_start(a)[c], depth_cur(a)[c], subarea_pre(a, c, i, j));
 }
 
 // limiterCFL + computeVolumeSub + computeSubArea + updateDepth
 // (nsfunctions.c:452-495, subgrid.c:1215-1224, 55-251, nsfunctions.c:239-247)
-template <class R>
+// FUSED: the surface update is made here too (see k_limiter)
+template <class R, bool FUSED>
 __global__ void __launch_bounds__(256) k_limiter_sub(const StepArgsT<R> *__restrict__ A)
 {
     const StepArgsT<R> &a = A[blockIdx.z];
     if (blockIdx.x >= a.gx2 || blockIdx.y >= a.gy2) return;
-    __shared__ double lds[16 * 2];
+    __shared__ double lds[16 * 3];
     if (tail_off(a)) return;
     // the new depth goes to the second buffer: the next step starts from it
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) a.ctl->flipPending = 1;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    double loss[2] = {0.0, 0.0};
+    double loss[3] = {0.0, 0.0, 0.0};
+    double (&lim)[2] = *reinterpret_cast<double (*)[2]>(loss);
     if (i < g.nx && j < g.ny) {
         const P &p = a.p;
         const int c = j * g.nx + i;
@@ -1241,9 +1295,10 @@ __global__ void __launch_bounds__(256) k_limiter_sub(const StepArgsT<R> *__restr
 #pragma unroll
         for (int q = 0; q < NS; q++) sv[q] = q < p.nScalar ? s_cur(a)[q * ss + c] : 0.0;
         uint8_t rmv;
-        const double h = limiter_cell(a, c, i, j, loss, rmv);
+        const double hs = FUSED ? fused_surface(a, c, i, j, loss[2]) : sf_new(a)[c];
+        const double h = limiter_cell(a, c, i, j, hs, lim, rmv);
         const double d = h - zb;
-        a.surf[c] = h;
+        sf_new(a)[c] = h;
         a.rmv[c] = rmv;
         a.cellV[c] = cellV;
 #pragma unroll
@@ -1253,7 +1308,8 @@ __global__ void __launch_bounds__(256) k_limiter_sub(const StepArgsT<R> *__restr
         depth_nxt(a)[c] = d < 0 ? 0 : d;
         subarea_cell(a, c, i, j, h, dOld, pre);
     }
-    store_vloss2(a, loss, lds);
+    if (FUSED) store_vloss<R, 3>(a, loss, lds);
+    else store_vloss<R, 2>(a, lim, lds);
 }
 
 // combineSubArea (subgrid.c:254-462 and the clip loop :466-478) of interior
@@ -1324,15 +1380,17 @@ __device__ inline void min_adjust(const StepArgsT<R> &a, int c, int e, int n, do
     }
 }
 
+// (hs: the surface the lookups were made at -- sf_start at Init, sf_new in
+// the step)
 template <class R>
-__device__ inline Combined combine_cell(const StepArgsT<R> &a, int c)
+__device__ inline Combined combine_cell(const StepArgsT<R> &a, int c, gptr<const double> hs)
 {
     const Grid &g = a.g;
     const P &p = a.p;
     const int i = c % g.nx, j = c / g.nx, N = g.N;
     const int e = nbE(g, c, i, j), n = nbN(g, c, i, j);
     const double dxdy = p.dx * p.dy;
-    const double Zc = a.Zraw[c], h = a.surf[c];
+    const double Zc = a.Zraw[c], h = hs[c];
     double Nx, Oy, Vx, Vy, Zx, Zy;
     if (e < N) { const double A = a.Np[c], B = a.Nm[e]; Nx = A >= B ? A : B; }
     else Nx = a.Np[c];
@@ -1345,7 +1403,7 @@ __device__ inline Combined combine_cell(const StepArgsT<R> &a, int c)
         Zx = Zc;
         if (Vx > 0 && Zx <= 0) Zx = dxdy;
         if (h < a.bottomXP[c]) { Vx = vm; Zx = 0.5 * Zc; }
-        if (a.surf[e] < a.bottomXP[c]) { Vx = vp; Zx = 0.5 * Zc; }
+        if (hs[e] < a.bottomXP[c]) { Vx = vp; Zx = 0.5 * Zc; }
     } else {
         Vx = 2 * a.Vxp[c];
         Zx = 2 * Zc;
@@ -1357,7 +1415,7 @@ __device__ inline Combined combine_cell(const StepArgsT<R> &a, int c)
         Zy = Zc;
         if (Vy > 0 && Zy <= 0) Zy = dxdy;
         if (h < a.bottomYP[c]) { Vy = vm; Zy = 0.5 * Zc; }
-        if (a.surf[n] < a.bottomYP[c]) { Vy = vp; Zy = 0.5 * Zc; }
+        if (hs[n] < a.bottomYP[c]) { Vy = vp; Zy = 0.5 * Zc; }
     } else {
         Vy = 2 * a.Vyp[c];
         Zy = 2 * Zc;
@@ -1393,7 +1451,7 @@ __global__ void __launch_bounds__(256) k_combine(const StepArgsT<R> *__restrict_
     if ((int)blockIdx.x < a.nbN) {
         const int c = blockIdx.x * blockDim.x + threadIdx.x;
         if (c >= g.N) return;
-        const Combined v = combine_cell(a, c);
+        const Combined v = combine_cell(a, c, sf_start(a));
         a.Nx[c] = v.Nx; a.Oy[c] = v.Oy; a.Vx[c] = v.Vx; a.Vy[c] = v.Vy;
         a.Zx[c] = v.Zx; a.Zy[c] = v.Zy; a.V[c] = v.V; a.Z[c] = v.Z;
         return;
@@ -1402,7 +1460,7 @@ __global__ void __launch_bounds__(256) k_combine(const StepArgsT<R> *__restrict_
     if (k >= g.Ng - 4) return;
     const int src = ghost_source(g, k, false);
     const int r = k - g.N;
-    const Combined v = combine_cell(a, src);
+    const Combined v = combine_cell(a, src, sf_start(a));
     a.V[k] = v.V; a.Z[k] = v.Z;
     a.Vx[k] = v.Vx; a.Vy[k] = v.Vy;
     a.Zx[k] = v.Zx; a.Zy[k] = v.Zy;
@@ -1432,7 +1490,8 @@ __device__ inline FaceVel face_vel_sub(const StepArgsT<R> &a, int c, const Combi
     const P &p = a.p;
     const int i = c % g.nx, j = c / g.nx;
     const int e = nbE(g, c, i, j), n = nbN(g, c, i, j);
-    const double hc = a.surf[c], he = a.surf[e], hn = a.surf[n];
+    const double *hs = sf_new(a);
+    const double hc = hs[c], he = hs[e], hn = hs[n];
     const double bXc = a.bottomXP[c], bYc = a.bottomYP[c];
     FaceVel r{0, 0, 0, 0, 0};
     double &fX = r.fX, &fY = r.fY;
@@ -1498,7 +1557,7 @@ __device__ inline void combine_faces_cell(const StepArgsT<R> &a, int k, double &
     const double dY = clipped(face_depth_y(a, k));
     double u, v, NxK, OyK;
     if (k < g.N) {
-        const Combined cv = combine_cell(a, k);
+        const Combined cv = combine_cell(a, k, sf_new(a));
         a.Nx[k] = cv.Nx; a.Oy[k] = cv.Oy; a.Vx[k] = cv.Vx; a.Vy[k] = cv.Vy;
         a.Zx[k] = cv.Zx; a.Zy[k] = cv.Zy; a.V[k] = cv.V; a.Z[k] = cv.Z;
         const FaceVel f = face_vel_sub(a, k, cv);
@@ -1512,7 +1571,7 @@ __device__ inline void combine_faces_cell(const StepArgsT<R> &a, int k, double &
     } else {
         const int r = k - g.N;
         const int sv = ghost_source(g, k, false), su = ghost_source(g, k, true);
-        const Combined cvv = combine_cell(a, sv);
+        const Combined cvv = combine_cell(a, sv, sf_new(a));
         if (k < g.Ng - 4) {
             // combineSubArea's ghost copies (subgrid.c:480-506); the iM / iP
             // ghosts of the top row take the corner cell's Oy before the
@@ -1537,13 +1596,13 @@ __device__ inline void combine_faces_cell(const StepArgsT<R> &a, int k, double &
         if (r >= g.nx && r < 2 * g.nx) {             // jM ghost: YM test of cell (i, 0)
             const int i = r - g.nx;
             const double bot = a.bottomYP[k];
-            if (a.surf[i] < bot - 0.0 && a.surf[k] - bot >= p.wtfh) a.wtfYP[k] = 1;
+            if (sf_new(a)[i] < bot - 0.0 && sf_new(a)[k] - bot >= p.wtfh) a.wtfYP[k] = 1;
         } else if (r >= 2 * g.nx + g.ny && r < 2 * g.nx + 2 * g.ny) {   // iM ghost
             const int jj = r - 2 * g.nx - g.ny;
             const double bot = a.bottomXP[k];
-            if (a.surf[jj * g.nx] < bot - 0.0 && a.surf[k] - bot >= p.wtfh) a.wtfXP[k] = 1;
+            if (sf_new(a)[jj * g.nx] < bot - 0.0 && sf_new(a)[k] - bot >= p.wtfh) a.wtfXP[k] = 1;
         }
-        u = face_vel_sub(a, su, su == sv ? cvv : combine_cell(a, su)).u;
+        u = face_vel_sub(a, su, su == sv ? cvv : combine_cell(a, su, sf_new(a))).u;
         v = face_vel_sub(a, sv, cvv).v;
     }
     a.depthXP[k] = dX;
@@ -1590,7 +1649,7 @@ __global__ void k_tidal_velocity_sub(const StepArgsT<R> *__restrict__ A)
     const bool isP = q < nP;
     const int c = isP ? a.tideLocP[q] : a.tideLocM[q - nP];
     const int i = c % g.nx, j = c / g.nx;
-    const double Fww = (a.surf[c] - a.surfOld[c]) * a.Z[c] / p.dt;
+    const double Fww = (sf_new(a)[c] - sf_start(a)[c]) * a.Z[c] / p.dt;
     const double Oyc = a.Oy[c];
     if (isP) {
         const double all = a.Fvv[nbS(g, c, i, j)] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww;
@@ -1708,7 +1767,7 @@ __device__ inline Row assemble_row_sub(const StepArgsT<R> &a, int c, int i, int 
     if (Vyc > 0) GYP = cy * Oyc * Oyc / Vyc / divider / (1 + q.DYc);
     if (Vys > 0) GYM = cy * Oys * Oys / Vys / divider / (1 + q.DYs);
     double D = a.Z[c] + GXP + GYP + GXM + GYM;
-    const double hc = a.surf[c];
+    const double hc = sf_start(a)[c];
     double z = hc * a.Z[c] - dt * (q.EnXc * Nxc - q.EnXw * Nxw) - dt * (q.EnYc * Oyc - q.EnYs * Oys);
     const uint8_t fl = a.flags[c];
     const int ninflow = p.bcType != 3 ? (fl >> 2) : 0;
@@ -1786,10 +1845,14 @@ template __global__ void k_finish<double>(const StepArgsT<double> *__restrict__)
 template __global__ void k_finish<float>(const StepArgsT<float> *__restrict__);
 template __global__ void k_finish_sub<double>(const StepArgsT<double> *__restrict__);
 template __global__ void k_finish_sub<float>(const StepArgsT<float> *__restrict__);
-template __global__ void k_limiter<double>(const StepArgsT<double> *__restrict__);
-template __global__ void k_limiter<float>(const StepArgsT<float> *__restrict__);
-template __global__ void k_limiter_sub<double>(const StepArgsT<double> *__restrict__);
-template __global__ void k_limiter_sub<float>(const StepArgsT<float> *__restrict__);
+template __global__ void k_limiter<double, false>(const StepArgsT<double> *__restrict__);
+template __global__ void k_limiter<double, true>(const StepArgsT<double> *__restrict__);
+template __global__ void k_limiter<float, false>(const StepArgsT<float> *__restrict__);
+template __global__ void k_limiter<float, true>(const StepArgsT<float> *__restrict__);
+template __global__ void k_limiter_sub<double, false>(const StepArgsT<double> *__restrict__);
+template __global__ void k_limiter_sub<double, true>(const StepArgsT<double> *__restrict__);
+template __global__ void k_limiter_sub<float, false>(const StepArgsT<float> *__restrict__);
+template __global__ void k_limiter_sub<float, true>(const StepArgsT<float> *__restrict__);
 template __global__ void k_predictor<double>(const StepArgsT<double> *__restrict__);
 template __global__ void k_predictor<float>(const StepArgsT<float> *__restrict__);
 template __global__ void k_predictor_sub<double>(const StepArgsT<double> *__restrict__);

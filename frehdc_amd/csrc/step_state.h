@@ -40,8 +40,10 @@ inline int step_call_check(int phasesRun, bool inBatch, StepCall call, int phase
     return phasesRun == 0 ? 0 : FREHD_ESTATE;
 }
 
-// which double buffers a download reads between phases: the depth updated
-// by updateData (phase 2) and the concentrations of scalarTransport (phase 3)
+// which double buffers a download reads between phases: the surface of
+// solveFreeSurface (phase 1), the depth updated by updateData (phase 2) and
+// the concentrations of scalarTransport (phase 3)
+inline bool step_state_new_surface(int phasesRun) { return phasesRun >= 2; }
 inline bool step_state_new_depth(int phasesRun) { return phasesRun >= 3; }
 inline bool step_state_new_scalar(int phasesRun) { return phasesRun >= 4; }
 
