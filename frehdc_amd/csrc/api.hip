@@ -252,14 +252,19 @@ inline DirTiling dir_tiling(int nx, int ny, int H, int remap, int target = 1024,
 
 inline size_t max_dir_blocks(int nx, int ny) { return (size_t)((nx + 255) / 256) * ny + 8; }
 
-// FREHD_GRAPHS=0 in the environment when a handle or batch is created: its
-// steps are launched kernel by kernel, never replayed as graphs (for
-// profilers: under rocprofv3 a process that replays the graphs of a second
-// handle has died in hipGraphLaunch)
+// How a handle's or batch's whole steps are launched, from FREHD_GRAPHS in
+// the environment at its creation: unset, streamed launches for groups of
+// at least 2^18 cells and captured graphs below; 0 kernel by kernel with a
+// wait for the solver flags (for profilers: under rocprofv3 a process that
+// replayed the graphs of a second handle has died in hipGraphLaunch); 1
+// graphs; 2 streamed launches
 inline int graphs_from_env()
 {
     const char *v = getenv("FREHD_GRAPHS");
-    return !(v && v[0] == '0');
+    if (!v || !v[0]) return kLaunchAuto;
+    if (v[0] == '0') return kLaunchEager;
+    if (v[0] == '2') return kLaunchStreamed;
+    return kLaunchGraphs;
 }
 
 // the most adjacent columns per lane the configuration's solver takes:

@@ -712,6 +712,53 @@ static int enqueue_tail(Exec &e)
 // (captured on its own) for that member; members whose tail ran stand down.
 // The same kernels with the same arguments as the eager path, so the results
 // are identical.
+// One step as plain launches with the same structure as a captured step:
+// the predictor, the assembly, the solver's pattern and the flag gather,
+// then -- without waiting -- the kernels after the solve, which stand down
+// for a member whose solve the pattern left open; the host then waits for
+// the flags (while the tail runs) and finishes any open solve as the
+// captured step does.  The host queues a step's launches well ahead of the
+// device, and consecutive launches on a stream follow each other more
+// closely than consecutive graph launches (about 14 us lie between the end
+// of one step graph and the start of the next on MI355X), so large grids
+// step faster this way.
+static int exec_step_streamed(Exec &e)
+{
+    const int n = e.count;
+    int k = 0, rc;
+    clear_flags(e);
+    {
+        const dim3 b2((unsigned)e.ext.blockX, (unsigned)(256 / e.ext.blockX)),
+            g2((unsigned)e.ext.gx2, (unsigned)e.ext.gy2, (unsigned)n);
+        if (e.sub) LAUNCH_ST(K_PRED, k_predictor_sub, g2, b2);
+        else LAUNCH_ST(K_PRED, k_predictor, g2, b2);
+        int r = enqueue_assemble(e);
+        if (e.planBlocks > 0) {
+            if (!r) r = enqueue_fused_solve(e);
+        } else {
+            if (!r) r = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
+            if (!r) r = enqueue_gather(e, 1);
+        }
+        if (!r) r = enqueue_tail(e);     // (its phase-4 bookkeeping advances the members' step)
+        if (r) return r;
+    }
+    if (e.planBlocks > 0) return fused_check(e);
+    if ((rc = wait_flags(e))) return rc;
+    bool open = false;
+    for (int m = 0; m < n; m++)
+        if (!flags(e, m)[F_HALTED] && flags(e, m)[e.mixed ? F_REFDONE : F_DONE] < 0) open = true;
+    // finish the open solves (behind the stood-down tail) and record the
+    // counts for the next pattern
+    rc = e.mixed ? solve_mixed_finish(e, k) : solve_fp64_finish(e, k);
+    if (rc) return rc;
+    if (open) {
+        if ((rc = enqueue_gather(e, 1))) return rc;
+        if ((rc = enqueue_tail(e))) return rc;
+        for (auto *m : e.members) m->tt -= 1;   // the step advanced once
+    }
+    return 0;
+}
+
 static int exec_step_graphs(Exec &e)
 {
     const int n = e.count;
@@ -797,12 +844,26 @@ static int exec_step_graphs(Exec &e)
     return 0;
 }
 
+// cells of a group above which its steps run as streamed launches rather
+// than as captured graphs (where a step's kernels take long enough that the
+// host's launches keep ahead of the device); FREHD_GRAPHS=0 / 1 / 2 forces
+// kernel-by-kernel steps / graphs / streamed launches
+constexpr long long kStreamCells = 1 << 18;
+
 int exec_step(Exec &e)
 {
+    if ((e.tim && e.tim->on) || e.count == 0) return exec_step_eager(e);
+    int mode = e.useGraphs;
+    if (mode == kLaunchAuto) {
+        long long cells = 0;
+        for (auto *m : e.members) cells += (long long)m->g.N;
+        mode = cells >= kStreamCells ? kLaunchStreamed : kLaunchGraphs;
+    }
+    if (mode == kLaunchStreamed) return exec_step_streamed(e);
     // graphs once the group has stepped unchanged for a few steps (a group
     // whose members change every few steps would spend more on captures
     // than the graphs save)
-    if (e.useGraphs && !(e.tim && e.tim->on) && e.count > 0 && ++e.stableSteps > 3) return exec_step_graphs(e);
+    if (mode == kLaunchGraphs && ++e.stableSteps > 3) return exec_step_graphs(e);
     return exec_step_eager(e);
 }
 

@@ -116,7 +116,7 @@ struct Exec {
     // (predictor, assembly, the pattern, the flag gather, the kernels after
     // the solve), and those last kernels alone, for a step whose solve the
     // pattern left open; rebuilt whenever the members' argument blocks change
-    int useGraphs = 1;
+    int useGraphs = 1;                  // launch mode: kLaunchEager, kLaunchGraphs, kLaunchStreamed, kLaunchAuto
     std::vector<std::pair<std::vector<int>, hipGraphExec_t>> heads;
     hipGraphExec_t tail = nullptr;
     // place in the solve of the launches being enqueued (see TimedLaunch)
@@ -138,6 +138,9 @@ struct Exec {
     std::vector<SolveBlock> hPlan;
     unsigned long long spinTicks = 0;
 };
+
+// how a group's whole steps are launched
+enum { kLaunchEager = 0, kLaunchGraphs = 1, kLaunchStreamed = 2, kLaunchAuto = 3 };
 
 // plan entries a group's device plan array holds (blocks of one fused solve)
 constexpr int kPlanCap = 4096;
