@@ -320,17 +320,17 @@ int enqueue_inner_iteration(Exec &e, int s, int it)
 int enqueue_refine(Exec &e, int s)
 {
     Tag tag(e, 3, s, 0);
-    LAUNCH(K_REFINE, k_refine, grid1(e.ext.nbRef, e.count), dim3(256), e.dRef, s);
+    LAUNCH(K_REFINE, k_refine, grid1(e.ext.nbRef, e.count), dim3(256), e.dRef, s, e.dPcg32);
     e.pendTest = s;
     return 0;
 }
 
-// one inner solve of n iterations: the set-up launch, n iterations and the
-// launch whose stop test sees the residual of iteration n (launches past a
-// member's convergence exit at once)
+// one inner solve of n iterations: the set-up launch and n iterations
+// (launches past a member's convergence exit at once); the stop test on the
+// residual of iteration n is made by the refinement sweep after it
 int enqueue_inner_solve(Exec &e, int s, int n)
 {
-    for (int it = 0; it <= n + 1; it++) {
+    for (int it = 0; it <= n; it++) {
         int rc = enqueue_inner_iteration(e, s, it);
         if (rc) return rc;
     }
@@ -482,7 +482,7 @@ int exec_time_solver(Exec &e, int iters, double *ms)
     int rc;
     if (e.mixed) {
         for (int q = 0; q < 4; q++)
-            LAUNCH(K_REFINE, k_refine, grid1(e.ext.nbRef, e.count), dim3(256), e.dRef, 0);
+            LAUNCH(K_REFINE, k_refine, grid1(e.ext.nbRef, e.count), dim3(256), e.dRef, 0, e.dPcg32);
         for (int it = 0; it <= iters; it++)
             if ((rc = enqueue_inner_iteration(e, 0, it))) return rc;
     } else {

@@ -489,19 +489,6 @@ __device__ inline void refine_store(const RefineArgs &a, int slot, const double 
     a.partC[a.nb + slot] = part[3];
 }
 
-__global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A, int s)
-{
-    const RefineArgs &a = A[blockIdx.z];
-    if ((int)blockIdx.x >= a.nb) return;
-    __shared__ double lds[16 * 4];
-    Ctl *ctl = a.ctl;
-    const int halted = ctl->halted, refDone = ctl->ref.doneAt;
-    if (halted || (refDone >= 0 && refDone < s)) return;
-    double part[4] = {0.0, 0.0, 0.0, 0.0};     // r32.r32, r32.z32, b.b, r.r
-    refine_tile(a, s, tile_of(blockIdx.x, a.nb, a.remap), part);
-    block_sum<4>(part, lds);
-    refine_store(a, blockIdx.x, part);
-}
 
 // ---------------------------------------------------------------------------
 // Inner FP32 solve of the mixed path: CG on the scaled, decoupled operator
@@ -685,6 +672,35 @@ __device__ inline Cg1Dec cg1_decide(const PcgArgs<float> &a, int it, double eps,
         }
     }
     return d;
+}
+
+// k_refine(s).  For s > 0, block 0 first makes the stop test the launch
+// after the inner solve's last iteration would make (the launch pattern
+// ends the inner solve at its last working iteration): an inner solve whose
+// test passes there is recorded as converged, as its own launch would have
+// recorded it (the solver statistics and the next pattern read this).
+__global__ void __launch_bounds__(256) k_refine(const RefineArgs *__restrict__ A, int s,
+                                                const PcgArgs<float> *__restrict__ P)
+{
+    const RefineArgs &a = A[blockIdx.z];
+    if ((int)blockIdx.x >= a.nb) return;
+    __shared__ double lds[16 * 4];
+    Ctl *ctl = a.ctl;
+    const int halted = ctl->halted, refDone = ctl->ref.doneAt;
+    if (halted || (refDone >= 0 && refDone < s)) return;
+    if (s > 0 && blockIdx.x == 0) {
+        const PcgArgs<float> &pa = P[blockIdx.z];
+        PcgScalars *sc = pa.sc;
+        const int doneAt = sc->doneAt, last = sc->iter;
+        if (doneAt < 0 && last >= 1) {
+            const Cg1Dec d = cg1_decide(pa, last + 1, ctl->ref.epsIn, sc->bNorm, lds, false);
+            if (d.stop && threadIdx.x == 0) sc->doneAt = last + 1;
+        }
+    }
+    double part[4] = {0.0, 0.0, 0.0, 0.0};     // r32.r32, r32.z32, b.b, r.r
+    refine_tile(a, s, tile_of(blockIdx.x, a.nb, a.remap), part);
+    block_sum<4>(part, lds);
+    refine_store(a, blockIdx.x, part);
 }
 
 // inner iteration it over tile lb (see above): the partials of q_it added to

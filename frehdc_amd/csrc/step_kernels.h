@@ -298,7 +298,7 @@ struct RefineArgs {
     int gx, gy, H, remap, nb, pad;
 };
 
-__global__ void k_refine(const RefineArgs *__restrict__ A, int s);
+__global__ void k_refine(const RefineArgs *__restrict__ A, int s, const PcgArgs<float> *__restrict__ P);
 
 // The outer stop test after refinement sweep s: |b - A x_s| < eps |b| (the
 // reference's criterion, rtc.c:54, on the FP64 residual) or s at the sweep
