@@ -372,6 +372,7 @@ size_t layout(frehd_dev *d, char *base)
     a.manning = nullptr;
     a.rmv = cv.take<uint8_t>(N);
     a.flags = cv.take<uint8_t>(N);
+    a.tface = cv.take<uint16_t>(N);
     const PartialSizes ps = partial_sizes(k);
     a.maxBlocks = (int)ps.maxBlocks;
     a.nb1 = (int)ps.nb1;
@@ -569,6 +570,29 @@ int build_flags(const frehd_config &k, const frehd_forcing &f, std::vector<uint8
     return 0;
 }
 
+// Which vvYP faces adjustTidalVelocity (nsfunctions.c:1034-1078) sets, as
+// the velocity interpolation of each cell c reads them, faces f = c, c + 1,
+// c - nx, c + 1 - nx in bit pairs 2q, 2q + 1: the face's own cell is a
+// tidal P cell (YP boundary), the cell north of it a tidal M cell (YM
+// boundary); bit 8: c is a tidal M cell of the first row (its jM ghost face).
+void build_tidal_faces(const frehd_config &k, const std::vector<uint8_t> &flags, std::vector<uint16_t> &tface)
+{
+    const int nx = k.nx, N = k.nx * k.ny;
+    auto bits = [&](int f) -> int {
+        if (f < 0 || f >= N) return 0;
+        int b = 0;
+        if (k.bcType != 2 && (flags[f] & 1)) b |= 1;
+        if (k.bcType != 1 && f + nx < N && (flags[f + nx] & 2)) b |= 2;
+        return b;
+    };
+    tface.assign(N, 0);
+    for (int c = 0; c < N; c++) {
+        int t = bits(c) | bits(c + 1) << 2 | bits(c - nx) << 4 | bits(c + 1 - nx) << 6;
+        if (c < nx && k.bcType != 1 && (flags[c] & 2)) t |= 256;
+        tface[c] = (uint16_t)t;
+    }
+}
+
 void fill_params(P &p, const frehd_config &k)
 {
     memset(&p, 0, sizeof p);
@@ -726,6 +750,9 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     std::vector<uint8_t> flags;
     if ((rc = build_flags(d->cfg, *f, flags))) return rc;
     if ((rc = upload(d, flags.data(), d->args.flags, flags.size()))) return rc;
+    std::vector<uint16_t> tface;
+    build_tidal_faces(d->cfg, flags, tface);
+    if ((rc = upload(d, tface.data(), d->args.tface, tface.size() * sizeof(uint16_t)))) return rc;
     HIPCHK(hipStreamSynchronize(d->stream));
     if ((rc = upload_member_args(d))) return rc;
     *dd = d;

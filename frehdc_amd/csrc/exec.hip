@@ -15,7 +15,7 @@
 namespace frehd {
 
 const char *kKernelNames[K_COUNT] = {"k_step_begin", "k_predictor", "k_assemble", "k_pcg_dir",
-    "k_pcg_res", "k_surface", "k_limiter", "k_faces", "k_tidal_velocity",
+    "k_pcg_res", "k_surface", "k_limiter", "k_faces",
     "k_finish", "k_scalar", "k_step_end", "k_combine",
     "k_subarea_init", "k_cg1_f32", "k_cg1_f32_setup", "k_cg1_f32_stop", "k_refine", "k_gather_flags",
     "k_pcg_dir_stop", "k_ssor", "k_pcg1", "k_pcg1_setup", "k_pcg1_stop", "k_pcg1_start", "k_solve_mixed"};
@@ -566,7 +566,6 @@ static int enqueue_update(Exec &e, bool fusedSurface)
         default: LAUNCH(K_LIM, (k_limiter_sub<float, true>), g2, b2, a32); break;
         }
         LAUNCH_ST(K_FACES, k_combine_faces, grid1(x.nb1, n), b1);
-        if (x.nbTide > 0) LAUNCH_ST(K_TIDEV, k_tidal_velocity_sub, grid1(x.nbTide, n), b1);
         LAUNCH_ST(K_FINISH, k_finish_sub, grid1(x.nb1, n), b1);
     } else {
         switch (v) {
@@ -576,7 +575,6 @@ static int enqueue_update(Exec &e, bool fusedSurface)
         default: LAUNCH(K_LIM, (k_limiter<float, true>), g2, b2, a32); break;
         }
         LAUNCH_ST(K_FACES, k_faces, grid1(x.nb1, n), b1);
-        if (x.nbTide > 0) LAUNCH_ST(K_TIDEV, k_tidal_velocity, grid1(x.nbTide, n), b1);
         LAUNCH_ST(K_FINISH, k_finish, grid1(x.nb1, n), b1);
     }
     return 0;
@@ -894,7 +892,6 @@ Extents member_extents(const frehd_dev_impl *d)
     x.nb1 = a.nb1;
     x.nbN = a.nbN;
     x.nbGhost = a.nbGhost;
-    x.nbTide = a.nbTide;
     x.nbRes = a.nbRes;
     const bool mixed = d->cfg.precision == FREHD_PREC_MIXED;
     x.nbDir = mixed ? d->pcg32.nbDir : d->pcg.nbDir;
@@ -923,7 +920,6 @@ static void merge(Exec &e, const frehd_dev_impl *d, bool first)
     x.nb1 = std::max(x.nb1, m.nb1);
     x.nbN = std::max(x.nbN, m.nbN);
     x.nbGhost = std::max(x.nbGhost, m.nbGhost);
-    x.nbTide = std::max(x.nbTide, m.nbTide);
     x.nbRes = std::max(x.nbRes, m.nbRes);
     x.nbDir = std::max(x.nbDir, m.nbDir);
     x.nbRef = std::max(x.nbRef, m.nbRef);

@@ -32,7 +32,7 @@ int fail(int code, const char *fmt, ...);
                                                    hipGetErrorString(e_));                \
     } while (0)
 
-enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES, K_TIDEV,
+enum KernelId { K_BEGIN, K_PRED, K_ASM, K_PQ, K_UPD, K_SURF, K_LIM, K_FACES,
                 K_FINISH, K_SCALAR, K_END, K_COMBINE, K_SUBINIT,
                 K_CG32, K_CG32_SET, K_CG32_STOP, K_REFINE, K_GATHER, K_PQ_STOP, K_SSOR,
                 K_PCG1, K_PCG1_SET, K_PCG1_STOP, K_PCG1_START, K_SOLVE, K_COUNT,
@@ -69,7 +69,7 @@ struct Timing {
 
 // launch extents of a group: the maximum over its members
 struct Extents {
-    int gx2, gy2, nb1, nbN, nbGhost, nbTide, nbRes, nbDir, nbRef;
+    int gx2, gy2, nb1, nbN, nbGhost, nbRes, nbDir, nbRef;
     int blockX;     // the 2-D stencil kernels' block: blockX x (256 / blockX) (shared by the group)
     int lanes;      // columns per lane of k_cg1 / k_pcg1 (shared by the group)
     int ssorN;      // largest N of the members preconditioned with SSOR (0: none)

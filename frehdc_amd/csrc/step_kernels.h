@@ -130,6 +130,9 @@ struct StepArgsT {
     sptr<R> uuYP, vvXP;
     gptr<double> depthA, depthB, cellV, manning;
     gptr<uint8_t> rmv, flags;      // flags: bit0 tideP cell, bit1 tideM cell, bits 2-7 inflow count
+    gptr<uint16_t> tface;          // per cell: which of the vvYP faces its velocity interpolation
+                                   // reads the tidal adjustment sets, and how (bit pairs 0-7),
+                                   // bit 8 its jM ghost face
     // operator and solver vectors
     gptr<double> cX, cY, D, b, x, r, q, pA, pB;
     gptr<float> cX32, cY32;        // mixed precision only (else NULL): the scaled FP32
@@ -486,7 +489,6 @@ template <class R> __global__ void k_assemble(const StepArgsT<R> *__restrict__ A
 template <class R> __global__ void k_surface(const StepArgsT<R> *__restrict__ A);
 template <class R, bool FUSED> __global__ void k_limiter(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_faces(const StepArgsT<R> *__restrict__ A);
-template <class R> __global__ void k_tidal_velocity(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_finish(const StepArgsT<R> *__restrict__ A);
 template <class R, bool BUDGET> __global__ void k_scalar(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_step_end(const StepArgsT<R> *__restrict__ A);
@@ -498,7 +500,6 @@ template <class R> __global__ void k_subarea_init(const StepArgsT<R> *__restrict
 template <class R, bool FUSED> __global__ void k_limiter_sub(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_combine(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_combine_faces(const StepArgsT<R> *__restrict__ A);
-template <class R> __global__ void k_tidal_velocity_sub(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_finish_sub(const StepArgsT<R> *__restrict__ A);
 template <class R, bool BUDGET> __global__ void k_scalar_sub(const StepArgsT<R> *__restrict__ A);
 

@@ -639,33 +639,88 @@ __device__ inline void faces_cell(const StepArgsT<R> &a, int k, double &loss)
     a.Fvv[k] = v * p.dx * dY;
 }
 
-// adjustTidalVelocity (nsfunctions.c:1034-1078): one thread per tidal cell.
-// Fvv is not recomputed afterwards (the reference keeps the old flux).
-template <class R>
-__global__ void k_tidal_velocity(const StepArgsT<R> *__restrict__ A)
+// adjustTidalVelocity (nsfunctions.c:1034-1078) and adjustTidalVelocitySub
+// (subgrid.c:1291-1332), folded into the velocity interpolation, the first
+// reader of their result.  The YP boundary sets vvYP of each tidal P cell,
+// the YM boundary vvYP of the face south of each tidal M cell, from the
+// step's fluxes (left as they are: the reference does not recompute them)
+// and the cell's surface change.  tidal_cell_v: that value for tidal cell c
+// (P: isP), rounded to the state precision as the store rounds it, and
+// whether it is set (the plain YM boundary leaves a face alone where the
+// flux or the depth is zero).
+template <class R, bool SUB>
+__device__ inline bool tidal_cell_v(const StepArgsT<R> &a, int c, bool isP, double &v)
 {
-    const StepArgsT<R> &a = A[blockIdx.z];
-    if (blockIdx.x >= a.nbTide) return;
-    if (tail_off(a)) return;
-    const Grid g = a.g;
+    const Grid &g = a.g;
     const P &p = a.p;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nP = p.bcType != 2 ? p.tideLocLengthP : 0;
-    const int nM = p.bcType != 1 ? p.tideLocLengthM : 0;
-    if (q >= nP + nM) return;
-    const bool isP = q < nP;
-    const int c = isP ? a.tideLocP[q] : a.tideLocM[q - nP];
     const int i = c % g.nx, j = c / g.nx;
-    const double Fww = (sf_new(a)[c] - sf_start(a)[c]) * p.dx * p.dy / p.dt;
-    const double dc = depth_nxt(a)[c];
-    if (isP) {
-        const double all = a.Fvv[nbS(g, c, i, j)] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww;
-        if (all != 0 && dc > 0) a.vvYP[c] = all / (dc * p.dx);
-        else a.vvYP[c] = 0;
+    const double Fww = SUB ? (sf_new(a)[c] - sf_start(a)[c]) * a.Z[c] / p.dt
+                           : (sf_new(a)[c] - sf_start(a)[c]) * p.dx * p.dy / p.dt;
+    const double all = isP ? a.Fvv[nbS(g, c, i, j)] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww
+                           : -a.Fvv[c] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww;
+    double w;
+    if (SUB) {
+        const double Oyc = a.Oy[c];
+        w = (all != 0 && Oyc > 0) ? (isP ? all / Oyc : -all / Oyc) : 0;
     } else {
-        const double all = -a.Fvv[c] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww;
-        if (all != 0 && dc > 0) a.vvYP[nbS(g, c, i, j)] = -all / (dc * p.dx);
+        const double dc = depth_nxt(a)[c];
+        const bool on = all != 0 && dc > 0;
+        if (!isP && !on) return false;
+        w = on ? (isP ? all / (dc * p.dx) : -all / (dc * p.dx)) : 0;
     }
+    v = static_cast<double>(static_cast<R>(w));
+    return true;
+}
+
+// the value vvYP[k] holds after adjustTidalVelocity[Sub], for an interior
+// face k the velocity interpolation reads: the YP boundary's value where k
+// is a tidal P cell (bit 0 of b), then the YM boundary's where the cell
+// north of k is a tidal M cell (bit 1; the reference's order), else v, the
+// face's value as loaded.  set: k changed.
+template <class R, bool SUB>
+__device__ inline double tidal_v(const StepArgsT<R> &a, int k, int b, double v, bool &set)
+{
+    double w;
+    if ((b & 1) && tidal_cell_v<R, SUB>(a, k, true, w)) {
+        v = w;
+        set = true;
+    }
+    if ((b & 2) && tidal_cell_v<R, SUB>(a, k + a.g.nx, false, w)) {
+        v = w;
+        set = true;
+    }
+    return v;
+}
+
+// velocityInterp's four vvYP values for cell c (flat-index neighbours, an
+// index below 0 reads 0) after the tidal adjustment; the thread stores the
+// adjusted values of the faces it owns: vvYP[c], and for a tidal M cell of
+// the first row its jM ghost face (no interior thread reads that face).
+// Two bytes per cell (tface) say which of the faces are tidal faces and of
+// which boundary; the adjustment's stores come after all of its loads.
+struct InterpV {
+    double c, c1, mn, mn1;
+};
+template <class R, bool SUB>
+__device__ inline InterpV interp_v(const StepArgsT<R> &a, int c, int i, int j)
+{
+    const Grid &g = a.g;
+    const int nx = g.nx;
+    const auto v = a.vvYP;
+    InterpV r{v[c], v[c + 1], c - nx >= 0 ? v[c - nx] : 0.0, c + 1 - nx >= 0 ? v[c + 1 - nx] : 0.0};
+    const int t = a.nbTide > 0 ? a.tface[c] : 0;
+    if (t) {
+        bool set = false, any = false;
+        r.c = tidal_v<R, SUB>(a, c, t, r.c, set);
+        r.c1 = tidal_v<R, SUB>(a, c + 1, t >> 2, r.c1, any);
+        r.mn = tidal_v<R, SUB>(a, c - nx, t >> 4, r.mn, any);
+        r.mn1 = tidal_v<R, SUB>(a, c + 1 - nx, t >> 6, r.mn1, any);
+        double w;
+        const bool ghost = (t & 256) && tidal_cell_v<R, SUB>(a, c, false, w);
+        if (set) v[c] = r.c;
+        if (ghost) v[nbS(g, c, i, j)] = w;
+    }
+    return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -719,13 +774,13 @@ __device__ inline void volume_update(const StepArgsT<R> &a, int c, int i, int j,
 
 // the block's volume-budget partials (budget slots 0..6)
 template <class R>
-__device__ inline void store_vol_budget(const StepArgsT<R> &a, VolBudget &b, double *lds)
+__device__ inline void store_vol_budget(const StepArgsT<R> &a, VolBudget &b, double *lds, int slot)
 {
     const DD r = block_dd_sum(b.res, lds);
     double v[5] = {b.vol, b.bnd, b.in, b.clip, b.moved};
     block_sum<5>(v, lds);
     if (threadIdx.x == 0) {
-        const int mb = a.maxBlocks, k = blockIdx.x;
+        const int mb = a.maxBlocks, k = slot;
         a.bpart[0 * mb + k] = r.hi;
         a.bpart[1 * mb + k] = r.lo;
         for (int q = 0; q < 5; q++) a.bpart[(2 + q) * mb + k] = v[q];
@@ -745,7 +800,10 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgsT<R> *__restrict__
     __shared__ double lds[16 * 5];
     if (tail_off(a)) return;
     const Grid g = a.g;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    // blocks from the last row down: the tidal YP boundary's extra work in
+    // the first blocks, not in the last ones
+    const int blk = a.nb1 - 1 - (int)blockIdx.x;
+    const int c = blk * blockDim.x + threadIdx.x;
     const P &p = a.p;
     double cflx = 0, cfly = 0;
     VolBudget vb{{0.0, 0.0}, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -753,13 +811,11 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgsT<R> *__restrict__
         const int tt = cur_tt(a);
         const int nx = g.nx, i = c % nx, j = c / nx;
         const auto u = a.uuXP;
-        const auto v = a.vvYP;
         const double um1 = c - 1 >= 0 ? u[c - 1] : 0.0;
         const double um1n = c - 1 + nx >= 0 ? u[c - 1 + nx] : 0.0;
-        const double vmn = c - nx >= 0 ? v[c - nx] : 0.0;
-        const double vmn1 = c + 1 - nx >= 0 ? v[c + 1 - nx] : 0.0;
+        const InterpV v = interp_v<R, false>(a, c, i, j);
         a.uuYP[c] = 0.25 * (u[c] + um1 + u[c + nx] + um1n);
-        a.vvXP[c] = 0.25 * (v[c] + v[c + 1] + vmn + vmn1);
+        a.vvXP[c] = 0.25 * (v.c + v.c1 + v.mn + v.mn1);
 
         volume_update(a, c, i, j, tt, true, vb);
 
@@ -778,7 +834,7 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgsT<R> *__restrict__
             if (a.depthYP[c] <= p.hD) a.CDYP[c] = coeff / pow(d, 2.0 / 3.0);
         }
         cflx = fabs(u[c]) * p.dt / p.dx;
-        cfly = fabs(v[c]) * p.dt / p.dy;
+        cfly = fabs(v.c) * p.dt / p.dy;
         a.CFLx[c] = cflx;
         a.CFLy[c] = cfly;
     }
@@ -790,11 +846,11 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgsT<R> *__restrict__
     const double mx = block_max(cflx, lds);
     const double my = block_max(cfly, lds);
     if (threadIdx.x == 0) {
-        a.cflpart[blockIdx.x] = mx;
-        a.cflpart[a.maxBlocks + blockIdx.x] = my;
+        a.cflpart[blk] = mx;
+        a.cflpart[a.maxBlocks + blk] = my;
         cfl_stop_test(a, mx, my);
     }
-    if (p.budget) store_vol_budget(a, vb, lds);
+    if (p.budget) store_vol_budget(a, vb, lds, blk);
 }
 
 // ---------------------------------------------------------------------------
@@ -1633,33 +1689,6 @@ __global__ void __launch_bounds__(256) k_combine_faces(const StepArgsT<R> *__res
     if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blockIdx.x] += loss[0];
 }
 
-// adjustTidalVelocitySub (subgrid.c:1291-1332)
-template <class R>
-__global__ void k_tidal_velocity_sub(const StepArgsT<R> *__restrict__ A)
-{
-    const StepArgsT<R> &a = A[blockIdx.z];
-    if (blockIdx.x >= a.nbTide) return;
-    if (tail_off(a)) return;
-    const Grid g = a.g;
-    const P &p = a.p;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nP = p.bcType != 2 ? p.tideLocLengthP : 0;
-    const int nM = p.bcType != 1 ? p.tideLocLengthM : 0;
-    if (q >= nP + nM) return;
-    const bool isP = q < nP;
-    const int c = isP ? a.tideLocP[q] : a.tideLocM[q - nP];
-    const int i = c % g.nx, j = c / g.nx;
-    const double Fww = (sf_new(a)[c] - sf_start(a)[c]) * a.Z[c] / p.dt;
-    const double Oyc = a.Oy[c];
-    if (isP) {
-        const double all = a.Fvv[nbS(g, c, i, j)] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww;
-        a.vvYP[c] = (all != 0 && Oyc > 0) ? all / Oyc : 0;
-    } else {
-        const double all = -a.Fvv[c] + a.Fuu[nbW(g, c, i, j)] - a.Fuu[c] - Fww;
-        a.vvYP[nbS(g, c, i, j)] = (all != 0 && Oyc > 0) ? -all / Oyc : 0;
-    }
-}
-
 // velocityInterp + volumeByFluxSub + updateCDSub (useSubDrag 0: h = V/Z;
 // 2: Volp; 3, 4: Casas)
 // + thinLayerDrag + CFL numbers (nsfunctions.c:337-347, subgrid.c:1227-1263,
@@ -1672,7 +1701,10 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgsT<R> *__restri
     __shared__ double lds[16 * 5];
     if (tail_off(a)) return;
     const Grid g = a.g;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    // blocks from the last row down: the tidal YP boundary's extra work in
+    // the first blocks, not in the last ones
+    const int blk = a.nb1 - 1 - (int)blockIdx.x;
+    const int c = blk * blockDim.x + threadIdx.x;
     const P &p = a.p;
     double cflx = 0, cfly = 0;
     VolBudget vb{{0.0, 0.0}, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1680,13 +1712,11 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgsT<R> *__restri
         const int tt = cur_tt(a);
         const int nx = g.nx, i = c % nx, j = c / nx;
         const auto u = a.uuXP;
-        const auto v = a.vvYP;
         const double um1 = c - 1 >= 0 ? u[c - 1] : 0.0;
         const double um1n = c - 1 + nx >= 0 ? u[c - 1 + nx] : 0.0;
-        const double vmn = c - nx >= 0 ? v[c - nx] : 0.0;
-        const double vmn1 = c + 1 - nx >= 0 ? v[c + 1 - nx] : 0.0;
+        const InterpV v = interp_v<R, true>(a, c, i, j);
         a.uuYP[c] = 0.25 * (u[c] + um1 + u[c + nx] + um1n);
-        a.vvXP[c] = 0.25 * (v[c] + v[c + 1] + vmn + vmn1);
+        a.vvXP[c] = 0.25 * (v.c + v.c1 + v.mn + v.mn1);
 
         volume_update(a, c, i, j, tt, p.bcType != 3, vb);
 
@@ -1725,7 +1755,7 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgsT<R> *__restri
             if (a.depthYP[c] <= p.hD) a.CDYP[c] = coeff / pow(d, 2.0 / 3.0);
         }
         cflx = fabs(u[c]) * p.dt / p.dx;
-        cfly = fabs(v[c]) * p.dt / p.dy;
+        cfly = fabs(v.c) * p.dt / p.dy;
         a.CFLx[c] = cflx;
         a.CFLy[c] = cfly;
     }
@@ -1736,11 +1766,11 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgsT<R> *__restri
     const double mx = block_max(cflx, lds);
     const double my = block_max(cfly, lds);
     if (threadIdx.x == 0) {
-        a.cflpart[blockIdx.x] = mx;
-        a.cflpart[a.maxBlocks + blockIdx.x] = my;
+        a.cflpart[blk] = mx;
+        a.cflpart[a.maxBlocks + blk] = my;
         cfl_stop_test(a, mx, my);
     }
-    if (p.budget) store_vol_budget(a, vb, lds);
+    if (p.budget) store_vol_budget(a, vb, lds, blk);
 }
 
 // matrixSourceTermSub + matrixCoeffSub (useSubgrid = 1, staggeredV = 1) +
@@ -1871,9 +1901,5 @@ template __global__ void k_subarea_init<double>(const StepArgsT<double> *__restr
 template __global__ void k_subarea_init<float>(const StepArgsT<float> *__restrict__);
 template __global__ void k_surface<double>(const StepArgsT<double> *__restrict__);
 template __global__ void k_surface<float>(const StepArgsT<float> *__restrict__);
-template __global__ void k_tidal_velocity<double>(const StepArgsT<double> *__restrict__);
-template __global__ void k_tidal_velocity<float>(const StepArgsT<float> *__restrict__);
-template __global__ void k_tidal_velocity_sub<double>(const StepArgsT<double> *__restrict__);
-template __global__ void k_tidal_velocity_sub<float>(const StepArgsT<float> *__restrict__);
 
 }  // namespace frehd
