@@ -660,6 +660,7 @@ void fill_extents(frehd_dev *d)
     a.nbGhost = (2 * k.nx + 2 * k.ny + 255) / 256;
     const int nt = (k.bcType != 2 ? k.tideLocLengthP : 0) + (k.bcType != 1 ? k.tideLocLengthM : 0);
     a.nbTide = (nt + 255) / 256;
+    a.rowsDown = k.bcType != 2;
     a.nbRes = (int)pcg_blocks(d->g.N);
 }
 

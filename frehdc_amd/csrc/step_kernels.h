@@ -149,6 +149,7 @@ struct StepArgsT {
     // 64x4 blocks, sweeps of 256 over N, ghost cells and tidal cells, and
     // the grid-stride solver grid
     int gx2, gy2, nbN, nbGhost, nbTide, nbRes;
+    int rowsDown;                  // the 2-D kernels' blocks take rows from the last down (row_block)
     // forcing (Nt+1 samples) and locations
     gptr<const double> tideP, tideM, inflow, tidalPS, tidalMS;
     // wind per step: speed and the cosine / sine of its direction (computed
@@ -415,6 +416,17 @@ template <class R>
 __device__ inline gptr<const double> depth_cur(const StepArgsT<R> &a) { return a.ctl->parity ? a.depthB : a.depthA; }
 template <class R>
 __device__ inline gptr<double> depth_nxt(const StepArgsT<R> &a) { return a.ctl->parity ? a.depthA : a.depthB; }
+// The row band of a 2-D stencil block.  Blocks are dispatched in index
+// order and the last ones set a kernel's end, so the bands start at the
+// tidal boundary, where the wet cells (and the subgrid lookups) and the
+// boundary work sit: from the last row down unless the tide enters only
+// through the YM boundary.
+template <class R>
+__device__ inline int row_block(const StepArgsT<R> &a)
+{
+    return a.rowsDown ? a.gy2 - 1 - (int)blockIdx.y : (int)blockIdx.y;
+}
+
 // the kernels after the solve run only for a member that is not halted,
 // whose solve has finished and whose step they have not completed yet (a
 // group launches them again when one member's solve needed more iterations)

@@ -214,7 +214,7 @@ __device__ inline void predictor_body(const StepArgsT<R> &a)
     if (a.ctl->halted || a.ctl->haltPending) return;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int j = row_block(a) * blockDim.y + threadIdx.y;
     if (i >= g.nx || j >= g.ny) return;
     const int c = j * g.nx + i;
     store_pred(a, c, i, j, predict<R, SUB>(a, c, i, j));
@@ -389,7 +389,7 @@ __device__ inline void store_vloss(const StepArgsT<R> &a, double (&loss)[NL], do
 {
     block_sum<NL>(loss, lds);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const int bid = blockIdx.x + a.gx2 * blockIdx.y;
+        const int bid = blockIdx.x + a.gx2 * row_block(a);
 #pragma unroll
         for (int q = 0; q < NL; q++) a.vpart[(size_t)q * a.maxBlocks + bid] += loss[q];
     }
@@ -436,7 +436,7 @@ __global__ void __launch_bounds__(256) k_limiter(const StepArgsT<R> *__restrict_
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) a.ctl->flipPending = 1;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int j = row_block(a) * blockDim.y + threadIdx.y;
     double loss[3] = {0.0, 0.0, 0.0};
     double (&lim)[2] = *reinterpret_cast<double (*)[2]>(loss);
     if (i < g.nx && j < g.ny) {
@@ -538,11 +538,14 @@ __global__ void __launch_bounds__(256) k_faces(const StepArgsT<R> *__restrict__ 
     if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
     if (tail_off(a)) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    // blocks from the last index down: the ghost faces, which recompute the
+    // faces they mirror, in the first blocks, not in the last ones
+    const int blk = a.nb1 - 1 - (int)blockIdx.x;
+    const int k = blk * blockDim.x + threadIdx.x;
     double loss[1] = {0.0};
     if (k < a.g.Ng) faces_cell(a, k, loss[0]);
     block_sum<1>(loss, lds);
-    if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blockIdx.x] += loss[0];
+    if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blk] += loss[0];
 }
 
 // updateVelocity + waterfallCorrection of interior cell c's east and north
@@ -800,9 +803,9 @@ __global__ void __launch_bounds__(256) k_finish(const StepArgsT<R> *__restrict__
     __shared__ double lds[16 * 5];
     if (tail_off(a)) return;
     const Grid g = a.g;
-    // blocks from the last row down: the tidal YP boundary's extra work in
-    // the first blocks, not in the last ones
-    const int blk = a.nb1 - 1 - (int)blockIdx.x;
+    // blocks start at the tidal boundary (row_block), whose faces take
+    // extra work here
+    const int blk = a.rowsDown ? a.nb1 - 1 - (int)blockIdx.x : (int)blockIdx.x;
     const int c = blk * blockDim.x + threadIdx.x;
     const P &p = a.p;
     double cflx = 0, cfly = 0;
@@ -909,7 +912,7 @@ __device__ inline void scalar_body(const StepArgsT<R> &a, double *lds)
     if (tail_off(a) || a.p.nScalar < 1) return;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int j = row_block(a) * blockDim.y + threadIdx.y;
     // every thread of the block stays to the end (the budget's block sums);
     // the cells outside the grid do nothing
     const bool valid = i < g.nx && j < g.ny;
@@ -1042,7 +1045,7 @@ __device__ inline void scalar_body(const StepArgsT<R> &a, double *lds)
             double v[4] = {mb.mass, mb.bnd, mb.in, mb.moved};
             block_sum<4>(v, lds);
             if (threadIdx.x == 0 && threadIdx.y == 0) {
-                const int mbk = a.maxBlocks, k = blockIdx.x + a.gx2 * blockIdx.y;
+                const int mbk = a.maxBlocks, k = blockIdx.x + a.gx2 * row_block(a);
                 const int s0 = kBudgetVolSlots + kBudgetScalarSlots * q;
                 a.bpart[(size_t)(s0 + 0) * mbk + k] = r.hi;
                 a.bpart[(size_t)(s0 + 1) * mbk + k] = r.lo;
@@ -1331,7 +1334,7 @@ __global__ void __launch_bounds__(256) k_limiter_sub(const StepArgsT<R> *__restr
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) a.ctl->flipPending = 1;
     const Grid g = a.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int j = row_block(a) * blockDim.y + threadIdx.y;
     double loss[3] = {0.0, 0.0, 0.0};
     double (&lim)[2] = *reinterpret_cast<double (*)[2]>(loss);
     if (i < g.nx && j < g.ny) {
@@ -1682,11 +1685,14 @@ __global__ void __launch_bounds__(256) k_combine_faces(const StepArgsT<R> *__res
     if (blockIdx.x >= a.nb1) return;
     __shared__ double lds[16];
     if (tail_off(a)) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    // blocks from the last index down: the ghost faces, which recompute the
+    // faces they mirror, in the first blocks, not in the last ones
+    const int blk = a.nb1 - 1 - (int)blockIdx.x;
+    const int k = blk * blockDim.x + threadIdx.x;
     double loss[1] = {0.0};
     if (k < a.g.Ng) combine_faces_cell(a, k, loss[0]);
     block_sum<1>(loss, lds);
-    if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blockIdx.x] += loss[0];
+    if (threadIdx.x == 0) a.vpart[3 * a.maxBlocks + blk] += loss[0];
 }
 
 // velocityInterp + volumeByFluxSub + updateCDSub (useSubDrag 0: h = V/Z;
@@ -1701,9 +1707,9 @@ __global__ void __launch_bounds__(256) k_finish_sub(const StepArgsT<R> *__restri
     __shared__ double lds[16 * 5];
     if (tail_off(a)) return;
     const Grid g = a.g;
-    // blocks from the last row down: the tidal YP boundary's extra work in
-    // the first blocks, not in the last ones
-    const int blk = a.nb1 - 1 - (int)blockIdx.x;
+    // blocks start at the tidal boundary (row_block), whose faces take
+    // extra work here
+    const int blk = a.rowsDown ? a.nb1 - 1 - (int)blockIdx.x : (int)blockIdx.x;
     const int c = blk * blockDim.x + threadIdx.x;
     const P &p = a.p;
     double cflx = 0, cfly = 0;
