@@ -662,6 +662,10 @@ void fill_extents(frehd_dev *d)
     a.nbTide = (nt + 255) / 256;
     a.rowsDown = k.bcType != 2;
     a.nbRes = (int)pcg_blocks(d->g.N);
+    // the assembly: one cell per thread, unless it also sums the FP64 cold
+    // start's partials (r.r, r.z, b.b: nbRes of them, for the solver's first
+    // launch to re-reduce)
+    a.nbAsm = (k.precision == FREHD_PREC_MIXED || k.warmStart != 0) ? a.nbN : a.nbRes;
 }
 
 int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **out, frehd_dev **dd)

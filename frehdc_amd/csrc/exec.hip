@@ -541,8 +541,8 @@ int exec_collect_timing(Exec &e)
 int enqueue_assemble(Exec &e)
 {
     const int n = e.count;
-    if (e.sub) LAUNCH_ST(K_ASM, k_assemble_sub, grid1(e.ext.nbRes, n), dim3(256));
-    else LAUNCH_ST(K_ASM, k_assemble, grid1(e.ext.nbRes, n), dim3(256));
+    if (e.sub) LAUNCH_ST(K_ASM, k_assemble_sub, grid1(e.ext.nbAsm, n), dim3(256));
+    else LAUNCH_ST(K_ASM, k_assemble, grid1(e.ext.nbAsm, n), dim3(256));
     return 0;
 }
 
@@ -893,6 +893,7 @@ Extents member_extents(const frehd_dev_impl *d)
     x.nbN = a.nbN;
     x.nbGhost = a.nbGhost;
     x.nbRes = a.nbRes;
+    x.nbAsm = a.nbAsm;
     const bool mixed = d->cfg.precision == FREHD_PREC_MIXED;
     x.nbDir = mixed ? d->pcg32.nbDir : d->pcg.nbDir;
     x.nbRef = mixed ? d->ref.nb : 0;
@@ -921,6 +922,7 @@ static void merge(Exec &e, const frehd_dev_impl *d, bool first)
     x.nbN = std::max(x.nbN, m.nbN);
     x.nbGhost = std::max(x.nbGhost, m.nbGhost);
     x.nbRes = std::max(x.nbRes, m.nbRes);
+    x.nbAsm = std::max(x.nbAsm, m.nbAsm);
     x.nbDir = std::max(x.nbDir, m.nbDir);
     x.nbRef = std::max(x.nbRef, m.nbRef);
     x.ssorN = std::max(x.ssorN, m.ssorN);

@@ -69,7 +69,7 @@ struct Timing {
 
 // launch extents of a group: the maximum over its members
 struct Extents {
-    int gx2, gy2, nb1, nbN, nbGhost, nbRes, nbDir, nbRef;
+    int gx2, gy2, nb1, nbN, nbGhost, nbRes, nbAsm, nbDir, nbRef;
     int blockX;     // the 2-D stencil kernels' block: blockX x (256 / blockX) (shared by the group)
     int lanes;      // columns per lane of k_cg1 / k_pcg1 (shared by the group)
     int ssorN;      // largest N of the members preconditioned with SSOR (0: none)

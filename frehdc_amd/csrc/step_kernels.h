@@ -150,6 +150,7 @@ struct StepArgsT {
     // the grid-stride solver grid
     int gx2, gy2, nbN, nbGhost, nbTide, nbRes;
     int rowsDown;                  // the 2-D kernels' blocks take rows from the last down (row_block)
+    int nbAsm;                     // the assembly's blocks (nbRes when it writes the solver's partials)
     // forcing (Nt+1 samples) and locations
     gptr<const double> tideP, tideM, inflow, tidalPS, tidalMS;
     // wind per step: speed and the cosine / sine of its direction (computed

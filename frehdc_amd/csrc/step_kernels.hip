@@ -1828,7 +1828,7 @@ __device__ inline void assemble_body(const StepArgsT<R> &a, double *lds)
     if (!a.ctl->halted) {
         const Grid g = a.g;
         const int tt = cur_tt(a);
-        for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += a.nbRes * blockDim.x) {
+        for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += a.nbAsm * blockDim.x) {
             const int i = c % g.nx, j = c / g.nx;
             const int w = nbW(g, c, i, j), s = nbS(g, c, i, j);
             RowEn q;
@@ -1844,6 +1844,7 @@ __device__ inline void assemble_body(const StepArgsT<R> &a, double *lds)
             store_row(a, c, r, part);
         }
     }
+    if (a.nbAsm != a.nbRes) return;   // (no partials wanted)
     block_sum<3>(part, lds);
     store_partials<3>(a.partA, a.nbRes, {part[0], part[1], part[2]});
 }
@@ -1852,7 +1853,7 @@ template <class R>
 __global__ void __launch_bounds__(256) k_assemble(const StepArgsT<R> *__restrict__ A)
 {
     const StepArgsT<R> &a = A[blockIdx.z];
-    if (blockIdx.x >= a.nbRes) return;
+    if (blockIdx.x >= a.nbAsm) return;
     __shared__ double lds[16 * 3];
     assemble_body<R, false>(a, lds);
 }
@@ -1861,7 +1862,7 @@ template <class R>
 __global__ void __launch_bounds__(256) k_assemble_sub(const StepArgsT<R> *__restrict__ A)
 {
     const StepArgsT<R> &a = A[blockIdx.z];
-    if (blockIdx.x >= a.nbRes) return;
+    if (blockIdx.x >= a.nbAsm) return;
     __shared__ double lds[16 * 3];
     assemble_body<R, true>(a, lds);
 }
