@@ -267,6 +267,17 @@ inline int graphs_from_env()
     return kLaunchGraphs;
 }
 
+// FREHD_PATTERN_CAP in the environment at a handle's or batch's creation:
+// at most that many solver launches enqueued ahead (FP64 iterations, or the
+// inner iterations of the mixed solve's first sweep, which is then its only
+// pattern sweep), so that solves are left open and finished by the host --
+// a test hook for that path; 0 or unset: the covering pattern
+inline int pattern_cap_from_env()
+{
+    const char *v = getenv("FREHD_PATTERN_CAP");
+    return v && v[0] ? std::max(0, atoi(v)) : 0;
+}
+
 // the most adjacent columns per lane the configuration's solver takes:
 // k_cg1 4 (mixed), k_pcg1 2 (FP64 Jacobi), k_pcg_dir 1 (SSOR)
 inline int max_lanes(const frehd_config &k)
@@ -719,6 +730,7 @@ int create_common(const frehd_config *cfg, const frehd_forcing *f, frehd_dev **o
     d->self.stream = d->stream;
     d->self.tim = &d->timing;
     d->self.useGraphs = graphs_from_env();
+    d->self.patternCap = pattern_cap_from_env();
     HIPCHK(alloc_flags(d->self, 1));
     d->midStep = 0;
     d->tt = 0;
@@ -1697,6 +1709,7 @@ int frehd_batch_create(int32_t capacity, frehd_batch_t **out)
     HIPCHK(hipStreamSynchronize(e.stream));
     e.tim = &b->timing;
     e.useGraphs = graphs_from_env();
+    e.patternCap = pattern_cap_from_env();
     *out = b;
     return 0;
 }

@@ -36,7 +36,8 @@ Timing::~Timing()
 // array so that one copy brings them to the host; a member whose solve has
 // not finished is marked open, so the kernels after the solve stand down
 // until it has.  testS >= 0: first the outer test of the mixed solve's
-// refinement sweep testS (R: the members' refinement arguments).
+// refinement sweep testS (R: the members' refinement arguments).  out =
+// nullptr: the solve is closed on the device only, no host write.
 __global__ void __launch_bounds__(256) k_gather_flags(const StepArgs *__restrict__ A, int count, int mixed,
                                                       int32_t *out, const RefineArgs *__restrict__ R, int testS,
                                                       int issue)
@@ -50,7 +51,7 @@ __global__ void __launch_bounds__(256) k_gather_flags(const StepArgs *__restrict
     if (testS >= 0) refine_test(R[m], testS, lds, !c->halted && c->ref.doneAt < 0);
     if (threadIdx.x != 0) return;
     close_solve(c, mixed != 0, issue != 0);
-    write_flags(c, out + (size_t)m * F_STRIDE);
+    if (out) write_flags(c, out + (size_t)m * F_STRIDE);
 }
 
 namespace {
@@ -116,13 +117,16 @@ void remember(Exec &e, const SolveCounts &counts)
 }
 
 // issue: the kernels after the solve follow this gather (members whose
-// solve has ended get their go-ahead)
-int enqueue_gather(Exec &e, int issue)
+// solve has ended get their go-ahead).  host: the flags go to the host
+// buffer too.  A gather the host does not wait for must not write them: it
+// could land after the host has cleared the flags for the next step, whose
+// wait would then take them for that step's (and miss an open solve).
+int enqueue_gather(Exec &e, int issue, bool host)
 {
     const int testS = e.mixed ? e.pendTest : -1;
     e.pendTest = -1;
-    LAUNCH(K_GATHER, k_gather_flags, dim3((unsigned)e.count), dim3(256), e.dArgs, e.count, e.mixed, e.hFlagsDev,
-           e.dRef, testS, issue);
+    LAUNCH(K_GATHER, k_gather_flags, dim3((unsigned)e.count), dim3(256), e.dArgs, e.count, e.mixed,
+           host ? e.hFlagsDev : (int32_t *)nullptr, e.dRef, testS, issue);
     return 0;
 }
 
@@ -163,7 +167,7 @@ void clear_flags(Exec &e)
 
 int gather(Exec &e)
 {
-    int rc = enqueue_gather(e, 0);
+    int rc = enqueue_gather(e, 0, true);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(e.stream));
     return 0;
@@ -256,7 +260,8 @@ void classify_timed(Exec &e)
 // until every member reports convergence (or a halt).
 int solve_fp64_pattern(Exec &e, int *it)
 {
-    const int n = fp64_launches(e, std::max(1, e.lastIters));
+    int n = fp64_launches(e, std::max(1, e.lastIters));
+    if (e.patternCap > 0) n = std::min(n, e.patternCap);
     for (int q = 0; q < n; q++) {
         int rc = enqueue_pcg_iteration(e, q);
         if (rc) return rc;
@@ -350,8 +355,10 @@ int solve_mixed_pattern(Exec &e, int *sweeps)
     int rc, s = 0;
     e.pendTest = -1;
     if ((rc = enqueue_refine(e, 0))) return rc;
-    for (int k = 0; k < e.mixPlan.n; k++) {
-        if ((rc = enqueue_inner_solve(e, s, e.mixPlan.v[k]))) return rc;
+    const int nk = e.patternCap > 0 ? std::min(e.mixPlan.n, 1) : e.mixPlan.n;
+    for (int k = 0; k < nk; k++) {
+        const int n = e.patternCap > 0 ? std::min(e.mixPlan.v[k], e.patternCap) : e.mixPlan.v[k];
+        if ((rc = enqueue_inner_solve(e, s, n))) return rc;
         if ((rc = enqueue_refine(e, ++s))) return rc;
     }
     *sweeps = s;
@@ -612,7 +619,7 @@ static int exec_phase_launch(Exec &e, int phase)
         if (!rc) rc = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
         if (!rc) rc = gather(e);
         if (!rc) rc = e.mixed ? solve_mixed_finish(e, k) : solve_fp64_finish(e, k);
-        if (!rc) rc = enqueue_gather(e, 1);
+        if (!rc) rc = enqueue_gather(e, 1, false);   // (the flags came with the finish)
         if (rc) return rc;
         LAUNCH_ST(K_SURF, k_surface, grid1(x.nb1, n), b1);
         return 0;
@@ -735,7 +742,7 @@ static int exec_step_streamed(Exec &e)
             if (!r) r = enqueue_fused_solve(e);
         } else {
             if (!r) r = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
-            if (!r) r = enqueue_gather(e, 1);
+            if (!r) r = enqueue_gather(e, 1, true);
         }
         if (!r) r = enqueue_tail(e);     // (its phase-4 bookkeeping advances the members' step)
         if (r) return r;
@@ -750,7 +757,7 @@ static int exec_step_streamed(Exec &e)
     rc = e.mixed ? solve_mixed_finish(e, k) : solve_fp64_finish(e, k);
     if (rc) return rc;
     if (open) {
-        if ((rc = enqueue_gather(e, 1))) return rc;
+        if ((rc = enqueue_gather(e, 1, false))) return rc;   // (not waited for)
         if ((rc = enqueue_tail(e))) return rc;
         for (auto *m : e.members) m->tt -= 1;   // the step advanced once
     }
@@ -801,7 +808,7 @@ static int exec_step_graphs(Exec &e)
                 if (!r) r = enqueue_fused_solve(e);
             } else {
                 if (!r) r = e.mixed ? solve_mixed_pattern(e, &k) : solve_fp64_pattern(e, &k);
-                if (!r) r = enqueue_gather(e, 1);
+                if (!r) r = enqueue_gather(e, 1, true);
             }
             if (!r) r = enqueue_tail(e);
             return r;
@@ -811,7 +818,10 @@ static int exec_step_graphs(Exec &e)
         for (auto *m : e.members) m->tt -= 1;
         e.heads.push_back({key, step});
     } else {
-        k = e.mixed ? e.mixPlan.n : fp64_launches(e, std::max(1, e.lastIters));
+        // (what the captured pattern enqueued: its sweeps, or its FP64 launches)
+        k = e.mixed ? (e.patternCap > 0 ? std::min(e.mixPlan.n, 1) : e.mixPlan.n)
+                    : fp64_launches(e, std::max(1, e.lastIters));
+        if (!e.mixed && e.patternCap > 0) k = std::min(k, e.patternCap);
     }
     clear_flags(e);
     HIPCHK(hipGraphLaunch(step, e.stream));
@@ -830,7 +840,7 @@ static int exec_step_graphs(Exec &e)
     rc = e.mixed ? solve_mixed_finish(e, k) : solve_fp64_finish(e, k);
     if (rc) return rc;
     if (open) {
-        if ((rc = enqueue_gather(e, 1))) return rc;
+        if ((rc = enqueue_gather(e, 1, false))) return rc;   // (not waited for)
         if (!e.tail) {
             rc = capture(e, [&] { return enqueue_tail(e); }, &e.tail);
             if (rc) return rc;

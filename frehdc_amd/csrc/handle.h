@@ -117,6 +117,7 @@ struct Exec {
     // the solve), and those last kernels alone, for a step whose solve the
     // pattern left open; rebuilt whenever the members' argument blocks change
     int useGraphs = 1;                  // launch mode: kLaunchEager, kLaunchGraphs, kLaunchStreamed, kLaunchAuto
+    int patternCap = 0;                 // > 0: at most that many solver launches enqueued ahead (test hook)
     std::vector<std::pair<std::vector<int>, hipGraphExec_t>> heads;
     hipGraphExec_t tail = nullptr;
     // place in the solve of the launches being enqueued (see TimedLaunch)
