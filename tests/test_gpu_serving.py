@@ -28,6 +28,26 @@ def test_serving_results_equal_solo_runs():
             assert np.array_equal(solo[k].view(np.int64), r.result[k].view(np.int64)), (r.rid, k)
 
 
+def test_serving_warm_fp64_equals_solo_runs():
+    # warm FP64 starts vary their iteration counts from step to step and
+    # member to member, so the shared pattern leaves solves open that the
+    # host finishes, with several buckets stepping from their own threads;
+    # the FP64 iterates do not depend on that, so each result is still the
+    # solo run's, bit for bit, with every step taken
+    print("case serving 16 requests, warm FP64, sizes 64..512, steps 40..160", flush=True)
+    reqs = serving.make_requests(16, rate=400.0, seed=21, lo=64, hi=512, steps_lo=40, steps_hi=160, subgrid_r=4)
+    srv = serving.Server(budget_bytes=20e9, quantum=16, collect=["surf", "depth"], warmStart=2)
+    srv.run(reqs)
+    for r in reqs:
+        d = DeviceScenario(r.scen, warmStart=2)
+        d.step(r.steps)
+        assert d.stats()["steps"] == r.steps
+        solo = d.download(["surf", "depth"])
+        d.close()
+        for k in ("surf", "depth"):
+            assert np.array_equal(solo[k], r.result[k]), (r.rid, k, float(np.max(np.abs(solo[k] - r.result[k]))))
+
+
 def test_serving_budget_limits_residency():
     print("case serving under a tight HBM budget", flush=True)
     reqs = serving.make_requests(6, rate=1000.0, seed=12, lo=128, hi=128, steps_lo=10, steps_hi=10)
