@@ -596,7 +596,8 @@ def main():
                                f"one scalar, Jacobi PCG eps=1e-10, form {head}; timed: steps "
                                f"{4 + args.spinup + args.warmup}-{3 + args.spinup + args.warmup + args.steps} "
                                f"of a {args.job or 3 + args.spinup + args.warmup + args.steps}-step run from rest "
-                               f"(forms.*.job: the average over all of its steps after the autotuner)",
+                               f"(the solver tiling tuned after 3 steps, the stencil block shape within the "
+                               f"spin-up; forms.*.job: the average over all of its steps after the tuners)",
                    "cells": N,
                    "form": head, "forms": forms_out},
         "roofline": roof,

@@ -7,9 +7,9 @@ steps and `--steps` steps timed.  Prints one line per form.
 
     rocprofv3 --kernel-trace --stats ... -- python3 -m frehdc_amd.profile_run --workload marsh --forms fp64,mixed_warm
 
-A process that steps several handles or a batch under rocprofv3 runs with
-FREHD_GRAPHS=0 in its environment (kernel-by-kernel launches: the per-kernel
-durations are the same, the gaps between launches are not graph replays).
+Every workload here has at least 2^18 cells, so its steps run as streamed
+launches (no captured graphs) unless FREHD_GRAPHS=1 in the environment
+forces graph replays.
 """
 from __future__ import annotations
 

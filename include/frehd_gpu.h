@@ -406,9 +406,10 @@ int frehd_dev_autotune(frehd_dev_t *dev, int32_t iters, int32_t *rows_per_tile, 
  * the solver flags -- runs as one persistent launch with grid-wide barriers
  * instead of one launch per sweep or iteration; results are bitwise those of
  * the launched solve.  Its blocks must be resident on the device together:
- * max_blocks < 0 switches it off (the launched solve), 0 (the default)
- * lets it take as many blocks as the device holds at once, > 0 caps it
- * (the default is off).  A
+ * max_blocks < 0 switches it off (the launched solve, where a handle
+ * starts: on MI355X a grid-wide barrier costs more than the kernel boundary
+ * it replaces), 0 lets it take as many blocks as the device holds at once,
+ * > 0 caps it.  A
  * caller that steps several handles or batches concurrently (one stream
  * each) divides frehd_fused_solve_capacity among them; a launch whose blocks
  * were not resident together ends after a 2 s barrier wait with FREHD_EHIP.
