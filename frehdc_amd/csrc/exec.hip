@@ -566,11 +566,15 @@ static int enqueue_update(Exec &e, bool fusedSurface)
     const auto *a32 = reinterpret_cast<const StepArgsT<float> *>(e.dArgs);
     const int v = (e.f32 ? 2 : 0) + (fusedSurface ? 1 : 0);
     if (e.sub) {
-        switch (v) {
-        case 0: LAUNCH(K_LIM, (k_limiter_sub<double, false>), g2, b2, e.dArgs); break;
-        case 1: LAUNCH(K_LIM, (k_limiter_sub<double, true>), g2, b2, e.dArgs); break;
-        case 2: LAUNCH(K_LIM, (k_limiter_sub<float, false>), g2, b2, a32); break;
-        default: LAUNCH(K_LIM, (k_limiter_sub<float, true>), g2, b2, a32); break;
+        switch (v + (x.otf ? 4 : 0)) {
+        case 0: LAUNCH(K_LIM, (k_limiter_sub<double, false, false>), g2, b2, e.dArgs); break;
+        case 1: LAUNCH(K_LIM, (k_limiter_sub<double, true, false>), g2, b2, e.dArgs); break;
+        case 2: LAUNCH(K_LIM, (k_limiter_sub<float, false, false>), g2, b2, a32); break;
+        case 3: LAUNCH(K_LIM, (k_limiter_sub<float, true, false>), g2, b2, a32); break;
+        case 4: LAUNCH(K_LIM, (k_limiter_sub<double, false, true>), g2, b2, e.dArgs); break;
+        case 5: LAUNCH(K_LIM, (k_limiter_sub<double, true, true>), g2, b2, e.dArgs); break;
+        case 6: LAUNCH(K_LIM, (k_limiter_sub<float, false, true>), g2, b2, a32); break;
+        default: LAUNCH(K_LIM, (k_limiter_sub<float, true, true>), g2, b2, a32); break;
         }
         LAUNCH_ST(K_FACES, k_combine_faces, grid1(x.nb1, n), b1);
         LAUNCH_ST(K_FINISH, k_finish_sub, grid1(x.nb1, n), b1);
@@ -911,6 +915,7 @@ Extents member_extents(const frehd_dev_impl *d)
     x.ssorN = d->pcg.ssor ? d->g.N : 0;
     x.warm = !mixed && d->pcg.warm;
     x.budget = d->args.p.budget;
+    x.otf = d->cfg.useSubgrid == 1 && d->cfg.subgridOnTheFly == 1;
     x.ssorLds = !d->pcg.ssor || d->g.N <= d->pcg.ldsCells;
     return x;
 }
@@ -938,6 +943,7 @@ static void merge(Exec &e, const frehd_dev_impl *d, bool first)
     x.ssorN = std::max(x.ssorN, m.ssorN);
     x.warm = x.warm || m.warm;
     x.budget = x.budget || m.budget;
+    x.otf = x.otf || m.otf;
     x.ssorLds = x.ssorLds && m.ssorLds;
     e.maxIter = std::max(e.maxIter, d->cfg.maxIter);
     e.maxRefine = std::max(e.maxRefine, d->cfg.maxRefine);

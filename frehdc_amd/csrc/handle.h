@@ -76,6 +76,7 @@ struct Extents {
     int warm;       // an FP64 member starts its solve from a nonzero x
     int ssorLds;    // every SSOR member fits k_ssor's LDS vector
     int budget;     // a member accounts its conservation budgets
+    int otf;        // a member evaluates its subgrid lookups on the fly
 };
 
 struct frehd_dev_impl;

@@ -510,7 +510,7 @@ __global__ void k_swap_surface(double *a, double *b, int n);
 template <class R> __global__ void k_predictor_sub(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_assemble_sub(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_subarea_init(const StepArgsT<R> *__restrict__ A);
-template <class R, bool FUSED> __global__ void k_limiter_sub(const StepArgsT<R> *__restrict__ A);
+template <class R, bool FUSED, bool OTF> __global__ void k_limiter_sub(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_combine(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_combine_faces(const StepArgsT<R> *__restrict__ A);
 template <class R> __global__ void k_finish_sub(const StepArgsT<R> *__restrict__ A);

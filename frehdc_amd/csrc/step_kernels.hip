@@ -1207,7 +1207,7 @@ __device__ inline SubPre subarea_pre(const StepArgsT<R> &a, int c, int i, int j)
 // computeSubArea for cell c (subgrid.c:55-251): walk the level index to the
 // surface, interpolate the half-cell volumes and face areas between the two
 // bracketing levels, zero what is dry.
-template <class R>
+template <class R, bool OTF>
 __device__ inline void subarea_cell(const StepArgsT<R> &a, int c, int i, int j, double h, double depthOld,
                                     const SubPre &q)
 {
@@ -1246,7 +1246,7 @@ __device__ inline void subarea_cell(const StepArgsT<R> &a, int c, int i, int j, 
     // (tables in the state precision: evaluated values are rounded to it,
     // so both hold the same numbers)
     double t1[10], t2[10];
-    if (a.tabs) {
+    if (!OTF || a.tabs) {
         const auto p1 = a.tabs + tab_index(c, j1, N, 0), p2 = a.tabs + tab_index(c, j2, N, 0);
 #pragma unroll
         for (int q = 0; q < 10; q++) {
@@ -1256,7 +1256,10 @@ __device__ inline void subarea_cell(const StepArgsT<R> &a, int c, int i, int j, 
     } else {
         const double hl[2] = {level_raw(p.surfmin, dsurf, j1), level_raw(p.surfmin, dsurf, j2)};
         double v[2][10];
-        cell_levels<2>(a.bed, i, j, hl, v);
+        switch (a.bed.r) {            // the common pixel counts unrolled
+        case 4: cell_levels<2, 4>(a.bed, i, j, hl, v); break;
+        default: cell_levels<2>(a.bed, i, j, hl, v); break;
+        }
 #pragma unroll
         for (int q = 0; q < 10; q++) {
             t1[q] = (R)v[0][q];
@@ -1317,13 +1320,15 @@ __global__ void __launch_bounds__(256) k_subarea_init(const StepArgsT<R> *__rest
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const int i = c % g.nx, j = c / g.nx;
-    subarea_cell(a, c, i, j, sf_start(a)[c], depth_cur(a)[c], subarea_pre(a, c, i, j));
+    subarea_cell<R, true>(a, c, i, j, sf_start(a)[c], depth_cur(a)[c], subarea_pre(a, c, i, j));
 }
 
 // limiterCFL + computeVolumeSub + computeSubArea + updateDepth
 // (nsfunctions.c:452-495, subgrid.c:1215-1224, 55-251, nsfunctions.c:239-247)
-// FUSED: the surface update is made here too (see k_limiter)
-template <class R, bool FUSED>
+// FUSED: the surface update is made here too (see k_limiter); OTF: a
+// member may evaluate its lookups on the fly from the pixels (the variant
+// for tables only keeps the registers, and the occupancy, of that path)
+template <class R, bool FUSED, bool OTF>
 __global__ void __launch_bounds__(256) k_limiter_sub(const StepArgsT<R> *__restrict__ A)
 {
     const StepArgsT<R> &a = A[blockIdx.z];
@@ -1365,7 +1370,7 @@ __global__ void __launch_bounds__(256) k_limiter_sub(const StepArgsT<R> *__restr
             if (q < p.nScalar) a.Sm[q * ss + c] = sv[q] * cellV;
         for (int q = NS; q < p.nScalar; q++) a.Sm[q * ss + c] = s_cur(a)[q * ss + c] * cellV;
         depth_nxt(a)[c] = d < 0 ? 0 : d;
-        subarea_cell(a, c, i, j, h, dOld, pre);
+        subarea_cell<R, OTF>(a, c, i, j, h, dOld, pre);
     }
     if (FUSED) store_vloss<R, 3>(a, loss, lds);
     else store_vloss<R, 2>(a, lim, lds);
@@ -1886,10 +1891,14 @@ template __global__ void k_limiter<double, false>(const StepArgsT<double> *__res
 template __global__ void k_limiter<double, true>(const StepArgsT<double> *__restrict__);
 template __global__ void k_limiter<float, false>(const StepArgsT<float> *__restrict__);
 template __global__ void k_limiter<float, true>(const StepArgsT<float> *__restrict__);
-template __global__ void k_limiter_sub<double, false>(const StepArgsT<double> *__restrict__);
-template __global__ void k_limiter_sub<double, true>(const StepArgsT<double> *__restrict__);
-template __global__ void k_limiter_sub<float, false>(const StepArgsT<float> *__restrict__);
-template __global__ void k_limiter_sub<float, true>(const StepArgsT<float> *__restrict__);
+template __global__ void k_limiter_sub<double, false, false>(const StepArgsT<double> *__restrict__);
+template __global__ void k_limiter_sub<double, false, true>(const StepArgsT<double> *__restrict__);
+template __global__ void k_limiter_sub<double, true, false>(const StepArgsT<double> *__restrict__);
+template __global__ void k_limiter_sub<double, true, true>(const StepArgsT<double> *__restrict__);
+template __global__ void k_limiter_sub<float, false, false>(const StepArgsT<float> *__restrict__);
+template __global__ void k_limiter_sub<float, false, true>(const StepArgsT<float> *__restrict__);
+template __global__ void k_limiter_sub<float, true, false>(const StepArgsT<float> *__restrict__);
+template __global__ void k_limiter_sub<float, true, true>(const StepArgsT<float> *__restrict__);
 template __global__ void k_predictor<double>(const StepArgsT<double> *__restrict__);
 template __global__ void k_predictor<float>(const StepArgsT<float> *__restrict__);
 template __global__ void k_predictor_sub<double>(const StepArgsT<double> *__restrict__);

@@ -44,12 +44,15 @@ struct FineBed {
 };
 
 // values of cell (i, j) at the NL raw levels h[0..NL-1]: out[l][T] in the
-// table order V Z Vxp Vxm Vyp Vym Np Nm Op Om
-template <int NL>
+// table order V Z Vxp Vxm Vyp Vym Np Nm Op Om.  RR > 0: the pixel count per
+// cell side known at compile time (= b.r), so the pixel loops unroll and
+// every pixel load of the cell is in flight at once; the sums keep their
+// order, so the values are bitwise those of RR = 0.
+template <int NL, int RR = 0>
 __device__ inline void cell_levels(const FineBed &b, int i, int j, const double (&h)[NL], double (&out)[NL][10])
 {
 #pragma clang fp contract(off)
-    const int r = b.r, h2 = (r + 1) / 2;
+    const int r = RR > 0 ? RR : b.r, h2 = (r + 1) / 2;
     const size_t fx = (size_t)b.nx * r;
     const double dA = b.dxf * b.dxf;
     const double *z0 = b.z + (size_t)j * r * fx + (size_t)i * r;   // pixel (x=0, y=0) of the cell
@@ -60,8 +63,10 @@ __device__ inline void cell_levels(const FineBed &b, int i, int j, const double 
         vol[l] = vxp[l] = vxm[l] = vyp[l] = vym[l] = wet[l] = 0.0;
         np[l] = nm[l] = op[l] = om[l] = 0.0;
     }
+#pragma unroll
     for (int y = 0; y < r; y++) {
         const double *row = z0 + (size_t)y * fx;
+#pragma unroll
         for (int x = 0; x < r; x++) {
             const double zz = row[x];
 #pragma unroll
@@ -76,6 +81,7 @@ __device__ inline void cell_levels(const FineBed &b, int i, int j, const double 
     }
     if (r > 1) {
         const bool east = i < b.nx - 1, west = i > 0, north = j < b.ny - 1, south = j > 0;
+#pragma unroll
         for (int y = 0; y < r; y++) {
             const double *row = z0 + (size_t)y * fx;
             const double zl = row[r - 1], zf = row[0];
@@ -89,6 +95,7 @@ __device__ inline void cell_levels(const FineBed &b, int i, int j, const double 
             }
         }
         const double *top = z0 + (size_t)(r - 1) * fx;
+#pragma unroll
         for (int x = 0; x < r; x++) {
             const double zl = top[x], zf = z0[x];
             const double zn = north ? top[x + fx] : 0.0, zs = south ? z0[(ptrdiff_t)x - (ptrdiff_t)fx] : 0.0;
