@@ -245,7 +245,7 @@ def main():
     reqs = make_requests(a.count, a.rate, lo=a.lo, hi=a.hi, steps_lo=a.steps_lo, steps_hi=a.steps_hi,
                          subgrid_r=a.subgrid, workers=a.workers, on_the_fly=not a.tables)
     print(f"# built {len(reqs)} requests in {time.perf_counter() - t0:.1f} s", flush=True)
-    forms = {"fp64": {}, "mixed_warm": {"precision": abi.PREC_MIXED, "warmStart": 2},
+    forms = {"fp64": {}, "fp64_warm": {"warmStart": 2}, "mixed_warm": {"precision": abi.PREC_MIXED, "warmStart": 2},
              "mixed_cold": {"precision": abi.PREC_MIXED, "warmStart": 0}}
     out = {}
     for f in a.forms.split(","):
