@@ -786,7 +786,7 @@ static int exec_step_graphs(Exec &e)
     int k = 0, rc;
     if (!step) {
         // a pattern is captured the second time it comes up; the first time
-        // the step runs eagerly
+        // the step streams its launches
         int *count = nullptr;
         for (auto &sk : e.seen)
             if (sk.first == key) count = &sk.second;
@@ -795,7 +795,7 @@ static int exec_step_graphs(Exec &e)
             e.seen.push_back({key, 0});
             count = &e.seen.back().second;
         }
-        if (++*count < 2) return exec_step_eager(e);
+        if (++*count < 2) return exec_step_streamed(e);
         if (e.heads.size() >= 16) exec_release_graphs(e);
         if (getenv("FREHD_GRAPH_LOG")) {
             fprintf(stderr, "frehd: capturing step graph %zu, pattern", e.heads.size());
@@ -874,8 +874,10 @@ int exec_step(Exec &e)
     if (mode == kLaunchStreamed) return exec_step_streamed(e);
     // graphs once the group has stepped unchanged for a few steps (a group
     // whose members change every few steps would spend more on captures
-    // than the graphs save)
+    // than the graphs save); until then, and for a pattern's first step,
+    // streamed launches (the same kernels, one wait for the flags)
     if (mode == kLaunchGraphs && ++e.stableSteps > 3) return exec_step_graphs(e);
+    if (mode == kLaunchGraphs) return exec_step_streamed(e);
     return exec_step_eager(e);
 }
 
