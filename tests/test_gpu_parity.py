@@ -78,9 +78,23 @@ def three_scalar_marsh():
     return sc
 
 
+def two_tide(sc, amp=0.3, period=1200.0):
+    """A second tidal boundary along the first row (bcType = 3): the YM
+    boundary's Dirichlet rows, surface values and tidal-face velocities
+    (adjustTidalVelocity's second loop, nsfunctions.c:1058-1077, whose faces
+    south of the first row are jM ghosts)."""
+    nx, steps = sc.cfg["nx"], len(sc.tideP) - 1
+    sc.cfg["bcType"] = 3
+    sc.cfg["tideLocLengthM"] = nx
+    sc.tideLocM = np.arange(nx, dtype=np.int32)
+    sc.tideM[:] = cases.tide_series(steps, sc.cfg["dt"], amp, period=period, phase=1.0)
+    sc.name += "_ym"
+    return sc
+
+
 def cases_small():
     return [cases.basin(nx=40, ny=36, Nt=200), fast_marsh(), tvd_marsh(), wind_basin(), wind_marsh(),
-            three_scalar_marsh()]
+            three_scalar_marsh(), two_tide(fast_marsh())]
 
 
 def compare(dev_fields, ref_fields, names, tag, ulp=ULP_FIELDS, rel=None):

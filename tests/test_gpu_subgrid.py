@@ -65,8 +65,22 @@ def generated_sub(**kw):
     return cases.marsh_subgrid_generated(n=28, r=4, Nt=300, period=1200.0, **kw)
 
 
+def ym_sub():
+    """The subgrid marsh with a second tidal boundary along its first row
+    (bcType = 3): the YM loop of adjustTidalVelocitySub (subgrid.c:1291-1332),
+    whose faces south of the first row are jM ghosts."""
+    sc = cases.marsh_subgrid(n=32, r=4, Nt=300, period=1200.0)
+    nx = sc.cfg["nx"]
+    sc.cfg["bcType"] = 3
+    sc.cfg["tideLocLengthM"] = nx
+    sc.tideLocM = np.arange(nx, dtype=np.int32)
+    sc.tideM[:] = cases.tide_series(len(sc.tideP) - 1, sc.cfg["dt"], 0.6, period=900.0)
+    sc.name += "_ym"
+    return sc
+
+
 def sub_cases():
-    return [cases.marsh_subgrid(n=32, r=4, Nt=300, period=1200.0),
+    return [ym_sub(), cases.marsh_subgrid(n=32, r=4, Nt=300, period=1200.0),
             cases.marsh_subgrid(n=24, r=8, Nt=300, period=1200.0, manning_cell=False), tvd_sub(), wind_sub(),
             volp_sub(), casas_sub(), minarea_sub(0), minarea_sub(2), generated_sub(useminV=2),
             generated_sub(useminA=0, useminV=0, subdrag=2), generated_sub(useminA=0, useminV=0, subdrag=4,
