@@ -29,6 +29,18 @@ def gen_sub(**kw):
     return cases.marsh_subgrid_generated(n=20, r=4, Nt=200, period=1200.0, manning_cell=False, **kw)
 
 
+def two_tide(sc, amp=0.3):
+    """A second tidal boundary along the first row (bcType = 3): the YM
+    Dirichlet rows and surface values and adjustTidalVelocity's second loop
+    (nsfunctions.c:1058-1077)."""
+    nx, steps = sc.cfg["nx"], len(sc.tideP) - 1
+    sc.cfg["bcType"] = 3
+    sc.cfg["tideLocLengthM"] = nx
+    sc.tideLocM = np.arange(nx, dtype=np.int32)
+    sc.tideM[:] = cases.tide_series(steps, sc.cfg["dt"], amp, period=1200.0, phase=1.0)
+    return sc
+
+
 SUB_FIELDS = ["V", "Z", "Vx", "Vy", "Zx", "Zy", "Nx", "Oy", "Vxp", "Vxm", "Vyp", "Vym", "Np",
               "Nm", "Op", "Om", "minVx", "minVy"]
 
@@ -38,6 +50,9 @@ CASES = [
     ("basin_jacobi", lambda: cases.basin(nx=30, ny=26, Nt=40), {}),
     ("basin_ssor", lambda: cases.basin(nx=30, ny=26, Nt=40), {"precond": 1}),
     ("marsh_jacobi", marsh_fast, {}),
+    # two tidal boundaries (bcType 3: tideP on the last row, tideM on the first)
+    ("marsh_ym", lambda: two_tide(marsh_fast()), {}),
+    ("marsh_subgrid4_ym", lambda: two_tide(marsh_sub()), {}),
     ("basin_constCD_thin", lambda: cases.basin(nx=28, ny=24, Nt=40), {"CDnotN": 1, "hD": 2.6}),
     # TVD superbee scalar advection (scalar.c:99-255)
     ("basin_tvd", lambda: cases.basin(nx=30, ny=26, Nt=40), {"scalarAdv": 2}),
